@@ -1,0 +1,132 @@
+/*
+ * cosnark.h -- C ABI of libcosnark.so, the gfx950 (MI355X) implementation of
+ * one prover rank's local work in the code-based collaborative SNARK.
+ *
+ * Plain C, POD arguments only: device buffers are caller-owned device
+ * pointers (hipMalloc'd or cosnark_dev_alloc'd), host buffers are ordinary
+ * pointers.  Field elements are 16 bytes {u64 real; u64 img;} exactly as the
+ * reference's virgo::fieldElement (src/fieldElement.hpp:96-97), so a
+ * std::vector<F> is passed as (uint64_t*)v.data() with no copy.  Digests are
+ * 32-byte BLAKE3 outputs (struct _hash, src/Blake3_hash.h:3-5).
+ *
+ * Every entry point returns 0 on success or a negative COSNARK_E_* code; no
+ * exception crosses the boundary.  cosnark_last_error() gives the message.
+ * Work is issued on the context's HIP stream; unless stated otherwise a call
+ * returns after its results are complete.  One context per host thread.
+ *
+ * Citations name the reference function each entry point replaces
+ * (paths relative to the reference repository root).
+ */
+#ifndef COSNARK_H
+#define COSNARK_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define COSNARK_OK        0
+#define COSNARK_E_ARG    -1   /* bad argument (size not a power of two, ...) */
+#define COSNARK_E_HIP    -2   /* HIP runtime error */
+#define COSNARK_E_NOMEM  -3   /* device allocation failed */
+#define COSNARK_E_CHECK  -4   /* prover self-check failed where the
+                                 reference prints an error and exits */
+#define COSNARK_E_RANGE  -5   /* size beyond what this build supports */
+
+typedef struct cosnark_ctx cosnark_ctx;
+
+/* ---- context ------------------------------------------------------- */
+int  cosnark_create(int device, cosnark_ctx **out);
+void cosnark_destroy(cosnark_ctx *ctx);
+const char *cosnark_last_error(const cosnark_ctx *ctx);
+int  cosnark_sync(cosnark_ctx *ctx);
+/* the hipStream_t the context issues work on */
+void *cosnark_stream(cosnark_ctx *ctx);
+/* library build string (arch, version) */
+const char *cosnark_version(void);
+
+/* ---- device memory helpers (for callers with no allocator of their own) */
+int cosnark_dev_alloc(cosnark_ctx *ctx, uint64_t bytes, void **out);
+int cosnark_dev_free(cosnark_ctx *ctx, void *p);
+int cosnark_h2d(cosnark_ctx *ctx, void *d_dst, const void *h_src, uint64_t bytes);
+int cosnark_d2h(cosnark_ctx *ctx, void *h_dst, const void *d_src, uint64_t bytes);
+int cosnark_d2d(cosnark_ctx *ctx, void *d_dst, const void *d_src, uint64_t bytes);
+
+/* ---- synthetic workload inputs ---------------------------------------
+ * n elements uniform in GF(p^2): splitmix64-seeded xoshiro256**, each
+ * coordinate masked to 61 bits with the value p rejected.  Host buffer. */
+void cosnark_synth_fill(uint64_t seed, uint64_t stream, uint64_t *h_out, uint64_t n);
+
+/* ---- NTT: fft(arr, logn, inverse), src/utils.cpp:904-964 --------------
+ * In place on d_arr (2^logn elements), natural order in and out.  Forward
+ * uses getRootOfUnity(logn) (src/utils.cpp:706-717); inverse uses its
+ * inverse and scales by 2^-logn.  Also covers pfft (src/utils.cpp:1583),
+ * which computes the same forward transform.  1 <= logn <= 28. */
+int cosnark_ntt(cosnark_ctx *ctx, uint64_t *d_arr, int logn, int inverse);
+
+/* ---- Reed-Solomon encode, the codeword of Whir_Commit_initial ----------
+ * src/Whir.cpp:158-163: codeword = pfft(coeffs || 0) of length rate*n.
+ * d_coeffs (n elements) is read only; d_codeword holds rate*n elements. */
+int cosnark_rs_encode(cosnark_ctx *ctx, const uint64_t *d_coeffs, uint64_t n,
+                      int rate, uint64_t *d_codeword);
+
+/* ---- Merkle tree: MT_commit_Blake + create_tree_blake -----------------
+ * src/merkle_tree.cpp:195-233 and :267-299.  Leaf i = BLAKE3 of elements
+ * 4i..4i+3 (64 raw bytes).  All levels are written back to back into
+ * d_nodes (level 0 first, root last), cosnark_merkle_node_count(n) digests.
+ * dup_left = 1 reproduces the reference (a parent hashes its left child
+ * twice, :287-292); dup_left = 0 hashes left||right.  n_elems: power of
+ * two, 8 <= n_elems <= 2^30. */
+uint64_t cosnark_merkle_node_count(uint64_t n_elems);
+int cosnark_merkle_commit(cosnark_ctx *ctx, const uint64_t *d_elems, uint64_t n_elems,
+                          uint8_t *d_nodes, int dup_left);
+/* open_tree_blake, src/merkle_tree.cpp:320-335: sibling digest per level
+ * below the root into h_path (32 * depth bytes); *depth = log2(n_elems/4). */
+int cosnark_merkle_open(cosnark_ctx *ctx, const uint8_t *d_nodes, uint64_t n_elems,
+                        uint64_t pos, uint8_t *h_path, int *depth);
+
+/* ---- Fiat-Shamir transcript, src/Fiat_Shamir.cpp ----------------------
+ * The reference keeps one global running value prev_challenge (starts at 0,
+ * Fiat_Shamir.cpp:5); here it is explicit state owned by the caller. */
+typedef struct { uint64_t prev[2]; } cosnark_transcript;
+/* hash_to_field (:29-34): absorbs elems in order, returns the new state */
+int cosnark_hash_to_field(cosnark_transcript *ts, const uint64_t *h_elems, uint64_t n,
+                          uint64_t out[2]);
+/* get_indexes (:37-56): l pairs (prev.real % N, prev.img % M) into idx[2l] */
+int cosnark_get_indexes(cosnark_transcript *ts, int l, uint32_t N, uint32_t M, uint32_t *idx);
+
+/* ---- product sumcheck ---------------------------------------------------
+ * _quadratic_sumcheck(y, v1, v2, N = 1), src/Distributed_Sumcheck.cpp:204-348
+ * with the tail of quadratic_sumcheck, src/sparse_eval.cpp:140-192.
+ * n = 2^m elements per vector (read only).  Round i (i < m - offset) takes
+ * its challenge from the transcript (hash_to_field({a,b,c})); the last
+ * `offset` rounds draw F::_random() four times each from libc random()
+ * (src/fieldElement.cpp:119-125,363-368), as the reference's
+ * sumcheck_offset rounds do.  Outputs: h_polys[6*i..6*i+5] = (a,b,c) of
+ * round i's a x^2 + b x + c; h_r[2*i..] = challenge i; h_final = {v1[0],
+ * v2[0]} after all folds.  Returns COSNARK_E_CHECK if some round has
+ * H(0) + H(1) != y (the reference exits there, :243-247). */
+int cosnark_product_sumcheck(cosnark_ctx *ctx, const uint64_t *d_v1, const uint64_t *d_v2,
+                             uint64_t n, const uint64_t y[2], cosnark_transcript *ts,
+                             int offset, uint64_t *h_polys, uint64_t *h_r,
+                             uint64_t h_final[4]);
+
+/* ---- plain single-rank prover -------------------------------------------
+ * The rank-local pipeline of one proof: RS-encode f at blowup `rate`
+ * (Whir_Commit_initial form), Merkle-commit the codeword (MT_commit_Blake,
+ * dup-left), then the all-Fiat-Shamir product sumcheck of f*g with claimed
+ * sum y = sum_x f(x) g(x).  Roots are not absorbed into the transcript
+ * (the reference never absorbs them).  Proof bytes (cosnark_plain_proof_bytes
+ * of them): root (32) | m rounds x (a,b,c) (48 each) | v1[0], v2[0] (32).
+ * times (optional, 4 doubles): encode, merkle, sumcheck, total ms measured
+ * with HIP events on the context stream. */
+uint64_t cosnark_plain_proof_bytes(uint64_t n);
+int cosnark_prove_plain(cosnark_ctx *ctx, const uint64_t *d_f, const uint64_t *d_g,
+                        uint64_t n, int rate, cosnark_transcript *ts,
+                        uint8_t *h_proof, double *times);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,681 @@
+/*
+ * oracle.c -- CPU restatement of the reference prover's rank-local path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker).  Loaded by tests/, by
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg; never by the
+ * product library.  Each function cites the reference file:line it follows
+ * (paths relative to the reference repository root).
+ *
+ * Pinning (what checks each function independently of this text):
+ *   field ops     -- oracle/_ref (reference src/fieldElement.cpp compiled),
+ *                    Python big-int brute force, algebraic identities
+ *   root of unity -- oracle/_ref getRootOfUnity; order 2^n exactly
+ *   blake3        -- oracle/_ref (reference Blake/ C compiled), published
+ *                    BLAKE3 known answers
+ *   fft / pfft    -- O(n^2) DFT by definition, inverse round trip
+ *   merkle        -- dup-left closed form (level-l node i is the l-fold
+ *                    self-hash chain of leaf 2^l*i), leaf digests vs _ref
+ *   hash_F        -- digest bytes vs _ref blake3 on the documented layout;
+ *                    7-bit packing bound; chain order (partially pinned)
+ *   sumcheck      -- H(0)+H(1)=y each round, final values = multilinear
+ *                    evaluation at the challenges (brute force)
+ *   beta/eval     -- brute-force eq polynomial, Lagrange identity
+ *   _random       -- oracle/_ref F::_random() on the same libc seed
+ *   whir          -- internal consistency (the reference's own verifier
+ *                    checks 1-2 run inside or_whir_open), query-fold closed
+ *                    form; exact byte order of the record: parity unpinned
+ *                    beyond those checks (reference Whir.cpp unbuildable).
+ */
+#include "oracle.h"
+
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+
+#define P61 0x1FFFFFFFFFFFFFFFULL
+
+/* ------------------------------------------------------------------ */
+/* field: src/fieldElement.cpp                                          */
+/* ------------------------------------------------------------------ */
+
+static uint64_t modp(unsigned __int128 t) {
+    /* 2^61 == 1 (mod p): fold the high part onto the low 61 bits */
+    uint64_t lo = (uint64_t)t & P61;
+    uint64_t hi = (uint64_t)(t >> 61);
+    uint64_t s = lo + hi;               /* < 2^64 for t < 2^125 */
+    s = (s & P61) + (s >> 61);
+    if (s >= P61) s -= P61;
+    return s;
+}
+
+static uint64_t addp(uint64_t a, uint64_t b) {
+    uint64_t s = a + b;
+    return s >= P61 ? s - P61 : s;
+}
+
+static uint64_t subp(uint64_t a, uint64_t b) {
+    return a >= b ? a - b : a + P61 - b;
+}
+
+or_fe or_add(or_fe a, or_fe b) {            /* fieldElement.cpp:34-47 */
+    or_fe r = { addp(a.real, b.real), addp(a.img, b.img) };
+    return r;
+}
+
+or_fe or_sub(or_fe a, or_fe b) {            /* fieldElement.cpp:80-96 */
+    or_fe r = { subp(a.real, b.real), subp(a.img, b.img) };
+    return r;
+}
+
+or_fe or_mul(or_fe a, or_fe b) {            /* fieldElement.cpp:49-78 */
+    /* (a.r + a.i I)(b.r + b.i I), I^2 = -1:
+     * real = ar*br - ai*bi, img = ar*bi + ai*br  (all mod p, canonical) */
+    uint64_t ac = modp((unsigned __int128)a.real * b.real);
+    uint64_t bd = modp((unsigned __int128)a.img * b.img);
+    uint64_t ad = modp((unsigned __int128)a.real * b.img);
+    uint64_t bc = modp((unsigned __int128)a.img * b.real);
+    or_fe r = { subp(ac, bd), addp(ad, bc) };
+    return r;
+}
+
+or_fe or_from_u64(uint64_t x) {             /* fieldElement.cpp:24-27 */
+    or_fe r = { x, 0 };
+    return r;
+}
+
+or_fe or_pow(or_fe x, unsigned __int128 e) {   /* fieldElement.cpp:323-335 */
+    or_fe ret = { 1, 0 }, t = x;
+    while (e) {
+        if (e & 1) ret = or_mul(ret, t);
+        t = or_mul(t, t);
+        e >>= 1;
+    }
+    return ret;
+}
+
+or_fe or_inv(or_fe a) {                     /* fieldElement.cpp:207-210 */
+    unsigned __int128 p = P61;
+    return or_pow(a, p * p - 2);
+}
+
+or_fe or_root_of_unity(int logn) {          /* utils.cpp:706-717 */
+    /* generator of order 2^62: 2^31 + 1033321771269002680 I */
+    or_fe g = { 2147483648ULL, 1033321771269002680ULL };
+    for (int i = 0; i < 62 - logn; ++i) g = or_mul(g, g);
+    return g;
+}
+
+void or_vec_mul(const or_fe *a, const or_fe *b, or_fe *o, uint64_t n) { for (uint64_t i = 0; i < n; ++i) o[i] = or_mul(a[i], b[i]); }
+void or_vec_add(const or_fe *a, const or_fe *b, or_fe *o, uint64_t n) { for (uint64_t i = 0; i < n; ++i) o[i] = or_add(a[i], b[i]); }
+void or_vec_sub(const or_fe *a, const or_fe *b, or_fe *o, uint64_t n) { for (uint64_t i = 0; i < n; ++i) o[i] = or_sub(a[i], b[i]); }
+void or_vec_inv(const or_fe *a, or_fe *o, uint64_t n) { for (uint64_t i = 0; i < n; ++i) o[i] = or_inv(a[i]); }
+
+/* ------------------------------------------------------------------ */
+/* BLAKE3 single-block compression (BLAKE3 specification; the reference */
+/* vendors BLAKE3 1.8.2 in Blake/, blake3_portable.c:8-82)              */
+/* ------------------------------------------------------------------ */
+
+static const uint32_t B3_IV[8] = {
+    0x6A09E667u, 0xBB67AE85u, 0x3C6EF372u, 0xA54FF53Au,
+    0x510E527Fu, 0x9B05688Cu, 0x1F83D9ABu, 0x5BE0CD19u };
+/* per-round message word permutation from the BLAKE3 paper, sec. 2.2 */
+static const uint8_t B3_PERM[16] = { 2, 6, 3, 10, 7, 0, 4, 13, 1, 11, 12, 5, 9, 14, 15, 8 };
+
+static inline uint32_t rotr(uint32_t x, int c) { return (x >> c) | (x << (32 - c)); }
+
+static inline void b3_g(uint32_t *v, int a, int b, int c, int d, uint32_t mx, uint32_t my) {
+    v[a] = v[a] + v[b] + mx; v[d] = rotr(v[d] ^ v[a], 16);
+    v[c] = v[c] + v[d];      v[b] = rotr(v[b] ^ v[c], 12);
+    v[a] = v[a] + v[b] + my; v[d] = rotr(v[d] ^ v[a], 8);
+    v[c] = v[c] + v[d];      v[b] = rotr(v[b] ^ v[c], 7);
+}
+
+/* one compression with chaining value IV, counter 0; out = first 8 words */
+static void b3_compress(const uint32_t m_in[16], uint32_t block_len, uint32_t flags, uint32_t out[8]) {
+    uint32_t v[16], m[16], t[16];
+    memcpy(m, m_in, sizeof(m));
+    for (int i = 0; i < 8; ++i) v[i] = B3_IV[i];
+    v[8] = B3_IV[0]; v[9] = B3_IV[1]; v[10] = B3_IV[2]; v[11] = B3_IV[3];
+    v[12] = 0; v[13] = 0; v[14] = block_len; v[15] = flags;
+    for (int round = 0; round < 7; ++round) {
+        b3_g(v, 0, 4, 8, 12, m[0], m[1]);
+        b3_g(v, 1, 5, 9, 13, m[2], m[3]);
+        b3_g(v, 2, 6, 10, 14, m[4], m[5]);
+        b3_g(v, 3, 7, 11, 15, m[6], m[7]);
+        b3_g(v, 0, 5, 10, 15, m[8], m[9]);
+        b3_g(v, 1, 6, 11, 12, m[10], m[11]);
+        b3_g(v, 2, 7, 8, 13, m[12], m[13]);
+        b3_g(v, 3, 4, 9, 14, m[14], m[15]);
+        for (int i = 0; i < 16; ++i) t[i] = m[B3_PERM[i]];
+        memcpy(m, t, sizeof(m));
+    }
+    for (int i = 0; i < 8; ++i) out[i] = v[i] ^ v[i + 8];
+}
+
+/* flags CHUNK_START|CHUNK_END|ROOT: a whole message that fits one block */
+#define B3_FLAGS_SINGLE 0x0Bu
+
+void or_blake3_small(const uint8_t *src, size_t len, uint8_t dst[32]) {
+    uint8_t blk[64];
+    uint32_t m[16], out[8];
+    memset(blk, 0, sizeof(blk));
+    if (len > 64) len = 64;
+    memcpy(blk, src, len);
+    for (int i = 0; i < 16; ++i)
+        m[i] = (uint32_t)blk[4 * i] | ((uint32_t)blk[4 * i + 1] << 8) |
+               ((uint32_t)blk[4 * i + 2] << 16) | ((uint32_t)blk[4 * i + 3] << 24);
+    b3_compress(m, (uint32_t)len, B3_FLAGS_SINGLE, out);
+    for (int i = 0; i < 8; ++i) {
+        dst[4 * i] = (uint8_t)out[i];
+        dst[4 * i + 1] = (uint8_t)(out[i] >> 8);
+        dst[4 * i + 2] = (uint8_t)(out[i] >> 16);
+        dst[4 * i + 3] = (uint8_t)(out[i] >> 24);
+    }
+}
+
+void or_blake3_hash64(const uint8_t src[64], uint8_t dst[32]) {   /* Blake3_hash.cpp:5-9 */
+    or_blake3_small(src, 64, dst);
+}
+
+void or_blake3_hash64_many(const uint8_t *src, uint8_t *dst, uint64_t n) {
+    for (uint64_t i = 0; i < n; ++i) or_blake3_hash64(src + 64 * i, dst + 32 * i);
+}
+
+/* ------------------------------------------------------------------ */
+/* Fiat-Shamir: src/Fiat_Shamir.cpp                                     */
+/* ------------------------------------------------------------------ */
+
+or_fe or_hash_F(or_fe x, or_fe y) {         /* Fiat_Shamir.cpp:8-27 */
+    /* 64-byte block: {x.real, x.img, 0, 0, y.real, y.img, 0, 0} as LE u64 */
+    uint64_t w[8] = { x.real, x.img, 0, 0, y.real, y.img, 0, 0 };
+    uint8_t data[64], h[32];
+    memcpy(data, w, 64);                    /* x86 is little-endian */
+    or_blake3_hash64(data, h);
+    /* digest bytes 0..7 -> real, 8..15 -> img, with a 7-bit stride */
+    uint64_t real = 0, img = 0;
+    for (int i = 0; i < 8; ++i) {
+        real += (1ULL << (7 * i)) * h[i];
+        img += (1ULL << (7 * i)) * h[i + 8];
+    }
+    or_fe r = { real, img };
+    return r;
+}
+
+or_fe or_hash_to_field(or_fe *prev, const or_fe *elems, uint64_t n) {  /* :29-34 */
+    for (uint64_t i = 0; i < n; ++i) *prev = or_hash_F(*prev, elems[i]);
+    return *prev;
+}
+
+void or_get_indexes(or_fe *prev, int l, uint32_t N, uint32_t M, uint32_t *idx) {  /* :37-56 */
+    for (int i = 0; i < l; ++i) {
+        *prev = or_hash_F(*prev, *prev);
+        idx[2 * i] = (uint32_t)(prev->real % N);
+        idx[2 * i + 1] = (uint32_t)(prev->img % M);
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* NTT: src/utils.cpp:904-964 (fft), :1583-1673 (pfft)                  */
+/* ------------------------------------------------------------------ */
+
+void or_fft(or_fe *arr, int logn, int inverse) {
+    uint64_t len = 1ULL << logn;
+    if (logn == 0) return;
+    /* bit-reversal permutation (utils.cpp:918-919,938-939) */
+    for (uint64_t i = 1, j = 0; i < len; ++i) {
+        uint64_t bit = len >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) { or_fe t = arr[i]; arr[i] = arr[j]; arr[j] = t; }
+    }
+    or_fe w1 = or_root_of_unity(logn);
+    if (inverse) w1 = or_inv(w1);
+    /* twiddle table w[k] = w1^k (utils.cpp:931-937) */
+    or_fe *w = (or_fe *)malloc(sizeof(or_fe) * (len > 1 ? len / 2 : 1));
+    w[0].real = 1; w[0].img = 0;
+    for (uint64_t k = 1; k < len / 2; ++k) w[k] = or_mul(w[k - 1], w1);
+    /* butterflies (utils.cpp:943-950) */
+    for (uint64_t i = 2; i <= len; i <<= 1) {
+        uint64_t h = i >> 1, step = len / i;
+        for (uint64_t j = 0; j < len; j += i)
+            for (uint64_t k = 0; k < h; ++k) {
+                or_fe u = arr[j + k];
+                or_fe v = or_mul(arr[j + k + h], w[step * k]);
+                arr[j + k] = or_add(u, v);
+                arr[j + k + h] = or_sub(u, v);
+            }
+    }
+    free(w);
+    if (inverse) {                          /* utils.cpp:952-963 */
+        or_fe ilen = or_inv(or_from_u64(len));
+        for (uint64_t i = 0; i < len; ++i) arr[i] = or_mul(arr[i], ilen);
+    }
+}
+
+/* pfft is a four-step evaluation of the same forward DFT (16 row transforms,
+ * twiddle, 16-point column transforms; utils.cpp:1583-1673) and falls back to
+ * fft for len <= 1024.  Its result equals fft(arr, logn, false). */
+void or_pfft(or_fe *arr, int logn) { or_fft(arr, logn, 0); }
+
+void or_rs_encode(const or_fe *arr, uint64_t n, int rate, or_fe *cw) {  /* Whir.cpp:158-163 */
+    uint64_t m = n * (uint64_t)rate;
+    memcpy(cw, arr, n * sizeof(or_fe));
+    memset(cw + n, 0, (m - n) * sizeof(or_fe));
+    int logm = 0;
+    while ((1ULL << logm) < m) ++logm;
+    or_pfft(cw, logm);
+}
+
+/* ------------------------------------------------------------------ */
+/* Merkle tree: src/merkle_tree.cpp                                     */
+/* ------------------------------------------------------------------ */
+
+uint64_t or_merkle_node_count(uint64_t n_elems) {
+    uint64_t leaves = n_elems / 4, total = 0;
+    for (uint64_t s = leaves; s >= 1; s /= 2) total += s;
+    return total;
+}
+
+void or_merkle_commit(const or_fe *elems, uint64_t n_elems, uint8_t *nodes, int dup_left) {
+    uint64_t leaves = n_elems / 4;
+    /* leaf i = BLAKE3(F[4i] || F[4i+1] || F[4i+2] || F[4i+3]), raw 64 bytes
+     * of {real,img} pairs (merkle_tree.cpp:211-220) */
+    for (uint64_t i = 0; i < leaves; ++i)
+        or_blake3_hash64((const uint8_t *)(elems + 4 * i), nodes + 32 * i);
+    /* levels (create_tree_blake, merkle_tree.cpp:267-299) */
+    uint8_t *prev = nodes, *cur = nodes + 32 * leaves;
+    for (uint64_t s = leaves / 2; s >= 1; s /= 2) {
+        for (uint64_t i = 0; i < s; ++i) {
+            uint8_t data[64];
+            memcpy(data, prev + 32 * (2 * i), 32);
+            /* the reference copies the left child into both halves (:287-292) */
+            memcpy(data + 32, prev + 32 * (dup_left ? 2 * i : 2 * i + 1), 32);
+            or_blake3_hash64(data, cur + 32 * i);
+        }
+        prev = cur;
+        cur += 32 * s;
+    }
+}
+
+int or_merkle_open(const uint8_t *nodes, uint64_t n_elems, uint64_t pos, uint8_t *path) {
+    /* open_tree_blake (merkle_tree.cpp:320-335): per level below the root,
+     * the sibling 2*(pos/2) + (1 - pos%2), then pos /= 2 */
+    uint64_t s = n_elems / 4, off = 0;
+    int lvl = 0;
+    if (pos >= s) return -1;
+    while (s > 1) {
+        uint64_t sib = 2 * (pos / 2) + (1 - (pos % 2));
+        memcpy(path + 32 * lvl, nodes + 32 * (off + sib), 32);
+        off += s;
+        s /= 2;
+        pos /= 2;
+        ++lvl;
+    }
+    return lvl;
+}
+
+/* ------------------------------------------------------------------ */
+/* multilinear helpers: src/utils.cpp                                   */
+/* ------------------------------------------------------------------ */
+
+void or_precompute_beta(const or_fe *r, int m, or_fe *B) {   /* utils.cpp:198-243 */
+    /* B[x] = prod_k (bit_k(x) ? r[k] : 1 - r[k]); bit 0 pairs with r[0]:
+     * the table doubles once per coordinate, consuming r[m-1] first */
+    B[0].real = 1; B[0].img = 0;
+    for (int i = 0; i < m; ++i) {
+        uint64_t sz = 1ULL << i;
+        or_fe ri = r[m - 1 - i];
+        for (uint64_t j = sz; j-- > 0;) {
+            or_fe t = or_mul(ri, B[j]);
+            B[2 * j] = or_sub(B[j], t);
+            B[2 * j + 1] = t;
+        }
+    }
+}
+
+or_fe or_evaluate_vector(const or_fe *v, uint64_t n, const or_fe *r) {  /* utils.cpp:1201-1214 */
+    or_fe *t = (or_fe *)malloc(sizeof(or_fe) * n);
+    memcpy(t, v, sizeof(or_fe) * n);
+    int m = 0;
+    while ((1ULL << m) < n) ++m;
+    or_fe one = { 1, 0 };
+    for (int i = 0; i < m; ++i) {
+        uint64_t L = 1ULL << (m - 1 - i);
+        for (uint64_t j = 0; j < L; ++j)
+            t[j] = or_add(or_mul(or_sub(one, r[i]), t[2 * j]), or_mul(r[i], t[2 * j + 1]));
+    }
+    or_fe res = t[0];
+    free(t);
+    return res;
+}
+
+void or_compute_lagrange_coeff(or_fe r, int degree, or_fe *L) {   /* utils.cpp:169-197 */
+    /* inverse DFT of (1, r, r^2, ..., r^{d-1}) */
+    L[0].real = 1; L[0].img = 0;
+    for (int i = 1; i < degree; ++i) L[i] = or_mul(L[i - 1], r);
+    int logd = 0;
+    while ((1 << logd) < degree) ++logd;
+    or_fft(L, logd, 1);
+}
+
+/* ------------------------------------------------------------------ */
+/* glibc challenge stream: src/fieldElement.cpp:119-125, 363-368        */
+/* ------------------------------------------------------------------ */
+
+static uint64_t random_number(void) {
+    uint64_t ret = (uint64_t)(random() % 10);
+    for (int i = 1; i < 20; ++i) ret = (ret * 10ULL + (uint64_t)(random() % 10)) % P61;
+    return ret;
+}
+
+or_fe or_random_element(void) {
+    or_fe r;
+    r.real = random_number() % P61;
+    r.img = random_number() % P61;
+    return r;
+}
+
+/* ------------------------------------------------------------------ */
+/* product sumcheck: src/Distributed_Sumcheck.cpp:204-348 at N = 1     */
+/* ------------------------------------------------------------------ */
+
+/* one round's polynomial: sum over pairs of (v[2j+1]-v[2j]) x + v[2j]
+ * multiplied for the two vectors (Distributed_Sumcheck.cpp:224-233,
+ * polynomial.cpp:133-135: (a x + b)(c x + d) = ac x^2 + (ad + bc) x + bd) */
+static void round_poly(const or_fe *v1, const or_fe *v2, uint64_t L, or_fe out[3]) {
+    or_fe a = { 0, 0 }, b = { 0, 0 }, c = { 0, 0 };
+    for (uint64_t j = 0; j < L; ++j) {
+        or_fe d1 = or_sub(v1[2 * j + 1], v1[2 * j]);
+        or_fe d2 = or_sub(v2[2 * j + 1], v2[2 * j]);
+        a = or_add(a, or_mul(d1, d2));
+        b = or_add(b, or_add(or_mul(d1, v2[2 * j]), or_mul(v1[2 * j], d2)));
+        c = or_add(c, or_mul(v1[2 * j], v2[2 * j]));
+    }
+    out[0] = a; out[1] = b; out[2] = c;
+}
+
+static or_fe quad_eval(const or_fe p[3], or_fe x) {   /* polynomial.cpp:111-113 */
+    return or_add(or_mul(or_add(or_mul(p[0], x), p[1]), x), p[2]);
+}
+
+static void fold(or_fe *v, uint64_t L, or_fe r) {     /* :257-261 */
+    for (uint64_t j = 0; j < L; ++j)
+        v[j] = or_add(or_mul(r, or_sub(v[2 * j + 1], v[2 * j])), v[2 * j]);
+}
+
+int or_product_sumcheck(or_fe y, or_fe *v1, or_fe *v2, uint64_t n, or_fe *prev,
+                        int offset, or_fe *polys, or_fe *r, or_fe *final_out) {
+    int m = 0;
+    while ((1ULL << m) < n) ++m;
+    int fs_rounds = m - offset;
+    if (fs_rounds < 0) fs_rounds = 0;
+    or_fe zero = { 0, 0 }, one = { 1, 0 };
+    for (int i = 0; i < m; ++i) {
+        uint64_t L = 1ULL << (m - 1 - i);
+        or_fe p[3];
+        round_poly(v1, v2, L, p);
+        /* consistency check (:243-247; sparse_eval.cpp:160-163) */
+        if (!(or_add(quad_eval(p, zero), quad_eval(p, one)).real == y.real &&
+              or_add(quad_eval(p, zero), quad_eval(p, one)).img == y.img))
+            return -(i + 1);
+        or_fe ch;
+        if (i < fs_rounds) {
+            ch = or_hash_to_field(prev, p, 3);       /* :248 */
+        } else {
+            /* tail rounds: four F::_random() draws, the last is kept
+             * (sparse_eval.cpp:164-168) */
+            ch = or_random_element();
+            ch = or_random_element();
+            ch = or_random_element();
+            ch = or_random_element();
+        }
+        y = quad_eval(p, ch);
+        polys[3 * i] = p[0]; polys[3 * i + 1] = p[1]; polys[3 * i + 2] = p[2];
+        r[i] = ch;
+        fold(v1, L, ch);
+        fold(v2, L, ch);
+    }
+    final_out[0] = v1[0];
+    final_out[1] = v2[0];
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* synthetic inputs                                                     */
+/* ------------------------------------------------------------------ */
+
+static uint64_t splitmix64(uint64_t *s) {
+    uint64_t z = (*s += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+static inline uint64_t rotl64(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+
+static uint64_t xoshiro_next(uint64_t s[4]) {
+    uint64_t result = rotl64(s[1] * 5, 7) * 9;
+    uint64_t t = s[1] << 17;
+    s[2] ^= s[0]; s[3] ^= s[1]; s[1] ^= s[2]; s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = rotl64(s[3], 45);
+    return result;
+}
+
+void or_synth_fill(uint64_t seed, uint64_t stream, or_fe *out, uint64_t n) {
+    uint64_t sm = seed ^ (stream * 0xD1B54A32D192ED03ULL), s[4];
+    for (int i = 0; i < 4; ++i) s[i] = splitmix64(&sm);
+    for (uint64_t i = 0; i < n; ++i) {
+        uint64_t x;
+        do { x = xoshiro_next(s) & P61; } while (x == P61);
+        out[i].real = x;
+        do { x = xoshiro_next(s) & P61; } while (x == P61);
+        out[i].img = x;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* WHIR: src/Whir.cpp, whir_k = 4                                       */
+/* ------------------------------------------------------------------ */
+
+#define WHIR_K 4
+
+int or_whir_query_count(int rate) {         /* Whir.cpp:253-258 (conjectured_security false) */
+    return (int)(-100.0 / log2(sqrt(1.0 / (double)rate)));
+}
+
+static int ilog2u(uint64_t x) { int l = 0; while ((1ULL << l) < x) ++l; return l; }
+
+or_fe or_rs_multifold_segment_tensor(const or_fe *seg, int seg_n, const or_fe *r, int nr,
+                                     uint64_t n, uint64_t idx) {   /* coPCS_utils.cpp:333-383 */
+    or_fe one = { 1, 0 };
+    or_fe len_inv = or_inv(or_from_u64((uint64_t)seg_n));
+    int fold_degree = seg_n, log_fold = ilog2u((uint64_t)fold_degree);
+    int logn = ilog2u(n);
+    or_fe omega = or_inv(or_root_of_unity(logn));
+    omega = or_pow(omega, n / (uint64_t)seg_n);
+    or_fe inv_omegas = or_pow(or_inv(or_root_of_unity(logn)), idx);
+    or_fe part[64];
+    for (int j = 0; j < seg_n / 2; ++j) {
+        or_fe s = or_add(seg[j], seg[j + seg_n / 2]);
+        or_fe d = or_sub(seg[j], seg[j + seg_n / 2]);
+        part[j] = or_add(or_mul(or_sub(one, r[0]), s), or_mul(or_mul(r[0], inv_omegas), d));
+        inv_omegas = or_mul(inv_omegas, omega);
+    }
+    if (nr > 1) {
+        or_fe pw[16];
+        for (int i = 0; i < log_fold; ++i) pw[i] = one;
+        or_fe om = or_inv(or_root_of_unity(logn));
+        if (log_fold > 1) pw[1] = or_mul(om, om);
+        for (int i = 2; i < log_fold; ++i) pw[i] = or_mul(pw[i - 1], pw[i - 1]);
+        for (int i = 1; i < log_fold; ++i) pw[i] = or_pow(pw[i], n / (uint64_t)seg_n);
+        int sn = seg_n;
+        for (int i = 1; i < log_fold; ++i) {
+            sn /= 2;
+            or_fe om_i = or_inv(or_root_of_unity(ilog2u(n >> i)));
+            or_fe pow_omega = or_pow(om_i, idx);
+            for (int j = 0; j < sn / 2; ++j) {
+                or_fe s = or_add(part[j], part[j + sn / 2]);
+                or_fe d = or_sub(part[j], part[j + sn / 2]);
+                part[j] = or_add(or_mul(or_sub(one, r[i]), s), or_mul(r[i], or_mul(pow_omega, d)));
+                pow_omega = or_mul(pow_omega, pw[i]);
+            }
+        }
+    }
+    return or_mul(len_inv, part[0]);
+}
+
+/* record writer */
+typedef struct { uint64_t *p; uint64_t n, cap; int overflow; } rec_t;
+static void rec_u64(rec_t *R, uint64_t x) { if (R->n < R->cap) R->p[R->n] = x; else R->overflow = 1; R->n++; }
+static void rec_fe(rec_t *R, or_fe x) { rec_u64(R, x.real); rec_u64(R, x.img); }
+static void rec_bytes32(rec_t *R, const uint8_t *h) { uint64_t w[4]; memcpy(w, h, 32); for (int i = 0; i < 4; ++i) rec_u64(R, w[i]); }
+
+/*
+ * WHIR opening record (u64 words):
+ *   [n_iter]
+ *   per iteration:
+ *     polys  2 x (a,b,c)           12 words
+ *     r      2 challenges           4 words
+ *     root   new commitment root    4 words
+ *     l      query count            1 word
+ *     q[l]   query positions as drawn (before scaling)   l words
+ *     replies l x 4 elements       8l words
+ *     y      l folded values       2l words
+ *     paths  l x depth digests     4*l*depth words (depth = log2(|cw_old|/4))
+ *     rb     update_betas challenge 2 words
+ *   final: arr (4 elements, 8 words), K (2 words), final root (4 words),
+ *          final r (log2|arr| elements)
+ */
+int64_t or_whir_open(const or_fe *arr_in, uint64_t n, const or_fe *z, or_fe K,
+                     int rate, int stop_log, uint64_t *record, uint64_t record_cap) {
+    rec_t R = { record, 0, record_cap, 0 };
+    int m = ilog2u(n);
+    or_fe zero = { 0, 0 }, one = { 1, 0 };
+    uint64_t stop = WHIR_K;
+    if (stop_log > 0 && (1ULL << stop_log) > stop) stop = 1ULL << stop_log;
+
+    or_fe *arr = (or_fe *)malloc(sizeof(or_fe) * n);
+    or_fe *b = (or_fe *)malloc(sizeof(or_fe) * n);
+    memcpy(arr, arr_in, sizeof(or_fe) * n);
+    or_precompute_beta(z, m, b);                           /* Whir.cpp:395 */
+
+    /* initial commitment (Whir_Commit_initial with the caller's rate) */
+    uint64_t cw_n = n * (uint64_t)rate;
+    or_fe *cw = (or_fe *)malloc(sizeof(or_fe) * cw_n);
+    or_rs_encode(arr, n, rate, cw);
+    uint8_t *mt = (uint8_t *)malloc(32 * or_merkle_node_count(cw_n));
+    or_merkle_commit(cw, cw_n, mt, 1);
+
+    uint64_t iters_pos = R.n;
+    rec_u64(&R, 0);
+    uint64_t iters = 0;
+    uint64_t cur_n = n;
+    int status = 0;
+    while (1) {
+        /* ---- Whir_Iteration (Whir.cpp:354-386) ---- */
+        or_fe rr[2];
+        for (int rd = 0; rd < 2; ++rd) {                   /* whir_sumcheck :233-249 */
+            uint64_t L = cur_n / 2;
+            or_fe p[3];
+            round_poly(arr, b, L, p);
+            or_fe s01 = or_add(quad_eval(p, zero), quad_eval(p, one));
+            if (s01.real != K.real || s01.img != K.img) { status = -10 - (int)iters; goto done; }
+            rr[rd] = or_random_element();
+            K = quad_eval(p, rr[rd]);
+            fold(arr, L, rr[rd]);
+            fold(b, L, rr[rd]);
+            cur_n = L;
+            rec_fe(&R, p[0]); rec_fe(&R, p[1]); rec_fe(&R, p[2]);
+        }
+        rec_fe(&R, rr[0]); rec_fe(&R, rr[1]);
+
+        /* new commitment at 2*rate (Whir_Commit_initial :158-190) */
+        uint64_t ncw_n = cur_n * (uint64_t)(2 * rate);
+        or_fe *ncw = (or_fe *)malloc(sizeof(or_fe) * ncw_n);
+        or_rs_encode(arr, cur_n, 2 * rate, ncw);
+        uint64_t nnodes = or_merkle_node_count(ncw_n);
+        uint8_t *nmt = (uint8_t *)malloc(32 * nnodes);
+        or_merkle_commit(ncw, ncw_n, nmt, 1);
+        rec_bytes32(&R, nmt + 32 * (nnodes - 1));
+
+        /* whir_query (:251-314) on the previous codeword */
+        int l = or_whir_query_count(rate);
+        rec_u64(&R, (uint64_t)l);
+        uint64_t *q = (uint64_t *)malloc(sizeof(uint64_t) * l);
+        for (int i = 0; i < l; ++i) {
+            q[i] = (uint64_t)(rand() % (int)(cw_n / 4));  /* whir_k <= 4 branch */
+            rec_u64(&R, q[i]);
+        }
+        or_fe *y = (or_fe *)malloc(sizeof(or_fe) * l);
+        for (int i = 0; i < l; ++i) {
+            or_fe rep[WHIR_K];
+            for (int j = 0; j < WHIR_K; ++j) {
+                rep[j] = cw[q[i] + (uint64_t)j * cw_n / WHIR_K];
+                rec_fe(&R, rep[j]);
+            }
+            y[i] = or_rs_multifold_segment_tensor(rep, WHIR_K, rr, 2, cw_n, q[i]);
+        }
+        for (int i = 0; i < l; ++i) rec_fe(&R, y[i]);
+        {
+            uint8_t path[32 * 64];
+            for (int i = 0; i < l; ++i) {
+                int d = or_merkle_open(mt, cw_n, q[i], path);
+                for (int k = 0; k < d; ++k) rec_bytes32(&R, path + 32 * k);
+            }
+        }
+        /* scale the queries to the new codeword (:373-374) */
+        uint64_t scale = (uint64_t)WHIR_K * ncw_n / cw_n;
+        for (int i = 0; i < l; ++i) q[i] *= scale;
+
+        /* update_betas (:325-342) */
+        or_fe rb = or_random_element();
+        rec_fe(&R, rb);
+        or_fe ch = rb;
+        int logncw = ilog2u(ncw_n);
+        or_fe omega = or_root_of_unity(logncw);
+        for (int i = 0; i < l; ++i) {
+            if (i > 0) ch = or_mul(ch, rb);
+            or_fe w = or_pow(omega, q[i]), pw = one;
+            for (uint64_t j = 0; j < cur_n; ++j) {
+                b[j] = or_add(b[j], or_mul(ch, pw));
+                pw = or_mul(pw, w);
+            }
+            K = or_add(K, or_mul(ch, y[i]));
+        }
+        free(q); free(y);
+        free(cw); free(mt);
+        cw = ncw; mt = nmt; cw_n = ncw_n;
+        ++iters;
+        rate *= 2;
+        if (cur_n <= stop) break;
+    }
+    record[iters_pos] = iters;
+
+    /* final steps of Whir_Open (:425-466) */
+    for (uint64_t i = 0; i < cur_n; ++i) rec_fe(&R, arr[i]);
+    rec_fe(&R, K);
+    {
+        /* Whir_Commit (:192-213) hashes an all-zero leaf vector of
+         * cw/whir_k digests: the root depends only on the size */
+        uint64_t leaves = cur_n * (uint64_t)rate / WHIR_K;
+        uint8_t cur[32], data[64];
+        memset(cur, 0, 32);
+        for (uint64_t s = leaves / 2; s >= 1; s /= 2) {
+            memcpy(data, cur, 32); memcpy(data + 32, cur, 32);
+            or_blake3_hash64(data, cur);
+        }
+        rec_bytes32(&R, cur);
+    }
+    {
+        or_fe Kc = K;
+        for (uint64_t i = 0; i < cur_n; ++i) Kc = or_sub(Kc, or_mul(arr[i], b[i]));
+        if (Kc.real != 0 || Kc.img != 0) { status = -2; goto done; }
+    }
+    for (int i = 0; i < ilog2u(cur_n); ++i) rec_fe(&R, or_random_element());
+done:
+    free(arr); free(b); free(cw); free(mt);
+    if (status) return status;
+    if (R.overflow) return -1;
+    return (int64_t)R.n;
+}

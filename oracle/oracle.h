@@ -1,0 +1,112 @@
+/*
+ * oracle.h -- CPU restatement of the reference prover's rank-local path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
+ * product (libcosnark.so) never links, loads or calls anything in here.
+ *
+ * Plain C (gcc), scalar, single-threaded unless a function says otherwise.
+ * Every function cites the reference file:line whose behaviour it restates
+ * (paths relative to the reference repository root).
+ *
+ * Element layout follows src/fieldElement.hpp:96-97: {u64 real; u64 img;},
+ * 16 bytes, canonical values in [0, 2^61-1).
+ */
+#ifndef COSNARK_ORACLE_H
+#define COSNARK_ORACLE_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { uint64_t real, img; } or_fe;
+
+/* ---- field GF((2^61-1)^2), src/fieldElement.cpp ---- */
+or_fe or_add(or_fe a, or_fe b);                 /* :34-47 */
+or_fe or_sub(or_fe a, or_fe b);                 /* :80-96 */
+or_fe or_mul(or_fe a, or_fe b);                 /* :49-78 */
+or_fe or_inv(or_fe a);                          /* :207-210 (x^(p^2-2)) */
+or_fe or_pow(or_fe a, unsigned __int128 e);     /* :323-335 */
+or_fe or_root_of_unity(int logn);               /* src/utils.cpp:706-717 */
+or_fe or_from_u64(uint64_t x);                  /* F(long long), :24-27 */
+
+/* array wrappers (for ctypes), element arrays as u64[2n] */
+void or_vec_mul(const or_fe *a, const or_fe *b, or_fe *out, uint64_t n);
+void or_vec_add(const or_fe *a, const or_fe *b, or_fe *out, uint64_t n);
+void or_vec_sub(const or_fe *a, const or_fe *b, or_fe *out, uint64_t n);
+void or_vec_inv(const or_fe *a, or_fe *out, uint64_t n);
+
+/* ---- BLAKE3 of one 64-byte block, src/Blake3_hash.cpp:5-9 ---- */
+void or_blake3_hash64(const uint8_t src[64], uint8_t dst[32]);
+/* general single-block BLAKE3 (len <= 64), for known-answer checks */
+void or_blake3_small(const uint8_t *src, size_t len, uint8_t dst[32]);
+void or_blake3_hash64_many(const uint8_t *src, uint8_t *dst, uint64_t n);
+
+/* ---- Fiat-Shamir, src/Fiat_Shamir.cpp ---- */
+or_fe or_hash_F(or_fe x, or_fe y);                              /* :8-27 */
+/* absorbs elems[0..n) into *prev (the reference's prev_challenge) */
+or_fe or_hash_to_field(or_fe *prev, const or_fe *elems, uint64_t n); /* :29-34 */
+/* writes l (N-index, M-index) pairs into idx[2l] */
+void or_get_indexes(or_fe *prev, int l, uint32_t N, uint32_t M, uint32_t *idx); /* :37-56 */
+
+/* ---- NTT, src/utils.cpp ---- */
+void or_fft(or_fe *arr, int logn, int inverse);     /* fft :904-964 */
+void or_pfft(or_fe *arr, int logn);                 /* pfft :1583-1673 (forward only) */
+/* Whir_Commit_initial's codeword: (arr || 0) zero-padded to rate*n, forward
+ * transform.  src/Whir.cpp:158-163 */
+void or_rs_encode(const or_fe *arr, uint64_t n, int rate, or_fe *codeword);
+
+/* ---- Merkle tree, src/merkle_tree.cpp ---- */
+/* number of 32-byte nodes over all levels for a commit of n_elems elements */
+uint64_t or_merkle_node_count(uint64_t n_elems);
+/* MT_commit_Blake (:195-233) + create_tree_blake (:267-299); levels stored
+ * back to back, level 0 first; root is the last node.  n_elems must be a
+ * power of two >= 4.  dup_left=1 is the reference's behaviour (the parent
+ * hashes its left child twice, :287-292); dup_left=0 hashes left||right. */
+void or_merkle_commit(const or_fe *elems, uint64_t n_elems, uint8_t *nodes, int dup_left);
+/* open_tree_blake (:320-335): sibling path, one digest per level below root */
+int or_merkle_open(const uint8_t *nodes, uint64_t n_elems, uint64_t pos, uint8_t *path);
+
+/* ---- multilinear helpers, src/utils.cpp ---- */
+void or_precompute_beta(const or_fe *r, int m, or_fe *B);        /* :198-243 */
+or_fe or_evaluate_vector(const or_fe *v, uint64_t n, const or_fe *r); /* :1201-1214 */
+void or_compute_lagrange_coeff(or_fe r, int degree, or_fe *L);   /* :169-197 */
+
+/* ---- glibc challenge stream, src/fieldElement.cpp:119-125,363-368 ---- */
+or_fe or_random_element(void);   /* F::_random(): uses libc random() */
+
+/* ---- product sumcheck, src/Distributed_Sumcheck.cpp:204-348 at one rank,
+ *      tail src/sparse_eval.cpp:140-192 ----
+ * v1, v2 (length n, power of two) are folded in place as the reference does.
+ * offset = number of trailing rounds that draw F::_random() challenges
+ * (the reference's sumcheck_offset); the leading rounds use the Fiat-Shamir
+ * chain *prev.  polys[3*round + {0,1,2}] = (a,b,c) of a x^2 + b x + c;
+ * r[round] = challenge.  final_out = {v1[0], v2[0]}.
+ * Returns 0, or -(round+1) if H(0)+H(1) != y (the reference exits there). */
+int or_product_sumcheck(or_fe y, or_fe *v1, or_fe *v2, uint64_t n, or_fe *prev,
+                        int offset, or_fe *polys, or_fe *r, or_fe *final_out);
+
+/* ---- synthetic inputs: splitmix64-seeded xoshiro256**, 61-bit masked,
+ *      value p rejected (workload definition, not a reference function) ---- */
+void or_synth_fill(uint64_t seed, uint64_t stream, or_fe *out, uint64_t n);
+
+/* ---- WHIR, src/Whir.cpp ---- */
+int or_whir_query_count(int rate);                                  /* :253-258 */
+or_fe or_rs_multifold_segment_tensor(const or_fe *seg, int seg_n, const or_fe *r,
+                                     int nr, uint64_t n, uint64_t idx); /* coPCS_utils.cpp:333-383 */
+/* Whir_Open (:388-467) with the iteration loop of Whir_Iteration (:354-386),
+ * whir_k = 4.  arr (n), codeword (rate*n, from or_rs_encode) and its Merkle
+ * nodes describe the initial commitment; z (log2 n coordinates) is the
+ * single claim point and K the claim.  Draws challenges from libc random()
+ * (the caller seeds it).  Writes a flat record; see oracle.c for layout.
+ * Returns the number of u64 words written, or a negative error. */
+int64_t or_whir_open(const or_fe *arr, uint64_t n, const or_fe *z, or_fe K,
+                     int rate, int stop_log, uint64_t *record, uint64_t record_cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,270 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so) and of oracle/_ref.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker.  Element arrays are numpy
+uint64 of shape (n, 2) = (real, img), the reference's element layout.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libref.so")
+
+P = (1 << 61) - 1
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class _Fe(ctypes.Structure):
+    _fields_ = [("real", ctypes.c_uint64), ("img", ctypes.c_uint64)]
+
+
+def fe(x):
+    return _Fe(int(x[0]), int(x[1]))
+
+
+def unfe(s):
+    return np.array([s.real, s.img], dtype=np.uint64)
+
+
+_o = None
+_r = None
+_libc = None
+
+
+def build():
+    """Compile liboracle.so (and _ref when the reference tree is present)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def oracle():
+    global _o
+    if _o is None:
+        if not os.path.exists(ORACLE_SO):
+            build()
+        L = ctypes.CDLL(ORACLE_SO)
+        vp, u64, i32 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int
+        for f in ("or_add", "or_sub", "or_mul"):
+            getattr(L, f).argtypes = [_Fe, _Fe]
+            getattr(L, f).restype = _Fe
+        L.or_inv.argtypes = [_Fe]
+        L.or_inv.restype = _Fe
+        L.or_root_of_unity.argtypes = [i32]
+        L.or_root_of_unity.restype = _Fe
+        for f in ("or_vec_mul", "or_vec_add", "or_vec_sub"):
+            getattr(L, f).argtypes = [vp, vp, vp, u64]
+        L.or_vec_inv.argtypes = [vp, vp, u64]
+        L.or_blake3_hash64.argtypes = [vp, vp]
+        L.or_blake3_small.argtypes = [vp, ctypes.c_size_t, vp]
+        L.or_blake3_hash64_many.argtypes = [vp, vp, u64]
+        L.or_hash_F.argtypes = [_Fe, _Fe]
+        L.or_hash_F.restype = _Fe
+        L.or_hash_to_field.argtypes = [vp, vp, u64]
+        L.or_hash_to_field.restype = _Fe
+        L.or_get_indexes.argtypes = [vp, i32, ctypes.c_uint32, ctypes.c_uint32, vp]
+        L.or_fft.argtypes = [vp, i32, i32]
+        L.or_pfft.argtypes = [vp, i32]
+        L.or_rs_encode.argtypes = [vp, u64, i32, vp]
+        L.or_merkle_node_count.argtypes = [u64]
+        L.or_merkle_node_count.restype = u64
+        L.or_merkle_commit.argtypes = [vp, u64, vp, i32]
+        L.or_merkle_open.argtypes = [vp, u64, u64, vp]
+        L.or_merkle_open.restype = i32
+        L.or_precompute_beta.argtypes = [vp, i32, vp]
+        L.or_evaluate_vector.argtypes = [vp, u64, vp]
+        L.or_evaluate_vector.restype = _Fe
+        L.or_compute_lagrange_coeff.argtypes = [_Fe, i32, vp]
+        L.or_random_element.restype = _Fe
+        L.or_product_sumcheck.argtypes = [_Fe, vp, vp, u64, vp, i32, vp, vp, vp]
+        L.or_product_sumcheck.restype = i32
+        L.or_synth_fill.argtypes = [u64, u64, vp, u64]
+        L.or_whir_query_count.argtypes = [i32]
+        L.or_whir_query_count.restype = i32
+        L.or_rs_multifold_segment_tensor.argtypes = [vp, i32, vp, i32, u64, u64]
+        L.or_rs_multifold_segment_tensor.restype = _Fe
+        L.or_whir_open.argtypes = [vp, u64, vp, _Fe, i32, i32, vp, u64]
+        L.or_whir_open.restype = ctypes.c_int64
+        _o = L
+    return _o
+
+
+def ref_available():
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    """The reference's own compiled field / BLAKE3 / polynomial code."""
+    global _r
+    if _r is None:
+        if not os.path.exists(REF_SO):
+            build()
+        if not os.path.exists(REF_SO):
+            return None
+        L = ctypes.CDLL(REF_SO)
+        vp, u64, i32 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int
+        L.ref_fe_layout.argtypes = [vp]
+        for f in ("ref_mul", "ref_add", "ref_sub"):
+            getattr(L, f).argtypes = [vp, vp, vp, u64]
+        L.ref_inv.argtypes = [vp, vp, u64]
+        L.ref_pow.argtypes = [vp, u64, u64, vp]
+        L.ref_root_of_unity.argtypes = [i32, vp]
+        L.ref_from_ll.argtypes = [ctypes.c_longlong, vp]
+        L.ref_random_elements.argtypes = [vp, u64]
+        L.ref_blake3_hash64.argtypes = [vp, vp]
+        L.ref_blake3_any.argtypes = [vp, ctypes.c_size_t, vp]
+        L.ref_linear_mul.argtypes = [vp, vp, vp]
+        L.ref_quadratic_eval.argtypes = [vp, vp, vp]
+        _r = L
+    return _r
+
+
+def libc():
+    global _libc
+    if _libc is None:
+        _libc = ctypes.CDLL("libc.so.6")
+        _libc.srandom.argtypes = [ctypes.c_uint]
+        _libc.srand.argtypes = [ctypes.c_uint]
+    return _libc
+
+
+# ---- convenience wrappers -------------------------------------------------
+
+def fe_array(n):
+    return np.zeros((n, 2), dtype=np.uint64)
+
+
+def synth(seed, stream, n):
+    out = fe_array(n)
+    oracle().or_synth_fill(seed, stream, _p(out), n)
+    return out
+
+
+def vec_op(name, a, b=None):
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    out = np.zeros_like(a)
+    if b is None:
+        getattr(oracle(), "or_vec_" + name)(_p(a), _p(out), a.shape[0])
+    else:
+        b = np.ascontiguousarray(b, dtype=np.uint64)
+        getattr(oracle(), "or_vec_" + name)(_p(a), _p(b), _p(out), a.shape[0])
+    return out
+
+
+def root_of_unity(logn):
+    return unfe(oracle().or_root_of_unity(logn))
+
+
+def blake3_64(block):
+    b = np.ascontiguousarray(np.frombuffer(bytes(block), dtype=np.uint8))
+    out = np.zeros(32, np.uint8)
+    oracle().or_blake3_hash64(_p(b), _p(out))
+    return out.tobytes()
+
+
+def blake3_small(data: bytes):
+    b = np.frombuffer(bytes(data) + b"\0", dtype=np.uint8).copy()
+    out = np.zeros(32, np.uint8)
+    oracle().or_blake3_small(_p(b), len(data), _p(out))
+    return out.tobytes()
+
+
+def hash_F(x, y):
+    return unfe(oracle().or_hash_F(fe(x), fe(y)))
+
+
+def hash_to_field(prev, elems):
+    """prev: array(2) state, updated in place; returns the new value."""
+    st = np.ascontiguousarray(prev, dtype=np.uint64)
+    e = np.ascontiguousarray(np.asarray(elems, dtype=np.uint64).reshape(-1, 2))
+    r = oracle().or_hash_to_field(_p(st), _p(e), e.shape[0])
+    prev[...] = st
+    return unfe(r)
+
+
+def fft(arr, inverse=False):
+    a = np.ascontiguousarray(arr, dtype=np.uint64).copy()
+    logn = a.shape[0].bit_length() - 1
+    oracle().or_fft(_p(a), logn, int(inverse))
+    return a
+
+
+def rs_encode(arr, rate):
+    a = np.ascontiguousarray(arr, dtype=np.uint64)
+    out = fe_array(a.shape[0] * rate)
+    oracle().or_rs_encode(_p(a), a.shape[0], rate, _p(out))
+    return out
+
+
+def merkle_commit(elems, dup_left=True):
+    a = np.ascontiguousarray(elems, dtype=np.uint64)
+    n = a.shape[0]
+    cnt = int(oracle().or_merkle_node_count(n))
+    nodes = np.zeros((cnt, 32), np.uint8)
+    oracle().or_merkle_commit(_p(a), n, _p(nodes), int(dup_left))
+    return nodes
+
+
+def merkle_open(nodes, n_elems, pos):
+    path = np.zeros((64, 32), np.uint8)
+    d = oracle().or_merkle_open(_p(np.ascontiguousarray(nodes)), n_elems, pos, _p(path))
+    if d < 0:
+        raise ValueError("position out of range")
+    return path[:d].copy()
+
+
+def precompute_beta(r):
+    r = np.ascontiguousarray(r, dtype=np.uint64)
+    m = r.shape[0]
+    B = fe_array(1 << m)
+    oracle().or_precompute_beta(_p(r), m, _p(B))
+    return B
+
+
+def evaluate_vector(v, r):
+    v = np.ascontiguousarray(v, dtype=np.uint64)
+    r = np.ascontiguousarray(r, dtype=np.uint64)
+    return unfe(oracle().or_evaluate_vector(_p(v), v.shape[0], _p(r)))
+
+
+def product_sumcheck(y, v1, v2, prev, offset=0):
+    """Returns (status, polys (m,3,2), r (m,2), final (2,2)); prev updated."""
+    a = np.ascontiguousarray(v1, dtype=np.uint64).copy()
+    b = np.ascontiguousarray(v2, dtype=np.uint64).copy()
+    n = a.shape[0]
+    m = n.bit_length() - 1
+    st = np.ascontiguousarray(prev, dtype=np.uint64)
+    polys = np.zeros((m, 3, 2), np.uint64)
+    r = np.zeros((max(m, 1), 2), np.uint64)
+    fin = np.zeros((2, 2), np.uint64)
+    rc = oracle().or_product_sumcheck(fe(y), _p(a), _p(b), n, _p(st), offset, _p(polys), _p(r), _p(fin))
+    prev[...] = st
+    return rc, polys, r[:m], fin
+
+
+def inner_product(a, b):
+    """sum a[i]*b[i] in the field (Python big ints, for claims)."""
+    prod = vec_op("mul", a, b)
+    re = int(np.sum(prod[:, 0].astype(object)) % P)
+    im = int(np.sum(prod[:, 1].astype(object)) % P)
+    return np.array([re, im], dtype=np.uint64)
+
+
+def plain_proof(f, g, rate, prev=None):
+    """The plain single-rank prover's proof bytes: Merkle root of the
+    rate-blowup codeword of f | per-round (a,b,c) | final v1[0], v2[0]."""
+    if prev is None:
+        prev = np.zeros(2, np.uint64)
+    cw = rs_encode(f, rate)
+    nodes = merkle_commit(cw, True)
+    y = inner_product(f, g)
+    rc, polys, r, fin = product_sumcheck(y, f, g, prev, 0)
+    assert rc == 0
+    return nodes[-1].tobytes() + polys.tobytes() + fin.tobytes(), y, r

@@ -1,0 +1,251 @@
+"""ctypes binding of libcosnark.so (the C ABI in include/cosnark.h).
+
+Host arrays are numpy uint64 arrays of shape (n, 2) -- one row per field
+element, columns (real, img) -- the byte layout of the reference's
+virgo::fieldElement (src/fieldElement.hpp:96-97).  Digests are uint8 (k, 32).
+
+The library is required: importing this module on a machine without the
+built libcosnark.so raises, there is no fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcosnark.so")
+
+P = (1 << 61) - 1
+
+E_OK, E_ARG, E_HIP, E_NOMEM, E_CHECK, E_RANGE = 0, -1, -2, -3, -4, -5
+
+
+class CosnarkError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"cosnark error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libcosnark.so not built at {LIB_PATH}; run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, u64, i32, u32 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_uint32
+        L.cosnark_create.argtypes = [i32, ctypes.POINTER(vp)]
+        L.cosnark_destroy.argtypes = [vp]
+        L.cosnark_destroy.restype = None
+        L.cosnark_last_error.argtypes = [vp]
+        L.cosnark_last_error.restype = ctypes.c_char_p
+        L.cosnark_sync.argtypes = [vp]
+        L.cosnark_stream.argtypes = [vp]
+        L.cosnark_stream.restype = vp
+        L.cosnark_version.restype = ctypes.c_char_p
+        L.cosnark_dev_alloc.argtypes = [vp, u64, ctypes.POINTER(vp)]
+        L.cosnark_dev_free.argtypes = [vp, vp]
+        for f in ("cosnark_h2d", "cosnark_d2h", "cosnark_d2d"):
+            getattr(L, f).argtypes = [vp, vp, vp, u64]
+        L.cosnark_synth_fill.argtypes = [u64, u64, vp, u64]
+        L.cosnark_synth_fill.restype = None
+        L.cosnark_ntt.argtypes = [vp, vp, i32, i32]
+        L.cosnark_rs_encode.argtypes = [vp, vp, u64, i32, vp]
+        L.cosnark_merkle_node_count.argtypes = [u64]
+        L.cosnark_merkle_node_count.restype = u64
+        L.cosnark_merkle_commit.argtypes = [vp, vp, u64, vp, i32]
+        L.cosnark_merkle_open.argtypes = [vp, vp, u64, u64, vp, ctypes.POINTER(i32)]
+        L.cosnark_hash_to_field.argtypes = [vp, vp, u64, vp]
+        L.cosnark_get_indexes.argtypes = [vp, i32, u32, u32, vp]
+        L.cosnark_product_sumcheck.argtypes = [vp, vp, vp, u64, vp, vp, i32, vp, vp, vp]
+        L.cosnark_plain_proof_bytes.argtypes = [u64]
+        L.cosnark_plain_proof_bytes.restype = u64
+        L.cosnark_prove_plain.argtypes = [vp, vp, vp, u64, i32, vp, vp, vp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def fe_array(n):
+    return np.zeros((n, 2), dtype=np.uint64)
+
+
+def synth(seed, stream, n):
+    """n uniform GF(p^2) elements of the workload generator."""
+    out = fe_array(n)
+    lib().cosnark_synth_fill(seed, stream, _p(out), n)
+    return out
+
+
+class Transcript(ctypes.Structure):
+    """The reference's running prev_challenge (src/Fiat_Shamir.cpp:5)."""
+    _fields_ = [("prev", ctypes.c_uint64 * 2)]
+
+    def value(self):
+        return (int(self.prev[0]), int(self.prev[1]))
+
+
+def hash_to_field(ts: Transcript, elems):
+    e = np.ascontiguousarray(np.asarray(elems, dtype=np.uint64).reshape(-1, 2))
+    out = np.zeros(2, np.uint64)
+    rc = lib().cosnark_hash_to_field(ctypes.byref(ts), _p(e), e.shape[0], _p(out))
+    if rc:
+        raise CosnarkError(rc, "hash_to_field")
+    return out
+
+
+def get_indexes(ts: Transcript, l, N, M):
+    idx = np.zeros((l, 2), np.uint32)
+    rc = lib().cosnark_get_indexes(ctypes.byref(ts), l, N, M, _p(idx))
+    if rc:
+        raise CosnarkError(rc, "get_indexes")
+    return idx
+
+
+def merkle_node_count(n_elems):
+    return int(lib().cosnark_merkle_node_count(n_elems))
+
+
+class DevBuf:
+    """A device allocation owned by a Context."""
+
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        ctx._ck(lib().cosnark_dev_alloc(ctx.h, self.nbytes, ctypes.byref(p)))
+        self.ptr = p
+        ctx._bufs.add(self)
+
+    def free(self):
+        if self.ptr and self.ctx.h:
+            lib().cosnark_dev_free(self.ctx.h, self.ptr)
+        self.ptr = None
+        self.ctx._bufs.discard(self)
+
+    def __hash__(self):
+        return id(self)
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        rc = lib().cosnark_create(device, ctypes.byref(h))
+        if rc:
+            raise CosnarkError(rc, "cosnark_create failed (no GPU?)")
+        self.h = h
+        self._bufs = set()
+
+    def close(self):
+        if self.h:
+            for b in list(self._bufs):
+                b.free()
+            lib().cosnark_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _ck(self, rc):
+        if rc:
+            raise CosnarkError(rc, lib().cosnark_last_error(self.h).decode())
+        return rc
+
+    # ---- memory ----
+    def alloc(self, nbytes):
+        return DevBuf(self, nbytes)
+
+    def upload(self, arr):
+        a = np.ascontiguousarray(arr)
+        b = DevBuf(self, a.nbytes)
+        self._ck(lib().cosnark_h2d(self.h, b.ptr, _p(a), a.nbytes))
+        return b
+
+    def download(self, buf, shape, dtype=np.uint64):
+        out = np.zeros(shape, dtype=dtype)
+        self._ck(lib().cosnark_d2h(self.h, _p(out), buf.ptr, out.nbytes))
+        return out
+
+    def sync(self):
+        self._ck(lib().cosnark_sync(self.h))
+
+    # ---- kernels ----
+    def ntt(self, arr, inverse=False):
+        a = np.ascontiguousarray(arr, dtype=np.uint64)
+        n = a.shape[0]
+        logn = n.bit_length() - 1
+        b = self.upload(a)
+        self._ck(lib().cosnark_ntt(self.h, b.ptr, logn, int(inverse)))
+        return self.download(b, a.shape)
+
+    def rs_encode(self, coeffs, rate):
+        a = np.ascontiguousarray(coeffs, dtype=np.uint64)
+        n = a.shape[0]
+        b = self.upload(a)
+        cw = self.alloc(16 * n * rate)
+        self._ck(lib().cosnark_rs_encode(self.h, b.ptr, n, rate, cw.ptr))
+        return self.download(cw, (n * rate, 2))
+
+    def merkle_commit(self, elems, dup_left=True, keep_device=False):
+        a = np.ascontiguousarray(elems, dtype=np.uint64)
+        n = a.shape[0]
+        b = self.upload(a)
+        cnt = merkle_node_count(n)
+        nodes = self.alloc(32 * cnt)
+        self._ck(lib().cosnark_merkle_commit(self.h, b.ptr, n, nodes.ptr, int(dup_left)))
+        host = self.download(nodes, (cnt, 32), np.uint8)
+        if keep_device:
+            return host, nodes
+        return host
+
+    def merkle_open(self, nodes_dev, n_elems, pos):
+        path = np.zeros((64, 32), np.uint8)
+        depth = ctypes.c_int()
+        self._ck(lib().cosnark_merkle_open(self.h, nodes_dev.ptr, n_elems, pos, _p(path), ctypes.byref(depth)))
+        return path[: depth.value].copy()
+
+    def product_sumcheck(self, v1, v2, y, ts: Transcript, offset=0):
+        a = np.ascontiguousarray(v1, dtype=np.uint64)
+        b = np.ascontiguousarray(v2, dtype=np.uint64)
+        n = a.shape[0]
+        m = n.bit_length() - 1
+        d1, d2 = self.upload(a), self.upload(b)
+        yy = np.asarray(y, dtype=np.uint64).reshape(2)
+        polys = np.zeros((m, 3, 2), np.uint64)
+        r = np.zeros((m, 2), np.uint64)
+        fin = np.zeros((2, 2), np.uint64)
+        rc = lib().cosnark_product_sumcheck(self.h, d1.ptr, d2.ptr, n, _p(yy), ctypes.byref(ts), offset,
+                                            _p(polys), _p(r), _p(fin))
+        return rc, polys, r, fin
+
+    def prove_plain(self, f, g, rate, ts: Transcript, dev_inputs=None):
+        """Returns (proof bytes, stage times ms [encode, merkle, sumcheck, total])."""
+        if dev_inputs is None:
+            a = np.ascontiguousarray(f, dtype=np.uint64)
+            b = np.ascontiguousarray(g, dtype=np.uint64)
+            n = a.shape[0]
+            df, dg = self.upload(a), self.upload(b)
+        else:
+            df, dg, n = dev_inputs
+        nb = int(lib().cosnark_plain_proof_bytes(n))
+        proof = np.zeros(nb, np.uint8)
+        times = np.zeros(4, np.float64)
+        self._ck(lib().cosnark_prove_plain(self.h, df.ptr, dg.ptr, n, rate, ctypes.byref(ts), _p(proof),
+                                           _p(times)))
+        return proof, times

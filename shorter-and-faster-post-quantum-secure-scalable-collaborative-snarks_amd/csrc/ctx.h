@@ -1,0 +1,81 @@
+// ctx.h -- library-internal context and launch helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include "gf.h"
+#include "cosnark.h"
+
+// Largest transform the twiddle tables cover: 2^28 points.
+constexpr int NTT_LOG_MAX = 28;
+constexpr int NTT_TW_BITS = 14;                 // two-level table split
+constexpr int NTT_TW_SIZE = 1 << NTT_TW_BITS;
+constexpr int NTT_LDS_ELEMS = 4096;             // elements per workgroup tile
+constexpr int NTT_THREADS = 256;
+constexpr int NTT_RADIX_LOG_MAX = 10;           // radix of one LDS pass <= 1024
+
+// Product-sumcheck device state (one per context).
+struct ScState {
+    gf::fe prev;          // transcript chain value (prev_challenge)
+    gf::fe y;             // current claim
+    uint32_t counter;     // last-workgroup-done counter (reset by the last WG)
+    int32_t err;          // 0, or round+1 of the first failed H(0)+H(1)=y check
+    int32_t derive_y;     // 1: take y = H(0)+H(1) of round 0 (claim computed here)
+    int32_t pad;
+    gf::fe polys[64 * 3];
+    gf::fe r[64];
+    gf::fe ext_r[64];     // externally drawn challenges (libc stream rounds)
+    gf::fe final_v[2];
+};
+
+struct cosnark_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // twiddle tables (device): omega_{2^28}^(e & mask), omega_{2^28}^((e >> 14) << 14),
+    // and the same for the inverse root; radix tables omega_1024^j (j < 512).
+    gf::fe *tw_lo = nullptr, *tw_hi = nullptr, *itw_lo = nullptr, *itw_hi = nullptr;
+    gf::fe *rtw = nullptr, *irtw = nullptr;
+    // growable scratch arena
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
+    ScState *sc = nullptr;            // device
+    gf::fe *sc_partials = nullptr;    // device, 3 * SC_MAX_BLOCKS elements
+    hipEvent_t ev[8] = {};
+};
+
+constexpr int SC_MAX_BLOCKS = 1024;
+
+int set_err(cosnark_ctx *ctx, int code, const std::string &msg);
+int hip_err(cosnark_ctx *ctx, hipError_t e, const char *what);
+// ensure the scratch arena holds at least `bytes`; returns base pointer
+int scratch_reserve(cosnark_ctx *ctx, size_t bytes, void **base);
+
+#define HIPCK(expr)                                              \
+    do {                                                         \
+        hipError_t e_ = (expr);                                  \
+        if (e_ != hipSuccess) return hip_err(ctx, e_, #expr);    \
+    } while (0)
+
+#define LAUNCHCK()                                                         \
+    do {                                                                   \
+        hipError_t e_ = hipGetLastError();                                 \
+        if (e_ != hipSuccess) return hip_err(ctx, e_, "kernel launch");    \
+    } while (0)
+
+inline bool is_pow2(uint64_t x) { return x && !(x & (x - 1)); }
+inline int ilog2(uint64_t x) { int l = 0; while ((1ull << l) < x) ++l; return l; }
+
+// ---- internal asynchronous launchers (work on ctx->stream) ----
+// NTT of 2^logn points: out = DFT(in || 0) where in has in_len <= 2^logn
+// elements (in may equal out when in_len == 2^logn).
+int ntt_launch(cosnark_ctx *ctx, const gf::fe *in, uint64_t in_len, gf::fe *out, int logn,
+               bool inverse);
+// Merkle commit of n_elems elements into nodes (all levels).
+int merkle_launch(cosnark_ctx *ctx, const gf::fe *elems, uint64_t n_elems, uint8_t *nodes,
+                  bool dup_left);
+int sumcheck_init(cosnark_ctx *ctx);
+// Product sumcheck rounds; state in ctx->sc (initialised by the caller).
+// scratch buffers: a1,a2,b1,b2 each of n/2 elements.
+int sumcheck_launch(cosnark_ctx *ctx, const gf::fe *v1, const gf::fe *v2, uint64_t n, int fs_rounds,
+                    gf::fe *buf);

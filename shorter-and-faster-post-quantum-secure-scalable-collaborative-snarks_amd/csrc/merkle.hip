@@ -1,0 +1,130 @@
+// merkle.hip -- BLAKE3 Merkle commit (MT_commit_Blake + create_tree_blake).
+//
+// Reference: src/merkle_tree.cpp:195-233 (leaf i = BLAKE3 of elements
+// 4i..4i+3, 64 raw bytes) and :267-299 (level l node i = BLAKE3 of
+// level l-1 nodes 2i and 2i -- the left child twice, :287-292).  Every level
+// is materialised because openings (open_tree_blake, :320-335) serve
+// siblings from all levels.
+//
+// One message per lane.  A workgroup of 256 lanes hashes a 1024-node block
+// of its first level (4 per lane, coalesced 64-byte loads), keeps the
+// digests in LDS and climbs the tree inside the block (10 more levels),
+// writing every node; the block roots feed the next launch.  A 2^22-leaf
+// tree takes three launches.
+#include "ctx.h"
+#include "blake3.h"
+
+using gf::fe;
+
+namespace {
+
+constexpr int MK_THREADS = 256;
+constexpr int MK_BLOCK = 1024;      // first-level nodes per workgroup
+
+struct LevelOffsets {
+    uint64_t off[40];               // digest offset of each level in `nodes`
+};
+
+struct Digest {
+    uint32_t w[8];
+};
+
+__device__ __forceinline__ void store_digest(uint8_t *nodes, uint64_t idx, const uint32_t h[8]) {
+    uint4 *p = reinterpret_cast<uint4 *>(nodes + 32 * idx);
+    p[0] = make_uint4(h[0], h[1], h[2], h[3]);
+    p[1] = make_uint4(h[4], h[5], h[6], h[7]);
+}
+
+// FROM_ELEMS: the first level is the leaf level (input = elements).
+// Otherwise the first level is level `lvl0` computed from level lvl0-1.
+template <bool FROM_ELEMS>
+__global__ __launch_bounds__(MK_THREADS) void merkle_block_kernel(
+    const fe *__restrict__ elems, const uint8_t *in_nodes, uint8_t *nodes,
+    LevelOffsets lo, int lvl0, int block_log, int dup_left) {
+    __shared__ Digest bufA[MK_BLOCK];
+    __shared__ Digest bufB[MK_BLOCK / 2];
+    const int tid = threadIdx.x;
+    const int T = 1 << block_log;                       // first-level nodes in this block
+    const uint64_t g0 = (uint64_t)blockIdx.x << block_log;
+
+    for (int i = tid; i < T; i += MK_THREADS) {
+        const uint64_t g = g0 + i;
+        uint32_t m[16], h[8];
+        if (FROM_ELEMS) {
+            const uint4 *src = reinterpret_cast<const uint4 *>(elems + 4 * g);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint4 q = src[j];
+                m[4 * j] = q.x; m[4 * j + 1] = q.y; m[4 * j + 2] = q.z; m[4 * j + 3] = q.w;
+            }
+        } else {
+            const uint4 *L = reinterpret_cast<const uint4 *>(in_nodes + 32 * (2 * g));
+            const uint4 *Rn = reinterpret_cast<const uint4 *>(in_nodes + 32 * (2 * g + (dup_left ? 0 : 1)));
+            const uint4 l0 = L[0], l1 = L[1], r0 = Rn[0], r1 = Rn[1];
+            m[0] = l0.x; m[1] = l0.y; m[2] = l0.z; m[3] = l0.w;
+            m[4] = l1.x; m[5] = l1.y; m[6] = l1.z; m[7] = l1.w;
+            m[8] = r0.x; m[9] = r0.y; m[10] = r0.z; m[11] = r0.w;
+            m[12] = r1.x; m[13] = r1.y; m[14] = r1.z; m[15] = r1.w;
+        }
+        b3::compress64(m, h);
+        store_digest(nodes, lo.off[lvl0] + g, h);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) bufA[i].w[j] = h[j];
+    }
+
+    // climb inside the block: level lvl0 + s has T >> s nodes
+    Digest *src = bufA, *dst = bufB;
+    for (int s = 1; s <= block_log; ++s) {
+        __syncthreads();
+        const int cnt = T >> s;
+        const uint64_t gbase = g0 >> s;
+        for (int i = tid; i < cnt; i += MK_THREADS) {
+            uint32_t m[16], h[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) m[j] = src[2 * i].w[j];
+            const int ri = dup_left ? 2 * i : 2 * i + 1;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) m[8 + j] = src[ri].w[j];
+            b3::compress64(m, h);
+            store_digest(nodes, lo.off[lvl0 + s] + gbase + i, h);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) dst[i].w[j] = h[j];
+        }
+        Digest *t = src; src = dst; dst = t;
+    }
+}
+
+}  // namespace
+
+int merkle_launch(cosnark_ctx *ctx, const fe *elems, uint64_t n_elems, uint8_t *nodes, bool dup_left) {
+    if (!is_pow2(n_elems) || n_elems < 8 || n_elems > (1ull << 30))
+        return set_err(ctx, COSNARK_E_ARG, "merkle: n_elems must be a power of two in [8, 2^30]");
+    const uint64_t leaves = n_elems / 4;
+    const int depth = ilog2(leaves);               // root level index
+    LevelOffsets lo{};
+    uint64_t off = 0;
+    for (int l = 0; l <= depth; ++l) {
+        lo.off[l] = off;
+        off += leaves >> l;
+    }
+    // leaf level + in-block levels
+    int block_log = ilog2(leaves < (uint64_t)MK_BLOCK ? leaves : (uint64_t)MK_BLOCK);
+    uint64_t blocks = leaves >> block_log;
+    hipLaunchKernelGGL(merkle_block_kernel<true>, dim3((unsigned)blocks), dim3(MK_THREADS), 0, ctx->stream,
+                       elems, (const uint8_t *)nullptr, nodes, lo, 0, block_log, (int)dup_left);
+    LAUNCHCK();
+    int top = block_log;                              // highest level written
+    uint64_t count = blocks;                          // nodes at level `top`
+    while (count > 1) {
+        const uint64_t next = count / 2;              // nodes at level top+1
+        int bl = ilog2(next < (uint64_t)MK_BLOCK ? next : (uint64_t)MK_BLOCK);
+        const uint64_t nb = next >> bl;
+        hipLaunchKernelGGL(merkle_block_kernel<false>, dim3((unsigned)nb), dim3(MK_THREADS), 0, ctx->stream,
+                           (const fe *)nullptr, nodes + 32 * lo.off[top], nodes, lo, top + 1, bl,
+                           (int)dup_left);
+        LAUNCHCK();
+        top += 1 + bl;
+        count = nb;
+    }
+    return COSNARK_OK;
+}

@@ -1,0 +1,36 @@
+"""NTT / Reed-Solomon encode on the GPU against the oracle (fft, pfft,
+Whir_Commit_initial codeword: src/utils.cpp:904-964, src/Whir.cpp:158-163)."""
+import numpy as np
+import pytest
+
+import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("logn", [1, 2, 3, 5, 8, 10, 11, 12, 13, 16, 17, 20])
+def test_ntt_forward_matches_oracle(ctx, logn):
+    a = O.synth(0xA0 + logn, 0, 1 << logn)
+    print(f"ntt forward logn={logn}", flush=True)
+    assert np.array_equal(ctx.ntt(a), O.fft(a))
+
+
+@pytest.mark.parametrize("logn", [1, 4, 9, 12, 15, 18])
+def test_ntt_inverse_matches_oracle(ctx, logn):
+    a = O.synth(0xB0 + logn, 1, 1 << logn)
+    print(f"ntt inverse logn={logn}", flush=True)
+    assert np.array_equal(ctx.ntt(a, inverse=True), O.fft(a, inverse=True))
+
+
+@pytest.mark.parametrize("logn", [21, 24])
+def test_ntt_round_trip_large(ctx, logn):
+    a = O.synth(0xC0 + logn, 0, 1 << logn)
+    print(f"ntt round trip logn={logn}", flush=True)
+    assert np.array_equal(ctx.ntt(ctx.ntt(a), inverse=True), a)
+
+
+@pytest.mark.parametrize("logn,rate", [(1, 4), (3, 2), (6, 4), (10, 4), (14, 4), (16, 4), (12, 8), (13, 16)])
+def test_rs_encode_matches_oracle(ctx, logn, rate):
+    a = O.synth(0xD0 + logn, 0, 1 << logn)
+    print(f"rs_encode logn={logn} rate={rate}", flush=True)
+    assert np.array_equal(ctx.rs_encode(a, rate), O.rs_encode(a, rate))
