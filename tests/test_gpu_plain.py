@@ -19,3 +19,12 @@ def test_plain_proof_bytes_match_oracle(ctx, m):
     ts = cosnark.Transcript()
     proof, times = ctx.prove_plain(f, g, 4, ts)
     assert proof.tobytes() == exp
+
+
+def test_plain_proof_cfg1_matches_fixture(ctx):
+    import json
+    import os
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "oracle_vectors.json")))["plain_proof_cfg1"]
+    f, gg = O.synth(g["seed"], 0, g["n"]), O.synth(g["seed"], 1, g["n"])
+    proof, _ = ctx.prove_plain(f, gg, g["rate"], cosnark.Transcript())
+    assert proof.tobytes().hex() == g["proof_hex"]
