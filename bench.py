@@ -38,6 +38,7 @@ def parse():
     ap.add_argument("--log-n", type=int, default=22)
     ap.add_argument("--rate", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--serial", action="store_true", help="run the stages one after another (profiling)")
     return ap.parse_args()
 
 
@@ -61,6 +62,8 @@ def main():
     f = cosnark.synth(SEED_CFG2, 0, n)
     g = cosnark.synth(SEED_CFG2, 1, n)
     ctx = cosnark.Context(0)
+    if a.serial:
+        cosnark.lib().cosnark_set_overlap(ctx.h, 0)
     df, dg = ctx.upload(f), ctx.upload(g)
     stage = []
     proof = None

@@ -43,6 +43,9 @@ const char *cosnark_last_error(const cosnark_ctx *ctx);
 int  cosnark_sync(cosnark_ctx *ctx);
 /* the hipStream_t the context issues work on */
 void *cosnark_stream(cosnark_ctx *ctx);
+/* plain prover scheduling: 1 (default) runs the sumcheck on a second stream
+ * beside the commit, 0 runs the stages one after another (for profiling) */
+int  cosnark_set_overlap(cosnark_ctx *ctx, int on);
 /* library build string (arch, version) */
 const char *cosnark_version(void);
 

@@ -47,6 +47,7 @@ def lib():
         L.cosnark_stream.argtypes = [vp]
         L.cosnark_stream.restype = vp
         L.cosnark_version.restype = ctypes.c_char_p
+        L.cosnark_set_overlap.argtypes = [vp, i32]
         L.cosnark_dev_alloc.argtypes = [vp, u64, ctypes.POINTER(vp)]
         L.cosnark_dev_free.argtypes = [vp, vp]
         for f in ("cosnark_h2d", "cosnark_d2h", "cosnark_d2d"):

@@ -66,13 +66,18 @@ int cosnark_create(int device, cosnark_ctx **out) {
     auto fail = [&](int rc) { cosnark_destroy(ctx); return rc; };
     hipError_t e = hipSetDevice(device);
     if (e != hipSuccess) { delete ctx; return COSNARK_E_HIP; }
+    if (const char *t = getenv("COSNARK_NTT_TILE")) {
+        const int v = atoi(t);
+        if (v >= 256 && v <= NTT_LDS_ELEMS && is_pow2((uint64_t)v)) ctx->ntt_tile = v;
+    }
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return fail(COSNARK_E_HIP);
+    if (hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) != hipSuccess) return fail(COSNARK_E_HIP);
     for (auto &ev : ctx->ev)
         if (hipEventCreate(&ev) != hipSuccess) return fail(COSNARK_E_HIP);
 
     // twiddle tables: omega = root of order 2^28 (and its inverse)
     std::vector<fe> lo(NTT_TW_SIZE), hi(NTT_TW_SIZE), ilo(NTT_TW_SIZE), ihi(NTT_TW_SIZE);
-    std::vector<fe> rt(512), irt(512);
+    std::vector<fe> rt(1024), irt(1024);
     {
         const fe w = gf::root_of_unity(NTT_LOG_MAX), wi = gf::inv(w);
         const fe W = gf::pow(w, NTT_TW_SIZE), Wi = gf::pow(wi, NTT_TW_SIZE);
@@ -85,7 +90,7 @@ int cosnark_create(int device, cosnark_ctx **out) {
         }
         const fe r = gf::root_of_unity(NTT_RADIX_LOG_MAX), ri = gf::inv(r);
         rt[0] = irt[0] = gf::one();
-        for (int i = 1; i < 512; ++i) {
+        for (int i = 1; i < 1024; ++i) {
             rt[i] = gf::mul(rt[i - 1], r);
             irt[i] = gf::mul(irt[i - 1], ri);
         }
@@ -93,21 +98,23 @@ int cosnark_create(int device, cosnark_ctx **out) {
     const size_t tb = NTT_TW_SIZE * sizeof(fe);
     if (hipMalloc(&ctx->tw_lo, tb) != hipSuccess || hipMalloc(&ctx->tw_hi, tb) != hipSuccess ||
         hipMalloc(&ctx->itw_lo, tb) != hipSuccess || hipMalloc(&ctx->itw_hi, tb) != hipSuccess ||
-        hipMalloc(&ctx->rtw, 512 * sizeof(fe)) != hipSuccess ||
-        hipMalloc(&ctx->irtw, 512 * sizeof(fe)) != hipSuccess)
+        hipMalloc(&ctx->rtw, 1024 * sizeof(fe)) != hipSuccess ||
+        hipMalloc(&ctx->irtw, 1024 * sizeof(fe)) != hipSuccess)
         return fail(COSNARK_E_NOMEM);
     if (hipMemcpy(ctx->tw_lo, lo.data(), tb, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(ctx->tw_hi, hi.data(), tb, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(ctx->itw_lo, ilo.data(), tb, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(ctx->itw_hi, ihi.data(), tb, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(ctx->rtw, rt.data(), 512 * sizeof(fe), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(ctx->irtw, irt.data(), 512 * sizeof(fe), hipMemcpyHostToDevice) != hipSuccess)
+        hipMemcpy(ctx->rtw, rt.data(), 1024 * sizeof(fe), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(ctx->irtw, irt.data(), 1024 * sizeof(fe), hipMemcpyHostToDevice) != hipSuccess)
         return fail(COSNARK_E_HIP);
     if (hipMalloc(&ctx->sc, sizeof(ScState)) != hipSuccess ||
-        hipMalloc(&ctx->sc_partials, 3 * SC_MAX_BLOCKS * sizeof(fe)) != hipSuccess)
+        hipMalloc(&ctx->sc_partials, 2 * 3 * SC_MAX_BLOCKS * sizeof(fe)) != hipSuccess ||
+        hipHostMalloc(&ctx->sc_host, sizeof(ScState), hipHostMallocDefault) != hipSuccess)
         return fail(COSNARK_E_NOMEM);
     if (hipMemset(ctx->sc, 0, sizeof(ScState)) != hipSuccess) return fail(COSNARK_E_HIP);
     if (sumcheck_init(ctx) != COSNARK_OK) return fail(COSNARK_E_HIP);
+    if (ntt_init(ctx) != COSNARK_OK) return fail(COSNARK_E_HIP);
     if (hipDeviceSynchronize() != hipSuccess) return fail(COSNARK_E_HIP);
     *out = ctx;
     return COSNARK_OK;
@@ -121,12 +128,20 @@ void cosnark_destroy(cosnark_ctx *ctx) {
     (void)hipFree(ctx->tw_lo); (void)hipFree(ctx->tw_hi); (void)hipFree(ctx->itw_lo); (void)hipFree(ctx->itw_hi);
     (void)hipFree(ctx->rtw); (void)hipFree(ctx->irtw);
     (void)hipFree(ctx->sc); (void)hipFree(ctx->sc_partials);
+    if (ctx->sc_host) (void)hipHostFree(ctx->sc_host);
     if (ctx->scratch) (void)hipFree(ctx->scratch);
+    if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
 
 const char *cosnark_last_error(const cosnark_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int cosnark_set_overlap(cosnark_ctx *ctx, int on) {
+    if (!ctx) return COSNARK_E_ARG;
+    ctx->overlap = on != 0;
+    return COSNARK_OK;
+}
 
 int cosnark_sync(cosnark_ctx *ctx) {
     HIPCK(hipStreamSynchronize(ctx->stream));
@@ -204,7 +219,7 @@ void cosnark_synth_fill(uint64_t seed, uint64_t stream, uint64_t *out, uint64_t 
 int cosnark_ntt(cosnark_ctx *ctx, uint64_t *d_arr, int logn, int inverse) {
     if (!ctx || !d_arr) return set_err(ctx, COSNARK_E_ARG, "ntt: null argument");
     if (logn == 0) return COSNARK_OK;
-    int rc = ntt_launch(ctx, (const fe *)d_arr, 1ull << logn, (fe *)d_arr, logn, inverse != 0);
+    int rc = ntt_launch(ctx, (const fe *)d_arr, 1ull << logn, (fe *)d_arr, logn, inverse != 0, ctx->stream);
     if (rc) return rc;
     return cosnark_sync(ctx);
 }
@@ -215,7 +230,7 @@ int cosnark_rs_encode(cosnark_ctx *ctx, const uint64_t *d_coeffs, uint64_t n, in
         return set_err(ctx, COSNARK_E_ARG, "rs_encode: n and rate must be powers of two");
     const int logm = ilog2(n * (uint64_t)rate);
     if (logm == 0) return cosnark_d2d(ctx, d_cw, d_coeffs, 16);
-    int rc = ntt_launch(ctx, (const fe *)d_coeffs, n, (fe *)d_cw, logm, false);
+    int rc = ntt_launch(ctx, (const fe *)d_coeffs, n, (fe *)d_cw, logm, false, ctx->stream);
     if (rc) return rc;
     return cosnark_sync(ctx);
 }
@@ -233,7 +248,7 @@ uint64_t cosnark_merkle_node_count(uint64_t n_elems) {
 int cosnark_merkle_commit(cosnark_ctx *ctx, const uint64_t *d_elems, uint64_t n_elems, uint8_t *d_nodes,
                           int dup_left) {
     if (!ctx || !d_elems || !d_nodes) return set_err(ctx, COSNARK_E_ARG, "merkle_commit: null argument");
-    int rc = merkle_launch(ctx, (const fe *)d_elems, n_elems, d_nodes, dup_left != 0);
+    int rc = merkle_launch(ctx, (const fe *)d_elems, n_elems, d_nodes, dup_left != 0, ctx->stream);
     if (rc) return rc;
     return cosnark_sync(ctx);
 }
@@ -291,32 +306,31 @@ int cosnark_get_indexes(cosnark_transcript *t, int l, uint32_t N, uint32_t M, ui
 // ---------------------------------------------------------------------
 
 // stage the sumcheck state: transcript, claim and libc-drawn tail challenges
-static int sc_stage_state(cosnark_ctx *ctx, const fe &prev, const fe &y, int derive_y, int m, int fs_rounds) {
-    ScState h;
-    memset(&h, 0, sizeof(h));
-    h.prev = prev;
-    h.y = y;
-    h.derive_y = derive_y;
-    for (int i = fs_rounds; i < m; ++i) {
+static int sc_stage_state(cosnark_ctx *ctx, const fe &prev, const fe &y, int derive_y, int m, int fs_rounds,
+                          hipStream_t s) {
+    if (fs_rounds < m) {
         // four draws per round, the last is kept (src/sparse_eval.cpp:164-168)
-        fe r = libc_random_element();
-        r = libc_random_element();
-        r = libc_random_element();
-        r = libc_random_element();
-        h.ext_r[i] = r;
+        for (int i = fs_rounds; i < m; ++i) {
+            fe r = libc_random_element();
+            r = libc_random_element();
+            r = libc_random_element();
+            r = libc_random_element();
+            ctx->sc_host->ext_r[i] = r;
+        }
+        // the pinned mirror is not touched again before this call's final sync
+        HIPCK(hipMemcpyAsync(ctx->sc->ext_r + fs_rounds, ctx->sc_host->ext_r + fs_rounds,
+                             sizeof(fe) * (m - fs_rounds), hipMemcpyHostToDevice, s));
     }
-    HIPCK(hipMemcpyAsync(ctx->sc, &h, sizeof(ScState), hipMemcpyHostToDevice, ctx->stream));
-    // the host struct must stay alive until the copy has run
-    HIPCK(hipStreamSynchronize(ctx->stream));
-    return COSNARK_OK;
+    return sumcheck_state_init(ctx, prev, y, derive_y != 0, s);
 }
 
+// one copy of the device state into the pinned mirror, then results out
 static int sc_collect(cosnark_ctx *ctx, int m, cosnark_transcript *t, uint64_t *h_polys, uint64_t *h_r,
-                      uint64_t *h_final) {
-    ScState h;
-    HIPCK(hipMemcpyAsync(&h, ctx->sc, sizeof(ScState), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCK(hipStreamSynchronize(ctx->stream));
-    if (t) { t->prev[0] = h.prev.re; t->prev[1] = h.prev.im; }
+                      uint64_t *h_final, hipStream_t s) {
+    HIPCK(hipMemcpyAsync(ctx->sc_host, ctx->sc, sizeof(ScState), hipMemcpyDeviceToHost, s));
+    HIPCK(hipStreamSynchronize(s));
+    const ScState &h = *ctx->sc_host;
+    if (t) { t->prev[0] = h.tr_prev[m].re; t->prev[1] = h.tr_prev[m].im; }
     if (h_polys) memcpy(h_polys, h.polys, sizeof(fe) * 3 * m);
     if (h_r) memcpy(h_r, h.r, sizeof(fe) * m);
     if (h_final) memcpy(h_final, h.final_v, sizeof(fe) * 2);
@@ -335,11 +349,11 @@ int cosnark_product_sumcheck(cosnark_ctx *ctx, const uint64_t *d_v1, const uint6
     void *base;
     int rc = scratch_reserve(ctx, sizeof(fe) * (n + n / 2), &base);
     if (rc) return rc;
-    rc = sc_stage_state(ctx, gf::mk(t->prev[0], t->prev[1]), gf::mk(y[0], y[1]), 0, m, fs_rounds);
+    rc = sc_stage_state(ctx, gf::mk(t->prev[0], t->prev[1]), gf::mk(y[0], y[1]), 0, m, fs_rounds, ctx->stream);
     if (rc) return rc;
-    rc = sumcheck_launch(ctx, (const fe *)d_v1, (const fe *)d_v2, n, fs_rounds, (fe *)base);
+    rc = sumcheck_launch(ctx, (const fe *)d_v1, (const fe *)d_v2, n, fs_rounds, (fe *)base, ctx->stream);
     if (rc) return rc;
-    return sc_collect(ctx, m, t, h_polys, h_r, h_final);
+    return sc_collect(ctx, m, t, h_polys, h_r, h_final, ctx->stream);
 }
 
 // ---------------------------------------------------------------------
@@ -366,26 +380,35 @@ int cosnark_prove_plain(cosnark_ctx *ctx, const uint64_t *d_f, const uint64_t *d
     fe *cw = (fe *)((char *)base + off_cw);
     uint8_t *nd = (uint8_t *)base + off_nodes;
     fe *scb = (fe *)((char *)base + off_sc);
-    rc = sc_stage_state(ctx, gf::mk(t->prev[0], t->prev[1]), gf::zero(), 1, m, m);
+    rc = sc_stage_state(ctx, gf::mk(t->prev[0], t->prev[1]), gf::zero(), 1, m, m, ctx->stream);
     if (rc) return rc;
+    // commit (encode + Merkle) on stream 1, the sumcheck beside it on stream 2:
+    // the transcript never absorbs the root, so the two are independent
+    hipStream_t s2 = ctx->overlap ? ctx->stream2 : ctx->stream;
     HIPCK(hipEventRecord(ctx->ev[0], ctx->stream));
-    if ((rc = ntt_launch(ctx, (const fe *)d_f, n, cw, ilog2(cw_n), false))) return rc;
+    if ((rc = ntt_launch(ctx, (const fe *)d_f, n, cw, ilog2(cw_n), false, ctx->stream))) return rc;
     HIPCK(hipEventRecord(ctx->ev[1], ctx->stream));
-    if ((rc = merkle_launch(ctx, cw, cw_n, nd, true))) return rc;
+    HIPCK(hipStreamWaitEvent(s2, ctx->ev[0], 0));
+    HIPCK(hipEventRecord(ctx->ev[4], s2));
+    if ((rc = sumcheck_launch(ctx, (const fe *)d_f, (const fe *)d_g, n, m, scb, s2))) return rc;
+    HIPCK(hipEventRecord(ctx->ev[3], s2));
+    if ((rc = merkle_launch(ctx, cw, cw_n, nd, true, ctx->stream))) return rc;
     HIPCK(hipEventRecord(ctx->ev[2], ctx->stream));
-    if ((rc = sumcheck_launch(ctx, (const fe *)d_f, (const fe *)d_g, n, m, scb))) return rc;
-    HIPCK(hipEventRecord(ctx->ev[3], ctx->stream));
-    HIPCK(hipMemcpyAsync(h_proof, nd + 32 * (nodes - 1), 32, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCK(hipStreamWaitEvent(ctx->stream, ctx->ev[3], 0));
+    HIPCK(hipEventRecord(ctx->ev[5], ctx->stream));
+    HIPCK(hipMemcpyAsync(ctx->sc->root, nd + 32 * (nodes - 1), 32, hipMemcpyDeviceToDevice, ctx->stream));
     std::vector<uint64_t> polys(6 * m), fin(4);
-    rc = sc_collect(ctx, m, t, polys.data(), nullptr, fin.data());
+    rc = sc_collect(ctx, m, t, polys.data(), nullptr, fin.data(), ctx->stream);
+    memcpy(h_proof, ctx->sc_host->root, 32);
     memcpy(h_proof + 32, polys.data(), 48ull * m);
     memcpy(h_proof + 32 + 48ull * m, fin.data(), 32);
     if (times) {
-        float a = 0, b = 0, c = 0;
+        float a = 0, b = 0, c = 0, tot = 0;
         (void)hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]);
         (void)hipEventElapsedTime(&b, ctx->ev[1], ctx->ev[2]);
-        (void)hipEventElapsedTime(&c, ctx->ev[2], ctx->ev[3]);
-        times[0] = a; times[1] = b; times[2] = c; times[3] = (double)a + b + c;
+        (void)hipEventElapsedTime(&c, ctx->ev[4], ctx->ev[3]);
+        (void)hipEventElapsedTime(&tot, ctx->ev[0], ctx->ev[5]);
+        times[0] = a; times[1] = b; times[2] = c; times[3] = tot;
     }
     return rc;
 }

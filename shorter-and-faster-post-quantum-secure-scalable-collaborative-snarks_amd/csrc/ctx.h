@@ -12,25 +12,27 @@ constexpr int NTT_TW_BITS = 14;                 // two-level table split
 constexpr int NTT_TW_SIZE = 1 << NTT_TW_BITS;
 constexpr int NTT_LDS_ELEMS = 4096;             // elements per workgroup tile
 constexpr int NTT_THREADS = 256;
-constexpr int NTT_RADIX_LOG_MAX = 10;           // radix of one LDS pass <= 1024
+constexpr int NTT_RADIX_LOG_MAX = 10;           // radix twiddle table: w_1024^j, j < 1024
+constexpr int NTT_PASS_LOG_MAX = 8;             // radix of one LDS pass <= 256
 
-// Product-sumcheck device state (one per context).
+// Product-sumcheck device state (one per context).  Transcript value and
+// claim are kept per round: tr_prev[k], tr_y[k] = state before round k.
 struct ScState {
-    gf::fe prev;          // transcript chain value (prev_challenge)
-    gf::fe y;             // current claim
-    uint32_t counter;     // last-workgroup-done counter (reset by the last WG)
-    int32_t err;          // 0, or round+1 of the first failed H(0)+H(1)=y check
-    int32_t derive_y;     // 1: take y = H(0)+H(1) of round 0 (claim computed here)
-    int32_t pad;
+    gf::fe tr_prev[65];
+    gf::fe tr_y[65];
     gf::fe polys[64 * 3];
     gf::fe r[64];
     gf::fe ext_r[64];     // externally drawn challenges (libc stream rounds)
     gf::fe final_v[2];
+    uint8_t root[32];     // plain prover: Merkle root, gathered for one copy-out
+    int32_t err;          // 0, or round+1 of the first failed H(0)+H(1)=y check
+    int32_t derive_y;     // 1: take y = H(0)+H(1) of round 0 (claim computed here)
 };
 
 struct cosnark_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;    // concurrent stage (sumcheck beside the commit)
     std::string err;
     // twiddle tables (device): omega_{2^28}^(e & mask), omega_{2^28}^((e >> 14) << 14),
     // and the same for the inverse root; radix tables omega_1024^j (j < 512).
@@ -40,11 +42,14 @@ struct cosnark_ctx {
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
     ScState *sc = nullptr;            // device
-    gf::fe *sc_partials = nullptr;    // device, 3 * SC_MAX_BLOCKS elements
+    ScState *sc_host = nullptr;       // pinned host mirror for copy-in / copy-out
+    gf::fe *sc_partials = nullptr;    // device, 2 x 3 * SC_MAX_BLOCKS elements (ping-pong)
     hipEvent_t ev[8] = {};
+    int ntt_tile = NTT_LDS_ELEMS;     // elements per NTT workgroup tile (power of two, <= 4096)
+    bool overlap = true;              // plain prover: sumcheck beside the commit
 };
 
-constexpr int SC_MAX_BLOCKS = 1024;
+constexpr int SC_MAX_BLOCKS = 256;     // workgroups per multi-workgroup sumcheck round
 
 int set_err(cosnark_ctx *ctx, int code, const std::string &msg);
 int hip_err(cosnark_ctx *ctx, hipError_t e, const char *what);
@@ -66,16 +71,18 @@ int scratch_reserve(cosnark_ctx *ctx, size_t bytes, void **base);
 inline bool is_pow2(uint64_t x) { return x && !(x & (x - 1)); }
 inline int ilog2(uint64_t x) { int l = 0; while ((1ull << l) < x) ++l; return l; }
 
-// ---- internal asynchronous launchers (work on ctx->stream) ----
+// ---- internal asynchronous launchers (work on the given stream) ----
 // NTT of 2^logn points: out = DFT(in || 0) where in has in_len <= 2^logn
 // elements (in may equal out when in_len == 2^logn).
 int ntt_launch(cosnark_ctx *ctx, const gf::fe *in, uint64_t in_len, gf::fe *out, int logn,
-               bool inverse);
+               bool inverse, hipStream_t s);
 // Merkle commit of n_elems elements into nodes (all levels).
 int merkle_launch(cosnark_ctx *ctx, const gf::fe *elems, uint64_t n_elems, uint8_t *nodes,
-                  bool dup_left);
+                  bool dup_left, hipStream_t s);
 int sumcheck_init(cosnark_ctx *ctx);
+int sumcheck_state_init(cosnark_ctx *ctx, const gf::fe &prev, const gf::fe &y, bool derive_y, hipStream_t s);
+int ntt_init(cosnark_ctx *ctx);
 // Product sumcheck rounds; state in ctx->sc (initialised by the caller).
 // scratch buffers: a1,a2,b1,b2 each of n/2 elements.
 int sumcheck_launch(cosnark_ctx *ctx, const gf::fe *v1, const gf::fe *v2, uint64_t n, int fs_rounds,
-                    gf::fe *buf);
+                    gf::fe *buf, hipStream_t s);

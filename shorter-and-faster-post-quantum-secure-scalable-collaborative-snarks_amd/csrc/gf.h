@@ -12,7 +12,7 @@
 #include <stdint.h>
 #include <hip/hip_runtime.h>
 
-#define GF_HD __host__ __device__ __forceinline__
+#define GF_HD __host__ __device__ __forceinline__ constexpr
 
 namespace gf {
 
@@ -23,7 +23,7 @@ struct alignas(16) fe {
     uint64_t re, im;
 };
 
-GF_HD fe mk(uint64_t re, uint64_t im) { fe r; r.re = re; r.im = im; return r; }
+GF_HD fe mk(uint64_t re, uint64_t im) { return fe{re, im}; }
 GF_HD fe zero() { return mk(0, 0); }
 GF_HD fe one() { return mk(1, 0); }
 GF_HD bool eq(const fe &a, const fe &b) { return a.re == b.re && a.im == b.im; }
@@ -48,23 +48,40 @@ GF_HD uint64_t mul61(uint64_t a, uint64_t b) {
 GF_HD uint64_t add61(uint64_t a, uint64_t b) { return canon(a + b); }
 GF_HD uint64_t sub61(uint64_t a, uint64_t b) { return a >= b ? a - b : a + P - b; }
 
-// x * 2^k mod p for canonical x, 0 <= k < 61: a 61-bit rotation
+// x * 2^k mod p for canonical x, 0 < k < 61: a 61-bit rotation (the
+// all-ones pattern p is the only fixed point, so canonical stays canonical)
 GF_HD uint64_t shl61(uint64_t x, int k) {
-    return canon(((x << k) & P) | (x >> (61 - k)));
+    return ((x << k) & P) | (x >> (61 - k));
 }
 
 GF_HD fe add(const fe &a, const fe &b) { return mk(add61(a.re, b.re), add61(a.im, b.im)); }
 GF_HD fe sub(const fe &a, const fe &b) { return mk(sub61(a.re, b.re), sub61(a.im, b.im)); }
 GF_HD fe neg(const fe &a) { return mk(sub61(0, a.re), sub61(0, a.im)); }
 
-// (a + bI)(c + dI) = (ac - bd) + ((a+b)(c+d) - ac - bd) I
+// (a + bI)(c + dI) = (ac - bd) + (ad + bc) I.  Both output coordinates are
+// sums of two 61x61-bit products (the real one with -b = p XOR b for
+// canonical b), accumulated from 31/30-bit limbs with one Mersenne fold per
+// coordinate: 16 v_mad_u64_u32 and two reductions (60 VALU instructions on
+// gfx950, against 92 for a three-multiply Karatsuba with three reductions).
+//   x = x1*2^31 + x0 (x0 < 2^31, x1 < 2^30); 2^62 == 2, 2^61 == 1 (mod p)
+GF_HD uint64_t dot2_61(uint64_t a, uint64_t b, uint64_t c, uint64_t d) {
+    // a*b + c*d mod p for canonical a, b, c, d
+    const uint64_t M31 = 0x7FFFFFFFull;
+    const uint32_t a0 = (uint32_t)(a & M31), a1 = (uint32_t)(a >> 31);
+    const uint32_t b0 = (uint32_t)(b & M31), b1 = (uint32_t)(b >> 31);
+    const uint32_t c0 = (uint32_t)(c & M31), c1 = (uint32_t)(c >> 31);
+    const uint32_t d0 = (uint32_t)(d & M31), d1 = (uint32_t)(d >> 31);
+    const uint64_t p00 = (uint64_t)a0 * b0 + (uint64_t)c0 * d0;                        // < 2^63
+    const uint64_t p01 = (uint64_t)a0 * b1 + (uint64_t)a1 * b0 + (uint64_t)c0 * d1 + (uint64_t)c1 * d0;  // < 2^63
+    const uint64_t p11 = (uint64_t)a1 * b1 + (uint64_t)c1 * d1;                        // < 2^61
+    // value = p11*2^62 + p01*2^31 + p00; p01*2^31 = (p01>>30)*2^61 + (p01 & M30)*2^31
+    const uint64_t s = (p11 << 1) + p00 + (p01 >> 30) + ((p01 & 0x3FFFFFFFull) << 31);  // < 2^64
+    const uint64_t r = (s & P) + (s >> 61);                                           // <= p + 7
+    return canon(r);
+}
+
 GF_HD fe mul(const fe &x, const fe &y) {
-    uint64_t ac = mul61(x.re, y.re);
-    uint64_t bd = mul61(x.im, y.im);
-    uint64_t t = mul61(x.re + x.im, y.re + y.im);
-    uint64_t im = t + (P - ac) + (P - bd);                  // < 3p
-    im = canon((im & P) + (im >> 61));
-    return mk(sub61(ac, bd), im);
+    return mk(dot2_61(x.re, y.re, x.im ^ P, y.im), dot2_61(x.re, y.im, x.im, y.re));
 }
 
 // multiply by I: (a + bI) I = -b + aI
@@ -75,7 +92,7 @@ GF_HD fe mul_neg_i(const fe &x) { return mk(x.im, sub61(0, x.re)); }
 GF_HD fe from_u64(uint64_t x) { return mk(x, 0); }
 
 GF_HD fe pow(fe x, uint64_t e) {
-    fe r = one();
+    fe r = mk(1, 0);
     while (e) {
         if (e & 1) r = mul(r, x);
         x = mul(x, x);
@@ -107,9 +124,53 @@ GF_HD fe inv(const fe &x) {
 constexpr uint64_t G_RE = 2147483648ull, G_IM = 1033321771269002680ull;
 
 GF_HD fe root_of_unity(int logn) {
-    fe g = mk(G_RE, G_IM);
+    fe g{G_RE, G_IM};
     for (int i = 0; i < 62 - logn; ++i) g = mul(g, g);
     return g;
+}
+
+// ---- multiplication by small roots of unity, for transform kernels ----
+// The reference's roots (src/utils.cpp:706-717) give w_4 = -I and
+// w_8 = 2^30 (1 - I); both are checked at compile time below.
+static_assert(root_of_unity(2).re == 0 && root_of_unity(2).im == P - 1, "w4 = -I");
+static_assert(root_of_unity(3).re == (1ull << 30) && root_of_unity(3).im == P - (1ull << 30),
+              "w8 = 2^30 (1 - I)");
+
+// x * w_8^k for odd k: w_8^k = 2^30 (a + b I) with (a, b) = (1,-1), (-1,-1),
+// (-1,1), (1,1) for k = 1, 3, 5, 7
+template <int K>
+GF_HD fe mul_w8(const fe &x) {
+    static_assert(K == 1 || K == 3 || K == 5 || K == 7, "odd power of w8");
+    const uint64_t s = add61(x.re, x.im);   // x + y
+    const uint64_t d = sub61(x.im, x.re);   // y - x
+    const uint64_t rs = shl61(s, 30), rd = shl61(d, 30);
+    if (K == 1) return mk(rs, rd);
+    if (K == 3) return mk(rd, sub61(0, rs));
+    if (K == 5) return mk(sub61(0, rs), sub61(0, rd));
+    return mk(sub61(0, rd), rs);
+}
+
+// x * w_M^E (w_M = root_of_unity(log2 M)), E reduced mod M at compile time;
+// powers that are +-1, +-I or odd powers of w_8 avoid the general multiply
+template <int M, int E>
+GF_HD fe mul_root(const fe &x) {
+    constexpr int e = ((E % M) + M) % M;
+    if constexpr (e == 0) {
+        return x;
+    } else if constexpr (2 * e == M) {
+        return neg(x);
+    } else if constexpr (4 * e == M) {
+        return mk(x.im, sub61(0, x.re));              // x * (-I)
+    } else if constexpr (4 * e == 3 * M) {
+        return mk(sub61(0, x.im), x.re);              // x * I
+    } else if constexpr ((8 * e) % M == 0) {
+        return mul_w8<(8 * e) / M>(x);
+    } else {
+        constexpr int lg = (M <= 1) ? 0 : (M == 2 ? 1 : M == 4 ? 2 : M == 8 ? 3 : M == 16 ? 4 : M == 32 ? 5 :
+                                         M == 64 ? 6 : M == 128 ? 7 : M == 256 ? 8 : M == 512 ? 9 : 10);
+        constexpr fe w = pow(root_of_unity(lg), (uint64_t)e);
+        return mul(x, w);
+    }
 }
 
 }  // namespace gf

@@ -96,7 +96,8 @@ __global__ __launch_bounds__(MK_THREADS) void merkle_block_kernel(
 
 }  // namespace
 
-int merkle_launch(cosnark_ctx *ctx, const fe *elems, uint64_t n_elems, uint8_t *nodes, bool dup_left) {
+int merkle_launch(cosnark_ctx *ctx, const fe *elems, uint64_t n_elems, uint8_t *nodes, bool dup_left,
+                  hipStream_t stream) {
     if (!is_pow2(n_elems) || n_elems < 8 || n_elems > (1ull << 30))
         return set_err(ctx, COSNARK_E_ARG, "merkle: n_elems must be a power of two in [8, 2^30]");
     const uint64_t leaves = n_elems / 4;
@@ -110,7 +111,7 @@ int merkle_launch(cosnark_ctx *ctx, const fe *elems, uint64_t n_elems, uint8_t *
     // leaf level + in-block levels
     int block_log = ilog2(leaves < (uint64_t)MK_BLOCK ? leaves : (uint64_t)MK_BLOCK);
     uint64_t blocks = leaves >> block_log;
-    hipLaunchKernelGGL(merkle_block_kernel<true>, dim3((unsigned)blocks), dim3(MK_THREADS), 0, ctx->stream,
+    hipLaunchKernelGGL(merkle_block_kernel<true>, dim3((unsigned)blocks), dim3(MK_THREADS), 0, stream,
                        elems, (const uint8_t *)nullptr, nodes, lo, 0, block_log, (int)dup_left);
     LAUNCHCK();
     int top = block_log;                              // highest level written
@@ -119,7 +120,7 @@ int merkle_launch(cosnark_ctx *ctx, const fe *elems, uint64_t n_elems, uint8_t *
         const uint64_t next = count / 2;              // nodes at level top+1
         int bl = ilog2(next < (uint64_t)MK_BLOCK ? next : (uint64_t)MK_BLOCK);
         const uint64_t nb = next >> bl;
-        hipLaunchKernelGGL(merkle_block_kernel<false>, dim3((unsigned)nb), dim3(MK_THREADS), 0, ctx->stream,
+        hipLaunchKernelGGL(merkle_block_kernel<false>, dim3((unsigned)nb), dim3(MK_THREADS), 0, stream,
                            (const fe *)nullptr, nodes + 32 * lo.off[top], nodes, lo, top + 1, bl,
                            (int)dup_left);
         LAUNCHCK();

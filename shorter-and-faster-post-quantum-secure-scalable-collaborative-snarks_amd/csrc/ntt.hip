@@ -5,7 +5,7 @@
 // out; the inverse uses w^-1 and scales by 2^-logn.  pfft
 // (src/utils.cpp:1583-1673) is the same forward transform.
 //
-// Design: a 2^L-point transform is split into passes of radix 2^r (r <= 10).
+// Design: a 2^L-point transform is split into passes of radix 2^r (r <= 8).
 // Pass t transforms one index digit of stride S_t = 2^(L - r_0 - .. - r_t)
 // for every (prefix, suffix) pair: a workgroup stages R x C elements
 // (R = 2^r points for C adjacent columns, C*R = 4096) in LDS, runs radix-2
@@ -15,6 +15,7 @@
 // is consecutive, so each store is a C-element contiguous run.  Zero padding
 // (Reed-Solomon encoding of n coefficients into rate*n points) is handled by
 // the first pass, which reads only the n real inputs.
+#include <type_traits>
 #include "ctx.h"
 
 using gf::fe;
@@ -34,120 +35,233 @@ __device__ __forceinline__ uint32_t brev(uint32_t x, int bits) {
     return bits == 0 ? 0u : (__brev(x) >> (32 - bits));
 }
 
+template <int N>
+constexpr int brev_c(int x) {
+    int r = 0;
+    for (int b = 1, q = N >> 1; b < N; b <<= 1, q >>= 1)
+        if (x & b) r |= q;
+    return r;
+}
+
 __device__ __forceinline__ fe tw28(const fe *lo, const fe *hi, uint32_t e) {
     return gf::mul(lo[e & (NTT_TW_SIZE - 1)], hi[e >> NTT_TW_BITS]);
 }
 
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F &&f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
+// In-register DFT of N points, decimation in frequency: natural order in,
+// bit-reversed order out (v[i] = X[brev(i)]).  All twiddles are compile-time
+// constants; powers of w_8 and +-I cost no general multiply.
+template <int N, bool INV>
+__device__ __forceinline__ void dif(fe *v) {
+    if constexpr (N > 1) {
+        static_for<0, N / 2>([&](auto kk) {
+            constexpr int k = decltype(kk)::value;
+            const fe a = v[k], b = v[k + N / 2];
+            v[k] = gf::add(a, b);
+            v[k + N / 2] = gf::mul_root<N, INV ? -k : k>(gf::sub(a, b));
+        });
+        dif<N / 2, INV>(v);
+        dif<N / 2, INV>(v + N / 2);
+    }
+}
+
+constexpr int Y_PAD = 1;   // one padding element per LDS row keeps 16-byte lanes off shared banks
+
+// One pass of radix R = R1 * R2 over C columns held in LDS as a 2-D
+// (four-step) transform: step 1 does R2*C in-register R1-point DFTs over
+// stride-R2 inputs and the w_R^(d2 k1) twiddle, step 2 does R1*C R2-point
+// DFTs, applies the inter-pass twiddle w_{N_t}^(lo k) and stores.
+template <int R1, int R2, bool LAST, bool INV>
 __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
     const fe *in, fe *out, NttPass p, const fe *__restrict__ twlo, const fe *__restrict__ twhi,
-    const fe *__restrict__ rtw, fe scale, int apply_scale) {
-    __shared__ fe tile[NTT_LDS_ELEMS];
-    __shared__ fe rt[512];
+    const fe *__restrict__ rtab, fe scale) {
+    constexpr int R = R1 * R2;
+    constexpr int LOGR = (R >= 256) ? 8 : (R >= 128) ? 7 : (R >= 64) ? 6 : (R >= 32) ? 5 : (R >= 16) ? 4 : (R >= 8) ? 3 : (R >= 4) ? 2 : (R >= 2) ? 1 : 0;
+    constexpr int LOGR2 = (R2 >= 16) ? 4 : (R2 >= 8) ? 3 : (R2 >= 4) ? 2 : (R2 >= 2) ? 1 : 0;
+    constexpr int LOGR1 = LOGR - LOGR2;
+    extern __shared__ fe ntt_lds[];
+    fe *wr = ntt_lds;          // w_R^e, e < R
+    fe *Y = ntt_lds + R;       // (R * C + R * Y_PAD) elements
     const int tid = threadIdx.x;
-    const int logR = p.logR, logC = p.logC;
-    const int R = 1 << logR, C = 1 << logC;
-    const int nelem = R << logC;
-
-    for (int j = tid; j < R / 2; j += NTT_THREADS) rt[j] = rtw[j << (NTT_RADIX_LOG_MAX - logR)];
+    const int logC = p.logC, C = 1 << logC;
+    const int CP = C + Y_PAD;   // padded LDS row length
+    for (int j = tid; j < R; j += NTT_THREADS) wr[j] = rtab[j << (NTT_RADIX_LOG_MAX - LOGR)];
 
     const uint64_t bid = blockIdx.x;
-    if (!p.last) {
-        // positions: hi * (R*S) + d * S + lo, this block: fixed hi, lo in [lo0, lo0 + C)
-        const int logS = p.logS;
-        const uint64_t lo_blocks_log = (uint64_t)(logS - logC);
-        const uint64_t hi = bid >> lo_blocks_log;
-        const uint64_t lo0 = (bid & ((1ull << lo_blocks_log) - 1)) << logC;
-        const uint64_t base = (hi << (logR + logS)) + lo0;
-        for (int idx = tid; idx < nelem; idx += NTT_THREADS) {
-            const int d = idx >> logC, c = idx & (C - 1);
-            const uint64_t pos = base + ((uint64_t)d << logS) + c;
-            fe v = pos < p.in_len ? in[pos] : gf::zero();
-            tile[(brev(d, logR) << logC) + c] = v;
+    // block -> positions
+    uint64_t hi = 0, lo0 = 0;              // non-last: pos = hi*(R*S) + d*S + lo0 + c
+    uint64_t rest = 0, c0 = 0, W0 = 0;      // last: row(c) = rest + (c0 + c)*W0, pos = row*R + d
+    const int logS = p.logS;
+    if (!LAST) {
+        const int lo_blocks_log = logS - logC;
+        hi = bid >> lo_blocks_log;
+        lo0 = (bid & ((1ull << lo_blocks_log) - 1)) << logC;
+    } else if (p.npass >= 2) {
+        const int cblocks_log = p.rbits[0] - logC;
+        rest = bid >> cblocks_log;
+        c0 = (bid & ((1ull << cblocks_log) - 1)) << logC;
+        W0 = 1ull << (p.logN - p.rbits[0] - p.rbits[p.npass - 1]);
+    }
+    __syncthreads();
+
+    // ---- step 1: R1-point DFTs over d1 for each (d2, c) ----
+    const int items1 = R2 << logC;
+    for (int it = tid; it < items1; it += NTT_THREADS) {
+        int c, d2;
+        uint64_t base;
+        if (!LAST) {
+            c = it & (C - 1);
+            d2 = it >> logC;
+            base = (hi << (LOGR + logS)) + lo0 + c;
+        } else {
+            d2 = it & (R2 - 1);
+            c = it >> LOGR2;
+            base = (rest + (c0 + c) * W0) << LOGR;
         }
-        __syncthreads();
-        for (int s = 0; s < logR; ++s) {
-            for (int b = tid; b < nelem / 2; b += NTT_THREADS) {
-                const int c = b & (C - 1), bb = b >> logC;
-                const int k = bb & ((1 << s) - 1), grp = bb >> s;
-                const int i0 = (grp << (s + 1)) + k, i1 = i0 + (1 << s);
-                const fe w = rt[k << (logR - 1 - s)];
-                const fe u = tile[(i0 << logC) + c];
-                const fe v = gf::mul(tile[(i1 << logC) + c], w);
-                tile[(i0 << logC) + c] = gf::add(u, v);
-                tile[(i1 << logC) + c] = gf::sub(u, v);
-            }
-            __syncthreads();
+        fe v[R1];
+#pragma unroll
+        for (int d1 = 0; d1 < R1; ++d1) {
+            const uint64_t d = (uint64_t)(R2 * d1 + d2);
+            const uint64_t pos = base + (LAST ? d : (d << logS));
+            v[d1] = pos < p.in_len ? in[pos] : gf::zero();
         }
-        const int logNt = logR + logS;
-        for (int idx = tid; idx < nelem; idx += NTT_THREADS) {
-            const int k = idx >> logC, c = idx & (C - 1);
+        dif<R1, INV>(v);
+        static_for<0, R1>([&](auto ii) {
+            constexpr int i = decltype(ii)::value;
+            constexpr int k1 = brev_c<R1>(i);
+            fe x = v[i];
+            if (k1 != 0 && d2 != 0) x = gf::mul(x, wr[d2 * k1]);
+            Y[(k1 * R2 + d2) * CP + c] = x;
+        });
+    }
+    __syncthreads();
+
+    // ---- step 2: R2-point DFTs over d2 for each (k1, c), twiddle, store ----
+    const int items2 = R1 << logC;
+    const int logNt = LOGR + logS;
+    uint64_t nat_rest = 0;
+    int kshift = 0;
+    if (LAST) {
+        uint64_t x = rest;
+        for (int t = p.npass - 2; t >= 1; --t) {
+            const uint64_t dgt = x & ((1ull << p.rbits[t]) - 1);
+            x >>= p.rbits[t];
+            nat_rest += dgt << p.pre[t];
+        }
+        kshift = p.logN - LOGR;
+    }
+    for (int it = tid; it < items2; it += NTT_THREADS) {
+        const int c = it & (C - 1), k1 = it >> logC;
+        fe v[R2];
+#pragma unroll
+        for (int d2 = 0; d2 < R2; ++d2) v[d2] = Y[(k1 * R2 + d2) * CP + c];
+        dif<R2, INV>(v);
+        if (!LAST) {
+            // inter-pass twiddle w_{N_t}^(lo k), k = k1 + R1 k2: one table product for
+            // w^(lo k1) and one for w^(lo R1), then a running product over k2
             const uint64_t lo = lo0 + c;
-            const uint32_t e = (uint32_t)((lo * (uint64_t)k) << (NTT_LOG_MAX - logNt));
-            fe v = tile[idx];
-            if (e) v = gf::mul(v, tw28(twlo, twhi, e));
-            out[(hi << logNt) + ((uint64_t)k << logS) + lo] = v;
-        }
-    } else {
-        // rows hi_c = rest + (c0 + c) * W0 of R contiguous elements
-        const int L = p.logN;
-        const int m = p.npass;
-        uint64_t rest = 0, c0 = 0, W0 = 0;
-        if (m >= 2) {
-            const int logW0 = L - p.rbits[0] - p.rbits[m - 1];
-            const int cblocks_log = p.rbits[0] - logC;
-            rest = bid >> cblocks_log;
-            c0 = (bid & ((1ull << cblocks_log) - 1)) << logC;
-            W0 = 1ull << logW0;
-        }
-        for (int idx = tid; idx < nelem; idx += NTT_THREADS) {
-            const int c = idx >> logR, d = idx & (R - 1);
-            const uint64_t hic = rest + (c0 + c) * W0;
-            const uint64_t pos = (hic << logR) + d;
-            fe v = pos < p.in_len ? in[pos] : gf::zero();
-            tile[(brev(d, logR) << logC) + c] = v;
-        }
-        __syncthreads();
-        for (int s = 0; s < logR; ++s) {
-            for (int b = tid; b < nelem / 2; b += NTT_THREADS) {
-                const int c = b & (C - 1), bb = b >> logC;
-                const int k = bb & ((1 << s) - 1), grp = bb >> s;
-                const int i0 = (grp << (s + 1)) + k, i1 = i0 + (1 << s);
-                const fe w = rt[k << (logR - 1 - s)];
-                const fe u = tile[(i0 << logC) + c];
-                const fe v = gf::mul(tile[(i1 << logC) + c], w);
-                tile[(i0 << logC) + c] = gf::add(u, v);
-                tile[(i1 << logC) + c] = gf::sub(u, v);
-            }
-            __syncthreads();
-        }
-        // digit reversal of `rest` (digits of passes 1..m-2)
-        uint64_t nat_rest = 0;
-        {
-            uint64_t x = rest;
-            for (int t = m - 2; t >= 1; --t) {
-                const uint64_t dgt = x & ((1ull << p.rbits[t]) - 1);
-                x >>= p.rbits[t];
-                nat_rest += dgt << p.pre[t];
-            }
-        }
-        const int kshift = L - p.rbits[m - 1];
-        for (int idx = tid; idx < nelem; idx += NTT_THREADS) {
-            const int k = idx >> logC, c = idx & (C - 1);
-            fe v = tile[idx];
-            if (apply_scale) v = gf::mul(v, scale);
-            out[nat_rest + c0 + c + ((uint64_t)k << kshift)] = v;
+            const int sh = NTT_LOG_MAX - logNt;
+            const uint32_t eb = (uint32_t)((lo * (uint64_t)k1) << sh);
+            const uint32_t es = (uint32_t)((lo * (uint64_t)R1) << sh);
+            fe t = tw28(twlo, twhi, eb);
+            const fe step = tw28(twlo, twhi, es);
+            static_for<0, R2>([&](auto kk) {
+                constexpr int k2 = decltype(kk)::value;
+                constexpr int i = brev_c<R2>(k2);
+                const uint64_t k = (uint64_t)(k1 + R1 * k2);
+                out[(hi << logNt) + (k << logS) + lo] = gf::mul(v[i], t);
+                if (k2 + 1 < R2) t = gf::mul(t, step);
+            });
+        } else {
+            static_for<0, R2>([&](auto ii) {
+                constexpr int i = decltype(ii)::value;
+                constexpr int k2 = brev_c<R2>(i);
+                const uint64_t k = (uint64_t)(k1 + R1 * k2);
+                fe x = v[i];
+                if (INV) x = gf::mul(x, scale);
+                out[nat_rest + c0 + c + (k << kshift)] = x;
+            });
         }
     }
 }
 
+using PassKernel = void (*)(const fe *, fe *, NttPass, const fe *, const fe *, const fe *, fe);
+
+template <int R1, int R2>
+PassKernel pass_kernel(bool last, bool inverse) {
+    return last ? (inverse ? ntt_pass_kernel<R1, R2, true, true> : ntt_pass_kernel<R1, R2, true, false>)
+                : (inverse ? ntt_pass_kernel<R1, R2, false, true> : ntt_pass_kernel<R1, R2, false, false>);
+}
+
+PassKernel pass_kernel_for(int r, bool last, bool inverse) {
+    switch (r) {
+        case 1: return pass_kernel<2, 1>(last, inverse);
+        case 2: return pass_kernel<2, 2>(last, inverse);
+        case 3: return pass_kernel<4, 2>(last, inverse);
+        case 4: return pass_kernel<4, 4>(last, inverse);
+        case 5: return pass_kernel<8, 4>(last, inverse);
+        case 6: return pass_kernel<8, 8>(last, inverse);
+        case 7: return pass_kernel<16, 8>(last, inverse);
+        case 8: return pass_kernel<16, 16>(last, inverse);
+        default: return nullptr;
+    }
+}
+
+template <int R1, int R2>
+int launch_pass(cosnark_ctx *ctx, hipStream_t stream, uint64_t blocks, const fe *in, fe *out,
+                const NttPass &q, bool last, bool inverse, const fe *twlo, const fe *twhi,
+                const fe *rtab, fe scale) {
+    PassKernel k = pass_kernel<R1, R2>(last, inverse);
+    const size_t lds = sizeof(fe) * ((size_t)R1 * R2 * ((1u << q.logC) + Y_PAD + 1));
+    hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(NTT_THREADS), lds, stream, in, out, q, twlo, twhi, rtab,
+                       scale);
+    LAUNCHCK();
+    return COSNARK_OK;
+}
+
+int dispatch_pass(cosnark_ctx *ctx, hipStream_t stream, int r, uint64_t blocks, const fe *in, fe *out,
+                  const NttPass &q, bool last, bool inverse, const fe *twlo, const fe *twhi, const fe *rtab,
+                  fe scale) {
+#define PASS(R1, R2) launch_pass<R1, R2>(ctx, stream, blocks, in, out, q, last, inverse, twlo, twhi, rtab, scale)
+    switch (r) {
+        case 1: return PASS(2, 1);
+        case 2: return PASS(2, 2);
+        case 3: return PASS(4, 2);
+        case 4: return PASS(4, 4);
+        case 5: return PASS(8, 4);
+        case 6: return PASS(8, 8);
+        case 7: return PASS(16, 8);
+        case 8: return PASS(16, 16);
+        default: return set_err(ctx, COSNARK_E_RANGE, "ntt: unsupported radix");
+    }
+#undef PASS
+}
+
 }  // namespace
 
-int ntt_launch(cosnark_ctx *ctx, const fe *in, uint64_t in_len, fe *out, int logn, bool inverse) {
+int ntt_init(cosnark_ctx *ctx) {
+    // the largest tile: 4096 elements + padding + the radix table (~74 KiB)
+    const int bytes = (int)(sizeof(fe) * (NTT_LDS_ELEMS + 256 * (Y_PAD + 1)));
+    for (int r = 1; r <= NTT_PASS_LOG_MAX; ++r)
+        for (int last = 0; last < 2; ++last)
+            for (int inv = 0; inv < 2; ++inv)
+                HIPCK(hipFuncSetAttribute((const void *)pass_kernel_for(r, last, inv),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    return COSNARK_OK;
+}
+
+int ntt_launch(cosnark_ctx *ctx, const fe *in, uint64_t in_len, fe *out, int logn, bool inverse,
+               hipStream_t stream) {
     if (logn < 1 || logn > NTT_LOG_MAX) return set_err(ctx, COSNARK_E_RANGE, "ntt: logn out of range");
-    // radix plan: fewest passes with r <= 10, bits spread evenly, larger first
-    int m = (logn + NTT_RADIX_LOG_MAX - 1) / NTT_RADIX_LOG_MAX;
-    if (logn > 16 && m < 3) m = 3;          // keep r <= 8 (C = 16) for large transforms
-    if (m > logn) m = logn;
+    // radix plan: fewest passes of at most 8 bits, bits spread evenly, larger first
+    const int m = (logn + NTT_PASS_LOG_MAX - 1) / NTT_PASS_LOG_MAX;
     NttPass p{};
     p.in_len = in_len;
     p.logN = logn;
@@ -157,17 +271,19 @@ int ntt_launch(cosnark_ctx *ctx, const fe *in, uint64_t in_len, fe *out, int log
     for (int t = 1; t < m; ++t) p.pre[t] = p.pre[t - 1] + p.rbits[t - 1];
     const fe *twlo = inverse ? ctx->itw_lo : ctx->tw_lo;
     const fe *twhi = inverse ? ctx->itw_hi : ctx->tw_hi;
-    const fe *rtw = inverse ? ctx->irtw : ctx->rtw;
-    fe scale = gf::inv(gf::from_u64(1ull << logn));
+    const fe *rtab = inverse ? ctx->irtw : ctx->rtw;
+    const fe scale = gf::inv(gf::from_u64(1ull << logn));
     int done_bits = 0;
     for (int t = 0; t < m; ++t) {
         const int r = p.rbits[t];
         NttPass q = p;
         q.logR = r;
-        q.last = (t == m - 1);
+        const bool last = (t == m - 1);
+        q.last = last;
         q.logS = logn - done_bits - r;
-        int logC = ilog2(NTT_LDS_ELEMS) - r;
-        if (!q.last) {
+        int logC = ilog2((uint64_t)ctx->ntt_tile) - r;
+        if (logC < 0) logC = 0;
+        if (!last) {
             if (logC > q.logS) logC = q.logS;
         } else {
             if (m == 1) logC = 0;
@@ -176,10 +292,9 @@ int ntt_launch(cosnark_ctx *ctx, const fe *in, uint64_t in_len, fe *out, int log
         q.logC = logC;
         if (t > 0) q.in_len = 1ull << logn;
         const uint64_t blocks = (1ull << logn) >> (r + logC);
-        hipLaunchKernelGGL(ntt_pass_kernel, dim3((unsigned)blocks), dim3(NTT_THREADS), 0, ctx->stream,
-                           t == 0 ? in : out, out, q, twlo, twhi, rtw, scale,
-                           (int)(inverse && q.last));
-        LAUNCHCK();
+        int rc = dispatch_pass(ctx, stream, r, blocks, t == 0 ? in : out, out, q, last, inverse, twlo, twhi,
+                               rtab, scale);
+        if (rc) return rc;
         done_bits += r;
     }
     return COSNARK_OK;

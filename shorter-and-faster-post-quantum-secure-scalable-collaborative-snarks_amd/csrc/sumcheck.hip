@@ -8,12 +8,16 @@
 // y = H(r), fold v[j] = r (v[2j+1] - v[2j]) + v[2j] (:257-261).
 //
 // Here b is formed as sum o1 o2 - a - c (o = v[2j+1]), three products per
-// pair.  Round i+1's evaluation is fused with round i's fold: one pass reads
-// four elements per vector, writes the two folded ones and accumulates the
-// next round's sums.  Workgroup partials are combined by the last workgroup
-// to finish (atomic counter), which also runs the transcript hashes, so the
-// large rounds need one launch each and no host round trip.  Once a vector
-// fits in LDS (<= 4096 elements) one workgroup runs all remaining rounds.
+// pair.  Round i's evaluation is fused with the fold by r[i-1]: one pass
+// reads four elements per vector, writes the two folded ones and sums the
+// round's three terms.  Each workgroup writes its partial sums; the next
+// round's kernel starts by reducing the previous round's partials in every
+// workgroup (a few KiB from L2) and running the transcript hashes
+// redundantly, on four lanes, so no atomics, no last-workgroup election and
+// no host round trip sit between rounds.  Transcript values are kept per
+// round (tr_prev[k], tr_y[k] = state before round k) so a kernel never reads
+// a slot it writes.  Once a vector fits in LDS one workgroup runs all
+// remaining rounds.
 #include "ctx.h"
 #include "transcript.h"
 
@@ -23,7 +27,8 @@ namespace {
 
 constexpr int SC_THREADS = 256;
 constexpr int SC_TAIL_THREADS = 1024;
-constexpr int SC_TAIL_ELEMS = 4096;
+constexpr int SC_TAIL_ELEMS = 4096;     // LDS-resident vectors (2 x 64 KiB)
+constexpr int SC_ITEMS = 4;             // pairs per thread per sweep in the multi-workgroup rounds
 
 __device__ __forceinline__ fe fold1(const fe &e, const fe &o, const fe &r) {
     return gf::add(gf::mul(r, gf::sub(o, e)), e);
@@ -42,161 +47,178 @@ __device__ __forceinline__ void wave_reduce3(fe &a, fe &b, fe &c) {
     }
 }
 
-// Scalar transcript state carried through the rounds by one thread.
-struct ScRound {
-    fe prev, y;
-    int err, derive_y;
-};
+// Block-wide sum of (A, C, O); the result is valid in wave 0.  Starts and
+// ends with a barrier.
+template <int NT>
+__device__ __forceinline__ void block_reduce3(fe &A, fe &C, fe &O, fe (*red)[NT / 64]) {
+    wave_reduce3(A, C, O);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) { red[0][wv] = A; red[1][wv] = C; red[2][wv] = O; }
+    __syncthreads();
+    if (wv == 0) {
+        A = red[0][0]; C = red[1][0]; O = red[2][0];
+        for (int w = 1; w < NT / 64; ++w) {
+            A = gf::add(A, red[0][w]); C = gf::add(C, red[1][w]); O = gf::add(O, red[2][w]);
+        }
+    }
+}
 
-// Combine (A = sum d1 d2, C = sum e1 e2, O = sum o1 o2) into round `round`'s
-// polynomial, check it, derive the challenge, update y.  One thread.
-// st->polys / st->r are written only; st->ext_r is read only.
-__device__ fe finalize_round(ScRound &s, ScState *st, int round, int fs_rounds, fe A, fe C, fe O) {
+// Round `round`'s polynomial from (A = sum d1 d2, C = sum e1 e2,
+// O = sum o1 o2): check H(0)+H(1) = y, derive the challenge (transcript or
+// libc stream), update (prev, y), return r.  Called by lanes 0..3 of a wave
+// with the same inputs (they share the four-lane hash); when `writer`,
+// lane 0 records the round in st.
+__device__ fe finalize_round(ScState *st, int round, int fs_rounds, bool derive_y, fe &prev, fe &y, fe A, fe C,
+                             fe O, bool writer) {
     const fe a = A, c = C;
     const fe b = gf::sub(gf::sub(O, A), C);
     const fe h01 = gf::add(gf::add(a, b), gf::add(c, c));   // H(0) + H(1)
-    if (round == 0 && s.derive_y) {
-        s.y = h01;
-    } else if (!gf::eq(h01, s.y) && s.err == 0) {
-        s.err = round + 1;
-    }
+    bool bad = false;
+    if (round == 0 && derive_y) y = h01;
+    else bad = !gf::eq(h01, y);
     fe r;
     if (round < fs_rounds) {
-        s.prev = ts::hash_F(s.prev, a);
-        s.prev = ts::hash_F(s.prev, b);
-        s.prev = ts::hash_F(s.prev, c);
-        r = s.prev;
+        prev = ts::hash_F_quad(prev, a);
+        prev = ts::hash_F_quad(prev, b);
+        prev = ts::hash_F_quad(prev, c);
+        r = prev;
     } else {
         r = st->ext_r[round];
     }
-    s.y = gf::add(gf::mul(gf::add(gf::mul(a, r), b), r), c);
-    st->polys[3 * round] = a;
-    st->polys[3 * round + 1] = b;
-    st->polys[3 * round + 2] = c;
-    st->r[round] = r;
+    y = gf::add(gf::mul(gf::add(gf::mul(a, r), b), r), c);
+    if (writer && (threadIdx.x & 3) == 0) {
+        st->polys[3 * round] = a;
+        st->polys[3 * round + 1] = b;
+        st->polys[3 * round + 2] = c;
+        st->r[round] = r;
+        st->tr_prev[round + 1] = prev;
+        st->tr_y[round + 1] = y;
+        if (bad && st->err == 0) st->err = round + 1;
+    }
     return r;
 }
 
-__device__ __forceinline__ ScRound load_round_state(const ScState *st) {
-    ScRound s;
-    s.prev = st->prev; s.y = st->y; s.err = st->err; s.derive_y = st->derive_y;
-    return s;
+// Reduce round `round`'s workgroup partials and finalize it in this
+// workgroup: (*sh_r, *sh_prev, *sh_y) receive the challenge and the state
+// after the round, visible to all threads on return.
+template <int NT>
+__device__ void finish_round(ScState *st, int round, int fs_rounds, const fe *part_in, int nb_in, fe (*red)[NT / 64],
+                             fe *sh_r, fe *sh_prev, fe *sh_y) {
+    fe A = gf::zero(), C = gf::zero(), O = gf::zero();
+    for (int b = threadIdx.x; b < nb_in; b += NT) {
+        A = gf::add(A, part_in[3 * b]);
+        C = gf::add(C, part_in[3 * b + 1]);
+        O = gf::add(O, part_in[3 * b + 2]);
+    }
+    block_reduce3<NT>(A, C, O, red);
+    if (threadIdx.x < 4) {
+        fe prev = st->tr_prev[round], y = st->tr_y[round];
+        const fe r = finalize_round(st, round, fs_rounds, st->derive_y != 0, prev, y, A, C, O, blockIdx.x == 0);
+        if (threadIdx.x == 0) { *sh_r = r; *sh_prev = prev; *sh_y = y; }
+    }
+    __syncthreads();
 }
 
-__device__ __forceinline__ void store_round_state(ScState *st, const ScRound &s) {
-    st->prev = s.prev; st->y = s.y; st->err = s.err;
-}
-
-// One multi-workgroup round.  FOLD: inputs have 4*npairs elements and are
-// folded with r[round-1] into outputs of 2*npairs; otherwise inputs have
-// 2*npairs elements and are used as they are.
+// One multi-workgroup round `round`.  FOLD (round > 0): inputs hold
+// 4*npairs elements, folded by r[round-1] (finalized here from part_in) into
+// outputs of 2*npairs; otherwise inputs hold 2*npairs elements.  Writes this
+// round's per-workgroup partials to part_out.
 template <bool FOLD>
 __global__ __launch_bounds__(SC_THREADS) void sc_round_kernel(
     const fe *__restrict__ s1, const fe *__restrict__ s2, fe *__restrict__ d1, fe *__restrict__ d2,
-    uint64_t npairs, int round, int fs_rounds, ScState *st, fe *partials) {
+    uint64_t npairs, int round, int fs_rounds, ScState *st, const fe *part_in, int nb_in, fe *part_out) {
     __shared__ fe red[3][SC_THREADS / 64];
-    __shared__ int is_last;
+    __shared__ fe sh_r, sh_prev, sh_y;
+    constexpr int NV = FOLD ? 4 : 2;
+    const uint64_t stride = (uint64_t)gridDim.x * SC_THREADS * SC_ITEMS;
+    uint64_t j0 = (uint64_t)blockIdx.x * SC_THREADS * SC_ITEMS + threadIdx.x;
+    fe x1[SC_ITEMS][NV], x2[SC_ITEMS][NV];
+    auto load = [&](uint64_t base) {
+#pragma unroll
+        for (int t = 0; t < SC_ITEMS; ++t) {
+            const uint64_t j = base + (uint64_t)t * SC_THREADS;
+            if (j < npairs) {
+#pragma unroll
+                for (int q = 0; q < NV; ++q) { x1[t][q] = s1[NV * j + q]; x2[t][q] = s2[NV * j + q]; }
+            }
+        }
+    };
+    // the first sweep's loads do not depend on the challenge: issue them first
+    load(j0);
     fe r = gf::zero();
-    if (FOLD) r = st->r[round - 1];
+    if (FOLD) {
+        finish_round<SC_THREADS>(st, round - 1, fs_rounds, part_in, nb_in, red, &sh_r, &sh_prev, &sh_y);
+        r = sh_r;
+    }
     fe A = gf::zero(), C = gf::zero(), O = gf::zero();
-    const uint64_t stride = (uint64_t)gridDim.x * SC_THREADS;
-    for (uint64_t j = (uint64_t)blockIdx.x * SC_THREADS + threadIdx.x; j < npairs; j += stride) {
-        fe e1, o1, e2, o2;
-        if (FOLD) {
-            e1 = fold1(s1[4 * j], s1[4 * j + 1], r);
-            o1 = fold1(s1[4 * j + 2], s1[4 * j + 3], r);
-            e2 = fold1(s2[4 * j], s2[4 * j + 1], r);
-            o2 = fold1(s2[4 * j + 2], s2[4 * j + 3], r);
-            d1[2 * j] = e1; d1[2 * j + 1] = o1;
-            d2[2 * j] = e2; d2[2 * j + 1] = o2;
-        } else {
-            e1 = s1[2 * j]; o1 = s1[2 * j + 1];
-            e2 = s2[2 * j]; o2 = s2[2 * j + 1];
+    while (j0 < npairs) {
+#pragma unroll
+        for (int t = 0; t < SC_ITEMS; ++t) {
+            const uint64_t j = j0 + (uint64_t)t * SC_THREADS;
+            if (j >= npairs) break;
+            fe e1, o1, e2, o2;
+            if (FOLD) {
+                e1 = fold1(x1[t][0], x1[t][1], r);
+                o1 = fold1(x1[t][2], x1[t][3], r);
+                e2 = fold1(x2[t][0], x2[t][1], r);
+                o2 = fold1(x2[t][2], x2[t][3], r);
+                d1[2 * j] = e1; d1[2 * j + 1] = o1;
+                d2[2 * j] = e2; d2[2 * j + 1] = o2;
+            } else {
+                e1 = x1[t][0]; o1 = x1[t][1];
+                e2 = x2[t][0]; o2 = x2[t][1];
+            }
+            A = gf::add(A, gf::mul(gf::sub(o1, e1), gf::sub(o2, e2)));
+            C = gf::add(C, gf::mul(e1, e2));
+            O = gf::add(O, gf::mul(o1, o2));
         }
-        A = gf::add(A, gf::mul(gf::sub(o1, e1), gf::sub(o2, e2)));
-        C = gf::add(C, gf::mul(e1, e2));
-        O = gf::add(O, gf::mul(o1, o2));
+        j0 += stride;
+        if (j0 < npairs) load(j0);
     }
-    wave_reduce3(A, C, O);
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (lane == 0) { red[0][wv] = A; red[1][wv] = C; red[2][wv] = O; }
-    __syncthreads();
+    block_reduce3<SC_THREADS>(A, C, O, red);
     if (threadIdx.x == 0) {
-        fe a = red[0][0], c = red[1][0], o = red[2][0];
-        for (int w = 1; w < SC_THREADS / 64; ++w) {
-            a = gf::add(a, red[0][w]); c = gf::add(c, red[1][w]); o = gf::add(o, red[2][w]);
-        }
-        partials[3 * blockIdx.x] = a;
-        partials[3 * blockIdx.x + 1] = c;
-        partials[3 * blockIdx.x + 2] = o;
-        __threadfence();
-        const uint32_t prev = atomicAdd(&st->counter, 1u);
-        is_last = (prev == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
-    // last workgroup: reduce all partials
-    A = gf::zero(); C = gf::zero(); O = gf::zero();
-    for (unsigned b = threadIdx.x; b < gridDim.x; b += SC_THREADS) {
-        const volatile uint64_t *p = reinterpret_cast<const volatile uint64_t *>(partials + 3 * b);
-        A = gf::add(A, gf::mk(p[0], p[1]));
-        C = gf::add(C, gf::mk(p[2], p[3]));
-        O = gf::add(O, gf::mk(p[4], p[5]));
-    }
-    wave_reduce3(A, C, O);
-    __syncthreads();
-    if (lane == 0) { red[0][wv] = A; red[1][wv] = C; red[2][wv] = O; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        fe a = red[0][0], c = red[1][0], o = red[2][0];
-        for (int w = 1; w < SC_THREADS / 64; ++w) {
-            a = gf::add(a, red[0][w]); c = gf::add(c, red[1][w]); o = gf::add(o, red[2][w]);
-        }
-        ScRound rs = load_round_state(st);
-        finalize_round(rs, st, round, fs_rounds, a, c, o);
-        store_round_state(st, rs);
-        st->counter = 0;
-        __threadfence();
+        part_out[3 * blockIdx.x] = A;
+        part_out[3 * blockIdx.x + 1] = C;
+        part_out[3 * blockIdx.x + 2] = O;
     }
 }
 
 // All rounds from `round0` to m-1 in one workgroup, vectors held in LDS.
-// If round0 > 0 the input (len elements) still needs the fold with
-// r[round0-1].  Writes the final folded values to st->final_v.
+// If round0 > 0 the input (len elements) still needs the fold by
+// r[round0-1], which is finalized here from part_in.  Writes the final
+// folded values and the closing transcript state.
 __global__ __launch_bounds__(SC_TAIL_THREADS) void sc_tail_kernel(
-    const fe *__restrict__ s1, const fe *__restrict__ s2, uint32_t len, int round0, int m,
-    int fs_rounds, ScState *st) {
+    const fe *__restrict__ s1, const fe *__restrict__ s2, uint32_t len, int round0, int m, int fs_rounds,
+    ScState *st, const fe *part_in, int nb_in) {
     extern __shared__ fe lds[];
     fe *x1 = lds, *x2 = lds + SC_TAIL_ELEMS;
     __shared__ fe red[3][SC_TAIL_THREADS / 64];
-    __shared__ ScRound srs;
-    __shared__ fe sh_r;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    __shared__ fe sh_r, sh_prev, sh_y;
+    const int tid = threadIdx.x;
     for (uint32_t i = tid; i < len; i += SC_TAIL_THREADS) { x1[i] = s1[i]; x2[i] = s2[i]; }
-    if (tid == 0) {
-        srs = load_round_state(st);
-        sh_r = round0 > 0 ? st->r[round0 - 1] : gf::zero();
+    const bool derive_y = st->derive_y != 0;
+    if (round0 > 0) {
+        finish_round<SC_TAIL_THREADS>(st, round0 - 1, fs_rounds, part_in, nb_in, red, &sh_r, &sh_prev, &sh_y);
+    } else {
+        if (tid == 0) { sh_prev = st->tr_prev[0]; sh_y = st->tr_y[0]; sh_r = gf::zero(); }
+        __syncthreads();
     }
-    __syncthreads();
-    uint32_t cur = len;      // elements currently held (before pending fold)
+    uint32_t cur = len;
     bool pending = round0 > 0;
     for (int round = round0; round <= m; ++round) {
-        // apply pending fold with r[round-1], then evaluate round `round`
         const fe r = sh_r;
-        const uint32_t after = pending ? cur / 2 : cur;     // elements after fold
+        const uint32_t after = pending ? cur / 2 : cur;
         if (round == m) {
-            // final fold only
             if (tid == 0) {
                 st->final_v[0] = pending ? fold1(x1[0], x1[1], r) : x1[0];
                 st->final_v[1] = pending ? fold1(x2[0], x2[1], r) : x2[0];
-                store_round_state(st, srs);
             }
             break;
         }
         const uint32_t npairs = after / 2;
         fe A = gf::zero(), C = gf::zero(), O = gf::zero();
-        fe w[2][4];
+        fe w0[4] = {}, w1[4] = {};
         int nmine = 0;
         for (uint32_t j = tid; j < npairs; j += SC_TAIL_THREADS) {
             fe e1, o1, e2, o2;
@@ -212,30 +234,36 @@ __global__ __launch_bounds__(SC_TAIL_THREADS) void sc_tail_kernel(
             A = gf::add(A, gf::mul(gf::sub(o1, e1), gf::sub(o2, e2)));
             C = gf::add(C, gf::mul(e1, e2));
             O = gf::add(O, gf::mul(o1, o2));
-            w[nmine][0] = e1; w[nmine][1] = o1; w[nmine][2] = e2; w[nmine][3] = o2;
+            if (nmine == 0) { w0[0] = e1; w0[1] = o1; w0[2] = e2; w0[3] = o2; }
+            else { w1[0] = e1; w1[1] = o1; w1[2] = e2; w1[3] = o2; }
             ++nmine;
         }
-        wave_reduce3(A, C, O);
-        if (lane == 0) { red[0][wv] = A; red[1][wv] = C; red[2][wv] = O; }
-        __syncthreads();   // all reads of x done; partials visible
+        block_reduce3<SC_TAIL_THREADS>(A, C, O, red);   // its barriers order all reads of x before the writes below
         if (pending) {
             int q = 0;
             for (uint32_t j = tid; j < npairs; j += SC_TAIL_THREADS, ++q) {
-                x1[2 * j] = w[q][0]; x1[2 * j + 1] = w[q][1];
-                x2[2 * j] = w[q][2]; x2[2 * j + 1] = w[q][3];
+                x1[2 * j] = q == 0 ? w0[0] : w1[0];
+                x1[2 * j + 1] = q == 0 ? w0[1] : w1[1];
+                x2[2 * j] = q == 0 ? w0[2] : w1[2];
+                x2[2 * j + 1] = q == 0 ? w0[3] : w1[3];
             }
         }
-        if (tid == 0) {
-            fe a = red[0][0], c = red[1][0], o = red[2][0];
-            for (int k = 1; k < SC_TAIL_THREADS / 64; ++k) {
-                a = gf::add(a, red[0][k]); c = gf::add(c, red[1][k]); o = gf::add(o, red[2][k]);
-            }
-            sh_r = finalize_round(srs, st, round, fs_rounds, a, c, o);
+        if (tid < 4) {
+            fe prev = sh_prev, y = sh_y;
+            const fe rr = finalize_round(st, round, fs_rounds, derive_y, prev, y, A, C, O, true);
+            if (tid == 0) { sh_prev = prev; sh_y = y; sh_r = rr; }
         }
         __syncthreads();
         cur = after;
         pending = true;
     }
+}
+
+__global__ void sc_init_kernel(ScState *st, fe prev, fe y, int derive_y) {
+    st->tr_prev[0] = prev;
+    st->tr_y[0] = y;
+    st->derive_y = derive_y;
+    st->err = 0;
 }
 
 }  // namespace
@@ -247,40 +275,51 @@ int sumcheck_init(cosnark_ctx *ctx) {
     return COSNARK_OK;
 }
 
-int sumcheck_launch(cosnark_ctx *ctx, const fe *v1, const fe *v2, uint64_t n, int fs_rounds, fe *buf) {
+int sumcheck_state_init(cosnark_ctx *ctx, const fe &prev, const fe &y, bool derive_y, hipStream_t stream) {
+    hipLaunchKernelGGL(sc_init_kernel, dim3(1), dim3(1), 0, stream, ctx->sc, prev, y, (int)derive_y);
+    LAUNCHCK();
+    return COSNARK_OK;
+}
+
+int sumcheck_launch(cosnark_ctx *ctx, const fe *v1, const fe *v2, uint64_t n, int fs_rounds, fe *buf,
+                    hipStream_t stream) {
     const int m = ilog2(n);
     // buffers: ping (n/2 per vector) and pong (n/4 per vector)
     fe *a1 = buf, *a2 = buf + n / 2, *b1 = buf + n, *b2 = buf + n + n / 4;
+    fe *part[2] = {ctx->sc_partials, ctx->sc_partials + 3 * SC_MAX_BLOCKS};
     const fe *s1 = v1, *s2 = v2;
     uint64_t len = n;        // elements in s1/s2
     int round = 0;
     bool pending = false;    // s1/s2 still need the fold with r[round-1]
     fe *outs[2][2] = {{a1, a2}, {b1, b2}};
-    int which = 0;
+    int which = 0, nb_prev = 0;
     while (len > (uint64_t)SC_TAIL_ELEMS) {
         const uint64_t after = pending ? len / 2 : len;
         const uint64_t npairs = after / 2;
-        uint64_t blocks = (npairs + SC_THREADS - 1) / SC_THREADS;
+        uint64_t blocks = (npairs + SC_THREADS * SC_ITEMS - 1) / (SC_THREADS * SC_ITEMS);
         if (blocks > (uint64_t)SC_MAX_BLOCKS) blocks = SC_MAX_BLOCKS;
+        const fe *pin = part[(round + 1) & 1];
+        fe *pout = part[round & 1];
         if (pending) {
             fe *o1 = outs[which][0], *o2 = outs[which][1];
-            hipLaunchKernelGGL(sc_round_kernel<true>, dim3((unsigned)blocks), dim3(SC_THREADS), 0, ctx->stream,
-                               s1, s2, o1, o2, npairs, round, fs_rounds, ctx->sc, ctx->sc_partials);
+            hipLaunchKernelGGL(sc_round_kernel<true>, dim3((unsigned)blocks), dim3(SC_THREADS), 0, stream,
+                               s1, s2, o1, o2, npairs, round, fs_rounds, ctx->sc, pin, nb_prev, pout);
             s1 = o1; s2 = o2;
             which ^= 1;
         } else {
-            hipLaunchKernelGGL(sc_round_kernel<false>, dim3((unsigned)blocks), dim3(SC_THREADS), 0, ctx->stream,
-                               s1, s2, (fe *)nullptr, (fe *)nullptr, npairs, round, fs_rounds, ctx->sc,
-                               ctx->sc_partials);
+            hipLaunchKernelGGL(sc_round_kernel<false>, dim3((unsigned)blocks), dim3(SC_THREADS), 0, stream,
+                               s1, s2, (fe *)nullptr, (fe *)nullptr, npairs, round, fs_rounds, ctx->sc, pin, 0,
+                               pout);
         }
         LAUNCHCK();
+        nb_prev = (int)blocks;
         len = after;
         pending = true;
         ++round;
     }
     const size_t lds = 2 * SC_TAIL_ELEMS * sizeof(fe);
-    hipLaunchKernelGGL(sc_tail_kernel, dim3(1), dim3(SC_TAIL_THREADS), lds, ctx->stream,
-                       s1, s2, (uint32_t)len, round, m, fs_rounds, ctx->sc);
+    hipLaunchKernelGGL(sc_tail_kernel, dim3(1), dim3(SC_TAIL_THREADS), lds, stream, s1, s2, (uint32_t)len, round, m,
+                       fs_rounds, ctx->sc, (const fe *)part[(round + 1) & 1], nb_prev);
     LAUNCHCK();
     return COSNARK_OK;
 }
