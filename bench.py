@@ -39,6 +39,7 @@ def parse():
     ap.add_argument("--rate", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--serial", action="store_true", help="run the stages one after another (profiling)")
+    ap.add_argument("--no-whir", action="store_true", help="skip the WHIR (config 3) leg")
     return ap.parse_args()
 
 
@@ -50,6 +51,38 @@ def cpu_baseline(f, g, rate, gpu_proof):
     proof, _, _ = O.plain_proof(f, g, rate)
     dt = time.perf_counter() - t0
     return dt * 1e3, proof == gpu_proof.tobytes()
+
+
+def whir_leg(ctx, reps=3):
+    """Config 3: WHIR commit + open of a 2^24-element polynomial, blowup 4,
+    whir_k = 4, folded down to 2^8 (8 iterations), libc seed 0xC05A0003."""
+    import ctypes
+    seed = 0xC05A0003
+    logn, rate, stop = 24, 4, 8
+    n = 1 << logn
+    arr = cosnark.synth(seed, 0, n)
+    z = cosnark.synth(seed, 1, logn)
+    d = ctx.upload(arr)
+    K = ctx.whir_claim(d, n, z)
+    libc = ctypes.CDLL("libc.so.6")
+    commit_ms, open_ms = [], []
+    rec = None
+    for _ in range(reps):
+        ctx.sync()
+        t0 = time.perf_counter()
+        cw, nodes = ctx.whir_commit(d, n, rate)
+        t1 = time.perf_counter()
+        libc.srandom(seed)
+        rc, rec = ctx.whir_open(d, n, z, K, rate, stop, cw, nodes)
+        t2 = time.perf_counter()
+        if rc != 0:
+            raise RuntimeError(f"whir_open failed: {rc}")
+        commit_ms.append((t1 - t0) * 1e3)
+        open_ms.append((t2 - t1) * 1e3)
+        cw.free(); nodes.free()
+    return {"workload": "WHIR 2^24 evaluations, blowup 4, whir_k 4, down to 2^8 (8 iterations)",
+            "commit_ms": round(min(commit_ms), 3), "open_ms": round(min(open_ms), 3),
+            "iterations": int(rec[0]), "record_words": int(rec.shape[0])}
 
 
 def main():
@@ -122,6 +155,9 @@ def main():
                      "alg_bytes_per_launch": int(per_launch_bytes),
                      "avg_launch_ms": round(per_launch_s * 1e3, 5)},
     }
+    if not a.no_whir:
+        df.free(); dg.free()
+        out["whir_cfg3"] = whir_leg(ctx)
     if not a.no_cpu:
         cpu_ms, same = cpu_baseline(f, g, a.rate, proof)
         out["cpu_baseline"] = {"value": round(cpu_ms, 1), "unit": "ms per rank share", "cores": 1,

@@ -129,6 +129,33 @@ int cosnark_prove_plain(cosnark_ctx *ctx, const uint64_t *d_f, const uint64_t *d
                         uint64_t n, int rate, cosnark_transcript *ts,
                         uint8_t *h_proof, double *times);
 
+/* ---- WHIR (whir_k = 4), src/Whir.cpp -----------------------------------
+ * Whir_Commit_initial (:158-190): d_codeword = RS encode of d_arr at `rate`
+ * (rate*n elements), d_nodes = its Merkle tree (MT_commit_Blake, 4 elements
+ * per leaf, cosnark_merkle_node_count(rate*n) digests). */
+int cosnark_whir_commit(cosnark_ctx *ctx, const uint64_t *d_arr, uint64_t n, int rate,
+                        uint64_t *d_codeword, uint8_t *d_nodes);
+/* The claim of Whir_Open (:392-395): K = sum_x arr[x] * eq(z, x), with the
+ * eq table of precompute_beta (src/utils.cpp:198-243); h_z holds log2(n)
+ * elements. */
+int cosnark_whir_claim(cosnark_ctx *ctx, const uint64_t *d_arr, uint64_t n, const uint64_t *h_z,
+                       uint64_t K[2]);
+/* Whir_Open (:388-467) over one claim (K, z), with the commitment from
+ * cosnark_whir_commit; d_arr is read only.  Draws its challenges from libc
+ * random()/rand() in the reference's order (seed with srandom).  Iterates
+ * until |arr| <= max(4, 2^stop_log) (stop_log <= 2: the reference's loop).
+ * Writes the opening record (u64 words, *words of them) to h_record:
+ *   [iterations]; per iteration: 2 x (a,b,c), 2 challenges, new root (4
+ *   words), l, l positions, l x 4 queried values, l folded values, l auth
+ *   paths (log2(|cw|/4) digests each), update_betas challenge; then the final
+ *   arr, K, the final commitment root and the closing challenges.
+ * Returns COSNARK_E_CHECK when the claim chain fails the reference's own
+ * checks (Whir.cpp:238-241, 454-457). */
+int cosnark_whir_open(cosnark_ctx *ctx, const uint64_t *d_arr, uint64_t n, const uint64_t *h_z,
+                      const uint64_t K[2], int rate, int stop_log, const uint64_t *d_codeword,
+                      const uint8_t *d_nodes, uint64_t *h_record, uint64_t record_cap,
+                      uint64_t *words);
+
 #ifdef __cplusplus
 }
 #endif

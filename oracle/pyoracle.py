@@ -268,3 +268,18 @@ def plain_proof(f, g, rate, prev=None):
     rc, polys, r, fin = product_sumcheck(y, f, g, prev, 0)
     assert rc == 0
     return nodes[-1].tobytes() + polys.tobytes() + fin.tobytes(), y, r
+
+
+def whir_open(arr, z, K, rate, stop_log=0, cap=1 << 22):
+    """Whir_Open record of the oracle (draws from libc random(); seed first)."""
+    a = np.ascontiguousarray(arr, dtype=np.uint64)
+    zz = np.ascontiguousarray(z, dtype=np.uint64)
+    rec = np.zeros(cap, np.uint64)
+    w = oracle().or_whir_open(_p(a), a.shape[0], _p(zz), fe(K), rate, stop_log, _p(rec), cap)
+    if w < 0:
+        return int(w), None
+    return 0, rec[:w].copy()
+
+
+def whir_claim(arr, z):
+    return inner_product(arr, precompute_beta(z))

@@ -66,6 +66,9 @@ def lib():
         L.cosnark_plain_proof_bytes.argtypes = [u64]
         L.cosnark_plain_proof_bytes.restype = u64
         L.cosnark_prove_plain.argtypes = [vp, vp, vp, u64, i32, vp, vp, vp]
+        L.cosnark_whir_commit.argtypes = [vp, vp, u64, i32, vp, vp]
+        L.cosnark_whir_claim.argtypes = [vp, vp, u64, vp, vp]
+        L.cosnark_whir_open.argtypes = [vp, vp, u64, vp, vp, i32, i32, vp, vp, vp, u64, ctypes.POINTER(u64)]
         _lib = L
     return _lib
 
@@ -250,3 +253,27 @@ class Context:
         self._ck(lib().cosnark_prove_plain(self.h, df.ptr, dg.ptr, n, rate, ctypes.byref(ts), _p(proof),
                                            _p(times)))
         return proof, times
+
+    # ---- WHIR ----
+    def whir_commit(self, arr_dev, n, rate):
+        """Whir_Commit_initial: (codeword, nodes) device buffers."""
+        cw = self.alloc(16 * n * rate)
+        nodes = self.alloc(32 * merkle_node_count(n * rate))
+        self._ck(lib().cosnark_whir_commit(self.h, arr_dev.ptr, n, rate, cw.ptr, nodes.ptr))
+        return cw, nodes
+
+    def whir_claim(self, arr_dev, n, z):
+        zz = np.ascontiguousarray(z, dtype=np.uint64)
+        K = np.zeros(2, np.uint64)
+        self._ck(lib().cosnark_whir_claim(self.h, arr_dev.ptr, n, _p(zz), _p(K)))
+        return K
+
+    def whir_open(self, arr_dev, n, z, K, rate, stop_log, cw, nodes, cap=1 << 22):
+        """Returns (status, record words)."""
+        zz = np.ascontiguousarray(z, dtype=np.uint64)
+        KK = np.ascontiguousarray(K, dtype=np.uint64).reshape(2)
+        rec = np.zeros(cap, np.uint64)
+        words = ctypes.c_uint64()
+        rc = lib().cosnark_whir_open(self.h, arr_dev.ptr, n, _p(zz), _p(KK), rate, stop_log, cw.ptr, nodes.ptr,
+                                     _p(rec), cap, ctypes.byref(words))
+        return rc, rec[: words.value].copy()

@@ -23,7 +23,7 @@ int hip_err(cosnark_ctx *ctx, hipError_t e, const char *what) {
 
 int scratch_reserve(cosnark_ctx *ctx, size_t bytes, void **base) {
     if (bytes > ctx->scratch_bytes) {
-        HIPCK(hipStreamSynchronize(ctx->stream));
+        HIPCK(hipDeviceSynchronize());
         if (ctx->scratch) HIPCK(hipFree(ctx->scratch));
         ctx->scratch = nullptr;
         ctx->scratch_bytes = 0;
@@ -34,6 +34,20 @@ int scratch_reserve(cosnark_ctx *ctx, size_t bytes, void **base) {
     return COSNARK_OK;
 }
 
+int ntt_tmp_reserve(cosnark_ctx *ctx, size_t bytes, void **base) {
+    if (bytes > ctx->ntt_tmp_bytes) {
+        // work on either stream may still use the old buffer
+        HIPCK(hipDeviceSynchronize());
+        if (ctx->ntt_tmp) HIPCK(hipFree(ctx->ntt_tmp));
+        ctx->ntt_tmp = nullptr;
+        ctx->ntt_tmp_bytes = 0;
+        HIPCK(hipMalloc(&ctx->ntt_tmp, bytes));
+        ctx->ntt_tmp_bytes = bytes;
+    }
+    *base = ctx->ntt_tmp;
+    return COSNARK_OK;
+}
+
 static size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 // ---------------------------------------------------------------------
@@ -41,13 +55,13 @@ static size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 // 363-368): F::_random() = two 20-digit decimal numbers from random()
 // ---------------------------------------------------------------------
 
-static uint64_t libc_random_number() {
+uint64_t libc_random_number() {
     uint64_t ret = (uint64_t)(random() % 10);
     for (int i = 1; i < 20; ++i) ret = (ret * 10ull + (uint64_t)(random() % 10)) % gf::P;
     return ret;
 }
 
-static fe libc_random_element() {
+fe libc_random_element() {
     fe r;
     r.re = libc_random_number() % gf::P;
     r.im = libc_random_number() % gf::P;
@@ -66,6 +80,10 @@ int cosnark_create(int device, cosnark_ctx **out) {
     auto fail = [&](int rc) { cosnark_destroy(ctx); return rc; };
     hipError_t e = hipSetDevice(device);
     if (e != hipSuccess) { delete ctx; return COSNARK_E_HIP; }
+    if (const char *t = getenv("COSNARK_NTT_PASSBITS")) {
+        const int v = atoi(t);
+        if (v >= 2 && v <= NTT_PASS_LOG_MAX) ctx->ntt_passbits = v;
+    }
     if (const char *t = getenv("COSNARK_NTT_TILE")) {
         const int v = atoi(t);
         if (v >= 256 && v <= NTT_LDS_ELEMS && is_pow2((uint64_t)v)) ctx->ntt_tile = v;
@@ -130,6 +148,7 @@ void cosnark_destroy(cosnark_ctx *ctx) {
     (void)hipFree(ctx->sc); (void)hipFree(ctx->sc_partials);
     if (ctx->sc_host) (void)hipHostFree(ctx->sc_host);
     if (ctx->scratch) (void)hipFree(ctx->scratch);
+    if (ctx->ntt_tmp) (void)hipFree(ctx->ntt_tmp);
     if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -321,7 +340,7 @@ static int sc_stage_state(cosnark_ctx *ctx, const fe &prev, const fe &y, int der
         HIPCK(hipMemcpyAsync(ctx->sc->ext_r + fs_rounds, ctx->sc_host->ext_r + fs_rounds,
                              sizeof(fe) * (m - fs_rounds), hipMemcpyHostToDevice, s));
     }
-    return sumcheck_state_init(ctx, prev, y, derive_y != 0, s);
+    return sumcheck_state_init(ctx, prev, y, derive_y, s);
 }
 
 // one copy of the device state into the pinned mirror, then results out

@@ -26,7 +26,7 @@ struct ScState {
     gf::fe final_v[2];
     uint8_t root[32];     // plain prover: Merkle root, gathered for one copy-out
     int32_t err;          // 0, or round+1 of the first failed H(0)+H(1)=y check
-    int32_t derive_y;     // 1: take y = H(0)+H(1) of round 0 (claim computed here)
+    int32_t derive_y;     // claim mode: SC_CLAIM_CHECK / _DERIVE / _UNCHECKED
 };
 
 struct cosnark_ctx {
@@ -41,11 +41,14 @@ struct cosnark_ctx {
     // growable scratch arena
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
+    void *ntt_tmp = nullptr;          // NTT passes before the natural-order one
+    size_t ntt_tmp_bytes = 0;
     ScState *sc = nullptr;            // device
     ScState *sc_host = nullptr;       // pinned host mirror for copy-in / copy-out
     gf::fe *sc_partials = nullptr;    // device, 2 x 3 * SC_MAX_BLOCKS elements (ping-pong)
     hipEvent_t ev[8] = {};
     int ntt_tile = NTT_LDS_ELEMS;     // elements per NTT workgroup tile (power of two, <= 4096)
+    int ntt_passbits = NTT_PASS_LOG_MAX;  // largest radix (bits) of one NTT pass
     bool overlap = true;              // plain prover: sumcheck beside the commit
 };
 
@@ -55,6 +58,7 @@ int set_err(cosnark_ctx *ctx, int code, const std::string &msg);
 int hip_err(cosnark_ctx *ctx, hipError_t e, const char *what);
 // ensure the scratch arena holds at least `bytes`; returns base pointer
 int scratch_reserve(cosnark_ctx *ctx, size_t bytes, void **base);
+int ntt_tmp_reserve(cosnark_ctx *ctx, size_t bytes, void **base);
 
 #define HIPCK(expr)                                              \
     do {                                                         \
@@ -68,6 +72,10 @@ int scratch_reserve(cosnark_ctx *ctx, size_t bytes, void **base);
         if (e_ != hipSuccess) return hip_err(ctx, e_, "kernel launch");    \
     } while (0)
 
+// F::_random() of the reference (src/fieldElement.cpp:119-125, 363-368):
+// two 20-digit decimal draws from libc random(), consumed on the calling thread
+gf::fe libc_random_element();
+
 inline bool is_pow2(uint64_t x) { return x && !(x & (x - 1)); }
 inline int ilog2(uint64_t x) { int l = 0; while ((1ull << l) < x) ++l; return l; }
 
@@ -80,7 +88,18 @@ int ntt_launch(cosnark_ctx *ctx, const gf::fe *in, uint64_t in_len, gf::fe *out,
 int merkle_launch(cosnark_ctx *ctx, const gf::fe *elems, uint64_t n_elems, uint8_t *nodes,
                   bool dup_left, hipStream_t s);
 int sumcheck_init(cosnark_ctx *ctx);
-int sumcheck_state_init(cosnark_ctx *ctx, const gf::fe &prev, const gf::fe &y, bool derive_y, hipStream_t s);
+// claim handling of round 0 / every round: check against y, derive y from
+// round 0, or leave unchecked (the caller verifies the claim chain)
+constexpr int SC_CLAIM_CHECK = 0, SC_CLAIM_DERIVE = 1, SC_CLAIM_UNCHECKED = 2;
+int sumcheck_state_init(cosnark_ctx *ctx, const gf::fe &prev, const gf::fe &y, int claim_mode, hipStream_t s);
+// one multi-workgroup round (fold by the previous round's challenge when
+// `fold`); *nb_out = workgroups whose partials the next launch reduces
+int sumcheck_round_launch(cosnark_ctx *ctx, const gf::fe *s1, const gf::fe *s2, gf::fe *d1, gf::fe *d2,
+                          uint64_t npairs, int round, bool fold, int fs_rounds, int nb_prev, hipStream_t s,
+                          int *nb_out);
+// finalize `round` and fold both vectors by its challenge into nout elements
+int sumcheck_fold_launch(cosnark_ctx *ctx, const gf::fe *s1, const gf::fe *s2, gf::fe *d1, gf::fe *d2, uint64_t nout,
+                         int round, int fs_rounds, int nb_prev, hipStream_t s);
 int ntt_init(cosnark_ctx *ctx);
 // Product sumcheck rounds; state in ctx->sc (initialised by the caller).
 // scratch buffers: a1,a2,b1,b2 each of n/2 elements.

@@ -6,11 +6,11 @@
 // is materialised because openings (open_tree_blake, :320-335) serve
 // siblings from all levels.
 //
-// One message per lane.  A workgroup of 256 lanes hashes a 1024-node block
-// of its first level (4 per lane, coalesced 64-byte loads), keeps the
-// digests in LDS and climbs the tree inside the block (10 more levels),
-// writing every node; the block roots feed the next launch.  A 2^22-leaf
-// tree takes three launches.
+// One message per lane.  A workgroup of 256 lanes takes a 1024-node block of
+// its first level: each lane hashes four consecutive nodes and the two
+// levels above them in registers, then the workgroup climbs the remaining
+// eight levels of the block through LDS, writing every node; the block roots
+// feed the next launch.  A 2^22-leaf tree takes three launches.
 #include "ctx.h"
 #include "blake3.h"
 
@@ -35,60 +35,105 @@ __device__ __forceinline__ void store_digest(uint8_t *nodes, uint64_t idx, const
     p[1] = make_uint4(h[4], h[5], h[6], h[7]);
 }
 
+__device__ __forceinline__ void hash_pair(const uint32_t l[8], const uint32_t r[8], uint32_t h[8]) {
+    uint32_t m[16];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) { m[j] = l[j]; m[8 + j] = r[j]; }
+    b3::compress64(m, h);
+}
+
 // FROM_ELEMS: the first level is the leaf level (input = elements).
 // Otherwise the first level is level `lvl0` computed from level lvl0-1.
+// A lane owns 2^RL consecutive first-level nodes (RL = min(2, block_log)) and
+// climbs RL levels in registers; the workgroup then climbs the remaining
+// block_log - RL levels through LDS.  All input loads of a lane are issued
+// before its first compression.
 template <bool FROM_ELEMS>
 __global__ __launch_bounds__(MK_THREADS) void merkle_block_kernel(
     const fe *__restrict__ elems, const uint8_t *in_nodes, uint8_t *nodes,
     LevelOffsets lo, int lvl0, int block_log, int dup_left) {
-    __shared__ Digest bufA[MK_BLOCK];
-    __shared__ Digest bufB[MK_BLOCK / 2];
+    __shared__ Digest bufA[MK_BLOCK / 4];
+    __shared__ Digest bufB[MK_BLOCK / 8];
     const int tid = threadIdx.x;
-    const int T = 1 << block_log;                       // first-level nodes in this block
+    const int RL = block_log < 2 ? block_log : 2;
+    const int per = 1 << RL;                            // first-level nodes per lane
+    const int lanes = (1 << block_log) >> RL;           // active lanes
     const uint64_t g0 = (uint64_t)blockIdx.x << block_log;
 
-    for (int i = tid; i < T; i += MK_THREADS) {
-        const uint64_t g = g0 + i;
-        uint32_t m[16], h[8];
-        if (FROM_ELEMS) {
-            const uint4 *src = reinterpret_cast<const uint4 *>(elems + 4 * g);
+    if (tid < lanes) {
+        const uint64_t gl = g0 + (uint64_t)tid * per;   // first node of this lane
+        uint32_t msg[4][16];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const uint4 q = src[j];
-                m[4 * j] = q.x; m[4 * j + 1] = q.y; m[4 * j + 2] = q.z; m[4 * j + 3] = q.w;
+        for (int i = 0; i < 4; ++i) {
+            if (i < per) {
+                const uint64_t g = gl + i;
+                if (FROM_ELEMS) {
+                    const uint4 *src = reinterpret_cast<const uint4 *>(elems + 4 * g);
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const uint4 q = src[j];
+                        msg[i][4 * j] = q.x; msg[i][4 * j + 1] = q.y; msg[i][4 * j + 2] = q.z; msg[i][4 * j + 3] = q.w;
+                    }
+                } else {
+                    const uint4 *L = reinterpret_cast<const uint4 *>(in_nodes + 32 * (2 * g));
+                    const uint4 *Rn = reinterpret_cast<const uint4 *>(in_nodes + 32 * (2 * g + (dup_left ? 0 : 1)));
+                    const uint4 l0 = L[0], l1 = L[1], r0 = Rn[0], r1 = Rn[1];
+                    uint32_t *m = msg[i];
+                    m[0] = l0.x; m[1] = l0.y; m[2] = l0.z; m[3] = l0.w;
+                    m[4] = l1.x; m[5] = l1.y; m[6] = l1.z; m[7] = l1.w;
+                    m[8] = r0.x; m[9] = r0.y; m[10] = r0.z; m[11] = r0.w;
+                    m[12] = r1.x; m[13] = r1.y; m[14] = r1.z; m[15] = r1.w;
+                }
+            }
+        }
+        uint32_t h[4][8];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (i < per) {
+                b3::compress64(msg[i], h[i]);
+                store_digest(nodes, lo.off[lvl0] + gl + i, h[i]);
+            }
+        }
+        // register levels: lvl0+1 (two nodes), lvl0+2 (one node)
+        if (RL >= 1) {
+            uint32_t p0[8], p1[8];
+            hash_pair(h[0], dup_left ? h[0] : h[1], p0);
+            store_digest(nodes, lo.off[lvl0 + 1] + (gl >> 1), p0);
+            if (RL >= 2) {
+                hash_pair(h[2], dup_left ? h[2] : h[3], p1);
+                store_digest(nodes, lo.off[lvl0 + 1] + (gl >> 1) + 1, p1);
+                uint32_t q[8];
+                hash_pair(p0, dup_left ? p0 : p1, q);
+                store_digest(nodes, lo.off[lvl0 + 2] + (gl >> 2), q);
+#pragma unroll
+                for (int j = 0; j < 8; ++j) bufA[tid].w[j] = q[j];
+            } else {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) bufA[tid].w[j] = p0[j];
             }
         } else {
-            const uint4 *L = reinterpret_cast<const uint4 *>(in_nodes + 32 * (2 * g));
-            const uint4 *Rn = reinterpret_cast<const uint4 *>(in_nodes + 32 * (2 * g + (dup_left ? 0 : 1)));
-            const uint4 l0 = L[0], l1 = L[1], r0 = Rn[0], r1 = Rn[1];
-            m[0] = l0.x; m[1] = l0.y; m[2] = l0.z; m[3] = l0.w;
-            m[4] = l1.x; m[5] = l1.y; m[6] = l1.z; m[7] = l1.w;
-            m[8] = r0.x; m[9] = r0.y; m[10] = r0.z; m[11] = r0.w;
-            m[12] = r1.x; m[13] = r1.y; m[14] = r1.z; m[15] = r1.w;
-        }
-        b3::compress64(m, h);
-        store_digest(nodes, lo.off[lvl0] + g, h);
 #pragma unroll
-        for (int j = 0; j < 8; ++j) bufA[i].w[j] = h[j];
+            for (int j = 0; j < 8; ++j) bufA[tid].w[j] = h[0][j];
+        }
     }
 
-    // climb inside the block: level lvl0 + s has T >> s nodes
+    // climb inside the block through LDS: level lvl0 + s has T >> s nodes
     Digest *src = bufA, *dst = bufB;
-    for (int s = 1; s <= block_log; ++s) {
+    for (int s = RL + 1; s <= block_log; ++s) {
         __syncthreads();
-        const int cnt = T >> s;
+        const int cnt = (1 << block_log) >> s;
         const uint64_t gbase = g0 >> s;
         for (int i = tid; i < cnt; i += MK_THREADS) {
-            uint32_t m[16], h[8];
+            uint32_t m[16], hh[8];
 #pragma unroll
             for (int j = 0; j < 8; ++j) m[j] = src[2 * i].w[j];
             const int ri = dup_left ? 2 * i : 2 * i + 1;
 #pragma unroll
             for (int j = 0; j < 8; ++j) m[8 + j] = src[ri].w[j];
-            b3::compress64(m, h);
-            store_digest(nodes, lo.off[lvl0 + s] + gbase + i, h);
+            b3::compress64(m, hh);
+            store_digest(nodes, lo.off[lvl0 + s] + gbase + i, hh);
 #pragma unroll
-            for (int j = 0; j < 8; ++j) dst[i].w[j] = h[j];
+            for (int j = 0; j < 8; ++j) dst[i].w[j] = hh[j];
         }
         Digest *t = src; src = dst; dst = t;
     }

@@ -31,10 +31,6 @@ struct NttPass {
     int pre[8];         // pre[t] = rbits[0] + .. + rbits[t-1]
 };
 
-__device__ __forceinline__ uint32_t brev(uint32_t x, int bits) {
-    return bits == 0 ? 0u : (__brev(x) >> (32 - bits));
-}
-
 template <int N>
 constexpr int brev_c(int x) {
     int r = 0;
@@ -85,7 +81,6 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
     constexpr int R = R1 * R2;
     constexpr int LOGR = (R >= 256) ? 8 : (R >= 128) ? 7 : (R >= 64) ? 6 : (R >= 32) ? 5 : (R >= 16) ? 4 : (R >= 8) ? 3 : (R >= 4) ? 2 : (R >= 2) ? 1 : 0;
     constexpr int LOGR2 = (R2 >= 16) ? 4 : (R2 >= 8) ? 3 : (R2 >= 4) ? 2 : (R2 >= 2) ? 1 : 0;
-    constexpr int LOGR1 = LOGR - LOGR2;
     extern __shared__ fe ntt_lds[];
     fe *wr = ntt_lds;          // w_R^e, e < R
     fe *Y = ntt_lds + R;       // (R * C + R * Y_PAD) elements
@@ -261,7 +256,8 @@ int ntt_launch(cosnark_ctx *ctx, const fe *in, uint64_t in_len, fe *out, int log
                hipStream_t stream) {
     if (logn < 1 || logn > NTT_LOG_MAX) return set_err(ctx, COSNARK_E_RANGE, "ntt: logn out of range");
     // radix plan: fewest passes of at most 8 bits, bits spread evenly, larger first
-    const int m = (logn + NTT_PASS_LOG_MAX - 1) / NTT_PASS_LOG_MAX;
+    const int pmax = ctx->ntt_passbits;
+    const int m = (logn + pmax - 1) / pmax;
     NttPass p{};
     p.in_len = in_len;
     p.logN = logn;
@@ -273,6 +269,15 @@ int ntt_launch(cosnark_ctx *ctx, const fe *in, uint64_t in_len, fe *out, int log
     const fe *twhi = inverse ? ctx->itw_hi : ctx->tw_hi;
     const fe *rtab = inverse ? ctx->irtw : ctx->rtw;
     const fe scale = gf::inv(gf::from_u64(1ull << logn));
+    // The last pass scatters rows into natural order, so it cannot run in
+    // place: passes 0..m-2 work in a context-owned temporary and the last
+    // one writes `out`.  (A single pass is one workgroup: reads finish
+    // before its writes start.)
+    fe *tmp = nullptr;
+    if (m >= 2) {
+        int rc = ntt_tmp_reserve(ctx, sizeof(fe) << logn, (void **)&tmp);
+        if (rc) return rc;
+    }
     int done_bits = 0;
     for (int t = 0; t < m; ++t) {
         const int r = p.rbits[t];
@@ -292,8 +297,9 @@ int ntt_launch(cosnark_ctx *ctx, const fe *in, uint64_t in_len, fe *out, int log
         q.logC = logC;
         if (t > 0) q.in_len = 1ull << logn;
         const uint64_t blocks = (1ull << logn) >> (r + logC);
-        int rc = dispatch_pass(ctx, stream, r, blocks, t == 0 ? in : out, out, q, last, inverse, twlo, twhi,
-                               rtab, scale);
+        const fe *src = (t == 0) ? in : tmp;
+        fe *dst = last ? out : tmp;
+        int rc = dispatch_pass(ctx, stream, r, blocks, src, dst, q, last, inverse, twlo, twhi, rtab, scale);
         if (rc) return rc;
         done_bits += r;
     }

@@ -69,14 +69,14 @@ __device__ __forceinline__ void block_reduce3(fe &A, fe &C, fe &O, fe (*red)[NT 
 // libc stream), update (prev, y), return r.  Called by lanes 0..3 of a wave
 // with the same inputs (they share the four-lane hash); when `writer`,
 // lane 0 records the round in st.
-__device__ fe finalize_round(ScState *st, int round, int fs_rounds, bool derive_y, fe &prev, fe &y, fe A, fe C,
+__device__ fe finalize_round(ScState *st, int round, int fs_rounds, int claim_mode, fe &prev, fe &y, fe A, fe C,
                              fe O, bool writer) {
     const fe a = A, c = C;
     const fe b = gf::sub(gf::sub(O, A), C);
     const fe h01 = gf::add(gf::add(a, b), gf::add(c, c));   // H(0) + H(1)
     bool bad = false;
-    if (round == 0 && derive_y) y = h01;
-    else bad = !gf::eq(h01, y);
+    if (round == 0 && claim_mode == SC_CLAIM_DERIVE) y = h01;
+    else if (claim_mode != SC_CLAIM_UNCHECKED) bad = !gf::eq(h01, y);
     fe r;
     if (round < fs_rounds) {
         prev = ts::hash_F_quad(prev, a);
@@ -114,7 +114,7 @@ __device__ void finish_round(ScState *st, int round, int fs_rounds, const fe *pa
     block_reduce3<NT>(A, C, O, red);
     if (threadIdx.x < 4) {
         fe prev = st->tr_prev[round], y = st->tr_y[round];
-        const fe r = finalize_round(st, round, fs_rounds, st->derive_y != 0, prev, y, A, C, O, blockIdx.x == 0);
+        const fe r = finalize_round(st, round, fs_rounds, st->derive_y, prev, y, A, C, O, blockIdx.x == 0);
         if (threadIdx.x == 0) { *sh_r = r; *sh_prev = prev; *sh_y = y; }
     }
     __syncthreads();
@@ -197,7 +197,7 @@ __global__ __launch_bounds__(SC_TAIL_THREADS) void sc_tail_kernel(
     __shared__ fe sh_r, sh_prev, sh_y;
     const int tid = threadIdx.x;
     for (uint32_t i = tid; i < len; i += SC_TAIL_THREADS) { x1[i] = s1[i]; x2[i] = s2[i]; }
-    const bool derive_y = st->derive_y != 0;
+    const int claim_mode = st->derive_y;
     if (round0 > 0) {
         finish_round<SC_TAIL_THREADS>(st, round0 - 1, fs_rounds, part_in, nb_in, red, &sh_r, &sh_prev, &sh_y);
     } else {
@@ -250,12 +250,28 @@ __global__ __launch_bounds__(SC_TAIL_THREADS) void sc_tail_kernel(
         }
         if (tid < 4) {
             fe prev = sh_prev, y = sh_y;
-            const fe rr = finalize_round(st, round, fs_rounds, derive_y, prev, y, A, C, O, true);
+            const fe rr = finalize_round(st, round, fs_rounds, claim_mode, prev, y, A, C, O, true);
             if (tid == 0) { sh_prev = prev; sh_y = y; sh_r = rr; }
         }
         __syncthreads();
         cur = after;
         pending = true;
+    }
+}
+
+// Finalize round `round` from part_in, then fold both vectors by its
+// challenge: d[j] = fold(s[2j], s[2j+1]) for j < nout (no evaluation).
+__global__ __launch_bounds__(SC_THREADS) void sc_fold_kernel(
+    const fe *__restrict__ s1, const fe *__restrict__ s2, fe *__restrict__ d1, fe *__restrict__ d2, uint64_t nout,
+    int round, int fs_rounds, ScState *st, const fe *part_in, int nb_in) {
+    __shared__ fe red[3][SC_THREADS / 64];
+    __shared__ fe sh_r, sh_prev, sh_y;
+    finish_round<SC_THREADS>(st, round, fs_rounds, part_in, nb_in, red, &sh_r, &sh_prev, &sh_y);
+    const fe r = sh_r;
+    for (uint64_t j = (uint64_t)blockIdx.x * SC_THREADS + threadIdx.x; j < nout; j += (uint64_t)gridDim.x * SC_THREADS) {
+        const fe a0 = s1[2 * j], a1 = s1[2 * j + 1], b0 = s2[2 * j], b1 = s2[2 * j + 1];
+        d1[j] = fold1(a0, a1, r);
+        d2[j] = fold1(b0, b1, r);
     }
 }
 
@@ -275,8 +291,38 @@ int sumcheck_init(cosnark_ctx *ctx) {
     return COSNARK_OK;
 }
 
-int sumcheck_state_init(cosnark_ctx *ctx, const fe &prev, const fe &y, bool derive_y, hipStream_t stream) {
-    hipLaunchKernelGGL(sc_init_kernel, dim3(1), dim3(1), 0, stream, ctx->sc, prev, y, (int)derive_y);
+int sumcheck_state_init(cosnark_ctx *ctx, const fe &prev, const fe &y, int claim_mode, hipStream_t stream) {
+    hipLaunchKernelGGL(sc_init_kernel, dim3(1), dim3(1), 0, stream, ctx->sc, prev, y, claim_mode);
+    LAUNCHCK();
+    return COSNARK_OK;
+}
+
+int sumcheck_round_launch(cosnark_ctx *ctx, const fe *s1, const fe *s2, fe *d1, fe *d2, uint64_t npairs, int round,
+                          bool fold, int fs_rounds, int nb_prev, hipStream_t stream, int *nb_out) {
+    fe *part[2] = {ctx->sc_partials, ctx->sc_partials + 3 * SC_MAX_BLOCKS};
+    uint64_t blocks = (npairs + SC_THREADS * SC_ITEMS - 1) / (SC_THREADS * SC_ITEMS);
+    if (blocks > (uint64_t)SC_MAX_BLOCKS) blocks = SC_MAX_BLOCKS;
+    if (blocks == 0) blocks = 1;
+    const fe *pin = part[(round + 1) & 1];
+    fe *pout = part[round & 1];
+    if (fold)
+        hipLaunchKernelGGL(sc_round_kernel<true>, dim3((unsigned)blocks), dim3(SC_THREADS), 0, stream, s1, s2, d1, d2,
+                           npairs, round, fs_rounds, ctx->sc, pin, nb_prev, pout);
+    else
+        hipLaunchKernelGGL(sc_round_kernel<false>, dim3((unsigned)blocks), dim3(SC_THREADS), 0, stream, s1, s2,
+                           (fe *)nullptr, (fe *)nullptr, npairs, round, fs_rounds, ctx->sc, pin, 0, pout);
+    LAUNCHCK();
+    *nb_out = (int)blocks;
+    return COSNARK_OK;
+}
+
+int sumcheck_fold_launch(cosnark_ctx *ctx, const fe *s1, const fe *s2, fe *d1, fe *d2, uint64_t nout, int round,
+                         int fs_rounds, int nb_prev, hipStream_t stream) {
+    fe *part[2] = {ctx->sc_partials, ctx->sc_partials + 3 * SC_MAX_BLOCKS};
+    uint64_t blocks = (nout + SC_THREADS - 1) / SC_THREADS;
+    if (blocks > (uint64_t)SC_MAX_BLOCKS) blocks = SC_MAX_BLOCKS;
+    hipLaunchKernelGGL(sc_fold_kernel, dim3((unsigned)blocks), dim3(SC_THREADS), 0, stream, s1, s2, d1, d2, nout,
+                       round, fs_rounds, ctx->sc, (const fe *)part[round & 1], nb_prev);
     LAUNCHCK();
     return COSNARK_OK;
 }

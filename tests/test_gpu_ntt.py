@@ -34,3 +34,37 @@ def test_rs_encode_matches_oracle(ctx, logn, rate):
     a = O.synth(0xD0 + logn, 0, 1 << logn)
     print(f"rs_encode logn={logn} rate={rate}", flush=True)
     assert np.array_equal(ctx.rs_encode(a, rate), O.rs_encode(a, rate))
+
+
+@pytest.mark.parametrize("logn", [25])
+def test_ntt_four_pass_matches_oracle(ctx, logn):
+    a = O.synth(0xF0 + logn, 0, 1 << logn)
+    print(f"ntt forward logn={logn}", flush=True)
+    assert np.array_equal(ctx.ntt(a), O.fft(a))
+
+
+def test_rs_encode_four_pass_matches_oracle(ctx):
+    a = O.synth(0xF1, 0, 1 << 22)
+    print("rs_encode 2^22 rate 8", flush=True)
+    assert np.array_equal(ctx.rs_encode(a, 8), O.rs_encode(a, 8))
+
+
+@pytest.mark.parametrize("passbits,logn", [(4, 20), (3, 20), (5, 20), (2, 11)])
+def test_ntt_many_pass_plans_match_oracle(passbits, logn):
+    # Regression: the last (natural-order) pass scatters rows, so it must not
+    # run in place; plans with 4-10 passes exercise every digit-reversal path.
+    import os
+    import cosnark
+    os.environ["COSNARK_NTT_PASSBITS"] = str(passbits)
+    try:
+        c = cosnark.Context(0)
+    finally:
+        del os.environ["COSNARK_NTT_PASSBITS"]
+    try:
+        a = O.synth(0x70 + logn, 0, 1 << logn)
+        print(f"ntt passbits={passbits} logn={logn}", flush=True)
+        assert np.array_equal(c.ntt(a), O.fft(a))
+        assert np.array_equal(c.ntt(a, inverse=True), O.fft(a, inverse=True))
+        assert np.array_equal(c.rs_encode(a[: (1 << logn) // 4], 4), O.rs_encode(a[: (1 << logn) // 4], 4))
+    finally:
+        c.close()

@@ -9,7 +9,7 @@ import pyoracle as O
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("m", [4, 10, 16])
+@pytest.mark.parametrize("m", [4, 10, 16, 20])
 def test_plain_proof_bytes_match_oracle(ctx, m):
     seed = 0xC05A0001
     n = 1 << m

@@ -26,7 +26,7 @@ def _run_both(ctx, m, seed, offset=0, libc_seed=None, prev0=(0, 0)):
     return (rc_o, polys_o, r_o, fin_o, prev), (rc_g, polys_g, r_g, fin_g, np.array(ts.value(), np.uint64))
 
 
-@pytest.mark.parametrize("m", [1, 2, 3, 5, 8, 11, 12, 13, 14, 16, 18])
+@pytest.mark.parametrize("m", [1, 2, 3, 5, 8, 11, 12, 13, 14, 16, 18, 20, 21])
 def test_sumcheck_matches_oracle(ctx, m):
     print(f"sumcheck m={m}", flush=True)
     o, g = _run_both(ctx, m, 0x5C00 + m)
