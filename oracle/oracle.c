@@ -679,3 +679,134 @@ done:
     if (R.overflow) return -1;
     return (int64_t)R.n;
 }
+
+/* ------------------------------------------------------------------ */
+/* coPCS rank-local kernels                                             */
+/* ------------------------------------------------------------------ */
+
+void or_copcs_encode(const or_fe *rows, int nrows, uint64_t L, int rate, int log_rate_offset, or_fe *out) {
+    int logL = ilog2u(L);
+    uint64_t M = L * (uint64_t)rate;
+    or_fe *tmp = (or_fe *)malloc(sizeof(or_fe) * M);
+    for (int i = 0; i < nrows; ++i) {                     /* coPCS.cpp:132-146 */
+        memcpy(tmp, rows + (uint64_t)i * L, sizeof(or_fe) * L);
+        or_fft(tmp, logL, 1);
+        or_fe omega = or_inv(or_root_of_unity(1 + log_rate_offset + logL));
+        or_fe mul = { 1, 0 };
+        for (uint64_t j = 0; j < L; ++j) {
+            tmp[j] = or_mul(mul, tmp[j]);
+            mul = or_mul(mul, omega);
+        }
+        memset(tmp + L, 0, sizeof(or_fe) * (M - L));
+        or_fft(tmp, ilog2u(M), 0);
+        memcpy(out + (uint64_t)i * M, tmp, sizeof(or_fe) * M);
+    }
+    free(tmp);
+}
+
+void or_distributed_mt(const or_fe *cw, int nrows, uint64_t Lc, uint8_t *base_nodes) {
+    /* column roots (coPCS.cpp:167-177) */
+    or_fe col[32];
+    uint8_t nodes[32 * 15];
+    for (uint64_t j = 0; j < Lc; ++j) {
+        for (int k = 0; k < 32; ++k) {
+            if (k < nrows) col[k] = cw[(uint64_t)k * Lc + j];
+            else { col[k].real = 0; col[k].img = 0; }
+        }
+        or_merkle_commit(col, 32, nodes, 1);
+        memcpy(base_nodes + 32 * j, nodes + 32 * 14, 32);
+    }
+    /* create_tree_blake over the column roots (dup-left) */
+    uint8_t *prev = base_nodes, *cur = base_nodes + 32 * Lc;
+    for (uint64_t s = Lc / 2; s >= 1; s /= 2) {
+        for (uint64_t i = 0; i < s; ++i) {
+            uint8_t data[64];
+            memcpy(data, prev + 32 * (2 * i), 32);
+            memcpy(data + 32, prev + 32 * (2 * i), 32);
+            or_blake3_hash64(data, cur + 32 * i);
+        }
+        prev = cur;
+        cur += 32 * s;
+    }
+}
+
+void or_pcs_sumcheck_phase1(const or_fe *data, const or_fe *B, uint64_t M, int d, or_fe *poly) {
+    int logd = ilog2u((uint64_t)d);
+    or_fe *h1 = (or_fe *)malloc(sizeof(or_fe) * 2 * d), *h2 = (or_fe *)malloc(sizeof(or_fe) * 2 * d);
+    for (int k = 0; k < 2 * d; ++k) { poly[k].real = 0; poly[k].img = 0; }
+    for (uint64_t j = 0; j < M / d; ++j) {                /* coPCS.cpp:553-567 */
+        memcpy(h1, data + j * d, sizeof(or_fe) * d);
+        memcpy(h2, B + j * d, sizeof(or_fe) * d);
+        or_fft(h1, logd, 1);
+        or_fft(h2, logd, 1);
+        memset(h1 + d, 0, sizeof(or_fe) * d);
+        memset(h2 + d, 0, sizeof(or_fe) * d);
+        /* coPCS.cpp:561 resizes h2 to 4d (it reads h1's new size), and
+         * :562 transforms its first 2d entries: fft's length assert
+         * (utils.cpp:912) is compiled out in the Release build (setup.sh:3
+         * -> -DNDEBUG), so the products below are of the 2d-point images. */
+        or_fft(h1, logd + 1, 0);
+        or_fft(h2, logd + 1, 0);
+        for (int k = 0; k < 2 * d; ++k) poly[k] = or_add(poly[k], or_mul(h1[k], h2[k]));
+    }
+    free(h1); free(h2);
+}
+
+void or_pcs_sumcheck_phase2(or_fe *data, or_fe *B, uint64_t M, int d, or_fe r) {
+    or_fe *L = (or_fe *)malloc(sizeof(or_fe) * d);
+    or_compute_lagrange_coeff(r, d, L);                   /* coPCS.cpp:574 */
+    for (uint64_t j = 0; j < M / d; ++j) {
+        or_fe s1 = { 0, 0 }, s2 = { 0, 0 };
+        for (int k = 0; k < d; ++k) {
+            s1 = or_add(s1, or_mul(L[k], data[j * d + k]));
+            s2 = or_add(s2, or_mul(L[k], B[j * d + k]));
+        }
+        data[j] = s1;
+        B[j] = s2;
+    }
+    free(L);
+}
+
+void or_rs_multifold5(const or_fe *cw, uint64_t n0, const or_fe *a, int fd, or_fe *out) {
+    /* partial codewords of growing length, exactly as coPCS_utils.cpp:389-433 */
+    uint64_t n = n0;
+    int log_fd = ilog2u((uint64_t)fd);
+    or_fe len_inv = or_inv(or_from_u64((uint64_t)fd));
+    uint64_t np = n0 / 2;
+    or_fe **part = (or_fe **)malloc(sizeof(or_fe *) * np);
+    for (uint64_t i = 0; i < np; ++i) part[i] = (or_fe *)calloc((size_t)fd, sizeof(or_fe));
+    or_fe omega = or_inv(or_root_of_unity(ilog2u(n)));
+    or_fe inv_omegas = { 1, 0 };
+    for (uint64_t j = 0; j < n0 / 2; ++j) {
+        part[j][0] = or_add(cw[j], cw[j + n / 2]);
+        part[j][1] = or_mul(inv_omegas, or_sub(cw[j], cw[j + n / 2]));
+        inv_omegas = or_mul(inv_omegas, omega);
+    }
+    if (fd > 2) {
+        or_fe pw[32];
+        for (int i = 0; i < log_fd; ++i) { pw[i].real = 1; pw[i].img = 0; }
+        pw[1] = or_mul(omega, omega);
+        for (int i = 2; i < log_fd; ++i) pw[i] = or_mul(pw[i - 1], pw[i - 1]);
+        for (int i = 1; i < log_fd; ++i) {
+            n = n / 2;
+            or_fe pow_omega = { 1, 0 };
+            uint64_t half = 1ULL << i;                    /* old length of part[j] */
+            for (uint64_t j = 0; j < n0 / (1ULL << (i + 1)); ++j) {
+                for (uint64_t k = 0; k < half; ++k) {
+                    or_fe v1 = or_add(part[j][k], part[j + n / 2][k]);
+                    or_fe v2 = or_mul(pow_omega, or_sub(part[j][k], part[j + n / 2][k]));
+                    part[j][k] = v1;
+                    part[j][k + half] = v2;
+                }
+                pow_omega = or_mul(pow_omega, pw[i]);
+            }
+        }
+    }
+    for (uint64_t i = 0; i < n0 / (uint64_t)fd; ++i) {
+        or_fe acc = { 0, 0 };
+        for (int j = 0; j < fd; ++j) acc = or_add(acc, or_mul(a[j], part[i][j]));
+        out[i] = or_mul(acc, len_inv);
+    }
+    for (uint64_t i = 0; i < np; ++i) free(part[i]);
+    free(part);
+}

@@ -106,6 +106,27 @@ or_fe or_rs_multifold_segment_tensor(const or_fe *seg, int seg_n, const or_fe *r
 int64_t or_whir_open(const or_fe *arr, uint64_t n, const or_fe *z, or_fe K,
                      int rate, int stop_log, uint64_t *record, uint64_t record_cap);
 
+/* ---- coPCS rank-local kernels ---- */
+/* Encode (src/coPCS.cpp:131-147) of nrows rows of L elements each: iNTT_L,
+ * times w^-j with w = root(1 + log_rate_offset + log2 L), zero-pad to
+ * rate*L, NTT.  out: nrows x rate*L, row major. */
+void or_copcs_encode(const or_fe *rows, int nrows, uint64_t L, int rate, int log_rate_offset, or_fe *out);
+/* distributed_MT at one rank (src/coPCS.cpp:152-183): the codeword matrix
+ * has 32 rows (BATCH_FACTOR, src/New_PCS.cpp:9) of Lc elements, rows >=
+ * nrows zero; column j -> MT_commit_Blake of its 32 elements (8 leaves) ->
+ * the column root; create_tree_blake over the Lc column roots.  base_nodes
+ * receives Lc + Lc/2 + ... + 1 digests (Com.Base_MT, level 0 first). */
+void or_distributed_mt(const or_fe *cw, int nrows, uint64_t Lc, uint8_t *base_nodes);
+/* pcs_sumcheck_phase1 before aggregation (src/coPCS.cpp:547-569): poly (2d)
+ * = sum over blocks j of NTT_2d(pad(iNTT_d(data block))) * same for B. */
+void or_pcs_sumcheck_phase1(const or_fe *data, const or_fe *B, uint64_t M, int d, or_fe *poly);
+/* pcs_sumcheck_phase2 (src/coPCS.cpp:571-587): data[j] = sum_k L[k]
+ * data[jd+k] (same for B), L = compute_lagrange_coeff(r, d); the first M/d
+ * entries of data and B are overwritten. */
+void or_pcs_sumcheck_phase2(or_fe *data, or_fe *B, uint64_t M, int d, or_fe r);
+/* RS_multifold5 (src/coPCS_utils.cpp:386-437): out has n/fd elements. */
+void or_rs_multifold5(const or_fe *cw, uint64_t n, const or_fe *a, int fd, or_fe *out);
+
 #ifdef __cplusplus
 }
 #endif

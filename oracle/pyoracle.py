@@ -92,6 +92,11 @@ def oracle():
         L.or_rs_multifold_segment_tensor.restype = _Fe
         L.or_whir_open.argtypes = [vp, u64, vp, _Fe, i32, i32, vp, u64]
         L.or_whir_open.restype = ctypes.c_int64
+        L.or_copcs_encode.argtypes = [vp, i32, u64, i32, i32, vp]
+        L.or_distributed_mt.argtypes = [vp, i32, u64, vp]
+        L.or_pcs_sumcheck_phase1.argtypes = [vp, vp, u64, i32, vp]
+        L.or_pcs_sumcheck_phase2.argtypes = [vp, vp, u64, i32, _Fe]
+        L.or_rs_multifold5.argtypes = [vp, u64, vp, i32, vp]
         _o = L
     return _o
 
@@ -283,3 +288,55 @@ def whir_open(arr, z, K, rate, stop_log=0, cap=1 << 22):
 
 def whir_claim(arr, z):
     return inner_product(arr, precompute_beta(z))
+
+
+# ---- coPCS rank-local kernels ----------------------------------------------
+
+def copcs_encode(rows, rate, log_rate_offset=0):
+    """rows: (nrows, L, 2).  Returns (nrows, rate*L, 2)."""
+    r = np.ascontiguousarray(rows, dtype=np.uint64)
+    nrows, L = r.shape[0], r.shape[1]
+    out = np.zeros((nrows, rate * L, 2), np.uint64)
+    oracle().or_copcs_encode(_p(r), nrows, L, rate, log_rate_offset, _p(out))
+    return out
+
+
+def distributed_mt(cw):
+    """cw: (nrows <= 32, Lc, 2).  Returns the Base_MT digests (2*Lc-1, 32)."""
+    c = np.ascontiguousarray(cw, dtype=np.uint64)
+    nrows, Lc = c.shape[0], c.shape[1]
+    nodes = np.zeros((2 * Lc - 1, 32), np.uint8)
+    oracle().or_distributed_mt(_p(c), nrows, Lc, _p(nodes))
+    return nodes
+
+
+def pcs_sumcheck_phase1(data, B, d):
+    a = np.ascontiguousarray(data, dtype=np.uint64)
+    b = np.ascontiguousarray(B, dtype=np.uint64)
+    poly = fe_array(2 * d)
+    oracle().or_pcs_sumcheck_phase1(_p(a), _p(b), a.shape[0], d, _p(poly))
+    return poly
+
+
+def pcs_sumcheck_phase2(data, B, d, r):
+    """Returns the folded (M/d) data and B (inputs are not modified)."""
+    a = np.ascontiguousarray(data, dtype=np.uint64).copy()
+    b = np.ascontiguousarray(B, dtype=np.uint64).copy()
+    M = a.shape[0]
+    oracle().or_pcs_sumcheck_phase2(_p(a), _p(b), M, d, fe(r))
+    return a[:M // d].copy(), b[:M // d].copy()
+
+
+def rs_multifold5(cw, a):
+    c = np.ascontiguousarray(cw, dtype=np.uint64)
+    aa = np.ascontiguousarray(a, dtype=np.uint64)
+    fd = aa.shape[0]
+    out = fe_array(c.shape[0] // fd)
+    oracle().or_rs_multifold5(_p(c), c.shape[0], _p(aa), fd, _p(out))
+    return out
+
+
+def lagrange_coeff(r, d):
+    L = fe_array(d)
+    oracle().or_compute_lagrange_coeff(fe(r), d, _p(L))
+    return L

@@ -15,12 +15,17 @@
 // is consecutive, so each store is a C-element contiguous run.  Zero padding
 // (Reed-Solomon encoding of n coefficients into rate*n points) is handled by
 // the first pass, which reads only the n real inputs.
-#include <type_traits>
 #include "ctx.h"
+#include "dft.h"
 
 using gf::fe;
 
 namespace {
+
+using dft::brev_c;
+using dft::dif;
+using dft::static_for;
+using dft::tw28;
 
 struct NttPass {
     uint64_t in_len;    // elements of `in` that are non-zero inputs
@@ -30,43 +35,6 @@ struct NttPass {
     int rbits[8];       // radix bits per pass, first pass first
     int pre[8];         // pre[t] = rbits[0] + .. + rbits[t-1]
 };
-
-template <int N>
-constexpr int brev_c(int x) {
-    int r = 0;
-    for (int b = 1, q = N >> 1; b < N; b <<= 1, q >>= 1)
-        if (x & b) r |= q;
-    return r;
-}
-
-__device__ __forceinline__ fe tw28(const fe *lo, const fe *hi, uint32_t e) {
-    return gf::mul(lo[e & (NTT_TW_SIZE - 1)], hi[e >> NTT_TW_BITS]);
-}
-
-template <int B, int E, class F>
-__device__ __forceinline__ void static_for(F &&f) {
-    if constexpr (B < E) {
-        f(std::integral_constant<int, B>{});
-        static_for<B + 1, E>(f);
-    }
-}
-
-// In-register DFT of N points, decimation in frequency: natural order in,
-// bit-reversed order out (v[i] = X[brev(i)]).  All twiddles are compile-time
-// constants; powers of w_8 and +-I cost no general multiply.
-template <int N, bool INV>
-__device__ __forceinline__ void dif(fe *v) {
-    if constexpr (N > 1) {
-        static_for<0, N / 2>([&](auto kk) {
-            constexpr int k = decltype(kk)::value;
-            const fe a = v[k], b = v[k + N / 2];
-            v[k] = gf::add(a, b);
-            v[k + N / 2] = gf::mul_root<N, INV ? -k : k>(gf::sub(a, b));
-        });
-        dif<N / 2, INV>(v);
-        dif<N / 2, INV>(v + N / 2);
-    }
-}
 
 constexpr int Y_PAD = 1;   // one padding element per LDS row keeps 16-byte lanes off shared banks
 
