@@ -156,6 +156,46 @@ int cosnark_whir_open(cosnark_ctx *ctx, const uint64_t *d_arr, uint64_t n, const
                       const uint8_t *d_nodes, uint64_t *h_record, uint64_t record_cap,
                       uint64_t *words);
 
+/* ---- coPCS rank-local kernels --------------------------------------------
+ * Encode, src/coPCS.cpp:131-147: each of the nrows rows of L elements
+ * (row-major in d_rows) -> iNTT_L, times w^-j with w = getRootOfUnity(1 +
+ * log_rate_offset + log2 L), zero-padded to rate*L, NTT.  log_rate_offset is
+ * log2 of the reference's global rate_offset (src/coPCS.cpp:22,46).
+ * d_codewords: nrows x rate*L elements, row-major.  L >= 2 and rate are
+ * powers of two; rate*L <= 2^28. */
+int cosnark_copcs_encode(cosnark_ctx *ctx, const uint64_t *d_rows, int nrows, uint64_t L, int rate,
+                         int log_rate_offset, uint64_t *d_codewords);
+/* distributed_MT at one rank, src/coPCS.cpp:152-183: the codeword matrix has
+ * 32 rows (BATCH_FACTOR, src/New_PCS.cpp:9) of Lc elements; d_cw holds the
+ * first nrows (row-major), the rest are zero.  Column j is committed with
+ * MT_commit_Blake (32 elements, 8 leaves) and its root is leaf j of a
+ * create_tree_blake tree.  d_nodes receives Com.Base_MT: Lc + Lc/2 + ... + 1
+ * digests, level 0 first (2 Lc - 1).  With the reference's dup-left parents
+ * only rows 0..3 reach a column root.  1 <= nrows <= 32; Lc a power of two
+ * <= 2^28.  The all-to-all and rank-0 tree of roots (:186-230) span ranks. */
+int cosnark_distributed_mt(cosnark_ctx *ctx, const uint64_t *d_cw, int nrows, uint64_t Lc, uint8_t *d_nodes);
+/* Block sumcheck, phase 1: pcs_sumcheck_phase1 before pcs_sumcheck_aggr
+ * (src/coPCS.cpp:547-569), equal to the block sum of
+ * quadratic_coSumcheck_phase1 (src/coSumcheck.cpp:150-180) before its
+ * r * H_r mask and sumcheck_aggr.  h_poly (2d elements) = sum over blocks j
+ * of NTT_2d(pad(iNTT_d(p1 block j))) * NTT_2d(pad(iNTT_d(p2 block j))).
+ * M = |p1| = |p2|; d and M powers of two, 2 <= d <= min(M, 2048). */
+int cosnark_cosumcheck_phase1(cosnark_ctx *ctx, const uint64_t *d_p1, const uint64_t *d_p2, uint64_t M, int d,
+                              uint64_t *h_poly);
+/* Block sumcheck, phase 2: pcs_sumcheck_phase2 (src/coPCS.cpp:571-587),
+ * equal to quadratic_coSumcheck_phase2 (src/coSumcheck.cpp:201-213) with
+ * L = compute_lagrange_coeff(., r, d) (src/utils.cpp:169-173):
+ * p[j] = sum_k L_k p[jd + k] for j < M/d, in place on both vectors (the
+ * reference then shrinks them to M/d).  2 <= d <= min(M, 8192). */
+int cosnark_cosumcheck_phase2(cosnark_ctx *ctx, uint64_t *d_p1, uint64_t *d_p2, uint64_t M, int d,
+                              const uint64_t r[2]);
+/* RS_multifold5, src/coPCS_utils.cpp:386-437: folds the n-element codeword
+ * d_cw by the fd challenges h_a (fd elements) into n/fd elements at d_out
+ * (the reference replaces the codeword; here d_out must not alias d_cw).
+ * fd and n powers of two, 2 <= fd <= min(n, 4096), n <= 2^28. */
+int cosnark_rs_multifold5(cosnark_ctx *ctx, const uint64_t *d_cw, uint64_t n, const uint64_t *h_a, int fd,
+                          uint64_t *d_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -69,6 +69,11 @@ def lib():
         L.cosnark_whir_commit.argtypes = [vp, vp, u64, i32, vp, vp]
         L.cosnark_whir_claim.argtypes = [vp, vp, u64, vp, vp]
         L.cosnark_whir_open.argtypes = [vp, vp, u64, vp, vp, i32, i32, vp, vp, vp, u64, ctypes.POINTER(u64)]
+        L.cosnark_copcs_encode.argtypes = [vp, vp, i32, u64, i32, i32, vp]
+        L.cosnark_distributed_mt.argtypes = [vp, vp, i32, u64, vp]
+        L.cosnark_cosumcheck_phase1.argtypes = [vp, vp, vp, u64, i32, vp]
+        L.cosnark_cosumcheck_phase2.argtypes = [vp, vp, vp, u64, i32, vp]
+        L.cosnark_rs_multifold5.argtypes = [vp, vp, u64, vp, i32, vp]
         _lib = L
     return _lib
 
@@ -277,3 +282,68 @@ class Context:
         rc = lib().cosnark_whir_open(self.h, arr_dev.ptr, n, _p(zz), _p(KK), rate, stop_log, cw.ptr, nodes.ptr,
                                      _p(rec), cap, ctypes.byref(words))
         return rc, rec[: words.value].copy()
+
+    # ---- coPCS rank-local kernels ----------------------------------------
+
+    def copcs_encode(self, rows, rate, log_rate_offset=0):
+        """Encode (src/coPCS.cpp:131-147) of rows (nrows, L, 2) -> (nrows, rate*L, 2)."""
+        r = np.ascontiguousarray(rows, dtype=np.uint64)
+        nrows, L = r.shape[0], r.shape[1]
+        d = self.upload(r)
+        out = self.alloc(16 * nrows * L * rate)
+        self._ck(lib().cosnark_copcs_encode(self.h, d.ptr, nrows, L, rate, log_rate_offset, out.ptr))
+        res = self.download(out, (nrows, L * rate, 2))
+        d.free(); out.free()
+        return res
+
+    def distributed_mt(self, cw):
+        """distributed_MT (src/coPCS.cpp:152-183) of cw (nrows, Lc, 2): Base_MT digests (2Lc-1, 32)."""
+        c = np.ascontiguousarray(cw, dtype=np.uint64)
+        nrows, Lc = c.shape[0], c.shape[1]
+        d = self.upload(c)
+        nodes = self.alloc(32 * (2 * Lc - 1))
+        self._ck(lib().cosnark_distributed_mt(self.h, d.ptr, nrows, Lc, nodes.ptr))
+        res = self.download(nodes, (2 * Lc - 1, 32), np.uint8)
+        d.free(); nodes.free()
+        return res
+
+    def cosumcheck_phase1(self, p1, p2, d, dev=None):
+        """Block sumcheck phase 1: the 2d-element round polynomial (2d, 2)."""
+        if dev is None:
+            a, b = self.upload(p1), self.upload(p2)
+            M = np.asarray(p1).shape[0]
+        else:
+            a, b, M = dev
+        out = np.zeros((2 * d, 2), np.uint64)
+        self._ck(lib().cosnark_cosumcheck_phase1(self.h, a.ptr, b.ptr, M, d, _p(out)))
+        if dev is None:
+            a.free(); b.free()
+        return out
+
+    def cosumcheck_phase2(self, p1, p2, d, r, dev=None):
+        """Block sumcheck phase 2: the folded (M/d, 2) vectors."""
+        rr = np.ascontiguousarray(np.asarray(r, dtype=np.uint64).reshape(2))
+        if dev is None:
+            a, b = self.upload(p1), self.upload(p2)
+            M = np.asarray(p1).shape[0]
+        else:
+            a, b, M = dev
+        self._ck(lib().cosnark_cosumcheck_phase2(self.h, a.ptr, b.ptr, M, d, _p(rr)))
+        if dev is not None:
+            return None
+        r1 = self.download(a, (M // d, 2))
+        r2 = self.download(b, (M // d, 2))
+        a.free(); b.free()
+        return r1, r2
+
+    def rs_multifold5(self, cw, a):
+        """RS_multifold5 (src/coPCS_utils.cpp:386-437): (n/fd, 2)."""
+        c = np.ascontiguousarray(cw, dtype=np.uint64)
+        aa = np.ascontiguousarray(a, dtype=np.uint64)
+        n, fd = c.shape[0], aa.shape[0]
+        d = self.upload(c)
+        out = self.alloc(16 * (n // fd))
+        self._ck(lib().cosnark_rs_multifold5(self.h, d.ptr, n, _p(aa), fd, out.ptr))
+        res = self.download(out, (n // fd, 2))
+        d.free(); out.free()
+        return res

@@ -128,11 +128,13 @@ int cosnark_create(int device, cosnark_ctx **out) {
         return fail(COSNARK_E_HIP);
     if (hipMalloc(&ctx->sc, sizeof(ScState)) != hipSuccess ||
         hipMalloc(&ctx->sc_partials, 2 * 3 * SC_MAX_BLOCKS * sizeof(fe)) != hipSuccess ||
-        hipHostMalloc(&ctx->sc_host, sizeof(ScState), hipHostMallocDefault) != hipSuccess)
+        hipHostMalloc(&ctx->sc_host, sizeof(ScState), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc(&ctx->co_host, CO_HOST_ELEMS * sizeof(fe), hipHostMallocDefault) != hipSuccess)
         return fail(COSNARK_E_NOMEM);
     if (hipMemset(ctx->sc, 0, sizeof(ScState)) != hipSuccess) return fail(COSNARK_E_HIP);
     if (sumcheck_init(ctx) != COSNARK_OK) return fail(COSNARK_E_HIP);
     if (ntt_init(ctx) != COSNARK_OK) return fail(COSNARK_E_HIP);
+    if (copcs_init(ctx) != COSNARK_OK) return fail(COSNARK_E_HIP);
     if (hipDeviceSynchronize() != hipSuccess) return fail(COSNARK_E_HIP);
     *out = ctx;
     return COSNARK_OK;
@@ -147,6 +149,7 @@ void cosnark_destroy(cosnark_ctx *ctx) {
     (void)hipFree(ctx->rtw); (void)hipFree(ctx->irtw);
     (void)hipFree(ctx->sc); (void)hipFree(ctx->sc_partials);
     if (ctx->sc_host) (void)hipHostFree(ctx->sc_host);
+    if (ctx->co_host) (void)hipHostFree(ctx->co_host);
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     if (ctx->ntt_tmp) (void)hipFree(ctx->ntt_tmp);
     if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);

@@ -1,0 +1,585 @@
+// copcs.hip -- rank-local kernels of the coPCS commitment and opening:
+// the coset encode of the committed rows, the block sumcheck's two phases
+// and RS_multifold5.  (The column tree of distributed_MT is the column mode
+// of the Merkle kernel.)
+//
+// Encode (src/coPCS.cpp:131-147): per row, iNTT_L, times w^-j with
+// w = root(1 + log2 rate_offset + log2 L), zero-pad to rate*L, NTT.  The
+// twist is one elementwise kernel between two library transforms.
+//
+// Block sumcheck.  pcs_sumcheck_phase1 (src/coPCS.cpp:547-569) and the
+// unmasked block sum of quadratic_coSumcheck_phase1 (src/coSumcheck.cpp:
+// 150-180) compute, over blocks j of d elements of p1 and p2,
+//     H[k] = sum_j P_j(w_2d^k) Q_j(w_2d^k),   k < 2d,
+// with P_j, Q_j the interpolants of the blocks on the d-th roots (iNTT_d,
+// pad, NTT_2d).  At even k = 2i an interpolant takes its block's own value
+// i, so H[2i] = sum_j p1[jd+i] p2[jd+i] needs no transform.  The odd points
+// w_2d^(2i+1) come from one d-point transform of the twisted coefficients:
+//     Y = NTT_d(t . iNTT_d(x)),  t_s = w_2d^s,  Y_i = d P(w_2d^(2i+1))
+// (both transforms unscaled; the 1/d^2 is applied once to the sums).  The
+// reference skips blocks whose p1 part is all zero (src/coSumcheck.cpp:
+// 161-167); their products are zero, so the sums agree without that test.
+//   d <= 32: one lane per (block, vector), the transforms in registers with
+//   compile-time twiddles; the two lanes of a block meet in LDS.
+//   64 <= d <= 2048: one workgroup per block, radix-2 stages in LDS.
+// Both write per-workgroup column sums; a small kernel adds them up.
+//
+// Phase 2 (src/coPCS.cpp:571-587, src/coSumcheck.cpp:201-213):
+// p[j] = sum_k L_k p[jd+k] with L = compute_lagrange_coeff(r, d), the
+// inverse transform of (1, r, ..., r^(d-1)) (src/utils.cpp:169-173).
+//
+// RS_multifold5 (src/coPCS_utils.cpp:386-437).  Its partial codewords after
+// log2(fd) butterfly levels are, for output i < n/fd and x_s = cw[i + s n/fd],
+//     part[i][k] = w_n^(-ik) X_k,   X = inverse DFT_fd(x) (unscaled),
+// and out[i] = (1/fd) sum_k a_k part[i][k]: one fd-point transform per
+// output and a Horner pass in w_n^-i.
+#include <cstring>
+#include <vector>
+#include "ctx.h"
+#include "dft.h"
+
+using gf::fe;
+
+namespace {
+
+constexpr int BSC_TB = 64;              // blocks per tile, small d
+constexpr int BSC_SMALL_MAX = 32;
+constexpr int BSC_D_MAX = 2048;
+constexpr int BSC_MAX_WG = 1024;
+constexpr int CO_THREADS = 256;
+constexpr int MF_SMALL_MAX = 32;
+constexpr int MF_FD_MAX = 4096;
+
+// ---- Encode: x[j] *= w^-j, w = root(k), e = j << (28 - k) ----------------
+__global__ __launch_bounds__(CO_THREADS) void coset_twist_kernel(fe *x, uint64_t L, int shift,
+                                                                 const fe *__restrict__ itlo,
+                                                                 const fe *__restrict__ ithi) {
+    const uint64_t j = (uint64_t)blockIdx.x * CO_THREADS + threadIdx.x;
+    if (j < L) x[j] = gf::mul(x[j], dft::tw28(itlo, ithi, (uint32_t)(j << shift)));
+}
+
+// ---- block sumcheck, phase 1 ----------------------------------------------
+
+// x (natural order) -> Y_i = d P(w_2d^(2i+1)), natural order, in place
+template <int D>
+__device__ __forceinline__ void odd_points(fe *x) {
+    dft::dif<D, true>(x);                                   // x[i] = X[brev(i)]
+    fe y[D];
+    dft::static_for<0, D>([&](auto ii) {
+        constexpr int i = decltype(ii)::value;
+        constexpr int s = dft::brev_c<D>(i);
+        y[s] = gf::mul_root<2 * D, s>(x[i]);                // X[s] w_2d^s
+    });
+    dft::dif<D, false>(y);                                  // y[i] = Y[brev(i)]
+    dft::static_for<0, D>([&](auto ii) {
+        constexpr int i = decltype(ii)::value;
+        x[dft::brev_c<D>(i)] = y[i];
+    });
+}
+
+// 128 lanes: lanes 0..63 take p1 of the tile's 64 blocks, lanes 64..127 the
+// same blocks of p2.  LDS row b (2D + 1 elements, padded) collects block b's
+// products: columns 0..D-1 even points, D..2D-1 odd points.
+template <int D>
+__global__ __launch_bounds__(2 * BSC_TB) void bsc_small_kernel(const fe *__restrict__ p1,
+                                                               const fe *__restrict__ p2,
+                                                               uint64_t nblocks, fe *__restrict__ part) {
+    extern __shared__ fe S[];
+    constexpr int ROW = 2 * D + 1, COLS = 2 * D, G = 2 * BSC_TB / COLS;
+    const int t = threadIdx.x, lane = t & (BSC_TB - 1), vec = t / BSC_TB;
+    const int col = t % COLS, grp = t / COLS;
+    const fe *src = vec ? p2 : p1;
+    fe *row = S + lane * ROW;
+    fe acc = gf::zero();
+    const uint64_t ntiles = (nblocks + BSC_TB - 1) / BSC_TB;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const uint64_t blk = tile * BSC_TB + lane;
+        fe x[D];
+        if (blk < nblocks) {
+            const fe *b = src + blk * D;
+#pragma unroll
+            for (int k = 0; k < D; ++k) x[k] = b[k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < D; ++k) x[k] = gf::zero();
+        }
+        if (vec) {
+#pragma unroll
+            for (int k = 0; k < D; ++k) row[k] = x[k];
+        }
+        __syncthreads();
+        if (!vec) {
+#pragma unroll
+            for (int k = 0; k < D; ++k) row[k] = gf::mul(x[k], row[k]);
+        }
+        odd_points<D>(x);
+        if (vec) {
+#pragma unroll
+            for (int k = 0; k < D; ++k) row[D + k] = x[k];
+        }
+        __syncthreads();
+        if (!vec) {
+#pragma unroll
+            for (int k = 0; k < D; ++k) row[D + k] = gf::mul(x[k], row[D + k]);
+        }
+        __syncthreads();
+        for (int r = grp; r < BSC_TB; r += G) acc = gf::add(acc, S[r * ROW + col]);
+        __syncthreads();
+    }
+    S[grp * COLS + col] = acc;
+    __syncthreads();
+    if (t < COLS) {
+        fe s = gf::zero();
+        for (int g = 0; g < G; ++g) s = gf::add(s, S[g * COLS + t]);
+        part[(uint64_t)blockIdx.x * COLS + t] = s;
+    }
+}
+
+__device__ __forceinline__ uint32_t brev_bits(uint32_t x, int bits) {
+    return bits ? __brev(x) >> (32 - bits) : 0;
+}
+
+// Radix-2 decimation-in-frequency stages over NA arrays of 2^logd elements
+// in LDS (natural order in, bit-reversed out), unscaled; the twiddle tables
+// passed in (root of order 2^28 or its inverse) set the direction.
+template <int NA>
+__device__ void lds_dif(fe *const *A, int logd, const fe *twlo, const fe *twhi) {
+    const int d = 1 << logd;
+    for (int lh = logd - 1; lh >= 0; --lh) {
+        const int h = 1 << lh;
+        for (int p = threadIdx.x; p < d / 2; p += CO_THREADS) {
+            const int off = p & (h - 1), i = ((p >> lh) << (lh + 1)) + off;
+            const fe w = dft::tw28(twlo, twhi, (uint32_t)off << (NTT_LOG_MAX - lh - 1));
+#pragma unroll
+            for (int a = 0; a < NA; ++a) {
+                const fe u = A[a][i], v = A[a][i + h];
+                A[a][i] = gf::add(u, v);
+                A[a][i + h] = gf::mul(gf::sub(u, v), w);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Radix-2 decimation-in-time stages: bit-reversed order in, natural out.
+template <int NA>
+__device__ void lds_dit(fe *const *A, int logd, const fe *twlo, const fe *twhi) {
+    const int d = 1 << logd;
+    for (int lh = 0; lh < logd; ++lh) {
+        const int h = 1 << lh;
+        for (int p = threadIdx.x; p < d / 2; p += CO_THREADS) {
+            const int off = p & (h - 1), i = ((p >> lh) << (lh + 1)) + off;
+            const fe w = dft::tw28(twlo, twhi, (uint32_t)off << (NTT_LOG_MAX - lh - 1));
+#pragma unroll
+            for (int a = 0; a < NA; ++a) {
+                const fe u = A[a][i], v = gf::mul(A[a][i + h], w);
+                A[a][i] = gf::add(u, v);
+                A[a][i + h] = gf::sub(u, v);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// one workgroup per block (grid-stride); thread t owns columns t + 256 q
+__global__ __launch_bounds__(CO_THREADS) void bsc_large_kernel(const fe *__restrict__ p1,
+                                                               const fe *__restrict__ p2, uint64_t nblocks,
+                                                               int logd, const fe *twlo, const fe *twhi,
+                                                               const fe *itlo, const fe *ithi,
+                                                               fe *__restrict__ part) {
+    extern __shared__ fe S[];
+    constexpr int QMAX = 2 * BSC_D_MAX / CO_THREADS;
+    const int d = 1 << logd, t = threadIdx.x, cols = 2 * d;
+    fe *A = S, *B = S + d;
+    fe *const AB[2] = {A, B};
+    fe acc[QMAX];
+#pragma unroll
+    for (int q = 0; q < QMAX; ++q) acc[q] = gf::zero();
+    for (uint64_t j = blockIdx.x; j < nblocks; j += gridDim.x) {
+        for (int k = t; k < d; k += CO_THREADS) {
+            A[k] = p1[j * d + k];
+            B[k] = p2[j * d + k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < QMAX; ++q) {
+            const int c = t + CO_THREADS * q;
+            if (c < d) acc[q] = gf::add(acc[q], gf::mul(A[c], B[c]));
+        }
+        __syncthreads();
+        lds_dif<2>(AB, logd, itlo, ithi);             // A[i] = X[brev(i)]
+        for (int i = t; i < d; i += CO_THREADS) {
+            const uint32_t s = brev_bits((uint32_t)i, logd);
+            const fe w = dft::tw28(twlo, twhi, s << (NTT_LOG_MAX - logd - 1));
+            A[i] = gf::mul(A[i], w);
+            B[i] = gf::mul(B[i], w);
+        }
+        __syncthreads();
+        lds_dit<2>(AB, logd, twlo, twhi);                   // A[k] = Y[k]
+#pragma unroll
+        for (int q = 0; q < QMAX; ++q) {
+            const int c = t + CO_THREADS * q;
+            if (c >= d && c < cols) acc[q] = gf::add(acc[q], gf::mul(A[c - d], B[c - d]));
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < QMAX; ++q) {
+        const int c = t + CO_THREADS * q;
+        if (c < cols) part[(uint64_t)blockIdx.x * cols + c] = acc[q];
+    }
+}
+
+// column sums over the workgroups; 64 columns per workgroup, 4 lanes each
+__global__ __launch_bounds__(CO_THREADS) void bsc_final_kernel(const fe *__restrict__ part, int nwg, int d,
+                                                               fe inv_d2, fe *__restrict__ out) {
+    __shared__ fe S[4][64];
+    const int cols = 2 * d, cl = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int c = blockIdx.x * 64 + cl;
+    fe s = gf::zero();
+    if (c < cols)
+        for (int w = g; w < nwg; w += 4) s = gf::add(s, part[(uint64_t)w * cols + c]);
+    S[g][cl] = s;
+    __syncthreads();
+    if (g == 0 && c < cols) {
+        s = gf::add(gf::add(S[0][cl], S[1][cl]), gf::add(S[2][cl], S[3][cl]));
+        if (c < d) out[2 * c] = s;
+        else out[2 * (c - d) + 1] = gf::mul(s, inv_d2);
+    }
+}
+
+// ---- block sumcheck, phase 2 ----------------------------------------------
+
+// one lane per output, both vectors (d <= 64)
+__global__ __launch_bounds__(CO_THREADS) void bsc_fold_small_kernel(const fe *__restrict__ p1,
+                                                                    const fe *__restrict__ p2, uint64_t nout,
+                                                                    int d, const fe *__restrict__ L,
+                                                                    fe *__restrict__ o1, fe *__restrict__ o2) {
+    __shared__ fe Ls[64];
+    if (threadIdx.x < d) Ls[threadIdx.x] = L[threadIdx.x];
+    __syncthreads();
+    const uint64_t j = (uint64_t)blockIdx.x * CO_THREADS + threadIdx.x;
+    if (j >= nout) return;
+    const fe *a = p1 + j * d, *b = p2 + j * d;
+    fe s1 = gf::zero(), s2 = gf::zero();
+    for (int k = 0; k < d; ++k) {
+        s1 = gf::add(s1, gf::mul(Ls[k], a[k]));
+        s2 = gf::add(s2, gf::mul(Ls[k], b[k]));
+    }
+    o1[j] = s1;
+    o2[j] = s2;
+}
+
+__device__ __forceinline__ fe shfl_xor_fe(const fe &x, int m) {
+    return gf::mk(__shfl_xor(x.re, m), __shfl_xor(x.im, m));
+}
+
+// one wave per output (d >= 128): lane l takes k = l + 64 q
+__global__ __launch_bounds__(CO_THREADS) void bsc_fold_wave_kernel(const fe *__restrict__ p1,
+                                                                   const fe *__restrict__ p2, uint64_t nout,
+                                                                   int d, const fe *__restrict__ L,
+                                                                   fe *__restrict__ o1, fe *__restrict__ o2) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t w0 = ((uint64_t)blockIdx.x * CO_THREADS + threadIdx.x) >> 6;
+    const uint64_t nw = ((uint64_t)gridDim.x * CO_THREADS) >> 6;
+    for (uint64_t j = w0; j < nout; j += nw) {
+        const fe *a = p1 + j * d, *b = p2 + j * d;
+        fe s1 = gf::zero(), s2 = gf::zero();
+        for (int k = lane; k < d; k += 64) {
+            const fe l = L[k];
+            s1 = gf::add(s1, gf::mul(l, a[k]));
+            s2 = gf::add(s2, gf::mul(l, b[k]));
+        }
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) {
+            s1 = gf::add(s1, shfl_xor_fe(s1, m));
+            s2 = gf::add(s2, shfl_xor_fe(s2, m));
+        }
+        if (lane == 0) {
+            o1[j] = s1;
+            o2[j] = s2;
+        }
+    }
+}
+
+// ---- RS_multifold5 ----------------------------------------------------------
+
+struct FoldWeights {
+    fe a[MF_SMALL_MAX];     // a_k / fd
+};
+
+template <int FD>
+__global__ __launch_bounds__(CO_THREADS) void multifold_small_kernel(const fe *__restrict__ cw, uint64_t nout,
+                                                                     int shift, const fe *__restrict__ itlo,
+                                                                     const fe *__restrict__ ithi,
+                                                                     FoldWeights wts, fe *__restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * CO_THREADS + threadIdx.x;
+    if (i >= nout) return;
+    fe x[FD];
+#pragma unroll
+    for (int s = 0; s < FD; ++s) x[s] = cw[i + (uint64_t)s * nout];
+    dft::dif<FD, true>(x);                                  // x[j] = X[brev(j)]
+    const fe w = dft::tw28(itlo, ithi, (uint32_t)(i << shift));   // w_n^-i
+    fe acc = gf::mul(wts.a[FD - 1], x[dft::brev_c<FD>(FD - 1)]);
+    dft::static_for<1, FD>([&](auto kk) {
+        constexpr int k = FD - 1 - decltype(kk)::value;
+        acc = gf::add(gf::mul(acc, w), gf::mul(wts.a[k], x[dft::brev_c<FD>(k)]));
+    });
+    out[i] = acc;
+}
+
+// one workgroup per output (grid-stride), the fd-point transform in LDS
+__global__ __launch_bounds__(CO_THREADS) void multifold_large_kernel(const fe *__restrict__ cw, uint64_t nout,
+                                                                     int logfd, int logn, const fe *itlo,
+                                                                     const fe *ithi, const fe *__restrict__ a,
+                                                                     fe *__restrict__ out) {
+    extern __shared__ fe S[];
+    const int fd = 1 << logfd, t = threadIdx.x;
+    fe *X = S, *R = S + fd;
+    fe *const XA[1] = {X};
+    for (uint64_t i = blockIdx.x; i < nout; i += gridDim.x) {
+        for (int s = t; s < fd; s += CO_THREADS) X[s] = cw[i + (uint64_t)s * nout];
+        __syncthreads();
+        lds_dif<1>(XA, logfd, itlo, ithi);            // X[j] = X_brev(j)
+        fe acc = gf::zero();
+        for (int j = t; j < fd; j += CO_THREADS) {
+            const uint32_t k = brev_bits((uint32_t)j, logfd);
+            const uint64_t e = (i * k) << (NTT_LOG_MAX - logn);     // i k < n
+            const fe w = dft::tw28(itlo, ithi, (uint32_t)e);
+            acc = gf::add(acc, gf::mul(gf::mul(a[k], w), X[j]));
+        }
+        R[t] = acc;
+        __syncthreads();
+        for (int h = CO_THREADS / 2; h >= 1; h >>= 1) {
+            if (t < h) R[t] = gf::add(R[t], R[t + h]);
+            __syncthreads();
+        }
+        if (t == 0) out[i] = R[0];
+        __syncthreads();
+    }
+}
+
+// ---- host helpers -------------------------------------------------------------
+
+// the reference's fft (src/utils.cpp:904-964) on the host, for d-element
+// weight vectors: natural order in and out, inverse scaled by 1/n
+void host_fft(std::vector<fe> &a, bool inverse) {
+    const int n = (int)a.size(), logn = ilog2((uint64_t)n);
+    for (int i = 1, j = 0; i < n; ++i) {
+        int bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) std::swap(a[i], a[j]);
+    }
+    for (int lg = 1; lg <= logn; ++lg) {
+        const int len = 1 << lg;
+        fe wl = gf::root_of_unity(lg);
+        if (inverse) wl = gf::inv(wl);
+        for (int s = 0; s < n; s += len) {
+            fe w = gf::one();
+            for (int k = 0; k < len / 2; ++k) {
+                const fe u = a[s + k], v = gf::mul(a[s + k + len / 2], w);
+                a[s + k] = gf::add(u, v);
+                a[s + k + len / 2] = gf::sub(u, v);
+                w = gf::mul(w, wl);
+            }
+        }
+    }
+    if (inverse) {
+        const fe ni = gf::inv(gf::from_u64((uint64_t)n));
+        for (auto &x : a) x = gf::mul(x, ni);
+    }
+}
+
+// compute_lagrange_coeff(., r, d), src/utils.cpp:169-173
+std::vector<fe> lagrange_weights(const fe &r, int d) {
+    std::vector<fe> L(d);
+    L[0] = gf::one();
+    for (int i = 1; i < d; ++i) L[i] = gf::mul(L[i - 1], r);
+    host_fft(L, true);
+    return L;
+}
+
+template <int D>
+void launch_bsc_small(const fe *p1, const fe *p2, uint64_t nblocks, fe *part, int nwg, hipStream_t s) {
+    const size_t lds = sizeof(fe) * BSC_TB * (2 * D + 1);
+    hipLaunchKernelGGL(bsc_small_kernel<D>, dim3(nwg), dim3(2 * BSC_TB), lds, s, p1, p2, nblocks, part);
+}
+
+template <int FD>
+void launch_mf_small(const fe *cw, uint64_t nout, int shift, const fe *itlo, const fe *ithi,
+                     const FoldWeights &w, fe *out, hipStream_t s) {
+    const unsigned g = (unsigned)((nout + CO_THREADS - 1) / CO_THREADS);
+    hipLaunchKernelGGL(multifold_small_kernel<FD>, dim3(g), dim3(CO_THREADS), 0, s, cw, nout, shift, itlo,
+                       ithi, w, out);
+}
+
+}  // namespace
+
+int copcs_init(cosnark_ctx *) {
+    const int small_lds = (int)(sizeof(fe) * BSC_TB * (2 * BSC_SMALL_MAX + 1));
+    const int large_lds = (int)(sizeof(fe) * 2 * BSC_D_MAX);
+    const int mf_lds = (int)(sizeof(fe) * (MF_FD_MAX + CO_THREADS));
+    if (hipFuncSetAttribute((const void *)bsc_small_kernel<32>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            small_lds) != hipSuccess ||
+        hipFuncSetAttribute((const void *)bsc_large_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            large_lds) != hipSuccess ||
+        hipFuncSetAttribute((const void *)multifold_large_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            mf_lds) != hipSuccess)
+        return COSNARK_E_HIP;
+    return COSNARK_OK;
+}
+
+extern "C" {
+
+int cosnark_copcs_encode(cosnark_ctx *ctx, const uint64_t *d_rows, int nrows, uint64_t L, int rate,
+                         int log_rate_offset, uint64_t *d_codewords) {
+    if (!ctx || !d_rows || !d_codewords) return set_err(ctx, COSNARK_E_ARG, "copcs_encode: null argument");
+    if (nrows < 1 || !is_pow2(L) || L < 2 || rate < 1 || !is_pow2((uint64_t)rate) || log_rate_offset < 0)
+        return set_err(ctx, COSNARK_E_ARG, "copcs_encode: need nrows >= 1, L >= 2 and rate powers of two");
+    const int logL = ilog2(L), logM = ilog2(L * (uint64_t)rate), k = 1 + log_rate_offset + logL;
+    if (logM > NTT_LOG_MAX || k > NTT_LOG_MAX)
+        return set_err(ctx, COSNARK_E_RANGE, "copcs_encode: transform or coset root beyond 2^28");
+    void *base;
+    int rc = scratch_reserve(ctx, sizeof(fe) * L, &base);
+    if (rc) return rc;
+    fe *tmp = (fe *)base;
+    const uint64_t M = L * (uint64_t)rate;
+    for (int i = 0; i < nrows; ++i) {
+        rc = ntt_launch(ctx, (const fe *)d_rows + (uint64_t)i * L, L, tmp, logL, true, ctx->stream);
+        if (rc) return rc;
+        hipLaunchKernelGGL(coset_twist_kernel, dim3((unsigned)((L + CO_THREADS - 1) / CO_THREADS)),
+                           dim3(CO_THREADS), 0, ctx->stream, tmp, L, NTT_LOG_MAX - k,
+                           (const fe *)ctx->itw_lo, (const fe *)ctx->itw_hi);
+        LAUNCHCK();
+        rc = ntt_launch(ctx, tmp, L, (fe *)d_codewords + (uint64_t)i * M, logM, false, ctx->stream);
+        if (rc) return rc;
+    }
+    return cosnark_sync(ctx);
+}
+
+int cosnark_distributed_mt(cosnark_ctx *ctx, const uint64_t *d_cw, int nrows, uint64_t Lc, uint8_t *d_nodes) {
+    if (!ctx || !d_cw || !d_nodes) return set_err(ctx, COSNARK_E_ARG, "distributed_mt: null argument");
+    int rc = merkle_columns_launch(ctx, (const fe *)d_cw, nrows, Lc, d_nodes, ctx->stream);
+    if (rc) return rc;
+    return cosnark_sync(ctx);
+}
+
+int cosnark_cosumcheck_phase1(cosnark_ctx *ctx, const uint64_t *d_p1, const uint64_t *d_p2, uint64_t M, int d,
+                              uint64_t *h_poly) {
+    if (!ctx || !d_p1 || !d_p2 || !h_poly) return set_err(ctx, COSNARK_E_ARG, "cosumcheck_phase1: null argument");
+    if (d < 2 || !is_pow2((uint64_t)d) || !is_pow2(M) || M < (uint64_t)d)
+        return set_err(ctx, COSNARK_E_ARG, "cosumcheck_phase1: d and M must be powers of two, 2 <= d <= M");
+    if (d > BSC_D_MAX) return set_err(ctx, COSNARK_E_RANGE, "cosumcheck_phase1: d > 2048");
+    const uint64_t nblocks = M / d;
+    const int cols = 2 * d;
+    int nwg;
+    if (d <= BSC_SMALL_MAX) {
+        const uint64_t tiles = (nblocks + BSC_TB - 1) / BSC_TB;
+        nwg = (int)(tiles < BSC_MAX_WG ? tiles : BSC_MAX_WG);
+    } else {
+        nwg = (int)(nblocks < BSC_MAX_WG ? nblocks : BSC_MAX_WG);
+    }
+    void *base;
+    int rc = scratch_reserve(ctx, sizeof(fe) * ((size_t)nwg * cols + cols), &base);
+    if (rc) return rc;
+    fe *part = (fe *)base, *out = part + (size_t)nwg * cols;
+    const fe *a = (const fe *)d_p1, *b = (const fe *)d_p2;
+    hipStream_t s = ctx->stream;
+    switch (d) {
+        case 2: launch_bsc_small<2>(a, b, nblocks, part, nwg, s); break;
+        case 4: launch_bsc_small<4>(a, b, nblocks, part, nwg, s); break;
+        case 8: launch_bsc_small<8>(a, b, nblocks, part, nwg, s); break;
+        case 16: launch_bsc_small<16>(a, b, nblocks, part, nwg, s); break;
+        case 32: launch_bsc_small<32>(a, b, nblocks, part, nwg, s); break;
+        default:
+            hipLaunchKernelGGL(bsc_large_kernel, dim3(nwg), dim3(CO_THREADS), sizeof(fe) * 2 * d, s, a, b,
+                               nblocks, ilog2((uint64_t)d), (const fe *)ctx->tw_lo, (const fe *)ctx->tw_hi,
+                               (const fe *)ctx->itw_lo, (const fe *)ctx->itw_hi, part);
+    }
+    LAUNCHCK();
+    const fe di = gf::inv(gf::from_u64((uint64_t)d));
+    hipLaunchKernelGGL(bsc_final_kernel, dim3((cols + 63) / 64), dim3(CO_THREADS), 0, s, (const fe *)part, nwg, d,
+                       gf::mul(di, di), out);
+    LAUNCHCK();
+    HIPCK(hipMemcpyAsync(ctx->co_host, out, sizeof(fe) * cols, hipMemcpyDeviceToHost, s));
+    rc = cosnark_sync(ctx);
+    if (rc) return rc;
+    memcpy(h_poly, ctx->co_host, sizeof(fe) * cols);
+    return COSNARK_OK;
+}
+
+int cosnark_cosumcheck_phase2(cosnark_ctx *ctx, uint64_t *d_p1, uint64_t *d_p2, uint64_t M, int d,
+                              const uint64_t r[2]) {
+    if (!ctx || !d_p1 || !d_p2 || !r) return set_err(ctx, COSNARK_E_ARG, "cosumcheck_phase2: null argument");
+    if (d < 2 || !is_pow2((uint64_t)d) || !is_pow2(M) || M < (uint64_t)d)
+        return set_err(ctx, COSNARK_E_ARG, "cosumcheck_phase2: d and M must be powers of two, 2 <= d <= M");
+    if (d > CO_HOST_ELEMS) return set_err(ctx, COSNARK_E_RANGE, "cosumcheck_phase2: d too large");
+    const uint64_t nout = M / d;
+    void *base;
+    int rc = scratch_reserve(ctx, sizeof(fe) * (2 * nout + d), &base);
+    if (rc) return rc;
+    fe *o1 = (fe *)base, *o2 = o1 + nout, *Ld = o2 + nout;
+    const std::vector<fe> L = lagrange_weights(gf::mk(r[0], r[1]), d);
+    memcpy(ctx->co_host, L.data(), sizeof(fe) * d);
+    hipStream_t s = ctx->stream;
+    HIPCK(hipMemcpyAsync(Ld, ctx->co_host, sizeof(fe) * d, hipMemcpyHostToDevice, s));
+    if (d <= 64) {
+        hipLaunchKernelGGL(bsc_fold_small_kernel, dim3((unsigned)((nout + CO_THREADS - 1) / CO_THREADS)),
+                           dim3(CO_THREADS), 0, s, (const fe *)d_p1, (const fe *)d_p2, nout, d, (const fe *)Ld,
+                           o1, o2);
+    } else {
+        const uint64_t waves = nout, wg = (waves + 3) / 4;
+        hipLaunchKernelGGL(bsc_fold_wave_kernel, dim3((unsigned)(wg < 4096 ? wg : 4096)), dim3(CO_THREADS), 0, s,
+                           (const fe *)d_p1, (const fe *)d_p2, nout, d, (const fe *)Ld, o1, o2);
+    }
+    LAUNCHCK();
+    HIPCK(hipMemcpyAsync(d_p1, o1, sizeof(fe) * nout, hipMemcpyDeviceToDevice, s));
+    HIPCK(hipMemcpyAsync(d_p2, o2, sizeof(fe) * nout, hipMemcpyDeviceToDevice, s));
+    return cosnark_sync(ctx);
+}
+
+int cosnark_rs_multifold5(cosnark_ctx *ctx, const uint64_t *d_cw, uint64_t n, const uint64_t *h_a, int fd,
+                          uint64_t *d_out) {
+    if (!ctx || !d_cw || !h_a || !d_out) return set_err(ctx, COSNARK_E_ARG, "rs_multifold5: null argument");
+    if (fd < 2 || !is_pow2((uint64_t)fd) || !is_pow2(n) || n < (uint64_t)fd)
+        return set_err(ctx, COSNARK_E_ARG, "rs_multifold5: fd and n must be powers of two, 2 <= fd <= n");
+    const int logn = ilog2(n), logfd = ilog2((uint64_t)fd);
+    if (logn > NTT_LOG_MAX || fd > MF_FD_MAX)
+        return set_err(ctx, COSNARK_E_RANGE, "rs_multifold5: n > 2^28 or fd > 4096");
+    if ((const void *)d_cw == (const void *)d_out)
+        return set_err(ctx, COSNARK_E_ARG, "rs_multifold5: output must not alias the codeword");
+    const uint64_t nout = n / fd;
+    const fe fi = gf::inv(gf::from_u64((uint64_t)fd));
+    hipStream_t s = ctx->stream;
+    const fe *itlo = (const fe *)ctx->itw_lo, *ithi = (const fe *)ctx->itw_hi;
+    const int shift = NTT_LOG_MAX - logn;
+    if (fd <= MF_SMALL_MAX) {
+        FoldWeights w{};
+        for (int k = 0; k < fd; ++k) w.a[k] = gf::mul(gf::mk(h_a[2 * k], h_a[2 * k + 1]), fi);
+        const fe *cw = (const fe *)d_cw;
+        fe *out = (fe *)d_out;
+        switch (fd) {
+            case 2: launch_mf_small<2>(cw, nout, shift, itlo, ithi, w, out, s); break;
+            case 4: launch_mf_small<4>(cw, nout, shift, itlo, ithi, w, out, s); break;
+            case 8: launch_mf_small<8>(cw, nout, shift, itlo, ithi, w, out, s); break;
+            case 16: launch_mf_small<16>(cw, nout, shift, itlo, ithi, w, out, s); break;
+            default: launch_mf_small<32>(cw, nout, shift, itlo, ithi, w, out, s); break;
+        }
+        LAUNCHCK();
+    } else {
+        void *base;
+        int rc = scratch_reserve(ctx, sizeof(fe) * fd, &base);
+        if (rc) return rc;
+        fe *ad = (fe *)base;
+        for (int k = 0; k < fd; ++k) ctx->co_host[k] = gf::mul(gf::mk(h_a[2 * k], h_a[2 * k + 1]), fi);
+        HIPCK(hipMemcpyAsync(ad, ctx->co_host, sizeof(fe) * fd, hipMemcpyHostToDevice, s));
+        const unsigned g = (unsigned)(nout < 4096 ? nout : 4096);
+        hipLaunchKernelGGL(multifold_large_kernel, dim3(g), dim3(CO_THREADS), sizeof(fe) * (fd + CO_THREADS), s,
+                           (const fe *)d_cw, nout, logfd, logn, itlo, ithi, (const fe *)ad, (fe *)d_out);
+        LAUNCHCK();
+    }
+    return cosnark_sync(ctx);
+}
+
+}  // extern "C"

@@ -45,6 +45,7 @@ struct cosnark_ctx {
     size_t ntt_tmp_bytes = 0;
     ScState *sc = nullptr;            // device
     ScState *sc_host = nullptr;       // pinned host mirror for copy-in / copy-out
+    gf::fe *co_host = nullptr;        // pinned staging, CO_HOST_ELEMS elements (small results, weights)
     gf::fe *sc_partials = nullptr;    // device, 2 x 3 * SC_MAX_BLOCKS elements (ping-pong)
     hipEvent_t ev[8] = {};
     int ntt_tile = NTT_LDS_ELEMS;     // elements per NTT workgroup tile (power of two, <= 4096)
@@ -52,6 +53,7 @@ struct cosnark_ctx {
     bool overlap = true;              // plain prover: sumcheck beside the commit
 };
 
+constexpr int CO_HOST_ELEMS = 8192;     // pinned staging for block-sumcheck results and weights
 constexpr int SC_MAX_BLOCKS = 256;     // workgroups per multi-workgroup sumcheck round
 
 int set_err(cosnark_ctx *ctx, int code, const std::string &msg);
@@ -87,7 +89,12 @@ int ntt_launch(cosnark_ctx *ctx, const gf::fe *in, uint64_t in_len, gf::fe *out,
 // Merkle commit of n_elems elements into nodes (all levels).
 int merkle_launch(cosnark_ctx *ctx, const gf::fe *elems, uint64_t n_elems, uint8_t *nodes,
                   bool dup_left, hipStream_t s);
+// distributed_MT tree: first level = column roots of a row-major nrows x Lc
+// matrix (rows nrows..31 zero), then the dup-left tree (2 Lc - 1 digests).
+int merkle_columns_launch(cosnark_ctx *ctx, const gf::fe *cw, int nrows, uint64_t Lc, uint8_t *nodes,
+                          hipStream_t s);
 int sumcheck_init(cosnark_ctx *ctx);
+int copcs_init(cosnark_ctx *ctx);
 // claim handling of round 0 / every round: check against y, derive y from
 // round 0, or leave unchecked (the caller verifies the claim chain)
 constexpr int SC_CLAIM_CHECK = 0, SC_CLAIM_DERIVE = 1, SC_CLAIM_UNCHECKED = 2;

@@ -11,6 +11,13 @@
 // levels above them in registers, then the workgroup climbs the remaining
 // eight levels of the block through LDS, writing every node; the block roots
 // feed the next launch.  A 2^22-leaf tree takes three launches.
+//
+// Column mode serves distributed_MT (src/coPCS.cpp:152-183): first-level
+// node j is the root of MT_commit_Blake over column j of a 32-row codeword
+// matrix (rows >= nrows are zero).  With the left child hashed twice, that
+// 8-leaf tree keeps only leaf 0 (rows 0..3 of the column) and self-hashes it
+// three times, so a lane reads 64 bytes per column and spends four
+// compressions on it.
 #include "ctx.h"
 #include "blake3.h"
 
@@ -42,16 +49,20 @@ __device__ __forceinline__ void hash_pair(const uint32_t l[8], const uint32_t r[
     b3::compress64(m, h);
 }
 
-// FROM_ELEMS: the first level is the leaf level (input = elements).
-// Otherwise the first level is level `lvl0` computed from level lvl0-1.
+// MODE_ELEMS: the first level is the leaf level (input = elements).
+// MODE_NODES: the first level is level `lvl0` computed from level lvl0-1.
+// MODE_COLUMNS: the first level holds the column roots of a row-major matrix
+// of `col_rows` rows of `col_len` elements (dup-left column trees only).
 // A lane owns 2^RL consecutive first-level nodes (RL = min(2, block_log)) and
 // climbs RL levels in registers; the workgroup then climbs the remaining
 // block_log - RL levels through LDS.  All input loads of a lane are issued
 // before its first compression.
-template <bool FROM_ELEMS>
+constexpr int MODE_ELEMS = 0, MODE_NODES = 1, MODE_COLUMNS = 2;
+
+template <int MODE>
 __global__ __launch_bounds__(MK_THREADS) void merkle_block_kernel(
     const fe *__restrict__ elems, const uint8_t *in_nodes, uint8_t *nodes,
-    LevelOffsets lo, int lvl0, int block_log, int dup_left) {
+    LevelOffsets lo, int lvl0, int block_log, int dup_left, uint64_t col_len, int col_rows) {
     __shared__ Digest bufA[MK_BLOCK / 4];
     __shared__ Digest bufB[MK_BLOCK / 8];
     const int tid = threadIdx.x;
@@ -67,11 +78,18 @@ __global__ __launch_bounds__(MK_THREADS) void merkle_block_kernel(
         for (int i = 0; i < 4; ++i) {
             if (i < per) {
                 const uint64_t g = gl + i;
-                if (FROM_ELEMS) {
+                if (MODE == MODE_ELEMS) {
                     const uint4 *src = reinterpret_cast<const uint4 *>(elems + 4 * g);
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         const uint4 q = src[j];
+                        msg[i][4 * j] = q.x; msg[i][4 * j + 1] = q.y; msg[i][4 * j + 2] = q.z; msg[i][4 * j + 3] = q.w;
+                    }
+                } else if (MODE == MODE_COLUMNS) {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        uint4 q = make_uint4(0, 0, 0, 0);
+                        if (j < col_rows) q = *reinterpret_cast<const uint4 *>(elems + (uint64_t)j * col_len + g);
                         msg[i][4 * j] = q.x; msg[i][4 * j + 1] = q.y; msg[i][4 * j + 2] = q.z; msg[i][4 * j + 3] = q.w;
                     }
                 } else {
@@ -91,6 +109,16 @@ __global__ __launch_bounds__(MK_THREADS) void merkle_block_kernel(
         for (int i = 0; i < 4; ++i) {
             if (i < per) {
                 b3::compress64(msg[i], h[i]);
+                if (MODE == MODE_COLUMNS) {
+                    // the column tree's levels 1..3 over its leaf 0
+#pragma unroll
+                    for (int l = 0; l < 3; ++l) {
+                        uint32_t t[8];
+                        hash_pair(h[i], h[i], t);
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) h[i][j] = t[j];
+                    }
+                }
                 store_digest(nodes, lo.off[lvl0] + gl + i, h[i]);
             }
         }
@@ -141,23 +169,28 @@ __global__ __launch_bounds__(MK_THREADS) void merkle_block_kernel(
 
 }  // namespace
 
-int merkle_launch(cosnark_ctx *ctx, const fe *elems, uint64_t n_elems, uint8_t *nodes, bool dup_left,
-                  hipStream_t stream) {
-    if (!is_pow2(n_elems) || n_elems < 8 || n_elems > (1ull << 30))
-        return set_err(ctx, COSNARK_E_ARG, "merkle: n_elems must be a power of two in [8, 2^30]");
-    const uint64_t leaves = n_elems / 4;
-    const int depth = ilog2(leaves);               // root level index
+namespace {
+
+LevelOffsets level_offsets(uint64_t leaves) {
     LevelOffsets lo{};
     uint64_t off = 0;
-    for (int l = 0; l <= depth; ++l) {
+    for (int l = 0; l <= ilog2(leaves); ++l) {
         lo.off[l] = off;
         off += leaves >> l;
     }
-    // leaf level + in-block levels
+    return lo;
+}
+
+// first launch over `leaves` first-level nodes, then one launch per up to
+// ten further levels until the root
+int build_tree(cosnark_ctx *ctx, int mode, const fe *elems, uint64_t leaves, uint8_t *nodes, bool dup_left,
+               uint64_t col_len, int col_rows, hipStream_t stream) {
+    const LevelOffsets lo = level_offsets(leaves);
     int block_log = ilog2(leaves < (uint64_t)MK_BLOCK ? leaves : (uint64_t)MK_BLOCK);
     uint64_t blocks = leaves >> block_log;
-    hipLaunchKernelGGL(merkle_block_kernel<true>, dim3((unsigned)blocks), dim3(MK_THREADS), 0, stream,
-                       elems, (const uint8_t *)nullptr, nodes, lo, 0, block_log, (int)dup_left);
+    auto first = mode == MODE_ELEMS ? merkle_block_kernel<MODE_ELEMS> : merkle_block_kernel<MODE_COLUMNS>;
+    hipLaunchKernelGGL(first, dim3((unsigned)blocks), dim3(MK_THREADS), 0, stream,
+                       elems, (const uint8_t *)nullptr, nodes, lo, 0, block_log, (int)dup_left, col_len, col_rows);
     LAUNCHCK();
     int top = block_log;                              // highest level written
     uint64_t count = blocks;                          // nodes at level `top`
@@ -165,12 +198,30 @@ int merkle_launch(cosnark_ctx *ctx, const fe *elems, uint64_t n_elems, uint8_t *
         const uint64_t next = count / 2;              // nodes at level top+1
         int bl = ilog2(next < (uint64_t)MK_BLOCK ? next : (uint64_t)MK_BLOCK);
         const uint64_t nb = next >> bl;
-        hipLaunchKernelGGL(merkle_block_kernel<false>, dim3((unsigned)nb), dim3(MK_THREADS), 0, stream,
+        hipLaunchKernelGGL(merkle_block_kernel<MODE_NODES>, dim3((unsigned)nb), dim3(MK_THREADS), 0, stream,
                            (const fe *)nullptr, nodes + 32 * lo.off[top], nodes, lo, top + 1, bl,
-                           (int)dup_left);
+                           (int)dup_left, (uint64_t)0, 0);
         LAUNCHCK();
         top += 1 + bl;
         count = nb;
     }
     return COSNARK_OK;
+}
+
+}  // namespace
+
+int merkle_launch(cosnark_ctx *ctx, const fe *elems, uint64_t n_elems, uint8_t *nodes, bool dup_left,
+                  hipStream_t stream) {
+    if (!is_pow2(n_elems) || n_elems < 8 || n_elems > (1ull << 30))
+        return set_err(ctx, COSNARK_E_ARG, "merkle: n_elems must be a power of two in [8, 2^30]");
+    return build_tree(ctx, MODE_ELEMS, elems, n_elems / 4, nodes, dup_left, 0, 0, stream);
+}
+
+int merkle_columns_launch(cosnark_ctx *ctx, const fe *cw, int nrows, uint64_t Lc, uint8_t *nodes,
+                          hipStream_t stream) {
+    if (!is_pow2(Lc) || Lc > (1ull << 28))
+        return set_err(ctx, COSNARK_E_ARG, "distributed_mt: Lc must be a power of two <= 2^28");
+    if (nrows < 1 || nrows > 32)
+        return set_err(ctx, COSNARK_E_ARG, "distributed_mt: nrows must be in [1, 32]");
+    return build_tree(ctx, MODE_COLUMNS, cw, Lc, nodes, true, Lc, nrows, stream);
 }
