@@ -196,6 +196,39 @@ int cosnark_cosumcheck_phase2(cosnark_ctx *ctx, uint64_t *d_p1, uint64_t *d_p2, 
 int cosnark_rs_multifold5(cosnark_ctx *ctx, const uint64_t *d_cw, uint64_t n, const uint64_t *h_a, int fd,
                           uint64_t *d_out);
 
+/* ---- batched plain proofs (no reference counterpart: the reference proves
+ * one polynomial at a time) ---------------------------------------------
+ * A batch holds `count` plain proofs (cosnark_prove_plain) of sizes
+ * sizes[i] (powers of two, n >= 4, 8 <= n*rate <= 2^28) at one blowup.
+ * cosnark_batch_create plans it once: device slots for every input and
+ * intermediate, and the launch tables.  The caller writes proof i's f and g
+ * into the slots cosnark_batch_input returns (sizes[i] elements each).
+ * cosnark_batch_prove runs every proof: the RS encodes of one size share
+ * each NTT launch, all trees share each Merkle launch and all sumchecks share
+ * each round's launch.  The pass is captured into a hipGraph at the first
+ * call and replayed afterwards (cosnark_batch_set_graph(b, 0): plain
+ * launches).  ts[i] is proof i's transcript (in and out); h_proofs receives
+ * the proofs back to back, proof i at cosnark_batch_proof_offset(b, i), each
+ * byte-identical to cosnark_prove_plain's for the same inputs and
+ * transcript.  ms (optional): device time of the pass. */
+typedef struct cosnark_batch cosnark_batch;
+int cosnark_batch_create(cosnark_ctx *ctx, int count, const uint64_t *sizes, int rate, cosnark_batch **out);
+void cosnark_batch_destroy(cosnark_batch *b);
+int cosnark_batch_input(cosnark_batch *b, int i, uint64_t **d_f, uint64_t **d_g);
+uint64_t cosnark_batch_proof_offset(const cosnark_batch *b, int i);
+int cosnark_batch_set_graph(cosnark_batch *b, int on);
+int cosnark_batch_prove(cosnark_batch *b, cosnark_transcript *ts, uint8_t *h_proofs, double *ms);
+
+/* ---- NTT plan autotuner (no reference counterpart) -----------------------
+ * Times every radix plan (largest pass radix 2^5..2^8, LDS tile 1024..4096
+ * elements) for 2^logn-point transforms, logn_lo <= logn <= logn_hi, with
+ * `batch` transforms per launch, and keeps the fastest per size in ctx.
+ * Plans differ only in speed; outputs are identical.  best_ms (optional,
+ * logn_hi - logn_lo + 1 doubles) receives the winning times.  Run it before
+ * creating batches (a captured graph keeps the plan it was captured with). */
+int cosnark_autotune(cosnark_ctx *ctx, int logn_lo, int logn_hi, int batch, double *best_ms);
+int cosnark_ntt_plan(cosnark_ctx *ctx, int logn, int *passbits, int *tile);
+
 #ifdef __cplusplus
 }
 #endif

@@ -74,6 +74,16 @@ def lib():
         L.cosnark_cosumcheck_phase1.argtypes = [vp, vp, vp, u64, i32, vp]
         L.cosnark_cosumcheck_phase2.argtypes = [vp, vp, vp, u64, i32, vp]
         L.cosnark_rs_multifold5.argtypes = [vp, vp, u64, vp, i32, vp]
+        L.cosnark_batch_create.argtypes = [vp, i32, vp, i32, ctypes.POINTER(vp)]
+        L.cosnark_batch_destroy.argtypes = [vp]
+        L.cosnark_batch_destroy.restype = None
+        L.cosnark_batch_input.argtypes = [vp, i32, ctypes.POINTER(vp), ctypes.POINTER(vp)]
+        L.cosnark_batch_proof_offset.argtypes = [vp, i32]
+        L.cosnark_batch_proof_offset.restype = u64
+        L.cosnark_batch_set_graph.argtypes = [vp, i32]
+        L.cosnark_batch_prove.argtypes = [vp, vp, vp, ctypes.POINTER(ctypes.c_double)]
+        L.cosnark_autotune.argtypes = [vp, i32, i32, i32, vp]
+        L.cosnark_ntt_plan.argtypes = [vp, i32, ctypes.POINTER(i32), ctypes.POINTER(i32)]
         _lib = L
     return _lib
 
@@ -347,3 +357,79 @@ class Context:
         res = self.download(out, (n // fd, 2))
         d.free(); out.free()
         return res
+
+    # ---- batches and tuning ---------------------------------------------
+
+    def autotune(self, logn_lo, logn_hi, batch=1):
+        best = np.zeros(logn_hi - logn_lo + 1, np.float64)
+        self._ck(lib().cosnark_autotune(self.h, logn_lo, logn_hi, batch, _p(best)))
+        return best
+
+    def ntt_plan(self, logn):
+        b, t = ctypes.c_int(0), ctypes.c_int(0)
+        self._ck(lib().cosnark_ntt_plan(self.h, logn, ctypes.byref(b), ctypes.byref(t)))
+        return b.value, t.value
+
+    def batch(self, sizes, rate):
+        return Batch(self, sizes, rate)
+
+
+class Batch:
+    """Many plain proofs of uneven size, proven together (cosnark_batch_*)."""
+
+    def __init__(self, ctx, sizes, rate):
+        self.ctx = ctx
+        self.sizes = [int(x) for x in sizes]
+        self.rate = rate
+        arr = np.ascontiguousarray(self.sizes, dtype=np.uint64)
+        h = ctypes.c_void_p()
+        ctx._ck(lib().cosnark_batch_create(ctx.h, len(self.sizes), _p(arr), rate, ctypes.byref(h)))
+        self.h = h
+        self.offsets = [int(lib().cosnark_batch_proof_offset(h, i)) for i in range(len(self.sizes) + 1)]
+
+    def close(self):
+        if self.h:
+            lib().cosnark_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def input_ptrs(self, i):
+        f, g = ctypes.c_void_p(), ctypes.c_void_p()
+        self.ctx._ck(lib().cosnark_batch_input(self.h, i, ctypes.byref(f), ctypes.byref(g)))
+        return f.value, g.value
+
+    def set_inputs(self, i, f, g):
+        pf, pg = self.input_ptrs(i)
+        a = np.ascontiguousarray(f, dtype=np.uint64)
+        b = np.ascontiguousarray(g, dtype=np.uint64)
+        assert a.shape == (self.sizes[i], 2) and b.shape == a.shape
+        self.ctx._ck(lib().cosnark_h2d(self.ctx.h, ctypes.c_void_p(pf), _p(a), a.nbytes))
+        self.ctx._ck(lib().cosnark_h2d(self.ctx.h, ctypes.c_void_p(pg), _p(b), b.nbytes))
+
+    def set_graph(self, on):
+        self.ctx._ck(lib().cosnark_batch_set_graph(self.h, int(bool(on))))
+
+    def prove(self, transcripts=None):
+        """Returns (list of proof bytes, list of transcripts after, device ms)."""
+        n = len(self.sizes)
+        ts = (Transcript * n)()
+        if transcripts is not None:
+            for i, t in enumerate(transcripts):
+                ts[i].prev[0], ts[i].prev[1] = t.prev[0], t.prev[1]
+        out = np.zeros(self.offsets[-1], np.uint8)
+        ms = ctypes.c_double(0)
+        self.ctx._ck(lib().cosnark_batch_prove(self.h, ctypes.cast(ts, ctypes.c_void_p), _p(out), ctypes.byref(ms)))
+        proofs = [out[self.offsets[i]:self.offsets[i + 1]].tobytes() for i in range(n)]
+        return proofs, list(ts), ms.value
+
+
+def cfg4_sizes(count=64, seed=0xC05A0004, lo=12, hi=20):
+    """Batched-proving workload: `count` sizes 2^e, e uniform on lo..hi from
+    the seeded generator (stream 2, real part modulo the range)."""
+    x = synth(seed, 2, count)[:, 0]
+    return [1 << (lo + int(v) % (hi - lo + 1)) for v in x]

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string>
+#include <vector>
 #include "gf.h"
 #include "cosnark.h"
 
@@ -17,7 +18,7 @@ constexpr int NTT_PASS_LOG_MAX = 8;             // radix of one LDS pass <= 256
 
 // Product-sumcheck device state (one per context).  Transcript value and
 // claim are kept per round: tr_prev[k], tr_y[k] = state before round k.
-struct ScState {
+struct alignas(16) ScState {
     gf::fe tr_prev[65];
     gf::fe tr_y[65];
     gf::fe polys[64 * 3];
@@ -50,6 +51,8 @@ struct cosnark_ctx {
     hipEvent_t ev[8] = {};
     int ntt_tile = NTT_LDS_ELEMS;     // elements per NTT workgroup tile (power of two, <= 4096)
     int ntt_passbits = NTT_PASS_LOG_MAX;  // largest radix (bits) of one NTT pass
+    // per-size plan (0 = the defaults above); set by the autotuner
+    struct { int passbits, tile; } ntt_plan[NTT_LOG_MAX + 1] = {};
     bool overlap = true;              // plain prover: sumcheck beside the commit
 };
 
@@ -86,6 +89,15 @@ inline int ilog2(uint64_t x) { int l = 0; while ((1ull << l) < x) ++l; return l;
 // elements (in may equal out when in_len == 2^logn).
 int ntt_launch(cosnark_ctx *ctx, const gf::fe *in, uint64_t in_len, gf::fe *out, int logn,
                bool inverse, hipStream_t s);
+// `batch` transforms of the same size in one launch per pass: item y reads
+// in_tab[y] (device array; or in + y * in_len when in_tab is null) and
+// writes out_tab[y] (or out + y * 2^logn).  Passes before the last use
+// tmp (batch * 2^logn elements) or, when null, the context's buffer.
+int ntt_launch_batch(cosnark_ctx *ctx, const gf::fe *in, const gf::fe *const *in_tab, uint64_t in_len,
+                     gf::fe *out, gf::fe *const *out_tab, int batch, int logn, bool inverse, hipStream_t s,
+                     gf::fe *tmp);
+int ntt_plan_bits(const cosnark_ctx *ctx, int logn);
+int ntt_plan_tile(const cosnark_ctx *ctx, int logn);
 // Merkle commit of n_elems elements into nodes (all levels).
 int merkle_launch(cosnark_ctx *ctx, const gf::fe *elems, uint64_t n_elems, uint8_t *nodes,
                   bool dup_left, hipStream_t s);
@@ -93,8 +105,53 @@ int merkle_launch(cosnark_ctx *ctx, const gf::fe *elems, uint64_t n_elems, uint8
 // matrix (rows nrows..31 zero), then the dup-left tree (2 Lc - 1 digests).
 int merkle_columns_launch(cosnark_ctx *ctx, const gf::fe *cw, int nrows, uint64_t Lc, uint8_t *nodes,
                           hipStream_t s);
+// Launch tables of a batch, staged on the host: reserve() hands out
+// 256-byte aligned offsets; the bytes are copied to one device block and
+// launches address their tables as block + offset.
+struct TableBuf {
+    std::vector<uint8_t> host;
+    size_t reserve(size_t bytes) {
+        const size_t o = (host.size() + 255) & ~(size_t)255;
+        host.resize(o + bytes);
+        return o;
+    }
+    template <class T>
+    T *at(size_t off) { return reinterpret_cast<T *>(host.data() + off); }
+};
+
+// Batches of trees: one launch per stage covers every tree of the batch.
+struct MerkleBatchPlan {
+    struct Launch {
+        int mode;
+        uint32_t blocks;
+        size_t jobs, blk_job;     // table offsets
+    };
+    std::vector<Launch> launches;
+};
+int merkle_batch_plan(cosnark_ctx *ctx, int count, const gf::fe *const *elems, const uint64_t *n_elems,
+                      uint8_t *const *nodes, bool dup_left, TableBuf &tb, MerkleBatchPlan &plan);
+int merkle_batch_run(cosnark_ctx *ctx, const MerkleBatchPlan &plan, const uint8_t *tables, hipStream_t s);
 int sumcheck_init(cosnark_ctx *ctx);
 int copcs_init(cosnark_ctx *ctx);
+// Batches of all-Fiat-Shamir product sumchecks (the plain prover's): proof
+// i folds v1[i], v2[i] (n[i] elements) in buf[i] (1.5 n[i] elements) with
+// partials at parts[i] (6 * SC_MAX_BLOCKS elements) and state st[i], whose
+// transcript starts from prev[i] and whose claim comes from round 0.
+struct ScBatchPlan {
+    struct Launch {
+        bool fold;
+        uint32_t blocks;
+        size_t jobs, blk_job;     // table offsets
+    };
+    std::vector<Launch> rounds;
+    size_t tail = 0;              // table offset: one tail job per proof
+    int count = 0;
+    const gf::fe *prev = nullptr; // device: transcript start of each proof
+};
+int sumcheck_batch_plan(cosnark_ctx *ctx, int count, const gf::fe *const *v1, const gf::fe *const *v2,
+                        const uint64_t *n, gf::fe *const *buf, gf::fe *const *parts, ScState *const *st,
+                        const gf::fe *prev, TableBuf &tb, ScBatchPlan &plan);
+int sumcheck_batch_run(cosnark_ctx *ctx, const ScBatchPlan &plan, const uint8_t *tables, hipStream_t s);
 // claim handling of round 0 / every round: check against y, derive y from
 // round 0, or leave unchecked (the caller verifies the claim chain)
 constexpr int SC_CLAIM_CHECK = 0, SC_CLAIM_DERIVE = 1, SC_CLAIM_UNCHECKED = 2;
@@ -112,3 +169,44 @@ int ntt_init(cosnark_ctx *ctx);
 // scratch buffers: a1,a2,b1,b2 each of n/2 elements.
 int sumcheck_launch(cosnark_ctx *ctx, const gf::fe *v1, const gf::fe *v2, uint64_t n, int fs_rounds,
                     gf::fe *buf, hipStream_t s);
+
+// ---- batches of plain proofs (shared by the batch API and the server) ----
+// Proof i reads f, g (n elements each) and works in `work`
+// (batch_work_bytes(n, rate) bytes: codeword, tree, sumcheck buffers and
+// partials, state).
+struct BatchProof {
+    const gf::fe *f, *g;
+    uint64_t n;
+    void *work;
+};
+struct BatchCore {
+    int count = 0, rate = 0;
+    std::vector<uint64_t> n;
+    std::vector<size_t> out_off;     // proof i's record in the output block
+    size_t out_bytes = 0;
+    struct Group {
+        int logn, B;
+        uint64_t in_len;
+        size_t in_tab, out_tab;      // table offsets: input and codeword pointers
+    };
+    std::vector<Group> groups;
+    MerkleBatchPlan mk;
+    ScBatchPlan sc;
+    size_t pack = 0;
+    TableBuf tb;
+    gf::fe *prev_dev = nullptr;      // device: count transcript starts
+    uint8_t *out_dev = nullptr;      // device: out_bytes of records
+};
+size_t align256(size_t x);
+size_t batch_work_bytes(uint64_t n, int rate);
+size_t batch_record_bytes(uint64_t n);
+int batch_core_build(cosnark_ctx *ctx, int count, const BatchProof *pr, int rate, gf::fe *prev_dev,
+                     uint8_t *out_dev, BatchCore &C);
+uint64_t batch_tmp_elems(const BatchCore &C);
+// The proving pass: prev_host (pinned) -> prev_dev; sumchecks on s2; NTT
+// groups (in chunks of tmp_elems) and trees on s1; join; records -> out_host
+// (pinned).  tables: the device copy of C.tb.
+int batch_core_enqueue(cosnark_ctx *ctx, const BatchCore &C, const uint8_t *tables, const gf::fe *prev_host,
+                       uint8_t *out_host, gf::fe *tmp, uint64_t tmp_elems, hipStream_t s1, hipStream_t s2,
+                       hipEvent_t fork, hipEvent_t join);
+int batch_unpack(const BatchCore &C, const uint8_t *out_host, uint8_t *const *proofs, cosnark_transcript *ts);

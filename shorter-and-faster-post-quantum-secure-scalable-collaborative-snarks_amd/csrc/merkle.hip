@@ -18,6 +18,9 @@
 // 8-leaf tree keeps only leaf 0 (rows 0..3 of the column) and self-hashes it
 // three times, so a lane reads 64 bytes per column and spends four
 // compressions on it.
+#include <cstring>
+#include <utility>
+#include <vector>
 #include "ctx.h"
 #include "blake3.h"
 
@@ -59,17 +62,33 @@ __device__ __forceinline__ void hash_pair(const uint32_t l[8], const uint32_t r[
 // before its first compression.
 constexpr int MODE_ELEMS = 0, MODE_NODES = 1, MODE_COLUMNS = 2;
 
+// One launch's work on one tree.  A batch launch covers several trees: its
+// workgroup b works on job blk_job[b], block b - first_block of it.
+struct MkJob {
+    const fe *elems;            // MODE_ELEMS, MODE_COLUMNS input
+    const uint8_t *in_nodes;    // MODE_NODES: level lvl0 - 1
+    uint8_t *nodes;             // all levels of the tree
+    LevelOffsets lo;
+    uint64_t col_len;
+    int lvl0, block_log, dup_left, col_rows;
+    uint32_t first_block;
+};
+
 template <int MODE>
-__global__ __launch_bounds__(MK_THREADS) void merkle_block_kernel(
-    const fe *__restrict__ elems, const uint8_t *in_nodes, uint8_t *nodes,
-    LevelOffsets lo, int lvl0, int block_log, int dup_left, uint64_t col_len, int col_rows) {
+__device__ __forceinline__ void merkle_block(const MkJob &J, uint64_t blk) {
     __shared__ Digest bufA[MK_BLOCK / 4];
     __shared__ Digest bufB[MK_BLOCK / 8];
+    const fe *__restrict__ elems = J.elems;
+    const uint8_t *in_nodes = J.in_nodes;
+    uint8_t *nodes = J.nodes;
+    const LevelOffsets &lo = J.lo;
+    const int lvl0 = J.lvl0, block_log = J.block_log, dup_left = J.dup_left, col_rows = J.col_rows;
+    const uint64_t col_len = J.col_len;
     const int tid = threadIdx.x;
     const int RL = block_log < 2 ? block_log : 2;
     const int per = 1 << RL;                            // first-level nodes per lane
     const int lanes = (1 << block_log) >> RL;           // active lanes
-    const uint64_t g0 = (uint64_t)blockIdx.x << block_log;
+    const uint64_t g0 = blk << block_log;
 
     if (tid < lanes) {
         const uint64_t gl = g0 + (uint64_t)tid * per;   // first node of this lane
@@ -167,6 +186,18 @@ __global__ __launch_bounds__(MK_THREADS) void merkle_block_kernel(
     }
 }
 
+template <int MODE>
+__global__ __launch_bounds__(MK_THREADS) void merkle_block_kernel(MkJob J) {
+    merkle_block<MODE>(J, blockIdx.x);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(MK_THREADS) void merkle_batch_kernel(const MkJob *__restrict__ jobs,
+                                                                  const uint16_t *__restrict__ blk_job) {
+    const MkJob &J = jobs[blk_job[blockIdx.x]];
+    merkle_block<MODE>(J, blockIdx.x - J.first_block);
+}
+
 }  // namespace
 
 namespace {
@@ -181,29 +212,48 @@ LevelOffsets level_offsets(uint64_t leaves) {
     return lo;
 }
 
-// first launch over `leaves` first-level nodes, then one launch per up to
-// ten further levels until the root
-int build_tree(cosnark_ctx *ctx, int mode, const fe *elems, uint64_t leaves, uint8_t *nodes, bool dup_left,
-               uint64_t col_len, int col_rows, hipStream_t stream) {
-    const LevelOffsets lo = level_offsets(leaves);
-    int block_log = ilog2(leaves < (uint64_t)MK_BLOCK ? leaves : (uint64_t)MK_BLOCK);
-    uint64_t blocks = leaves >> block_log;
-    auto first = mode == MODE_ELEMS ? merkle_block_kernel<MODE_ELEMS> : merkle_block_kernel<MODE_COLUMNS>;
-    hipLaunchKernelGGL(first, dim3((unsigned)blocks), dim3(MK_THREADS), 0, stream,
-                       elems, (const uint8_t *)nullptr, nodes, lo, 0, block_log, (int)dup_left, col_len, col_rows);
-    LAUNCHCK();
-    int top = block_log;                              // highest level written
+// The launches that build one tree: the first over `leaves` first-level
+// nodes, then one per up to ten further levels until the root.  Returns the
+// (job, workgroups) of each launch.
+std::vector<std::pair<MkJob, uint64_t>> tree_stages(int mode, const fe *elems, uint64_t leaves, uint8_t *nodes,
+                                                    bool dup_left, uint64_t col_len, int col_rows) {
+    std::vector<std::pair<MkJob, uint64_t>> st;
+    MkJob J{};
+    J.elems = elems;
+    J.nodes = nodes;
+    J.lo = level_offsets(leaves);
+    J.col_len = col_len;
+    J.col_rows = col_rows;
+    J.dup_left = (int)dup_left;
+    J.lvl0 = 0;
+    J.block_log = ilog2(leaves < (uint64_t)MK_BLOCK ? leaves : (uint64_t)MK_BLOCK);
+    uint64_t blocks = leaves >> J.block_log;
+    st.push_back({J, blocks});
+    int top = J.block_log;                            // highest level written
     uint64_t count = blocks;                          // nodes at level `top`
     while (count > 1) {
         const uint64_t next = count / 2;              // nodes at level top+1
-        int bl = ilog2(next < (uint64_t)MK_BLOCK ? next : (uint64_t)MK_BLOCK);
-        const uint64_t nb = next >> bl;
-        hipLaunchKernelGGL(merkle_block_kernel<MODE_NODES>, dim3((unsigned)nb), dim3(MK_THREADS), 0, stream,
-                           (const fe *)nullptr, nodes + 32 * lo.off[top], nodes, lo, top + 1, bl,
-                           (int)dup_left, (uint64_t)0, 0);
-        LAUNCHCK();
-        top += 1 + bl;
+        MkJob U = J;
+        U.elems = nullptr;
+        U.in_nodes = nodes + 32 * J.lo.off[top];
+        U.lvl0 = top + 1;
+        U.block_log = ilog2(next < (uint64_t)MK_BLOCK ? next : (uint64_t)MK_BLOCK);
+        const uint64_t nb = next >> U.block_log;
+        st.push_back({U, nb});
+        top += 1 + U.block_log;
         count = nb;
+    }
+    return st;
+}
+
+int build_tree(cosnark_ctx *ctx, int mode, const fe *elems, uint64_t leaves, uint8_t *nodes, bool dup_left,
+               uint64_t col_len, int col_rows, hipStream_t stream) {
+    const auto st = tree_stages(mode, elems, leaves, nodes, dup_left, col_len, col_rows);
+    for (size_t i = 0; i < st.size(); ++i) {
+        auto k = i > 0 ? merkle_block_kernel<MODE_NODES>
+                       : mode == MODE_ELEMS ? merkle_block_kernel<MODE_ELEMS> : merkle_block_kernel<MODE_COLUMNS>;
+        hipLaunchKernelGGL(k, dim3((unsigned)st[i].second), dim3(MK_THREADS), 0, stream, st[i].first);
+        LAUNCHCK();
     }
     return COSNARK_OK;
 }
@@ -224,4 +274,49 @@ int merkle_columns_launch(cosnark_ctx *ctx, const fe *cw, int nrows, uint64_t Lc
     if (nrows < 1 || nrows > 32)
         return set_err(ctx, COSNARK_E_ARG, "distributed_mt: nrows must be in [1, 32]");
     return build_tree(ctx, MODE_COLUMNS, cw, Lc, nodes, true, Lc, nrows, stream);
+}
+
+// ---- batches of trees -----------------------------------------------------
+
+int merkle_batch_plan(cosnark_ctx *ctx, int count, const fe *const *elems, const uint64_t *n_elems,
+                      uint8_t *const *nodes, bool dup_left, TableBuf &tb, MerkleBatchPlan &plan) {
+    std::vector<std::vector<std::pair<MkJob, uint64_t>>> per(count);
+    size_t nst = 0;
+    for (int i = 0; i < count; ++i) {
+        if (!is_pow2(n_elems[i]) || n_elems[i] < 8 || n_elems[i] > (1ull << 30))
+            return set_err(ctx, COSNARK_E_ARG, "merkle batch: n_elems must be powers of two in [8, 2^30]");
+        per[i] = tree_stages(MODE_ELEMS, elems[i], n_elems[i] / 4, nodes[i], dup_left, 0, 0);
+        nst = std::max(nst, per[i].size());
+    }
+    plan.launches.clear();
+    for (size_t s = 0; s < nst; ++s) {
+        std::vector<MkJob> jobs;
+        std::vector<uint16_t> map;
+        for (int i = 0; i < count; ++i) {
+            if (s >= per[i].size()) continue;
+            MkJob J = per[i][s].first;
+            J.first_block = (uint32_t)map.size();
+            for (uint64_t b = 0; b < per[i][s].second; ++b) map.push_back((uint16_t)jobs.size());
+            jobs.push_back(J);
+        }
+        if (jobs.size() > 0xFFFF || map.size() > 0x7FFFFFFFu)
+            return set_err(ctx, COSNARK_E_RANGE, "merkle batch: too many trees or blocks in one launch");
+        MerkleBatchPlan::Launch L{s == 0 ? MODE_ELEMS : MODE_NODES, (uint32_t)map.size(),
+                                  tb.reserve(sizeof(MkJob) * jobs.size()), 0};
+        L.blk_job = tb.reserve(sizeof(uint16_t) * map.size());
+        memcpy(tb.at<MkJob>(L.jobs), jobs.data(), sizeof(MkJob) * jobs.size());
+        memcpy(tb.at<uint16_t>(L.blk_job), map.data(), sizeof(uint16_t) * map.size());
+        plan.launches.push_back(L);
+    }
+    return COSNARK_OK;
+}
+
+int merkle_batch_run(cosnark_ctx *ctx, const MerkleBatchPlan &plan, const uint8_t *tables, hipStream_t stream) {
+    for (const auto &L : plan.launches) {
+        auto k = L.mode == MODE_ELEMS ? merkle_batch_kernel<MODE_ELEMS> : merkle_batch_kernel<MODE_NODES>;
+        hipLaunchKernelGGL(k, dim3(L.blocks), dim3(MK_THREADS), 0, stream, (const MkJob *)(tables + L.jobs),
+                           (const uint16_t *)(tables + L.blk_job));
+        LAUNCHCK();
+    }
+    return COSNARK_OK;
 }

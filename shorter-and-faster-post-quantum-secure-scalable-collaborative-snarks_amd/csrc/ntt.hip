@@ -28,6 +28,11 @@ using dft::static_for;
 using dft::tw28;
 
 struct NttPass {
+    // batch item y = blockIdx.y reads in_tab[y] (or in + y * in_stride) and
+    // writes out_tab[y] (or out + y * out_stride)
+    const fe *const *in_tab;
+    fe *const *out_tab;
+    uint64_t in_stride, out_stride;
     uint64_t in_len;    // elements of `in` that are non-zero inputs
     int logN, logR, logS, logC;
     int last;           // 1: stride-1 pass writing natural order
@@ -44,7 +49,7 @@ constexpr int Y_PAD = 1;   // one padding element per LDS row keeps 16-byte lane
 // DFTs, applies the inter-pass twiddle w_{N_t}^(lo k) and stores.
 template <int R1, int R2, bool LAST, bool INV>
 __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
-    const fe *in, fe *out, NttPass p, const fe *__restrict__ twlo, const fe *__restrict__ twhi,
+    const fe *in0, fe *out0, NttPass p, const fe *__restrict__ twlo, const fe *__restrict__ twhi,
     const fe *__restrict__ rtab, fe scale) {
     constexpr int R = R1 * R2;
     constexpr int LOGR = (R >= 256) ? 8 : (R >= 128) ? 7 : (R >= 64) ? 6 : (R >= 32) ? 5 : (R >= 16) ? 4 : (R >= 8) ? 3 : (R >= 4) ? 2 : (R >= 2) ? 1 : 0;
@@ -55,6 +60,9 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
     const int tid = threadIdx.x;
     const int logC = p.logC, C = 1 << logC;
     const int CP = C + Y_PAD;   // padded LDS row length
+    const uint32_t by = blockIdx.y;
+    const fe *in = p.in_tab ? p.in_tab[by] : in0 + by * p.in_stride;
+    fe *out = p.out_tab ? p.out_tab[by] : out0 + by * p.out_stride;
     for (int j = tid; j < R; j += NTT_THREADS) wr[j] = rtab[j << (NTT_RADIX_LOG_MAX - LOGR)];
 
     const uint64_t bid = blockIdx.x;
@@ -178,21 +186,21 @@ PassKernel pass_kernel_for(int r, bool last, bool inverse) {
 }
 
 template <int R1, int R2>
-int launch_pass(cosnark_ctx *ctx, hipStream_t stream, uint64_t blocks, const fe *in, fe *out,
+int launch_pass(cosnark_ctx *ctx, hipStream_t stream, uint64_t blocks, int batch, const fe *in, fe *out,
                 const NttPass &q, bool last, bool inverse, const fe *twlo, const fe *twhi,
                 const fe *rtab, fe scale) {
     PassKernel k = pass_kernel<R1, R2>(last, inverse);
     const size_t lds = sizeof(fe) * ((size_t)R1 * R2 * ((1u << q.logC) + Y_PAD + 1));
-    hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(NTT_THREADS), lds, stream, in, out, q, twlo, twhi, rtab,
-                       scale);
+    hipLaunchKernelGGL(k, dim3((unsigned)blocks, (unsigned)batch), dim3(NTT_THREADS), lds, stream, in, out, q, twlo,
+                       twhi, rtab, scale);
     LAUNCHCK();
     return COSNARK_OK;
 }
 
-int dispatch_pass(cosnark_ctx *ctx, hipStream_t stream, int r, uint64_t blocks, const fe *in, fe *out,
+int dispatch_pass(cosnark_ctx *ctx, hipStream_t stream, int r, uint64_t blocks, int batch, const fe *in, fe *out,
                   const NttPass &q, bool last, bool inverse, const fe *twlo, const fe *twhi, const fe *rtab,
                   fe scale) {
-#define PASS(R1, R2) launch_pass<R1, R2>(ctx, stream, blocks, in, out, q, last, inverse, twlo, twhi, rtab, scale)
+#define PASS(R1, R2) launch_pass<R1, R2>(ctx, stream, blocks, batch, in, out, q, last, inverse, twlo, twhi, rtab, scale)
     switch (r) {
         case 1: return PASS(2, 1);
         case 2: return PASS(2, 2);
@@ -220,12 +228,25 @@ int ntt_init(cosnark_ctx *ctx) {
     return COSNARK_OK;
 }
 
-int ntt_launch(cosnark_ctx *ctx, const fe *in, uint64_t in_len, fe *out, int logn, bool inverse,
-               hipStream_t stream) {
+int ntt_plan_bits(const cosnark_ctx *ctx, int logn) {
+    const int b = ctx->ntt_plan[logn].passbits;
+    return b ? b : ctx->ntt_passbits;
+}
+
+int ntt_plan_tile(const cosnark_ctx *ctx, int logn) {
+    const int t = ctx->ntt_plan[logn].tile;
+    return t ? t : ctx->ntt_tile;
+}
+
+int ntt_launch_batch(cosnark_ctx *ctx, const fe *in, const fe *const *in_tab, uint64_t in_len, fe *out,
+                     fe *const *out_tab, int batch, int logn, bool inverse, hipStream_t stream, fe *tmp_own) {
     if (logn < 1 || logn > NTT_LOG_MAX) return set_err(ctx, COSNARK_E_RANGE, "ntt: logn out of range");
-    // radix plan: fewest passes of at most 8 bits, bits spread evenly, larger first
-    const int pmax = ctx->ntt_passbits;
+    if (batch < 1 || batch > 65535) return set_err(ctx, COSNARK_E_RANGE, "ntt: batch out of range");
+    // radix plan: fewest passes of at most `passbits` bits, spread evenly, larger first
+    const int pmax = ntt_plan_bits(ctx, logn);
+    const int tile = ntt_plan_tile(ctx, logn);
     const int m = (logn + pmax - 1) / pmax;
+    const uint64_t N = 1ull << logn;
     NttPass p{};
     p.in_len = in_len;
     p.logN = logn;
@@ -236,14 +257,14 @@ int ntt_launch(cosnark_ctx *ctx, const fe *in, uint64_t in_len, fe *out, int log
     const fe *twlo = inverse ? ctx->itw_lo : ctx->tw_lo;
     const fe *twhi = inverse ? ctx->itw_hi : ctx->tw_hi;
     const fe *rtab = inverse ? ctx->irtw : ctx->rtw;
-    const fe scale = gf::inv(gf::from_u64(1ull << logn));
+    const fe scale = gf::inv(gf::from_u64(N));
     // The last pass scatters rows into natural order, so it cannot run in
-    // place: passes 0..m-2 work in a context-owned temporary and the last
-    // one writes `out`.  (A single pass is one workgroup: reads finish
-    // before its writes start.)
-    fe *tmp = nullptr;
-    if (m >= 2) {
-        int rc = ntt_tmp_reserve(ctx, sizeof(fe) << logn, (void **)&tmp);
+    // place: passes 0..m-2 work in a context-owned temporary (one N-element
+    // slice per batch item) and the last one writes `out`.  (A single pass
+    // is one workgroup per item: reads finish before its writes start.)
+    fe *tmp = tmp_own;
+    if (m >= 2 && !tmp) {
+        int rc = ntt_tmp_reserve(ctx, (sizeof(fe) << logn) * (size_t)batch, (void **)&tmp);
         if (rc) return rc;
     }
     int done_bits = 0;
@@ -254,7 +275,7 @@ int ntt_launch(cosnark_ctx *ctx, const fe *in, uint64_t in_len, fe *out, int log
         const bool last = (t == m - 1);
         q.last = last;
         q.logS = logn - done_bits - r;
-        int logC = ilog2((uint64_t)ctx->ntt_tile) - r;
+        int logC = ilog2((uint64_t)tile) - r;
         if (logC < 0) logC = 0;
         if (!last) {
             if (logC > q.logS) logC = q.logS;
@@ -263,13 +284,36 @@ int ntt_launch(cosnark_ctx *ctx, const fe *in, uint64_t in_len, fe *out, int log
             else if (logC > p.rbits[0]) logC = p.rbits[0];
         }
         q.logC = logC;
-        if (t > 0) q.in_len = 1ull << logn;
-        const uint64_t blocks = (1ull << logn) >> (r + logC);
-        const fe *src = (t == 0) ? in : tmp;
-        fe *dst = last ? out : tmp;
-        int rc = dispatch_pass(ctx, stream, r, blocks, src, dst, q, last, inverse, twlo, twhi, rtab, scale);
+        if (t > 0) q.in_len = N;
+        const uint64_t blocks = N >> (r + logC);
+        const fe *src;
+        fe *dst;
+        if (t == 0) {
+            src = in;
+            q.in_tab = in_tab;
+            q.in_stride = in_len;
+        } else {
+            src = tmp;
+            q.in_tab = nullptr;
+            q.in_stride = N;
+        }
+        if (last) {
+            dst = out;
+            q.out_tab = out_tab;
+            q.out_stride = N;
+        } else {
+            dst = tmp;
+            q.out_tab = nullptr;
+            q.out_stride = N;
+        }
+        int rc = dispatch_pass(ctx, stream, r, blocks, batch, src, dst, q, last, inverse, twlo, twhi, rtab, scale);
         if (rc) return rc;
         done_bits += r;
     }
     return COSNARK_OK;
+}
+
+int ntt_launch(cosnark_ctx *ctx, const fe *in, uint64_t in_len, fe *out, int logn, bool inverse,
+               hipStream_t stream) {
+    return ntt_launch_batch(ctx, in, nullptr, in_len, out, nullptr, 1, logn, inverse, stream, nullptr);
 }

@@ -18,6 +18,9 @@
 // round (tr_prev[k], tr_y[k] = state before round k) so a kernel never reads
 // a slot it writes.  Once a vector fits in LDS one workgroup runs all
 // remaining rounds.
+#include <algorithm>
+#include <cstring>
+#include <vector>
 #include "ctx.h"
 #include "transcript.h"
 
@@ -104,7 +107,7 @@ __device__ fe finalize_round(ScState *st, int round, int fs_rounds, int claim_mo
 // after the round, visible to all threads on return.
 template <int NT>
 __device__ void finish_round(ScState *st, int round, int fs_rounds, const fe *part_in, int nb_in, fe (*red)[NT / 64],
-                             fe *sh_r, fe *sh_prev, fe *sh_y) {
+                             fe *sh_r, fe *sh_prev, fe *sh_y, bool writer) {
     fe A = gf::zero(), C = gf::zero(), O = gf::zero();
     for (int b = threadIdx.x; b < nb_in; b += NT) {
         A = gf::add(A, part_in[3 * b]);
@@ -114,25 +117,46 @@ __device__ void finish_round(ScState *st, int round, int fs_rounds, const fe *pa
     block_reduce3<NT>(A, C, O, red);
     if (threadIdx.x < 4) {
         fe prev = st->tr_prev[round], y = st->tr_y[round];
-        const fe r = finalize_round(st, round, fs_rounds, st->derive_y, prev, y, A, C, O, blockIdx.x == 0);
+        const fe r = finalize_round(st, round, fs_rounds, st->derive_y, prev, y, A, C, O, writer);
         if (threadIdx.x == 0) { *sh_r = r; *sh_prev = prev; *sh_y = y; }
     }
     __syncthreads();
 }
+
+// One launch's work on one proof.  A batch launch covers several proofs:
+// its workgroup b works on job blk_job[b], block b - first_block of it.
+struct ScJob {
+    const fe *s1, *s2;        // inputs
+    fe *d1, *d2;              // folded outputs (FOLD rounds)
+    uint64_t npairs;          // pairs this round (tail: elements in s1/s2)
+    ScState *st;
+    const fe *part_in;        // previous round's workgroup partials
+    fe *part_out;
+    int nb_in, nb;            // partials in part_in; this job's workgroups
+    int round, m, fs_rounds;
+    uint32_t first_block;
+};
 
 // One multi-workgroup round `round`.  FOLD (round > 0): inputs hold
 // 4*npairs elements, folded by r[round-1] (finalized here from part_in) into
 // outputs of 2*npairs; otherwise inputs hold 2*npairs elements.  Writes this
 // round's per-workgroup partials to part_out.
 template <bool FOLD>
-__global__ __launch_bounds__(SC_THREADS) void sc_round_kernel(
-    const fe *__restrict__ s1, const fe *__restrict__ s2, fe *__restrict__ d1, fe *__restrict__ d2,
-    uint64_t npairs, int round, int fs_rounds, ScState *st, const fe *part_in, int nb_in, fe *part_out) {
+__device__ __forceinline__ void sc_round(const ScJob &J, uint32_t blk) {
     __shared__ fe red[3][SC_THREADS / 64];
     __shared__ fe sh_r, sh_prev, sh_y;
+    const fe *__restrict__ s1 = J.s1;
+    const fe *__restrict__ s2 = J.s2;
+    fe *__restrict__ d1 = J.d1;
+    fe *__restrict__ d2 = J.d2;
+    const uint64_t npairs = J.npairs;
+    const int round = J.round, fs_rounds = J.fs_rounds, nb_in = J.nb_in;
+    ScState *st = J.st;
+    const fe *part_in = J.part_in;
+    fe *part_out = J.part_out;
     constexpr int NV = FOLD ? 4 : 2;
-    const uint64_t stride = (uint64_t)gridDim.x * SC_THREADS * SC_ITEMS;
-    uint64_t j0 = (uint64_t)blockIdx.x * SC_THREADS * SC_ITEMS + threadIdx.x;
+    const uint64_t stride = (uint64_t)J.nb * SC_THREADS * SC_ITEMS;
+    uint64_t j0 = (uint64_t)blk * SC_THREADS * SC_ITEMS + threadIdx.x;
     fe x1[SC_ITEMS][NV], x2[SC_ITEMS][NV];
     auto load = [&](uint64_t base) {
 #pragma unroll
@@ -148,7 +172,7 @@ __global__ __launch_bounds__(SC_THREADS) void sc_round_kernel(
     load(j0);
     fe r = gf::zero();
     if (FOLD) {
-        finish_round<SC_THREADS>(st, round - 1, fs_rounds, part_in, nb_in, red, &sh_r, &sh_prev, &sh_y);
+        finish_round<SC_THREADS>(st, round - 1, fs_rounds, part_in, nb_in, red, &sh_r, &sh_prev, &sh_y, blk == 0);
         r = sh_r;
     }
     fe A = gf::zero(), C = gf::zero(), O = gf::zero();
@@ -178,19 +202,35 @@ __global__ __launch_bounds__(SC_THREADS) void sc_round_kernel(
     }
     block_reduce3<SC_THREADS>(A, C, O, red);
     if (threadIdx.x == 0) {
-        part_out[3 * blockIdx.x] = A;
-        part_out[3 * blockIdx.x + 1] = C;
-        part_out[3 * blockIdx.x + 2] = O;
+        part_out[3 * blk] = A;
+        part_out[3 * blk + 1] = C;
+        part_out[3 * blk + 2] = O;
     }
+}
+
+template <bool FOLD>
+__global__ __launch_bounds__(SC_THREADS) void sc_round_kernel(ScJob J) {
+    sc_round<FOLD>(J, blockIdx.x);
+}
+
+template <bool FOLD>
+__global__ __launch_bounds__(SC_THREADS) void sc_round_batch_kernel(const ScJob *__restrict__ jobs,
+                                                                     const uint16_t *__restrict__ blk_job) {
+    const ScJob &J = jobs[blk_job[blockIdx.x]];
+    sc_round<FOLD>(J, blockIdx.x - J.first_block);
 }
 
 // All rounds from `round0` to m-1 in one workgroup, vectors held in LDS.
 // If round0 > 0 the input (len elements) still needs the fold by
 // r[round0-1], which is finalized here from part_in.  Writes the final
 // folded values and the closing transcript state.
-__global__ __launch_bounds__(SC_TAIL_THREADS) void sc_tail_kernel(
-    const fe *__restrict__ s1, const fe *__restrict__ s2, uint32_t len, int round0, int m, int fs_rounds,
-    ScState *st, const fe *part_in, int nb_in) {
+__device__ __forceinline__ void sc_tail(const ScJob &J) {
+    const fe *__restrict__ s1 = J.s1;
+    const fe *__restrict__ s2 = J.s2;
+    const uint32_t len = (uint32_t)J.npairs;
+    const int round0 = J.round, m = J.m, fs_rounds = J.fs_rounds, nb_in = J.nb_in;
+    ScState *st = J.st;
+    const fe *part_in = J.part_in;
     extern __shared__ fe lds[];
     fe *x1 = lds, *x2 = lds + SC_TAIL_ELEMS;
     __shared__ fe red[3][SC_TAIL_THREADS / 64];
@@ -199,7 +239,7 @@ __global__ __launch_bounds__(SC_TAIL_THREADS) void sc_tail_kernel(
     for (uint32_t i = tid; i < len; i += SC_TAIL_THREADS) { x1[i] = s1[i]; x2[i] = s2[i]; }
     const int claim_mode = st->derive_y;
     if (round0 > 0) {
-        finish_round<SC_TAIL_THREADS>(st, round0 - 1, fs_rounds, part_in, nb_in, red, &sh_r, &sh_prev, &sh_y);
+        finish_round<SC_TAIL_THREADS>(st, round0 - 1, fs_rounds, part_in, nb_in, red, &sh_r, &sh_prev, &sh_y, true);
     } else {
         if (tid == 0) { sh_prev = st->tr_prev[0]; sh_y = st->tr_y[0]; sh_r = gf::zero(); }
         __syncthreads();
@@ -259,6 +299,13 @@ __global__ __launch_bounds__(SC_TAIL_THREADS) void sc_tail_kernel(
     }
 }
 
+__global__ __launch_bounds__(SC_TAIL_THREADS) void sc_tail_kernel(ScJob J) { sc_tail(J); }
+
+// one workgroup per proof
+__global__ __launch_bounds__(SC_TAIL_THREADS) void sc_tail_batch_kernel(const ScJob *__restrict__ jobs) {
+    sc_tail(jobs[blockIdx.x]);
+}
+
 // Finalize round `round` from part_in, then fold both vectors by its
 // challenge: d[j] = fold(s[2j], s[2j+1]) for j < nout (no evaluation).
 __global__ __launch_bounds__(SC_THREADS) void sc_fold_kernel(
@@ -266,7 +313,7 @@ __global__ __launch_bounds__(SC_THREADS) void sc_fold_kernel(
     int round, int fs_rounds, ScState *st, const fe *part_in, int nb_in) {
     __shared__ fe red[3][SC_THREADS / 64];
     __shared__ fe sh_r, sh_prev, sh_y;
-    finish_round<SC_THREADS>(st, round, fs_rounds, part_in, nb_in, red, &sh_r, &sh_prev, &sh_y);
+    finish_round<SC_THREADS>(st, round, fs_rounds, part_in, nb_in, red, &sh_r, &sh_prev, &sh_y, blockIdx.x == 0);
     const fe r = sh_r;
     for (uint64_t j = (uint64_t)blockIdx.x * SC_THREADS + threadIdx.x; j < nout; j += (uint64_t)gridDim.x * SC_THREADS) {
         const fe a0 = s1[2 * j], a1 = s1[2 * j + 1], b0 = s2[2 * j], b1 = s2[2 * j + 1];
@@ -282,11 +329,25 @@ __global__ void sc_init_kernel(ScState *st, fe prev, fe y, int derive_y) {
     st->err = 0;
 }
 
+// proof i's state (tail[i].st): transcript from prev[i], claim derived
+// from round 0
+__global__ void sc_init_batch_kernel(const ScJob *tail, const fe *prev, int count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    ScState *st = tail[i].st;
+    st->tr_prev[0] = prev[i];
+    st->tr_y[0] = gf::zero();
+    st->derive_y = SC_CLAIM_DERIVE;
+    st->err = 0;
+}
+
 }  // namespace
 
 int sumcheck_init(cosnark_ctx *ctx) {
     // the tail kernel keeps two 4096-element vectors (128 KiB) in LDS
     HIPCK(hipFuncSetAttribute((const void *)sc_tail_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)(2 * SC_TAIL_ELEMS * sizeof(fe))));
+    HIPCK(hipFuncSetAttribute((const void *)sc_tail_batch_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)(2 * SC_TAIL_ELEMS * sizeof(fe))));
     return COSNARK_OK;
 }
@@ -303,14 +364,21 @@ int sumcheck_round_launch(cosnark_ctx *ctx, const fe *s1, const fe *s2, fe *d1, 
     uint64_t blocks = (npairs + SC_THREADS * SC_ITEMS - 1) / (SC_THREADS * SC_ITEMS);
     if (blocks > (uint64_t)SC_MAX_BLOCKS) blocks = SC_MAX_BLOCKS;
     if (blocks == 0) blocks = 1;
-    const fe *pin = part[(round + 1) & 1];
-    fe *pout = part[round & 1];
+    ScJob J{};
+    J.s1 = s1; J.s2 = s2;
+    J.d1 = fold ? d1 : nullptr; J.d2 = fold ? d2 : nullptr;
+    J.npairs = npairs;
+    J.st = ctx->sc;
+    J.part_in = part[(round + 1) & 1];
+    J.part_out = part[round & 1];
+    J.nb_in = fold ? nb_prev : 0;
+    J.nb = (int)blocks;
+    J.round = round;
+    J.fs_rounds = fs_rounds;
     if (fold)
-        hipLaunchKernelGGL(sc_round_kernel<true>, dim3((unsigned)blocks), dim3(SC_THREADS), 0, stream, s1, s2, d1, d2,
-                           npairs, round, fs_rounds, ctx->sc, pin, nb_prev, pout);
+        hipLaunchKernelGGL(sc_round_kernel<true>, dim3((unsigned)blocks), dim3(SC_THREADS), 0, stream, J);
     else
-        hipLaunchKernelGGL(sc_round_kernel<false>, dim3((unsigned)blocks), dim3(SC_THREADS), 0, stream, s1, s2,
-                           (fe *)nullptr, (fe *)nullptr, npairs, round, fs_rounds, ctx->sc, pin, 0, pout);
+        hipLaunchKernelGGL(sc_round_kernel<false>, dim3((unsigned)blocks), dim3(SC_THREADS), 0, stream, J);
     LAUNCHCK();
     *nb_out = (int)blocks;
     return COSNARK_OK;
@@ -327,45 +395,135 @@ int sumcheck_fold_launch(cosnark_ctx *ctx, const fe *s1, const fe *s2, fe *d1, f
     return COSNARK_OK;
 }
 
-int sumcheck_launch(cosnark_ctx *ctx, const fe *v1, const fe *v2, uint64_t n, int fs_rounds, fe *buf,
-                    hipStream_t stream) {
+namespace {
+
+// The launches of one proof's sumcheck as sumcheck_launch issues them: the
+// multi-workgroup rounds while a vector exceeds the LDS tail, then the tail.
+// buf: n + n/2 elements (ping n/2 per vector, pong n/4 per vector); part:
+// two slots of 3 * SC_MAX_BLOCKS partials.
+std::vector<ScJob> sumcheck_jobs(const fe *v1, const fe *v2, uint64_t n, int fs_rounds, fe *buf, fe *part0,
+                                 ScState *st) {
     const int m = ilog2(n);
-    // buffers: ping (n/2 per vector) and pong (n/4 per vector)
     fe *a1 = buf, *a2 = buf + n / 2, *b1 = buf + n, *b2 = buf + n + n / 4;
-    fe *part[2] = {ctx->sc_partials, ctx->sc_partials + 3 * SC_MAX_BLOCKS};
+    fe *part[2] = {part0, part0 + 3 * SC_MAX_BLOCKS};
+    fe *outs[2][2] = {{a1, a2}, {b1, b2}};
+    std::vector<ScJob> jobs;
     const fe *s1 = v1, *s2 = v2;
     uint64_t len = n;        // elements in s1/s2
-    int round = 0;
+    int round = 0, which = 0, nb_prev = 0;
     bool pending = false;    // s1/s2 still need the fold with r[round-1]
-    fe *outs[2][2] = {{a1, a2}, {b1, b2}};
-    int which = 0, nb_prev = 0;
     while (len > (uint64_t)SC_TAIL_ELEMS) {
         const uint64_t after = pending ? len / 2 : len;
         const uint64_t npairs = after / 2;
         uint64_t blocks = (npairs + SC_THREADS * SC_ITEMS - 1) / (SC_THREADS * SC_ITEMS);
         if (blocks > (uint64_t)SC_MAX_BLOCKS) blocks = SC_MAX_BLOCKS;
-        const fe *pin = part[(round + 1) & 1];
-        fe *pout = part[round & 1];
+        ScJob J{};
+        J.s1 = s1; J.s2 = s2;
+        J.npairs = npairs;
+        J.st = st;
+        J.part_in = part[(round + 1) & 1];
+        J.part_out = part[round & 1];
+        J.nb_in = pending ? nb_prev : 0;
+        J.nb = (int)blocks;
+        J.round = round;
+        J.m = m;
+        J.fs_rounds = fs_rounds;
         if (pending) {
-            fe *o1 = outs[which][0], *o2 = outs[which][1];
-            hipLaunchKernelGGL(sc_round_kernel<true>, dim3((unsigned)blocks), dim3(SC_THREADS), 0, stream,
-                               s1, s2, o1, o2, npairs, round, fs_rounds, ctx->sc, pin, nb_prev, pout);
-            s1 = o1; s2 = o2;
+            J.d1 = outs[which][0]; J.d2 = outs[which][1];
+            s1 = J.d1; s2 = J.d2;
             which ^= 1;
-        } else {
-            hipLaunchKernelGGL(sc_round_kernel<false>, dim3((unsigned)blocks), dim3(SC_THREADS), 0, stream,
-                               s1, s2, (fe *)nullptr, (fe *)nullptr, npairs, round, fs_rounds, ctx->sc, pin, 0,
-                               pout);
         }
-        LAUNCHCK();
+        jobs.push_back(J);
         nb_prev = (int)blocks;
         len = after;
         pending = true;
         ++round;
     }
+    ScJob T{};
+    T.s1 = s1; T.s2 = s2;
+    T.npairs = len;
+    T.st = st;
+    T.part_in = part[(round + 1) & 1];
+    T.nb_in = nb_prev;
+    T.round = round;
+    T.m = m;
+    T.fs_rounds = fs_rounds;
+    jobs.push_back(T);
+    return jobs;
+}
+
+}  // namespace
+
+int sumcheck_launch(cosnark_ctx *ctx, const fe *v1, const fe *v2, uint64_t n, int fs_rounds, fe *buf,
+                    hipStream_t stream) {
+    const auto jobs = sumcheck_jobs(v1, v2, n, fs_rounds, buf, ctx->sc_partials, ctx->sc);
+    for (size_t i = 0; i + 1 < jobs.size(); ++i) {
+        const ScJob &J = jobs[i];
+        if (J.round > 0)
+            hipLaunchKernelGGL(sc_round_kernel<true>, dim3((unsigned)J.nb), dim3(SC_THREADS), 0, stream, J);
+        else
+            hipLaunchKernelGGL(sc_round_kernel<false>, dim3((unsigned)J.nb), dim3(SC_THREADS), 0, stream, J);
+        LAUNCHCK();
+    }
     const size_t lds = 2 * SC_TAIL_ELEMS * sizeof(fe);
-    hipLaunchKernelGGL(sc_tail_kernel, dim3(1), dim3(SC_TAIL_THREADS), lds, stream, s1, s2, (uint32_t)len, round, m,
-                       fs_rounds, ctx->sc, (const fe *)part[(round + 1) & 1], nb_prev);
+    hipLaunchKernelGGL(sc_tail_kernel, dim3(1), dim3(SC_TAIL_THREADS), lds, stream, jobs.back());
+    LAUNCHCK();
+    return COSNARK_OK;
+}
+
+// ---- batches of proofs ------------------------------------------------------
+
+int sumcheck_batch_plan(cosnark_ctx *ctx, int count, const fe *const *v1, const fe *const *v2, const uint64_t *n,
+                        fe *const *buf, fe *const *parts, ScState *const *st, const fe *prev, TableBuf &tb,
+                        ScBatchPlan &plan) {
+    std::vector<std::vector<ScJob>> per(count);
+    size_t nr = 0;
+    for (int i = 0; i < count; ++i) {
+        const int m = ilog2(n[i]);
+        per[i] = sumcheck_jobs(v1[i], v2[i], n[i], m, buf[i], parts[i], st[i]);
+        nr = std::max(nr, per[i].size() - 1);
+    }
+    plan.rounds.clear();
+    plan.count = count;
+    plan.prev = prev;
+    for (size_t r = 0; r < nr; ++r) {
+        std::vector<ScJob> jobs;
+        std::vector<uint16_t> map;
+        for (int i = 0; i < count; ++i) {
+            if (r + 1 >= per[i].size()) continue;
+            ScJob J = per[i][r];
+            J.first_block = (uint32_t)map.size();
+            for (int b = 0; b < J.nb; ++b) map.push_back((uint16_t)jobs.size());
+            jobs.push_back(J);
+        }
+        if (jobs.size() > 0xFFFF) return set_err(ctx, COSNARK_E_RANGE, "sumcheck batch: too many proofs");
+        ScBatchPlan::Launch L{r > 0, (uint32_t)map.size(), tb.reserve(sizeof(ScJob) * jobs.size()), 0};
+        L.blk_job = tb.reserve(sizeof(uint16_t) * map.size());
+        memcpy(tb.at<ScJob>(L.jobs), jobs.data(), sizeof(ScJob) * jobs.size());
+        memcpy(tb.at<uint16_t>(L.blk_job), map.data(), sizeof(uint16_t) * map.size());
+        plan.rounds.push_back(L);
+    }
+    plan.tail = tb.reserve(sizeof(ScJob) * count);
+    for (int i = 0; i < count; ++i) tb.at<ScJob>(plan.tail)[i] = per[i].back();
+    return COSNARK_OK;
+}
+
+int sumcheck_batch_run(cosnark_ctx *ctx, const ScBatchPlan &plan, const uint8_t *tables, hipStream_t stream) {
+    const ScJob *tail = (const ScJob *)(tables + plan.tail);
+    hipLaunchKernelGGL(sc_init_batch_kernel, dim3((plan.count + 255) / 256), dim3(256), 0, stream, tail, plan.prev,
+                       plan.count);
+    LAUNCHCK();
+    for (const auto &L : plan.rounds) {
+        if (L.fold)
+            hipLaunchKernelGGL(sc_round_batch_kernel<true>, dim3(L.blocks), dim3(SC_THREADS), 0, stream,
+                               (const ScJob *)(tables + L.jobs), (const uint16_t *)(tables + L.blk_job));
+        else
+            hipLaunchKernelGGL(sc_round_batch_kernel<false>, dim3(L.blocks), dim3(SC_THREADS), 0, stream,
+                               (const ScJob *)(tables + L.jobs), (const uint16_t *)(tables + L.blk_job));
+        LAUNCHCK();
+    }
+    const size_t lds = 2 * SC_TAIL_ELEMS * sizeof(fe);
+    hipLaunchKernelGGL(sc_tail_batch_kernel, dim3(plan.count), dim3(SC_TAIL_THREADS), lds, stream, tail);
     LAUNCHCK();
     return COSNARK_OK;
 }

@@ -1,0 +1,94 @@
+// tune.hip -- the NTT radix plan per transform size, picked by timing.
+//
+// A 2^L-point transform runs as ceil(L / b) LDS passes of radix at most 2^b,
+// each workgroup staging a tile of T elements.  Every (b, T) computes the
+// same transform (the result is exact field arithmetic), so the choice only
+// moves time: fewer passes mean fewer HBM sweeps but larger in-register
+// DFTs and fewer columns per tile.  The tuner times each candidate on the
+// device, with `batch` transforms per launch as the batched prover issues
+// them, and keeps the fastest per size in the context.
+#include <vector>
+#include "ctx.h"
+
+using gf::fe;
+
+extern "C" {
+
+int cosnark_autotune(cosnark_ctx *ctx, int logn_lo, int logn_hi, int batch, double *best_ms) {
+    if (!ctx) return COSNARK_E_ARG;
+    if (logn_lo < 1 || logn_hi > NTT_LOG_MAX || logn_lo > logn_hi || batch < 1)
+        return set_err(ctx, COSNARK_E_ARG, "autotune: need 1 <= logn_lo <= logn_hi <= 28, batch >= 1");
+    HIPCK(hipSetDevice(ctx->device));
+    const uint64_t maxn = (uint64_t)batch << logn_hi;
+    fe *in = nullptr, *out = nullptr, *tmp = nullptr;
+    fe **itab = nullptr, **otab = nullptr;
+    auto cleanup = [&]() {
+        for (void *p : {(void *)in, (void *)out, (void *)tmp, (void *)itab, (void *)otab})
+            if (p) (void)hipFree(p);
+    };
+    if (hipMalloc(&in, sizeof(fe) * maxn) != hipSuccess || hipMalloc(&out, sizeof(fe) * maxn) != hipSuccess ||
+        hipMalloc(&tmp, sizeof(fe) * maxn) != hipSuccess || hipMalloc(&itab, sizeof(fe *) * batch) != hipSuccess ||
+        hipMalloc(&otab, sizeof(fe *) * batch) != hipSuccess) {
+        cleanup();
+        return set_err(ctx, COSNARK_E_NOMEM, "autotune: allocation failed");
+    }
+    // inputs: any values do (the plan does not change what is computed)
+    if (hipMemsetAsync(in, 0, sizeof(fe) * maxn, ctx->stream) != hipSuccess) {
+        cleanup();
+        return set_err(ctx, COSNARK_E_HIP, "autotune: memset failed");
+    }
+    hipEvent_t e0, e1;
+    HIPCK(hipEventCreate(&e0));
+    HIPCK(hipEventCreate(&e1));
+    const int bits_c[] = {5, 6, 7, 8};
+    const int tile_c[] = {1024, 2048, 4096};
+    int rc = COSNARK_OK;
+    for (int logn = logn_lo; logn <= logn_hi && rc == COSNARK_OK; ++logn) {
+        const uint64_t N = 1ull << logn;
+        std::vector<const fe *> ih(batch);
+        std::vector<fe *> oh(batch);
+        for (int y = 0; y < batch; ++y) { ih[y] = in + y * N; oh[y] = out + y * N; }
+        if (hipMemcpy(itab, ih.data(), sizeof(fe *) * batch, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(otab, oh.data(), sizeof(fe *) * batch, hipMemcpyHostToDevice) != hipSuccess) {
+            rc = set_err(ctx, COSNARK_E_HIP, "autotune: table copy failed");
+            break;
+        }
+        float best = 1e30f;
+        int bb = 0, bt = 0;
+        for (int bits : bits_c) {
+            for (int tile : tile_c) {
+                if ((1 << bits) > tile) continue;
+                ctx->ntt_plan[logn].passbits = bits;
+                ctx->ntt_plan[logn].tile = tile;
+                float t = 1e30f;
+                for (int rep = 0; rep < 4 && rc == COSNARK_OK; ++rep) {
+                    HIPCK(hipEventRecord(e0, ctx->stream));
+                    rc = ntt_launch_batch(ctx, nullptr, (const fe *const *)itab, N, nullptr, otab, batch, logn, false,
+                                          ctx->stream, tmp);
+                    HIPCK(hipEventRecord(e1, ctx->stream));
+                    HIPCK(hipEventSynchronize(e1));
+                    float ms = 0;
+                    (void)hipEventElapsedTime(&ms, e0, e1);
+                    if (rep > 0 && ms < t) t = ms;      // the first run warms up
+                }
+                if (t < best) { best = t; bb = bits; bt = tile; }
+            }
+        }
+        ctx->ntt_plan[logn].passbits = bb;
+        ctx->ntt_plan[logn].tile = bt;
+        if (best_ms) best_ms[logn - logn_lo] = best;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    cleanup();
+    return rc;
+}
+
+int cosnark_ntt_plan(cosnark_ctx *ctx, int logn, int *passbits, int *tile) {
+    if (!ctx || logn < 1 || logn > NTT_LOG_MAX) return COSNARK_E_ARG;
+    if (passbits) *passbits = ntt_plan_bits(ctx, logn);
+    if (tile) *tile = ntt_plan_tile(ctx, logn);
+    return COSNARK_OK;
+}
+
+}  // extern "C"
