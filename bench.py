@@ -40,6 +40,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--serial", action="store_true", help="run the stages one after another (profiling)")
     ap.add_argument("--no-whir", action="store_true", help="skip the WHIR (config 3) leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched (config 4) leg")
     return ap.parse_args()
 
 
@@ -83,6 +84,56 @@ def whir_leg(ctx, reps=3):
     return {"workload": "WHIR 2^24 evaluations, blowup 4, whir_k 4, down to 2^8 (8 iterations)",
             "commit_ms": round(min(commit_ms), 3), "open_ms": round(min(open_ms), 3),
             "iterations": int(rec[0]), "record_words": int(rec.shape[0])}
+
+
+def batched_leg(reps=5):
+    """Config 4: 64 proofs of 2^e elements, e uniform on 12..20 (seed
+    0xC05A0004), RS blowup 4: one batched pass (graph replay, autotuned NTT
+    plans) against 64 sequential plain proofs on the same context."""
+    seed = 0xC05A0004
+    sizes = cosnark.cfg4_sizes()
+    ctx = cosnark.Context(0)
+    t0 = time.perf_counter()
+    ctx.autotune(14, 22, batch=8)
+    tune_s = time.perf_counter() - t0
+    b = ctx.batch(sizes, 4)
+    dev = []
+    for i, n in enumerate(sizes):
+        f, g = cosnark.synth(seed, 2 * i, n), cosnark.synth(seed, 2 * i + 1, n)
+        b.set_inputs(i, f, g)
+        dev.append((ctx.upload(f), ctx.upload(g), n))
+    # 64 sequential plain proofs (inputs resident), after one warm pass
+    seq = []
+    for rep in range(2):
+        ctx.sync()
+        t0 = time.perf_counter()
+        proofs_seq = [ctx.prove_plain(None, None, 4, cosnark.Transcript(), dev_inputs=d)[0].tobytes() for d in dev]
+        seq.append(time.perf_counter() - t0)
+    seq_s = min(seq)
+    # batched: graph captured on the first pass
+    for _ in range(2):
+        proofs_b, _, _ = b.prove()
+    walls, dev_ms = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        proofs_b, _, ms = b.prove()
+        walls.append(time.perf_counter() - t0)
+        dev_ms.append(ms)
+    bat_s = min(walls)
+    same = proofs_b == proofs_seq
+    b.close()
+    for f, g, _ in dev:
+        f.free(); g.free()
+    ctx.close()
+    return {"workload": "64 proofs, n = 2^e with e uniform on 12..20 (seed 0xC05A0004), RS blowup 4",
+            "elements": int(sum(sizes)),
+            "proofs_per_s": round(64 / bat_s, 1), "ms_per_batch": round(bat_s * 1e3, 3),
+            "device_ms_per_batch": round(min(dev_ms), 3),
+            "sequential_plain_proofs_per_s": round(64 / seq_s, 1),
+            "sequential_plain_ms": round(seq_s * 1e3, 3),
+            "speedup_vs_sequential": round(seq_s / bat_s, 2),
+            "autotune_s": round(tune_s, 2),
+            "proofs_identical_to_sequential": bool(same)}
 
 
 def main():
@@ -155,9 +206,11 @@ def main():
                      "alg_bytes_per_launch": int(per_launch_bytes),
                      "avg_launch_ms": round(per_launch_s * 1e3, 5)},
     }
+    df.free(); dg.free()
     if not a.no_whir:
-        df.free(); dg.free()
         out["whir_cfg3"] = whir_leg(ctx)
+    if not a.no_batch:
+        out["batched_cfg4"] = batched_leg()
     if not a.no_cpu:
         cpu_ms, same = cpu_baseline(f, g, a.rate, proof)
         out["cpu_baseline"] = {"value": round(cpu_ms, 1), "unit": "ms per rank share", "cores": 1,
