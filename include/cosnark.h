@@ -219,6 +219,36 @@ uint64_t cosnark_batch_proof_offset(const cosnark_batch *b, int i);
 int cosnark_batch_set_graph(cosnark_batch *b, int on);
 int cosnark_batch_prove(cosnark_batch *b, cosnark_transcript *ts, uint8_t *h_proofs, double *ms);
 
+/* ---- proof server (no reference counterpart: the reference proves one
+ * request per process run) ---------------------------------------------------
+ * Continuous batching of plain proofs under a fixed HBM budget.  The server
+ * owns a context on `device`, a scheduler thread and an arena of `budget`
+ * bytes of device memory, from which every request's working memory and
+ * every pass's tables come.  Each tick it retires finished passes, admits
+ * queued requests in arrival order while their memory fits, and launches
+ * them as one batched pass on a free slot (one of `slots` stream pairs), so
+ * passes overlap on the device.  Requests are n = 2^k <= 2^max_log_n
+ * elements at blowup `rate`; budget must hold `slots` NTT temporaries of
+ * rate * 2^max_log_n elements plus one largest request.
+ *   submit: d_f, d_g are device arrays on the same device (n elements each,
+ *           read during the proof: keep them until wait returns); prev is
+ *           the transcript start (null: zero).  Thread-safe.
+ *   wait:   blocks until the proof is ready; h_proof receives
+ *           cosnark_plain_proof_bytes(n) bytes, identical to
+ *           cosnark_prove_plain's for the same inputs and transcript;
+ *           latency_ms = completion - submission; releases the ticket.
+ *   stats:  passes, completed requests, most requests in flight at once,
+ *           peak arena bytes, arena bytes, queued requests. */
+typedef struct cosnark_server cosnark_server;
+int cosnark_server_create(int device, uint64_t budget, int rate, int max_log_n, int slots, cosnark_server **out);
+int cosnark_server_submit(cosnark_server *s, const uint64_t *d_f, const uint64_t *d_g, uint64_t n,
+                          const uint64_t prev[2], uint64_t *ticket);
+int cosnark_server_poll(cosnark_server *s, uint64_t ticket, int *done);
+int cosnark_server_wait(cosnark_server *s, uint64_t ticket, uint8_t *h_proof, uint64_t prev_out[2],
+                        double *latency_ms);
+int cosnark_server_stats(cosnark_server *s, uint64_t out[6]);
+void cosnark_server_destroy(cosnark_server *s);
+
 /* ---- NTT plan autotuner (no reference counterpart) -----------------------
  * Times every radix plan (largest pass radix 2^5..2^8, LDS tile 1024..4096
  * elements) for 2^logn-point transforms, logn_lo <= logn <= logn_hi, with

@@ -84,6 +84,13 @@ def lib():
         L.cosnark_batch_prove.argtypes = [vp, vp, vp, ctypes.POINTER(ctypes.c_double)]
         L.cosnark_autotune.argtypes = [vp, i32, i32, i32, vp]
         L.cosnark_ntt_plan.argtypes = [vp, i32, ctypes.POINTER(i32), ctypes.POINTER(i32)]
+        L.cosnark_server_create.argtypes = [i32, u64, i32, i32, i32, ctypes.POINTER(vp)]
+        L.cosnark_server_submit.argtypes = [vp, vp, vp, u64, vp, ctypes.POINTER(u64)]
+        L.cosnark_server_poll.argtypes = [vp, u64, ctypes.POINTER(i32)]
+        L.cosnark_server_wait.argtypes = [vp, u64, vp, vp, ctypes.POINTER(ctypes.c_double)]
+        L.cosnark_server_stats.argtypes = [vp, vp]
+        L.cosnark_server_destroy.argtypes = [vp]
+        L.cosnark_server_destroy.restype = None
         _lib = L
     return _lib
 
@@ -433,3 +440,60 @@ def cfg4_sizes(count=64, seed=0xC05A0004, lo=12, hi=20):
     the seeded generator (stream 2, real part modulo the range)."""
     x = synth(seed, 2, count)[:, 0]
     return [1 << (lo + int(v) % (hi - lo + 1)) for v in x]
+
+
+class Server:
+    """Continuous-batching proof server (cosnark_server_*)."""
+
+    def __init__(self, budget_bytes, rate=4, max_log_n=22, slots=3, device=0):
+        h = ctypes.c_void_p()
+        rc = lib().cosnark_server_create(device, budget_bytes, rate, max_log_n, slots, ctypes.byref(h))
+        if rc != 0:
+            raise CosnarkError(rc, "server_create failed")
+        self.h = h
+        self.rate = rate
+
+    def submit(self, d_f, d_g, n, prev=(0, 0)):
+        """d_f, d_g: device pointers (int) or DevBuf; returns the ticket."""
+        pf = d_f.ptr if hasattr(d_f, "ptr") else ctypes.c_void_p(d_f)
+        pg = d_g.ptr if hasattr(d_g, "ptr") else ctypes.c_void_p(d_g)
+        pv = np.ascontiguousarray([int(prev[0]), int(prev[1])], dtype=np.uint64)
+        t = ctypes.c_uint64(0)
+        rc = lib().cosnark_server_submit(self.h, pf, pg, n, _p(pv), ctypes.byref(t))
+        if rc != 0:
+            raise CosnarkError(rc, "server_submit failed")
+        return t.value
+
+    def poll(self, ticket):
+        d = ctypes.c_int(0)
+        rc = lib().cosnark_server_poll(self.h, ticket, ctypes.byref(d))
+        if rc != 0:
+            raise CosnarkError(rc, "server_poll failed")
+        return bool(d.value)
+
+    def wait(self, ticket, n):
+        """Returns (proof bytes, transcript after, latency ms)."""
+        proof = np.zeros(int(lib().cosnark_plain_proof_bytes(n)), np.uint8)
+        pv = np.zeros(2, np.uint64)
+        lat = ctypes.c_double(0)
+        rc = lib().cosnark_server_wait(self.h, ticket, _p(proof), _p(pv), ctypes.byref(lat))
+        if rc != 0:
+            raise CosnarkError(rc, "server request failed")
+        return proof.tobytes(), (int(pv[0]), int(pv[1])), lat.value
+
+    def stats(self):
+        out = np.zeros(6, np.uint64)
+        lib().cosnark_server_stats(self.h, _p(out))
+        keys = ("passes", "completed", "max_inflight", "arena_peak_bytes", "arena_bytes", "queued")
+        return {k: int(v) for k, v in zip(keys, out)}
+
+    def close(self):
+        if self.h:
+            lib().cosnark_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

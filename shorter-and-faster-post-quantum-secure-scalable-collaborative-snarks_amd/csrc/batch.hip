@@ -173,7 +173,8 @@ int batch_core_enqueue(cosnark_ctx *ctx, const BatchCore &C, const uint8_t *tabl
 
 // proof i's bytes and transcript out of the copied records; the index of
 // the first proof whose sumcheck self-check failed, or -1
-int batch_unpack(const BatchCore &C, const uint8_t *out_host, uint8_t *const *proofs, cosnark_transcript *ts) {
+int batch_unpack(const BatchCore &C, const uint8_t *out_host, uint8_t *const *proofs, cosnark_transcript *ts,
+                 int32_t *errs) {
     int bad = -1;
     for (int i = 0; i < C.count; ++i) {
         const uint64_t pb = cosnark_plain_proof_bytes(C.n[i]);
@@ -182,6 +183,7 @@ int batch_unpack(const BatchCore &C, const uint8_t *out_host, uint8_t *const *pr
         if (ts) memcpy(ts[i].prev, rec + pb, 16);
         int32_t err;
         memcpy(&err, rec + pb + 16, 4);
+        if (errs) errs[i] = err;
         if (err && bad < 0) bad = i;
     }
     return bad;
@@ -358,7 +360,7 @@ int cosnark_batch_prove(cosnark_batch *b, cosnark_transcript *ts, uint8_t *h_pro
         dst[i] = h_proofs + o;
         o += cosnark_plain_proof_bytes(b->core.n[i]);
     }
-    const int bad = batch_unpack(b->core, b->out_host, dst.data(), ts);
+    const int bad = batch_unpack(b->core, b->out_host, dst.data(), ts, nullptr);
     if (bad >= 0)
         return set_err(ctx, COSNARK_E_CHECK, "batch_prove: sumcheck self-check failed in proof " + std::to_string(bad));
     return COSNARK_OK;

@@ -209,4 +209,7 @@ uint64_t batch_tmp_elems(const BatchCore &C);
 int batch_core_enqueue(cosnark_ctx *ctx, const BatchCore &C, const uint8_t *tables, const gf::fe *prev_host,
                        uint8_t *out_host, gf::fe *tmp, uint64_t tmp_elems, hipStream_t s1, hipStream_t s2,
                        hipEvent_t fork, hipEvent_t join);
-int batch_unpack(const BatchCore &C, const uint8_t *out_host, uint8_t *const *proofs, cosnark_transcript *ts);
+// proof i's bytes (to proofs[i] when non-null), transcript and self-check
+// flag out of the copied records; returns the first failed proof or -1
+int batch_unpack(const BatchCore &C, const uint8_t *out_host, uint8_t *const *proofs, cosnark_transcript *ts,
+                 int32_t *errs);
