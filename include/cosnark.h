@@ -258,6 +258,10 @@ void cosnark_server_destroy(cosnark_server *s);
  * creating batches (a captured graph keeps the plan it was captured with). */
 int cosnark_autotune(cosnark_ctx *ctx, int logn_lo, int logn_hi, int batch, double *best_ms);
 int cosnark_ntt_plan(cosnark_ctx *ctx, int logn, int *passbits, int *tile);
+/* Measured throughput of the library's GF(p^2) multiply on this device
+ * (independent multiply chains, no memory traffic): the compute ceiling the
+ * transform kernels are reported against.  No reference counterpart. */
+int cosnark_fmul_rate(cosnark_ctx *ctx, double *per_s);
 
 #ifdef __cplusplus
 }

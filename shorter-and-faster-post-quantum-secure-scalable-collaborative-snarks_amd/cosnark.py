@@ -84,6 +84,7 @@ def lib():
         L.cosnark_batch_prove.argtypes = [vp, vp, vp, ctypes.POINTER(ctypes.c_double)]
         L.cosnark_autotune.argtypes = [vp, i32, i32, i32, vp]
         L.cosnark_ntt_plan.argtypes = [vp, i32, ctypes.POINTER(i32), ctypes.POINTER(i32)]
+        L.cosnark_fmul_rate.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
         L.cosnark_server_create.argtypes = [i32, u64, i32, i32, i32, ctypes.POINTER(vp)]
         L.cosnark_server_submit.argtypes = [vp, vp, vp, u64, vp, ctypes.POINTER(u64)]
         L.cosnark_server_poll.argtypes = [vp, u64, ctypes.POINTER(i32)]
@@ -371,6 +372,11 @@ class Context:
         best = np.zeros(logn_hi - logn_lo + 1, np.float64)
         self._ck(lib().cosnark_autotune(self.h, logn_lo, logn_hi, batch, _p(best)))
         return best
+
+    def fmul_rate(self):
+        r = ctypes.c_double(0)
+        self._ck(lib().cosnark_fmul_rate(self.h, ctypes.byref(r)))
+        return r.value
 
     def ntt_plan(self, logn):
         b, t = ctypes.c_int(0), ctypes.c_int(0)

@@ -414,6 +414,7 @@ int cosnark_prove_plain(cosnark_ctx *ctx, const uint64_t *d_f, const uint64_t *d
     HIPCK(hipEventRecord(ctx->ev[4], s2));
     if ((rc = sumcheck_launch(ctx, (const fe *)d_f, (const fe *)d_g, n, m, scb, s2))) return rc;
     HIPCK(hipEventRecord(ctx->ev[3], s2));
+    HIPCK(hipEventRecord(ctx->ev[6], ctx->stream));
     if ((rc = merkle_launch(ctx, cw, cw_n, nd, true, ctx->stream))) return rc;
     HIPCK(hipEventRecord(ctx->ev[2], ctx->stream));
     HIPCK(hipStreamWaitEvent(ctx->stream, ctx->ev[3], 0));
@@ -427,7 +428,7 @@ int cosnark_prove_plain(cosnark_ctx *ctx, const uint64_t *d_f, const uint64_t *d
     if (times) {
         float a = 0, b = 0, c = 0, tot = 0;
         (void)hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]);
-        (void)hipEventElapsedTime(&b, ctx->ev[1], ctx->ev[2]);
+        (void)hipEventElapsedTime(&b, ctx->ev[6], ctx->ev[2]);
         (void)hipEventElapsedTime(&c, ctx->ev[4], ctx->ev[3]);
         (void)hipEventElapsedTime(&tot, ctx->ev[0], ctx->ev[5]);
         times[0] = a; times[1] = b; times[2] = c; times[3] = tot;

@@ -12,7 +12,59 @@
 
 using gf::fe;
 
+namespace {
+
+constexpr int FR_CHAINS = 8;
+
+// Eight independent chains x <- x * b per lane: a dependency-free stream of
+// GF(p^2) multiplies, the ceiling the transform kernels are measured against.
+__global__ __launch_bounds__(256) void fmul_rate_kernel(fe b, int iters, fe *out) {
+    const uint64_t gid = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    fe a[FR_CHAINS];
+#pragma unroll
+    for (int k = 0; k < FR_CHAINS; ++k) a[k] = gf::mk(gid + 1 + k, (gid * 7 + k) & 0xFFFFFFFFFFFull);
+    for (int i = 0; i < iters; ++i) {
+#pragma unroll
+        for (int k = 0; k < FR_CHAINS; ++k) a[k] = gf::mul(a[k], b);
+    }
+    fe s = a[0];
+#pragma unroll
+    for (int k = 1; k < FR_CHAINS; ++k) s = gf::add(s, a[k]);
+    out[gid] = s;
+}
+
+}  // namespace
+
 extern "C" {
+
+int cosnark_fmul_rate(cosnark_ctx *ctx, double *per_s) {
+    if (!ctx || !per_s) return COSNARK_E_ARG;
+    HIPCK(hipSetDevice(ctx->device));
+    constexpr unsigned blocks = 8192;
+    constexpr int iters = 256;
+    void *out;
+    int rc = scratch_reserve(ctx, sizeof(fe) * 256 * (size_t)blocks, &out);
+    if (rc) return rc;
+    hipEvent_t e0, e1;
+    HIPCK(hipEventCreate(&e0));
+    HIPCK(hipEventCreate(&e1));
+    const fe b = gf::root_of_unity(20);
+    float best = 1e30f;
+    for (int rep = 0; rep < 3; ++rep) {
+        HIPCK(hipEventRecord(e0, ctx->stream));
+        hipLaunchKernelGGL(fmul_rate_kernel, dim3(blocks), dim3(256), 0, ctx->stream, b, iters, (fe *)out);
+        LAUNCHCK();
+        HIPCK(hipEventRecord(e1, ctx->stream));
+        HIPCK(hipEventSynchronize(e1));
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (rep > 0 && ms < best) best = ms;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *per_s = (double)blocks * 256 * iters * FR_CHAINS / (best / 1e3);
+    return COSNARK_OK;
+}
 
 int cosnark_autotune(cosnark_ctx *ctx, int logn_lo, int logn_hi, int batch, double *best_ms) {
     if (!ctx) return COSNARK_E_ARG;
