@@ -56,6 +56,12 @@ int cosnark_h2d(cosnark_ctx *ctx, void *d_dst, const void *h_src, uint64_t bytes
 int cosnark_d2h(cosnark_ctx *ctx, void *h_dst, const void *d_src, uint64_t bytes);
 int cosnark_d2d(cosnark_ctx *ctx, void *d_dst, const void *d_src, uint64_t bytes);
 
+/* ---- host field operations (GF((2^61-1)^2), canonical in and out) -------
+ * operator* and operator+ of virgo::fieldElement, src/fieldElement.cpp:34-78;
+ * for the few host-side combinations around the kernels (sumcheck masks). */
+void cosnark_field_mul(const uint64_t a[2], const uint64_t b[2], uint64_t out[2]);
+void cosnark_field_add(const uint64_t a[2], const uint64_t b[2], uint64_t out[2]);
+
 /* ---- synthetic workload inputs ---------------------------------------
  * n elements uniform in GF(p^2): splitmix64-seeded xoshiro256**, each
  * coordinate masked to 61 bits with the value p rejected.  Host buffer. */
@@ -84,6 +90,10 @@ int cosnark_rs_encode(cosnark_ctx *ctx, const uint64_t *d_coeffs, uint64_t n,
 uint64_t cosnark_merkle_node_count(uint64_t n_elems);
 int cosnark_merkle_commit(cosnark_ctx *ctx, const uint64_t *d_elems, uint64_t n_elems,
                           uint8_t *d_nodes, int dup_left);
+/* create_tree_blake (src/merkle_tree.cpp:267-299) over given level-0
+ * digests: d_nodes holds n_leaves digests (level 0, a power of two); levels
+ * 1.. are written after them, root last (2 n_leaves - 1 digests). */
+int cosnark_merkle_tree_from_leaves(cosnark_ctx *ctx, uint64_t n_leaves, uint8_t *d_nodes, int dup_left);
 /* open_tree_blake, src/merkle_tree.cpp:320-335: sibling digest per level
  * below the root into h_path (32 * depth bytes); *depth = log2(n_elems/4). */
 int cosnark_merkle_open(cosnark_ctx *ctx, const uint8_t *d_nodes, uint64_t n_elems,
@@ -189,6 +199,10 @@ int cosnark_cosumcheck_phase1(cosnark_ctx *ctx, const uint64_t *d_p1, const uint
  * reference then shrinks them to M/d).  2 <= d <= min(M, 8192). */
 int cosnark_cosumcheck_phase2(cosnark_ctx *ctx, uint64_t *d_p1, uint64_t *d_p2, uint64_t M, int d,
                               const uint64_t r[2]);
+/* The same fold with caller-given weights h_L (d elements):
+ * quadratic_coSumcheck_phase2(p1, p2, L, d), src/coSumcheck.cpp:201-213. */
+int cosnark_cosumcheck_fold(cosnark_ctx *ctx, uint64_t *d_p1, uint64_t *d_p2, uint64_t M, int d,
+                            const uint64_t *h_L);
 /* RS_multifold5, src/coPCS_utils.cpp:386-437: folds the n-element codeword
  * d_cw by the fd challenges h_a (fd elements) into n/fd elements at d_out
  * (the reference replaces the codeword; here d_out must not alias d_cw).

@@ -1,0 +1,458 @@
+// cosnark_dropin.hpp -- the reference prover's rank-local entry points, with
+// their names, argument order and meaning, running on the GPU through the
+// cosnark C ABI (cosnark.h).  Header only; link libcosnark.so.
+//
+// F is the caller's field element: the reference's virgo::fieldElement
+// (src/fieldElement.hpp:96-97), 16 bytes, u64 real then u64 img, canonical.
+// H is its digest type: _hash (src/Blake3_hash.h:3-5), uint8_t arr[32].
+// A reference source file adopts an entry point by forwarding to it, e.g.
+//     void merkle_tree::merkle_tree_prover::MT_commit_Blake(F *l, vector<vector<_hash>> &h, int N)
+//     { cosnark::dropin::MT_commit_Blake(l, h, N); }
+//
+// Conventions kept from the reference: vectors by reference, mutated and
+// resized as the reference does; the Fiat-Shamir running value is one
+// state per thread here (the reference's global prev_challenge,
+// src/Fiat_Shamir.cpp:5), read and set with transcript(); the globals
+// sumcheck_offset (src/main.cpp:36) and rate_offset (src/coPCS.cpp:22) are
+// sumcheck_offset() and rate_offset().  Differences: where the reference
+// prints a message and calls exit(-1), these throw cosnark::dropin::error;
+// work runs on a per-thread context on GPU 0 (COSNARK_DEVICE selects
+// another), and each call copies its inputs to the device and its results
+// back.  Cross-rank steps (MPI aggregation, all-to-all, the rank-0 tree of
+// roots for N > 1) are outside this layer: the functions take N and accept
+// N = 1, where those steps are the identity or a single root.
+#pragma once
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <utility>
+#include <vector>
+#include "cosnark.h"
+
+namespace cosnark {
+namespace dropin {
+
+struct error : std::runtime_error {
+    int code;
+    error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+// ---- process state mirroring the reference's globals ----------------------
+inline int &sumcheck_offset() { static int v = 0; return v; }          // src/main.cpp:36
+inline unsigned &rate_offset() { static unsigned v = 1; return v; }     // src/coPCS.cpp:22,46
+inline cosnark_transcript &transcript() {                               // src/Fiat_Shamir.cpp:5
+    thread_local cosnark_transcript t{{0, 0}};
+    return t;
+}
+
+namespace detail {
+
+struct Ctx {
+    cosnark_ctx *c = nullptr;
+    ~Ctx() { if (c) cosnark_destroy(c); }
+};
+
+inline cosnark_ctx *ctx() {
+    thread_local Ctx h;
+    if (!h.c) {
+        const char *d = std::getenv("COSNARK_DEVICE");
+        const int rc = cosnark_create(d ? std::atoi(d) : 0, &h.c);
+        if (rc) throw error(rc, "cosnark: no GPU context (is the HIP device visible?)");
+    }
+    return h.c;
+}
+
+inline void check(int rc, const char *what) {
+    if (rc) throw error(rc, std::string(what) + ": " + cosnark_last_error(ctx()));
+}
+
+template <class F>
+void elem_type() {
+    static_assert(sizeof(F) == 16 && std::is_trivially_copyable<F>::value,
+                  "field element must be 16 bytes: u64 real, u64 img");
+}
+
+template <class H>
+void digest_type() {
+    static_assert(sizeof(H) == 32 && std::is_trivially_copyable<H>::value, "digest must be 32 bytes");
+}
+
+// device buffer, freed on scope exit
+class Dev {
+  public:
+    explicit Dev(uint64_t bytes) : n_(bytes) {
+        check(cosnark_dev_alloc(ctx(), bytes ? bytes : 16, &p_), "device allocation");
+    }
+    Dev(const void *host, uint64_t bytes) : Dev(bytes) { put(host, bytes); }
+    ~Dev() { if (p_) cosnark_dev_free(ctx(), p_); }
+    Dev(const Dev &) = delete;
+    Dev &operator=(const Dev &) = delete;
+    void put(const void *host, uint64_t bytes) { if (bytes) check(cosnark_h2d(ctx(), p_, host, bytes), "copy in"); }
+    void get(void *host, uint64_t bytes) const { if (bytes) check(cosnark_d2h(ctx(), host, p_, bytes), "copy out"); }
+    uint64_t *u64() const { return static_cast<uint64_t *>(p_); }
+    uint8_t *u8() const { return static_cast<uint8_t *>(p_); }
+
+  private:
+    void *p_ = nullptr;
+    uint64_t n_;
+};
+
+inline int ilog2(uint64_t x) { int l = 0; while ((1ull << l) < x) ++l; return l; }
+inline bool pow2(uint64_t x) { return x && !(x & (x - 1)); }
+
+// all levels of a tree, level 0 first, into vector<vector<H>>
+template <class H>
+void unpack_levels(const std::vector<uint8_t> &nodes, uint64_t leaves, std::vector<std::vector<H>> &hashes) {
+    const int depth = ilog2(leaves);
+    if ((int)hashes.size() < depth + 1) hashes.resize(depth + 1);
+    uint64_t off = 0;
+    for (int l = 0; l <= depth; ++l) {
+        const uint64_t cnt = leaves >> l;
+        hashes[l].resize(cnt);
+        std::memcpy(hashes[l].data(), nodes.data() + 32 * off, 32 * cnt);
+        off += cnt;
+    }
+}
+
+template <class F>
+F make(uint64_t re, uint64_t im) {
+    F x;
+    uint64_t w[2] = {re, im};
+    std::memcpy(&x, w, 16);
+    return x;
+}
+
+}  // namespace detail
+
+// ---- NTT -------------------------------------------------------------------
+
+// fft(arr, logn, flag), src/utils.cpp:904-964: in place, natural order;
+// flag = inverse (scaled by 2^-logn).  arr.size() must be 2^logn.
+template <class F>
+void fft(std::vector<F> &arr, int logn, bool flag) {
+    detail::elem_type<F>();
+    if (arr.size() != (1ull << logn)) throw error(COSNARK_E_ARG, "fft: arr.size() != 2^logn");
+    if (logn == 0) return;
+    detail::Dev d(arr.data(), 16 * arr.size());
+    detail::check(cosnark_ntt(detail::ctx(), d.u64(), logn, flag ? 1 : 0), "fft");
+    d.get(arr.data(), 16 * arr.size());
+}
+
+// pfft(arr, logn, flag), src/utils.cpp:1583-1673: the forward transform
+// (the reference ignores flag and does not scale).
+template <class F>
+void pfft(std::vector<F> &arr, int logn, bool) { fft(arr, logn, false); }
+
+// ---- Fiat-Shamir, src/Fiat_Shamir.cpp ----------------------------------------
+
+// hash_to_field(elements), :29-34: absorbs the elements into the thread's
+// transcript and returns its new value
+template <class F>
+F hash_to_field(const std::vector<F> &elements) {
+    detail::elem_type<F>();
+    uint64_t out[2];
+    detail::check(cosnark_hash_to_field(&transcript(), reinterpret_cast<const uint64_t *>(elements.data()),
+                                        elements.size(), out), "hash_to_field");
+    return detail::make<F>(out[0], out[1]);
+}
+
+// get_indexes(l, N, M, pos), :37-56 (pos = -1: all l pairs)
+inline std::vector<std::vector<uint32_t>> get_indexes(int l, int N, int M, int pos) {
+    std::vector<uint32_t> flat(2 * (size_t)(l > 0 ? l : 0));
+    detail::check(cosnark_get_indexes(&transcript(), l, (uint32_t)N, (uint32_t)M, flat.data()), "get_indexes");
+    std::vector<std::vector<uint32_t>> idx;
+    for (int i = 0; i < l; ++i) {
+        if (pos != -1 && flat[2 * i] != (uint32_t)pos) continue;
+        idx.push_back({flat[2 * i], flat[2 * i + 1]});
+    }
+    return idx;
+}
+
+// ---- Merkle tree, src/merkle_tree.cpp ----------------------------------------
+
+// MT_commit_Blake(leafs, hashes, N), :195-233: leaf i = BLAKE3 of elements
+// 4i..4i+3; every level into hashes (level 0 first).  N: power of two >= 8
+// (the reference's N <= 2 branch hashes 32 uninitialised bytes).
+template <class F, class H>
+void MT_commit_Blake(F *leafs, std::vector<std::vector<H>> &hashes, int N) {
+    detail::elem_type<F>();
+    detail::digest_type<H>();
+    if (N < 8 || !detail::pow2((uint64_t)N)) throw error(COSNARK_E_ARG, "MT_commit_Blake: N must be a power of two >= 8");
+    const uint64_t cnt = cosnark_merkle_node_count((uint64_t)N);
+    detail::Dev e(leafs, 16ull * N), nodes(32 * cnt);
+    detail::check(cosnark_merkle_commit(detail::ctx(), e.u64(), (uint64_t)N, nodes.u8(), 1), "MT_commit_Blake");
+    std::vector<uint8_t> h(32 * cnt);
+    nodes.get(h.data(), h.size());
+    detail::unpack_levels(h, (uint64_t)N / 4, hashes);
+}
+
+// create_tree_blake(ele_num, hashes, element_size, alloc_required),
+// :267-299: levels 1.. over hashes[0] (ele_num digests, a power of two),
+// each parent the hash of its left child twice.
+template <class H>
+void create_tree_blake(int ele_num, std::vector<std::vector<H>> &hashes, int = 32, bool = false) {
+    detail::digest_type<H>();
+    if (ele_num < 1 || !detail::pow2((uint64_t)ele_num) || hashes.empty() || (int)hashes[0].size() < ele_num)
+        throw error(COSNARK_E_ARG, "create_tree_blake: need hashes[0] with ele_num (a power of two) digests");
+    const uint64_t total = 2ull * ele_num - 1;
+    detail::Dev nodes(32 * total);
+    nodes.put(hashes[0].data(), 32ull * ele_num);
+    detail::check(cosnark_merkle_tree_from_leaves(detail::ctx(), (uint64_t)ele_num, nodes.u8(), 1), "create_tree_blake");
+    std::vector<uint8_t> h(32 * total);
+    nodes.get(h.data(), h.size());
+    detail::unpack_levels(h, (uint64_t)ele_num, hashes);
+}
+
+// open_tree_blake(MT_hashes, pos), :320-335: the sibling of pos on every
+// level below the root (a lookup in the host copy of the tree)
+template <class H>
+std::vector<H> open_tree_blake(std::vector<std::vector<H>> &MT_hashes, int pos) {
+    if (MT_hashes.empty() || pos < 0 || pos >= (int)MT_hashes[0].size())
+        throw error(COSNARK_E_ARG, "open_tree_blake: position out of range");
+    std::vector<H> path;
+    for (size_t i = 0; i + 1 < MT_hashes.size(); ++i) {
+        const size_t sib = 2 * (size_t)(pos / 2) + (1 - pos % 2);
+        if (sib >= MT_hashes[i].size()) throw error(COSNARK_E_ARG, "open_tree_blake: tree level too short");
+        path.push_back(MT_hashes[i][sib]);
+        pos /= 2;
+    }
+    return path;
+}
+
+// ---- product sumcheck ----------------------------------------------------------
+
+// _quadratic_sumcheck(y, v1, v2, N), src/Distributed_Sumcheck.cpp:204-348 at
+// N = 1 with the tail of src/sparse_eval.cpp:140-192: the first
+// log2|v| - sumcheck_offset() rounds take Fiat-Shamir challenges from the
+// thread's transcript, the rest draw F::_random() from libc random().
+// Returns {(v1 folded, r), (v2 folded, r)}, r the challenges in round order.
+// Throws where the reference exits (H(0) + H(1) != y).  v1, v2 are not
+// modified (the reference leaves them partly folded).
+template <class F>
+std::vector<std::pair<F, std::vector<F>>> _quadratic_sumcheck(F y, std::vector<F> &v1, std::vector<F> &v2, int N) {
+    detail::elem_type<F>();
+    if (N != 1) throw error(COSNARK_E_ARG, "_quadratic_sumcheck: one rank (N = 1) only");
+    const uint64_t n = v1.size();
+    if (v2.size() != n || !detail::pow2(n) || n < 2) throw error(COSNARK_E_ARG, "_quadratic_sumcheck: sizes");
+    const int m = detail::ilog2(n);
+    detail::Dev a(v1.data(), 16 * n), b(v2.data(), 16 * n);
+    std::vector<uint64_t> polys(6 * (size_t)m), r(2 * (size_t)m), fin(4);
+    uint64_t yy[2];
+    std::memcpy(yy, &y, 16);
+    detail::check(cosnark_product_sumcheck(detail::ctx(), a.u64(), b.u64(), n, yy, &transcript(), sumcheck_offset(),
+                                           polys.data(), r.data(), fin.data()),
+                  "_quadratic_sumcheck");
+    std::vector<F> rr(m);
+    for (int i = 0; i < m; ++i) rr[i] = detail::make<F>(r[2 * i], r[2 * i + 1]);
+    return {std::make_pair(detail::make<F>(fin[0], fin[1]), rr), std::make_pair(detail::make<F>(fin[2], fin[3]), rr)};
+}
+
+// ---- WHIR, src/Whir.cpp (whir_k = 4) -----------------------------------------------
+
+// Whir_Commit_initial(arr, codeword, MT, rate, kappa), :158-190: codeword =
+// the rate*|arr| evaluations of arr's coefficients, MT its tree
+template <class F, class H>
+void Whir_Commit_initial(std::vector<F> &arr, std::vector<F> &codeword, std::vector<std::vector<H>> &MT, int rate,
+                         int kappa) {
+    detail::elem_type<F>();
+    detail::digest_type<H>();
+    if (kappa != 4) throw error(COSNARK_E_ARG, "Whir_Commit_initial: whir_k = 4 only");
+    const uint64_t n = arr.size(), cw = n * (uint64_t)rate, cnt = cosnark_merkle_node_count(cw);
+    detail::Dev a(arr.data(), 16 * n), c(16 * cw), nodes(32 * cnt);
+    detail::check(cosnark_whir_commit(detail::ctx(), a.u64(), n, rate, c.u64(), nodes.u8()), "Whir_Commit_initial");
+    codeword.resize(cw);
+    c.get(codeword.data(), 16 * cw);
+    std::vector<uint8_t> h(32 * cnt);
+    nodes.get(h.data(), h.size());
+    detail::unpack_levels(h, cw / 4, MT);
+}
+
+// The opening record of the thread's last Whir_Open (cosnark_whir_open's
+// layout), for callers that serialise or verify it.
+inline std::vector<uint64_t> &last_whir_record() {
+    thread_local std::vector<uint64_t> rec;
+    return rec;
+}
+
+// Whir_Open(arr, codeword, MT, claims, rate), :388-467, one claim (K, z):
+// the iterations down to |arr| <= 4, challenges from libc random()/rand()
+// in the reference's order.  Throws where the reference's checks exit.
+template <class F, class H>
+void Whir_Open(std::vector<F> &arr, std::vector<F> &codeword, std::vector<std::vector<H>> &MT,
+               std::vector<std::pair<F, std::vector<F>>> claims, int rate) {
+    detail::elem_type<F>();
+    detail::digest_type<H>();
+    if (claims.size() != 1) throw error(COSNARK_E_ARG, "Whir_Open: one claim only");
+    const uint64_t n = arr.size(), cw = n * (uint64_t)rate, cnt = cosnark_merkle_node_count(cw);
+    if (codeword.size() != cw || MT.empty() || MT[0].size() != cw / 4)
+        throw error(COSNARK_E_ARG, "Whir_Open: codeword / tree do not match arr and rate");
+    detail::Dev a(arr.data(), 16 * n), c(codeword.data(), 16 * cw), nodes(32 * cnt);
+    uint64_t off = 0;
+    for (size_t l = 0; l < MT.size(); ++l) {
+        if (off + MT[l].size() > cnt) throw error(COSNARK_E_ARG, "Whir_Open: tree larger than expected");
+        detail::check(cosnark_h2d(detail::ctx(), nodes.u8() + 32 * off, MT[l].data(), 32 * MT[l].size()), "copy in");
+        off += MT[l].size();
+    }
+    if (off != cnt) throw error(COSNARK_E_ARG, "Whir_Open: tree smaller than expected");
+    uint64_t K[2];
+    std::memcpy(K, &claims[0].first, 16);
+    auto &rec = last_whir_record();
+    rec.assign(1u << 22, 0);
+    uint64_t words = 0;
+    detail::check(cosnark_whir_open(detail::ctx(), a.u64(), n, reinterpret_cast<const uint64_t *>(claims[0].second.data()),
+                                    K, rate, 0, c.u64(), nodes.u8(), rec.data(), rec.size(), &words),
+                  "Whir_Open");
+    rec.resize(words);
+}
+
+// ---- coPCS, src/coPCS.cpp and src/coPCS_utils.cpp ----------------------------------
+
+// Encode(Codeword, N, rate), src/coPCS.cpp:131-147: every non-empty row
+// (before the first empty one, at most 32) becomes its coset encoding at
+// blowup rate[0], with w = root(1 + log2 rate_offset() + log2 |row|).
+template <class F>
+void Encode(std::vector<std::vector<F>> &Codeword, int, std::vector<int> rate) {
+    detail::elem_type<F>();
+    size_t rows = 0;
+    while (rows < Codeword.size() && rows < 32 && !Codeword[rows].empty()) ++rows;
+    if (rows == 0) return;
+    const uint64_t L = Codeword[0].size(), M = L * (uint64_t)rate.at(0);
+    for (size_t i = 1; i < rows; ++i)
+        if (Codeword[i].size() != L) throw error(COSNARK_E_ARG, "Encode: rows of different sizes");
+    detail::Dev in(16 * L * rows), out(16 * M * rows);
+    for (size_t i = 0; i < rows; ++i)
+        detail::check(cosnark_h2d(detail::ctx(), in.u8() + 16 * L * i, Codeword[i].data(), 16 * L), "copy in");
+    detail::check(cosnark_copcs_encode(detail::ctx(), in.u64(), (int)rows, L, rate[0], detail::ilog2(rate_offset()),
+                                       out.u64()),
+                  "Encode");
+    for (size_t i = 0; i < rows; ++i) {
+        Codeword[i].resize(M);
+        detail::check(cosnark_d2h(detail::ctx(), Codeword[i].data(), out.u8() + 16 * M * i, 16 * M), "copy out");
+    }
+}
+
+// distributed_MT(Codeword, Com, N), src/coPCS.cpp:152-230, one rank: Base_MT
+// is the tree over the column roots; at N = 1 Root_MT is that one root.
+// MTT is the reference's MT struct (Base_MT, Root_MT of vector<vector<_hash>>).
+template <class F, class MTT>
+void distributed_MT(std::vector<std::vector<F>> &Codeword, MTT &Com, int N) {
+    detail::elem_type<F>();
+    int rows = 0;
+    while (rows < (int)Codeword.size() && !Codeword[rows].empty()) ++rows;
+    if (rows == 0) throw error(COSNARK_E_ARG, "distributed_MT: empty codeword");
+    if (rows > 32) throw error(COSNARK_E_ARG, "distributed_MT: more than 32 rows");
+    const uint64_t Lc = Codeword[0].size();
+    detail::Dev cw(16 * Lc * rows), nodes(32 * (2 * Lc - 1));
+    for (int i = 0; i < rows; ++i)
+        detail::check(cosnark_h2d(detail::ctx(), cw.u8() + 16 * Lc * i, Codeword[i].data(), 16 * Lc), "copy in");
+    detail::check(cosnark_distributed_mt(detail::ctx(), cw.u64(), rows, Lc, nodes.u8()), "distributed_MT");
+    std::vector<uint8_t> h(32 * (2 * Lc - 1));
+    nodes.get(h.data(), h.size());
+    Com.Base_MT.clear();
+    detail::unpack_levels(h, Lc, Com.Base_MT);
+    if (N == 1) {
+        Com.Root_MT.assign(1, {Com.Base_MT.back()[0]});
+    }
+}
+
+// pcs_sumcheck_phase1(Commit_data, B, degree, N), src/coPCS.cpp:547-569:
+// this rank's 2*degree-element polynomial (pcs_sumcheck_aggr, the MPI
+// aggregation, is the caller's).
+template <class F>
+std::vector<F> pcs_sumcheck_phase1(std::vector<F> &Commit_data, std::vector<F> &B, int degree, int) {
+    detail::elem_type<F>();
+    const uint64_t M = Commit_data.size();
+    if (B.size() != M) throw error(COSNARK_E_ARG, "pcs_sumcheck_phase1: sizes");
+    detail::Dev a(Commit_data.data(), 16 * M), b(B.data(), 16 * M);
+    std::vector<F> poly(2 * (size_t)degree);
+    detail::check(cosnark_cosumcheck_phase1(detail::ctx(), a.u64(), b.u64(), M, degree,
+                                            reinterpret_cast<uint64_t *>(poly.data())),
+                  "pcs_sumcheck_phase1");
+    return poly;
+}
+
+// pcs_sumcheck_phase2(Commit_data, B, degree, r), src/coPCS.cpp:571-587
+template <class F>
+void pcs_sumcheck_phase2(std::vector<F> &Commit_data, std::vector<F> &B, int degree, F r) {
+    detail::elem_type<F>();
+    const uint64_t M = Commit_data.size();
+    if (B.size() != M) throw error(COSNARK_E_ARG, "pcs_sumcheck_phase2: sizes");
+    detail::Dev a(Commit_data.data(), 16 * M), b(B.data(), 16 * M);
+    uint64_t rr[2];
+    std::memcpy(rr, &r, 16);
+    detail::check(cosnark_cosumcheck_phase2(detail::ctx(), a.u64(), b.u64(), M, degree, rr), "pcs_sumcheck_phase2");
+    Commit_data.resize(M / degree);
+    B.resize(M / degree);
+    a.get(Commit_data.data(), 16 * (M / degree));
+    b.get(B.data(), 16 * (M / degree));
+}
+
+// quadratic_coSumcheck_phase1(p1, p2, R, r, d, iter, N),
+// src/coSumcheck.cpp:150-198, this rank's part: the block sum plus r * H_r
+// with H_r = R[iter] and H_r[0] += the even entries of R[iter+1..]
+// (sumcheck_aggr, the MPI aggregation, is the caller's).
+template <class F>
+std::vector<F> quadratic_coSumcheck_phase1(std::vector<F> &p1, std::vector<F> &p2, std::vector<std::vector<F>> R, F r,
+                                           int d, int iter, int) {
+    detail::elem_type<F>();
+    const uint64_t M = p1.size();
+    if (p2.size() != M) throw error(COSNARK_E_ARG, "quadratic_coSumcheck_phase1: sizes");
+    detail::Dev a(p1.data(), 16 * M), b(p2.data(), 16 * M);
+    std::vector<F> H(2 * (size_t)d);
+    detail::check(cosnark_cosumcheck_phase1(detail::ctx(), a.u64(), b.u64(), M, d, reinterpret_cast<uint64_t *>(H.data())),
+                  "quadratic_coSumcheck_phase1");
+    // the mask (src/coSumcheck.cpp:182-193): H_r = R[iter], H_r[0] += the
+    // even entries of R[iter+1..]; H[i] += r * H_r[i]
+    std::vector<F> Hr = R.at(iter);
+    if (Hr.size() < H.size()) throw error(COSNARK_E_ARG, "quadratic_coSumcheck_phase1: R[iter] shorter than 2d");
+    auto w = [](const F &x) { const uint64_t *p = reinterpret_cast<const uint64_t *>(&x); return p; };
+    uint64_t h0[2] = {w(Hr[0])[0], w(Hr[0])[1]};
+    for (size_t j = iter + 1; j < R.size(); ++j)
+        for (size_t k = 0; k < R[j].size() / 2; ++k) cosnark_field_add(h0, w(R[j][2 * k]), h0);
+    std::memcpy(&Hr[0], h0, 16);
+    uint64_t rr[2];
+    std::memcpy(rr, &r, 16);
+    for (size_t i = 0; i < H.size(); ++i) {
+        uint64_t t[2], h[2];
+        cosnark_field_mul(rr, w(Hr[i]), t);
+        cosnark_field_add(w(H[i]), t, h);
+        std::memcpy(&H[i], h, 16);
+    }
+    return H;
+}
+
+// quadratic_coSumcheck_phase2(p1, p2, L, d), src/coSumcheck.cpp:201-213
+template <class F>
+void quadratic_coSumcheck_phase2(std::vector<F> &p1, std::vector<F> &p2, std::vector<F> L, int d) {
+    detail::elem_type<F>();
+    const uint64_t M = p1.size();
+    if (p2.size() != M || (int)L.size() < d) throw error(COSNARK_E_ARG, "quadratic_coSumcheck_phase2: sizes");
+    detail::Dev a(p1.data(), 16 * M), b(p2.data(), 16 * M);
+    detail::check(cosnark_cosumcheck_fold(detail::ctx(), a.u64(), b.u64(), M, d, reinterpret_cast<const uint64_t *>(L.data())),
+                  "quadratic_coSumcheck_phase2");
+    p1.resize(M / d);
+    p2.resize(M / d);
+    a.get(p1.data(), 16 * (M / d));
+    b.get(p2.data(), 16 * (M / d));
+}
+
+// RS_multifold5(codeword, a, fold_degree), src/coPCS_utils.cpp:386-437:
+// the codeword is replaced by its fold (|codeword| / fold_degree elements)
+template <class F>
+void RS_multifold5(std::vector<F> &codeword, std::vector<F> a, int fold_degree) {
+    detail::elem_type<F>();
+    if ((int)a.size() != fold_degree) throw error(COSNARK_E_ARG, "RS_multifold5: a.size() != fold_degree");
+    const uint64_t n = codeword.size();
+    detail::Dev c(codeword.data(), 16 * n), out(16 * (n / fold_degree));
+    detail::check(cosnark_rs_multifold5(detail::ctx(), c.u64(), n, reinterpret_cast<const uint64_t *>(a.data()),
+                                        fold_degree, out.u64()),
+                  "RS_multifold5");
+    codeword.resize(n / fold_degree);
+    out.get(codeword.data(), 16 * codeword.size());
+}
+
+}  // namespace dropin
+}  // namespace cosnark

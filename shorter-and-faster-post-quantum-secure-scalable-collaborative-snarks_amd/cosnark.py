@@ -59,6 +59,7 @@ def lib():
         L.cosnark_merkle_node_count.argtypes = [u64]
         L.cosnark_merkle_node_count.restype = u64
         L.cosnark_merkle_commit.argtypes = [vp, vp, u64, vp, i32]
+        L.cosnark_merkle_tree_from_leaves.argtypes = [vp, u64, vp, i32]
         L.cosnark_merkle_open.argtypes = [vp, vp, u64, u64, vp, ctypes.POINTER(i32)]
         L.cosnark_hash_to_field.argtypes = [vp, vp, u64, vp]
         L.cosnark_get_indexes.argtypes = [vp, i32, u32, u32, vp]
@@ -73,6 +74,7 @@ def lib():
         L.cosnark_distributed_mt.argtypes = [vp, vp, i32, u64, vp]
         L.cosnark_cosumcheck_phase1.argtypes = [vp, vp, vp, u64, i32, vp]
         L.cosnark_cosumcheck_phase2.argtypes = [vp, vp, vp, u64, i32, vp]
+        L.cosnark_cosumcheck_fold.argtypes = [vp, vp, vp, u64, i32, vp]
         L.cosnark_rs_multifold5.argtypes = [vp, vp, u64, vp, i32, vp]
         L.cosnark_batch_create.argtypes = [vp, i32, vp, i32, ctypes.POINTER(vp)]
         L.cosnark_batch_destroy.argtypes = [vp]

@@ -72,6 +72,18 @@ extern "C" {
 
 const char *cosnark_version(void) { return "cosnark-hip 0.1 gfx950"; }
 
+void cosnark_field_mul(const uint64_t a[2], const uint64_t b[2], uint64_t out[2]) {
+    const fe r = gf::mul(gf::mk(a[0], a[1]), gf::mk(b[0], b[1]));
+    out[0] = r.re;
+    out[1] = r.im;
+}
+
+void cosnark_field_add(const uint64_t a[2], const uint64_t b[2], uint64_t out[2]) {
+    const fe r = gf::add(gf::mk(a[0], a[1]), gf::mk(b[0], b[1]));
+    out[0] = r.re;
+    out[1] = r.im;
+}
+
 int cosnark_create(int device, cosnark_ctx **out) {
     if (!out) return COSNARK_E_ARG;
     *out = nullptr;
@@ -271,6 +283,13 @@ int cosnark_merkle_commit(cosnark_ctx *ctx, const uint64_t *d_elems, uint64_t n_
                           int dup_left) {
     if (!ctx || !d_elems || !d_nodes) return set_err(ctx, COSNARK_E_ARG, "merkle_commit: null argument");
     int rc = merkle_launch(ctx, (const fe *)d_elems, n_elems, d_nodes, dup_left != 0, ctx->stream);
+    if (rc) return rc;
+    return cosnark_sync(ctx);
+}
+
+int cosnark_merkle_tree_from_leaves(cosnark_ctx *ctx, uint64_t n_leaves, uint8_t *d_nodes, int dup_left) {
+    if (!ctx || !d_nodes) return set_err(ctx, COSNARK_E_ARG, "merkle_tree_from_leaves: null argument");
+    int rc = merkle_from_leaves_launch(ctx, n_leaves, d_nodes, dup_left != 0, ctx->stream);
     if (rc) return rc;
     return cosnark_sync(ctx);
 }

@@ -509,19 +509,18 @@ int cosnark_cosumcheck_phase1(cosnark_ctx *ctx, const uint64_t *d_p1, const uint
     return COSNARK_OK;
 }
 
-int cosnark_cosumcheck_phase2(cosnark_ctx *ctx, uint64_t *d_p1, uint64_t *d_p2, uint64_t M, int d,
-                              const uint64_t r[2]) {
-    if (!ctx || !d_p1 || !d_p2 || !r) return set_err(ctx, COSNARK_E_ARG, "cosumcheck_phase2: null argument");
+int cosnark_cosumcheck_fold(cosnark_ctx *ctx, uint64_t *d_p1, uint64_t *d_p2, uint64_t M, int d,
+                            const uint64_t *h_L) {
+    if (!ctx || !d_p1 || !d_p2 || !h_L) return set_err(ctx, COSNARK_E_ARG, "cosumcheck_fold: null argument");
     if (d < 2 || !is_pow2((uint64_t)d) || !is_pow2(M) || M < (uint64_t)d)
-        return set_err(ctx, COSNARK_E_ARG, "cosumcheck_phase2: d and M must be powers of two, 2 <= d <= M");
-    if (d > CO_HOST_ELEMS) return set_err(ctx, COSNARK_E_RANGE, "cosumcheck_phase2: d too large");
+        return set_err(ctx, COSNARK_E_ARG, "cosumcheck_fold: d and M must be powers of two, 2 <= d <= M");
+    if (d > CO_HOST_ELEMS) return set_err(ctx, COSNARK_E_RANGE, "cosumcheck_fold: d too large");
     const uint64_t nout = M / d;
     void *base;
     int rc = scratch_reserve(ctx, sizeof(fe) * (2 * nout + d), &base);
     if (rc) return rc;
     fe *o1 = (fe *)base, *o2 = o1 + nout, *Ld = o2 + nout;
-    const std::vector<fe> L = lagrange_weights(gf::mk(r[0], r[1]), d);
-    memcpy(ctx->co_host, L.data(), sizeof(fe) * d);
+    memcpy(ctx->co_host, h_L, sizeof(fe) * d);
     hipStream_t s = ctx->stream;
     HIPCK(hipMemcpyAsync(Ld, ctx->co_host, sizeof(fe) * d, hipMemcpyHostToDevice, s));
     if (d <= 64) {
@@ -537,6 +536,15 @@ int cosnark_cosumcheck_phase2(cosnark_ctx *ctx, uint64_t *d_p1, uint64_t *d_p2, 
     HIPCK(hipMemcpyAsync(d_p1, o1, sizeof(fe) * nout, hipMemcpyDeviceToDevice, s));
     HIPCK(hipMemcpyAsync(d_p2, o2, sizeof(fe) * nout, hipMemcpyDeviceToDevice, s));
     return cosnark_sync(ctx);
+}
+
+int cosnark_cosumcheck_phase2(cosnark_ctx *ctx, uint64_t *d_p1, uint64_t *d_p2, uint64_t M, int d,
+                              const uint64_t r[2]) {
+    if (!ctx || !r) return set_err(ctx, COSNARK_E_ARG, "cosumcheck_phase2: null argument");
+    if (d < 2 || !is_pow2((uint64_t)d) || d > CO_HOST_ELEMS)
+        return set_err(ctx, COSNARK_E_ARG, "cosumcheck_phase2: d must be a power of two in [2, 8192]");
+    const std::vector<fe> L = lagrange_weights(gf::mk(r[0], r[1]), d);
+    return cosnark_cosumcheck_fold(ctx, d_p1, d_p2, M, d, (const uint64_t *)L.data());
 }
 
 int cosnark_rs_multifold5(cosnark_ctx *ctx, const uint64_t *d_cw, uint64_t n, const uint64_t *h_a, int fd,

@@ -101,6 +101,8 @@ int ntt_plan_tile(const cosnark_ctx *ctx, int logn);
 // Merkle commit of n_elems elements into nodes (all levels).
 int merkle_launch(cosnark_ctx *ctx, const gf::fe *elems, uint64_t n_elems, uint8_t *nodes,
                   bool dup_left, hipStream_t s);
+// levels 1.. of a tree whose `leaves` level-0 digests are in nodes[0 .. leaves)
+int merkle_from_leaves_launch(cosnark_ctx *ctx, uint64_t leaves, uint8_t *nodes, bool dup_left, hipStream_t s);
 // distributed_MT tree: first level = column roots of a row-major nrows x Lc
 // matrix (rows nrows..31 zero), then the dup-left tree (2 Lc - 1 digests).
 int merkle_columns_launch(cosnark_ctx *ctx, const gf::fe *cw, int nrows, uint64_t Lc, uint8_t *nodes,

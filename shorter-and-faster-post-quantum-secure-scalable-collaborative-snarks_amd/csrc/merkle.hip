@@ -246,6 +246,29 @@ std::vector<std::pair<MkJob, uint64_t>> tree_stages(int mode, const fe *elems, u
     return st;
 }
 
+// Levels 1.. above `leaves` level-0 digests already in nodes[0 .. leaves).
+std::vector<std::pair<MkJob, uint64_t>> upper_stages(uint64_t leaves, uint8_t *nodes, bool dup_left) {
+    std::vector<std::pair<MkJob, uint64_t>> st;
+    MkJob J{};
+    J.nodes = nodes;
+    J.lo = level_offsets(leaves);
+    J.dup_left = (int)dup_left;
+    int top = 0;
+    uint64_t count = leaves;
+    while (count > 1) {
+        const uint64_t next = count / 2;
+        MkJob U = J;
+        U.in_nodes = nodes + 32 * J.lo.off[top];
+        U.lvl0 = top + 1;
+        U.block_log = ilog2(next < (uint64_t)MK_BLOCK ? next : (uint64_t)MK_BLOCK);
+        const uint64_t nb = next >> U.block_log;
+        st.push_back({U, nb});
+        top += 1 + U.block_log;
+        count = nb;
+    }
+    return st;
+}
+
 int build_tree(cosnark_ctx *ctx, int mode, const fe *elems, uint64_t leaves, uint8_t *nodes, bool dup_left,
                uint64_t col_len, int col_rows, hipStream_t stream) {
     const auto st = tree_stages(mode, elems, leaves, nodes, dup_left, col_len, col_rows);
@@ -265,6 +288,17 @@ int merkle_launch(cosnark_ctx *ctx, const fe *elems, uint64_t n_elems, uint8_t *
     if (!is_pow2(n_elems) || n_elems < 8 || n_elems > (1ull << 30))
         return set_err(ctx, COSNARK_E_ARG, "merkle: n_elems must be a power of two in [8, 2^30]");
     return build_tree(ctx, MODE_ELEMS, elems, n_elems / 4, nodes, dup_left, 0, 0, stream);
+}
+
+int merkle_from_leaves_launch(cosnark_ctx *ctx, uint64_t leaves, uint8_t *nodes, bool dup_left, hipStream_t stream) {
+    if (!is_pow2(leaves) || leaves > (1ull << 30))
+        return set_err(ctx, COSNARK_E_ARG, "merkle: leaf count must be a power of two <= 2^30");
+    for (const auto &st : upper_stages(leaves, nodes, dup_left)) {
+        hipLaunchKernelGGL(merkle_block_kernel<MODE_NODES>, dim3((unsigned)st.second), dim3(MK_THREADS), 0, stream,
+                           st.first);
+        LAUNCHCK();
+    }
+    return COSNARK_OK;
 }
 
 int merkle_columns_launch(cosnark_ctx *ctx, const fe *cw, int nrows, uint64_t Lc, uint8_t *nodes,

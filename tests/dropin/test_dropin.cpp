@@ -1,0 +1,220 @@
+// The drop-in layer (include/cosnark_dropin.hpp) driven as the reference's
+// callers drive its functions, with the reference's type shapes (F =
+// {u64 real, img}, _hash = {u8 arr[32]}, MT = {Base_MT, Root_MT}), checked
+// against the oracle (oracle/oracle.h).  Exit status 0 when every check
+// passes; prints the first failure otherwise.
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+#include "cosnark_dropin.hpp"
+#include "../../oracle/oracle.h"
+
+using std::vector;
+namespace D = cosnark::dropin;
+
+struct F { uint64_t real, img; };
+struct _hash { uint8_t arr[32]; };
+struct MT { vector<vector<_hash>> Base_MT, Root_MT; };
+
+static int failures = 0;
+#define CHECK(cond, what)                                                  \
+    do {                                                                   \
+        if (!(cond)) { std::printf("FAIL: %s\n", what); ++failures; }      \
+        else std::printf("ok: %s\n", what);                                \
+        std::fflush(stdout);                                               \
+    } while (0)
+
+static vector<F> synth(uint64_t seed, uint64_t stream, uint64_t n) {
+    vector<F> v(n);
+    or_synth_fill(seed, stream, reinterpret_cast<or_fe *>(v.data()), n);
+    return v;
+}
+static bool same(const void *a, const void *b, size_t bytes) { return std::memcmp(a, b, bytes) == 0; }
+static or_fe O(const F &x) { return or_fe{x.real, x.img}; }
+
+int main() {
+    const uint64_t seed = 0xC05A0040;
+    // fft / inverse
+    {
+        vector<F> a = synth(seed, 0, 1 << 12), ref = a;
+        D::fft(a, 12, false);
+        or_fft(reinterpret_cast<or_fe *>(ref.data()), 12, 0);
+        CHECK(same(a.data(), ref.data(), 16 * a.size()), "fft forward 2^12");
+        D::fft(a, 12, true);
+        or_fft(reinterpret_cast<or_fe *>(ref.data()), 12, 1);
+        CHECK(same(a.data(), ref.data(), 16 * a.size()), "fft inverse 2^12");
+    }
+    // MT_commit_Blake, create_tree_blake, open_tree_blake
+    {
+        const int N = 1 << 12;
+        vector<F> e = synth(seed, 1, N);
+        vector<vector<_hash>> H;
+        D::MT_commit_Blake(e.data(), H, N);
+        vector<uint8_t> nodes(32 * or_merkle_node_count(N));
+        or_merkle_commit(reinterpret_cast<or_fe *>(e.data()), N, nodes.data(), 1);
+        bool ok = H.size() == 11;
+        uint64_t off = 0;
+        for (size_t l = 0; ok && l < H.size(); ++l) {
+            ok = same(H[l].data(), nodes.data() + 32 * off, 32 * H[l].size());
+            off += H[l].size();
+        }
+        CHECK(ok, "MT_commit_Blake 2^12 elements, every level");
+        vector<vector<_hash>> T(H.size());
+        T[0] = H[0];
+        D::create_tree_blake((int)H[0].size(), T, 32, true);
+        ok = true;
+        for (size_t l = 0; l < H.size(); ++l) ok = ok && same(T[l].data(), H[l].data(), 32 * H[l].size());
+        CHECK(ok, "create_tree_blake over the leaf digests");
+        vector<_hash> path = D::open_tree_blake(H, 77);
+        vector<uint8_t> opath(32 * 64);
+        const int depth = or_merkle_open(nodes.data(), N, 77, opath.data());
+        CHECK((int)path.size() == depth && same(path.data(), opath.data(), 32 * depth), "open_tree_blake");
+    }
+    // Fiat-Shamir
+    {
+        D::transcript() = cosnark_transcript{{0, 0}};
+        vector<F> el = synth(seed, 2, 5);
+        F r = D::hash_to_field(el);
+        or_fe prev{0, 0};
+        or_fe ro = or_hash_to_field(&prev, reinterpret_cast<or_fe *>(el.data()), 5);
+        CHECK(r.real == ro.real && r.img == ro.img, "hash_to_field");
+        auto idx = D::get_indexes(6, 1 << 10, 1 << 8, -1);
+        vector<uint32_t> oi(12);
+        or_get_indexes(&prev, 6, 1 << 10, 1 << 8, oi.data());
+        bool ok = idx.size() == 6;
+        for (int i = 0; ok && i < 6; ++i) ok = idx[i][0] == oi[2 * i] && idx[i][1] == oi[2 * i + 1];
+        CHECK(ok, "get_indexes");
+    }
+    // _quadratic_sumcheck, all Fiat-Shamir
+    {
+        D::transcript() = cosnark_transcript{{0, 0}};
+        D::sumcheck_offset() = 0;
+        const uint64_t n = 1 << 14;
+        vector<F> v1 = synth(seed, 3, n), v2 = synth(seed, 4, n);
+        or_fe y{0, 0};
+        for (uint64_t i = 0; i < n; ++i) y = or_add(y, or_mul(O(v1[i]), O(v2[i])));
+        auto claims = D::_quadratic_sumcheck(F{y.real, y.img}, v1, v2, 1);
+        vector<F> a = v1, b = v2, polys(3 * 14), r(14), fin(2);
+        or_fe prev{0, 0};
+        const int rc = or_product_sumcheck(y, reinterpret_cast<or_fe *>(a.data()), reinterpret_cast<or_fe *>(b.data()),
+                                           n, &prev, 0, reinterpret_cast<or_fe *>(polys.data()),
+                                           reinterpret_cast<or_fe *>(r.data()), reinterpret_cast<or_fe *>(fin.data()));
+        CHECK(rc == 0 && claims.size() == 2 && same(&claims[0].first, &fin[0], 16) && same(&claims[1].first, &fin[1], 16) &&
+                  claims[0].second.size() == 14 && same(claims[0].second.data(), r.data(), 16 * 14),
+              "_quadratic_sumcheck 2^14");
+        CHECK(D::transcript().prev[0] == prev.real && D::transcript().prev[1] == prev.img, "transcript after sumcheck");
+    }
+    // WHIR commit + open against the oracle's record
+    {
+        const uint64_t n = 1 << 12;
+        const int rate = 4;
+        vector<F> arr = synth(seed, 5, n), z = synth(seed, 6, 12), cw;
+        vector<vector<_hash>> tree;
+        D::Whir_Commit_initial(arr, cw, tree, rate, 4);
+        vector<F> ocw(n * rate);
+        or_rs_encode(reinterpret_cast<or_fe *>(arr.data()), n, rate, reinterpret_cast<or_fe *>(ocw.data()));
+        CHECK(same(cw.data(), ocw.data(), 16 * cw.size()), "Whir_Commit_initial codeword");
+        vector<F> beta(n);
+        or_precompute_beta(reinterpret_cast<or_fe *>(z.data()), 12, reinterpret_cast<or_fe *>(beta.data()));
+        or_fe K{0, 0};
+        for (uint64_t i = 0; i < n; ++i) K = or_add(K, or_mul(O(arr[i]), O(beta[i])));
+        srandom(1234);
+        D::Whir_Open(arr, cw, tree, {{F{K.real, K.img}, z}}, rate);
+        vector<uint64_t> orec(1 << 20);
+        srandom(1234);
+        const int64_t w = or_whir_open(reinterpret_cast<or_fe *>(arr.data()), n, reinterpret_cast<or_fe *>(z.data()), K,
+                                       rate, 0, orec.data(), orec.size());
+        const auto &rec = D::last_whir_record();
+        CHECK(w > 0 && (int64_t)rec.size() == w && same(rec.data(), orec.data(), 8 * w), "Whir_Open record 2^12");
+    }
+    // coPCS commit: Encode + distributed_MT
+    {
+        D::rate_offset() = 64;
+        const uint64_t L = 1 << 10;
+        vector<vector<F>> Codeword(32);
+        vector<F> rows;
+        for (int i = 0; i < 17; ++i) {
+            Codeword[i] = synth(seed, 10 + i, L);
+            rows.insert(rows.end(), Codeword[i].begin(), Codeword[i].end());
+        }
+        D::Encode(Codeword, 1, {4});
+        vector<F> ocw(17 * 4 * L);
+        or_copcs_encode(reinterpret_cast<or_fe *>(rows.data()), 17, L, 4, 6, reinterpret_cast<or_fe *>(ocw.data()));
+        bool ok = Codeword[17].empty();
+        for (int i = 0; ok && i < 17; ++i) ok = Codeword[i].size() == 4 * L && same(Codeword[i].data(), &ocw[i * 4 * L], 64 * L);
+        CHECK(ok, "Encode 17 rows of 2^10, rate 4, rate_offset 64");
+        MT Com;
+        D::distributed_MT(Codeword, Com, 1);
+        vector<uint8_t> base(32 * (2 * 4 * L - 1));
+        or_distributed_mt(reinterpret_cast<or_fe *>(ocw.data()), 17, 4 * L, base.data());
+        ok = Com.Base_MT.size() == 13 && Com.Root_MT.size() == 1 && Com.Root_MT[0].size() == 1;
+        uint64_t off = 0;
+        for (size_t l = 0; ok && l < Com.Base_MT.size(); ++l) {
+            ok = same(Com.Base_MT[l].data(), base.data() + 32 * off, 32 * Com.Base_MT[l].size());
+            off += Com.Base_MT[l].size();
+        }
+        ok = ok && same(Com.Root_MT[0][0].arr, Com.Base_MT.back()[0].arr, 32);
+        CHECK(ok, "distributed_MT Base_MT and Root_MT");
+    }
+    // block sumcheck: pcs phases, coSumcheck phases with the mask
+    {
+        const uint64_t M = 1 << 12;
+        vector<F> p1 = synth(seed, 30, M), p2 = synth(seed, 31, M);
+        vector<F> poly = D::pcs_sumcheck_phase1(p1, p2, 8, 1), opoly(16);
+        or_pcs_sumcheck_phase1(reinterpret_cast<or_fe *>(p1.data()), reinterpret_cast<or_fe *>(p2.data()), M, 8,
+                               reinterpret_cast<or_fe *>(opoly.data()));
+        CHECK(poly.size() == 16 && same(poly.data(), opoly.data(), 16 * 16), "pcs_sumcheck_phase1 d=8");
+        vector<F> q1 = p1, q2 = p2;
+        F r = synth(seed, 32, 1)[0];
+        D::pcs_sumcheck_phase2(q1, q2, 8, r);
+        vector<F> o1 = p1, o2 = p2;
+        or_pcs_sumcheck_phase2(reinterpret_cast<or_fe *>(o1.data()), reinterpret_cast<or_fe *>(o2.data()), M, 8, O(r));
+        CHECK(q1.size() == M / 8 && same(q1.data(), o1.data(), 16 * (M / 8)) && same(q2.data(), o2.data(), 16 * (M / 8)),
+              "pcs_sumcheck_phase2 d=8");
+        // quadratic_coSumcheck: mask from R, fold with given weights
+        vector<vector<F>> R = {synth(seed, 33, 64), synth(seed, 34, 16), synth(seed, 35, 8)};
+        vector<F> H = D::quadratic_coSumcheck_phase1(p1, p2, R, r, 32, 0, 1);
+        vector<F> oh(64);
+        or_pcs_sumcheck_phase1(reinterpret_cast<or_fe *>(p1.data()), reinterpret_cast<or_fe *>(p2.data()), M, 32,
+                               reinterpret_cast<or_fe *>(oh.data()));
+        or_fe h0 = O(R[0][0]);
+        for (int j = 1; j < 3; ++j)
+            for (size_t k = 0; k < R[j].size() / 2; ++k) h0 = or_add(h0, O(R[j][2 * k]));
+        bool ok = H.size() == 64;
+        for (int i = 0; ok && i < 64; ++i) {
+            const or_fe hr = i == 0 ? h0 : O(R[0][i]);
+            const or_fe want = or_add(O(oh[i]), or_mul(O(r), hr));
+            ok = H[i].real == want.real && H[i].img == want.img;
+        }
+        CHECK(ok, "quadratic_coSumcheck_phase1 d=32 with the R mask");
+        vector<F> L(32);
+        or_compute_lagrange_coeff(O(r), 32, reinterpret_cast<or_fe *>(L.data()));
+        vector<F> c1 = p1, c2 = p2, e1 = p1, e2 = p2;
+        D::quadratic_coSumcheck_phase2(c1, c2, L, 32);
+        or_pcs_sumcheck_phase2(reinterpret_cast<or_fe *>(e1.data()), reinterpret_cast<or_fe *>(e2.data()), M, 32, O(r));
+        CHECK(c1.size() == M / 32 && same(c1.data(), e1.data(), 16 * (M / 32)) && same(c2.data(), e2.data(), 16 * (M / 32)),
+              "quadratic_coSumcheck_phase2 d=32");
+    }
+    // RS_multifold5
+    {
+        vector<F> cw = synth(seed, 40, 1 << 14), a = synth(seed, 41, 32), ref(cw.size() / 32);
+        or_rs_multifold5(reinterpret_cast<or_fe *>(cw.data()), cw.size(), reinterpret_cast<or_fe *>(a.data()), 32,
+                         reinterpret_cast<or_fe *>(ref.data()));
+        D::RS_multifold5(cw, a, 32);
+        CHECK(cw.size() == ref.size() && same(cw.data(), ref.data(), 16 * ref.size()), "RS_multifold5 fd=32");
+    }
+    // errors where the reference exits
+    {
+        bool threw = false;
+        try {
+            vector<F> v1 = synth(seed, 50, 64), v2 = synth(seed, 51, 64);
+            D::_quadratic_sumcheck(F{1, 2}, v1, v2, 1);
+        } catch (const D::error &e) {
+            threw = e.code == COSNARK_E_CHECK;
+        }
+        CHECK(threw, "_quadratic_sumcheck with a wrong claim throws");
+    }
+    std::printf("%s (%d failures)\n", failures ? "FAILED" : "PASSED", failures);
+    return failures ? 1 : 0;
+}
