@@ -175,6 +175,15 @@ int cosnark_whir_open(cosnark_ctx *ctx, const uint64_t *d_arr, uint64_t n, const
  * powers of two; rate*L <= 2^28. */
 int cosnark_copcs_encode(cosnark_ctx *ctx, const uint64_t *d_rows, int nrows, uint64_t L, int rate,
                          int log_rate_offset, uint64_t *d_codewords);
+/* encode_concatenated, src/coPCS.cpp:653-677: d_cprime = (data || 0) of
+ * rate0*n elements, forward NTT (RS encode); with S = rate0*n/degree,
+ * segment j < S takes cprime[j + k*S] for k < degree, zero-pads to
+ * degree*rate1 and takes the forward NTT: row j of d_C (S rows of
+ * degree*rate1 elements, row-major; commit_concatenated, :679-689, hashes
+ * d_C as one vector with cosnark_merkle_commit).  n, rate0, rate1, degree
+ * powers of two, degree <= rate0*n <= 2^28, 2 <= degree*rate1 <= 4096. */
+int cosnark_encode_concatenated(cosnark_ctx *ctx, const uint64_t *d_data, uint64_t n, int rate0, int rate1,
+                                int degree, uint64_t *d_cprime, uint64_t *d_C);
 /* distributed_MT at one rank, src/coPCS.cpp:152-183: the codeword matrix has
  * 32 rows (BATCH_FACTOR, src/New_PCS.cpp:9) of Lc elements; d_cw holds the
  * first nrows (row-major), the rest are zero.  Column j is committed with

@@ -358,6 +358,40 @@ void distributed_MT(std::vector<std::vector<F>> &Codeword, MTT &Com, int N) {
     }
 }
 
+// encode_concatenated(Commit_Data, C_prime, C, rate, degree),
+// src/coPCS.cpp:653-677: C_prime = RS encode of Commit_Data at rate[0];
+// C[j] = the rate[1]-blowup encoding of segment j (C_prime[j + k S], k < degree)
+template <class F>
+void encode_concatenated(std::vector<F> &Commit_Data, std::vector<F> &C_prime, std::vector<std::vector<F>> &C,
+                         std::vector<int> rate, int degree) {
+    detail::elem_type<F>();
+    const uint64_t n = Commit_Data.size(), m = n * (uint64_t)rate.at(0);
+    const uint64_t S = m / (uint64_t)degree, DP = (uint64_t)degree * (uint64_t)rate.at(1);
+    detail::Dev d(Commit_Data.data(), 16 * n), cp(16 * m), c(16 * S * DP);
+    detail::check(cosnark_encode_concatenated(detail::ctx(), d.u64(), n, rate[0], rate[1], degree, cp.u64(), c.u64()),
+                  "encode_concatenated");
+    C_prime.resize(m);
+    cp.get(C_prime.data(), 16 * m);
+    C.resize(S);
+    for (uint64_t j = 0; j < S; ++j) {
+        C[j].resize(DP);
+        detail::check(cosnark_d2h(detail::ctx(), C[j].data(), c.u8() + 16 * DP * j, 16 * DP), "copy out");
+    }
+}
+
+// commit_concatenated(C, Com, N), src/coPCS.cpp:679-689 (one rank):
+// MT_commit_Blake over the rows of C back to back; at N = 1 Root_MT is its root
+template <class F, class MTT>
+void commit_concatenated(std::vector<std::vector<F>> &C, MTT &Com, int N) {
+    detail::elem_type<F>();
+    std::vector<F> leaves;
+    for (auto &row : C) leaves.insert(leaves.end(), row.begin(), row.end());
+    Com.Base_MT.clear();
+    Com.Root_MT.clear();
+    MT_commit_Blake(leaves.data(), Com.Base_MT, (int)leaves.size());
+    if (N == 1) Com.Root_MT.assign(1, {Com.Base_MT.back()[0]});
+}
+
 // pcs_sumcheck_phase1(Commit_data, B, degree, N), src/coPCS.cpp:547-569:
 // this rank's 2*degree-element polynomial (pcs_sumcheck_aggr, the MPI
 // aggregation, is the caller's).

@@ -810,3 +810,17 @@ void or_rs_multifold5(const or_fe *cw, uint64_t n0, const or_fe *a, int fd, or_f
     for (uint64_t i = 0; i < np; ++i) free(part[i]);
     free(part);
 }
+
+void or_encode_concatenated(const or_fe *data, uint64_t n, int rate0, int rate1, int degree, or_fe *cprime, or_fe *C) {
+    const uint64_t m = n * (uint64_t)rate0;              /* coPCS.cpp:658-668 */
+    or_rs_encode(data, n, rate0, cprime);
+    const uint64_t S = m / (uint64_t)degree, D = (uint64_t)degree * (uint64_t)rate1;
+    or_fe *seg = (or_fe *)malloc(sizeof(or_fe) * D);
+    for (uint64_t j = 0; j < S; ++j) {                   /* :669-676 */
+        memset(seg, 0, sizeof(or_fe) * D);
+        for (int k = 0; k < degree; ++k) seg[k] = cprime[j + (uint64_t)k * S];
+        or_fft(seg, ilog2u(D), 0);
+        memcpy(C + j * D, seg, sizeof(or_fe) * D);
+    }
+    free(seg);
+}

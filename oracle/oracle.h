@@ -126,6 +126,11 @@ void or_pcs_sumcheck_phase1(const or_fe *data, const or_fe *B, uint64_t M, int d
 void or_pcs_sumcheck_phase2(or_fe *data, or_fe *B, uint64_t M, int d, or_fe r);
 /* RS_multifold5 (src/coPCS_utils.cpp:386-437): out has n/fd elements. */
 void or_rs_multifold5(const or_fe *cw, uint64_t n, const or_fe *a, int fd, or_fe *out);
+/* encode_concatenated (src/coPCS.cpp:653-677): cprime = (data || 0) of
+ * length rate0*n, forward NTT; with S = rate0*n/degree, segment j < S takes
+ * cprime[j + k*S] for k < degree, zero-pads to degree*rate1 and takes the
+ * forward NTT.  C: S rows of degree*rate1 elements, row-major. */
+void or_encode_concatenated(const or_fe *data, uint64_t n, int rate0, int rate1, int degree, or_fe *cprime, or_fe *C);
 
 #ifdef __cplusplus
 }

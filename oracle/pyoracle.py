@@ -97,6 +97,7 @@ def oracle():
         L.or_pcs_sumcheck_phase1.argtypes = [vp, vp, u64, i32, vp]
         L.or_pcs_sumcheck_phase2.argtypes = [vp, vp, u64, i32, _Fe]
         L.or_rs_multifold5.argtypes = [vp, u64, vp, i32, vp]
+        L.or_encode_concatenated.argtypes = [vp, u64, i32, i32, i32, vp, vp]
         _o = L
     return _o
 
@@ -340,3 +341,15 @@ def lagrange_coeff(r, d):
     L = fe_array(d)
     oracle().or_compute_lagrange_coeff(fe(r), d, _p(L))
     return L
+
+
+def encode_concatenated(data, rate0, rate1, degree):
+    """Returns (cprime (rate0*n, 2), C (S, degree*rate1, 2))."""
+    d = np.ascontiguousarray(data, dtype=np.uint64)
+    n = d.shape[0]
+    m = n * rate0
+    S, D = m // degree, degree * rate1
+    cp = fe_array(m)
+    C = np.zeros((S, D, 2), np.uint64)
+    oracle().or_encode_concatenated(_p(d), n, rate0, rate1, degree, _p(cp), _p(C))
+    return cp, C

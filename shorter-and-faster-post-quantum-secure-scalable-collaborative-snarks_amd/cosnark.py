@@ -72,6 +72,7 @@ def lib():
         L.cosnark_whir_open.argtypes = [vp, vp, u64, vp, vp, i32, i32, vp, vp, vp, u64, ctypes.POINTER(u64)]
         L.cosnark_copcs_encode.argtypes = [vp, vp, i32, u64, i32, i32, vp]
         L.cosnark_distributed_mt.argtypes = [vp, vp, i32, u64, vp]
+        L.cosnark_encode_concatenated.argtypes = [vp, vp, u64, i32, i32, i32, vp, vp]
         L.cosnark_cosumcheck_phase1.argtypes = [vp, vp, vp, u64, i32, vp]
         L.cosnark_cosumcheck_phase2.argtypes = [vp, vp, vp, u64, i32, vp]
         L.cosnark_cosumcheck_fold.argtypes = [vp, vp, vp, u64, i32, vp]
@@ -315,6 +316,19 @@ class Context:
         res = self.download(out, (nrows, L * rate, 2))
         d.free(); out.free()
         return res
+
+    def encode_concatenated(self, data, rate0, rate1, degree):
+        """encode_concatenated (src/coPCS.cpp:653-677): (cprime, C (S, degree*rate1, 2))."""
+        d = np.ascontiguousarray(data, dtype=np.uint64)
+        n = d.shape[0]
+        m = n * rate0
+        S, D = m // degree, degree * rate1
+        a = self.upload(d)
+        cp, C = self.alloc(16 * m), self.alloc(16 * S * D)
+        self._ck(lib().cosnark_encode_concatenated(self.h, a.ptr, n, rate0, rate1, degree, cp.ptr, C.ptr))
+        r = self.download(cp, (m, 2)), self.download(C, (S, D, 2))
+        a.free(); cp.free(); C.free()
+        return r
 
     def distributed_mt(self, cw):
         """distributed_MT (src/coPCS.cpp:152-183) of cw (nrows, Lc, 2): Base_MT digests (2Lc-1, 32)."""

@@ -33,6 +33,7 @@
 //     part[i][k] = w_n^(-ik) X_k,   X = inverse DFT_fd(x) (unscaled),
 // and out[i] = (1/fd) sum_k a_k part[i][k]: one fd-point transform per
 // output and a Horner pass in w_n^-i.
+#include <algorithm>
 #include <cstring>
 #include <vector>
 #include "ctx.h"
@@ -359,6 +360,45 @@ __global__ __launch_bounds__(CO_THREADS) void multifold_large_kernel(const fe *_
     }
 }
 
+// ---- encode_concatenated: segment transforms --------------------------------
+// Segment j of cprime viewed as degree x S (column j: cprime[j + k S]),
+// zero-padded to DP = degree * rate1 points, forward DFT, stored as row j of
+// C.  A workgroup takes G consecutive segments (G * DP <= 4096): loads run
+// over j (coalesced), the transform runs as radix-2 decimation-in-time
+// stages over bit-reversed LDS positions, and the G rows leave as one
+// contiguous run.
+__global__ __launch_bounds__(CO_THREADS) void seg_ntt_kernel(const fe *__restrict__ cp, uint64_t S, int degree,
+                                                             int logdp, int G, const fe *twlo, const fe *twhi,
+                                                             fe *__restrict__ C) {
+    extern __shared__ fe X[];
+    const int DP = 1 << logdp, t = threadIdx.x;
+    const uint64_t ngroups = S / (uint64_t)G;
+    for (uint64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+        const uint64_t j0 = grp * G;
+        for (int e = t; e < G * DP; e += CO_THREADS) {
+            const int g = e % G, k = e / G;
+            fe v = gf::zero();
+            if (k < degree) v = cp[j0 + g + (uint64_t)k * S];
+            X[g * DP + (int)brev_bits((uint32_t)k, logdp)] = v;
+        }
+        __syncthreads();
+        for (int lh = 0; lh < logdp; ++lh) {
+            const int h = 1 << lh;
+            for (int p = t; p < G * DP / 2; p += CO_THREADS) {
+                const int g = p >> (logdp - 1), q = p & (DP / 2 - 1);
+                const int off = q & (h - 1), i = g * DP + ((q >> lh) << (lh + 1)) + off;
+                const fe w = dft::tw28(twlo, twhi, (uint32_t)off << (NTT_LOG_MAX - lh - 1));
+                const fe u = X[i], v = gf::mul(X[i + h], w);
+                X[i] = gf::add(u, v);
+                X[i + h] = gf::sub(u, v);
+            }
+            __syncthreads();
+        }
+        for (int e = t; e < G * DP; e += CO_THREADS) C[j0 * DP + e] = X[e];
+        __syncthreads();
+    }
+}
+
 // ---- host helpers -------------------------------------------------------------
 
 // the reference's fft (src/utils.cpp:904-964) on the host, for d-element
@@ -420,6 +460,9 @@ int copcs_init(cosnark_ctx *) {
     const int small_lds = (int)(sizeof(fe) * BSC_TB * (2 * BSC_SMALL_MAX + 1));
     const int large_lds = (int)(sizeof(fe) * 2 * BSC_D_MAX);
     const int mf_lds = (int)(sizeof(fe) * (MF_FD_MAX + CO_THREADS));
+    if (hipFuncSetAttribute((const void *)seg_ntt_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)(sizeof(fe) * 4096)) != hipSuccess)
+        return COSNARK_E_HIP;
     if (hipFuncSetAttribute((const void *)bsc_small_kernel<32>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             small_lds) != hipSuccess ||
         hipFuncSetAttribute((const void *)bsc_large_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -455,6 +498,33 @@ int cosnark_copcs_encode(cosnark_ctx *ctx, const uint64_t *d_rows, int nrows, ui
         rc = ntt_launch(ctx, tmp, L, (fe *)d_codewords + (uint64_t)i * M, logM, false, ctx->stream);
         if (rc) return rc;
     }
+    return cosnark_sync(ctx);
+}
+
+int cosnark_encode_concatenated(cosnark_ctx *ctx, const uint64_t *d_data, uint64_t n, int rate0, int rate1,
+                                int degree, uint64_t *d_cprime, uint64_t *d_C) {
+    if (!ctx || !d_data || !d_cprime || !d_C) return set_err(ctx, COSNARK_E_ARG, "encode_concatenated: null argument");
+    const uint64_t m = n * (uint64_t)rate0, DP = (uint64_t)degree * (uint64_t)rate1;
+    if (!is_pow2(n) || rate0 < 1 || !is_pow2((uint64_t)rate0) || rate1 < 1 || !is_pow2((uint64_t)rate1) ||
+        degree < 1 || !is_pow2((uint64_t)degree) || (uint64_t)degree > m || DP < 2)
+        return set_err(ctx, COSNARK_E_ARG, "encode_concatenated: n, rates, degree powers of two, degree <= rate0*n, "
+                                           "degree*rate1 >= 2");
+    if (m > (1ull << NTT_LOG_MAX) || DP > 4096)
+        return set_err(ctx, COSNARK_E_RANGE, "encode_concatenated: rate0*n > 2^28 or degree*rate1 > 4096");
+    hipStream_t s = ctx->stream;
+    if (m == 1) {
+        HIPCK(hipMemcpyAsync(d_cprime, d_data, sizeof(fe), hipMemcpyDeviceToDevice, s));
+    } else {
+        int rc = ntt_launch(ctx, (const fe *)d_data, n, (fe *)d_cprime, ilog2(m), false, s);
+        if (rc) return rc;
+    }
+    const uint64_t S = m / degree;
+    const uint64_t G = std::min<uint64_t>(S, 4096 / DP);
+    const uint64_t groups = S / G;
+    hipLaunchKernelGGL(seg_ntt_kernel, dim3((unsigned)(groups < 4096 ? groups : 4096)), dim3(CO_THREADS),
+                       sizeof(fe) * G * DP, s, (const fe *)d_cprime, S, degree, ilog2(DP), (int)G,
+                       (const fe *)ctx->tw_lo, (const fe *)ctx->tw_hi, (fe *)d_C);
+    LAUNCHCK();
     return cosnark_sync(ctx);
 }
 

@@ -157,6 +157,24 @@ int main() {
         ok = ok && same(Com.Root_MT[0][0].arr, Com.Base_MT.back()[0].arr, 32);
         CHECK(ok, "distributed_MT Base_MT and Root_MT");
     }
+    // encode_concatenated + commit_concatenated
+    {
+        vector<F> data = synth(seed, 20, 1 << 10), cprime;
+        vector<vector<F>> C;
+        D::encode_concatenated(data, cprime, C, {4, 4}, 8);
+        vector<F> ocp(4 << 10), oC((4 << 10) / 8 * 32);
+        or_encode_concatenated(reinterpret_cast<or_fe *>(data.data()), 1 << 10, 4, 4, 8,
+                               reinterpret_cast<or_fe *>(ocp.data()), reinterpret_cast<or_fe *>(oC.data()));
+        bool ok = cprime.size() == ocp.size() && same(cprime.data(), ocp.data(), 16 * ocp.size()) && C.size() == 512;
+        for (size_t j = 0; ok && j < C.size(); ++j) ok = C[j].size() == 32 && same(C[j].data(), &oC[32 * j], 16 * 32);
+        CHECK(ok, "encode_concatenated 2^10, rates {4,4}, degree 8");
+        MT Com;
+        D::commit_concatenated(C, Com, 1);
+        vector<uint8_t> nodes(32 * or_merkle_node_count(oC.size()));
+        or_merkle_commit(reinterpret_cast<or_fe *>(oC.data()), oC.size(), nodes.data(), 1);
+        CHECK(Com.Root_MT.size() == 1 && same(Com.Root_MT[0][0].arr, nodes.data() + nodes.size() - 32, 32),
+              "commit_concatenated root");
+    }
     // block sumcheck: pcs phases, coSumcheck phases with the mask
     {
         const uint64_t M = 1 << 12;

@@ -159,3 +159,17 @@ def test_copcs_argument_errors(ctx):
     assert lib.cosnark_distributed_mt(ctx.h, buf.ptr, 2, 3, out.ptr) == cosnark.E_ARG
     assert lib.cosnark_copcs_encode(ctx.h, buf.ptr, 1, 1, 2, 0, out.ptr) == cosnark.E_ARG
     buf.free(); out.free()
+
+
+@pytest.mark.parametrize("logn,rate0,rate1,degree", [(10, 4, 4, 8), (12, 4, 4, 32), (6, 2, 2, 2), (8, 4, 8, 512),
+                                                     (3, 1, 2, 1), (18, 4, 4, 8)])
+def test_encode_concatenated_matches_oracle(ctx, logn, rate0, rate1, degree):
+    print(f"encode_concatenated logn={logn} rates=({rate0},{rate1}) degree={degree}", flush=True)
+    data = O.synth(0xC05A0018 + logn, degree, 1 << logn)
+    gcp, gC = ctx.encode_concatenated(data, rate0, rate1, degree)
+    ocp, oC = O.encode_concatenated(data, rate0, rate1, degree)
+    assert np.array_equal(gcp, ocp) and np.array_equal(gC, oC)
+    if logn == 10:
+        # commit_concatenated: the tree over the rows back to back
+        nodes = ctx.merkle_commit(gC.reshape(-1, 2), True)
+        assert np.array_equal(nodes, O.merkle_commit(oC.reshape(-1, 2), True))

@@ -191,3 +191,19 @@ def test_multifold5_of_a_codeword_is_the_folded_codeword():
     for k in range(fd):
         g = O.vec_op("add", g, O.vec_op("mul", f[k::fd], np.tile(a[k], (m // fd, 1))))
     assert np.array_equal(out, O.rs_encode(g, rate))
+
+
+@pytest.mark.parametrize("logn,rate0,rate1,degree", [(6, 4, 4, 8), (5, 2, 4, 32), (4, 4, 2, 2), (7, 4, 4, 32)])
+def test_encode_concatenated_segments_are_codewords(logn, rate0, rate1, degree):
+    n = 1 << logn
+    data = O.synth(1300 + logn, degree, n)
+    cp, C = O.encode_concatenated(data, rate0, rate1, degree)
+    assert np.array_equal(cp, O.rs_encode(data, rate0))
+    S, D = n * rate0 // degree, degree * rate1
+    assert C.shape == (S, D, 2)
+    # row j evaluates sum_k cp[j + k S] x^k at the D-th roots of unity
+    wD = root(D.bit_length() - 1)
+    for j in {0, 1, S - 1}:
+        coeffs = cp[j::S][:degree]
+        for t in {0, 1, D // 2, D - 1}:
+            assert tup(C[j, t]) == horner(coeffs, fpow(wD, t))
