@@ -102,6 +102,10 @@ int cosnark_create(int device, cosnark_ctx **out) {
     }
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return fail(COSNARK_E_HIP);
     if (hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) != hipSuccess) return fail(COSNARK_E_HIP);
+    if (const char *e = getenv("COSNARK_SC_BLOCKS")) {
+        const int b = atoi(e);
+        if (b >= 1 && b <= SC_MAX_BLOCKS) ctx->sc_blocks = b;
+    }
     for (auto &ev : ctx->ev)
         if (hipEventCreate(&ev) != hipSuccess) return fail(COSNARK_E_HIP);
 
@@ -423,19 +427,22 @@ int cosnark_prove_plain(cosnark_ctx *ctx, const uint64_t *d_f, const uint64_t *d
     fe *scb = (fe *)((char *)base + off_sc);
     rc = sc_stage_state(ctx, gf::mk(t->prev[0], t->prev[1]), gf::zero(), 1, m, m, ctx->stream);
     if (rc) return rc;
-    // commit (encode + Merkle) on stream 1, the sumcheck beside it on stream 2:
-    // the transcript never absorbs the root, so the two are independent
-    hipStream_t s2 = ctx->overlap ? ctx->stream2 : ctx->stream;
-    HIPCK(hipEventRecord(ctx->ev[0], ctx->stream));
-    if ((rc = ntt_launch(ctx, (const fe *)d_f, n, cw, ilog2(cw_n), false, ctx->stream))) return rc;
-    HIPCK(hipEventRecord(ctx->ev[1], ctx->stream));
-    HIPCK(hipStreamWaitEvent(s2, ctx->ev[0], 0));
+    // commit (encode + Merkle) on s1, the sumcheck beside it on s2: the
+    // transcript never absorbs the root, so the two are independent
+    hipStream_t s1 = ctx->stream, s2 = ctx->overlap ? ctx->stream2 : ctx->stream;
+    HIPCK(hipEventRecord(ctx->ev[7], ctx->stream));
+    if (s1 != ctx->stream) HIPCK(hipStreamWaitEvent(s1, ctx->ev[7], 0));
+    HIPCK(hipEventRecord(ctx->ev[0], s1));
+    if ((rc = ntt_launch(ctx, (const fe *)d_f, n, cw, ilog2(cw_n), false, s1))) return rc;
+    HIPCK(hipEventRecord(ctx->ev[1], s1));
+    HIPCK(hipStreamWaitEvent(s2, ctx->ev[7], 0));
     HIPCK(hipEventRecord(ctx->ev[4], s2));
     if ((rc = sumcheck_launch(ctx, (const fe *)d_f, (const fe *)d_g, n, m, scb, s2))) return rc;
     HIPCK(hipEventRecord(ctx->ev[3], s2));
-    HIPCK(hipEventRecord(ctx->ev[6], ctx->stream));
-    if ((rc = merkle_launch(ctx, cw, cw_n, nd, true, ctx->stream))) return rc;
-    HIPCK(hipEventRecord(ctx->ev[2], ctx->stream));
+    HIPCK(hipEventRecord(ctx->ev[6], s1));
+    if ((rc = merkle_launch(ctx, cw, cw_n, nd, true, s1))) return rc;
+    HIPCK(hipEventRecord(ctx->ev[2], s1));
+    HIPCK(hipStreamWaitEvent(ctx->stream, ctx->ev[2], 0));
     HIPCK(hipStreamWaitEvent(ctx->stream, ctx->ev[3], 0));
     HIPCK(hipEventRecord(ctx->ev[5], ctx->stream));
     HIPCK(hipMemcpyAsync(ctx->sc->root, nd + 32 * (nodes - 1), 32, hipMemcpyDeviceToDevice, ctx->stream));

@@ -54,10 +54,11 @@ struct cosnark_ctx {
     // per-size plan (0 = the defaults above); set by the autotuner
     struct { int passbits, tile; } ntt_plan[NTT_LOG_MAX + 1] = {};
     bool overlap = true;              // plain prover: sumcheck beside the commit
+    int sc_blocks = 256;              // workgroups of a multi-workgroup sumcheck round (<= SC_MAX_BLOCKS)
 };
 
 constexpr int CO_HOST_ELEMS = 8192;     // pinned staging for block-sumcheck results and weights
-constexpr int SC_MAX_BLOCKS = 256;     // workgroups per multi-workgroup sumcheck round
+constexpr int SC_MAX_BLOCKS = 2048;    // partial slots per round: the most workgroups a sumcheck round may use
 
 int set_err(cosnark_ctx *ctx, int code, const std::string &msg);
 int hip_err(cosnark_ctx *ctx, hipError_t e, const char *what);

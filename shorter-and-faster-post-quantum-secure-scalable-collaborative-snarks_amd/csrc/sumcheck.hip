@@ -362,7 +362,7 @@ int sumcheck_round_launch(cosnark_ctx *ctx, const fe *s1, const fe *s2, fe *d1, 
                           bool fold, int fs_rounds, int nb_prev, hipStream_t stream, int *nb_out) {
     fe *part[2] = {ctx->sc_partials, ctx->sc_partials + 3 * SC_MAX_BLOCKS};
     uint64_t blocks = (npairs + SC_THREADS * SC_ITEMS - 1) / (SC_THREADS * SC_ITEMS);
-    if (blocks > (uint64_t)SC_MAX_BLOCKS) blocks = SC_MAX_BLOCKS;
+    if (blocks > (uint64_t)ctx->sc_blocks) blocks = ctx->sc_blocks;
     if (blocks == 0) blocks = 1;
     ScJob J{};
     J.s1 = s1; J.s2 = s2;
@@ -388,7 +388,7 @@ int sumcheck_fold_launch(cosnark_ctx *ctx, const fe *s1, const fe *s2, fe *d1, f
                          int fs_rounds, int nb_prev, hipStream_t stream) {
     fe *part[2] = {ctx->sc_partials, ctx->sc_partials + 3 * SC_MAX_BLOCKS};
     uint64_t blocks = (nout + SC_THREADS - 1) / SC_THREADS;
-    if (blocks > (uint64_t)SC_MAX_BLOCKS) blocks = SC_MAX_BLOCKS;
+    if (blocks > (uint64_t)ctx->sc_blocks) blocks = ctx->sc_blocks;
     hipLaunchKernelGGL(sc_fold_kernel, dim3((unsigned)blocks), dim3(SC_THREADS), 0, stream, s1, s2, d1, d2, nout,
                        round, fs_rounds, ctx->sc, (const fe *)part[round & 1], nb_prev);
     LAUNCHCK();
@@ -402,7 +402,7 @@ namespace {
 // buf: n + n/2 elements (ping n/2 per vector, pong n/4 per vector); part:
 // two slots of 3 * SC_MAX_BLOCKS partials.
 std::vector<ScJob> sumcheck_jobs(const fe *v1, const fe *v2, uint64_t n, int fs_rounds, fe *buf, fe *part0,
-                                 ScState *st) {
+                                 ScState *st, int max_blocks) {
     const int m = ilog2(n);
     fe *a1 = buf, *a2 = buf + n / 2, *b1 = buf + n, *b2 = buf + n + n / 4;
     fe *part[2] = {part0, part0 + 3 * SC_MAX_BLOCKS};
@@ -416,7 +416,7 @@ std::vector<ScJob> sumcheck_jobs(const fe *v1, const fe *v2, uint64_t n, int fs_
         const uint64_t after = pending ? len / 2 : len;
         const uint64_t npairs = after / 2;
         uint64_t blocks = (npairs + SC_THREADS * SC_ITEMS - 1) / (SC_THREADS * SC_ITEMS);
-        if (blocks > (uint64_t)SC_MAX_BLOCKS) blocks = SC_MAX_BLOCKS;
+        if (blocks > (uint64_t)max_blocks) blocks = max_blocks;
         ScJob J{};
         J.s1 = s1; J.s2 = s2;
         J.npairs = npairs;
@@ -456,7 +456,7 @@ std::vector<ScJob> sumcheck_jobs(const fe *v1, const fe *v2, uint64_t n, int fs_
 
 int sumcheck_launch(cosnark_ctx *ctx, const fe *v1, const fe *v2, uint64_t n, int fs_rounds, fe *buf,
                     hipStream_t stream) {
-    const auto jobs = sumcheck_jobs(v1, v2, n, fs_rounds, buf, ctx->sc_partials, ctx->sc);
+    const auto jobs = sumcheck_jobs(v1, v2, n, fs_rounds, buf, ctx->sc_partials, ctx->sc, ctx->sc_blocks);
     for (size_t i = 0; i + 1 < jobs.size(); ++i) {
         const ScJob &J = jobs[i];
         if (J.round > 0)
@@ -480,7 +480,7 @@ int sumcheck_batch_plan(cosnark_ctx *ctx, int count, const fe *const *v1, const 
     size_t nr = 0;
     for (int i = 0; i < count; ++i) {
         const int m = ilog2(n[i]);
-        per[i] = sumcheck_jobs(v1[i], v2[i], n[i], m, buf[i], parts[i], st[i]);
+        per[i] = sumcheck_jobs(v1[i], v2[i], n[i], m, buf[i], parts[i], st[i], ctx->sc_blocks);
         nr = std::max(nr, per[i].size() - 1);
     }
     plan.rounds.clear();
