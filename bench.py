@@ -42,6 +42,7 @@ def parse():
     ap.add_argument("--no-whir", action="store_true", help="skip the WHIR (config 3) leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched (config 4) leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving (config 5) leg")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default NTT radix plan")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="serving leg arrival rate (requests/s)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="first run three 256 MiB device-to-device copies (a kernel of known traffic, for "
@@ -208,6 +209,11 @@ def main():
     ctx = cosnark.Context(0)
     if a.serial:
         cosnark.lib().cosnark_set_overlap(ctx.h, 0)
+    log_cw = (n * a.rate).bit_length() - 1
+    t0 = time.perf_counter()
+    if not a.no_tune:
+        ctx.autotune(log_cw, log_cw, 1)        # radix plan of the encode (outside the timed region)
+    tune_s = time.perf_counter() - t0
     df, dg = ctx.upload(f), ctx.upload(g)
     stage = []
     proof = None
@@ -237,7 +243,6 @@ def main():
     # roofline of the dominant stage: algorithmic HBM bytes of the stage
     # (inputs read once, outputs written once) over its launches
     cw_n = n * a.rate
-    log_cw = cw_n.bit_length() - 1
     alg = {
         "rs_encode": 16 * n + 16 * cw_n,                       # read n, write rate*n elements
         "merkle": 16 * cw_n + 32 * (cw_n // 4) + 64 * (cw_n // 4 - 1),
@@ -304,7 +309,9 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by a 256 MiB copy "
                                        "(profiles/pmc_traffic.json)" if traffic else None,
-                     "launches_per_step": launches[dom], "ntt_passbits": passbits,
+                     "launches_per_step": launches[dom], "ntt_plan": {"passbits": passbits,
+                                                                      "tile": ctx.ntt_plan(log_cw)[1],
+                                                                      "autotune_s": round(tune_s, 3)},
                      "alg_bytes_per_launch": int(per_launch_bytes),
                      "avg_launch_ms": round(per_launch_s * 1e3, 5),
                      "valu": {"achieved_fmul_per_s": round(fmul[dom] / (dom_ms / 1e3), 1),
