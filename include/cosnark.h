@@ -175,6 +175,20 @@ int cosnark_whir_open(cosnark_ctx *ctx, const uint64_t *d_arr, uint64_t n, const
  * powers of two; rate*L <= 2^28. */
 int cosnark_copcs_encode(cosnark_ctx *ctx, const uint64_t *d_rows, int nrows, uint64_t L, int rate,
                          int log_rate_offset, uint64_t *d_codewords);
+/* The segments and folds of delegate_checks (src/coPCS.cpp:740-760):
+ * for each query q < l, segment q = d_cw[h_idx[q] + s * (n/fd)], s < fd,
+ * and h_y[q] = RS_multifold_segment(segment q, h_L, n, h_idx[q])
+ * (src/coPCS_utils.cpp:172-230), which equals RS_multifold5's output at
+ * h_idx[q].  h_segments (optional): l x fd elements.  h_idx[q] < n/fd;
+ * fd and n powers of two, 2 <= fd <= min(n, 4096); l <= 65536. */
+int cosnark_multifold_segments(cosnark_ctx *ctx, const uint64_t *d_cw, uint64_t n, const uint64_t *h_L, int fd,
+                               const uint32_t *h_idx, int l, uint64_t *h_segments, uint64_t *h_y);
+/* The batching step of the opening, src/coPCS.cpp:1349-1353:
+ * d_out[k] = sum_{i < nrows} h_c[i] * d_rows[i * len + k] (rows back to
+ * back; the reference combines the first batches - 1 rows of the codeword
+ * and of the committed data this way).  1 <= nrows <= 32. */
+int cosnark_row_combine(cosnark_ctx *ctx, const uint64_t *d_rows, int nrows, uint64_t len, const uint64_t *h_c,
+                        uint64_t *d_out);
 /* encode_concatenated, src/coPCS.cpp:653-677: d_cprime = (data || 0) of
  * rate0*n elements, forward NTT (RS encode); with S = rate0*n/degree,
  * segment j < S takes cprime[j + k*S] for k < degree, zero-pads to

@@ -824,3 +824,51 @@ void or_encode_concatenated(const or_fe *data, uint64_t n, int rate0, int rate1,
     }
     free(seg);
 }
+
+or_fe or_rs_multifold_segment(const or_fe *segment, const or_fe *L, int fd, uint64_t n, uint64_t idx) {
+    /* coPCS_utils.cpp:172-230, with partial codewords of growing length */
+    const int log_fd = ilog2u((uint64_t)fd);
+    uint64_t seg_n = (uint64_t)fd;
+    or_fe len_inv = or_inv(or_from_u64((uint64_t)fd));
+    or_fe w = or_inv(or_root_of_unity(ilog2u(n)));                   /* :177-179 */
+    or_fe omega = or_pow(w, (unsigned __int128)(n / seg_n));
+    or_fe inv_omegas = or_pow(w, (unsigned __int128)idx);           /* :182-184 */
+    or_fe part[64][64];
+    for (uint64_t j = 0; j < seg_n / 2; ++j) {                      /* :187-192 */
+        part[j][0] = or_add(segment[j], segment[j + seg_n / 2]);
+        part[j][1] = or_mul(inv_omegas, or_sub(segment[j], segment[j + seg_n / 2]));
+        inv_omegas = or_mul(inv_omegas, omega);
+    }
+    if (fd > 2) {
+        or_fe pw[32], pwo[32];
+        for (int i = 0; i < log_fd; ++i) { pw[i].real = 1; pw[i].img = 0; pwo[i] = pw[i]; }
+        pw[1] = or_mul(w, w);                                        /* :199-200 */
+        for (int i = 2; i < log_fd; ++i) pw[i] = or_mul(pw[i - 1], pw[i - 1]);
+        for (int i = 1; i < log_fd; ++i) pwo[i] = or_pow(pw[i], (unsigned __int128)idx);
+        for (int i = 1; i < log_fd; ++i) pw[i] = or_pow(pw[i], (unsigned __int128)(n / (uint64_t)fd));
+        for (int i = 1; i < log_fd; ++i) {                           /* :211-225 */
+            seg_n /= 2;
+            const uint64_t half = 1ULL << i;
+            for (uint64_t j = 0; j < seg_n / 2; ++j) {
+                for (uint64_t k = 0; k < half; ++k) {
+                    or_fe v1 = or_add(part[j][k], part[j + seg_n / 2][k]);
+                    or_fe v2 = or_mul(pwo[i], or_sub(part[j][k], part[j + seg_n / 2][k]));
+                    part[j][k] = v1;
+                    part[j][k + half] = v2;
+                }
+                pwo[i] = or_mul(pwo[i], pw[i]);
+            }
+        }
+    }
+    or_fe y = { 0, 0 };
+    for (int i = 0; i < fd; ++i) y = or_add(y, or_mul(L[i], part[0][i]));
+    return or_mul(len_inv, y);
+}
+
+void or_row_combine(const or_fe *rows, int nrows, uint64_t len, const or_fe *c, or_fe *out) {
+    for (uint64_t k = 0; k < len; ++k) {                             /* coPCS.cpp:1349-1353 */
+        or_fe acc = { 0, 0 };
+        for (int i = 0; i < nrows; ++i) acc = or_add(acc, or_mul(c[i], rows[(uint64_t)i * len + k]));
+        out[k] = acc;
+    }
+}

@@ -131,6 +131,12 @@ void or_rs_multifold5(const or_fe *cw, uint64_t n, const or_fe *a, int fd, or_fe
  * cprime[j + k*S] for k < degree, zero-pads to degree*rate1 and takes the
  * forward NTT.  C: S rows of degree*rate1 elements, row-major. */
 void or_encode_concatenated(const or_fe *data, uint64_t n, int rate0, int rate1, int degree, or_fe *cprime, or_fe *C);
+/* RS_multifold_segment (src/coPCS_utils.cpp:172-230): the fold of one
+ * fd-element segment cw[idx + s*(n/fd)], s < fd, by the weights L. */
+or_fe or_rs_multifold_segment(const or_fe *segment, const or_fe *L, int fd, uint64_t n, uint64_t idx);
+/* The batching step of the opening (src/coPCS.cpp:1349-1353):
+ * out[k] = sum_{i < nrows} c[i] * rows[i][k] (rows row-major, len each). */
+void or_row_combine(const or_fe *rows, int nrows, uint64_t len, const or_fe *c, or_fe *out);
 
 #ifdef __cplusplus
 }

@@ -98,6 +98,9 @@ def oracle():
         L.or_pcs_sumcheck_phase2.argtypes = [vp, vp, u64, i32, _Fe]
         L.or_rs_multifold5.argtypes = [vp, u64, vp, i32, vp]
         L.or_encode_concatenated.argtypes = [vp, u64, i32, i32, i32, vp, vp]
+        L.or_rs_multifold_segment.argtypes = [vp, vp, i32, u64, u64]
+        L.or_rs_multifold_segment.restype = _Fe
+        L.or_row_combine.argtypes = [vp, i32, u64, vp, vp]
         _o = L
     return _o
 
@@ -353,3 +356,17 @@ def encode_concatenated(data, rate0, rate1, degree):
     C = np.zeros((S, D, 2), np.uint64)
     oracle().or_encode_concatenated(_p(d), n, rate0, rate1, degree, _p(cp), _p(C))
     return cp, C
+
+
+def rs_multifold_segment(segment, L, n, idx):
+    seg = np.ascontiguousarray(segment, dtype=np.uint64)
+    LL = np.ascontiguousarray(L, dtype=np.uint64)
+    return unfe(oracle().or_rs_multifold_segment(_p(seg), _p(LL), LL.shape[0], n, idx))
+
+
+def row_combine(rows, c):
+    r = np.ascontiguousarray(rows, dtype=np.uint64)
+    cc = np.ascontiguousarray(c, dtype=np.uint64)
+    out = fe_array(r.shape[1])
+    oracle().or_row_combine(_p(r), r.shape[0], r.shape[1], _p(cc), _p(out))
+    return out

@@ -73,6 +73,8 @@ def lib():
         L.cosnark_copcs_encode.argtypes = [vp, vp, i32, u64, i32, i32, vp]
         L.cosnark_distributed_mt.argtypes = [vp, vp, i32, u64, vp]
         L.cosnark_encode_concatenated.argtypes = [vp, vp, u64, i32, i32, i32, vp, vp]
+        L.cosnark_multifold_segments.argtypes = [vp, vp, u64, vp, i32, vp, i32, vp, vp]
+        L.cosnark_row_combine.argtypes = [vp, vp, i32, u64, vp, vp]
         L.cosnark_cosumcheck_phase1.argtypes = [vp, vp, vp, u64, i32, vp]
         L.cosnark_cosumcheck_phase2.argtypes = [vp, vp, vp, u64, i32, vp]
         L.cosnark_cosumcheck_fold.argtypes = [vp, vp, vp, u64, i32, vp]
@@ -329,6 +331,29 @@ class Context:
         r = self.download(cp, (m, 2)), self.download(C, (S, D, 2))
         a.free(); cp.free(); C.free()
         return r
+
+    def multifold_segments(self, cw, L, idx):
+        """delegate_checks' segments and folds: (segments (l, fd, 2), y (l, 2))."""
+        c = np.ascontiguousarray(cw, dtype=np.uint64)
+        LL = np.ascontiguousarray(L, dtype=np.uint64)
+        ii = np.ascontiguousarray(idx, dtype=np.uint32)
+        n, fd, l = c.shape[0], LL.shape[0], ii.shape[0]
+        d = self.upload(c)
+        seg = np.zeros((l, fd, 2), np.uint64)
+        y = np.zeros((l, 2), np.uint64)
+        self._ck(lib().cosnark_multifold_segments(self.h, d.ptr, n, _p(LL), fd, _p(ii), l, _p(seg), _p(y)))
+        d.free()
+        return seg, y
+
+    def row_combine(self, rows, c):
+        r = np.ascontiguousarray(rows, dtype=np.uint64)
+        cc = np.ascontiguousarray(c, dtype=np.uint64)
+        d = self.upload(r)
+        out = self.alloc(16 * r.shape[1])
+        self._ck(lib().cosnark_row_combine(self.h, d.ptr, r.shape[0], r.shape[1], _p(cc), out.ptr))
+        res = self.download(out, (r.shape[1], 2))
+        d.free(); out.free()
+        return res
 
     def distributed_mt(self, cw):
         """distributed_MT (src/coPCS.cpp:152-183) of cw (nrows, Lc, 2): Base_MT digests (2Lc-1, 32)."""

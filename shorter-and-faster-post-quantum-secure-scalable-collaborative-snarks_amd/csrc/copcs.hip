@@ -309,16 +309,25 @@ struct FoldWeights {
     fe a[MF_SMALL_MAX];     // a_k / fd
 };
 
+// Output q is the fold at position i = idx[q] (idx null: i = q, all nout
+// outputs); seg (optional) receives the fd gathered elements of each output.
 template <int FD>
 __global__ __launch_bounds__(CO_THREADS) void multifold_small_kernel(const fe *__restrict__ cw, uint64_t nout,
                                                                      int shift, const fe *__restrict__ itlo,
                                                                      const fe *__restrict__ ithi,
-                                                                     FoldWeights wts, fe *__restrict__ out) {
-    const uint64_t i = (uint64_t)blockIdx.x * CO_THREADS + threadIdx.x;
-    if (i >= nout) return;
+                                                                     FoldWeights wts, fe *__restrict__ out,
+                                                                     const uint32_t *__restrict__ idx, uint64_t nq,
+                                                                     fe *__restrict__ seg) {
+    const uint64_t q = (uint64_t)blockIdx.x * CO_THREADS + threadIdx.x;
+    if (q >= nq) return;
+    const uint64_t i = idx ? (uint64_t)idx[q] : q;
     fe x[FD];
 #pragma unroll
     for (int s = 0; s < FD; ++s) x[s] = cw[i + (uint64_t)s * nout];
+    if (seg) {
+#pragma unroll
+        for (int s = 0; s < FD; ++s) seg[q * FD + s] = x[s];
+    }
     dft::dif<FD, true>(x);                                  // x[j] = X[brev(j)]
     const fe w = dft::tw28(itlo, ithi, (uint32_t)(i << shift));   // w_n^-i
     fe acc = gf::mul(wts.a[FD - 1], x[dft::brev_c<FD>(FD - 1)]);
@@ -326,20 +335,26 @@ __global__ __launch_bounds__(CO_THREADS) void multifold_small_kernel(const fe *_
         constexpr int k = FD - 1 - decltype(kk)::value;
         acc = gf::add(gf::mul(acc, w), gf::mul(wts.a[k], x[dft::brev_c<FD>(k)]));
     });
-    out[i] = acc;
+    out[q] = acc;
 }
 
 // one workgroup per output (grid-stride), the fd-point transform in LDS
 __global__ __launch_bounds__(CO_THREADS) void multifold_large_kernel(const fe *__restrict__ cw, uint64_t nout,
                                                                      int logfd, int logn, const fe *itlo,
                                                                      const fe *ithi, const fe *__restrict__ a,
-                                                                     fe *__restrict__ out) {
+                                                                     fe *__restrict__ out,
+                                                                     const uint32_t *__restrict__ idx, uint64_t nq,
+                                                                     fe *__restrict__ seg) {
     extern __shared__ fe S[];
     const int fd = 1 << logfd, t = threadIdx.x;
     fe *X = S, *R = S + fd;
     fe *const XA[1] = {X};
-    for (uint64_t i = blockIdx.x; i < nout; i += gridDim.x) {
-        for (int s = t; s < fd; s += CO_THREADS) X[s] = cw[i + (uint64_t)s * nout];
+    for (uint64_t q = blockIdx.x; q < nq; q += gridDim.x) {
+        const uint64_t i = idx ? (uint64_t)idx[q] : q;
+        for (int s = t; s < fd; s += CO_THREADS) {
+            X[s] = cw[i + (uint64_t)s * nout];
+            if (seg) seg[q * fd + s] = X[s];
+        }
         __syncthreads();
         lds_dif<1>(XA, logfd, itlo, ithi);            // X[j] = X_brev(j)
         fe acc = gf::zero();
@@ -355,8 +370,23 @@ __global__ __launch_bounds__(CO_THREADS) void multifold_large_kernel(const fe *_
             if (t < h) R[t] = gf::add(R[t], R[t + h]);
             __syncthreads();
         }
-        if (t == 0) out[i] = R[0];
+        if (t == 0) out[q] = R[0];
         __syncthreads();
+    }
+}
+
+// ---- the opening's batching step: out[k] = sum_i c_i rows[i][k] -------------
+struct RowWeights {
+    fe c[32];
+};
+
+__global__ __launch_bounds__(CO_THREADS) void row_combine_kernel(const fe *__restrict__ rows, int nrows, uint64_t len,
+                                                                 RowWeights w, fe *__restrict__ out) {
+    for (uint64_t k = (uint64_t)blockIdx.x * CO_THREADS + threadIdx.x; k < len;
+         k += (uint64_t)gridDim.x * CO_THREADS) {
+        fe acc = gf::zero();
+        for (int i = 0; i < nrows; ++i) acc = gf::add(acc, gf::mul(w.c[i], rows[(uint64_t)i * len + k]));
+        out[k] = acc;
     }
 }
 
@@ -448,10 +478,10 @@ void launch_bsc_small(const fe *p1, const fe *p2, uint64_t nblocks, fe *part, in
 
 template <int FD>
 void launch_mf_small(const fe *cw, uint64_t nout, int shift, const fe *itlo, const fe *ithi,
-                     const FoldWeights &w, fe *out, hipStream_t s) {
-    const unsigned g = (unsigned)((nout + CO_THREADS - 1) / CO_THREADS);
+                     const FoldWeights &w, fe *out, const uint32_t *idx, uint64_t nq, fe *seg, hipStream_t s) {
+    const unsigned g = (unsigned)((nq + CO_THREADS - 1) / CO_THREADS);
     hipLaunchKernelGGL(multifold_small_kernel<FD>, dim3(g), dim3(CO_THREADS), 0, s, cw, nout, shift, itlo,
-                       ithi, w, out);
+                       ithi, w, out, idx, nq, seg);
 }
 
 }  // namespace
@@ -617,16 +647,13 @@ int cosnark_cosumcheck_phase2(cosnark_ctx *ctx, uint64_t *d_p1, uint64_t *d_p2, 
     return cosnark_cosumcheck_fold(ctx, d_p1, d_p2, M, d, (const uint64_t *)L.data());
 }
 
-int cosnark_rs_multifold5(cosnark_ctx *ctx, const uint64_t *d_cw, uint64_t n, const uint64_t *h_a, int fd,
-                          uint64_t *d_out) {
-    if (!ctx || !d_cw || !h_a || !d_out) return set_err(ctx, COSNARK_E_ARG, "rs_multifold5: null argument");
-    if (fd < 2 || !is_pow2((uint64_t)fd) || !is_pow2(n) || n < (uint64_t)fd)
-        return set_err(ctx, COSNARK_E_ARG, "rs_multifold5: fd and n must be powers of two, 2 <= fd <= n");
+}  // extern "C"
+
+// RS_multifold5 outputs at positions idx[0..nq) (idx null: all n/fd), with
+// their gathered segments in seg (optional); everything on ctx->stream.
+static int multifold_launch(cosnark_ctx *ctx, const fe *cw, uint64_t n, const uint64_t *h_a, int fd, fe *out,
+                            const uint32_t *idx, uint64_t nq, fe *seg, fe *a_dev) {
     const int logn = ilog2(n), logfd = ilog2((uint64_t)fd);
-    if (logn > NTT_LOG_MAX || fd > MF_FD_MAX)
-        return set_err(ctx, COSNARK_E_RANGE, "rs_multifold5: n > 2^28 or fd > 4096");
-    if ((const void *)d_cw == (const void *)d_out)
-        return set_err(ctx, COSNARK_E_ARG, "rs_multifold5: output must not alias the codeword");
     const uint64_t nout = n / fd;
     const fe fi = gf::inv(gf::from_u64((uint64_t)fd));
     hipStream_t s = ctx->stream;
@@ -635,28 +662,88 @@ int cosnark_rs_multifold5(cosnark_ctx *ctx, const uint64_t *d_cw, uint64_t n, co
     if (fd <= MF_SMALL_MAX) {
         FoldWeights w{};
         for (int k = 0; k < fd; ++k) w.a[k] = gf::mul(gf::mk(h_a[2 * k], h_a[2 * k + 1]), fi);
-        const fe *cw = (const fe *)d_cw;
-        fe *out = (fe *)d_out;
         switch (fd) {
-            case 2: launch_mf_small<2>(cw, nout, shift, itlo, ithi, w, out, s); break;
-            case 4: launch_mf_small<4>(cw, nout, shift, itlo, ithi, w, out, s); break;
-            case 8: launch_mf_small<8>(cw, nout, shift, itlo, ithi, w, out, s); break;
-            case 16: launch_mf_small<16>(cw, nout, shift, itlo, ithi, w, out, s); break;
-            default: launch_mf_small<32>(cw, nout, shift, itlo, ithi, w, out, s); break;
+            case 2: launch_mf_small<2>(cw, nout, shift, itlo, ithi, w, out, idx, nq, seg, s); break;
+            case 4: launch_mf_small<4>(cw, nout, shift, itlo, ithi, w, out, idx, nq, seg, s); break;
+            case 8: launch_mf_small<8>(cw, nout, shift, itlo, ithi, w, out, idx, nq, seg, s); break;
+            case 16: launch_mf_small<16>(cw, nout, shift, itlo, ithi, w, out, idx, nq, seg, s); break;
+            default: launch_mf_small<32>(cw, nout, shift, itlo, ithi, w, out, idx, nq, seg, s); break;
         }
         LAUNCHCK();
     } else {
+        for (int k = 0; k < fd; ++k) ctx->co_host[k] = gf::mul(gf::mk(h_a[2 * k], h_a[2 * k + 1]), fi);
+        HIPCK(hipMemcpyAsync(a_dev, ctx->co_host, sizeof(fe) * fd, hipMemcpyHostToDevice, s));
+        const unsigned g = (unsigned)(nq < 4096 ? nq : 4096);
+        hipLaunchKernelGGL(multifold_large_kernel, dim3(g), dim3(CO_THREADS), sizeof(fe) * (fd + CO_THREADS), s, cw,
+                           nout, logfd, logn, itlo, ithi, (const fe *)a_dev, out, idx, nq, seg);
+        LAUNCHCK();
+    }
+    return COSNARK_OK;
+}
+
+extern "C" {
+
+int cosnark_rs_multifold5(cosnark_ctx *ctx, const uint64_t *d_cw, uint64_t n, const uint64_t *h_a, int fd,
+                          uint64_t *d_out) {
+    if (!ctx || !d_cw || !h_a || !d_out) return set_err(ctx, COSNARK_E_ARG, "rs_multifold5: null argument");
+    if (fd < 2 || !is_pow2((uint64_t)fd) || !is_pow2(n) || n < (uint64_t)fd)
+        return set_err(ctx, COSNARK_E_ARG, "rs_multifold5: fd and n must be powers of two, 2 <= fd <= n");
+    if (ilog2(n) > NTT_LOG_MAX || fd > MF_FD_MAX)
+        return set_err(ctx, COSNARK_E_RANGE, "rs_multifold5: n > 2^28 or fd > 4096");
+    if ((const void *)d_cw == (const void *)d_out)
+        return set_err(ctx, COSNARK_E_ARG, "rs_multifold5: output must not alias the codeword");
+    fe *ad = nullptr;
+    if (fd > MF_SMALL_MAX) {
         void *base;
         int rc = scratch_reserve(ctx, sizeof(fe) * fd, &base);
         if (rc) return rc;
-        fe *ad = (fe *)base;
-        for (int k = 0; k < fd; ++k) ctx->co_host[k] = gf::mul(gf::mk(h_a[2 * k], h_a[2 * k + 1]), fi);
-        HIPCK(hipMemcpyAsync(ad, ctx->co_host, sizeof(fe) * fd, hipMemcpyHostToDevice, s));
-        const unsigned g = (unsigned)(nout < 4096 ? nout : 4096);
-        hipLaunchKernelGGL(multifold_large_kernel, dim3(g), dim3(CO_THREADS), sizeof(fe) * (fd + CO_THREADS), s,
-                           (const fe *)d_cw, nout, logfd, logn, itlo, ithi, (const fe *)ad, (fe *)d_out);
-        LAUNCHCK();
+        ad = (fe *)base;
     }
+    int rc = multifold_launch(ctx, (const fe *)d_cw, n, h_a, fd, (fe *)d_out, nullptr, n / fd, nullptr, ad);
+    if (rc) return rc;
+    return cosnark_sync(ctx);
+}
+
+int cosnark_multifold_segments(cosnark_ctx *ctx, const uint64_t *d_cw, uint64_t n, const uint64_t *h_L, int fd,
+                               const uint32_t *h_idx, int l, uint64_t *h_segments, uint64_t *h_y) {
+    if (!ctx || !d_cw || !h_L || !h_idx || !h_y) return set_err(ctx, COSNARK_E_ARG, "multifold_segments: null argument");
+    if (fd < 2 || !is_pow2((uint64_t)fd) || !is_pow2(n) || n < (uint64_t)fd || l < 0)
+        return set_err(ctx, COSNARK_E_ARG, "multifold_segments: fd and n must be powers of two, 2 <= fd <= n");
+    if (ilog2(n) > NTT_LOG_MAX || fd > MF_FD_MAX || l > 65536)
+        return set_err(ctx, COSNARK_E_RANGE, "multifold_segments: n > 2^28, fd > 4096 or more than 65536 queries");
+    if (l == 0) return COSNARK_OK;
+    for (int q = 0; q < l; ++q)
+        if ((uint64_t)h_idx[q] >= n / fd) return set_err(ctx, COSNARK_E_ARG, "multifold_segments: index out of range");
+    // device block: indexes, weights (fd > 32), outputs, segments
+    const size_t off_a = align256(sizeof(uint32_t) * l), off_y = off_a + align256(sizeof(fe) * fd);
+    const size_t off_seg = off_y + align256(sizeof(fe) * l);
+    const size_t total = off_seg + (h_segments ? sizeof(fe) * (size_t)l * fd : 0);
+    void *base;
+    int rc = scratch_reserve(ctx, total, &base);
+    if (rc) return rc;
+    uint8_t *b = (uint8_t *)base;
+    hipStream_t s = ctx->stream;
+    HIPCK(hipMemcpyAsync(b, h_idx, sizeof(uint32_t) * l, hipMemcpyHostToDevice, s));
+    HIPCK(hipStreamSynchronize(s));     // h_idx is pageable; the weights below reuse the staging buffer
+    fe *seg = h_segments ? (fe *)(b + off_seg) : nullptr;
+    rc = multifold_launch(ctx, (const fe *)d_cw, n, h_L, fd, (fe *)(b + off_y), (const uint32_t *)b, (uint64_t)l,
+                          seg, (fe *)(b + off_a));
+    if (rc) return rc;
+    HIPCK(hipMemcpyAsync(h_y, b + off_y, sizeof(fe) * l, hipMemcpyDeviceToHost, s));
+    if (h_segments) HIPCK(hipMemcpyAsync(h_segments, seg, sizeof(fe) * (size_t)l * fd, hipMemcpyDeviceToHost, s));
+    return cosnark_sync(ctx);
+}
+
+int cosnark_row_combine(cosnark_ctx *ctx, const uint64_t *d_rows, int nrows, uint64_t len, const uint64_t *h_c,
+                        uint64_t *d_out) {
+    if (!ctx || !d_rows || !h_c || !d_out) return set_err(ctx, COSNARK_E_ARG, "row_combine: null argument");
+    if (nrows < 1 || nrows > 32 || len == 0) return set_err(ctx, COSNARK_E_ARG, "row_combine: 1 <= nrows <= 32, len > 0");
+    RowWeights w{};
+    for (int i = 0; i < nrows; ++i) w.c[i] = gf::mk(h_c[2 * i], h_c[2 * i + 1]);
+    const uint64_t g = (len + CO_THREADS - 1) / CO_THREADS;
+    hipLaunchKernelGGL(row_combine_kernel, dim3((unsigned)(g < 8192 ? g : 8192)), dim3(CO_THREADS), 0, ctx->stream,
+                       (const fe *)d_rows, nrows, len, w, (fe *)d_out);
+    LAUNCHCK();
     return cosnark_sync(ctx);
 }
 

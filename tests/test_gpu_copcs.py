@@ -173,3 +173,29 @@ def test_encode_concatenated_matches_oracle(ctx, logn, rate0, rate1, degree):
         # commit_concatenated: the tree over the rows back to back
         nodes = ctx.merkle_commit(gC.reshape(-1, 2), True)
         assert np.array_equal(nodes, O.merkle_commit(oC.reshape(-1, 2), True))
+
+
+@pytest.mark.parametrize("logn,fd,l", [(10, 8, 50), (20, 32, 333), (16, 64, 100), (14, 128, 40), (12, 2, 7)])
+def test_multifold_segments_match_oracle(ctx, logn, fd, l):
+    """delegate_checks (src/coPCS.cpp:740-760): gathered segments and their
+    RS_multifold_segment folds at query positions."""
+    print(f"multifold_segments logn={logn} fd={fd} l={l}", flush=True)
+    n = 1 << logn
+    cw, L = O.synth(0xC05A0019 + logn, fd, n), O.synth(0xC05A0019 + logn, fd + 1, fd)
+    idx = (O.synth(0xC05A0019 + logn, fd + 2, l)[:, 0] % (n // fd)).astype(np.uint32)
+    seg, y = ctx.multifold_segments(cw, L, idx)
+    for q in range(l):
+        want_seg = cw[int(idx[q])::n // fd][:fd]
+        assert np.array_equal(seg[q], want_seg)
+        if fd <= 64:
+            assert np.array_equal(y[q], O.rs_multifold_segment(want_seg, L, n, int(idx[q])))
+    if n <= 1 << 16:
+        full = O.rs_multifold5(cw, L)
+        assert np.array_equal(y, full[idx.astype(np.int64)])
+
+
+@pytest.mark.parametrize("nrows,logl", [(16, 20), (1, 10), (31, 12)])
+def test_row_combine_matches_oracle(ctx, nrows, logl):
+    rows = np.stack([O.synth(0xC05A001A + logl, i, 1 << logl) for i in range(nrows)])
+    c = O.synth(0xC05A001A, 99, nrows)
+    assert np.array_equal(ctx.row_combine(rows, c), O.row_combine(rows, c))

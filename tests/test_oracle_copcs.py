@@ -207,3 +207,27 @@ def test_encode_concatenated_segments_are_codewords(logn, rate0, rate1, degree):
         coeffs = cp[j::S][:degree]
         for t in {0, 1, D // 2, D - 1}:
             assert tup(C[j, t]) == horner(coeffs, fpow(wD, t))
+
+
+@pytest.mark.parametrize("logn,fd", [(8, 2), (8, 8), (10, 32), (6, 4)])
+def test_multifold_segment_is_the_fold_at_idx(logn, fd):
+    # RS_multifold_segment of the fd-element segment at idx equals RS_multifold5 at idx
+    n = 1 << logn
+    cw = O.synth(1400 + logn, fd, n)
+    L = O.synth(1400 + logn, fd + 1, fd)
+    full = O.rs_multifold5(cw, L)
+    for idx in {0, 1, 5 % (n // fd), n // fd - 1}:
+        seg = cw[idx::n // fd][:fd]
+        assert np.array_equal(O.rs_multifold_segment(seg, L, n, idx), full[idx])
+
+
+def test_row_combine_is_the_weighted_sum():
+    rows = np.stack([O.synth(1500, i, 64) for i in range(5)])
+    c = O.synth(1501, 0, 5)
+    out = O.row_combine(rows, c)
+    want = np.zeros((64, 2), np.uint64)
+    for i in range(5):
+        want = O.vec_op("add", want, O.vec_op("mul", rows[i], np.tile(c[i], (64, 1))))
+    assert np.array_equal(out, want)
+    # one row: the scaled row; unit weights: the plain sum
+    assert np.array_equal(O.row_combine(rows[:1], c[:1]), O.vec_op("mul", rows[0], np.tile(c[0], (64, 1))))
