@@ -226,6 +226,41 @@ int cosnark_cosumcheck_phase2(cosnark_ctx *ctx, uint64_t *d_p1, uint64_t *d_p2, 
  * quadratic_coSumcheck_phase2(p1, p2, L, d), src/coSumcheck.cpp:201-213. */
 int cosnark_cosumcheck_fold(cosnark_ctx *ctx, uint64_t *d_p1, uint64_t *d_p2, uint64_t M, int d,
                             const uint64_t *h_L);
+/* Cubic block sumcheck, phase 1: the block sum of cubic_coSumcheck_phase1
+ * (src/coSumcheck.cpp:9-45) before its r * H_r mask and sumcheck_aggr.
+ * With h1, h2, h3, h4 = blocks of p1, p2, b1, p3 (:17-20), each iNTT_d,
+ * zero-padded to 4d and NTT_4d, h_poly (4d elements) = sum over blocks of
+ * (h1 h2 - h4) h3.  M = |p1| = |p2| = |p3| = |b1|; d and M powers of two,
+ * 2 <= d <= min(M, 1024). */
+int cosnark_cosumcheck3_phase1(cosnark_ctx *ctx, const uint64_t *d_p1, const uint64_t *d_p2, const uint64_t *d_p3,
+                               const uint64_t *d_b1, uint64_t M, int d, uint64_t *h_poly);
+/* Cubic block sumcheck, phase 2: cubic_coSumcheck_phase2(p1, p2, p3, b1, L, d)
+ * (src/coSumcheck.cpp:64-148): p[j] = sum_k L_k p[jd + k] on each of the four
+ * vectors, in place (an all-zero block gives 0, as the reference's skip
+ * does).  2 <= d <= min(M, 8192). */
+int cosnark_cosumcheck3_fold(cosnark_ctx *ctx, uint64_t *d_p1, uint64_t *d_p2, uint64_t *d_p3, uint64_t *d_b1,
+                             uint64_t M, int d, const uint64_t *h_L);
+/* The same with L = compute_lagrange_coeff(., r, d), as cubic_coSumcheck
+ * derives it (src/coSumcheck.cpp:421). */
+int cosnark_cosumcheck3_phase2(cosnark_ctx *ctx, uint64_t *d_p1, uint64_t *d_p2, uint64_t *d_p3, uint64_t *d_b1,
+                               uint64_t M, int d, const uint64_t r[2]);
+
+/* ---- rank 0's share arithmetic (host only, no context) --------------------
+ * unpack_secret_share(secret, _k, k), src/MPI_utils.cpp:2769-2785: the n
+ * shares in h_secret (values at w_2n^(2j+1) of a polynomial of degree < n)
+ * are replaced, as the reference leaves them, by the polynomial's values at
+ * the n-th roots of unity, and h_out[j] = h_secret[n j / kk] for j < kk.
+ * n a power of two <= 8192, 1 <= kk <= n.  Returns COSNARK_E_ARG otherwise. */
+int cosnark_share_unpack(uint64_t *h_secret, int n, int kk, uint64_t *h_out);
+/* Rank 0's combine in sumcheck_aggr(poly, b, N), src/MPI_utils.cpp:2604-2645,
+ * given the gathered polynomials: h_polys holds nranks rows of len elements
+ * (row m = rank m's poly); coordinate i's shares are unpacked
+ * (cosnark_share_unpack with kk = n = nranks) and
+ * h_out[i] = sum_{j < nranks/4} b_j * unpacked[2j].  h_b holds nranks/4
+ * elements (the reference's ones(N/4) or b2); for nranks < 4 the sum is
+ * empty and h_out is zero, as in the reference.  nranks a power of two
+ * <= 8192. */
+int cosnark_sumcheck_aggr(const uint64_t *h_polys, int nranks, int len, const uint64_t *h_b, uint64_t *h_out);
 /* RS_multifold5, src/coPCS_utils.cpp:386-437: folds the n-element codeword
  * d_cw by the fd challenges h_a (fd elements) into n/fd elements at d_out
  * (the reference replaces the codeword; here d_out must not alias d_cw).

@@ -379,6 +379,31 @@ void encode_concatenated(std::vector<F> &Commit_Data, std::vector<F> &C_prime, s
     }
 }
 
+namespace detail {
+// H[i] += r * H_r[i] with H_r = R[iter] and H_r[0] += R[j][stride * k] for
+// j > iter, k < |R[j]| / stride (the coSumcheck masks: stride 2 at
+// src/coSumcheck.cpp:182-193, stride 4 at :48-58)
+template <class F>
+void add_mask(std::vector<F> &H, const std::vector<std::vector<F>> &R, int iter, const F &r, size_t stride,
+              const char *what) {
+    std::vector<F> Hr = R.at(iter);
+    if (Hr.size() < H.size()) throw error(COSNARK_E_ARG, std::string(what) + ": R[iter] shorter than the polynomial");
+    auto w = [](const F &x) { return reinterpret_cast<const uint64_t *>(&x); };
+    uint64_t h0[2] = {w(Hr[0])[0], w(Hr[0])[1]};
+    for (size_t j = iter + 1; j < R.size(); ++j)
+        for (size_t k = 0; k < R[j].size() / stride; ++k) cosnark_field_add(h0, w(R[j][stride * k]), h0);
+    std::memcpy(&Hr[0], h0, 16);
+    uint64_t rr[2];
+    std::memcpy(rr, &r, 16);
+    for (size_t i = 0; i < H.size(); ++i) {
+        uint64_t t[2], h[2];
+        cosnark_field_mul(rr, w(Hr[i]), t);
+        cosnark_field_add(w(H[i]), t, h);
+        std::memcpy(&H[i], h, 16);
+    }
+}
+}  // namespace detail
+
 // commit_concatenated(C, Com, N), src/coPCS.cpp:679-689 (one rank):
 // MT_commit_Blake over the rows of C back to back; at N = 1 Root_MT is its root
 template <class F, class MTT>
@@ -438,23 +463,8 @@ std::vector<F> quadratic_coSumcheck_phase1(std::vector<F> &p1, std::vector<F> &p
     std::vector<F> H(2 * (size_t)d);
     detail::check(cosnark_cosumcheck_phase1(detail::ctx(), a.u64(), b.u64(), M, d, reinterpret_cast<uint64_t *>(H.data())),
                   "quadratic_coSumcheck_phase1");
-    // the mask (src/coSumcheck.cpp:182-193): H_r = R[iter], H_r[0] += the
-    // even entries of R[iter+1..]; H[i] += r * H_r[i]
-    std::vector<F> Hr = R.at(iter);
-    if (Hr.size() < H.size()) throw error(COSNARK_E_ARG, "quadratic_coSumcheck_phase1: R[iter] shorter than 2d");
-    auto w = [](const F &x) { const uint64_t *p = reinterpret_cast<const uint64_t *>(&x); return p; };
-    uint64_t h0[2] = {w(Hr[0])[0], w(Hr[0])[1]};
-    for (size_t j = iter + 1; j < R.size(); ++j)
-        for (size_t k = 0; k < R[j].size() / 2; ++k) cosnark_field_add(h0, w(R[j][2 * k]), h0);
-    std::memcpy(&Hr[0], h0, 16);
-    uint64_t rr[2];
-    std::memcpy(rr, &r, 16);
-    for (size_t i = 0; i < H.size(); ++i) {
-        uint64_t t[2], h[2];
-        cosnark_field_mul(rr, w(Hr[i]), t);
-        cosnark_field_add(w(H[i]), t, h);
-        std::memcpy(&H[i], h, 16);
-    }
+    // the mask (src/coSumcheck.cpp:182-193)
+    detail::add_mask(H, R, iter, r, 2, "quadratic_coSumcheck_phase1");
     return H;
 }
 
@@ -471,6 +481,79 @@ void quadratic_coSumcheck_phase2(std::vector<F> &p1, std::vector<F> &p2, std::ve
     p2.resize(M / d);
     a.get(p1.data(), 16 * (M / d));
     b.get(p2.data(), 16 * (M / d));
+}
+
+// cubic_coSumcheck_phase1(p1, p2, p3, b1, b2, R, r, d, iter, N),
+// src/coSumcheck.cpp:9-62, this rank's part: the cubic block sum (blocks
+// whose p1 part is zero skipped) plus r * H_r with H_r = R[iter] and
+// H_r[0] += the entries 4k of R[iter+1..].  sumcheck_aggr(H, b2, N), the
+// MPI aggregation, is the caller's: rank 0 combines the gathered parts
+// with sumcheck_aggr_gathered.
+template <class F>
+std::vector<F> cubic_coSumcheck_phase1(std::vector<F> &p1, std::vector<F> &p2, std::vector<F> &p3, std::vector<F> &b1,
+                                       std::vector<F> &, std::vector<std::vector<F>> R, F r, int d, int iter, int) {
+    detail::elem_type<F>();
+    const uint64_t M = p1.size();
+    if (p2.size() != M || p3.size() != M || b1.size() != M) throw error(COSNARK_E_ARG, "cubic_coSumcheck_phase1: sizes");
+    detail::Dev a(p1.data(), 16 * M), b(p2.data(), 16 * M), c(p3.data(), 16 * M), e(b1.data(), 16 * M);
+    std::vector<F> H(4 * (size_t)d);
+    detail::check(cosnark_cosumcheck3_phase1(detail::ctx(), a.u64(), b.u64(), c.u64(), e.u64(), M, d,
+                                             reinterpret_cast<uint64_t *>(H.data())),
+                  "cubic_coSumcheck_phase1");
+    detail::add_mask(H, R, iter, r, 4, "cubic_coSumcheck_phase1");
+    return H;
+}
+
+// cubic_coSumcheck_phase2(p1, p2, p3, b1, L, d), src/coSumcheck.cpp:64-148
+template <class F>
+void cubic_coSumcheck_phase2(std::vector<F> &p1, std::vector<F> &p2, std::vector<F> &p3, std::vector<F> &b1,
+                             std::vector<F> L, int d) {
+    detail::elem_type<F>();
+    const uint64_t M = p1.size();
+    if (p2.size() != M || p3.size() != M || b1.size() != M || (int)L.size() < d)
+        throw error(COSNARK_E_ARG, "cubic_coSumcheck_phase2: sizes");
+    detail::Dev a(p1.data(), 16 * M), b(p2.data(), 16 * M), c(p3.data(), 16 * M), e(b1.data(), 16 * M);
+    detail::check(cosnark_cosumcheck3_fold(detail::ctx(), a.u64(), b.u64(), c.u64(), e.u64(), M, d,
+                                           reinterpret_cast<const uint64_t *>(L.data())),
+                  "cubic_coSumcheck_phase2");
+    for (auto *v : {&p1, &p2, &p3, &b1}) v->resize(M / d);
+    a.get(p1.data(), 16 * (M / d));
+    b.get(p2.data(), 16 * (M / d));
+    c.get(p3.data(), 16 * (M / d));
+    e.get(b1.data(), 16 * (M / d));
+}
+
+// unpack_secret_share(secret, _k, k), src/MPI_utils.cpp:2769-2785 (host):
+// secret is left transformed as the reference leaves it
+template <class F>
+std::vector<F> unpack_secret_share(std::vector<F> &secret, int _k, int) {
+    detail::elem_type<F>();
+    std::vector<F> v((size_t)_k);
+    const int rc = cosnark_share_unpack(reinterpret_cast<uint64_t *>(secret.data()), (int)secret.size(), _k,
+                                        reinterpret_cast<uint64_t *>(v.data()));
+    if (rc) throw error(rc, "unpack_secret_share: sizes");
+    return v;
+}
+
+// rank 0's combine in sumcheck_aggr(poly, b, N), src/MPI_utils.cpp:2604-2645,
+// over the polynomials gathered from the N = |P| ranks (P[m] = rank m's)
+template <class F>
+std::vector<F> sumcheck_aggr_gathered(const std::vector<std::vector<F>> &P, const std::vector<F> &b) {
+    detail::elem_type<F>();
+    const int N = (int)P.size();
+    const size_t len = N ? P[0].size() : 0;
+    std::vector<F> flat;
+    for (auto &row : P) {
+        if (row.size() != len) throw error(COSNARK_E_ARG, "sumcheck_aggr: ranks' polynomials differ in size");
+        flat.insert(flat.end(), row.begin(), row.end());
+    }
+    if ((int)b.size() < N / 4) throw error(COSNARK_E_ARG, "sumcheck_aggr: b shorter than N/4");
+    std::vector<F> out(len);
+    const int rc = cosnark_sumcheck_aggr(reinterpret_cast<const uint64_t *>(flat.data()), N, (int)len,
+                                         reinterpret_cast<const uint64_t *>(b.data()),
+                                         reinterpret_cast<uint64_t *>(out.data()));
+    if (rc) throw error(rc, "sumcheck_aggr: sizes");
+    return out;
 }
 
 // RS_multifold5(codeword, a, fold_degree), src/coPCS_utils.cpp:386-437:

@@ -872,3 +872,86 @@ void or_row_combine(const or_fe *rows, int nrows, uint64_t len, const or_fe *c, 
         out[k] = acc;
     }
 }
+
+void or_cosumcheck3_phase1(const or_fe *p1, const or_fe *p2, const or_fe *p3, const or_fe *b1, uint64_t M, int d,
+                           or_fe *H) {
+    int logd = ilog2u((uint64_t)d);
+    const or_fe *src[4] = { p1, p2, b1, p3 };                         /* h1, h2, h3, h4: coSumcheck.cpp:17-20 */
+    or_fe *h[4];
+    for (int v = 0; v < 4; ++v) h[v] = (or_fe *)malloc(sizeof(or_fe) * 4 * d);
+    for (int k = 0; k < 4 * d; ++k) { H[k].real = 0; H[k].img = 0; }  /* :11 */
+    for (uint64_t j = 0; j < M / d; ++j) {
+        int zero = 1;                                                  /* :22-28 */
+        for (int k = 0; k < d; ++k)
+            if (p1[j * d + k].real || p1[j * d + k].img) { zero = 0; break; }
+        if (zero) continue;
+        for (int v = 0; v < 4; ++v) {
+            memcpy(h[v], src[v] + j * d, sizeof(or_fe) * d);
+            or_fft(h[v], logd, 1);                                     /* :29-32 */
+            memset(h[v] + d, 0, sizeof(or_fe) * 3 * d);                /* :33-36 */
+            or_fft(h[v], logd + 2, 0);                                 /* :37-40 */
+        }
+        for (int k = 0; k < 4 * d; ++k)                                /* :41-43 */
+            H[k] = or_add(H[k], or_mul(or_sub(or_mul(h[0][k], h[1][k]), h[3][k]), h[2][k]));
+    }
+    for (int v = 0; v < 4; ++v) free(h[v]);
+}
+
+void or_cosumcheck3_phase2(or_fe *p1, or_fe *p2, or_fe *p3, or_fe *b1, uint64_t M, int d, const or_fe *L) {
+    or_fe *vec[4] = { p1, p2, p3, b1 };                                /* coSumcheck.cpp:66-141 */
+    for (uint64_t j = 0; j < M / d; ++j)
+        for (int v = 0; v < 4; ++v) {
+            or_fe *p = vec[v];
+            int zero = 1;
+            for (int k = 0; k < d; ++k)
+                if (p[j * d + k].real || p[j * d + k].img) { zero = 0; break; }
+            or_fe s = { 0, 0 };
+            if (!zero) {
+                s = or_mul(L[0], p[j * d]);
+                for (int k = 1; k < d; ++k) s = or_add(s, or_mul(L[k], p[j * d + k]));
+            }
+            p[j] = s;
+        }
+}
+
+void or_unpack_secret_share(or_fe *secret, int n, int kk, or_fe *v) {
+    int logn = ilog2u((uint64_t)n);
+    or_fft(secret, logn, 1);                                           /* MPI_utils.cpp:2771 */
+    or_fe omega = or_inv(or_root_of_unity(1 + logn));              /* :2773-2774 */
+    or_fe mul = { 1, 0 };
+    for (int i = 0; i < n; ++i) {                                      /* :2775-2779 */
+        secret[i] = or_mul(mul, secret[i]);
+        mul = or_mul(omega, mul);
+    }
+    or_fft(secret, logn, 0);                                           /* :2781 */
+    for (int j = 0; j < kk; ++j) v[j] = secret[(uint64_t)n * j / kk];  /* :2782-2783 */
+}
+
+void or_compute_secret_shares(const or_fe *v, uint64_t M, int N, int k, int kk, const or_fe *pads, or_fe *shares) {
+    int B = 2 * N > kk ? 2 * N : kk;
+    or_fe *buff = (or_fe *)malloc(sizeof(or_fe) * B);
+    uint64_t counter = 0, groups = M / k;
+    for (uint64_t i = 0; i < groups; ++i) {                            /* MPI_utils.cpp:2748-2766 */
+        for (int j = 0; j < B; ++j) { buff[j].real = 0; buff[j].img = 0; }
+        for (int j = 0; j < k; ++j) buff[j] = v[counter++];
+        for (int j = k; j < kk; ++j)
+            if (pads) buff[j] = pads[i * (uint64_t)(kk - k) + (j - k)];
+        or_fft(buff, ilog2u((uint64_t)kk), 1);                         /* :2760 */
+        for (int j = kk; j < 2 * N; ++j) { buff[j].real = 0; buff[j].img = 0; }   /* :2761 */
+        or_fft(buff, ilog2u((uint64_t)(2 * N)), 0);                    /* :2762 */
+        for (int j = 0; j < N; ++j) shares[(uint64_t)j * groups + i] = buff[2 * j + 1];
+    }
+    free(buff);
+}
+
+void or_sumcheck_aggr(const or_fe *polys, int N, int len, const or_fe *b, or_fe *out) {
+    or_fe *P = (or_fe *)malloc(sizeof(or_fe) * N), *v = (or_fe *)malloc(sizeof(or_fe) * N);
+    for (int i = 0; i < len; ++i) {                                    /* MPI_utils.cpp:2630-2644 */
+        for (int m = 0; m < N; ++m) P[m] = polys[(uint64_t)m * len + i];   /* transpose */
+        or_unpack_secret_share(P, N, N, v);
+        or_fe acc = { 0, 0 };
+        for (int j = 0; j < N / 4; ++j) acc = or_add(acc, or_mul(b[j], v[2 * j]));
+        out[i] = acc;
+    }
+    free(P); free(v);
+}

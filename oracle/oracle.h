@@ -137,6 +137,24 @@ or_fe or_rs_multifold_segment(const or_fe *segment, const or_fe *L, int fd, uint
 /* The batching step of the opening (src/coPCS.cpp:1349-1353):
  * out[k] = sum_{i < nrows} c[i] * rows[i][k] (rows row-major, len each). */
 void or_row_combine(const or_fe *rows, int nrows, uint64_t len, const or_fe *c, or_fe *out);
+/* cubic_coSumcheck_phase1's block sum (src/coSumcheck.cpp:9-45), before the
+ * mask and sumcheck_aggr: H (4d elements). */
+void or_cosumcheck3_phase1(const or_fe *p1, const or_fe *p2, const or_fe *p3, const or_fe *b1, uint64_t M, int d,
+                           or_fe *H);
+/* cubic_coSumcheck_phase2 (src/coSumcheck.cpp:64-148), in place, weights L. */
+void or_cosumcheck3_phase2(or_fe *p1, or_fe *p2, or_fe *p3, or_fe *b1, uint64_t M, int d, const or_fe *L);
+/* unpack_secret_share (src/MPI_utils.cpp:2769-2785): secret (n shares) is
+ * transformed in place as the reference leaves it; v receives kk values. */
+void or_unpack_secret_share(or_fe *secret, int n, int kk, or_fe *v);
+/* compute_secret_shares (src/MPI_utils.cpp:2738-2767) for M values, k per
+ * group, buffers of kk, N parties: the pads (entries k..kk-1 of each
+ * group's buffer, (M/k) * (kk - k) elements, group-major) come from `pads`
+ * (the reference draws random(); NULL gives zeros, as its non-private
+ * branch).  shares: N rows of M/k elements. */
+void or_compute_secret_shares(const or_fe *v, uint64_t M, int N, int k, int kk, const or_fe *pads, or_fe *shares);
+/* rank 0's combine in sumcheck_aggr (src/MPI_utils.cpp:2604-2645): polys
+ * N rows of len (rank-major), b N/4 weights; out len elements. */
+void or_sumcheck_aggr(const or_fe *polys, int N, int len, const or_fe *b, or_fe *out);
 
 #ifdef __cplusplus
 }

@@ -101,6 +101,11 @@ def oracle():
         L.or_rs_multifold_segment.argtypes = [vp, vp, i32, u64, u64]
         L.or_rs_multifold_segment.restype = _Fe
         L.or_row_combine.argtypes = [vp, i32, u64, vp, vp]
+        L.or_cosumcheck3_phase1.argtypes = [vp, vp, vp, vp, u64, i32, vp]
+        L.or_cosumcheck3_phase2.argtypes = [vp, vp, vp, vp, u64, i32, vp]
+        L.or_unpack_secret_share.argtypes = [vp, i32, i32, vp]
+        L.or_compute_secret_shares.argtypes = [vp, u64, i32, i32, i32, vp, vp]
+        L.or_sumcheck_aggr.argtypes = [vp, i32, i32, vp, vp]
         _o = L
     return _o
 
@@ -369,4 +374,49 @@ def row_combine(rows, c):
     cc = np.ascontiguousarray(c, dtype=np.uint64)
     out = fe_array(r.shape[1])
     oracle().or_row_combine(_p(r), r.shape[0], r.shape[1], _p(cc), _p(out))
+    return out
+
+
+def cosumcheck3_phase1(p1, p2, p3, b1, d):
+    """Cubic block sum: 4d elements."""
+    a = [np.ascontiguousarray(x, dtype=np.uint64) for x in (p1, p2, p3, b1)]
+    H = fe_array(4 * d)
+    oracle().or_cosumcheck3_phase1(_p(a[0]), _p(a[1]), _p(a[2]), _p(a[3]), a[0].shape[0], d, _p(H))
+    return H
+
+
+def cosumcheck3_phase2(p1, p2, p3, b1, d, L):
+    """Returns the four folded (M/d) vectors (inputs are not modified)."""
+    a = [np.ascontiguousarray(x, dtype=np.uint64).copy() for x in (p1, p2, p3, b1)]
+    LL = np.ascontiguousarray(L, dtype=np.uint64)
+    M = a[0].shape[0]
+    oracle().or_cosumcheck3_phase2(_p(a[0]), _p(a[1]), _p(a[2]), _p(a[3]), M, d, _p(LL))
+    return [x[:M // d].copy() for x in a]
+
+
+def unpack_secret_share(secret, kk):
+    """Returns (the transformed shares as the reference leaves them, the kk values)."""
+    s = np.ascontiguousarray(secret, dtype=np.uint64).copy()
+    v = fe_array(kk)
+    oracle().or_unpack_secret_share(_p(s), s.shape[0], kk, _p(v))
+    return s, v
+
+
+def compute_secret_shares(v, N, k, kk, pads=None):
+    """v: (M, 2).  Returns shares (N, M/k, 2)."""
+    a = np.ascontiguousarray(v, dtype=np.uint64)
+    M = a.shape[0]
+    sh = np.zeros((N, M // k, 2), np.uint64)
+    pp = None if pads is None else np.ascontiguousarray(pads, dtype=np.uint64)
+    oracle().or_compute_secret_shares(_p(a), M, N, k, kk, None if pp is None else _p(pp), _p(sh))
+    return sh
+
+
+def sumcheck_aggr(polys, b):
+    """polys: (N, len, 2) gathered per rank; b: (>= N/4, 2).  Returns (len, 2)."""
+    P = np.ascontiguousarray(polys, dtype=np.uint64)
+    N, ln = P.shape[0], P.shape[1]
+    bb = np.ascontiguousarray(b, dtype=np.uint64) if b is not None and len(b) else fe_array(1)
+    out = fe_array(ln)
+    oracle().or_sumcheck_aggr(_p(P), N, ln, _p(bb), _p(out))
     return out

@@ -78,6 +78,11 @@ def lib():
         L.cosnark_cosumcheck_phase1.argtypes = [vp, vp, vp, u64, i32, vp]
         L.cosnark_cosumcheck_phase2.argtypes = [vp, vp, vp, u64, i32, vp]
         L.cosnark_cosumcheck_fold.argtypes = [vp, vp, vp, u64, i32, vp]
+        L.cosnark_cosumcheck3_phase1.argtypes = [vp, vp, vp, vp, vp, u64, i32, vp]
+        L.cosnark_cosumcheck3_fold.argtypes = [vp, vp, vp, vp, vp, u64, i32, vp]
+        L.cosnark_cosumcheck3_phase2.argtypes = [vp, vp, vp, vp, vp, u64, i32, vp]
+        L.cosnark_share_unpack.argtypes = [vp, i32, i32, vp]
+        L.cosnark_sumcheck_aggr.argtypes = [vp, i32, i32, vp, vp]
         L.cosnark_rs_multifold5.argtypes = [vp, vp, u64, vp, i32, vp]
         L.cosnark_batch_create.argtypes = [vp, i32, vp, i32, ctypes.POINTER(vp)]
         L.cosnark_batch_destroy.argtypes = [vp]
@@ -379,6 +384,42 @@ class Context:
             a.free(); b.free()
         return out
 
+    def cosumcheck3_phase1(self, p1, p2, p3, b1, d, dev=None):
+        """Cubic block sumcheck phase 1: the 4d-element round polynomial (4d, 2).
+        dev: (four device buffers, M) already resident."""
+        if dev is None:
+            bufs = [self.upload(x) for x in (p1, p2, p3, b1)]
+            M = np.asarray(p1).shape[0]
+        else:
+            bufs, M = dev
+        out = np.zeros((4 * d, 2), np.uint64)
+        self._ck(lib().cosnark_cosumcheck3_phase1(self.h, *[b.ptr for b in bufs], M, d, _p(out)))
+        if dev is None:
+            for b in bufs:
+                b.free()
+        return out
+
+    def cosumcheck3_fold(self, p1, p2, p3, b1, d, L=None, r=None, dev=None):
+        """Cubic block sumcheck phase 2 with weights L (or L from r): the four
+        folded (M/d, 2) vectors (None when dev is given; folded in place)."""
+        if dev is None:
+            bufs = [self.upload(x) for x in (p1, p2, p3, b1)]
+            M = np.asarray(p1).shape[0]
+        else:
+            bufs, M = dev
+        if L is not None:
+            LL = np.ascontiguousarray(L, dtype=np.uint64)
+            self._ck(lib().cosnark_cosumcheck3_fold(self.h, *[b.ptr for b in bufs], M, d, _p(LL)))
+        else:
+            rr = np.ascontiguousarray(np.asarray(r, dtype=np.uint64).reshape(2))
+            self._ck(lib().cosnark_cosumcheck3_phase2(self.h, *[b.ptr for b in bufs], M, d, _p(rr)))
+        if dev is not None:
+            return None
+        res = [self.download(b, (M // d, 2)) for b in bufs]
+        for b in bufs:
+            b.free()
+        return res
+
     def cosumcheck_phase2(self, p1, p2, d, r, dev=None):
         """Block sumcheck phase 2: the folded (M/d, 2) vectors."""
         rr = np.ascontiguousarray(np.asarray(r, dtype=np.uint64).reshape(2))
@@ -544,3 +585,27 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def share_unpack(shares, kk):
+    """unpack_secret_share (src/MPI_utils.cpp:2769-2785) on the host:
+    (the shares transformed as the reference leaves them, the kk values)."""
+    s = np.ascontiguousarray(shares, dtype=np.uint64).copy()
+    out = np.zeros((kk, 2), np.uint64)
+    rc = lib().cosnark_share_unpack(_p(s), s.shape[0], kk, _p(out))
+    if rc:
+        raise CosnarkError(rc, "share_unpack: bad sizes")
+    return s, out
+
+
+def sumcheck_aggr(polys, b):
+    """Rank 0's combine of sumcheck_aggr (src/MPI_utils.cpp:2604-2645) over
+    the gathered polynomials (N, len, 2) with weights b (N/4, 2)."""
+    P = np.ascontiguousarray(polys, dtype=np.uint64)
+    N, ln = P.shape[0], P.shape[1]
+    bb = np.ascontiguousarray(b if b is not None and len(b) else np.zeros((1, 2)), dtype=np.uint64)
+    out = np.zeros((ln, 2), np.uint64)
+    rc = lib().cosnark_sumcheck_aggr(_p(P), N, ln, _p(bb), _p(out))
+    if rc:
+        raise CosnarkError(rc, "sumcheck_aggr: bad sizes")
+    return out

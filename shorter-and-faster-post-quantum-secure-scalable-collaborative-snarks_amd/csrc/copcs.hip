@@ -46,6 +46,8 @@ namespace {
 constexpr int BSC_TB = 64;              // blocks per tile, small d
 constexpr int BSC_SMALL_MAX = 32;
 constexpr int BSC_D_MAX = 2048;
+constexpr int BSC3_TILE_LOG = 9;        // cubic form: elements of each vector per tile, d <= 512
+constexpr int BSC3_D_MAX = 1024;
 constexpr int BSC_MAX_WG = 1024;
 constexpr int CO_THREADS = 256;
 constexpr int MF_SMALL_MAX = 32;
@@ -142,13 +144,15 @@ __device__ __forceinline__ uint32_t brev_bits(uint32_t x, int bits) {
 
 // Radix-2 decimation-in-frequency stages over NA arrays of 2^logd elements
 // in LDS (natural order in, bit-reversed out), unscaled; the twiddle tables
-// passed in (root of order 2^28 or its inverse) set the direction.
+// passed in (root of order 2^28 or its inverse) set the direction.  With
+// logn > logd each array holds 2^(logn - logd) consecutive independent
+// blocks (a stage's pairs never cross a block).
 template <int NA>
-__device__ void lds_dif(fe *const *A, int logd, const fe *twlo, const fe *twhi) {
-    const int d = 1 << logd;
+__device__ void lds_dif(fe *const *A, int logd, const fe *twlo, const fe *twhi, int logn = -1) {
+    const int n = 1 << (logn < 0 ? logd : logn);
     for (int lh = logd - 1; lh >= 0; --lh) {
         const int h = 1 << lh;
-        for (int p = threadIdx.x; p < d / 2; p += CO_THREADS) {
+        for (int p = threadIdx.x; p < n / 2; p += CO_THREADS) {
             const int off = p & (h - 1), i = ((p >> lh) << (lh + 1)) + off;
             const fe w = dft::tw28(twlo, twhi, (uint32_t)off << (NTT_LOG_MAX - lh - 1));
 #pragma unroll
@@ -164,11 +168,11 @@ __device__ void lds_dif(fe *const *A, int logd, const fe *twlo, const fe *twhi) 
 
 // Radix-2 decimation-in-time stages: bit-reversed order in, natural out.
 template <int NA>
-__device__ void lds_dit(fe *const *A, int logd, const fe *twlo, const fe *twhi) {
-    const int d = 1 << logd;
+__device__ void lds_dit(fe *const *A, int logd, const fe *twlo, const fe *twhi, int logn = -1) {
+    const int n = 1 << (logn < 0 ? logd : logn);
     for (int lh = 0; lh < logd; ++lh) {
         const int h = 1 << lh;
-        for (int p = threadIdx.x; p < d / 2; p += CO_THREADS) {
+        for (int p = threadIdx.x; p < n / 2; p += CO_THREADS) {
             const int off = p & (h - 1), i = ((p >> lh) << (lh + 1)) + off;
             const fe w = dft::tw28(twlo, twhi, (uint32_t)off << (NTT_LOG_MAX - lh - 1));
 #pragma unroll
@@ -231,11 +235,13 @@ __global__ __launch_bounds__(CO_THREADS) void bsc_large_kernel(const fe *__restr
     }
 }
 
-// column sums over the workgroups; 64 columns per workgroup, 4 lanes each
-__global__ __launch_bounds__(CO_THREADS) void bsc_final_kernel(const fe *__restrict__ part, int nwg, int d,
-                                                               fe inv_d2, fe *__restrict__ out) {
+// column sums over the workgroups; 64 columns per workgroup, 4 lanes each.
+// Column c = co * d + i is the point w^(ncos i + co) (w of order ncos * d);
+// columns with co > 0 are multiplied by `scale`.
+__global__ __launch_bounds__(CO_THREADS) void bsc_final_kernel(const fe *__restrict__ part, int nwg, int d, int ncos,
+                                                               fe scale, fe *__restrict__ out) {
     __shared__ fe S[4][64];
-    const int cols = 2 * d, cl = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int cols = ncos * d, cl = threadIdx.x & 63, g = threadIdx.x >> 6;
     const int c = blockIdx.x * 64 + cl;
     fe s = gf::zero();
     if (c < cols)
@@ -244,31 +250,141 @@ __global__ __launch_bounds__(CO_THREADS) void bsc_final_kernel(const fe *__restr
     __syncthreads();
     if (g == 0 && c < cols) {
         s = gf::add(gf::add(S[0][cl], S[1][cl]), gf::add(S[2][cl], S[3][cl]));
-        if (c < d) out[2 * c] = s;
-        else out[2 * (c - d) + 1] = gf::mul(s, inv_d2);
+        const int co = c / d, i = c - co * d;
+        out[ncos * i + co] = co ? gf::mul(s, scale) : s;
+    }
+}
+
+// Cubic block sum (cubic_coSumcheck_phase1, src/coSumcheck.cpp:11-45):
+//     H[k] = sum_j (P1 P2 - P4)(w_4d^k) P3(w_4d^k),   k < 4d,
+// with P1, P2, P3, P4 the interpolants of block j of p1, p2, b1, p3.  For
+// k = 4i + s, s = 0 is the blocks' own values; s = 1, 2, 3 are the
+// d-point transforms NTT_d(c_m w_4d^(s m) / d) of the unscaled inverse
+// transform c of each block, so each block takes one inverse and three
+// forward transforms per vector.  A workgroup takes tiles of T = max(d, 512)
+// elements of each vector (T/d blocks); each thread accumulates the four
+// cosets of the elements it owns, and the tile's blocks are added up per
+// column at the end.
+__global__ __launch_bounds__(CO_THREADS) void bsc3_kernel(const fe *__restrict__ p1, const fe *__restrict__ p2,
+                                                          const fe *__restrict__ p3, const fe *__restrict__ b1,
+                                                          uint64_t M, int logd, int logT, const fe *twlo,
+                                                          const fe *twhi, const fe *itlo, const fe *ithi, fe inv_d,
+                                                          fe *__restrict__ part) {
+    extern __shared__ fe S[];
+    constexpr int QMAX = BSC3_D_MAX / CO_THREADS;
+    const int d = 1 << logd, T = 1 << logT, t = threadIdx.x, QT = T / CO_THREADS;
+    fe *const A[4] = {S, S + T, S + 2 * T, S + 3 * T};      // p1, p2, p3, b1
+    fe *const B[1] = {S + 4 * T};
+    const fe *const src[4] = {p1, p2, p3, b1};
+    fe acc[4][QMAX], tmp[QMAX], f[QMAX];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int q = 0; q < QMAX; ++q) acc[c][q] = gf::zero();
+    const uint64_t ntiles = (M + T - 1) >> logT;
+    const int G = T >> logd;
+    int *live = (int *)(S + 5 * T);      // per block of the tile: its p1 part has a non-zero element
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const uint64_t base = tile << logT;
+        for (int g = t; g < G; g += CO_THREADS) live[g] = 0;
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < QMAX; ++q) {
+            if (q >= QT) break;
+            const int e = t + CO_THREADS * q;
+            const uint64_t pos = base + e;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) A[v][e] = pos < M ? src[v][pos] : gf::zero();
+            if (A[0][e].re | A[0][e].im) atomicOr(&live[e >> logd], 1);
+        }
+        __syncthreads();
+        // the reference skips blocks whose p1 part is zero (src/coSumcheck.cpp:
+        // 22-28); zeroing such a block's b1 part removes it from every coset
+#pragma unroll
+        for (int q = 0; q < QMAX; ++q) {
+            if (q >= QT) break;
+            const int e = t + CO_THREADS * q;
+            if (!live[e >> logd]) {
+                A[3][e] = gf::zero();
+            } else {
+                const fe x = gf::sub(gf::mul(A[0][e], A[1][e]), A[2][e]);
+                acc[0][q] = gf::add(acc[0][q], gf::mul(x, A[3][e]));
+            }
+        }
+        __syncthreads();
+        lds_dif<4>(A, logd, itlo, ithi, logT);                  // A[v][e] = c[brev(e mod d)] (unscaled)
+        dft::static_for<1, 4>([&](auto cc) {
+            constexpr int co = decltype(cc)::value;
+#pragma unroll
+            for (int q = 0; q < QMAX; ++q) {
+                if (q >= QT) break;
+                const uint32_t m = brev_bits((uint32_t)((t + CO_THREADS * q) & (d - 1)), logd);
+                f[q] = gf::mul(dft::tw28(twlo, twhi, (co * m) << (NTT_LOG_MAX - logd - 2)), inv_d);
+            }
+            dft::static_for<0, 4>([&](auto vv) {
+                constexpr int v = decltype(vv)::value;
+#pragma unroll
+                for (int q = 0; q < QMAX; ++q) {
+                    if (q >= QT) break;
+                    const int e = t + CO_THREADS * q;
+                    B[0][e] = gf::mul(A[v][e], f[q]);
+                }
+                __syncthreads();
+                lds_dit<1>(B, logd, twlo, twhi, logT);          // B[e] = P_v(w_4d^(4 (e mod d) + co))
+#pragma unroll
+                for (int q = 0; q < QMAX; ++q) {
+                    if (q >= QT) break;
+                    const fe y = B[0][t + CO_THREADS * q];
+                    if (v == 0) tmp[q] = y;
+                    else if (v == 1) tmp[q] = gf::mul(tmp[q], y);
+                    else if (v == 2) tmp[q] = gf::sub(tmp[q], y);
+                    else acc[co][q] = gf::add(acc[co][q], gf::mul(tmp[q], y));
+                }
+                __syncthreads();
+            });
+        });
+    }
+    // element slots -> columns: column (co, i) adds the slots co*T + g*d + i
+#pragma unroll
+    for (int co = 0; co < 4; ++co)
+#pragma unroll
+        for (int q = 0; q < QMAX; ++q) {
+            if (q >= QT) break;
+            S[co * T + t + CO_THREADS * q] = acc[co][q];
+        }
+    __syncthreads();
+    for (int c = t; c < 4 * d; c += CO_THREADS) {
+        const int co = c >> logd, i = c & (d - 1);
+        fe sum = gf::zero();
+        for (int g = 0; g < G; ++g) sum = gf::add(sum, S[co * T + (g << logd) + i]);
+        part[(uint64_t)blockIdx.x * 4 * d + c] = sum;
     }
 }
 
 // ---- block sumcheck, phase 2 ----------------------------------------------
 
-// one lane per output, both vectors (d <= 64)
-__global__ __launch_bounds__(CO_THREADS) void bsc_fold_small_kernel(const fe *__restrict__ p1,
-                                                                    const fe *__restrict__ p2, uint64_t nout,
-                                                                    int d, const fe *__restrict__ L,
-                                                                    fe *__restrict__ o1, fe *__restrict__ o2) {
+struct FoldPtrs {
+    const fe *in[4];
+    fe *out[4];
+};
+
+// one lane per output, NA vectors (d <= 64)
+template <int NA>
+__global__ __launch_bounds__(CO_THREADS) void bsc_fold_small_kernel(FoldPtrs P, uint64_t nout, int d,
+                                                                    const fe *__restrict__ L) {
     __shared__ fe Ls[64];
     if (threadIdx.x < d) Ls[threadIdx.x] = L[threadIdx.x];
     __syncthreads();
     const uint64_t j = (uint64_t)blockIdx.x * CO_THREADS + threadIdx.x;
     if (j >= nout) return;
-    const fe *a = p1 + j * d, *b = p2 + j * d;
-    fe s1 = gf::zero(), s2 = gf::zero();
-    for (int k = 0; k < d; ++k) {
-        s1 = gf::add(s1, gf::mul(Ls[k], a[k]));
-        s2 = gf::add(s2, gf::mul(Ls[k], b[k]));
-    }
-    o1[j] = s1;
-    o2[j] = s2;
+    fe s[NA];
+#pragma unroll
+    for (int a = 0; a < NA; ++a) s[a] = gf::zero();
+    for (int k = 0; k < d; ++k)
+#pragma unroll
+        for (int a = 0; a < NA; ++a) s[a] = gf::add(s[a], gf::mul(Ls[k], P.in[a][j * d + k]));
+#pragma unroll
+    for (int a = 0; a < NA; ++a) P.out[a][j] = s[a];
 }
 
 __device__ __forceinline__ fe shfl_xor_fe(const fe &x, int m) {
@@ -276,30 +392,28 @@ __device__ __forceinline__ fe shfl_xor_fe(const fe &x, int m) {
 }
 
 // one wave per output (d >= 128): lane l takes k = l + 64 q
-__global__ __launch_bounds__(CO_THREADS) void bsc_fold_wave_kernel(const fe *__restrict__ p1,
-                                                                   const fe *__restrict__ p2, uint64_t nout,
-                                                                   int d, const fe *__restrict__ L,
-                                                                   fe *__restrict__ o1, fe *__restrict__ o2) {
+template <int NA>
+__global__ __launch_bounds__(CO_THREADS) void bsc_fold_wave_kernel(FoldPtrs P, uint64_t nout, int d,
+                                                                   const fe *__restrict__ L) {
     const int lane = threadIdx.x & 63;
     const uint64_t w0 = ((uint64_t)blockIdx.x * CO_THREADS + threadIdx.x) >> 6;
     const uint64_t nw = ((uint64_t)gridDim.x * CO_THREADS) >> 6;
     for (uint64_t j = w0; j < nout; j += nw) {
-        const fe *a = p1 + j * d, *b = p2 + j * d;
-        fe s1 = gf::zero(), s2 = gf::zero();
+        fe s[NA];
+#pragma unroll
+        for (int a = 0; a < NA; ++a) s[a] = gf::zero();
         for (int k = lane; k < d; k += 64) {
             const fe l = L[k];
-            s1 = gf::add(s1, gf::mul(l, a[k]));
-            s2 = gf::add(s2, gf::mul(l, b[k]));
+#pragma unroll
+            for (int a = 0; a < NA; ++a) s[a] = gf::add(s[a], gf::mul(l, P.in[a][j * d + k]));
         }
 #pragma unroll
-        for (int m = 32; m >= 1; m >>= 1) {
-            s1 = gf::add(s1, shfl_xor_fe(s1, m));
-            s2 = gf::add(s2, shfl_xor_fe(s2, m));
-        }
-        if (lane == 0) {
-            o1[j] = s1;
-            o2[j] = s2;
-        }
+        for (int m = 32; m >= 1; m >>= 1)
+#pragma unroll
+            for (int a = 0; a < NA; ++a) s[a] = gf::add(s[a], shfl_xor_fe(s[a], m));
+        if (lane == 0)
+#pragma unroll
+            for (int a = 0; a < NA; ++a) P.out[a][j] = s[a];
     }
 }
 
@@ -498,9 +612,64 @@ int copcs_init(cosnark_ctx *) {
         hipFuncSetAttribute((const void *)bsc_large_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                             large_lds) != hipSuccess ||
         hipFuncSetAttribute((const void *)multifold_large_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            mf_lds) != hipSuccess)
+                            mf_lds) != hipSuccess ||
+        hipFuncSetAttribute((const void *)bsc3_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)(sizeof(fe) * 5 * BSC3_D_MAX + sizeof(int) * (1 << BSC3_TILE_LOG))) != hipSuccess)
         return COSNARK_E_HIP;
     return COSNARK_OK;
+}
+
+// p[j] = sum_k L_k p[jd + k] for each of the na vectors, in place (through
+// scratch: a block's outputs overwrite inputs of other blocks)
+static int block_fold(cosnark_ctx *ctx, uint64_t *const *v, int na, uint64_t M, int d, const uint64_t *h_L,
+                      const char *what) {
+    for (int a = 0; a < na; ++a)
+        if (!v[a]) return set_err(ctx, COSNARK_E_ARG, std::string(what) + ": null argument");
+    if (d < 2 || !is_pow2((uint64_t)d) || !is_pow2(M) || M < (uint64_t)d)
+        return set_err(ctx, COSNARK_E_ARG, std::string(what) + ": d and M must be powers of two, 2 <= d <= M");
+    if (d > CO_HOST_ELEMS) return set_err(ctx, COSNARK_E_RANGE, std::string(what) + ": d too large");
+    const uint64_t nout = M / d;
+    void *base;
+    int rc = scratch_reserve(ctx, sizeof(fe) * (na * nout + d), &base);
+    if (rc) return rc;
+    FoldPtrs P{};
+    for (int a = 0; a < na; ++a) {
+        P.in[a] = (const fe *)v[a];
+        P.out[a] = (fe *)base + a * nout;
+    }
+    fe *Ld = (fe *)base + na * nout;
+    memcpy(ctx->co_host, h_L, sizeof(fe) * d);
+    hipStream_t s = ctx->stream;
+    HIPCK(hipMemcpyAsync(Ld, ctx->co_host, sizeof(fe) * d, hipMemcpyHostToDevice, s));
+    if (d <= 64) {
+        const dim3 g((unsigned)((nout + CO_THREADS - 1) / CO_THREADS));
+        if (na == 2) hipLaunchKernelGGL(bsc_fold_small_kernel<2>, g, dim3(CO_THREADS), 0, s, P, nout, d, (const fe *)Ld);
+        else hipLaunchKernelGGL(bsc_fold_small_kernel<4>, g, dim3(CO_THREADS), 0, s, P, nout, d, (const fe *)Ld);
+    } else {
+        const uint64_t wg = (nout + 3) / 4;
+        const dim3 g((unsigned)(wg < 4096 ? wg : 4096));
+        if (na == 2) hipLaunchKernelGGL(bsc_fold_wave_kernel<2>, g, dim3(CO_THREADS), 0, s, P, nout, d, (const fe *)Ld);
+        else hipLaunchKernelGGL(bsc_fold_wave_kernel<4>, g, dim3(CO_THREADS), 0, s, P, nout, d, (const fe *)Ld);
+    }
+    LAUNCHCK();
+    for (int a = 0; a < na; ++a)
+        HIPCK(hipMemcpyAsync(v[a], P.out[a], sizeof(fe) * nout, hipMemcpyDeviceToDevice, s));
+    return cosnark_sync(ctx);
+}
+
+// unpack_secret_share on the host (src/MPI_utils.cpp:2769-2785): the n
+// shares are values of a polynomial of degree < n at w_2n^(2j+1); returns
+// its values at the n-th roots of unity in place of the shares
+static void unpack_values(std::vector<fe> &x) {
+    const int n = (int)x.size();
+    host_fft(x, true);
+    const fe wi = gf::inv(gf::root_of_unity(1 + ilog2((uint64_t)n)));
+    fe m = gf::one();
+    for (int i = 0; i < n; ++i) {
+        x[i] = gf::mul(x[i], m);
+        m = gf::mul(m, wi);
+    }
+    host_fft(x, false);
 }
 
 extern "C" {
@@ -599,7 +768,7 @@ int cosnark_cosumcheck_phase1(cosnark_ctx *ctx, const uint64_t *d_p1, const uint
     }
     LAUNCHCK();
     const fe di = gf::inv(gf::from_u64((uint64_t)d));
-    hipLaunchKernelGGL(bsc_final_kernel, dim3((cols + 63) / 64), dim3(CO_THREADS), 0, s, (const fe *)part, nwg, d,
+    hipLaunchKernelGGL(bsc_final_kernel, dim3((cols + 63) / 64), dim3(CO_THREADS), 0, s, (const fe *)part, nwg, d, 2,
                        gf::mul(di, di), out);
     LAUNCHCK();
     HIPCK(hipMemcpyAsync(ctx->co_host, out, sizeof(fe) * cols, hipMemcpyDeviceToHost, s));
@@ -612,30 +781,8 @@ int cosnark_cosumcheck_phase1(cosnark_ctx *ctx, const uint64_t *d_p1, const uint
 int cosnark_cosumcheck_fold(cosnark_ctx *ctx, uint64_t *d_p1, uint64_t *d_p2, uint64_t M, int d,
                             const uint64_t *h_L) {
     if (!ctx || !d_p1 || !d_p2 || !h_L) return set_err(ctx, COSNARK_E_ARG, "cosumcheck_fold: null argument");
-    if (d < 2 || !is_pow2((uint64_t)d) || !is_pow2(M) || M < (uint64_t)d)
-        return set_err(ctx, COSNARK_E_ARG, "cosumcheck_fold: d and M must be powers of two, 2 <= d <= M");
-    if (d > CO_HOST_ELEMS) return set_err(ctx, COSNARK_E_RANGE, "cosumcheck_fold: d too large");
-    const uint64_t nout = M / d;
-    void *base;
-    int rc = scratch_reserve(ctx, sizeof(fe) * (2 * nout + d), &base);
-    if (rc) return rc;
-    fe *o1 = (fe *)base, *o2 = o1 + nout, *Ld = o2 + nout;
-    memcpy(ctx->co_host, h_L, sizeof(fe) * d);
-    hipStream_t s = ctx->stream;
-    HIPCK(hipMemcpyAsync(Ld, ctx->co_host, sizeof(fe) * d, hipMemcpyHostToDevice, s));
-    if (d <= 64) {
-        hipLaunchKernelGGL(bsc_fold_small_kernel, dim3((unsigned)((nout + CO_THREADS - 1) / CO_THREADS)),
-                           dim3(CO_THREADS), 0, s, (const fe *)d_p1, (const fe *)d_p2, nout, d, (const fe *)Ld,
-                           o1, o2);
-    } else {
-        const uint64_t waves = nout, wg = (waves + 3) / 4;
-        hipLaunchKernelGGL(bsc_fold_wave_kernel, dim3((unsigned)(wg < 4096 ? wg : 4096)), dim3(CO_THREADS), 0, s,
-                           (const fe *)d_p1, (const fe *)d_p2, nout, d, (const fe *)Ld, o1, o2);
-    }
-    LAUNCHCK();
-    HIPCK(hipMemcpyAsync(d_p1, o1, sizeof(fe) * nout, hipMemcpyDeviceToDevice, s));
-    HIPCK(hipMemcpyAsync(d_p2, o2, sizeof(fe) * nout, hipMemcpyDeviceToDevice, s));
-    return cosnark_sync(ctx);
+    uint64_t *v[4] = {d_p1, d_p2, nullptr, nullptr};
+    return block_fold(ctx, v, 2, M, d, h_L, "cosumcheck_fold");
 }
 
 int cosnark_cosumcheck_phase2(cosnark_ctx *ctx, uint64_t *d_p1, uint64_t *d_p2, uint64_t M, int d,
@@ -645,6 +792,98 @@ int cosnark_cosumcheck_phase2(cosnark_ctx *ctx, uint64_t *d_p1, uint64_t *d_p2, 
         return set_err(ctx, COSNARK_E_ARG, "cosumcheck_phase2: d must be a power of two in [2, 8192]");
     const std::vector<fe> L = lagrange_weights(gf::mk(r[0], r[1]), d);
     return cosnark_cosumcheck_fold(ctx, d_p1, d_p2, M, d, (const uint64_t *)L.data());
+}
+
+int cosnark_cosumcheck3_phase1(cosnark_ctx *ctx, const uint64_t *d_p1, const uint64_t *d_p2, const uint64_t *d_p3,
+                               const uint64_t *d_b1, uint64_t M, int d, uint64_t *h_poly) {
+    if (!ctx || !d_p1 || !d_p2 || !d_p3 || !d_b1 || !h_poly)
+        return set_err(ctx, COSNARK_E_ARG, "cosumcheck3_phase1: null argument");
+    if (d < 2 || !is_pow2((uint64_t)d) || !is_pow2(M) || M < (uint64_t)d)
+        return set_err(ctx, COSNARK_E_ARG, "cosumcheck3_phase1: d and M must be powers of two, 2 <= d <= M");
+    if (d > BSC3_D_MAX) return set_err(ctx, COSNARK_E_RANGE, "cosumcheck3_phase1: d > 1024");
+    const int logd = ilog2((uint64_t)d), logT = std::max(logd, BSC3_TILE_LOG);
+    const uint64_t ntiles = (M + (1ull << logT) - 1) >> logT;
+    const int nwg = (int)(ntiles < 512 ? ntiles : 512), cols = 4 * d;
+    void *base;
+    int rc = scratch_reserve(ctx, sizeof(fe) * ((size_t)nwg * cols + cols), &base);
+    if (rc) return rc;
+    fe *part = (fe *)base, *out = part + (size_t)nwg * cols;
+    hipStream_t s = ctx->stream;
+    const fe di = gf::inv(gf::from_u64((uint64_t)d));
+    const size_t lds = sizeof(fe) * 5 * ((size_t)1 << logT) + sizeof(int) * ((size_t)1 << (logT - logd));
+    hipLaunchKernelGGL(bsc3_kernel, dim3(nwg), dim3(CO_THREADS), lds, s,
+                       (const fe *)d_p1, (const fe *)d_p2, (const fe *)d_p3, (const fe *)d_b1, M, logd, logT,
+                       (const fe *)ctx->tw_lo, (const fe *)ctx->tw_hi, (const fe *)ctx->itw_lo,
+                       (const fe *)ctx->itw_hi, di, part);
+    LAUNCHCK();
+    hipLaunchKernelGGL(bsc_final_kernel, dim3((cols + 63) / 64), dim3(CO_THREADS), 0, s, (const fe *)part, nwg, d, 4,
+                       gf::one(), out);
+    LAUNCHCK();
+    HIPCK(hipMemcpyAsync(ctx->co_host, out, sizeof(fe) * cols, hipMemcpyDeviceToHost, s));
+    rc = cosnark_sync(ctx);
+    if (rc) return rc;
+    memcpy(h_poly, ctx->co_host, sizeof(fe) * cols);
+    return COSNARK_OK;
+}
+
+int cosnark_cosumcheck3_fold(cosnark_ctx *ctx, uint64_t *d_p1, uint64_t *d_p2, uint64_t *d_p3, uint64_t *d_b1,
+                             uint64_t M, int d, const uint64_t *h_L) {
+    if (!ctx || !h_L) return set_err(ctx, COSNARK_E_ARG, "cosumcheck3_fold: null argument");
+    uint64_t *v[4] = {d_p1, d_p2, d_p3, d_b1};
+    return block_fold(ctx, v, 4, M, d, h_L, "cosumcheck3_fold");
+}
+
+int cosnark_cosumcheck3_phase2(cosnark_ctx *ctx, uint64_t *d_p1, uint64_t *d_p2, uint64_t *d_p3, uint64_t *d_b1,
+                               uint64_t M, int d, const uint64_t r[2]) {
+    if (!ctx || !r) return set_err(ctx, COSNARK_E_ARG, "cosumcheck3_phase2: null argument");
+    if (d < 2 || !is_pow2((uint64_t)d) || d > CO_HOST_ELEMS)
+        return set_err(ctx, COSNARK_E_ARG, "cosumcheck3_phase2: d must be a power of two in [2, 8192]");
+    const std::vector<fe> L = lagrange_weights(gf::mk(r[0], r[1]), d);
+    return cosnark_cosumcheck3_fold(ctx, d_p1, d_p2, d_p3, d_b1, M, d, (const uint64_t *)L.data());
+}
+
+int cosnark_share_unpack(uint64_t *h_secret, int n, int kk, uint64_t *h_out) {
+    if (!h_secret || !h_out) return COSNARK_E_ARG;
+    if (n < 1 || n > CO_HOST_ELEMS || !is_pow2((uint64_t)n) || kk < 1 || kk > n) return COSNARK_E_ARG;
+    std::vector<fe> x(n);
+    for (int i = 0; i < n; ++i) x[i] = gf::mk(h_secret[2 * i], h_secret[2 * i + 1]);
+    unpack_values(x);
+    for (int i = 0; i < n; ++i) {
+        h_secret[2 * i] = x[i].re;
+        h_secret[2 * i + 1] = x[i].im;
+    }
+    for (int j = 0; j < kk; ++j) {
+        const fe y = x[(uint64_t)n * j / kk];
+        h_out[2 * j] = y.re;
+        h_out[2 * j + 1] = y.im;
+    }
+    return COSNARK_OK;
+}
+
+int cosnark_sumcheck_aggr(const uint64_t *h_polys, int nranks, int len, const uint64_t *h_b, uint64_t *h_out) {
+    if (!h_polys || !h_out || (nranks >= 4 && !h_b)) return COSNARK_E_ARG;
+    if (nranks < 1 || nranks > CO_HOST_ELEMS || !is_pow2((uint64_t)nranks) || len < 0) return COSNARK_E_ARG;
+    // out[i] = sum_{j < N/4} b_j U(P[.][i])[2j] with U the unpack map; U is
+    // linear, so out[i] = sum_m w_m P[m][i], w_m = sum_j b_j U(e_m)[2j]
+    const int N = nranks;
+    std::vector<fe> w(N, gf::zero()), e(N);
+    for (int m = 0; m < N && N >= 4; ++m) {
+        std::fill(e.begin(), e.end(), gf::zero());
+        e[m] = gf::one();
+        unpack_values(e);
+        for (int j = 0; j < N / 4; ++j)
+            w[m] = gf::add(w[m], gf::mul(gf::mk(h_b[2 * j], h_b[2 * j + 1]), e[2 * j]));
+    }
+    for (int i = 0; i < len; ++i) {
+        fe acc = gf::zero();
+        for (int m = 0; m < N; ++m) {
+            const uint64_t *p = h_polys + 2 * ((uint64_t)m * len + i);
+            acc = gf::add(acc, gf::mul(w[m], gf::mk(p[0], p[1])));
+        }
+        h_out[2 * i] = acc.re;
+        h_out[2 * i + 1] = acc.im;
+    }
+    return COSNARK_OK;
 }
 
 }  // extern "C"

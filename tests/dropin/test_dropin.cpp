@@ -214,6 +214,56 @@ int main() {
         CHECK(c1.size() == M / 32 && same(c1.data(), e1.data(), 16 * (M / 32)) && same(c2.data(), e2.data(), 16 * (M / 32)),
               "quadratic_coSumcheck_phase2 d=32");
     }
+    // cubic coSumcheck phases with the mask; rank 0's share arithmetic
+    {
+        const uint64_t M = 1 << 12;
+        vector<F> p1 = synth(seed, 60, M), p2 = synth(seed, 61, M), p3 = synth(seed, 62, M), b1 = synth(seed, 63, M);
+        for (int k = 0; k < 16; ++k) p1[16 * 5 + k] = F{0, 0};           // a skipped block
+        vector<F> b2 = synth(seed, 64, 4);
+        F r = synth(seed, 65, 1)[0];
+        vector<vector<F>> R = {synth(seed, 66, 64), synth(seed, 67, 32), synth(seed, 68, 12)};
+        vector<F> H = D::cubic_coSumcheck_phase1(p1, p2, p3, b1, b2, R, r, 16, 0, 1), oh(64);
+        or_cosumcheck3_phase1(reinterpret_cast<or_fe *>(p1.data()), reinterpret_cast<or_fe *>(p2.data()),
+                              reinterpret_cast<or_fe *>(p3.data()), reinterpret_cast<or_fe *>(b1.data()), M, 16,
+                              reinterpret_cast<or_fe *>(oh.data()));
+        or_fe h0 = O(R[0][0]);
+        for (int j = 1; j < 3; ++j)
+            for (size_t k = 0; k < R[j].size() / 4; ++k) h0 = or_add(h0, O(R[j][4 * k]));
+        bool ok = H.size() == 64;
+        for (int i = 0; ok && i < 64; ++i) {
+            const or_fe hr = i == 0 ? h0 : O(R[0][i]);
+            const or_fe want = or_add(O(oh[i]), or_mul(O(r), hr));
+            ok = H[i].real == want.real && H[i].img == want.img;
+        }
+        CHECK(ok, "cubic_coSumcheck_phase1 d=16 with the R mask and a zero block");
+        vector<F> L(16);
+        or_compute_lagrange_coeff(O(r), 16, reinterpret_cast<or_fe *>(L.data()));
+        vector<F> c[4] = {p1, p2, p3, b1}, e[4] = {p1, p2, p3, b1};
+        D::cubic_coSumcheck_phase2(c[0], c[1], c[2], c[3], L, 16);
+        or_cosumcheck3_phase2(reinterpret_cast<or_fe *>(e[0].data()), reinterpret_cast<or_fe *>(e[1].data()),
+                              reinterpret_cast<or_fe *>(e[2].data()), reinterpret_cast<or_fe *>(e[3].data()), M, 16,
+                              reinterpret_cast<or_fe *>(L.data()));
+        ok = true;
+        for (int a = 0; a < 4; ++a) ok = ok && c[a].size() == M / 16 && same(c[a].data(), e[a].data(), 16 * (M / 16));
+        CHECK(ok, "cubic_coSumcheck_phase2 d=16");
+        // unpack_secret_share leaves the shares transformed, returns _k values
+        vector<F> sh = synth(seed, 69, 16), osh = sh, v, ov(8);
+        v = D::unpack_secret_share(sh, 8, 4);
+        or_unpack_secret_share(reinterpret_cast<or_fe *>(osh.data()), 16, 8, reinterpret_cast<or_fe *>(ov.data()));
+        CHECK(v.size() == 8 && same(v.data(), ov.data(), 16 * 8) && same(sh.data(), osh.data(), 16 * 16),
+              "unpack_secret_share 16 shares, _k = 8");
+        // rank 0 combining 8 ranks' polynomials with b2
+        vector<vector<F>> P;
+        vector<F> flat;
+        for (int m = 0; m < 8; ++m) {
+            P.push_back(synth(seed, 70 + m, 64));
+            flat.insert(flat.end(), P.back().begin(), P.back().end());
+        }
+        vector<F> agg = D::sumcheck_aggr_gathered(P, b2), oagg(64);
+        or_sumcheck_aggr(reinterpret_cast<or_fe *>(flat.data()), 8, 64, reinterpret_cast<or_fe *>(b2.data()),
+                         reinterpret_cast<or_fe *>(oagg.data()));
+        CHECK(agg.size() == 64 && same(agg.data(), oagg.data(), 16 * 64), "sumcheck_aggr over 8 ranks' polynomials");
+    }
     // RS_multifold5
     {
         vector<F> cw = synth(seed, 40, 1 << 14), a = synth(seed, 41, 32), ref(cw.size() / 32);
