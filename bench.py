@@ -41,6 +41,7 @@ def parse():
     ap.add_argument("--serial", action="store_true", help="run the stages one after another (profiling)")
     ap.add_argument("--no-whir", action="store_true", help="skip the WHIR (config 3) leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched (config 4) leg")
+    ap.add_argument("--no-cosc", action="store_true", help="skip the coSumcheck rounds leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving (config 5) leg")
     ap.add_argument("--no-tune", action="store_true", help="keep the default NTT radix plan")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="serving leg arrival rate (requests/s)")
@@ -90,6 +91,51 @@ def whir_leg(ctx, reps=3):
     return {"workload": "WHIR 2^24 evaluations, blowup 4, whir_k 4, down to 2^8 (8 iterations)",
             "commit_ms": round(min(commit_ms), 3), "open_ms": round(min(open_ms), 3),
             "iterations": int(rec[0]), "record_words": int(rec.shape[0])}
+
+
+def cosumcheck_leg(ctx, reps=3):
+    """The coSumcheck rounds of one rank's 2^22-element share at the
+    reference's degrees (src/coPIOP.cpp:1409-1420): cubic over (vL, vR, vO,
+    beta) with degrees 16, 512, 512 (the reference's trailing degree-2 round
+    finds one element left and has no blocks) and quadratic over two vectors
+    with degrees 256, 512, 32.  A round is the block sum (phase 1), its
+    Fiat-Shamir challenge and the fold (phase 2); inputs resident in HBM,
+    restored from device copies between repetitions (untimed)."""
+    seed, M = 0xC05A0006, 1 << 22
+    src = [ctx.upload(cosnark.synth(seed, i, M)) for i in range(4)]
+    work = [ctx.alloc(16 * M) for _ in range(4)]
+    res = {}
+    for name, nv, degrees in (("cubic", 4, (16, 512, 512)), ("quadratic", 2, (256, 512, 32))):
+        best, rounds = None, None
+        for _ in range(reps):
+            for i in range(nv):
+                cosnark.lib().cosnark_d2d(ctx.h, work[i].ptr, src[i].ptr, 16 * M)
+            ctx.sync()
+            ts = cosnark.Transcript()
+            size, per = M, []
+            t0 = time.perf_counter()
+            for d in degrees:
+                r0 = time.perf_counter()
+                if nv == 4:
+                    H = ctx.cosumcheck3_phase1(None, None, None, None, d, dev=(work, size))
+                    c = cosnark.hash_to_field(ts, H)
+                    ctx.cosumcheck3_fold(None, None, None, None, d, r=c, dev=(work, size))
+                else:
+                    H = ctx.cosumcheck_phase1(None, None, d, dev=(work[0], work[1], size))
+                    c = cosnark.hash_to_field(ts, H)
+                    ctx.cosumcheck_phase2(None, None, d, c, dev=(work[0], work[1], size))
+                per.append((time.perf_counter() - r0) * 1e3)
+                size //= d
+            total = (time.perf_counter() - t0) * 1e3
+            if best is None or total < best:
+                best, rounds = total, per
+        res[name] = {"degrees": list(degrees), "vectors": nv, "ms": round(best, 3),
+                     "round_ms": [round(x, 3) for x in rounds],
+                     "elements_per_s": round(nv * M / (best / 1e3), 1)}
+    for b in src + work:
+        b.free()
+    res["workload"] = "coSumcheck rounds over one rank's 2^22-element vectors (cubic: 4 vectors, quadratic: 2)"
+    return res
 
 
 def batched_leg(reps=5):
@@ -324,6 +370,8 @@ def main():
     df.free(); dg.free()
     if not a.no_whir:
         out["whir_cfg3"] = whir_leg(ctx)
+    if not a.no_cosc:
+        out["cosumcheck_share"] = cosumcheck_leg(ctx)
     if not a.no_batch:
         out["batched_cfg4"] = batched_leg()
     if not a.no_serve:

@@ -48,6 +48,7 @@ constexpr int BSC_SMALL_MAX = 32;
 constexpr int BSC_D_MAX = 2048;
 constexpr int BSC3_TILE_LOG = 9;        // cubic form: elements of each vector per tile, d <= 512
 constexpr int BSC3_D_MAX = 1024;
+constexpr int BSC3_SMALL_MAX = 16;      // cubic form: register transforms up to this d
 constexpr int BSC_MAX_WG = 1024;
 constexpr int CO_THREADS = 256;
 constexpr int MF_SMALL_MAX = 32;
@@ -235,21 +236,22 @@ __global__ __launch_bounds__(CO_THREADS) void bsc_large_kernel(const fe *__restr
     }
 }
 
-// column sums over the workgroups; 64 columns per workgroup, 4 lanes each.
+// column sums over the workgroups; 16 columns per workgroup, 16 lanes each.
 // Column c = co * d + i is the point w^(ncos i + co) (w of order ncos * d);
 // columns with co > 0 are multiplied by `scale`.
+constexpr int FIN_COLS = 16, FIN_LANES = CO_THREADS / FIN_COLS;
 __global__ __launch_bounds__(CO_THREADS) void bsc_final_kernel(const fe *__restrict__ part, int nwg, int d, int ncos,
                                                                fe scale, fe *__restrict__ out) {
-    __shared__ fe S[4][64];
-    const int cols = ncos * d, cl = threadIdx.x & 63, g = threadIdx.x >> 6;
-    const int c = blockIdx.x * 64 + cl;
+    __shared__ fe S[FIN_LANES][FIN_COLS];
+    const int cols = ncos * d, cl = threadIdx.x % FIN_COLS, g = threadIdx.x / FIN_COLS;
+    const int c = blockIdx.x * FIN_COLS + cl;
     fe s = gf::zero();
     if (c < cols)
-        for (int w = g; w < nwg; w += 4) s = gf::add(s, part[(uint64_t)w * cols + c]);
+        for (int w = g; w < nwg; w += FIN_LANES) s = gf::add(s, part[(uint64_t)w * cols + c]);
     S[g][cl] = s;
     __syncthreads();
     if (g == 0 && c < cols) {
-        s = gf::add(gf::add(S[0][cl], S[1][cl]), gf::add(S[2][cl], S[3][cl]));
+        for (int h = 1; h < FIN_LANES; ++h) s = gf::add(s, S[h][cl]);
         const int co = c / d, i = c - co * d;
         out[ncos * i + co] = co ? gf::mul(s, scale) : s;
     }
@@ -361,6 +363,91 @@ __global__ __launch_bounds__(CO_THREADS) void bsc3_kernel(const fe *__restrict__
     }
 }
 
+// Cubic block sum for d <= 16, transforms in registers: a workgroup takes
+// 64 blocks per tile; wave v (0..3: p1, p2, p3, b1) holds vector v of the
+// tile's blocks, one block per lane.  A lane takes its block's inverse
+// transform once, then for each coset s = 1, 2, 3 the d-point transform of
+// the coefficients twisted by w_4d^(s m) / d (compile-time roots); coset 0
+// is the block itself.  Each coset's values meet in LDS, where thread t
+// forms (P1 P2 - P4) P3 for the points k = t mod d of blocks t/d + 256j/d
+// (64 d pairs per coset: for d = 2 threads 128..255 form none).
+template <int D>
+__global__ __launch_bounds__(CO_THREADS) void bsc3_small_kernel(const fe *__restrict__ p1, const fe *__restrict__ p2,
+                                                                const fe *__restrict__ p3, const fe *__restrict__ b1,
+                                                                uint64_t nblocks, fe inv_d, fe *__restrict__ part) {
+    extern __shared__ fe S[];
+    constexpr int RP = D + 1, BT = 64, VS = BT * RP, NP = BT * D, J = (NP + CO_THREADS - 1) / CO_THREADS;
+    __shared__ int live[BT];
+    const int t = threadIdx.x, lane = t & (BT - 1), vec = t / BT;
+    const fe *src = vec == 0 ? p1 : vec == 1 ? p2 : vec == 2 ? p3 : b1;
+    fe *row = S + vec * VS + lane * RP;
+    const int k = t & (D - 1);
+    fe acc[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[c] = gf::zero();
+    // (P1 P2 - P4) P3 at point k of blocks t/D + j * 256/D
+    auto products = [&](fe &a) {
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            const int p = t + CO_THREADS * j;       // (block, point) pair; 128 pairs for d = 2
+            if (p >= NP) break;
+            const int b = p / D;
+            if (live[b]) {
+                const fe *c = S + b * RP + k;
+                a = gf::add(a, gf::mul(gf::sub(gf::mul(c[0], c[VS]), c[2 * VS]), c[3 * VS]));
+            }
+        }
+    };
+    const uint64_t ntiles = (nblocks + BT - 1) / BT;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const uint64_t blk = tile * BT + lane;
+        fe x[D];
+        int nz = 0;
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            x[i] = blk < nblocks ? src[blk * D + i] : gf::zero();
+            nz |= (x[i].re | x[i].im) != 0;
+            row[i] = x[i];
+        }
+        if (vec == 0) live[lane] = nz;     // blocks whose p1 part is zero are skipped
+        __syncthreads();
+        products(acc[0]);
+        __syncthreads();
+        dft::dif<D, true>(x);                                   // x[i] = D c[brev(i)]
+#pragma unroll
+        for (int i = 0; i < D; ++i) x[i] = gf::mul(x[i], inv_d);
+        dft::static_for<1, 4>([&](auto cc) {
+            constexpr int co = decltype(cc)::value;
+            fe y[D];
+            dft::static_for<0, D>([&](auto ii) {
+                constexpr int i = decltype(ii)::value;
+                constexpr int m = dft::brev_c<D>(i);
+                y[m] = gf::mul_root<4 * D, co * m>(x[i]);
+            });
+            dft::dif<D, false>(y);                              // y[i] = P(w_4d^(4 brev(i) + co))
+            dft::static_for<0, D>([&](auto ii) {
+                constexpr int i = decltype(ii)::value;
+                row[dft::brev_c<D>(i)] = y[i];
+            });
+            __syncthreads();
+            products(acc[co]);
+            __syncthreads();
+        });
+    }
+    // column (co, k) adds the threads with t mod D = k (S holds at least
+    // 4 * CO_THREADS elements: the launch sizes it for this as well)
+    static_assert(4 * BT * RP >= 4 * CO_THREADS || D < 4, "launch size covers the reduction");
+#pragma unroll
+    for (int c = 0; c < 4; ++c) S[c * CO_THREADS + t] = acc[c];
+    __syncthreads();
+    if (t < 4 * D) {
+        const int co = t / D, kk = t & (D - 1);
+        fe sum = gf::zero();
+        for (int u = kk; u < CO_THREADS; u += D) sum = gf::add(sum, S[co * CO_THREADS + u]);
+        part[(uint64_t)blockIdx.x * 4 * D + t] = sum;
+    }
+}
+
 // ---- block sumcheck, phase 2 ----------------------------------------------
 
 struct FoldPtrs {
@@ -368,27 +455,31 @@ struct FoldPtrs {
     fe *out[4];
 };
 
-// one lane per output, NA vectors (d <= 64)
-template <int NA>
-__global__ __launch_bounds__(CO_THREADS) void bsc_fold_small_kernel(FoldPtrs P, uint64_t nout, int d,
-                                                                    const fe *__restrict__ L) {
-    __shared__ fe Ls[64];
-    if (threadIdx.x < d) Ls[threadIdx.x] = L[threadIdx.x];
-    __syncthreads();
-    const uint64_t j = (uint64_t)blockIdx.x * CO_THREADS + threadIdx.x;
-    if (j >= nout) return;
-    fe s[NA];
-#pragma unroll
-    for (int a = 0; a < NA; ++a) s[a] = gf::zero();
-    for (int k = 0; k < d; ++k)
-#pragma unroll
-        for (int a = 0; a < NA; ++a) s[a] = gf::add(s[a], gf::mul(Ls[k], P.in[a][j * d + k]));
-#pragma unroll
-    for (int a = 0; a < NA; ++a) P.out[a][j] = s[a];
-}
-
 __device__ __forceinline__ fe shfl_xor_fe(const fe &x, int m) {
     return gf::mk(__shfl_xor(x.re, m), __shfl_xor(x.im, m));
+}
+
+// Phase 2 for d <= 64 over NA vectors: lane l of a d-lane group takes
+// element k = l mod d of one output (loads coalesced across the wave), and
+// the group adds its d products by butterfly shuffles.
+template <int NA>
+__global__ __launch_bounds__(CO_THREADS) void bsc_fold_seg_kernel(FoldPtrs P, uint64_t total, int logd,
+                                                                  const fe *__restrict__ L) {
+    const int d = 1 << logd;
+    const fe l = L[threadIdx.x & (d - 1)];
+    const uint64_t step = (uint64_t)gridDim.x * CO_THREADS;
+    for (uint64_t e = (uint64_t)blockIdx.x * CO_THREADS + threadIdx.x; e < total; e += step) {
+        fe s[NA];
+#pragma unroll
+        for (int a = 0; a < NA; ++a) s[a] = gf::mul(l, P.in[a][e]);
+        for (int m = d >> 1; m >= 1; m >>= 1)
+#pragma unroll
+            for (int a = 0; a < NA; ++a)
+                s[a] = gf::add(s[a], gf::mk(__shfl_xor(s[a].re, m), __shfl_xor(s[a].im, m)));
+        if ((e & (d - 1)) == 0)
+#pragma unroll
+            for (int a = 0; a < NA; ++a) P.out[a][e >> logd] = s[a];
+    }
 }
 
 // one wave per output (d >= 128): lane l takes k = l + 64 q
@@ -613,6 +704,8 @@ int copcs_init(cosnark_ctx *) {
                             large_lds) != hipSuccess ||
         hipFuncSetAttribute((const void *)multifold_large_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                             mf_lds) != hipSuccess ||
+        hipFuncSetAttribute((const void *)bsc3_small_kernel<16>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)(sizeof(fe) * 4 * 64 * 17)) != hipSuccess ||
         hipFuncSetAttribute((const void *)bsc3_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)(sizeof(fe) * 5 * BSC3_D_MAX + sizeof(int) * (1 << BSC3_TILE_LOG))) != hipSuccess)
         return COSNARK_E_HIP;
@@ -642,9 +735,11 @@ static int block_fold(cosnark_ctx *ctx, uint64_t *const *v, int na, uint64_t M, 
     hipStream_t s = ctx->stream;
     HIPCK(hipMemcpyAsync(Ld, ctx->co_host, sizeof(fe) * d, hipMemcpyHostToDevice, s));
     if (d <= 64) {
-        const dim3 g((unsigned)((nout + CO_THREADS - 1) / CO_THREADS));
-        if (na == 2) hipLaunchKernelGGL(bsc_fold_small_kernel<2>, g, dim3(CO_THREADS), 0, s, P, nout, d, (const fe *)Ld);
-        else hipLaunchKernelGGL(bsc_fold_small_kernel<4>, g, dim3(CO_THREADS), 0, s, P, nout, d, (const fe *)Ld);
+        const uint64_t g = (M + CO_THREADS - 1) / CO_THREADS;
+        const dim3 grid((unsigned)(g < 16384 ? g : 16384));
+        const int logd = ilog2((uint64_t)d);
+        if (na == 2) hipLaunchKernelGGL(bsc_fold_seg_kernel<2>, grid, dim3(CO_THREADS), 0, s, P, M, logd, (const fe *)Ld);
+        else hipLaunchKernelGGL(bsc_fold_seg_kernel<4>, grid, dim3(CO_THREADS), 0, s, P, M, logd, (const fe *)Ld);
     } else {
         const uint64_t wg = (nout + 3) / 4;
         const dim3 g((unsigned)(wg < 4096 ? wg : 4096));
@@ -768,7 +863,7 @@ int cosnark_cosumcheck_phase1(cosnark_ctx *ctx, const uint64_t *d_p1, const uint
     }
     LAUNCHCK();
     const fe di = gf::inv(gf::from_u64((uint64_t)d));
-    hipLaunchKernelGGL(bsc_final_kernel, dim3((cols + 63) / 64), dim3(CO_THREADS), 0, s, (const fe *)part, nwg, d, 2,
+    hipLaunchKernelGGL(bsc_final_kernel, dim3((cols + FIN_COLS - 1) / FIN_COLS), dim3(CO_THREADS), 0, s, (const fe *)part, nwg, d, 2,
                        gf::mul(di, di), out);
     LAUNCHCK();
     HIPCK(hipMemcpyAsync(ctx->co_host, out, sizeof(fe) * cols, hipMemcpyDeviceToHost, s));
@@ -802,21 +897,33 @@ int cosnark_cosumcheck3_phase1(cosnark_ctx *ctx, const uint64_t *d_p1, const uin
         return set_err(ctx, COSNARK_E_ARG, "cosumcheck3_phase1: d and M must be powers of two, 2 <= d <= M");
     if (d > BSC3_D_MAX) return set_err(ctx, COSNARK_E_RANGE, "cosumcheck3_phase1: d > 1024");
     const int logd = ilog2((uint64_t)d), logT = std::max(logd, BSC3_TILE_LOG);
-    const uint64_t ntiles = (M + (1ull << logT) - 1) >> logT;
-    const int nwg = (int)(ntiles < 512 ? ntiles : 512), cols = 4 * d;
+    const uint64_t ntiles = d <= BSC3_SMALL_MAX ? (M / d + 63) / 64 : (M + (1ull << logT) - 1) >> logT;
+    const int nwg = (int)(ntiles < 1024 ? ntiles : 1024), cols = 4 * d;
     void *base;
     int rc = scratch_reserve(ctx, sizeof(fe) * ((size_t)nwg * cols + cols), &base);
     if (rc) return rc;
     fe *part = (fe *)base, *out = part + (size_t)nwg * cols;
     hipStream_t s = ctx->stream;
     const fe di = gf::inv(gf::from_u64((uint64_t)d));
-    const size_t lds = sizeof(fe) * 5 * ((size_t)1 << logT) + sizeof(int) * ((size_t)1 << (logT - logd));
-    hipLaunchKernelGGL(bsc3_kernel, dim3(nwg), dim3(CO_THREADS), lds, s,
-                       (const fe *)d_p1, (const fe *)d_p2, (const fe *)d_p3, (const fe *)d_b1, M, logd, logT,
-                       (const fe *)ctx->tw_lo, (const fe *)ctx->tw_hi, (const fe *)ctx->itw_lo,
-                       (const fe *)ctx->itw_hi, di, part);
+    const fe *a1 = (const fe *)d_p1, *a2 = (const fe *)d_p2, *a3 = (const fe *)d_p3, *a4 = (const fe *)d_b1;
+    if (d <= BSC3_SMALL_MAX) {
+        const uint64_t nb = M / d;
+        // the tile's rows, and the four per-thread coset sums at the end
+        const size_t lds = sizeof(fe) * std::max<size_t>(4 * 64 * (d + 1), 4 * CO_THREADS);
+        switch (d) {
+            case 2: hipLaunchKernelGGL(bsc3_small_kernel<2>, dim3(nwg), dim3(CO_THREADS), lds, s, a1, a2, a3, a4, nb, di, part); break;
+            case 4: hipLaunchKernelGGL(bsc3_small_kernel<4>, dim3(nwg), dim3(CO_THREADS), lds, s, a1, a2, a3, a4, nb, di, part); break;
+            case 8: hipLaunchKernelGGL(bsc3_small_kernel<8>, dim3(nwg), dim3(CO_THREADS), lds, s, a1, a2, a3, a4, nb, di, part); break;
+            default: hipLaunchKernelGGL(bsc3_small_kernel<16>, dim3(nwg), dim3(CO_THREADS), lds, s, a1, a2, a3, a4, nb, di, part); break;
+        }
+    } else {
+        const size_t lds = sizeof(fe) * 5 * ((size_t)1 << logT) + sizeof(int) * ((size_t)1 << (logT - logd));
+        hipLaunchKernelGGL(bsc3_kernel, dim3(nwg), dim3(CO_THREADS), lds, s, a1, a2, a3, a4, M, logd, logT,
+                           (const fe *)ctx->tw_lo, (const fe *)ctx->tw_hi, (const fe *)ctx->itw_lo,
+                           (const fe *)ctx->itw_hi, di, part);
+    }
     LAUNCHCK();
-    hipLaunchKernelGGL(bsc_final_kernel, dim3((cols + 63) / 64), dim3(CO_THREADS), 0, s, (const fe *)part, nwg, d, 4,
+    hipLaunchKernelGGL(bsc_final_kernel, dim3((cols + FIN_COLS - 1) / FIN_COLS), dim3(CO_THREADS), 0, s, (const fe *)part, nwg, d, 4,
                        gf::one(), out);
     LAUNCHCK();
     HIPCK(hipMemcpyAsync(ctx->co_host, out, sizeof(fe) * cols, hipMemcpyDeviceToHost, s));
