@@ -165,6 +165,40 @@ int cosnark_whir_open(cosnark_ctx *ctx, const uint64_t *d_arr, uint64_t n, const
                       const uint64_t K[2], int rate, int stop_log, const uint64_t *d_codeword,
                       const uint8_t *d_nodes, uint64_t *h_record, uint64_t record_cap,
                       uint64_t *words);
+/* get_whir_rates (src/Whir.cpp:18-137) without its printing: for arrays of
+ * sizes[0..count) sorted largest first, the rate rates[i] each is committed
+ * at so that Whir_Batch_Open meets it with a codeword of the running size;
+ * the first is `rate`.  Host only.  COSNARK_E_ARG for unsorted sizes. */
+int cosnark_whir_rates(const uint64_t *sizes, int count, int rate, int whir_k, int *rates);
+/* Whir_Batch_Open (src/Whir.cpp:705-899): one opening for count arrays of
+ * sizes sorted largest first.  d_arrays[i], d_betas[i] (sizes[i] elements:
+ * the array and its eq weights), d_codewords[i] (cw_sizes[i] elements, the
+ * encoding at cosnark_whir_rates' rate) and d_nodes[i] (its dup-left tree as
+ * cosnark_whir_commit writes it) are device pointers; h_evals holds the
+ * claims (only entry 0 is read: the reference recomputes the others by
+ * aggregate_arrays, :672-703).  The fold schedule (:731-755) runs Whir_Iteration
+ * with fold factor kappa, and where an array size is met, aggregate_arrays
+ * and aggregate_commitments (:590-645) merge the arrays of that size into the
+ * running one; aggregate_commitments leaves a tree over all-zero leaf digests
+ * (the reference never fills its leaf vector), which the next iteration's
+ * queries open.  After an aggregation at the last scheduled step the
+ * reference reads past its kappa list (:827); the last kappa is used here.
+ * Challenges come from libc random()/rand() in the reference's order
+ * (seed first).  The record (u64 words): [steps], per step [kind]: 0 an
+ * iteration: kappa, log2(kappa) x (a,b,c), the log2(kappa) challenges, new
+ * root, l, l positions, l x kappa replies, l folds, l auth paths, the
+ * update_betas challenge; 1 an aggregation: cnt, challenges1[cnt],
+ * challenges2[cnt] (entry 0 = 1), evals[cnt x cnt] = <arr_i, b_j>, K, l,
+ * l positions, l auth paths on the running tree, the new tree's root; then
+ * the final arr, K, the final root (Whir_Commit over zero digests) and the
+ * closing challenges.  COSNARK_E_CHECK when one of the reference's checks
+ * fails (a round, the closing K = <arr, b>, the closing codeword = RS(arr))
+ * or where the reference exits (a query beyond the tree, codeword sizes
+ * that differ at an aggregation).  whir_k a power of two in [2, 64]. */
+int cosnark_whir_batch_open(cosnark_ctx *ctx, int count, const uint64_t *const *d_arrays, const uint64_t *sizes,
+                            const uint64_t *const *d_codewords, const uint64_t *cw_sizes,
+                            const uint8_t *const *d_nodes, const uint64_t *h_evals, const uint64_t *const *d_betas,
+                            int rate, int whir_k, uint64_t *h_record, uint64_t record_cap, uint64_t *words);
 
 /* ---- coPCS rank-local kernels --------------------------------------------
  * Encode, src/coPCS.cpp:131-147: each of the nrows rows of L elements

@@ -546,135 +546,378 @@ static void rec_bytes32(rec_t *R, const uint8_t *h) { uint64_t w[4]; memcpy(w, h
  *     rb     update_betas challenge 2 words
  *   final: arr (4 elements, 8 words), K (2 words), final root (4 words),
  *          final r (log2|arr| elements)
+ * With a general fold factor kappa an iteration holds log2(kappa) polys and
+ * challenges and kappa replies per query (or_whir_batch_open's record).
  */
+
+/* A WHIR prover's running state: arr and its weights b (n entries), the
+ * codeword (cw_n entries) and the tree it is opened against: the dup-left
+ * nodes mt, or, after aggregate_commitments, a tree over all-zero leaf
+ * digests (mt NULL) with mt_leaves leaves. */
+typedef struct {
+    or_fe *arr, *b;
+    uint64_t n;
+    or_fe *cw;
+    uint64_t cw_n;
+    uint8_t *mt;
+    uint64_t mt_leaves;
+    or_fe K;
+    int rate;
+} whir_state;
+
+/* node of level l of a dup-left tree over zero digests: the l-fold
+ * self-hash of the zero digest (levels 0..levels) */
+static void zero_chain(int levels, uint8_t *out) {
+    uint8_t d[64];
+    memset(out, 0, 32);
+    for (int l = 1; l <= levels; ++l) {
+        memcpy(d, out + 32 * (l - 1), 32);
+        memcpy(d + 32, out + 32 * (l - 1), 32);
+        or_blake3_hash64(d, out + 32 * l);
+    }
+}
+
+/* open_tree_blake (merkle_tree.cpp:320-335) on the state's tree; -1 where
+ * the reference exits (position beyond the leaves, :321-330) */
+static int state_path(const whir_state *S, uint64_t pos, uint8_t *path) {
+    if (pos >= S->mt_leaves) return -1;
+    if (S->mt) return or_merkle_open(S->mt, S->cw_n, pos, path);
+    int d = ilog2u(S->mt_leaves);
+    uint8_t *z = (uint8_t *)malloc(32 * (d + 1));
+    zero_chain(d, z);
+    memcpy(path, z, 32 * (size_t)d);
+    free(z);
+    return d;
+}
+
+static int fe_eq(or_fe a, or_fe b) { return a.real == b.real && a.img == b.img; }
+
+/* Whir_Iteration (Whir.cpp:354-386) with whir_k = kappa; 0, or a negative
+ * status where the reference reports an error */
+static int whir_iteration(whir_state *S, int kappa, rec_t *R, int iter) {
+    or_fe zero = { 0, 0 }, one = { 1, 0 };
+    int k = ilog2u((uint64_t)kappa);
+    or_fe rr[32];
+    for (int rd = 0; rd < k; ++rd) {                       /* whir_sumcheck :233-249 */
+        uint64_t L = S->n / 2;
+        or_fe p[3];
+        round_poly(S->arr, S->b, L, p);
+        if (!fe_eq(or_add(quad_eval(p, zero), quad_eval(p, one)), S->K)) return -10 - iter;
+        rr[rd] = or_random_element();
+        S->K = quad_eval(p, rr[rd]);
+        fold(S->arr, L, rr[rd]);
+        fold(S->b, L, rr[rd]);
+        S->n = L;
+        rec_fe(R, p[0]); rec_fe(R, p[1]); rec_fe(R, p[2]);
+    }
+    for (int rd = 0; rd < k; ++rd) rec_fe(R, rr[rd]);
+
+    /* Whir_Commit_initial at 2*rate (:158-190) */
+    uint64_t ncw_n = S->n * (uint64_t)(2 * S->rate);
+    or_fe *ncw = (or_fe *)malloc(sizeof(or_fe) * ncw_n);
+    or_rs_encode(S->arr, S->n, 2 * S->rate, ncw);
+    uint64_t nnodes = or_merkle_node_count(ncw_n);
+    uint8_t *nmt = (uint8_t *)malloc(32 * nnodes);
+    or_merkle_commit(ncw, ncw_n, nmt, 1);
+    rec_bytes32(R, nmt + 32 * (nnodes - 1));
+
+    /* whir_query (:251-314) on the current codeword */
+    int l = or_whir_query_count(S->rate);
+    rec_u64(R, (uint64_t)l);
+    uint64_t *q = (uint64_t *)malloc(sizeof(uint64_t) * l);
+    uint64_t qmod = kappa > 4 ? S->cw_n / kappa : S->cw_n / 4;   /* :262-267 */
+    for (int i = 0; i < l; ++i) {
+        q[i] = (uint64_t)(rand() % (int)qmod);
+        rec_u64(R, q[i]);
+    }
+    or_fe *y = (or_fe *)malloc(sizeof(or_fe) * l);
+    or_fe *rep = (or_fe *)malloc(sizeof(or_fe) * kappa);
+    for (int i = 0; i < l; ++i) {
+        for (int j = 0; j < kappa; ++j) {
+            rep[j] = S->cw[q[i] + (uint64_t)j * S->cw_n / kappa];
+            rec_fe(R, rep[j]);
+        }
+        y[i] = or_rs_multifold_segment_tensor(rep, kappa, rr, k, S->cw_n, q[i]);
+    }
+    free(rep);
+    for (int i = 0; i < l; ++i) rec_fe(R, y[i]);
+    int status = 0;
+    {
+        uint8_t path[32 * 64];
+        for (int i = 0; i < l && !status; ++i) {
+            int d = state_path(S, q[i], path);
+            if (d < 0) { status = -20 - iter; break; }
+            for (int t = 0; t < d; ++t) rec_bytes32(R, path + 32 * t);
+        }
+    }
+    if (!status) {
+        uint64_t scale = (uint64_t)kappa * ncw_n / S->cw_n;      /* :373-374 */
+        for (int i = 0; i < l; ++i) q[i] *= scale;
+        or_fe rb = or_random_element();                          /* update_betas :325-342 */
+        rec_fe(R, rb);
+        or_fe ch = rb;
+        or_fe omega = or_root_of_unity(ilog2u(ncw_n));
+        for (int i = 0; i < l; ++i) {
+            if (i > 0) ch = or_mul(ch, rb);
+            or_fe w = or_pow(omega, q[i]), pw = one;
+            for (uint64_t j = 0; j < S->n; ++j) {
+                S->b[j] = or_add(S->b[j], or_mul(ch, pw));
+                pw = or_mul(pw, w);
+            }
+            S->K = or_add(S->K, or_mul(ch, y[i]));               /* :383-384 */
+        }
+    }
+    free(q); free(y);
+    free(S->cw); free(S->mt);
+    S->cw = ncw; S->mt = nmt; S->cw_n = ncw_n; S->mt_leaves = ncw_n / 4;
+    return status;
+}
+
+/* The closing steps of Whir_Open / Whir_Batch_Open (Whir.cpp:425-449,
+ * :848-873) with whir_k = kappa: arr, K, the root of Whir_Commit (:192-213,
+ * a tree over |arr|*rate/kappa all-zero leaf digests), and the final
+ * challenges; verification steps 1 (the codeword is arr's encoding) and 2
+ * (K = <arr, b>) */
+static int whir_final(whir_state *S, int kappa, rec_t *R) {
+    for (uint64_t i = 0; i < S->n; ++i) rec_fe(R, S->arr[i]);
+    rec_fe(R, S->K);
+    {
+        uint64_t leaves = S->n * (uint64_t)S->rate / kappa;
+        int d = ilog2u(leaves);
+        uint8_t *z = (uint8_t *)malloc(32 * (d + 1));
+        zero_chain(d, z);
+        rec_bytes32(R, z + 32 * d);
+        free(z);
+    }
+    {
+        uint64_t ncw_n = S->n * (uint64_t)S->rate;
+        or_fe *ncw = (or_fe *)malloc(sizeof(or_fe) * ncw_n);
+        or_rs_encode(S->arr, S->n, S->rate, ncw);
+        int same = ncw_n == S->cw_n && !memcmp(ncw, S->cw, sizeof(or_fe) * ncw_n);
+        free(ncw);
+        if (!same) return -3;
+    }
+    or_fe Kc = S->K;
+    for (uint64_t i = 0; i < S->n; ++i) Kc = or_sub(Kc, or_mul(S->arr[i], S->b[i]));
+    if (Kc.real != 0 || Kc.img != 0) return -2;
+    for (int i = 0; i < ilog2u(S->n); ++i) rec_fe(R, or_random_element());
+    return 0;
+}
+
+static void state_free(whir_state *S) { free(S->arr); free(S->b); free(S->cw); free(S->mt); }
+
 int64_t or_whir_open(const or_fe *arr_in, uint64_t n, const or_fe *z, or_fe K,
                      int rate, int stop_log, uint64_t *record, uint64_t record_cap) {
     rec_t R = { record, 0, record_cap, 0 };
-    int m = ilog2u(n);
-    or_fe zero = { 0, 0 }, one = { 1, 0 };
     uint64_t stop = WHIR_K;
     if (stop_log > 0 && (1ULL << stop_log) > stop) stop = 1ULL << stop_log;
-
-    or_fe *arr = (or_fe *)malloc(sizeof(or_fe) * n);
-    or_fe *b = (or_fe *)malloc(sizeof(or_fe) * n);
-    memcpy(arr, arr_in, sizeof(or_fe) * n);
-    or_precompute_beta(z, m, b);                           /* Whir.cpp:395 */
-
-    /* initial commitment (Whir_Commit_initial with the caller's rate) */
-    uint64_t cw_n = n * (uint64_t)rate;
-    or_fe *cw = (or_fe *)malloc(sizeof(or_fe) * cw_n);
-    or_rs_encode(arr, n, rate, cw);
-    uint8_t *mt = (uint8_t *)malloc(32 * or_merkle_node_count(cw_n));
-    or_merkle_commit(cw, cw_n, mt, 1);
+    whir_state S;
+    S.n = n;
+    S.arr = (or_fe *)malloc(sizeof(or_fe) * n);
+    S.b = (or_fe *)malloc(sizeof(or_fe) * n);
+    memcpy(S.arr, arr_in, sizeof(or_fe) * n);
+    or_precompute_beta(z, ilog2u(n), S.b);                 /* Whir.cpp:395 */
+    S.K = K;
+    S.rate = rate;
+    /* the initial commitment (Whir_Commit_initial with the caller's rate) */
+    S.cw_n = n * (uint64_t)rate;
+    S.cw = (or_fe *)malloc(sizeof(or_fe) * S.cw_n);
+    or_rs_encode(S.arr, n, rate, S.cw);
+    S.mt = (uint8_t *)malloc(32 * or_merkle_node_count(S.cw_n));
+    or_merkle_commit(S.cw, S.cw_n, S.mt, 1);
+    S.mt_leaves = S.cw_n / 4;
 
     uint64_t iters_pos = R.n;
     rec_u64(&R, 0);
     uint64_t iters = 0;
-    uint64_t cur_n = n;
     int status = 0;
-    while (1) {
-        /* ---- Whir_Iteration (Whir.cpp:354-386) ---- */
-        or_fe rr[2];
-        for (int rd = 0; rd < 2; ++rd) {                   /* whir_sumcheck :233-249 */
-            uint64_t L = cur_n / 2;
-            or_fe p[3];
-            round_poly(arr, b, L, p);
-            or_fe s01 = or_add(quad_eval(p, zero), quad_eval(p, one));
-            if (s01.real != K.real || s01.img != K.img) { status = -10 - (int)iters; goto done; }
-            rr[rd] = or_random_element();
-            K = quad_eval(p, rr[rd]);
-            fold(arr, L, rr[rd]);
-            fold(b, L, rr[rd]);
-            cur_n = L;
-            rec_fe(&R, p[0]); rec_fe(&R, p[1]); rec_fe(&R, p[2]);
-        }
-        rec_fe(&R, rr[0]); rec_fe(&R, rr[1]);
-
-        /* new commitment at 2*rate (Whir_Commit_initial :158-190) */
-        uint64_t ncw_n = cur_n * (uint64_t)(2 * rate);
-        or_fe *ncw = (or_fe *)malloc(sizeof(or_fe) * ncw_n);
-        or_rs_encode(arr, cur_n, 2 * rate, ncw);
-        uint64_t nnodes = or_merkle_node_count(ncw_n);
-        uint8_t *nmt = (uint8_t *)malloc(32 * nnodes);
-        or_merkle_commit(ncw, ncw_n, nmt, 1);
-        rec_bytes32(&R, nmt + 32 * (nnodes - 1));
-
-        /* whir_query (:251-314) on the previous codeword */
-        int l = or_whir_query_count(rate);
-        rec_u64(&R, (uint64_t)l);
-        uint64_t *q = (uint64_t *)malloc(sizeof(uint64_t) * l);
-        for (int i = 0; i < l; ++i) {
-            q[i] = (uint64_t)(rand() % (int)(cw_n / 4));  /* whir_k <= 4 branch */
-            rec_u64(&R, q[i]);
-        }
-        or_fe *y = (or_fe *)malloc(sizeof(or_fe) * l);
-        for (int i = 0; i < l; ++i) {
-            or_fe rep[WHIR_K];
-            for (int j = 0; j < WHIR_K; ++j) {
-                rep[j] = cw[q[i] + (uint64_t)j * cw_n / WHIR_K];
-                rec_fe(&R, rep[j]);
-            }
-            y[i] = or_rs_multifold_segment_tensor(rep, WHIR_K, rr, 2, cw_n, q[i]);
-        }
-        for (int i = 0; i < l; ++i) rec_fe(&R, y[i]);
-        {
-            uint8_t path[32 * 64];
-            for (int i = 0; i < l; ++i) {
-                int d = or_merkle_open(mt, cw_n, q[i], path);
-                for (int k = 0; k < d; ++k) rec_bytes32(&R, path + 32 * k);
-            }
-        }
-        /* scale the queries to the new codeword (:373-374) */
-        uint64_t scale = (uint64_t)WHIR_K * ncw_n / cw_n;
-        for (int i = 0; i < l; ++i) q[i] *= scale;
-
-        /* update_betas (:325-342) */
-        or_fe rb = or_random_element();
-        rec_fe(&R, rb);
-        or_fe ch = rb;
-        int logncw = ilog2u(ncw_n);
-        or_fe omega = or_root_of_unity(logncw);
-        for (int i = 0; i < l; ++i) {
-            if (i > 0) ch = or_mul(ch, rb);
-            or_fe w = or_pow(omega, q[i]), pw = one;
-            for (uint64_t j = 0; j < cur_n; ++j) {
-                b[j] = or_add(b[j], or_mul(ch, pw));
-                pw = or_mul(pw, w);
-            }
-            K = or_add(K, or_mul(ch, y[i]));
-        }
-        free(q); free(y);
-        free(cw); free(mt);
-        cw = ncw; mt = nmt; cw_n = ncw_n;
+    while (1) {                                            /* :413-422 */
+        status = whir_iteration(&S, WHIR_K, &R, (int)iters);
+        if (status) break;
         ++iters;
-        rate *= 2;
-        if (cur_n <= stop) break;
+        S.rate *= 2;
+        if (S.n <= stop) break;
     }
-    record[iters_pos] = iters;
+    if (iters_pos < record_cap) record[iters_pos] = iters;
+    if (!status) status = whir_final(&S, WHIR_K, &R);
+    state_free(&S);
+    if (status) return status;
+    if (R.overflow) return -1;
+    return (int64_t)R.n;
+}
 
-    /* final steps of Whir_Open (:425-466) */
-    for (uint64_t i = 0; i < cur_n; ++i) rec_fe(&R, arr[i]);
-    rec_fe(&R, K);
-    {
-        /* Whir_Commit (:192-213) hashes an all-zero leaf vector of
-         * cw/whir_k digests: the root depends only on the size */
-        uint64_t leaves = cur_n * (uint64_t)rate / WHIR_K;
-        uint8_t cur[32], data[64];
-        memset(cur, 0, 32);
-        for (uint64_t s = leaves / 2; s >= 1; s /= 2) {
-            memcpy(data, cur, 32); memcpy(data + 32, cur, 32);
-            or_blake3_hash64(data, cur);
+void or_whir_rates(const uint64_t *sizes, int count, int rate, int whir_k, int *rates) {
+    /* get_whir_rates (Whir.cpp:18-137) for sizes already sorted largest
+     * first: the rate each array is committed at */
+    int w = 0, i = 1;
+    uint64_t size = sizes[0];
+    rates[w++] = rate;
+    while (i < count && sizes[i] == size) { rates[w++] = rate; ++i; }      /* :43-51 */
+    while (i < count) {                                                    /* :53-73 */
+        if (size / (uint64_t)whir_k > sizes[i]) {
+            size /= (uint64_t)whir_k;
+            rate *= 2;
+        } else {
+            size = sizes[i];
+            rate *= 2;
+            rates[w++] = rate;
+            uint64_t s2 = sizes[i++];
+            while (i < count && sizes[i] == s2) { rates[w++] = rate; ++i; }
         }
-        rec_bytes32(&R, cur);
     }
-    {
-        or_fe Kc = K;
-        for (uint64_t i = 0; i < cur_n; ++i) Kc = or_sub(Kc, or_mul(arr[i], b[i]));
-        if (Kc.real != 0 || Kc.img != 0) { status = -2; goto done; }
+}
+
+int or_whir_kappas(const uint64_t *sizes, int count, int whir_k, int *kappas, int *aggr, int cap) {
+    /* the fold schedule of Whir_Batch_Open (Whir.cpp:731-755) */
+    int nk = 0, i = 1;
+    uint64_t size = sizes[0];
+    for (int t = 1; t < count; ++t)
+        if (sizes[t] > sizes[t - 1] || sizes[t] == 0) return -2;      /* not largest first */
+    while (1) {
+        if (nk >= cap) return -1;
+        if (i >= count || size / (uint64_t)whir_k > sizes[i]) {
+            kappas[nk] = whir_k;
+            size /= (uint64_t)whir_k;
+            aggr[nk++] = 0;
+        } else {
+            kappas[nk] = (int)(size / sizes[i]);
+            uint64_t s2 = sizes[i];
+            while (i < count && sizes[i] == s2) ++i;
+            size /= (uint64_t)kappas[nk];
+            aggr[nk++] = 1;
+        }
+        if (size <= (uint64_t)whir_k) break;
     }
-    for (int i = 0; i < ilog2u(cur_n); ++i) rec_fe(&R, or_random_element());
-done:
-    free(arr); free(b); free(cw); free(mt);
+    return nk;
+}
+
+/*
+ * Whir_Batch_Open record (u64 words):
+ *   [n_steps]
+ *   per step [kind]:
+ *     0 iteration:   kappa, then the iteration as above with log2(kappa)
+ *                    polys and challenges and kappa replies per query
+ *     1 aggregation: cnt (running array + cnt-1 arrays of its size),
+ *                    challenge1[cnt], challenge2[cnt] (entry 0 set to 1),
+ *                    evals[cnt x cnt] (<arr_i, b_j>), K, l, q[l], l paths
+ *                    on the running tree, root of the new (zero-digest) tree
+ *   final as in Whir_Open, with whir_k as it stands at the end.
+ */
+int64_t or_whir_batch_open(int count, const or_fe *const *arrays, const uint64_t *sizes,
+                           const or_fe *const *codewords, const uint64_t *cw_sizes, const uint8_t *const *trees,
+                           const or_fe *evals, const or_fe *const *betas, int rate, int whir_k,
+                           uint64_t *record, uint64_t record_cap) {
+    rec_t R = { record, 0, record_cap, 0 };
+    int kap[64], agg[64];
+    int nk = or_whir_kappas(sizes, count, whir_k, kap, agg, 64);
+    if (nk < 0) return -4;
+    whir_state S;
+    S.n = sizes[0];
+    S.arr = (or_fe *)malloc(sizeof(or_fe) * S.n);
+    S.b = (or_fe *)malloc(sizeof(or_fe) * S.n);
+    memcpy(S.arr, arrays[0], sizeof(or_fe) * S.n);
+    memcpy(S.b, betas[0], sizeof(or_fe) * S.n);
+    S.cw_n = cw_sizes[0];
+    S.cw = (or_fe *)malloc(sizeof(or_fe) * S.cw_n);
+    memcpy(S.cw, codewords[0], sizeof(or_fe) * S.cw_n);
+    S.mt = (uint8_t *)malloc(32 * or_merkle_node_count(S.cw_n));
+    memcpy(S.mt, trees[0], 32 * or_merkle_node_count(S.cw_n));
+    S.mt_leaves = S.cw_n / 4;
+    S.K = evals[0];                                        /* :714 */
+    S.rate = rate;
+
+    uint64_t steps_pos = R.n;
+    rec_u64(&R, 0);
+    uint64_t steps = 0;
+    int status = 0, iter = 0, ri = 1, wk = whir_k;
+    while (1) {                                            /* :762-843 */
+        wk = kap[iter];
+        if (!agg[iter] || wk != 1) {
+            rec_u64(&R, 0); rec_u64(&R, (uint64_t)wk); ++steps;
+            if ((status = whir_iteration(&S, wk, &R, iter))) break;
+            S.rate *= 2;
+        }
+        ++iter;
+        if (agg[iter - 1]) {
+            /* arrays of the running array's size join it (:786-804) */
+            int cnt = 1;
+            while (ri + cnt - 1 < count && sizes[ri + cnt - 1] == S.n) ++cnt;
+            for (int j = ri + cnt - 1; j < count; ++j)
+                if (sizes[j] == S.n) { status = -5; break; }      /* a later equal size: input not sorted */
+            if (status) break;
+            rec_u64(&R, 1); rec_u64(&R, (uint64_t)cnt); ++steps;
+            or_fe *c1 = (or_fe *)malloc(sizeof(or_fe) * cnt), *c2 = (or_fe *)malloc(sizeof(or_fe) * cnt);
+            for (int i = 0; i < cnt; ++i) c1[i] = or_random_element();    /* :809-812 */
+            for (int i = 0; i < cnt; ++i) c2[i] = or_random_element();
+            c1[0].real = 1; c1[0].img = 0; c2[0].real = 1; c2[0].img = 0;
+            for (int i = 0; i < cnt; ++i) rec_fe(&R, c1[i]);
+            for (int i = 0; i < cnt; ++i) rec_fe(&R, c2[i]);
+            /* aggregate_arrays(arr, b, arr_aggr, B_aggr, c2, c1) (:672-703) */
+            const or_fe **A = (const or_fe **)malloc(sizeof(or_fe *) * cnt);
+            const or_fe **Bv = (const or_fe **)malloc(sizeof(or_fe *) * cnt);
+            or_fe *a0 = (or_fe *)malloc(sizeof(or_fe) * S.n), *b0 = (or_fe *)malloc(sizeof(or_fe) * S.n);
+            memcpy(a0, S.arr, sizeof(or_fe) * S.n);
+            memcpy(b0, S.b, sizeof(or_fe) * S.n);
+            A[0] = a0; Bv[0] = b0;
+            for (int i = 1; i < cnt; ++i) { A[i] = arrays[ri + i - 1]; Bv[i] = betas[ri + i - 1]; }
+            or_fe K = { 0, 0 };
+            for (int i = 0; i < cnt; ++i)
+                for (int j = 0; j < cnt; ++j) {
+                    or_fe e = { 0, 0 };
+                    for (uint64_t x = 0; x < S.n; ++x) e = or_add(e, or_mul(A[i][x], Bv[j][x]));
+                    rec_fe(&R, e);
+                    K = or_add(K, or_mul(or_mul(c2[i], c1[j]), e));
+                }
+            for (int i = 1; i < cnt; ++i)
+                for (uint64_t x = 0; x < S.n; ++x) {
+                    S.arr[x] = or_add(S.arr[x], or_mul(c2[i], A[i][x]));
+                    S.b[x] = or_add(S.b[x], or_mul(c1[i], Bv[i][x]));
+                }
+            S.K = K;
+            rec_fe(&R, K);
+            /* the next fold factor (:827-831); past the end of the schedule
+             * the reference reads beyond its vector, here the last entry */
+            wk = iter + 1 < nk ? kap[iter + 1] : kap[nk - 1];
+            if (ri + cnt - 1 < count && S.n / (uint64_t)wk <= sizes[ri + cnt - 1]) wk = (int)(S.n / sizes[ri + cnt - 1]);
+            /* aggregate_commitments(codeword, MT, cw_aggr, MT_aggr, c2, rate') (:590-645) */
+            int rate2 = (int)(S.cw_n / S.n);
+            for (int i = 1; i < cnt && !status; ++i)
+                if (cw_sizes[ri + i - 1] != S.cw_n) status = -6;
+            if (!status) {
+                for (int i = 1; i < cnt; ++i) {
+                    const or_fe *ci = codewords[ri + i - 1];
+                    for (uint64_t x = 0; x < S.cw_n; ++x) S.cw[x] = or_add(S.cw[x], or_mul(c2[i], ci[x]));
+                }
+                int l = or_whir_query_count(rate2);
+                rec_u64(&R, (uint64_t)l);
+                uint64_t *q = (uint64_t *)malloc(sizeof(uint64_t) * l);
+                for (int i = 0; i < l; ++i) { q[i] = (uint64_t)(rand() % (int)(S.cw_n / 4)); rec_u64(&R, q[i]); }
+                uint8_t path[32 * 64];
+                for (int i = 0; i < l && !status; ++i) {
+                    int d = state_path(&S, q[i], path);
+                    if (d < 0) { status = -7; break; }
+                    for (int t = 0; t < d; ++t) rec_bytes32(&R, path + 32 * t);
+                }
+                free(q);
+                free(S.mt);
+                S.mt = NULL;
+                S.mt_leaves = S.cw_n / (uint64_t)wk;
+                int d = ilog2u(S.mt_leaves);
+                uint8_t *z = (uint8_t *)malloc(32 * (d + 1));
+                zero_chain(d, z);
+                rec_bytes32(&R, z + 32 * d);
+                free(z);
+            }
+            ri += cnt - 1;
+            free(c1); free(c2); free(A); free(Bv); free(a0); free(b0);
+            if (status) break;
+        }
+        if (S.n <= (uint64_t)wk) break;
+        if (iter >= nk) { status = -8; break; }
+    }
+    if (steps_pos < record_cap) record[steps_pos] = steps;
+    if (!status) status = whir_final(&S, wk, &R);
+    state_free(&S);
     if (status) return status;
     if (R.overflow) return -1;
     return (int64_t)R.n;

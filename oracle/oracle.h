@@ -105,6 +105,21 @@ or_fe or_rs_multifold_segment_tensor(const or_fe *seg, int seg_n, const or_fe *r
  * Returns the number of u64 words written, or a negative error. */
 int64_t or_whir_open(const or_fe *arr, uint64_t n, const or_fe *z, or_fe K,
                      int rate, int stop_log, uint64_t *record, uint64_t record_cap);
+/* get_whir_rates (src/Whir.cpp:18-137): the rate each array (sizes sorted
+ * largest first) is committed at, starting from `rate`. */
+void or_whir_rates(const uint64_t *sizes, int count, int rate, int whir_k, int *rates);
+/* Whir_Batch_Open's fold schedule (src/Whir.cpp:731-755): kappas[] and
+ * aggr[] per step; returns the step count (-1 beyond cap). */
+int or_whir_kappas(const uint64_t *sizes, int count, int whir_k, int *kappas, int *aggr, int cap);
+/* Whir_Batch_Open (src/Whir.cpp:705-899) with the prover's record; arrays
+ * sorted largest first, each with its codeword (cw_sizes[i] elements, the
+ * rate from or_whir_rates), its dup-left tree, claim evals[i] and eq
+ * weights betas[i]; rate = codeword 0's.  Returns the word count or a
+ * negative status. */
+int64_t or_whir_batch_open(int count, const or_fe *const *arrays, const uint64_t *sizes,
+                           const or_fe *const *codewords, const uint64_t *cw_sizes, const uint8_t *const *trees,
+                           const or_fe *evals, const or_fe *const *betas, int rate, int whir_k,
+                           uint64_t *record, uint64_t record_cap);
 
 /* ---- coPCS rank-local kernels ---- */
 /* Encode (src/coPCS.cpp:131-147) of nrows rows of L elements each: iNTT_L,

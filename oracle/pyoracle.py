@@ -106,6 +106,11 @@ def oracle():
         L.or_unpack_secret_share.argtypes = [vp, i32, i32, vp]
         L.or_compute_secret_shares.argtypes = [vp, u64, i32, i32, i32, vp, vp]
         L.or_sumcheck_aggr.argtypes = [vp, i32, i32, vp, vp]
+        L.or_whir_rates.argtypes = [vp, i32, i32, i32, vp]
+        L.or_whir_kappas.argtypes = [vp, i32, i32, vp, vp, i32]
+        L.or_whir_kappas.restype = i32
+        L.or_whir_batch_open.argtypes = [i32, vp, vp, vp, vp, vp, vp, vp, i32, i32, vp, u64]
+        L.or_whir_batch_open.restype = ctypes.c_int64
         _o = L
     return _o
 
@@ -420,3 +425,57 @@ def sumcheck_aggr(polys, b):
     out = fe_array(ln)
     oracle().or_sumcheck_aggr(_p(P), N, ln, _p(bb), _p(out))
     return out
+
+
+# ---- batched WHIR ---------------------------------------------------------
+
+def whir_rates(sizes, rate, whir_k=4):
+    """get_whir_rates: the commitment rate of each array (sizes largest first)."""
+    sz = np.ascontiguousarray(sizes, dtype=np.uint64)
+    out = np.zeros(len(sizes), np.int32)
+    oracle().or_whir_rates(_p(sz), len(sizes), rate, whir_k, _p(out))
+    return [int(x) for x in out]
+
+
+def whir_kappas(sizes, whir_k=4):
+    """Whir_Batch_Open's fold schedule: (kappas, aggr)."""
+    sz = np.ascontiguousarray(sizes, dtype=np.uint64)
+    k = np.zeros(64, np.int32)
+    a = np.zeros(64, np.int32)
+    nk = oracle().or_whir_kappas(_p(sz), len(sizes), whir_k, _p(k), _p(a), 64)
+    return [int(x) for x in k[:nk]], [bool(x) for x in a[:nk]]
+
+
+def whir_batch_inputs(arrays, zs, rate, whir_k=4):
+    """The prover's inputs to Whir_Batch_Open: arrays (largest first), each
+    committed at its get_whir_rates rate, its dup-left tree, eq weights
+    precompute_beta(z_i) and claim <arr_i, beta_i>."""
+    sizes = [int(np.asarray(a).shape[0]) for a in arrays]
+    rates = whir_rates(sizes, rate, whir_k)
+    cws = [rs_encode(a, r) for a, r in zip(arrays, rates)]
+    trees = [merkle_commit(c) for c in cws]
+    betas = [precompute_beta(z) for z in zs]
+    evals = np.stack([inner_product(a, b) for a, b in zip(arrays, betas)])
+    return {"arrays": [np.ascontiguousarray(a, dtype=np.uint64) for a in arrays], "sizes": sizes, "rates": rates,
+            "codewords": cws, "trees": trees, "betas": betas, "evals": evals, "rate": rate, "whir_k": whir_k}
+
+
+def _ptrs(arrs):
+    return (ctypes.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+
+
+def whir_batch_open(inp, cap=1 << 22):
+    """Whir_Batch_Open record of the oracle (libc random()/rand(): seed first)."""
+    n = len(inp["arrays"])
+    sizes = np.ascontiguousarray(inp["sizes"], dtype=np.uint64)
+    cws = [np.ascontiguousarray(c) for c in inp["codewords"]]
+    cw_sizes = np.ascontiguousarray([c.shape[0] for c in cws], dtype=np.uint64)
+    trees = [np.ascontiguousarray(t) for t in inp["trees"]]
+    betas = [np.ascontiguousarray(b) for b in inp["betas"]]
+    ev = np.ascontiguousarray(inp["evals"], dtype=np.uint64)
+    rec = np.zeros(cap, np.uint64)
+    w = oracle().or_whir_batch_open(n, _ptrs(inp["arrays"]), _p(sizes), _ptrs(cws), _p(cw_sizes), _ptrs(trees),
+                                    _p(ev), _ptrs(betas), inp["rate"], inp["whir_k"], _p(rec), cap)
+    if w < 0:
+        return int(w), None
+    return 0, rec[:w].copy()

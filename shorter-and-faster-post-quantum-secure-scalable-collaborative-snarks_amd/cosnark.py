@@ -82,6 +82,9 @@ def lib():
         L.cosnark_cosumcheck3_fold.argtypes = [vp, vp, vp, vp, vp, u64, i32, vp]
         L.cosnark_cosumcheck3_phase2.argtypes = [vp, vp, vp, vp, vp, u64, i32, vp]
         L.cosnark_share_unpack.argtypes = [vp, i32, i32, vp]
+        L.cosnark_whir_rates.argtypes = [vp, i32, i32, i32, vp]
+        L.cosnark_whir_batch_open.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp, vp, i32, i32, vp, u64,
+                                              ctypes.POINTER(u64)]
         L.cosnark_sumcheck_aggr.argtypes = [vp, i32, i32, vp, vp]
         L.cosnark_rs_multifold5.argtypes = [vp, vp, u64, vp, i32, vp]
         L.cosnark_batch_create.argtypes = [vp, i32, vp, i32, ctypes.POINTER(vp)]
@@ -384,6 +387,31 @@ class Context:
             a.free(); b.free()
         return out
 
+    def whir_batch_open(self, arrays, codewords, trees, evals, betas, rate, whir_k=4, cap=1 << 22):
+        """Whir_Batch_Open (src/Whir.cpp:705-899) over host arrays (largest
+        first) with their codewords, trees (node arrays), claims and eq
+        weights; uploads them, returns (status, record).  Seed libc first."""
+        ups = []
+        def up(x):
+            b = self.upload(np.ascontiguousarray(x))
+            ups.append(b)
+            return b.ptr.value
+        n = len(arrays)
+        pa = (ctypes.c_void_p * n)(*[up(a) for a in arrays])
+        pc = (ctypes.c_void_p * n)(*[up(c) for c in codewords])
+        pt = (ctypes.c_void_p * n)(*[up(t) for t in trees])
+        pb = (ctypes.c_void_p * n)(*[up(b) for b in betas])
+        sizes = np.ascontiguousarray([np.asarray(a).shape[0] for a in arrays], dtype=np.uint64)
+        cws = np.ascontiguousarray([np.asarray(c).shape[0] for c in codewords], dtype=np.uint64)
+        ev = np.ascontiguousarray(evals, dtype=np.uint64)
+        rec = np.zeros(cap, np.uint64)
+        words = ctypes.c_uint64(0)
+        rc = lib().cosnark_whir_batch_open(self.h, n, pa, _p(sizes), pc, _p(cws), pt, _p(ev), pb, rate, whir_k,
+                                           _p(rec), cap, ctypes.byref(words))
+        for b in ups:
+            b.free()
+        return rc, rec[:words.value].copy()
+
     def cosumcheck3_phase1(self, p1, p2, p3, b1, d, dev=None):
         """Cubic block sumcheck phase 1: the 4d-element round polynomial (4d, 2).
         dev: (four device buffers, M) already resident."""
@@ -609,3 +637,13 @@ def sumcheck_aggr(polys, b):
     if rc:
         raise CosnarkError(rc, "sumcheck_aggr: bad sizes")
     return out
+
+
+def whir_rates(sizes, rate, whir_k=4):
+    """get_whir_rates (src/Whir.cpp:18-137): commitment rates, sizes largest first."""
+    sz = np.ascontiguousarray(sizes, dtype=np.uint64)
+    out = np.zeros(len(sizes), np.int32)
+    rc = lib().cosnark_whir_rates(_p(sz), len(sizes), rate, whir_k, _p(out))
+    if rc:
+        raise CosnarkError(rc, "whir_rates: sizes must be sorted largest first")
+    return [int(x) for x in out]

@@ -64,25 +64,43 @@ __global__ __launch_bounds__(WT) void dot_kernel(const fe *__restrict__ a, const
 
 // Query replies and Merkle auth paths (whir_query :262-305, open_tree_blake
 // src/merkle_tree.cpp:320-335: the sibling of pos at each level is pos ^ 1).
-__global__ __launch_bounds__(WT) void gather_kernel(const fe *__restrict__ cw, uint64_t cw_n,
-                                                    const uint8_t *__restrict__ nodes, const uint32_t *__restrict__ q,
-                                                    int l, int depth, fe *rep, uint8_t *paths) {
-    const int total_rep = l * WK, total_path = l * depth;
+// Query i takes kappa replies cw[q_i + j cw_n/kappa].  The tree is the
+// codeword's dup-left tree (nodes, cw_n/4 leaves) or, when zero_levels is
+// given, a tree over all-zero leaf digests whose level-l nodes all equal
+// zero_levels[l] (the tree aggregate_commitments leaves, Whir.cpp:633-644).
+__global__ __launch_bounds__(WT) void gather_kernel(const fe *__restrict__ cw, uint64_t cw_n, int kappa,
+                                                    const uint8_t *__restrict__ nodes,
+                                                    const uint8_t *__restrict__ zero_levels,
+                                                    const uint32_t *__restrict__ q, int l, int depth, fe *rep,
+                                                    uint8_t *paths) {
+    const int total_rep = l * kappa, total_path = l * depth;
     for (int idx = blockIdx.x * WT + threadIdx.x; idx < total_rep + total_path; idx += gridDim.x * WT) {
         if (idx < total_rep) {
-            const int i = idx / WK, j = idx % WK;
-            rep[idx] = cw[q[i] + (uint64_t)j * (cw_n / WK)];
+            const int i = idx / kappa, j = idx % kappa;
+            rep[idx] = cw[q[i] + (uint64_t)j * (cw_n / kappa)];
         } else {
             const int k = idx - total_rep, i = k / depth, lvl = k % depth;
-            uint64_t off = 0, sz = cw_n / 4;
-            for (int t = 0; t < lvl; ++t) { off += sz; sz >>= 1; }
-            const uint64_t sib = ((uint64_t)q[i] >> lvl) ^ 1ull;
-            const uint4 *src = reinterpret_cast<const uint4 *>(nodes + 32 * (off + sib));
+            const uint8_t *sp;
+            if (zero_levels) {
+                sp = zero_levels + 32 * lvl;
+            } else {
+                uint64_t off = 0, sz = cw_n / 4;
+                for (int t = 0; t < lvl; ++t) { off += sz; sz >>= 1; }
+                const uint64_t sib = ((uint64_t)q[i] >> lvl) ^ 1ull;
+                sp = nodes + 32 * (off + sib);
+            }
+            const uint4 *src = reinterpret_cast<const uint4 *>(sp);
             uint4 *dst = reinterpret_cast<uint4 *>(paths + 32 * (uint64_t)k);
             dst[0] = src[0];
             dst[1] = src[1];
         }
     }
+}
+
+// dst[x] += w * src[x]: one term of aggregate_arrays / aggregate_commitments
+__global__ __launch_bounds__(WT) void axpy_kernel(fe *dst, const fe *__restrict__ src, fe w, uint64_t n) {
+    for (uint64_t x = (uint64_t)blockIdx.x * WT + threadIdx.x; x < n; x += (uint64_t)gridDim.x * WT)
+        dst[x] = gf::add(dst[x], gf::mul(w, src[x]));
 }
 
 // update_betas: b[j] += sum_i c_i w^(q_i' j), w = root of order N = 2^logN.
@@ -188,6 +206,65 @@ int query_count(int rate) {   // Whir.cpp:253-258, conjectured_security = false
 
 fe quad_eval(const fe *p, const fe &x) {
     return gf::add(gf::mul(gf::add(gf::mul(p[0], x), p[1]), x), p[2]);
+}
+
+// nodes of levels 0..levels of a dup-left tree over all-zero leaf digests:
+// level l is the l-fold self-hash of the zero digest
+std::vector<uint8_t> zero_chain(int levels) {
+    std::vector<uint8_t> z(32 * (size_t)(levels + 1), 0);
+    for (int l = 1; l <= levels; ++l) {
+        uint32_t msg[16], h[8];
+        memcpy(msg, z.data() + 32 * (l - 1), 32);
+        memcpy(msg + 8, z.data() + 32 * (l - 1), 32);
+        b3::compress64(msg, h);
+        memcpy(z.data() + 32 * l, h, 32);
+    }
+    return z;
+}
+
+// Whir_Batch_Open's fold schedule (Whir.cpp:731-755): kappas and aggregation
+// flags; -1 for input not sorted largest first or a schedule over 64 steps
+int whir_schedule(const uint64_t *sizes, int count, int whir_k, int *kap, int *agg) {
+    for (int t = 1; t < count; ++t)
+        if (sizes[t] > sizes[t - 1] || sizes[t] == 0) return -1;
+    int nk = 0, i = 1;
+    uint64_t size = sizes[0];
+    while (true) {
+        if (nk >= 64) return -1;
+        if (i >= count || size / (uint64_t)whir_k > sizes[i]) {
+            kap[nk] = whir_k;
+            size /= (uint64_t)whir_k;
+            agg[nk++] = 0;
+        } else {
+            kap[nk] = (int)(size / sizes[i]);
+            const uint64_t s2 = sizes[i];
+            while (i < count && sizes[i] == s2) ++i;
+            size /= (uint64_t)kap[nk];
+            agg[nk++] = 1;
+        }
+        if (size <= (uint64_t)whir_k) break;
+    }
+    return nk;
+}
+
+// whir_sumcheck (Whir.cpp:233-249) with whir_k = 2^k: k rounds on (arr, b)
+// of n entries with the external challenges of rounds r0 .. r0+k-1 (each
+// round's launch folds the previous round's pair first), leaving the folded
+// n/2^k entries in (arr_n, b_n); t[0..1] hold n/2, t[2..3] n/4 elements
+int whir_rounds(cosnark_ctx *ctx, const fe *arr, const fe *b, fe *arr_n, fe *b_n, uint64_t n, int k, int r0,
+                fe *const *t, hipStream_t s) {
+    int nb = 0, rc;
+    if ((rc = sumcheck_round_launch(ctx, arr, b, nullptr, nullptr, n / 2, r0, false, 0, 0, s, &nb))) return rc;
+    const fe *s1 = arr, *s2 = b;
+    for (int j = 1; j < k; ++j) {
+        fe *d1 = t[2 * ((j - 1) & 1)], *d2 = t[2 * ((j - 1) & 1) + 1];
+        int nb2 = 0;
+        if ((rc = sumcheck_round_launch(ctx, s1, s2, d1, d2, n >> (j + 1), r0 + j, true, 0, nb, s, &nb2))) return rc;
+        nb = nb2;
+        s1 = d1;
+        s2 = d2;
+    }
+    return sumcheck_fold_launch(ctx, s1, s2, arr_n, b_n, n >> k, r0 + k - 1, 0, nb, s);
 }
 
 struct Iter {
@@ -388,9 +465,9 @@ int cosnark_whir_open(cosnark_ctx *ctx, const uint64_t *d_arr, uint64_t n, const
         // whir_query on the previous codeword
         {
             const int work = it.l * WK + it.l * it.depth;
-            hipLaunchKernelGGL(gather_kernel, dim3((work + WT - 1) / WT), dim3(WT), 0, s, cw_old, it.cw_n, nd_old,
-                               (const uint32_t *)(dq + t * qmax), it.l, it.depth, rep + it.rep_off,
-                               paths + 32 * it.path_off);
+            hipLaunchKernelGGL(gather_kernel, dim3((work + WT - 1) / WT), dim3(WT), 0, s, cw_old, it.cw_n, WK, nd_old,
+                               (const uint8_t *)nullptr, (const uint32_t *)(dq + t * qmax), it.l, it.depth,
+                               rep + it.rep_off, paths + 32 * it.path_off);
             LAUNCHCK();
         }
         // update_betas on the folded b
@@ -485,6 +562,427 @@ int cosnark_whir_open(cosnark_ctx *ctx, const uint64_t *d_arr, uint64_t n, const
     *words = w;
     if (overflow) return set_err(ctx, COSNARK_E_ARG, "whir_open: record buffer too small");
     if (bad) return set_err(ctx, COSNARK_E_CHECK, "whir_open: claim chain check failed");
+    return COSNARK_OK;
+}
+
+
+int cosnark_whir_rates(const uint64_t *sizes, int count, int rate, int whir_k, int *rates) {
+    if (!sizes || !rates || count < 1 || rate < 1 || whir_k < 2) return COSNARK_E_ARG;
+    for (int t = 1; t < count; ++t)
+        if (sizes[t] > sizes[t - 1] || sizes[t] == 0) return COSNARK_E_ARG;
+    int w = 0, i = 1;
+    uint64_t size = sizes[0];
+    rates[w++] = rate;                                                     // Whir.cpp:38-51
+    while (i < count && sizes[i] == size) { rates[w++] = rate; ++i; }
+    while (i < count) {                                                    // :53-73
+        if (size / (uint64_t)whir_k > sizes[i]) {
+            size /= (uint64_t)whir_k;
+            rate *= 2;
+        } else {
+            size = sizes[i];
+            rate *= 2;
+            rates[w++] = rate;
+            const uint64_t s2 = sizes[i++];
+            while (i < count && sizes[i] == s2) { rates[w++] = rate; ++i; }
+        }
+    }
+    return COSNARK_OK;
+}
+
+int cosnark_whir_batch_open(cosnark_ctx *ctx, int count, const uint64_t *const *d_arrays, const uint64_t *sizes,
+                            const uint64_t *const *d_codewords, const uint64_t *cw_sizes,
+                            const uint8_t *const *d_nodes, const uint64_t *h_evals, const uint64_t *const *d_betas,
+                            int rate, int whir_k, uint64_t *h_record, uint64_t record_cap, uint64_t *words) {
+    if (!ctx || count < 1 || !d_arrays || !sizes || !d_codewords || !cw_sizes || !d_nodes || !h_evals || !d_betas ||
+        !h_record || !words)
+        return set_err(ctx, COSNARK_E_ARG, "whir_batch_open: null argument");
+    if (whir_k < 2 || !is_pow2((uint64_t)whir_k) || whir_k > 64 || rate < 2 || !is_pow2((uint64_t)rate))
+        return set_err(ctx, COSNARK_E_ARG, "whir_batch_open: whir_k a power of two in [2, 64], rate >= 2 a power of two");
+    for (int i = 0; i < count; ++i) {
+        if (!d_arrays[i] || !d_codewords[i] || !d_nodes[i] || !d_betas[i])
+            return set_err(ctx, COSNARK_E_ARG, "whir_batch_open: null array, codeword, tree or weights");
+        if (!is_pow2(sizes[i]) || sizes[i] < 2 || !is_pow2(cw_sizes[i]) || cw_sizes[i] < 8 ||
+            cw_sizes[i] > (1ull << NTT_LOG_MAX) || cw_sizes[i] < 2 * sizes[i])
+            return set_err(ctx, COSNARK_E_ARG, "whir_batch_open: sizes powers of two, 8 <= codeword <= 2^28");
+    }
+    int kap[64], agg[64];
+    const int nk = whir_schedule(sizes, count, whir_k, kap, agg);
+    if (nk < 0) return set_err(ctx, COSNARK_E_ARG, "whir_batch_open: arrays must be sorted largest first");
+    for (int t = 0; t < nk; ++t)
+        if (kap[t] > 64) return set_err(ctx, COSNARK_E_RANGE, "whir_batch_open: a fold factor above 64");
+
+    const uint64_t n0 = sizes[0], cw0 = cw_sizes[0];
+    hipStream_t s = ctx->stream;
+    // ---- device workspace ----
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) / 256 * 256; return o; };
+    const size_t o_arrA = take(16 * n0), o_bA = take(16 * n0), o_arrB = take(16 * n0), o_bB = take(16 * n0);
+    const size_t o_t0 = take(16 * (n0 / 2)), o_t1 = take(16 * (n0 / 2));
+    const size_t o_t2 = take(16 * (n0 / 4 + 1)), o_t3 = take(16 * (n0 / 4 + 1));
+    const size_t o_cw[2] = {take(16 * cw0), take(16 * cw0)};
+    const size_t o_nd[2] = {take(32 * cosnark_merkle_node_count(cw0)), take(32 * cosnark_merkle_node_count(cw0))};
+    const size_t o_fin = take(16 * cw0);
+    constexpr int PARTS = 64;
+    const size_t STEP_Q = WQ_MAX;
+    const size_t o_q = take(4 * STEP_Q * 64), o_qs = take(4 * STEP_Q * 64), o_c = take(16 * STEP_Q * 64);
+    const size_t o_z = take(32 * 29 * 64);
+    // replies and paths: bounded by the steps' sizes, reserved below once known
+    void *base;
+    int rc;
+
+    // ---- pass 1: the schedule, every libc draw in the reference's order, and sizes ----
+    struct Step {
+        int kind;                 // 0 iteration, 1 aggregation
+        int kappa, k, r0, l, depth, rate;
+        uint64_t n, cw_n, cw_new_n;
+        fe r[6], rb;
+        std::vector<uint32_t> q;
+        size_t rep_off, path_off;
+        int cnt, first;           // aggregation: members first .. first+cnt-2 join the running array
+        std::vector<fe> c1, c2;
+        int zero_src;             // queried tree: -1 real, else index of the zero chain
+        int zero_new;             // aggregation: index of the zero chain it leaves
+        uint64_t zero_leaves;
+        size_t eval_off;
+    };
+    std::vector<Step> steps;
+    std::vector<std::vector<uint8_t>> chains;
+    std::vector<fe> ext(64, gf::zero());
+    int nround = 0;
+    {
+        uint64_t n = n0, cw_n = cw0;
+        int rt = rate, iter = 0, ri = 1, wk = whir_k, zero_cur = -1;
+        uint64_t tree_leaves = cw_n / 4;
+        size_t rep_off = 0, path_off = 0, eval_off = 0;
+        while (true) {
+            wk = kap[iter];
+            if (!agg[iter] || wk != 1) {
+                Step st{};
+                st.kind = 0;
+                st.kappa = wk;
+                st.k = ilog2((uint64_t)wk);
+                st.r0 = nround;
+                st.n = n;
+                st.cw_n = cw_n;
+                st.rate = rt;
+                if (nround + st.k > 64) return set_err(ctx, COSNARK_E_RANGE, "whir_batch_open: over 64 rounds");
+                for (int j = 0; j < st.k; ++j) ext[nround + j] = st.r[j] = libc_random_element();
+                nround += st.k;
+                const uint64_t nn = n / (uint64_t)wk;
+                st.cw_new_n = nn * (uint64_t)(2 * rt);
+                if (st.cw_new_n < 8 || st.cw_new_n > cw0)
+                    return set_err(ctx, COSNARK_E_RANGE, "whir_batch_open: a codeword outside [8, the first]");
+                st.l = query_count(rt);
+                if (st.l > WQ_MAX) return set_err(ctx, COSNARK_E_RANGE, "whir_batch_open: more than 256 queries");
+                const uint64_t qmod = wk > 4 ? cw_n / (uint64_t)wk : cw_n / 4;    // Whir.cpp:262-267
+                st.q.resize(st.l);
+                for (int i = 0; i < st.l; ++i) {
+                    st.q[i] = (uint32_t)((uint64_t)rand() % qmod);
+                    if (st.q[i] >= tree_leaves)
+                        return set_err(ctx, COSNARK_E_CHECK, "whir_batch_open: query beyond the tree (the reference exits)");
+                }
+                st.rb = libc_random_element();
+                st.depth = ilog2(tree_leaves);
+                st.zero_src = zero_cur;
+                st.rep_off = rep_off;
+                st.path_off = path_off;
+                rep_off += (size_t)st.l * wk;
+                path_off += (size_t)st.l * st.depth;
+                steps.push_back(st);
+                n = nn;
+                cw_n = st.cw_new_n;
+                tree_leaves = cw_n / 4;
+                zero_cur = -1;
+                rt *= 2;
+            }
+            ++iter;
+            if (agg[iter - 1]) {
+                Step st{};
+                st.kind = 1;
+                st.first = ri;
+                st.cnt = 1;
+                while (ri + st.cnt - 1 < count && sizes[ri + st.cnt - 1] == n) ++st.cnt;
+                for (int j = ri + st.cnt - 1; j < count; ++j)
+                    if (sizes[j] == n) return set_err(ctx, COSNARK_E_ARG, "whir_batch_open: arrays not sorted");
+                for (int i = 1; i < st.cnt; ++i)
+                    if (cw_sizes[ri + i - 1] != cw_n)
+                        return set_err(ctx, COSNARK_E_CHECK, "whir_batch_open: codeword sizes differ at aggregation "
+                                                             "(commit at the rates of cosnark_whir_rates)");
+                st.c1.resize(st.cnt);
+                st.c2.resize(st.cnt);
+                for (auto &x : st.c1) x = libc_random_element();                  // Whir.cpp:809-812
+                for (auto &x : st.c2) x = libc_random_element();
+                st.c1[0] = gf::one();
+                st.c2[0] = gf::one();
+                st.n = n;
+                st.cw_n = cw_n;
+                st.eval_off = eval_off;
+                eval_off += (size_t)st.cnt * st.cnt;
+                // the next fold factor (:827-831); past the end of the schedule the
+                // reference reads beyond its vector, here the last entry
+                wk = iter + 1 < nk ? kap[iter + 1] : kap[nk - 1];
+                if (ri + st.cnt - 1 < count && n / (uint64_t)wk <= sizes[ri + st.cnt - 1])
+                    wk = (int)(n / sizes[ri + st.cnt - 1]);
+                st.rate = (int)(cw_n / n);                                         // aggregate_commitments' rate
+                st.l = query_count(st.rate);
+                if (st.l > WQ_MAX) return set_err(ctx, COSNARK_E_RANGE, "whir_batch_open: more than 256 queries");
+                st.q.resize(st.l);
+                for (int i = 0; i < st.l; ++i) {
+                    st.q[i] = (uint32_t)((uint64_t)rand() % (cw_n / 4));
+                    if (st.q[i] >= tree_leaves)
+                        return set_err(ctx, COSNARK_E_CHECK, "whir_batch_open: query beyond the tree (the reference exits)");
+                }
+                st.depth = ilog2(tree_leaves);
+                st.zero_src = zero_cur;
+                st.rep_off = rep_off;
+                st.path_off = path_off;
+                path_off += (size_t)st.l * st.depth;
+                st.zero_leaves = cw_n / (uint64_t)wk;
+                if (st.zero_leaves < 1) return set_err(ctx, COSNARK_E_RANGE, "whir_batch_open: empty tree");
+                if ((int)chains.size() >= 64) return set_err(ctx, COSNARK_E_RANGE, "whir_batch_open: over 64 aggregations");
+                chains.push_back(zero_chain(ilog2(st.zero_leaves)));
+                st.zero_new = (int)chains.size() - 1;
+                zero_cur = st.zero_new;
+                tree_leaves = st.zero_leaves;
+                ri += st.cnt - 1;
+                steps.push_back(st);
+            }
+            if (n <= (uint64_t)wk) break;
+            if (iter >= nk) return set_err(ctx, COSNARK_E_CHECK, "whir_batch_open: schedule ended early");
+        }
+        // the closing challenges (Whir.cpp:861-863)
+        Step fin{};
+        fin.kind = 2;
+        fin.n = n;
+        fin.cw_n = cw_n;
+        fin.rate = rt;
+        fin.kappa = wk;
+        fin.k = ilog2(n);
+        for (int j = 0; j < fin.k && j < 6; ++j) fin.r[j] = libc_random_element();
+        if (fin.k > 6) return set_err(ctx, COSNARK_E_RANGE, "whir_batch_open: final array above 64 elements");
+        steps.push_back(fin);
+        const size_t o_rep = take(16 * (rep_off + 1)), o_path = take(32 * (path_off + 1));
+        const size_t o_parts = take(16 * PARTS * (eval_off + 1));
+        if ((rc = scratch_reserve(ctx, off, &base))) return rc;
+        char *B = (char *)base;
+        fe *arrA = (fe *)(B + o_arrA), *bA = (fe *)(B + o_bA), *arrB = (fe *)(B + o_arrB), *bB = (fe *)(B + o_bB);
+        fe *const tt[4] = {(fe *)(B + o_t0), (fe *)(B + o_t1), (fe *)(B + o_t2), (fe *)(B + o_t3)};
+        fe *cwbuf[2] = {(fe *)(B + o_cw[0]), (fe *)(B + o_cw[1])};
+        uint8_t *ndbuf[2] = {(uint8_t *)(B + o_nd[0]), (uint8_t *)(B + o_nd[1])};
+        fe *finbuf = (fe *)(B + o_fin);
+        uint32_t *dq = (uint32_t *)(B + o_q), *dqs = (uint32_t *)(B + o_qs);
+        fe *dc = (fe *)(B + o_c);
+        uint8_t *dz = (uint8_t *)(B + o_z);
+        fe *rep = (fe *)(B + o_rep);
+        uint8_t *paths = (uint8_t *)(B + o_path);
+        fe *parts = (fe *)(B + o_parts);
+
+        // ---- host staging for the uploads (alive until the final sync) ----
+        std::vector<uint32_t> hq(STEP_Q * 64, 0), hqs(STEP_Q * 64, 0);
+        std::vector<fe> hc(STEP_Q * 64, gf::zero());
+        std::vector<uint8_t> hz(32 * 29 * 64, 0);
+        for (size_t t = 0; t < steps.size(); ++t) {
+            const Step &st = steps[t];
+            if (st.kind == 0) {
+                const uint64_t scale = (uint64_t)st.kappa * st.cw_new_n / st.cw_n;   // :373-374
+                fe c = st.rb;
+                for (int i = 0; i < st.l; ++i) {
+                    if (i > 0) c = gf::mul(c, st.rb);
+                    hq[t * STEP_Q + i] = st.q[i];
+                    hqs[t * STEP_Q + i] = (uint32_t)(st.q[i] * scale);
+                    hc[t * STEP_Q + i] = c;
+                }
+            } else if (st.kind == 1) {
+                for (int i = 0; i < st.l; ++i) hq[t * STEP_Q + i] = st.q[i];
+            }
+        }
+        if (steps.size() > 64) return set_err(ctx, COSNARK_E_RANGE, "whir_batch_open: over 64 steps");
+        for (size_t c = 0; c < chains.size(); ++c) memcpy(hz.data() + 32 * 29 * c, chains[c].data(), chains[c].size());
+        HIPCK(hipMemcpyAsync(dq, hq.data(), 4 * hq.size(), hipMemcpyHostToDevice, s));
+        HIPCK(hipMemcpyAsync(dqs, hqs.data(), 4 * hqs.size(), hipMemcpyHostToDevice, s));
+        HIPCK(hipMemcpyAsync(dc, hc.data(), sizeof(fe) * hc.size(), hipMemcpyHostToDevice, s));
+        HIPCK(hipMemcpyAsync(dz, hz.data(), hz.size(), hipMemcpyHostToDevice, s));
+        HIPCK(hipMemcpyAsync(ctx->sc->ext_r, ext.data(), sizeof(fe) * 64, hipMemcpyHostToDevice, s));
+        if ((rc = sumcheck_state_init(ctx, gf::zero(), gf::zero(), SC_CLAIM_UNCHECKED, s))) return rc;
+
+        // ---- pass 2: the device work, step by step ----
+        HIPCK(hipMemcpyAsync(arrA, d_arrays[0], 16 * n0, hipMemcpyDeviceToDevice, s));
+        HIPCK(hipMemcpyAsync(bA, d_betas[0], 16 * n0, hipMemcpyDeviceToDevice, s));
+        fe *arr = arrA, *b = bA;
+        const fe *cw_cur = (const fe *)d_codewords[0];
+        const uint8_t *nd_cur = d_nodes[0];
+        int cw_slot = -1;          // which cwbuf holds cw_cur (-1: the caller's codeword)
+        auto grid = [](uint64_t n) { uint64_t g = (n + WT - 1) / WT; return dim3((unsigned)(g < 4096 ? g : 4096)); };
+        std::vector<uint8_t> roots(32 * steps.size());
+        for (size_t t = 0; t < steps.size(); ++t) {
+            const Step &st = steps[t];
+            if (st.kind == 0) {
+                fe *arr_n = (arr == arrA) ? arrB : arrA;
+                fe *b_n = (arr == arrA) ? bB : bA;
+                if ((rc = whir_rounds(ctx, arr, b, arr_n, b_n, st.n, st.k, st.r0, tt, s))) return rc;
+                const int slot = cw_slot == 0 ? 1 : 0;
+                fe *cw_new = cwbuf[slot];
+                uint8_t *nd_new = ndbuf[slot];
+                if ((rc = ntt_launch(ctx, arr_n, st.n / st.kappa, cw_new, ilog2(st.cw_new_n), false, s))) return rc;
+                if ((rc = merkle_launch(ctx, cw_new, st.cw_new_n, nd_new, true, s))) return rc;
+                const int work = st.l * st.kappa + st.l * st.depth;
+                hipLaunchKernelGGL(gather_kernel, dim3((work + WT - 1) / WT), dim3(WT), 0, s, cw_cur, st.cw_n, st.kappa,
+                                   nd_cur, st.zero_src >= 0 ? (const uint8_t *)(dz + 32 * 29 * st.zero_src) : nullptr,
+                                   (const uint32_t *)(dq + t * STEP_Q), st.l, st.depth, rep + st.rep_off,
+                                   paths + 32 * st.path_off);
+                LAUNCHCK();
+                const uint64_t nb = st.n / st.kappa;
+                const uint64_t threads = (nb + BETA_J - 1) / BETA_J;
+                hipLaunchKernelGGL(betas_kernel, dim3((unsigned)((threads + WT - 1) / WT)), dim3(WT), 0, s, b_n, nb,
+                                   (const uint32_t *)(dqs + t * STEP_Q), (const fe *)(dc + t * STEP_Q), st.l,
+                                   ilog2(st.cw_new_n), (const fe *)ctx->tw_lo, (const fe *)ctx->tw_hi);
+                LAUNCHCK();
+                const uint64_t cnt = cosnark_merkle_node_count(st.cw_new_n);
+                HIPCK(hipMemcpyAsync(roots.data() + 32 * t, nd_new + 32 * (cnt - 1), 32, hipMemcpyDeviceToHost, s));
+                arr = arr_n;
+                b = b_n;
+                cw_cur = cw_new;
+                nd_cur = nd_new;
+                cw_slot = slot;
+            } else if (st.kind == 1) {
+                // evals[i][j] = <A_i, B_j> before the running pair changes (:681-690)
+                for (int i = 0; i < st.cnt; ++i)
+                    for (int j = 0; j < st.cnt; ++j) {
+                        const fe *A = i ? (const fe *)d_arrays[st.first + i - 1] : arr;
+                        const fe *Bv = j ? (const fe *)d_betas[st.first + j - 1] : b;
+                        hipLaunchKernelGGL(dot_kernel, dim3(PARTS), dim3(WT), 0, s, A, Bv, st.n,
+                                           parts + PARTS * (st.eval_off + (size_t)i * st.cnt + j));
+                        LAUNCHCK();
+                    }
+                // auth paths on the running tree (aggregate_commitments :621-630)
+                const int work = st.l * st.depth;
+                if (work > 0) {
+                    hipLaunchKernelGGL(gather_kernel, dim3((work + WT - 1) / WT), dim3(WT), 0, s, cw_cur, st.cw_n, 0,
+                                       nd_cur,
+                                       st.zero_src >= 0 ? (const uint8_t *)(dz + 32 * 29 * st.zero_src) : nullptr,
+                                       (const uint32_t *)(dq + t * STEP_Q), st.l, st.depth, rep,
+                                       paths + 32 * st.path_off);
+                    LAUNCHCK();
+                }
+                // arr += c2_i A_i, b += c1_i B_i (:691-698); codeword += c2_i cw_i (:606-610)
+                for (int i = 1; i < st.cnt; ++i) {
+                    hipLaunchKernelGGL(axpy_kernel, grid(st.n), dim3(WT), 0, s, arr,
+                                       (const fe *)d_arrays[st.first + i - 1], st.c2[i], st.n);
+                    LAUNCHCK();
+                    hipLaunchKernelGGL(axpy_kernel, grid(st.n), dim3(WT), 0, s, b,
+                                       (const fe *)d_betas[st.first + i - 1], st.c1[i], st.n);
+                    LAUNCHCK();
+                }
+                if (cw_slot < 0) {                     // never write the caller's codeword
+                    HIPCK(hipMemcpyAsync(cwbuf[0], cw_cur, 16 * st.cw_n, hipMemcpyDeviceToDevice, s));
+                    cw_slot = 0;
+                }
+                fe *cw_w = cwbuf[cw_slot];
+                for (int i = 1; i < st.cnt; ++i) {
+                    hipLaunchKernelGGL(axpy_kernel, grid(st.cw_n), dim3(WT), 0, s, cw_w,
+                                       (const fe *)d_codewords[st.first + i - 1], st.c2[i], st.cw_n);
+                    LAUNCHCK();
+                }
+                cw_cur = cw_w;
+                nd_cur = nullptr;
+            } else {
+                // verification step 1 (:850-856): arr's encoding at the final rate is the codeword
+                if ((rc = ntt_launch(ctx, arr, st.n, finbuf, ilog2(st.n * (uint64_t)st.rate), false, s))) return rc;
+            }
+        }
+
+        // ---- results back ----
+        const Step &fs = steps.back();
+        const uint64_t fcw = fs.n * (uint64_t)fs.rate;
+        std::vector<fe> h_rep(rep_off + 1), h_parts(PARTS * (eval_off + 1)), fin_arr(fs.n), fin_b(fs.n);
+        std::vector<fe> fin_cw(fcw), run_cw(fs.cw_n);
+        std::vector<uint8_t> h_paths(32 * (path_off + 1));
+        HIPCK(hipMemcpyAsync(h_rep.data(), rep, sizeof(fe) * (rep_off + 1), hipMemcpyDeviceToHost, s));
+        HIPCK(hipMemcpyAsync(h_paths.data(), paths, 32 * (path_off + 1), hipMemcpyDeviceToHost, s));
+        HIPCK(hipMemcpyAsync(h_parts.data(), parts, sizeof(fe) * h_parts.size(), hipMemcpyDeviceToHost, s));
+        HIPCK(hipMemcpyAsync(fin_arr.data(), arr, sizeof(fe) * fs.n, hipMemcpyDeviceToHost, s));
+        HIPCK(hipMemcpyAsync(fin_b.data(), b, sizeof(fe) * fs.n, hipMemcpyDeviceToHost, s));
+        HIPCK(hipMemcpyAsync(fin_cw.data(), finbuf, sizeof(fe) * fcw, hipMemcpyDeviceToHost, s));
+        HIPCK(hipMemcpyAsync(run_cw.data(), cw_cur, sizeof(fe) * fs.cw_n, hipMemcpyDeviceToHost, s));
+        HIPCK(hipMemcpyAsync(ctx->sc_host, ctx->sc, sizeof(ScState), hipMemcpyDeviceToHost, s));
+        HIPCK(hipStreamSynchronize(s));
+        const ScState &scs = *ctx->sc_host;
+
+        // ---- the record and the reference's checks, in step order ----
+        uint64_t w = 0;
+        bool overflow = false, bad = false;
+        auto put = [&](uint64_t v) { if (w < record_cap) h_record[w] = v; else overflow = true; ++w; };
+        auto putfe = [&](const fe &v) { put(v.re); put(v.im); };
+        auto put32 = [&](const uint8_t *h) { uint64_t x[4]; memcpy(x, h, 32); for (int i = 0; i < 4; ++i) put(x[i]); };
+        fe K = gf::mk(h_evals[0], h_evals[1]);                                  // Whir.cpp:714
+        put(steps.size() - 1);
+        for (size_t t = 0; t + 1 < steps.size(); ++t) {
+            const Step &st = steps[t];
+            if (st.kind == 0) {
+                put(0);
+                put((uint64_t)st.kappa);
+                for (int rd = 0; rd < st.k; ++rd) {
+                    const fe *p = scs.polys + 3 * (st.r0 + rd);
+                    if (!gf::eq(gf::add(quad_eval(p, gf::zero()), quad_eval(p, gf::one())), K)) bad = true;
+                    K = quad_eval(p, st.r[rd]);
+                    putfe(p[0]); putfe(p[1]); putfe(p[2]);
+                }
+                for (int rd = 0; rd < st.k; ++rd) putfe(st.r[rd]);
+                put32(roots.data() + 32 * t);
+                put((uint64_t)st.l);
+                for (int i = 0; i < st.l; ++i) put(st.q[i]);
+                std::vector<fe> y(st.l);
+                for (int i = 0; i < st.l; ++i) {
+                    const fe *seg = h_rep.data() + st.rep_off + (size_t)i * st.kappa;
+                    for (int j = 0; j < st.kappa; ++j) putfe(seg[j]);
+                    y[i] = segment_fold(seg, st.kappa, st.r, st.k, st.cw_n, st.q[i]);
+                }
+                for (int i = 0; i < st.l; ++i) putfe(y[i]);
+                for (size_t i = 0; i < (size_t)st.l * st.depth; ++i) put32(h_paths.data() + 32 * (st.path_off + i));
+                putfe(st.rb);
+                fe c = st.rb;
+                for (int i = 0; i < st.l; ++i) {                                 // :383-384
+                    if (i > 0) c = gf::mul(c, st.rb);
+                    K = gf::add(K, gf::mul(c, y[i]));
+                }
+            } else {
+                put(1);
+                put((uint64_t)st.cnt);
+                for (const fe &x : st.c1) putfe(x);
+                for (const fe &x : st.c2) putfe(x);
+                fe Ka = gf::zero();
+                for (int i = 0; i < st.cnt; ++i)
+                    for (int j = 0; j < st.cnt; ++j) {
+                        fe e = gf::zero();
+                        const fe *pp = h_parts.data() + PARTS * (st.eval_off + (size_t)i * st.cnt + j);
+                        for (int u = 0; u < PARTS; ++u) e = gf::add(e, pp[u]);
+                        putfe(e);
+                        Ka = gf::add(Ka, gf::mul(gf::mul(st.c2[i], st.c1[j]), e));
+                    }
+                K = Ka;
+                putfe(K);
+                put((uint64_t)st.l);
+                for (int i = 0; i < st.l; ++i) put(st.q[i]);
+                for (size_t i = 0; i < (size_t)st.l * st.depth; ++i) put32(h_paths.data() + 32 * (st.path_off + i));
+                const auto &z = chains[st.zero_new];
+                put32(z.data() + z.size() - 32);
+            }
+        }
+        for (uint64_t i = 0; i < fs.n; ++i) putfe(fin_arr[i]);
+        putfe(K);
+        {
+            const auto z = zero_chain(ilog2(fs.n * (uint64_t)fs.rate / (uint64_t)fs.kappa));   // Whir_Commit :192-213
+            put32(z.data() + z.size() - 32);
+        }
+        if (fcw != fs.cw_n || memcmp(fin_cw.data(), run_cw.data(), sizeof(fe) * fcw) != 0) bad = true;
+        fe Kc = K;                                                                // verification step 2
+        for (uint64_t i = 0; i < fs.n; ++i) Kc = gf::sub(Kc, gf::mul(fin_arr[i], fin_b[i]));
+        if (!gf::eq(Kc, gf::zero())) bad = true;
+        for (int j = 0; j < fs.k; ++j) putfe(fs.r[j]);
+        *words = w;
+        if (overflow) return set_err(ctx, COSNARK_E_ARG, "whir_batch_open: record buffer too small");
+        if (bad) return set_err(ctx, COSNARK_E_CHECK, "whir_batch_open: claim chain check failed");
+    }
     return COSNARK_OK;
 }
 
