@@ -170,19 +170,19 @@ int cosnark_whir_open(cosnark_ctx *ctx, const uint64_t *d_arr, uint64_t n, const
  * at so that Whir_Batch_Open meets it with a codeword of the running size;
  * the first is `rate`.  Host only.  COSNARK_E_ARG for unsorted sizes. */
 int cosnark_whir_rates(const uint64_t *sizes, int count, int rate, int whir_k, int *rates);
-/* Whir_Batch_Open (src/Whir.cpp:705-899): one opening for count arrays of
+/* Whir_Batch_Open (src/Whir.cpp:706-899): one opening for count arrays of
  * sizes sorted largest first.  d_arrays[i], d_betas[i] (sizes[i] elements:
  * the array and its eq weights), d_codewords[i] (cw_sizes[i] elements, the
  * encoding at cosnark_whir_rates' rate) and d_nodes[i] (its dup-left tree as
  * cosnark_whir_commit writes it) are device pointers; h_evals holds the
  * claims (only entry 0 is read: the reference recomputes the others by
- * aggregate_arrays, :672-703).  The fold schedule (:731-755) runs Whir_Iteration
+ * aggregate_arrays, :670-703).  The fold schedule (:736-755) runs Whir_Iteration
  * with fold factor kappa, and where an array size is met, aggregate_arrays
- * and aggregate_commitments (:590-645) merge the arrays of that size into the
+ * and aggregate_commitments (:547-606) merge the arrays of that size into the
  * running one; aggregate_commitments leaves a tree over all-zero leaf digests
  * (the reference never fills its leaf vector), which the next iteration's
  * queries open.  After an aggregation at the last scheduled step the
- * reference reads past its kappa list (:827); the last kappa is used here.
+ * reference reads past its kappa list (:826); the last kappa is used here.
  * Challenges come from libc random()/rand() in the reference's order
  * (seed first).  The record (u64 words): [steps], per step [kind]: 0 an
  * iteration: kappa, log2(kappa) x (a,b,c), the log2(kappa) challenges, new
@@ -195,6 +195,18 @@ int cosnark_whir_rates(const uint64_t *sizes, int count, int rate, int whir_k, i
  * fails (a round, the closing K = <arr, b>, the closing codeword = RS(arr))
  * or where the reference exits (a query beyond the tree, codeword sizes
  * that differ at an aggregation).  whir_k a power of two in [2, 64]. */
+/* aggregate_claims (src/Whir.cpp:608-668) for claims on narrays arrays of n
+ * elements (device pointers): the eq weights beta_j of each of the npoints
+ * claim points (h_points: npoints x log2 n elements, point j's coordinates
+ * as precompute_beta takes them), Sums[i][j] = <beta_j, arrays_i>, the
+ * challenges h_beta_ch[npoints] then h_arr_ch[narrays] from libc (in that
+ * order), d_b_aggr = sum_j beta_ch_j beta_j, d_arr = sum_i arr_ch_i arrays_i
+ * and K = sum_ij beta_ch_j arr_ch_i Sums[i][j].  The reference adds to arr
+ * and b_aggr as given (its callers pass empty vectors); here they are
+ * overwritten.  1..256 arrays and points. */
+int cosnark_whir_aggregate_claims(cosnark_ctx *ctx, int narrays, const uint64_t *const *d_arrays, uint64_t n,
+                                  int npoints, const uint64_t *h_points, uint64_t *d_arr, uint64_t *d_b_aggr,
+                                  uint64_t K[2], uint64_t *h_beta_ch, uint64_t *h_arr_ch);
 int cosnark_whir_batch_open(cosnark_ctx *ctx, int count, const uint64_t *const *d_arrays, const uint64_t *sizes,
                             const uint64_t *const *d_codewords, const uint64_t *cw_sizes,
                             const uint8_t *const *d_nodes, const uint64_t *h_evals, const uint64_t *const *d_betas,

@@ -25,6 +25,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -253,13 +254,13 @@ std::vector<std::pair<F, std::vector<F>>> _quadratic_sumcheck(F y, std::vector<F
 // ---- WHIR, src/Whir.cpp (whir_k = 4) -----------------------------------------------
 
 // Whir_Commit_initial(arr, codeword, MT, rate, kappa), :158-190: codeword =
-// the rate*|arr| evaluations of arr's coefficients, MT its tree
+// the rate*|arr| evaluations of arr's coefficients, MT its tree (any kappa)
 template <class F, class H>
 void Whir_Commit_initial(std::vector<F> &arr, std::vector<F> &codeword, std::vector<std::vector<H>> &MT, int rate,
                          int kappa) {
     detail::elem_type<F>();
     detail::digest_type<H>();
-    if (kappa != 4) throw error(COSNARK_E_ARG, "Whir_Commit_initial: whir_k = 4 only");
+    (void)kappa;   // the reference only fills an unused buffer with it (:165-175)
     const uint64_t n = arr.size(), cw = n * (uint64_t)rate, cnt = cosnark_merkle_node_count(cw);
     detail::Dev a(arr.data(), 16 * n), c(16 * cw), nodes(32 * cnt);
     detail::check(cosnark_whir_commit(detail::ctx(), a.u64(), n, rate, c.u64(), nodes.u8()), "Whir_Commit_initial");
@@ -554,6 +555,122 @@ std::vector<F> sumcheck_aggr_gathered(const std::vector<std::vector<F>> &P, cons
                                          reinterpret_cast<uint64_t *>(out.data()));
     if (rc) throw error(rc, "sumcheck_aggr: sizes");
     return out;
+}
+
+namespace detail {
+// a tree as MT_commit_Blake leaves it (levels, leaves first) onto the device
+template <class H>
+void tree_in(const std::vector<std::vector<H>> &MT, uint64_t cw, Dev &nodes, const char *what) {
+    const uint64_t cnt = cosnark_merkle_node_count(cw);
+    if (MT.empty() || MT[0].size() != cw / 4) throw error(COSNARK_E_ARG, std::string(what) + ": tree does not match");
+    uint64_t off = 0;
+    for (size_t l = 0; l < MT.size(); ++l) {
+        if (off + MT[l].size() > cnt) throw error(COSNARK_E_ARG, std::string(what) + ": tree larger than expected");
+        check(cosnark_h2d(ctx(), nodes.u8() + 32 * off, MT[l].data(), 32 * MT[l].size()), "copy in");
+        off += MT[l].size();
+    }
+    if (off != cnt) throw error(COSNARK_E_ARG, std::string(what) + ": tree smaller than expected");
+}
+}  // namespace detail
+
+// get_whir_rates(rate), src/Whir.cpp:18-137.  The reference reads the global
+// whir_sizes and writes whir_rates; here the sizes (sorted largest first)
+// come in and the rates go out.
+inline std::vector<int> get_whir_rates(const std::vector<int> &sizes, int rate, int whir_k = 4) {
+    std::vector<uint64_t> sz(sizes.begin(), sizes.end());
+    std::vector<int> rates(sizes.size());
+    if (sizes.empty() || cosnark_whir_rates(sz.data(), (int)sz.size(), rate, whir_k, rates.data()))
+        throw error(COSNARK_E_ARG, "get_whir_rates: sizes must be sorted largest first");
+    return rates;
+}
+
+// Whir_Batch_Open(arrays, codewords, MTs, evals, betas, rate, Verify),
+// src/Whir.cpp:706-899, with whir_k the global's value.  The arrays are
+// sorted largest first, each committed at get_whir_rates' rate.  The
+// reference consumes its vector arguments as it aggregates them; they are
+// left empty here.  Its record is in last_whir_record().  A failed check
+// throws whatever Verify says (the reference prints for Verify = false).
+template <class F, class H>
+void Whir_Batch_Open(std::vector<std::vector<F>> &arrays, std::vector<std::vector<F>> &codewords,
+                     std::vector<std::vector<std::vector<H>>> &MTs, std::vector<F> evals,
+                     std::vector<std::vector<F>> &betas, int rate, bool, int whir_k = 4) {
+    detail::elem_type<F>();
+    detail::digest_type<H>();
+    const size_t c = arrays.size();
+    if (!c || codewords.size() != c || MTs.size() != c || betas.size() != c || evals.empty())
+        throw error(COSNARK_E_ARG, "Whir_Batch_Open: one codeword, tree and weight vector per array");
+    std::vector<std::unique_ptr<detail::Dev>> keep;
+    std::vector<const uint64_t *> pa(c), pc(c), pb(c);
+    std::vector<const uint8_t *> pt(c);
+    std::vector<uint64_t> sizes(c), cws(c);
+    for (size_t i = 0; i < c; ++i) {
+        sizes[i] = arrays[i].size();
+        cws[i] = codewords[i].size();
+        if (betas[i].size() != sizes[i]) throw error(COSNARK_E_ARG, "Whir_Batch_Open: weights and array sizes differ");
+        keep.emplace_back(new detail::Dev(arrays[i].data(), 16 * sizes[i]));
+        pa[i] = keep.back()->u64();
+        keep.emplace_back(new detail::Dev(codewords[i].data(), 16 * cws[i]));
+        pc[i] = keep.back()->u64();
+        keep.emplace_back(new detail::Dev(betas[i].data(), 16 * sizes[i]));
+        pb[i] = keep.back()->u64();
+        keep.emplace_back(new detail::Dev(32 * cosnark_merkle_node_count(cws[i])));
+        detail::tree_in(MTs[i], cws[i], *keep.back(), "Whir_Batch_Open");
+        pt[i] = keep.back()->u8();
+    }
+    auto &rec = last_whir_record();
+    rec.assign(1u << 22, 0);
+    uint64_t words = 0;
+    detail::check(cosnark_whir_batch_open(detail::ctx(), (int)c, pa.data(), sizes.data(), pc.data(), cws.data(),
+                                          pt.data(), reinterpret_cast<const uint64_t *>(evals.data()), pb.data(), rate,
+                                          whir_k, rec.data(), rec.size(), &words),
+                  "Whir_Batch_Open");
+    rec.resize(words);
+    arrays.clear();
+    codewords.clear();
+    MTs.clear();
+    betas.clear();
+}
+
+// aggregate_claims(arrays, arr, b_aggr, claims, K), src/Whir.cpp:608-668:
+// arr and b_aggr receive the combined array and weights (the reference adds
+// to them as given; its callers pass empty vectors), K the combined claim;
+// returns {weight challenges, array challenges} drawn from libc.
+template <class F>
+std::vector<std::vector<F>> aggregate_claims(std::vector<std::vector<F>> &arrays, std::vector<F> &arr,
+                                             std::vector<F> &b_aggr,
+                                             std::vector<std::vector<std::pair<F, std::vector<F>>>> claims, F &K) {
+    detail::elem_type<F>();
+    if (arrays.empty()) throw error(COSNARK_E_ARG, "aggregate_claims: no arrays");
+    const uint64_t n = arrays[0].size();
+    std::vector<F> pts;
+    int np = 0;
+    for (auto &ci : claims)
+        for (auto &cl : ci) {
+            if ((uint64_t(1) << cl.second.size()) != n) throw error(COSNARK_E_ARG, "aggregate_claims: point size");
+            pts.insert(pts.end(), cl.second.begin(), cl.second.end());
+            ++np;
+        }
+    std::vector<std::unique_ptr<detail::Dev>> keep;
+    std::vector<const uint64_t *> pa;
+    for (auto &a : arrays) {
+        if (a.size() != n) throw error(COSNARK_E_ARG, "aggregate_claims: arrays differ in size");
+        keep.emplace_back(new detail::Dev(a.data(), 16 * n));
+        pa.push_back(keep.back()->u64());
+    }
+    detail::Dev da(16 * n), db(16 * n);
+    std::vector<F> bc((size_t)np), ac(arrays.size());
+    uint64_t k[2];
+    detail::check(cosnark_whir_aggregate_claims(detail::ctx(), (int)arrays.size(), pa.data(), n, np,
+                                                reinterpret_cast<const uint64_t *>(pts.data()), da.u64(), db.u64(), k,
+                                                reinterpret_cast<uint64_t *>(bc.data()),
+                                                reinterpret_cast<uint64_t *>(ac.data())),
+                  "aggregate_claims");
+    arr.resize(n);
+    b_aggr.resize(n);
+    da.get(arr.data(), 16 * n);
+    db.get(b_aggr.data(), 16 * n);
+    std::memcpy(&K, k, 16);
+    return {bc, ac};
 }
 
 // RS_multifold5(codeword, a, fold_degree), src/coPCS_utils.cpp:386-437:

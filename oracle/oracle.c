@@ -674,7 +674,7 @@ static int whir_iteration(whir_state *S, int kappa, rec_t *R, int iter) {
 }
 
 /* The closing steps of Whir_Open / Whir_Batch_Open (Whir.cpp:425-449,
- * :848-873) with whir_k = kappa: arr, K, the root of Whir_Commit (:192-213,
+ * :844-863) with whir_k = kappa: arr, K, the root of Whir_Commit (:192-213,
  * a tree over |arr|*rate/kappa all-zero leaf digests), and the final
  * challenges; verification steps 1 (the codeword is arr's encoding) and 2
  * (K = <arr, b>) */
@@ -768,7 +768,7 @@ void or_whir_rates(const uint64_t *sizes, int count, int rate, int whir_k, int *
 }
 
 int or_whir_kappas(const uint64_t *sizes, int count, int whir_k, int *kappas, int *aggr, int cap) {
-    /* the fold schedule of Whir_Batch_Open (Whir.cpp:731-755) */
+    /* the fold schedule of Whir_Batch_Open (Whir.cpp:736-755) */
     int nk = 0, i = 1;
     uint64_t size = sizes[0];
     for (int t = 1; t < count; ++t)
@@ -852,7 +852,7 @@ int64_t or_whir_batch_open(int count, const or_fe *const *arrays, const uint64_t
             c1[0].real = 1; c1[0].img = 0; c2[0].real = 1; c2[0].img = 0;
             for (int i = 0; i < cnt; ++i) rec_fe(&R, c1[i]);
             for (int i = 0; i < cnt; ++i) rec_fe(&R, c2[i]);
-            /* aggregate_arrays(arr, b, arr_aggr, B_aggr, c2, c1) (:672-703) */
+            /* aggregate_arrays(arr, b, arr_aggr, B_aggr, c2, c1) (:670-703) */
             const or_fe **A = (const or_fe **)malloc(sizeof(or_fe *) * cnt);
             const or_fe **Bv = (const or_fe **)malloc(sizeof(or_fe *) * cnt);
             or_fe *a0 = (or_fe *)malloc(sizeof(or_fe) * S.n), *b0 = (or_fe *)malloc(sizeof(or_fe) * S.n);
@@ -875,11 +875,11 @@ int64_t or_whir_batch_open(int count, const or_fe *const *arrays, const uint64_t
                 }
             S.K = K;
             rec_fe(&R, K);
-            /* the next fold factor (:827-831); past the end of the schedule
+            /* the next fold factor (:826-829); past the end of the schedule
              * the reference reads beyond its vector, here the last entry */
             wk = iter + 1 < nk ? kap[iter + 1] : kap[nk - 1];
             if (ri + cnt - 1 < count && S.n / (uint64_t)wk <= sizes[ri + cnt - 1]) wk = (int)(S.n / sizes[ri + cnt - 1]);
-            /* aggregate_commitments(codeword, MT, cw_aggr, MT_aggr, c2, rate') (:590-645) */
+            /* aggregate_commitments(codeword, MT, cw_aggr, MT_aggr, c2, rate') (:547-606) */
             int rate2 = (int)(S.cw_n / S.n);
             for (int i = 1; i < cnt && !status; ++i)
                 if (cw_sizes[ri + i - 1] != S.cw_n) status = -6;
@@ -921,6 +921,38 @@ int64_t or_whir_batch_open(int count, const or_fe *const *arrays, const uint64_t
     if (status) return status;
     if (R.overflow) return -1;
     return (int64_t)R.n;
+}
+
+void or_whir_aggregate_claims(int narrays, const or_fe *const *arrays, uint64_t n, int npoints, const or_fe *points,
+                              or_fe *arr, or_fe *b_aggr, or_fe *K, or_fe *beta_ch, or_fe *arr_ch) {
+    /* aggregate_claims (Whir.cpp:608-668) with arr and b_aggr empty on entry */
+    int m = ilog2u(n);
+    or_fe **betas = (or_fe **)malloc(sizeof(or_fe *) * npoints);
+    for (int j = 0; j < npoints; ++j) {                     /* :615-622 */
+        betas[j] = (or_fe *)malloc(sizeof(or_fe) * n);
+        or_precompute_beta(points + (size_t)j * m, m, betas[j]);
+    }
+    or_fe *sums = (or_fe *)malloc(sizeof(or_fe) * narrays * npoints);
+    for (int i = 0; i < narrays; ++i)                       /* :625-633 */
+        for (int j = 0; j < npoints; ++j) {
+            or_fe s = { 0, 0 };
+            for (uint64_t k = 0; k < n; ++k) s = or_add(s, or_mul(betas[j][k], arrays[i][k]));
+            sums[i * npoints + j] = s;
+        }
+    for (int j = 0; j < npoints; ++j) beta_ch[j] = or_random_element();   /* :638-644 */
+    for (int i = 0; i < narrays; ++i) arr_ch[i] = or_random_element();
+    or_fe Ka = { 0, 0 };
+    for (int i = 0; i < narrays; ++i)                       /* :645-651 */
+        for (int j = 0; j < npoints; ++j)
+            Ka = or_add(Ka, or_mul(or_mul(beta_ch[j], arr_ch[i]), sums[i * npoints + j]));
+    for (uint64_t k = 0; k < n; ++k) { b_aggr[k].real = b_aggr[k].img = 0; arr[k].real = arr[k].img = 0; }
+    for (int j = 0; j < npoints; ++j)                       /* :652-656 */
+        for (uint64_t k = 0; k < n; ++k) b_aggr[k] = or_add(b_aggr[k], or_mul(beta_ch[j], betas[j][k]));
+    for (int i = 0; i < narrays; ++i)                       /* :657-661 */
+        for (uint64_t k = 0; k < n; ++k) arr[k] = or_add(arr[k], or_mul(arr_ch[i], arrays[i][k]));
+    *K = Ka;
+    for (int j = 0; j < npoints; ++j) free(betas[j]);
+    free(betas); free(sums);
 }
 
 /* ------------------------------------------------------------------ */

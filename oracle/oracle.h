@@ -108,14 +108,19 @@ int64_t or_whir_open(const or_fe *arr, uint64_t n, const or_fe *z, or_fe K,
 /* get_whir_rates (src/Whir.cpp:18-137): the rate each array (sizes sorted
  * largest first) is committed at, starting from `rate`. */
 void or_whir_rates(const uint64_t *sizes, int count, int rate, int whir_k, int *rates);
-/* Whir_Batch_Open's fold schedule (src/Whir.cpp:731-755): kappas[] and
+/* Whir_Batch_Open's fold schedule (src/Whir.cpp:736-755): kappas[] and
  * aggr[] per step; returns the step count (-1 beyond cap). */
 int or_whir_kappas(const uint64_t *sizes, int count, int whir_k, int *kappas, int *aggr, int cap);
-/* Whir_Batch_Open (src/Whir.cpp:705-899) with the prover's record; arrays
+/* Whir_Batch_Open (src/Whir.cpp:706-899) with the prover's record; arrays
  * sorted largest first, each with its codeword (cw_sizes[i] elements, the
  * rate from or_whir_rates), its dup-left tree, claim evals[i] and eq
  * weights betas[i]; rate = codeword 0's.  Returns the word count or a
  * negative status. */
+/* aggregate_claims (src/Whir.cpp:608-668) with arr, b_aggr empty on entry:
+ * npoints claim points of log2 n elements each; beta_ch[npoints],
+ * arr_ch[narrays] receive the libc-drawn challenges. */
+void or_whir_aggregate_claims(int narrays, const or_fe *const *arrays, uint64_t n, int npoints, const or_fe *points,
+                              or_fe *arr, or_fe *b_aggr, or_fe *K, or_fe *beta_ch, or_fe *arr_ch);
 int64_t or_whir_batch_open(int count, const or_fe *const *arrays, const uint64_t *sizes,
                            const or_fe *const *codewords, const uint64_t *cw_sizes, const uint8_t *const *trees,
                            const or_fe *evals, const or_fe *const *betas, int rate, int whir_k,

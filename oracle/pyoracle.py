@@ -107,6 +107,7 @@ def oracle():
         L.or_compute_secret_shares.argtypes = [vp, u64, i32, i32, i32, vp, vp]
         L.or_sumcheck_aggr.argtypes = [vp, i32, i32, vp, vp]
         L.or_whir_rates.argtypes = [vp, i32, i32, i32, vp]
+        L.or_whir_aggregate_claims.argtypes = [i32, vp, u64, i32, vp, vp, vp, vp, vp, vp]
         L.or_whir_kappas.argtypes = [vp, i32, i32, vp, vp, i32]
         L.or_whir_kappas.restype = i32
         L.or_whir_batch_open.argtypes = [i32, vp, vp, vp, vp, vp, vp, vp, i32, i32, vp, u64]
@@ -479,3 +480,16 @@ def whir_batch_open(inp, cap=1 << 22):
     if w < 0:
         return int(w), None
     return 0, rec[:w].copy()
+
+
+def whir_aggregate_claims(arrays, points):
+    """aggregate_claims: (arr, b_aggr, K, beta_ch, arr_ch); seed libc first."""
+    arrs = [np.ascontiguousarray(a, dtype=np.uint64) for a in arrays]
+    n = arrs[0].shape[0]
+    pts = np.ascontiguousarray(points, dtype=np.uint64)
+    arr, b = fe_array(n), fe_array(n)
+    K = fe_array(1)
+    bc, ac = fe_array(pts.shape[0]), fe_array(len(arrs))
+    oracle().or_whir_aggregate_claims(len(arrs), _ptrs(arrs), n, pts.shape[0], _p(pts), _p(arr), _p(b), _p(K),
+                                      _p(bc), _p(ac))
+    return arr, b, K[0], bc, ac

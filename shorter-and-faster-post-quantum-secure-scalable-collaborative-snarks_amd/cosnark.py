@@ -83,6 +83,7 @@ def lib():
         L.cosnark_cosumcheck3_phase2.argtypes = [vp, vp, vp, vp, vp, u64, i32, vp]
         L.cosnark_share_unpack.argtypes = [vp, i32, i32, vp]
         L.cosnark_whir_rates.argtypes = [vp, i32, i32, i32, vp]
+        L.cosnark_whir_aggregate_claims.argtypes = [vp, i32, vp, u64, i32, vp, vp, vp, vp, vp, vp]
         L.cosnark_whir_batch_open.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp, vp, i32, i32, vp, u64,
                                               ctypes.POINTER(u64)]
         L.cosnark_sumcheck_aggr.argtypes = [vp, i32, i32, vp, vp]
@@ -411,6 +412,26 @@ class Context:
         for b in ups:
             b.free()
         return rc, rec[:words.value].copy()
+
+    def whir_aggregate_claims(self, arrays, points):
+        """aggregate_claims (src/Whir.cpp:608-668): (arr, b_aggr, K, beta_ch,
+        arr_ch) for host arrays of equal size and claim points (npoints,
+        log2 n, 2).  Seed libc first."""
+        bufs = [self.upload(np.ascontiguousarray(a)) for a in arrays]
+        n = np.asarray(arrays[0]).shape[0]
+        pts = np.ascontiguousarray(points, dtype=np.uint64)
+        npts = pts.shape[0]
+        pa = (ctypes.c_void_p * len(arrays))(*[b.ptr.value for b in bufs])
+        da, db = self.alloc(16 * n), self.alloc(16 * n)
+        K = np.zeros(2, np.uint64)
+        bc = np.zeros((npts, 2), np.uint64)
+        ac = np.zeros((len(arrays), 2), np.uint64)
+        self._ck(lib().cosnark_whir_aggregate_claims(self.h, len(arrays), pa, n, npts, _p(pts), da.ptr, db.ptr,
+                                                      _p(K), _p(bc), _p(ac)))
+        arr, b = self.download(da, (n, 2)), self.download(db, (n, 2))
+        for x in bufs + [da, db]:
+            x.free()
+        return arr, b, K, bc, ac
 
     def cosumcheck3_phase1(self, p1, p2, p3, b1, d, dev=None):
         """Cubic block sumcheck phase 1: the 4d-element round polynomial (4d, 2).

@@ -67,7 +67,7 @@ __global__ __launch_bounds__(WT) void dot_kernel(const fe *__restrict__ a, const
 // Query i takes kappa replies cw[q_i + j cw_n/kappa].  The tree is the
 // codeword's dup-left tree (nodes, cw_n/4 leaves) or, when zero_levels is
 // given, a tree over all-zero leaf digests whose level-l nodes all equal
-// zero_levels[l] (the tree aggregate_commitments leaves, Whir.cpp:633-644).
+// zero_levels[l] (the tree aggregate_commitments leaves, Whir.cpp:591-605).
 __global__ __launch_bounds__(WT) void gather_kernel(const fe *__restrict__ cw, uint64_t cw_n, int kappa,
                                                     const uint8_t *__restrict__ nodes,
                                                     const uint8_t *__restrict__ zero_levels,
@@ -95,6 +95,12 @@ __global__ __launch_bounds__(WT) void gather_kernel(const fe *__restrict__ cw, u
             dst[1] = src[1];
         }
     }
+}
+
+// dst[x] = w * src[x]
+__global__ __launch_bounds__(WT) void scale_kernel(fe *dst, const fe *__restrict__ src, fe w, uint64_t n) {
+    for (uint64_t x = (uint64_t)blockIdx.x * WT + threadIdx.x; x < n; x += (uint64_t)gridDim.x * WT)
+        dst[x] = gf::mul(w, src[x]);
 }
 
 // dst[x] += w * src[x]: one term of aggregate_arrays / aggregate_commitments
@@ -222,7 +228,7 @@ std::vector<uint8_t> zero_chain(int levels) {
     return z;
 }
 
-// Whir_Batch_Open's fold schedule (Whir.cpp:731-755): kappas and aggregation
+// Whir_Batch_Open's fold schedule (Whir.cpp:736-755): kappas and aggregation
 // flags; -1 for input not sorted largest first or a schedule over 64 steps
 int whir_schedule(const uint64_t *sizes, int count, int whir_k, int *kap, int *agg) {
     for (int t = 1; t < count; ++t)
@@ -718,7 +724,7 @@ int cosnark_whir_batch_open(cosnark_ctx *ctx, int count, const uint64_t *const *
                 st.cw_n = cw_n;
                 st.eval_off = eval_off;
                 eval_off += (size_t)st.cnt * st.cnt;
-                // the next fold factor (:827-831); past the end of the schedule the
+                // the next fold factor (:826-829); past the end of the schedule the
                 // reference reads beyond its vector, here the last entry
                 wk = iter + 1 < nk ? kap[iter + 1] : kap[nk - 1];
                 if (ri + st.cnt - 1 < count && n / (uint64_t)wk <= sizes[ri + st.cnt - 1])
@@ -845,7 +851,7 @@ int cosnark_whir_batch_open(cosnark_ctx *ctx, int count, const uint64_t *const *
                 nd_cur = nd_new;
                 cw_slot = slot;
             } else if (st.kind == 1) {
-                // evals[i][j] = <A_i, B_j> before the running pair changes (:681-690)
+                // evals[i][j] = <A_i, B_j> before the running pair changes (:680-688)
                 for (int i = 0; i < st.cnt; ++i)
                     for (int j = 0; j < st.cnt; ++j) {
                         const fe *A = i ? (const fe *)d_arrays[st.first + i - 1] : arr;
@@ -854,7 +860,7 @@ int cosnark_whir_batch_open(cosnark_ctx *ctx, int count, const uint64_t *const *
                                            parts + PARTS * (st.eval_off + (size_t)i * st.cnt + j));
                         LAUNCHCK();
                     }
-                // auth paths on the running tree (aggregate_commitments :621-630)
+                // auth paths on the running tree (aggregate_commitments :576-589)
                 const int work = st.l * st.depth;
                 if (work > 0) {
                     hipLaunchKernelGGL(gather_kernel, dim3((work + WT - 1) / WT), dim3(WT), 0, s, cw_cur, st.cw_n, 0,
@@ -864,7 +870,7 @@ int cosnark_whir_batch_open(cosnark_ctx *ctx, int count, const uint64_t *const *
                                        paths + 32 * st.path_off);
                     LAUNCHCK();
                 }
-                // arr += c2_i A_i, b += c1_i B_i (:691-698); codeword += c2_i cw_i (:606-610)
+                // arr += c2_i A_i, b += c1_i B_i (:689-695); codeword += c2_i cw_i (:560-566)
                 for (int i = 1; i < st.cnt; ++i) {
                     hipLaunchKernelGGL(axpy_kernel, grid(st.n), dim3(WT), 0, s, arr,
                                        (const fe *)d_arrays[st.first + i - 1], st.c2[i], st.n);
@@ -886,7 +892,7 @@ int cosnark_whir_batch_open(cosnark_ctx *ctx, int count, const uint64_t *const *
                 cw_cur = cw_w;
                 nd_cur = nullptr;
             } else {
-                // verification step 1 (:850-856): arr's encoding at the final rate is the codeword
+                // verification step 1 (:846-853): arr's encoding at the final rate is the codeword
                 if ((rc = ntt_launch(ctx, arr, st.n, finbuf, ilog2(st.n * (uint64_t)st.rate), false, s))) return rc;
             }
         }
@@ -983,6 +989,86 @@ int cosnark_whir_batch_open(cosnark_ctx *ctx, int count, const uint64_t *const *
         if (overflow) return set_err(ctx, COSNARK_E_ARG, "whir_batch_open: record buffer too small");
         if (bad) return set_err(ctx, COSNARK_E_CHECK, "whir_batch_open: claim chain check failed");
     }
+    return COSNARK_OK;
+}
+
+int cosnark_whir_aggregate_claims(cosnark_ctx *ctx, int narrays, const uint64_t *const *d_arrays, uint64_t n,
+                                  int npoints, const uint64_t *h_points, uint64_t *d_arr, uint64_t *d_b_aggr,
+                                  uint64_t K[2], uint64_t *h_beta_ch, uint64_t *h_arr_ch) {
+    if (!ctx || !d_arrays || !h_points || !d_arr || !d_b_aggr || !K || !h_beta_ch || !h_arr_ch)
+        return set_err(ctx, COSNARK_E_ARG, "whir_aggregate_claims: null argument");
+    if (narrays < 1 || narrays > 256 || npoints < 1 || npoints > 256 || !is_pow2(n) || n < 2 || n > (1ull << 28))
+        return set_err(ctx, COSNARK_E_ARG, "whir_aggregate_claims: 1..256 arrays and points, n a power of two >= 2");
+    for (int i = 0; i < narrays; ++i)
+        if (!d_arrays[i]) return set_err(ctx, COSNARK_E_ARG, "whir_aggregate_claims: null array");
+    const int m = ilog2(n), lb = m / 2, hb = m - lb;
+    constexpr int PARTS = 64;
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) / 256 * 256; return o; };
+    const size_t o_beta = take(16 * n), o_lo = take(16 * ((size_t)1 << lb) * npoints),
+                 o_hi = take(16 * ((size_t)1 << hb) * npoints);
+    const size_t o_parts = take(16 * PARTS * (size_t)narrays * npoints);
+    void *base;
+    int rc = scratch_reserve(ctx, off, &base);
+    if (rc) return rc;
+    char *B = (char *)base;
+    fe *beta = (fe *)(B + o_beta), *dlo = (fe *)(B + o_lo), *dhi = (fe *)(B + o_hi), *parts = (fe *)(B + o_parts);
+    hipStream_t s = ctx->stream;
+    std::vector<fe> hlo, hhi;
+    for (int j = 0; j < npoints; ++j) {                                  // precompute_beta per point (:615-622)
+        const fe *z = (const fe *)h_points + (size_t)j * m;
+        std::vector<fe> lo = eq_half(z, 0, lb), hi = eq_half(z, lb, hb);
+        hlo.insert(hlo.end(), lo.begin(), lo.end());
+        hhi.insert(hhi.end(), hi.begin(), hi.end());
+    }
+    HIPCK(hipMemcpyAsync(dlo, hlo.data(), sizeof(fe) * hlo.size(), hipMemcpyHostToDevice, s));
+    HIPCK(hipMemcpyAsync(dhi, hhi.data(), sizeof(fe) * hhi.size(), hipMemcpyHostToDevice, s));
+    // Sums[i][j] = <beta_j, arrays[i]> (:625-633), one eq table at a time
+    auto grid = [](uint64_t x) { uint64_t g = (x + WT - 1) / WT; return dim3((unsigned)(g < 4096 ? g : 4096)); };
+    for (int j = 0; j < npoints; ++j) {
+        hipLaunchKernelGGL(beta_kernel, dim3(1024), dim3(WT), 0, s, (const fe *)(dlo + ((size_t)j << lb)),
+                           (const fe *)(dhi + ((size_t)j << hb)), beta, n, lb);
+        LAUNCHCK();
+        for (int i = 0; i < narrays; ++i) {
+            hipLaunchKernelGGL(dot_kernel, dim3(PARTS), dim3(WT), 0, s, (const fe *)beta, (const fe *)d_arrays[i], n,
+                               parts + PARTS * ((size_t)i * npoints + j));
+            LAUNCHCK();
+        }
+    }
+    // the verifier's challenges (:638-644)
+    std::vector<fe> bc(npoints), ac(narrays);
+    for (auto &x : bc) x = libc_random_element();
+    for (auto &x : ac) x = libc_random_element();
+    // b_aggr = sum_j bc_j beta_j (:652-656), arr = sum_i ac_i arrays_i (:657-661)
+    for (int j = 0; j < npoints; ++j) {
+        hipLaunchKernelGGL(beta_kernel, dim3(1024), dim3(WT), 0, s, (const fe *)(dlo + ((size_t)j << lb)),
+                           (const fe *)(dhi + ((size_t)j << hb)), beta, n, lb);
+        LAUNCHCK();
+        if (j == 0) hipLaunchKernelGGL(scale_kernel, grid(n), dim3(WT), 0, s, (fe *)d_b_aggr, (const fe *)beta, bc[j], n);
+        else hipLaunchKernelGGL(axpy_kernel, grid(n), dim3(WT), 0, s, (fe *)d_b_aggr, (const fe *)beta, bc[j], n);
+        LAUNCHCK();
+    }
+    for (int i = 0; i < narrays; ++i) {
+        if (i == 0)
+            hipLaunchKernelGGL(scale_kernel, grid(n), dim3(WT), 0, s, (fe *)d_arr, (const fe *)d_arrays[i], ac[i], n);
+        else
+            hipLaunchKernelGGL(axpy_kernel, grid(n), dim3(WT), 0, s, (fe *)d_arr, (const fe *)d_arrays[i], ac[i], n);
+        LAUNCHCK();
+    }
+    std::vector<fe> hp(PARTS * (size_t)narrays * npoints);
+    HIPCK(hipMemcpyAsync(hp.data(), parts, sizeof(fe) * hp.size(), hipMemcpyDeviceToHost, s));
+    HIPCK(hipStreamSynchronize(s));
+    fe Ka = gf::zero();                                                  // :645-651
+    for (int i = 0; i < narrays; ++i)
+        for (int j = 0; j < npoints; ++j) {
+            fe e = gf::zero();
+            for (int u = 0; u < PARTS; ++u) e = gf::add(e, hp[PARTS * ((size_t)i * npoints + j) + u]);
+            Ka = gf::add(Ka, gf::mul(gf::mul(bc[j], ac[i]), e));
+        }
+    K[0] = Ka.re;
+    K[1] = Ka.im;
+    for (int j = 0; j < npoints; ++j) { h_beta_ch[2 * j] = bc[j].re; h_beta_ch[2 * j + 1] = bc[j].im; }
+    for (int i = 0; i < narrays; ++i) { h_arr_ch[2 * i] = ac[i].re; h_arr_ch[2 * i + 1] = ac[i].im; }
     return COSNARK_OK;
 }
 

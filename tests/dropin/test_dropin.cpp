@@ -264,6 +264,83 @@ int main() {
                          reinterpret_cast<or_fe *>(oagg.data()));
         CHECK(agg.size() == 64 && same(agg.data(), oagg.data(), 16 * 64), "sumcheck_aggr over 8 ranks' polynomials");
     }
+    // batched WHIR: rates, Whir_Batch_Open, aggregate_claims
+    {
+        const std::vector<int> logs = {12, 10, 8, 8};
+        std::vector<int> sizes;
+        for (int lg : logs) sizes.push_back(1 << lg);
+        std::vector<int> rates = D::get_whir_rates(sizes, 4);
+        std::vector<uint64_t> usz(sizes.begin(), sizes.end());
+        std::vector<int> orates(sizes.size());
+        or_whir_rates(usz.data(), (int)usz.size(), 4, 4, orates.data());
+        CHECK(rates == orates, "get_whir_rates");
+        vector<vector<F>> arrays, codewords, betas;
+        vector<vector<vector<_hash>>> MTs;
+        vector<F> evals;
+        vector<vector<uint8_t>> flat;
+        for (size_t i = 0; i < logs.size(); ++i) {
+            arrays.push_back(synth(seed, 80 + i, sizes[i]));
+            vector<F> z = synth(seed, 90 + i, logs[i]), b(sizes[i]), cw;
+            or_precompute_beta(reinterpret_cast<or_fe *>(z.data()), logs[i], reinterpret_cast<or_fe *>(b.data()));
+            or_fe K = {0, 0};
+            for (int k = 0; k < sizes[i]; ++k) K = or_add(K, or_mul(O(arrays[i][k]), O(b[k])));
+            evals.push_back(F{K.real, K.img});
+            betas.push_back(b);
+            vector<vector<_hash>> mt;
+            D::Whir_Commit_initial(arrays[i], cw, mt, rates[i], 4);
+            codewords.push_back(cw);
+            MTs.push_back(mt);
+            vector<uint8_t> nodes(32 * or_merkle_node_count(cw.size()));
+            or_merkle_commit(reinterpret_cast<const or_fe *>(cw.data()), cw.size(), nodes.data(), 1);
+            flat.push_back(nodes);
+        }
+        vector<const or_fe *> oa, oc, ob;
+        vector<const uint8_t *> ot;
+        vector<uint64_t> ocs;
+        for (size_t i = 0; i < logs.size(); ++i) {
+            oa.push_back(reinterpret_cast<const or_fe *>(arrays[i].data()));
+            oc.push_back(reinterpret_cast<const or_fe *>(codewords[i].data()));
+            ob.push_back(reinterpret_cast<const or_fe *>(betas[i].data()));
+            ot.push_back(flat[i].data());
+            ocs.push_back(codewords[i].size());
+        }
+        vector<uint64_t> orec(1 << 20);
+        srandom(41);
+        const int64_t ow = or_whir_batch_open((int)logs.size(), oa.data(), usz.data(), oc.data(), ocs.data(), ot.data(),
+                                              reinterpret_cast<const or_fe *>(evals.data()), ob.data(), 4, 4,
+                                              orec.data(), orec.size());
+        srandom(41);
+        D::Whir_Batch_Open(arrays, codewords, MTs, evals, betas, 4, true);
+        const auto &rec = D::last_whir_record();
+        CHECK(ow > 0 && (int64_t)rec.size() == ow && same(rec.data(), orec.data(), 8 * ow) && arrays.empty(),
+              "Whir_Batch_Open 2^12, 2^10, 2^8, 2^8");
+        // aggregate_claims over three arrays of 2^8 with two points each
+        vector<vector<F>> arrs = {synth(seed, 100, 256), synth(seed, 101, 256), synth(seed, 102, 256)};
+        vector<vector<std::pair<F, vector<F>>>> claims(3);
+        vector<F> pts;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 2; ++j) {
+                vector<F> z = synth(seed, 110 + 2 * i + j, 8);
+                claims[i].push_back({F{0, 0}, z});
+                pts.insert(pts.end(), z.begin(), z.end());
+            }
+        vector<F> arr, b_aggr, oarr(256), ob2(256), obc(6), oac(3);
+        F K;
+        srandom(43);
+        auto ch = D::aggregate_claims(arrs, arr, b_aggr, claims, K);
+        vector<const or_fe *> oas = {reinterpret_cast<const or_fe *>(arrs[0].data()),
+                                     reinterpret_cast<const or_fe *>(arrs[1].data()),
+                                     reinterpret_cast<const or_fe *>(arrs[2].data())};
+        or_fe oK;
+        srandom(43);
+        or_whir_aggregate_claims(3, oas.data(), 256, 6, reinterpret_cast<const or_fe *>(pts.data()),
+                                 reinterpret_cast<or_fe *>(oarr.data()), reinterpret_cast<or_fe *>(ob2.data()), &oK,
+                                 reinterpret_cast<or_fe *>(obc.data()), reinterpret_cast<or_fe *>(oac.data()));
+        CHECK(ch.size() == 2 && same(ch[0].data(), obc.data(), 16 * 6) && same(ch[1].data(), oac.data(), 16 * 3) &&
+                  same(arr.data(), oarr.data(), 16 * 256) && same(b_aggr.data(), ob2.data(), 16 * 256) &&
+                  K.real == oK.real && K.img == oK.img,
+              "aggregate_claims 3 arrays x 2 points");
+    }
     // RS_multifold5
     {
         vector<F> cw = synth(seed, 40, 1 << 14), a = synth(seed, 41, 32), ref(cw.size() / 32);

@@ -1,4 +1,4 @@
-"""Batched WHIR (Whir_Batch_Open, src/Whir.cpp:705-899) on the GPU against the
+"""Batched WHIR (Whir_Batch_Open, src/Whir.cpp:706-899) on the GPU against the
 oracle: the whole opening record, bit for bit, for fold factors 2, 4 and 8,
 aggregations of equal and smaller arrays (including one at the first step
 with kappa = 1 and one at the last scheduled step), and the error behaviour
@@ -77,3 +77,17 @@ def test_batch_open_rejects_unsorted(ctx):
     O.libc().srandom(3)
     rc, _ = _gpu(ctx, inp)
     assert rc == cosnark.E_ARG
+
+
+@pytest.mark.parametrize("narr,npts,logn", [(1, 1, 6), (3, 2, 12), (4, 8, 16)])
+def test_aggregate_claims_matches_oracle(ctx, narr, npts, logn):
+    print(f"aggregate_claims narr={narr} npts={npts} logn={logn}", flush=True)
+    n = 1 << logn
+    arrays = [O.synth(0xC05A0054 + logn, i, n) for i in range(narr)]
+    points = np.stack([O.synth(0xC05A0055 + logn, j, logn) for j in range(npts)])
+    O.libc().srandom(23)
+    want = O.whir_aggregate_claims(arrays, points)
+    O.libc().srandom(23)
+    got = ctx.whir_aggregate_claims(arrays, points)
+    for g, w in zip(got, want):
+        assert np.array_equal(np.asarray(g, np.uint64).reshape(-1), np.asarray(w, np.uint64).reshape(-1))

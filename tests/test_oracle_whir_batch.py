@@ -1,6 +1,6 @@
 """The oracle's batched WHIR (src/Whir.cpp): get_whir_rates (:18-137), the
-fold schedule and aggregation of Whir_Batch_Open (:705-899),
-aggregate_arrays (:672-703) and aggregate_commitments (:590-645), with
+fold schedule and aggregation of Whir_Batch_Open (:706-899),
+aggregate_arrays (:670-703) and aggregate_commitments (:547-606), with
 Whir_Iteration at any fold factor kappa.
 
 Pins: the schedules against hand-derived values; the reference's own checks
@@ -119,3 +119,26 @@ def test_library_whir_rates_reject_unsorted():
     import cosnark
     with pytest.raises(cosnark.CosnarkError):
         cosnark.whir_rates([1 << 8, 1 << 10], 4, 4)
+
+
+@pytest.mark.parametrize("narr,npts,logn", [(1, 1, 6), (3, 2, 8), (2, 5, 10)])
+def test_aggregate_claims_closed_form(narr, npts, logn):
+    n = 1 << logn
+    arrays = [O.synth(1500 + logn, i, n) for i in range(narr)]
+    points = np.stack([O.synth(1600 + logn, j, logn) for j in range(npts)])
+    O.libc().srandom(17)
+    arr, b, K, bc, ac = O.whir_aggregate_claims(arrays, points)
+    # the challenges: npts for the weights, then narr for the arrays, from libc
+    O.libc().srandom(17)
+    draws = [O.unfe(O.oracle().or_random_element()) for _ in range(npts + narr)]
+    assert [tuple(int(v) for v in x) for x in bc] == [tuple(int(v) for v in d) for d in draws[:npts]]
+    assert [tuple(int(v) for v in x) for x in ac] == [tuple(int(v) for v in d) for d in draws[npts:]]
+    # K = <arr, b_aggr> = sum_ij bc_j ac_i MLE(A_i)(z_j)
+    assert np.array_equal(O.inner_product(arr, b), K)
+    want = np.zeros(2, np.uint64)
+    for i in range(narr):
+        for j in range(npts):
+            e = np.array(O.evaluate_vector(arrays[i], points[j]), np.uint64).reshape(1, 2)
+            w = O.vec_op("mul", bc[j:j + 1], ac[i:i + 1])
+            want = O.vec_op("add", want.reshape(1, 2), O.vec_op("mul", w, e)).reshape(2)
+    assert np.array_equal(want, K)
