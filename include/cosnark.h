@@ -99,6 +99,22 @@ int cosnark_merkle_tree_from_leaves(cosnark_ctx *ctx, uint64_t n_leaves, uint8_t
 int cosnark_merkle_open(cosnark_ctx *ctx, const uint8_t *d_nodes, uint64_t n_elems,
                         uint64_t pos, uint8_t *h_path, int *depth);
 
+/* Authentication-path verification, q paths in one launch: what
+ * verify_claim_opt_blake (src/merkle_tree.cpp:337-383) stands for.  The
+ * reference's version hashes with my_hhash, reads siblings from the prover's
+ * tree and returns true without comparing against the root; this one checks
+ * each path against the root.  Path j: the leaf's 4 elements at
+ * d_leaves[4j..4j+3] (BLAKE3 of their 64 bytes), its position d_pos[j]
+ * (< 2^depth), depth sibling digests at d_paths + 32*depth*j (level 0 first,
+ * as cosnark_merkle_open gives them), parents H(left || left) when dup_left
+ * (the reference's trees: there the root depends on leaf 0 alone, so a
+ * path at position p checks its leaf only when p = 0 and its level-L
+ * sibling only when p >> L = 1) or H(left || right).  d_ok[j] = 1 when the path reaches d_root (32
+ * device bytes); *h_failures counts the others. */
+int cosnark_merkle_verify(cosnark_ctx *ctx, const uint64_t *d_leaves, const uint32_t *d_pos, const uint8_t *d_paths,
+                          int depth, uint64_t q, const uint8_t *d_root, int dup_left, uint8_t *d_ok,
+                          uint64_t *h_failures);
+
 /* ---- Fiat-Shamir transcript, src/Fiat_Shamir.cpp ----------------------
  * The reference keeps one global running value prev_challenge (starts at 0,
  * Fiat_Shamir.cpp:5); here it is explicit state owned by the caller. */
@@ -138,6 +154,16 @@ uint64_t cosnark_plain_proof_bytes(uint64_t n);
 int cosnark_prove_plain(cosnark_ctx *ctx, const uint64_t *d_f, const uint64_t *d_g,
                         uint64_t n, int rate, cosnark_transcript *ts,
                         uint8_t *h_proof, double *times);
+/* The verifier of a plain proof's sumcheck (the checks the reference's
+ * prover makes, src/Distributed_Sumcheck.cpp:243-248): with y the claimed
+ * sum and the transcript t as the prover started it, each round's
+ * polynomial (a, b, c) must be canonical and satisfy H(0) + H(1) = y; then
+ * r = hash_to_field({a, b, c}) and y = H(r); at the end y = v1[0] v2[0].
+ * Host only.  h_r (optional) receives the m challenges; *bad_round (optional)
+ * the failing round (m for the final product).  COSNARK_E_CHECK on failure.
+ * The root is not checked: a plain proof carries no openings. */
+int cosnark_verify_plain(const uint8_t *h_proof, uint64_t n, const uint64_t y[2], cosnark_transcript *t,
+                         uint64_t *h_r, int *bad_round);
 
 /* ---- WHIR (whir_k = 4), src/Whir.cpp -----------------------------------
  * Whir_Commit_initial (:158-190): d_codeword = RS encode of d_arr at `rate`

@@ -83,6 +83,8 @@ def lib():
         L.cosnark_cosumcheck3_phase2.argtypes = [vp, vp, vp, vp, vp, u64, i32, vp]
         L.cosnark_share_unpack.argtypes = [vp, i32, i32, vp]
         L.cosnark_whir_rates.argtypes = [vp, i32, i32, i32, vp]
+        L.cosnark_merkle_verify.argtypes = [vp, vp, vp, vp, i32, u64, vp, i32, vp, ctypes.POINTER(u64)]
+        L.cosnark_verify_plain.argtypes = [vp, u64, vp, vp, vp, ctypes.POINTER(i32)]
         L.cosnark_whir_aggregate_claims.argtypes = [vp, i32, vp, u64, i32, vp, vp, vp, vp, vp, vp]
         L.cosnark_whir_batch_open.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp, vp, i32, i32, vp, u64,
                                               ctypes.POINTER(u64)]
@@ -433,6 +435,24 @@ class Context:
             x.free()
         return arr, b, K, bc, ac
 
+    def merkle_verify(self, leaves, pos, paths, root, dup_left=True):
+        """Authentication paths on the GPU: leaves (q, 4, 2), pos (q,), paths
+        (q, depth, 32), root (32,) -> per-path 0/1 flags."""
+        lv = np.ascontiguousarray(leaves, dtype=np.uint64)
+        ps = np.ascontiguousarray(pos, dtype=np.uint32)
+        pt = np.ascontiguousarray(paths, dtype=np.uint8)
+        q, depth = ps.shape[0], (pt.shape[1] if pt.ndim == 3 else 0)
+        bufs = [self.upload(x) for x in (lv, ps, pt if pt.size else np.zeros(32, np.uint8),
+                                         np.ascontiguousarray(root, dtype=np.uint8))]
+        ok = self.alloc(max(q, 1))
+        bad = ctypes.c_uint64(0)
+        self._ck(lib().cosnark_merkle_verify(self.h, bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, depth, q, bufs[3].ptr,
+                                              int(dup_left), ok.ptr, ctypes.byref(bad)))
+        flags = self.download(ok, (max(q, 1),), np.uint8)[:q]
+        for b in bufs + [ok]:
+            b.free()
+        return flags, int(bad.value)
+
     def cosumcheck3_phase1(self, p1, p2, p3, b1, d, dev=None):
         """Cubic block sumcheck phase 1: the 4d-element round polynomial (4d, 2).
         dev: (four device buffers, M) already resident."""
@@ -668,3 +688,15 @@ def whir_rates(sizes, rate, whir_k=4):
     if rc:
         raise CosnarkError(rc, "whir_rates: sizes must be sorted largest first")
     return [int(x) for x in out]
+
+
+def verify_plain(proof, n, y, ts=None):
+    """cosnark_verify_plain: (status, challenges, bad_round)."""
+    pr = np.frombuffer(bytes(proof), dtype=np.uint8).copy()
+    yy = np.ascontiguousarray(np.asarray(y, dtype=np.uint64).reshape(2))
+    t = ts if ts is not None else Transcript()
+    m = int(n).bit_length() - 1
+    r = np.zeros((m, 2), np.uint64)
+    bad = ctypes.c_int(0)
+    rc = lib().cosnark_verify_plain(_p(pr), n, _p(yy), ctypes.byref(t), _p(r), ctypes.byref(bad))
+    return rc, r, bad.value
