@@ -93,6 +93,61 @@ def whir_leg(ctx, reps=3):
             "iterations": int(rec[0]), "record_words": int(rec.shape[0])}
 
 
+def whir_batch_leg(ctx, reps=3):
+    """Whir_Batch_Open as the aggregator runs it: arrays of 2^22, 2^20, 2^20
+    and 2^18 elements, each committed at get_whir_rates' rate (blowup 4 for
+    the largest), whir_k 4, libc seed 0xC05A0007; one opening merges the
+    equal-size arrays as the fold reaches them.  Inputs, codewords and trees
+    resident in HBM; the time is the opening (commits reported apart)."""
+    import ctypes
+    seed, rate = 0xC05A0007, 4
+    logs = [22, 20, 20, 18]
+    sizes = [1 << lg for lg in logs]
+    rates = cosnark.whir_rates(sizes, rate)
+    L = cosnark.lib()
+    keep, arrs, cws, nds, bets = [], [], [], [], []
+    evals = np.zeros((len(logs), 2), np.uint64)
+    commit_ms = 0.0
+    for i, (lg, r) in enumerate(zip(logs, rates)):
+        n = 1 << lg
+        a = ctx.upload(cosnark.synth(seed, 2 * i, n))
+        z = np.ascontiguousarray(cosnark.synth(seed, 2 * i + 1, lg))
+        b = ctx.alloc(16 * n)
+        ctx._ck(L.cosnark_precompute_beta(ctx.h, cosnark._p(z), lg, b.ptr))
+        evals[i] = ctx.whir_claim(a, n, z)
+        cw = ctx.alloc(16 * n * r)
+        nd = ctx.alloc(32 * cosnark.merkle_node_count(n * r))
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx._ck(L.cosnark_whir_commit(ctx.h, a.ptr, n, r, cw.ptr, nd.ptr))
+        commit_ms += (time.perf_counter() - t0) * 1e3
+        keep += [a, b, cw, nd]
+        arrs.append(a.ptr.value); bets.append(b.ptr.value); cws.append(cw.ptr.value); nds.append(nd.ptr.value)
+    P = lambda v: (ctypes.c_void_p * len(v))(*v)
+    sz = np.ascontiguousarray(sizes, dtype=np.uint64)
+    cs = np.ascontiguousarray([s_ * r for s_, r in zip(sizes, rates)], dtype=np.uint64)
+    rec = np.zeros(1 << 22, np.uint64)
+    words = ctypes.c_uint64(0)
+    libc = ctypes.CDLL("libc.so.6")
+    times = []
+    for _ in range(reps):
+        libc.srandom(seed)
+        ctx.sync()
+        t0 = time.perf_counter()
+        rc = L.cosnark_whir_batch_open(ctx.h, len(logs), P(arrs), cosnark._p(sz), P(cws), cosnark._p(cs), P(nds),
+                                       cosnark._p(evals), P(bets), rate, 4, cosnark._p(rec), rec.shape[0],
+                                       ctypes.byref(words))
+        times.append((time.perf_counter() - t0) * 1e3)
+        if rc != 0:
+            raise RuntimeError(f"whir_batch_open failed: {rc}")
+    for b in keep:
+        b.free()
+    return {"workload": "Whir_Batch_Open of 2^22, 2^20, 2^20, 2^18 (rates " + ", ".join(map(str, rates)) +
+                        "), whir_k 4, down to 4 elements",
+            "commit_ms": round(commit_ms, 3), "open_ms": round(min(times), 3), "steps": int(rec[0]),
+            "record_words": int(words.value)}
+
+
 def cosumcheck_leg(ctx, reps=3):
     """The coSumcheck rounds of one rank's 2^22-element share at the
     reference's degrees (src/coPIOP.cpp:1409-1420): cubic over (vL, vR, vO,
@@ -370,6 +425,8 @@ def main():
     df.free(); dg.free()
     if not a.no_whir:
         out["whir_cfg3"] = whir_leg(ctx)
+    if not a.no_whir:
+        out["whir_batch_open"] = whir_batch_leg(ctx)
     if not a.no_cosc:
         out["cosumcheck_share"] = cosumcheck_leg(ctx)
     if not a.no_batch:

@@ -221,6 +221,10 @@ int cosnark_whir_rates(const uint64_t *sizes, int count, int rate, int whir_k, i
  * fails (a round, the closing K = <arr, b>, the closing codeword = RS(arr))
  * or where the reference exits (a query beyond the tree, codeword sizes
  * that differ at an aggregation).  whir_k a power of two in [2, 64]. */
+/* precompute_beta(r, B), src/utils.cpp:198-243: the 2^m eq weights
+ * B[x] = prod_k (bit_k(x) ? r_k : 1 - r_k) of the point h_z (m elements,
+ * r_0 on bit 0) into d_out.  1 <= m <= 28. */
+int cosnark_precompute_beta(cosnark_ctx *ctx, const uint64_t *h_z, int m, uint64_t *d_out);
 /* aggregate_claims (src/Whir.cpp:608-668) for claims on narrays arrays of n
  * elements (device pointers): the eq weights beta_j of each of the npoints
  * claim points (h_points: npoints x log2 n elements, point j's coordinates

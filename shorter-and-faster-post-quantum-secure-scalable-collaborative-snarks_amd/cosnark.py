@@ -83,6 +83,7 @@ def lib():
         L.cosnark_cosumcheck3_phase2.argtypes = [vp, vp, vp, vp, vp, u64, i32, vp]
         L.cosnark_share_unpack.argtypes = [vp, i32, i32, vp]
         L.cosnark_whir_rates.argtypes = [vp, i32, i32, i32, vp]
+        L.cosnark_precompute_beta.argtypes = [vp, vp, i32, vp]
         L.cosnark_merkle_verify.argtypes = [vp, vp, vp, vp, i32, u64, vp, i32, vp, ctypes.POINTER(u64)]
         L.cosnark_verify_plain.argtypes = [vp, u64, vp, vp, vp, ctypes.POINTER(i32)]
         L.cosnark_whir_aggregate_claims.argtypes = [vp, i32, vp, u64, i32, vp, vp, vp, vp, vp, vp]

@@ -1072,4 +1072,21 @@ int cosnark_whir_aggregate_claims(cosnark_ctx *ctx, int narrays, const uint64_t 
     return COSNARK_OK;
 }
 
+int cosnark_precompute_beta(cosnark_ctx *ctx, const uint64_t *h_z, int m, uint64_t *d_out) {
+    if (!ctx || !h_z || !d_out) return set_err(ctx, COSNARK_E_ARG, "precompute_beta: null argument");
+    if (m < 1 || m > NTT_LOG_MAX) return set_err(ctx, COSNARK_E_ARG, "precompute_beta: 1 <= m <= 28");
+    const int lb = m / 2, hb = m - lb;
+    std::vector<fe> lo = eq_half((const fe *)h_z, 0, lb), hi = eq_half((const fe *)h_z, lb, hb);
+    void *base;
+    int rc = scratch_reserve(ctx, sizeof(fe) * (lo.size() + hi.size()), &base);
+    if (rc) return rc;
+    fe *dlo = (fe *)base, *dhi = dlo + lo.size();
+    HIPCK(hipMemcpyAsync(dlo, lo.data(), sizeof(fe) * lo.size(), hipMemcpyHostToDevice, ctx->stream));
+    HIPCK(hipMemcpyAsync(dhi, hi.data(), sizeof(fe) * hi.size(), hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(beta_kernel, dim3(1024), dim3(WT), 0, ctx->stream, (const fe *)dlo, (const fe *)dhi,
+                       (fe *)d_out, 1ull << m, lb);
+    LAUNCHCK();
+    return cosnark_sync(ctx);
+}
+
 }  // extern "C"

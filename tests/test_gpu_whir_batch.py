@@ -91,3 +91,13 @@ def test_aggregate_claims_matches_oracle(ctx, narr, npts, logn):
     got = ctx.whir_aggregate_claims(arrays, points)
     for g, w in zip(got, want):
         assert np.array_equal(np.asarray(g, np.uint64).reshape(-1), np.asarray(w, np.uint64).reshape(-1))
+
+
+@pytest.mark.parametrize("m", [1, 5, 12, 20])
+def test_precompute_beta_matches_oracle(ctx, m):
+    z = O.synth(0xC05A0056, m, m)
+    out = ctx.alloc(16 << m)
+    ctx._ck(cosnark.lib().cosnark_precompute_beta(ctx.h, cosnark._p(np.ascontiguousarray(z)), m, out.ptr))
+    got = ctx.download(out, (1 << m, 2))
+    out.free()
+    assert np.array_equal(got, O.precompute_beta(z))
