@@ -31,7 +31,7 @@ namespace {
 constexpr int SC_THREADS = 256;
 constexpr int SC_TAIL_THREADS = 1024;
 constexpr int SC_TAIL_ELEMS = 4096;     // LDS-resident vectors (2 x 64 KiB)
-constexpr int SC_ITEMS = 4;             // pairs per thread per sweep in the multi-workgroup rounds
+constexpr int SC_ITEMS = 1;             // pairs per thread per sweep in the multi-workgroup rounds
 
 __device__ __forceinline__ fe fold1(const fe &e, const fe &o, const fe &r) {
     return gf::add(gf::mul(r, gf::sub(o, e)), e);
