@@ -406,6 +406,11 @@ void cosnark_server_destroy(cosnark_server *s);
  * creating batches (a captured graph keeps the plan it was captured with). */
 int cosnark_autotune(cosnark_ctx *ctx, int logn_lo, int logn_hi, int batch, double *best_ms);
 int cosnark_ntt_plan(cosnark_ctx *ctx, int logn, int *passbits, int *tile);
+/* Sets the plan for 2^logn-point transforms (as cosnark_autotune would):
+ * passes of at most `passbits` bits (2..8), tiles of `tile` elements
+ * (a power of two in 256..4096).  For callers that tune end to end, e.g.
+ * the encode beside an overlapped sumcheck. */
+int cosnark_set_ntt_plan(cosnark_ctx *ctx, int logn, int passbits, int tile);
 /* Measured throughput of the library's GF(p^2) multiply on this device
  * (independent multiply chains, no memory traffic): the compute ceiling the
  * transform kernels are reported against.  No reference counterpart. */

@@ -84,6 +84,7 @@ def lib():
         L.cosnark_share_unpack.argtypes = [vp, i32, i32, vp]
         L.cosnark_whir_rates.argtypes = [vp, i32, i32, i32, vp]
         L.cosnark_precompute_beta.argtypes = [vp, vp, i32, vp]
+        L.cosnark_set_ntt_plan.argtypes = [vp, i32, i32, i32]
         L.cosnark_merkle_verify.argtypes = [vp, vp, vp, vp, i32, u64, vp, i32, vp, ctypes.POINTER(u64)]
         L.cosnark_verify_plain.argtypes = [vp, u64, vp, vp, vp, ctypes.POINTER(i32)]
         L.cosnark_whir_aggregate_claims.argtypes = [vp, i32, vp, u64, i32, vp, vp, vp, vp, vp, vp]
@@ -534,6 +535,9 @@ class Context:
         b, t = ctypes.c_int(0), ctypes.c_int(0)
         self._ck(lib().cosnark_ntt_plan(self.h, logn, ctypes.byref(b), ctypes.byref(t)))
         return b.value, t.value
+
+    def set_ntt_plan(self, logn, passbits, tile):
+        self._ck(lib().cosnark_set_ntt_plan(self.h, logn, passbits, tile))
 
     def batch(self, sizes, rate):
         return Batch(self, sizes, rate)

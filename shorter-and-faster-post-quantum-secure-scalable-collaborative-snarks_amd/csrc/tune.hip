@@ -143,4 +143,13 @@ int cosnark_ntt_plan(cosnark_ctx *ctx, int logn, int *passbits, int *tile) {
     return COSNARK_OK;
 }
 
+int cosnark_set_ntt_plan(cosnark_ctx *ctx, int logn, int passbits, int tile) {
+    if (!ctx || logn < 1 || logn > NTT_LOG_MAX) return COSNARK_E_ARG;
+    if (passbits < 2 || passbits > NTT_PASS_LOG_MAX || tile < 256 || tile > NTT_LDS_ELEMS || !is_pow2((uint64_t)tile))
+        return set_err(ctx, COSNARK_E_ARG, "set_ntt_plan: passbits in [2, 8], tile a power of two in [256, 4096]");
+    ctx->ntt_plan[logn].passbits = passbits;
+    ctx->ntt_plan[logn].tile = tile;
+    return COSNARK_OK;
+}
+
 }  // extern "C"
