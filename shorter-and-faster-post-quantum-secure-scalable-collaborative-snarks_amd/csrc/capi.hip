@@ -433,14 +433,18 @@ int cosnark_prove_plain(cosnark_ctx *ctx, const uint64_t *d_f, const uint64_t *d
     HIPCK(hipEventRecord(ctx->ev[7], ctx->stream));
     if (s1 != ctx->stream) HIPCK(hipStreamWaitEvent(s1, ctx->ev[7], 0));
     HIPCK(hipEventRecord(ctx->ev[0], s1));
-    if ((rc = ntt_launch(ctx, (const fe *)d_f, n, cw, ilog2(cw_n), false, s1))) return rc;
+    bool leaves_done = false;
+    if ((rc = ntt_launch_leaves(ctx, (const fe *)d_f, n, cw, ilog2(cw_n), nd, &leaves_done, s1))) return rc;
     HIPCK(hipEventRecord(ctx->ev[1], s1));
     HIPCK(hipStreamWaitEvent(s2, ctx->ev[7], 0));
     HIPCK(hipEventRecord(ctx->ev[4], s2));
     if ((rc = sumcheck_launch(ctx, (const fe *)d_f, (const fe *)d_g, n, m, scb, s2))) return rc;
     HIPCK(hipEventRecord(ctx->ev[3], s2));
     HIPCK(hipEventRecord(ctx->ev[6], s1));
-    if ((rc = merkle_launch(ctx, cw, cw_n, nd, true, s1))) return rc;
+    // the leaf level came from the encode's last pass where the plan allowed it
+    if ((rc = leaves_done ? merkle_from_leaves_launch(ctx, cw_n / 4, nd, true, s1)
+                          : merkle_launch(ctx, cw, cw_n, nd, true, s1)))
+        return rc;
     HIPCK(hipEventRecord(ctx->ev[2], s1));
     HIPCK(hipStreamWaitEvent(ctx->stream, ctx->ev[2], 0));
     HIPCK(hipStreamWaitEvent(ctx->stream, ctx->ev[3], 0));

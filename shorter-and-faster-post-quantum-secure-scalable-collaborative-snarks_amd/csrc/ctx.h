@@ -94,9 +94,17 @@ int ntt_launch(cosnark_ctx *ctx, const gf::fe *in, uint64_t in_len, gf::fe *out,
 // in_tab[y] (device array; or in + y * in_len when in_tab is null) and
 // writes out_tab[y] (or out + y * 2^logn).  Passes before the last use
 // tmp (batch * 2^logn elements) or, when null, the context's buffer.
+// leaves (optional): the last pass also writes item 0's Merkle leaf digests
+// (BLAKE3 of each 4 consecutive outputs, 2^logn / 4 digests) when the plan
+// allows it; *leaves_done tells whether it did.
 int ntt_launch_batch(cosnark_ctx *ctx, const gf::fe *in, const gf::fe *const *in_tab, uint64_t in_len,
                      gf::fe *out, gf::fe *const *out_tab, int batch, int logn, bool inverse, hipStream_t s,
-                     gf::fe *tmp);
+                     gf::fe *tmp, uint8_t *leaves = nullptr, bool *leaves_done = nullptr);
+// RS encode (forward NTT of in || 0) of 2^logn points with the codeword's
+// Merkle leaf digests written by the last pass where the plan allows it
+// (*leaves_done); otherwise the caller hashes the leaves itself.
+int ntt_launch_leaves(cosnark_ctx *ctx, const gf::fe *in, uint64_t in_len, gf::fe *out, int logn, uint8_t *leaves,
+                      bool *leaves_done, hipStream_t s);
 int ntt_plan_bits(const cosnark_ctx *ctx, int logn);
 int ntt_plan_tile(const cosnark_ctx *ctx, int logn);
 // Merkle commit of n_elems elements into nodes (all levels).

@@ -15,6 +15,7 @@
 // is consecutive, so each store is a C-element contiguous run.  Zero padding
 // (Reed-Solomon encoding of n coefficients into rate*n points) is handled by
 // the first pass, which reads only the n real inputs.
+#include "blake3.h"
 #include "ctx.h"
 #include "dft.h"
 
@@ -39,6 +40,7 @@ struct NttPass {
     int npass;
     int rbits[8];       // radix bits per pass, first pass first
     int pre[8];         // pre[t] = rbits[0] + .. + rbits[t-1]
+    uint8_t *leaves;    // last pass, batch item 0: Merkle leaf digests of the output (or null)
 };
 
 constexpr int Y_PAD = 1;   // one padding element per LDS row keeps 16-byte lanes off shared banks
@@ -128,9 +130,15 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
         }
         kshift = p.logN - LOGR;
     }
+    // the fused leaf hashing below needs the last item's outputs after the
+    // loop (it is enabled only when each thread has at most one item)
+    const bool leaves_on = LAST && p.leaves && by == 0 && items2 <= NTT_THREADS;
+    fe v[R2];
+    int my_c = 0, my_k1 = 0;
     for (int it = tid; it < items2; it += NTT_THREADS) {
         const int c = it & (C - 1), k1 = it >> logC;
-        fe v[R2];
+        my_c = c;
+        my_k1 = k1;
 #pragma unroll
         for (int d2 = 0; d2 < R2; ++d2) v[d2] = Y[(k1 * R2 + d2) * CP + c];
         dif<R2, INV>(v);
@@ -155,10 +163,44 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
                 constexpr int i = decltype(ii)::value;
                 constexpr int k2 = brev_c<R2>(i);
                 const uint64_t k = (uint64_t)(k1 + R1 * k2);
-                fe x = v[i];
-                if (INV) x = gf::mul(x, scale);
-                out[nat_rest + c0 + c + (k << kshift)] = x;
+                if (INV) v[i] = gf::mul(v[i], scale);
+                out[nat_rest + c0 + c + (k << kshift)] = v[i];
             });
+        }
+    }
+    if (leaves_on) {
+        // Merkle leaves of the output (MT_commit_Blake, src/merkle_tree.cpp:
+        // 211-220: leaf = BLAKE3 of 4 consecutive elements).  A leaf is 4
+        // consecutive c at one k.  Each thread's outputs are still in v:
+        // restage them as Z[k][c] in the exchange buffer (after every
+        // thread's step-2 reads of it), then each thread hashes whole leaves.
+        // The condition is uniform over the workgroup, so are the barriers.
+        __syncthreads();
+        if (tid < items2) {
+            static_for<0, R2>([&](auto ii) {
+                constexpr int i = decltype(ii)::value;
+                constexpr int k2 = brev_c<R2>(i);
+                Y[(my_k1 + R1 * k2) * CP + my_c] = v[i];
+            });
+        }
+        __syncthreads();
+        const int qpr = C / 4, nleaf = (R * C) / 4;
+        for (int l = tid; l < nleaf; l += NTT_THREADS) {
+            const int k = l / qpr, q = l % qpr;
+            uint32_t m[16], h[8];
+            const fe *src = Y + k * CP + 4 * q;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                m[4 * e] = (uint32_t)src[e].re;
+                m[4 * e + 1] = (uint32_t)(src[e].re >> 32);
+                m[4 * e + 2] = (uint32_t)src[e].im;
+                m[4 * e + 3] = (uint32_t)(src[e].im >> 32);
+            }
+            b3::compress64(m, h);
+            const uint64_t pos = nat_rest + c0 + 4 * q + ((uint64_t)k << kshift);
+            uint4 *dst = reinterpret_cast<uint4 *>(p.leaves + 32 * (pos >> 2));
+            dst[0] = make_uint4(h[0], h[1], h[2], h[3]);
+            dst[1] = make_uint4(h[4], h[5], h[6], h[7]);
         }
     }
 }
@@ -239,7 +281,9 @@ int ntt_plan_tile(const cosnark_ctx *ctx, int logn) {
 }
 
 int ntt_launch_batch(cosnark_ctx *ctx, const fe *in, const fe *const *in_tab, uint64_t in_len, fe *out,
-                     fe *const *out_tab, int batch, int logn, bool inverse, hipStream_t stream, fe *tmp_own) {
+                     fe *const *out_tab, int batch, int logn, bool inverse, hipStream_t stream, fe *tmp_own,
+                     uint8_t *leaves, bool *leaves_done) {
+    if (leaves_done) *leaves_done = false;
     if (logn < 1 || logn > NTT_LOG_MAX) return set_err(ctx, COSNARK_E_RANGE, "ntt: logn out of range");
     if (batch < 1 || batch > 65535) return set_err(ctx, COSNARK_E_RANGE, "ntt: batch out of range");
     // radix plan: fewest passes of at most `passbits` bits, spread evenly, larger first
@@ -301,6 +345,13 @@ int ntt_launch_batch(cosnark_ctx *ctx, const fe *in, const fe *const *in_tab, ui
             dst = out;
             q.out_tab = out_tab;
             q.out_stride = N;
+            // fused leaf digests: whole leaves per tile (C >= 4) and at most
+            // one step-2 item per thread (R1 * C <= NTT_THREADS)
+            const int r1 = r >= 7 ? 16 : r >= 5 ? 8 : r >= 3 ? 4 : 2;
+            if (leaves && m >= 2 && logC >= 2 && (r1 << logC) <= NTT_THREADS) {
+                q.leaves = leaves;
+                if (leaves_done) *leaves_done = true;
+            }
         } else {
             dst = tmp;
             q.out_tab = nullptr;
@@ -311,6 +362,12 @@ int ntt_launch_batch(cosnark_ctx *ctx, const fe *in, const fe *const *in_tab, ui
         done_bits += r;
     }
     return COSNARK_OK;
+}
+
+int ntt_launch_leaves(cosnark_ctx *ctx, const fe *in, uint64_t in_len, fe *out, int logn, uint8_t *leaves,
+                      bool *leaves_done, hipStream_t stream) {
+    return ntt_launch_batch(ctx, in, nullptr, in_len, out, nullptr, 1, logn, false, stream, nullptr, leaves,
+                            leaves_done);
 }
 
 int ntt_launch(cosnark_ctx *ctx, const fe *in, uint64_t in_len, fe *out, int logn, bool inverse,
