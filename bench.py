@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--no-whir", action="store_true", help="skip the WHIR (config 3) leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched (config 4) leg")
     ap.add_argument("--no-cosc", action="store_true", help="skip the coSumcheck rounds leg")
+    ap.add_argument("--profile-leg", choices=["whir", "whir_batch", "batched", "cosumcheck"],
+                    help="run only this leg and print its JSON (for kernel traces of one workload)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving (config 5) leg")
     ap.add_argument("--no-tune", action="store_true", help="keep the default NTT radix plan")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="serving leg arrival rate (requests/s)")
@@ -304,6 +306,13 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    if a.profile_leg:
+        ctx = cosnark.Context(0)
+        legs = {"whir": lambda: whir_leg(ctx), "whir_batch": lambda: whir_batch_leg(ctx),
+                "batched": batched_leg, "cosumcheck": lambda: cosumcheck_leg(ctx)}
+        print(json.dumps({"leg": a.profile_leg, "result": legs[a.profile_leg]()}))
+        ctx.close()
+        return
     n = 1 << a.log_n
     f = cosnark.synth(SEED_CFG2, 0, n)
     g = cosnark.synth(SEED_CFG2, 1, n)
