@@ -55,6 +55,7 @@ struct cosnark_ctx {
     struct { int passbits, tile; } ntt_plan[NTT_LOG_MAX + 1] = {};
     bool overlap = true;              // plain prover: sumcheck beside the commit
     int sc_blocks = 256;              // workgroups of a multi-workgroup sumcheck round (<= SC_MAX_BLOCKS)
+    uint64_t sc_tail_elems = 4096;    // vectors this short finish in the one-workgroup tail (power of two, >= 4)
 };
 
 constexpr int CO_HOST_ELEMS = 8192;     // pinned staging for block-sumcheck results and weights

@@ -16,8 +16,11 @@
 // redundantly, on four lanes, so no atomics, no last-workgroup election and
 // no host round trip sit between rounds.  Transcript values are kept per
 // round (tr_prev[k], tr_y[k] = state before round k) so a kernel never reads
-// a slot it writes.  Once a vector fits in LDS one workgroup runs all
-// remaining rounds.
+// a slot it writes.  Once a vector is small (sc_tail_elems) one workgroup
+// runs all remaining rounds: its wave 0 runs the transcript while waves 1..3
+// fold by the challenge just drawn and form the next round's sums as
+// quadratics in the challenge still being hashed, so a round costs the hash
+// chain and little else.
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -29,8 +32,9 @@ using gf::fe;
 namespace {
 
 constexpr int SC_THREADS = 256;
-constexpr int SC_TAIL_THREADS = 1024;
-constexpr int SC_TAIL_ELEMS = 4096;     // LDS-resident vectors (2 x 64 KiB)
+constexpr int SC_TAIL_THREADS = 256;    // wave 0: transcript; waves 1..3: folds and sums
+constexpr int SC_TAIL_CT = SC_TAIL_THREADS - 64;
+constexpr int SC_TAIL_WAVES = SC_TAIL_CT / 64;
 constexpr int SC_ITEMS = 1;             // pairs per thread per sweep in the multi-workgroup rounds
 
 __device__ __forceinline__ fe fold1(const fe &e, const fe &o, const fe &r) {
@@ -116,9 +120,12 @@ __device__ void finish_round(ScState *st, int round, int fs_rounds, const fe *pa
     }
     block_reduce3<NT>(A, C, O, red);
     if (threadIdx.x < 4) {
+        // the transcript chain is serial: let it issue ahead of co-resident waves
+        __builtin_amdgcn_s_setprio(3);
         fe prev = st->tr_prev[round], y = st->tr_y[round];
         const fe r = finalize_round(st, round, fs_rounds, st->derive_y, prev, y, A, C, O, writer);
         if (threadIdx.x == 0) { *sh_r = r; *sh_prev = prev; *sh_y = y; }
+        __builtin_amdgcn_s_setprio(0);
     }
     __syncthreads();
 }
@@ -127,7 +134,8 @@ __device__ void finish_round(ScState *st, int round, int fs_rounds, const fe *pa
 // its workgroup b works on job blk_job[b], block b - first_block of it.
 struct ScJob {
     const fe *s1, *s2;        // inputs
-    fe *d1, *d2;              // folded outputs (FOLD rounds)
+    fe *d1, *d2;              // folded outputs (FOLD rounds; the tail's first)
+    fe *e1, *e2;              // the tail's second output buffers (it alternates d, e)
     uint64_t npairs;          // pairs this round (tail: elements in s1/s2)
     ScState *st;
     const fe *part_in;        // previous round's workgroup partials
@@ -220,82 +228,190 @@ __global__ __launch_bounds__(SC_THREADS) void sc_round_batch_kernel(const ScJob 
     sc_round<FOLD>(J, blockIdx.x - J.first_block);
 }
 
-// All rounds from `round0` to m-1 in one workgroup, vectors held in LDS.
-// If round0 > 0 the input (len elements) still needs the fold by
-// r[round0-1], which is finalized here from part_in.  Writes the final
-// folded values and the closing transcript state.
+// A round's sums as quadratics in the challenge r not yet drawn: for each of
+// A = sum d1 d2, C = sum e1 e2, O = sum o1 o2 the value at r = 0, the value at
+// r = 1 and the r^2 coefficient (q[0..2] A, q[3..5] C, q[6..8] O).  A
+// next-round pair whose folded values are e = x0 + r (x1 - x0) and
+// o = x2 + r (x3 - x2) (x = the four unfolded elements, y likewise) adds
+// x0 y0, x1 y1 and (x1 - x0)(y1 - y0) to C; the same for O with x2, x3; and
+// for A with d(0) = x2 - x0, d(1) = x3 - x1, slope (x3 - x2) - (x1 - x0).
+__device__ __forceinline__ void quad_acc(fe *q, const fe *x, const fe *y) {
+    const fe qe1 = gf::sub(x[1], x[0]), qo1 = gf::sub(x[3], x[2]);
+    const fe qe2 = gf::sub(y[1], y[0]), qo2 = gf::sub(y[3], y[2]);
+    q[0] = gf::add(q[0], gf::mul(gf::sub(x[2], x[0]), gf::sub(y[2], y[0])));
+    q[1] = gf::add(q[1], gf::mul(gf::sub(x[3], x[1]), gf::sub(y[3], y[1])));
+    q[2] = gf::add(q[2], gf::mul(gf::sub(qo1, qe1), gf::sub(qo2, qe2)));
+    q[3] = gf::add(q[3], gf::mul(x[0], y[0]));
+    q[4] = gf::add(q[4], gf::mul(x[1], y[1]));
+    q[5] = gf::add(q[5], gf::mul(qe1, qe2));
+    q[6] = gf::add(q[6], gf::mul(x[2], y[2]));
+    q[7] = gf::add(q[7], gf::mul(x[3], y[3]));
+    q[8] = gf::add(q[8], gf::mul(qo1, qo2));
+}
+
+// f(r) = f0 + r (f1 - f0 - f2) + r^2 f2
+__device__ __forceinline__ fe quad_eval(const fe &f0, const fe &f1, const fe &f2, const fe &r) {
+    return gf::add(f0, gf::mul(r, gf::add(gf::sub(gf::sub(f1, f0), f2), gf::mul(r, f2))));
+}
+
+__device__ __forceinline__ void wave_sum9(fe *q) {
+#pragma unroll
+    for (int msk = 32; msk >= 1; msk >>= 1)
+#pragma unroll
+        for (int t = 0; t < 9; ++t) q[t] = gf::add(q[t], shfl_xor_fe(q[t], msk));
+}
+
+// All rounds from `round0` to m-1 in one workgroup of four waves.  The input
+// (len elements) is v(round0) when round0 == 0, else v(round0 - 1), still to
+// be folded by r[round0 - 1], which is finalized here from part_in.  Phase k
+// finalizes round i = round0 + k on wave 0 (lanes 0..3: sums, claim check,
+// transcript hashes) while waves 1..3 fold the current vector by r[i - 1]
+// into the next buffer (d, e, d, ...: never the input) and accumulate round
+// i + 1's sums as quadratics in r[i]; one barrier ends each phase.  Vectors
+// move through global memory (L2), so the workgroup needs no LDS beyond the
+// exchange slots and fits on a CU beside other kernels' workgroups.
 __device__ __forceinline__ void sc_tail(const ScJob &J) {
-    const fe *__restrict__ s1 = J.s1;
-    const fe *__restrict__ s2 = J.s2;
-    const uint32_t len = (uint32_t)J.npairs;
+    __shared__ fe qpart[2][SC_TAIL_WAVES][9];
+    __shared__ fe sh_r[2];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const bool hash_wave = wave == 0;
+    const int ct = tid - 64;
     const int round0 = J.round, m = J.m, fs_rounds = J.fs_rounds, nb_in = J.nb_in;
     ScState *st = J.st;
-    const fe *part_in = J.part_in;
-    extern __shared__ fe lds[];
-    fe *x1 = lds, *x2 = lds + SC_TAIL_ELEMS;
-    __shared__ fe red[3][SC_TAIL_THREADS / 64];
-    __shared__ fe sh_r, sh_prev, sh_y;
-    const int tid = threadIdx.x;
-    for (uint32_t i = tid; i < len; i += SC_TAIL_THREADS) { x1[i] = s1[i]; x2[i] = s2[i]; }
-    const int claim_mode = st->derive_y;
-    if (round0 > 0) {
-        finish_round<SC_TAIL_THREADS>(st, round0 - 1, fs_rounds, part_in, nb_in, red, &sh_r, &sh_prev, &sh_y, true);
+    const bool pending = round0 > 0;
+    const fe *c1 = J.s1, *c2 = J.s2;
+    fe *const o1[2] = {J.d1, J.e1};
+    fe *const o2[2] = {J.d2, J.e2};
+    int ob = 0;
+    uint32_t cur = (uint32_t)J.npairs;      // elements in c1 / c2
+    fe prev = gf::zero(), y = gf::zero(), rr = gf::zero();
+    int claim_mode = 0;
+
+    // ---- before phase 0 ----
+    if (hash_wave) {
+        claim_mode = st->derive_y;
+        const int r0 = pending ? round0 - 1 : 0;
+        prev = st->tr_prev[r0];
+        y = st->tr_y[r0];
+        if (pending) {
+            fe A = gf::zero(), C = gf::zero(), O = gf::zero();
+            for (int b = lane; b < nb_in; b += 64) {
+                A = gf::add(A, J.part_in[3 * b]);
+                C = gf::add(C, J.part_in[3 * b + 1]);
+                O = gf::add(O, J.part_in[3 * b + 2]);
+            }
+            wave_reduce3(A, C, O);
+            if (lane < 4) {
+                __builtin_amdgcn_s_setprio(3);
+                rr = finalize_round(st, round0 - 1, fs_rounds, claim_mode, prev, y, A, C, O, true);
+                __builtin_amdgcn_s_setprio(0);
+            }
+        }
+        if (lane == 0) sh_r[0] = rr;
     } else {
-        if (tid == 0) { sh_prev = st->tr_prev[0]; sh_y = st->tr_y[0]; sh_r = gf::zero(); }
+        fe q[9];
+#pragma unroll
+        for (int t = 0; t < 9; ++t) q[t] = gf::zero();
+        if (pending) {
+            // round round0's sums in r[round0 - 1], from groups of 4 of the input
+            for (uint32_t g = ct; g < cur / 4; g += SC_TAIL_CT) {
+                fe x[4], z[4];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) { x[e] = c1[4 * g + e]; z[e] = c2[4 * g + e]; }
+                quad_acc(q, x, z);
+            }
+        } else {
+            // round 0's sums directly (constant terms)
+            for (uint32_t j = ct; j < cur / 2; j += SC_TAIL_CT) {
+                const fe e1 = c1[2 * j], od1 = c1[2 * j + 1], e2 = c2[2 * j], od2 = c2[2 * j + 1];
+                q[0] = gf::add(q[0], gf::mul(gf::sub(od1, e1), gf::sub(od2, e2)));
+                q[3] = gf::add(q[3], gf::mul(e1, e2));
+                q[6] = gf::add(q[6], gf::mul(od1, od2));
+            }
+        }
+        wave_sum9(q);
+        if (lane == 0)
+#pragma unroll
+            for (int t = 0; t < 9; ++t) qpart[0][wave - 1][t] = q[t];
+    }
+    __syncthreads();
+
+    // ---- phases ----
+    for (int k = 0; round0 + k < m; ++k) {
+        const int i = round0 + k, s = k & 1;
+        const bool fold = pending || k > 0;    // the current vector is v(i - 1)
+        if (hash_wave) {
+            if (lane < 4) {
+                __builtin_amdgcn_s_setprio(3);
+                fe f[9];
+#pragma unroll
+                for (int t = 0; t < 9; ++t) {
+                    f[t] = qpart[s][0][t];
+#pragma unroll
+                    for (int w = 1; w < SC_TAIL_WAVES; ++w) f[t] = gf::add(f[t], qpart[s][w][t]);
+                }
+                fe A = f[0], C = f[3], O = f[6];
+                if (fold) {
+                    A = quad_eval(f[0], f[1], f[2], rr);
+                    C = quad_eval(f[3], f[4], f[5], rr);
+                    O = quad_eval(f[6], f[7], f[8], rr);
+                }
+                rr = finalize_round(st, i, fs_rounds, claim_mode, prev, y, A, C, O, true);
+                if (lane == 0) sh_r[s ^ 1] = rr;
+                __builtin_amdgcn_s_setprio(0);
+            }
+        } else {
+            const bool next = i + 1 < m;       // round i + 1 exists: accumulate its sums
+            fe q[9];
+#pragma unroll
+            for (int t = 0; t < 9; ++t) q[t] = gf::zero();
+            if (fold) {
+                // v(i) = fold(v(i - 1), r[i - 1]); groups of 8 -> 4 (the last phase: 4 -> 2)
+                const fe r = sh_r[s];
+                fe *w1 = o1[ob], *w2 = o2[ob];
+                const uint32_t gs = cur >= 8 ? 8 : cur;
+                for (uint32_t g = ct; g < cur / gs; g += SC_TAIL_CT) {
+                    fe x[4], z[4];
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        if (2 * e < (int)gs) {
+                            x[e] = fold1(c1[gs * g + 2 * e], c1[gs * g + 2 * e + 1], r);
+                            z[e] = fold1(c2[gs * g + 2 * e], c2[gs * g + 2 * e + 1], r);
+                            w1[(gs / 2) * g + e] = x[e];
+                            w2[(gs / 2) * g + e] = z[e];
+                        }
+                    }
+                    if (next) quad_acc(q, x, z);
+                }
+            } else if (next) {
+                // round 0 was direct: round 1's sums from groups of 4 of v(0)
+                for (uint32_t g = ct; g < cur / 4; g += SC_TAIL_CT) {
+                    fe x[4], z[4];
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) { x[e] = c1[4 * g + e]; z[e] = c2[4 * g + e]; }
+                    quad_acc(q, x, z);
+                }
+            }
+            if (next) {
+                wave_sum9(q);
+                if (lane == 0)
+#pragma unroll
+                    for (int t = 0; t < 9; ++t) qpart[s ^ 1][wave - 1][t] = q[t];
+            }
+        }
+        if (fold) {
+            c1 = o1[ob];
+            c2 = o2[ob];
+            ob ^= 1;
+            cur /= 2;
+        }
         __syncthreads();
     }
-    uint32_t cur = len;
-    bool pending = round0 > 0;
-    for (int round = round0; round <= m; ++round) {
-        const fe r = sh_r;
-        const uint32_t after = pending ? cur / 2 : cur;
-        if (round == m) {
-            if (tid == 0) {
-                st->final_v[0] = pending ? fold1(x1[0], x1[1], r) : x1[0];
-                st->final_v[1] = pending ? fold1(x2[0], x2[1], r) : x2[0];
-            }
-            break;
-        }
-        const uint32_t npairs = after / 2;
-        fe A = gf::zero(), C = gf::zero(), O = gf::zero();
-        fe w0[4] = {}, w1[4] = {};
-        int nmine = 0;
-        for (uint32_t j = tid; j < npairs; j += SC_TAIL_THREADS) {
-            fe e1, o1, e2, o2;
-            if (pending) {
-                e1 = fold1(x1[4 * j], x1[4 * j + 1], r);
-                o1 = fold1(x1[4 * j + 2], x1[4 * j + 3], r);
-                e2 = fold1(x2[4 * j], x2[4 * j + 1], r);
-                o2 = fold1(x2[4 * j + 2], x2[4 * j + 3], r);
-            } else {
-                e1 = x1[2 * j]; o1 = x1[2 * j + 1];
-                e2 = x2[2 * j]; o2 = x2[2 * j + 1];
-            }
-            A = gf::add(A, gf::mul(gf::sub(o1, e1), gf::sub(o2, e2)));
-            C = gf::add(C, gf::mul(e1, e2));
-            O = gf::add(O, gf::mul(o1, o2));
-            if (nmine == 0) { w0[0] = e1; w0[1] = o1; w0[2] = e2; w0[3] = o2; }
-            else { w1[0] = e1; w1[1] = o1; w1[2] = e2; w1[3] = o2; }
-            ++nmine;
-        }
-        block_reduce3<SC_TAIL_THREADS>(A, C, O, red);   // its barriers order all reads of x before the writes below
-        if (pending) {
-            int q = 0;
-            for (uint32_t j = tid; j < npairs; j += SC_TAIL_THREADS, ++q) {
-                x1[2 * j] = q == 0 ? w0[0] : w1[0];
-                x1[2 * j + 1] = q == 0 ? w0[1] : w1[1];
-                x2[2 * j] = q == 0 ? w0[2] : w1[2];
-                x2[2 * j + 1] = q == 0 ? w0[3] : w1[3];
-            }
-        }
-        if (tid < 4) {
-            fe prev = sh_prev, y = sh_y;
-            const fe rr = finalize_round(st, round, fs_rounds, claim_mode, prev, y, A, C, O, true);
-            if (tid == 0) { sh_prev = prev; sh_y = y; sh_r = rr; }
-        }
-        __syncthreads();
-        cur = after;
-        pending = true;
+    // v(m - 1) (two elements) is in c1 / c2; r[m - 1] in sh_r
+    if (tid == 0) {
+        const fe r = sh_r[(m - round0) & 1];
+        st->final_v[0] = fold1(c1[0], c1[1], r);
+        st->final_v[1] = fold1(c2[0], c2[1], r);
     }
 }
 
@@ -344,11 +460,7 @@ __global__ void sc_init_batch_kernel(const ScJob *tail, const fe *prev, int coun
 }  // namespace
 
 int sumcheck_init(cosnark_ctx *ctx) {
-    // the tail kernel keeps two 4096-element vectors (128 KiB) in LDS
-    HIPCK(hipFuncSetAttribute((const void *)sc_tail_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)(2 * SC_TAIL_ELEMS * sizeof(fe))));
-    HIPCK(hipFuncSetAttribute((const void *)sc_tail_batch_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)(2 * SC_TAIL_ELEMS * sizeof(fe))));
+    (void)ctx;
     return COSNARK_OK;
 }
 
@@ -398,11 +510,13 @@ int sumcheck_fold_launch(cosnark_ctx *ctx, const fe *s1, const fe *s2, fe *d1, f
 namespace {
 
 // The launches of one proof's sumcheck as sumcheck_launch issues them: the
-// multi-workgroup rounds while a vector exceeds the LDS tail, then the tail.
+// multi-workgroup rounds while a vector exceeds tail_elems, then the tail.
 // buf: n + n/2 elements (ping n/2 per vector, pong n/4 per vector); part:
-// two slots of 3 * SC_MAX_BLOCKS partials.
+// two slots of 3 * SC_MAX_BLOCKS partials.  The tail folds into the ping-pong
+// slot not holding its input first (a folded vector of m elements lands in a
+// slot of at least m), then alternates; it never writes v1 / v2.
 std::vector<ScJob> sumcheck_jobs(const fe *v1, const fe *v2, uint64_t n, int fs_rounds, fe *buf, fe *part0,
-                                 ScState *st, int max_blocks) {
+                                 ScState *st, int max_blocks, uint64_t tail_elems) {
     const int m = ilog2(n);
     fe *a1 = buf, *a2 = buf + n / 2, *b1 = buf + n, *b2 = buf + n + n / 4;
     fe *part[2] = {part0, part0 + 3 * SC_MAX_BLOCKS};
@@ -412,7 +526,7 @@ std::vector<ScJob> sumcheck_jobs(const fe *v1, const fe *v2, uint64_t n, int fs_
     uint64_t len = n;        // elements in s1/s2
     int round = 0, which = 0, nb_prev = 0;
     bool pending = false;    // s1/s2 still need the fold with r[round-1]
-    while (len > (uint64_t)SC_TAIL_ELEMS) {
+    while (len > tail_elems) {
         const uint64_t after = pending ? len / 2 : len;
         const uint64_t npairs = after / 2;
         uint64_t blocks = (npairs + SC_THREADS * SC_ITEMS - 1) / (SC_THREADS * SC_ITEMS);
@@ -441,6 +555,8 @@ std::vector<ScJob> sumcheck_jobs(const fe *v1, const fe *v2, uint64_t n, int fs_
     }
     ScJob T{};
     T.s1 = s1; T.s2 = s2;
+    T.d1 = outs[which][0]; T.d2 = outs[which][1];
+    T.e1 = outs[which ^ 1][0]; T.e2 = outs[which ^ 1][1];
     T.npairs = len;
     T.st = st;
     T.part_in = part[(round + 1) & 1];
@@ -456,7 +572,8 @@ std::vector<ScJob> sumcheck_jobs(const fe *v1, const fe *v2, uint64_t n, int fs_
 
 int sumcheck_launch(cosnark_ctx *ctx, const fe *v1, const fe *v2, uint64_t n, int fs_rounds, fe *buf,
                     hipStream_t stream) {
-    const auto jobs = sumcheck_jobs(v1, v2, n, fs_rounds, buf, ctx->sc_partials, ctx->sc, ctx->sc_blocks);
+    const auto jobs =
+        sumcheck_jobs(v1, v2, n, fs_rounds, buf, ctx->sc_partials, ctx->sc, ctx->sc_blocks, ctx->sc_tail_elems);
     for (size_t i = 0; i + 1 < jobs.size(); ++i) {
         const ScJob &J = jobs[i];
         if (J.round > 0)
@@ -465,8 +582,7 @@ int sumcheck_launch(cosnark_ctx *ctx, const fe *v1, const fe *v2, uint64_t n, in
             hipLaunchKernelGGL(sc_round_kernel<false>, dim3((unsigned)J.nb), dim3(SC_THREADS), 0, stream, J);
         LAUNCHCK();
     }
-    const size_t lds = 2 * SC_TAIL_ELEMS * sizeof(fe);
-    hipLaunchKernelGGL(sc_tail_kernel, dim3(1), dim3(SC_TAIL_THREADS), lds, stream, jobs.back());
+    hipLaunchKernelGGL(sc_tail_kernel, dim3(1), dim3(SC_TAIL_THREADS), 0, stream, jobs.back());
     LAUNCHCK();
     return COSNARK_OK;
 }
@@ -480,7 +596,7 @@ int sumcheck_batch_plan(cosnark_ctx *ctx, int count, const fe *const *v1, const 
     size_t nr = 0;
     for (int i = 0; i < count; ++i) {
         const int m = ilog2(n[i]);
-        per[i] = sumcheck_jobs(v1[i], v2[i], n[i], m, buf[i], parts[i], st[i], ctx->sc_blocks);
+        per[i] = sumcheck_jobs(v1[i], v2[i], n[i], m, buf[i], parts[i], st[i], ctx->sc_blocks, ctx->sc_tail_elems);
         nr = std::max(nr, per[i].size() - 1);
     }
     plan.rounds.clear();
@@ -522,8 +638,7 @@ int sumcheck_batch_run(cosnark_ctx *ctx, const ScBatchPlan &plan, const uint8_t 
                                (const ScJob *)(tables + L.jobs), (const uint16_t *)(tables + L.blk_job));
         LAUNCHCK();
     }
-    const size_t lds = 2 * SC_TAIL_ELEMS * sizeof(fe);
-    hipLaunchKernelGGL(sc_tail_batch_kernel, dim3(plan.count), dim3(SC_TAIL_THREADS), lds, stream, tail);
+    hipLaunchKernelGGL(sc_tail_batch_kernel, dim3(plan.count), dim3(SC_TAIL_THREADS), 0, stream, tail);
     LAUNCHCK();
     return COSNARK_OK;
 }

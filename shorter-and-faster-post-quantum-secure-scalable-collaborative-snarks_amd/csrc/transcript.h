@@ -60,8 +60,12 @@ __device__ __forceinline__ uint32_t quad_perm(uint32_t x) {
     return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, CTRL, 0xF, 0xF, true);
 }
 
-__device__ __forceinline__ uint32_t sel4(int q, uint32_t w0, uint32_t w1, uint32_t w2, uint32_t w3) {
-    return q == 0 ? w0 : (q == 1 ? w1 : (q == 2 ? w2 : w3));
+// word w_q for lane q, branch-free: k0 / k1 are all-ones where bit 0 / bit 1
+// of q is set (nested ?: on q compiles to divergent branches)
+__device__ __forceinline__ uint32_t sel4(uint32_t k0, uint32_t k1, uint32_t w0, uint32_t w1, uint32_t w2,
+                                         uint32_t w3) {
+    const uint32_t a = (w0 & ~k0) | (w1 & k0), b = (w2 & ~k0) | (w3 & k0);
+    return (a & ~k1) | (b & k1);
 }
 
 __device__ __forceinline__ void g1(uint32_t &a, uint32_t &b, uint32_t &c, uint32_t &d, uint32_t x, uint32_t y) {
@@ -72,23 +76,24 @@ __device__ __forceinline__ void g1(uint32_t &a, uint32_t &b, uint32_t &c, uint32
 }
 
 __device__ __forceinline__ gf::fe hash_F_quad(const gf::fe &x, const gf::fe &y) {
-    const int q = threadIdx.x & 3;
+    const uint32_t q = threadIdx.x & 3;
+    const uint32_t k0 = 0u - (q & 1), k1 = 0u - (q >> 1);
     const uint32_t m[16] = {(uint32_t)x.re, (uint32_t)(x.re >> 32), (uint32_t)x.im, (uint32_t)(x.im >> 32),
                             0u, 0u, 0u, 0u,
                             (uint32_t)y.re, (uint32_t)(y.re >> 32), (uint32_t)y.im, (uint32_t)(y.im >> 32),
                             0u, 0u, 0u, 0u};
-    uint32_t a = sel4(q, b3::IV0, b3::IV1, b3::IV2, b3::IV3);
-    uint32_t b = sel4(q, b3::IV4, b3::IV5, b3::IV6, b3::IV7);
+    uint32_t a = sel4(k0, k1, b3::IV0, b3::IV1, b3::IV2, b3::IV3);
+    uint32_t b = sel4(k0, k1, b3::IV4, b3::IV5, b3::IV6, b3::IV7);
     uint32_t c = a;
-    uint32_t d = sel4(q, 0u, 0u, 64u, b3::FLAGS_ROOT_BLOCK);
+    uint32_t d = sel4(k0, k1, 0u, 0u, 64u, b3::FLAGS_ROOT_BLOCK);
 #pragma unroll
     for (int r = 0; r < 7; ++r) {
         const int *S = B3_SCHED.s[r];
-        g1(a, b, c, d, sel4(q, m[S[0]], m[S[2]], m[S[4]], m[S[6]]), sel4(q, m[S[1]], m[S[3]], m[S[5]], m[S[7]]));
+        g1(a, b, c, d, sel4(k0, k1, m[S[0]], m[S[2]], m[S[4]], m[S[6]]), sel4(k0, k1, m[S[1]], m[S[3]], m[S[5]], m[S[7]]));
         b = quad_perm<qp(1, 2, 3, 0)>(b);
         c = quad_perm<qp(2, 3, 0, 1)>(c);
         d = quad_perm<qp(3, 0, 1, 2)>(d);
-        g1(a, b, c, d, sel4(q, m[S[8]], m[S[10]], m[S[12]], m[S[14]]), sel4(q, m[S[9]], m[S[11]], m[S[13]], m[S[15]]));
+        g1(a, b, c, d, sel4(k0, k1, m[S[8]], m[S[10]], m[S[12]], m[S[14]]), sel4(k0, k1, m[S[9]], m[S[11]], m[S[13]], m[S[15]]));
         b = quad_perm<qp(3, 0, 1, 2)>(b);
         c = quad_perm<qp(2, 3, 0, 1)>(c);
         d = quad_perm<qp(1, 2, 3, 0)>(d);
