@@ -173,4 +173,52 @@ GF_HD fe mul_root(const fe &x) {
     }
 }
 
+// ---- weak forms for the transform kernels ----
+// A weak component lies in [0, W], W = 2^61 + 2^32: congruent to its value
+// mod p, not reduced.  mul(w, x) with canonical w and weak x is exact and
+// canonical (x's high limb x >> 31 <= 2^30 + 2 keeps dot2_61's partial sums
+// below 2^64; only the first operand goes through the P ^ b negation).  The
+// butterflies add and subtract without reducing and fold once per radix-4
+// layer; canon_w gives the canonical form for storage.
+constexpr uint64_t P2 = 2 * P, P4 = 4 * P;
+
+// x < 2^64  ->  (x & p) + (x >> 61) <= p + 7
+GF_HD uint64_t fold(uint64_t x) { return (x & P) + (x >> 61); }
+// weak -> canonical: one fold leaves [0, p]
+GF_HD uint64_t canon_w(uint64_t x) {
+    x = fold(x);
+    return x == P ? 0 : x;
+}
+GF_HD fe canon_w(const fe &a) { return mk(canon_w(a.re), canon_w(a.im)); }
+// y < 2^63  ->  y * 2^30 as a weak value: y = h 2^31 + l, 2^61 == 1
+GF_HD uint64_t mul2_30(uint64_t y) { return ((y & 0x7FFFFFFFull) << 30) + (y >> 31); }
+
+// x * w_M^E on a weak x, weak result; E reduced mod M at compile time
+template <int M, int E>
+GF_HD fe mul_root_w(const fe &x) {
+    constexpr int e = ((E % M) + M) % M;
+    if constexpr (e == 0) {
+        return x;
+    } else if constexpr (2 * e == M) {
+        return mk(fold(P2 - x.re), fold(P2 - x.im));
+    } else if constexpr (4 * e == M) {
+        return mk(x.im, fold(P2 - x.re));             // x * (-I)
+    } else if constexpr (4 * e == 3 * M) {
+        return mk(fold(P2 - x.im), x.re);             // x * I
+    } else if constexpr ((8 * e) % M == 0) {
+        // w_8^K = 2^30 (a + b I), (a, b) = (1,-1), (-1,-1), (-1,1), (1,1) for K = 1, 3, 5, 7;
+        // negations enter as swapped differences or 4p - sum (all < 2^63)
+        constexpr int K = (8 * e) / M;
+        if constexpr (K == 1) return mk(mul2_30(x.re + x.im), mul2_30(x.im + (P2 - x.re)));
+        if constexpr (K == 3) return mk(mul2_30(x.im + (P2 - x.re)), mul2_30(P4 - (x.re + x.im)));
+        if constexpr (K == 5) return mk(mul2_30(P4 - (x.re + x.im)), mul2_30(x.re + (P2 - x.im)));
+        return mk(mul2_30(x.re + (P2 - x.im)), mul2_30(x.re + x.im));
+    } else {
+        constexpr int lg = (M <= 1) ? 0 : (M == 2 ? 1 : M == 4 ? 2 : M == 8 ? 3 : M == 16 ? 4 : M == 32 ? 5 :
+                                         M == 64 ? 6 : M == 128 ? 7 : M == 256 ? 8 : M == 512 ? 9 : 10);
+        constexpr fe w = pow(root_of_unity(lg), (uint64_t)e);
+        return mul(w, x);
+    }
+}
+
 }  // namespace gf

@@ -12,7 +12,9 @@
 // decimation-in-time stages there, multiplies by the inter-pass twiddle
 // w_{N_t}^(lo*k) and writes back in place.  The last pass (stride 1) writes
 // in natural order; its workgroup takes C rows whose leading output digit
-// is consecutive, so each store is a C-element contiguous run.  Zero padding
+// is consecutive, so each store is a C-element contiguous run.  Inside a
+// pass values are weak (gf.h: unreduced sums, one fold per radix-4 layer);
+// the twiddle products and the last pass's stores are canonical.  Zero padding
 // (Reed-Solomon encoding of n coefficients into rate*n points) is handled by
 // the first pass, which reads only the n real inputs.
 #include "blake3.h"
@@ -24,7 +26,7 @@ using gf::fe;
 namespace {
 
 using dft::brev_c;
-using dft::dif;
+using dft::dif_w;
 using dft::static_for;
 using dft::tw28;
 
@@ -105,12 +107,12 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
             const uint64_t pos = base + (LAST ? d : (d << logS));
             v[d1] = pos < p.in_len ? in[pos] : gf::zero();
         }
-        dif<R1, INV>(v);
+        dif_w<R1, INV>(v);
         static_for<0, R1>([&](auto ii) {
             constexpr int i = decltype(ii)::value;
             constexpr int k1 = brev_c<R1>(i);
             fe x = v[i];
-            if (k1 != 0 && d2 != 0) x = gf::mul(x, wr[d2 * k1]);
+            if (k1 != 0 && d2 != 0) x = gf::mul(wr[d2 * k1], x);
             Y[(k1 * R2 + d2) * CP + c] = x;
         });
     }
@@ -141,7 +143,7 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
         my_k1 = k1;
 #pragma unroll
         for (int d2 = 0; d2 < R2; ++d2) v[d2] = Y[(k1 * R2 + d2) * CP + c];
-        dif<R2, INV>(v);
+        dif_w<R2, INV>(v);
         if (!LAST) {
             // inter-pass twiddle w_{N_t}^(lo k), k = k1 + R1 k2: one table product for
             // w^(lo k1) and one for w^(lo R1), then a running product over k2
@@ -155,7 +157,7 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
                 constexpr int k2 = decltype(kk)::value;
                 constexpr int i = brev_c<R2>(k2);
                 const uint64_t k = (uint64_t)(k1 + R1 * k2);
-                out[(hi << logNt) + (k << logS) + lo] = gf::mul(v[i], t);
+                out[(hi << logNt) + (k << logS) + lo] = gf::mul(t, v[i]);
                 if (k2 + 1 < R2) t = gf::mul(t, step);
             });
         } else {
@@ -163,7 +165,7 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
                 constexpr int i = decltype(ii)::value;
                 constexpr int k2 = brev_c<R2>(i);
                 const uint64_t k = (uint64_t)(k1 + R1 * k2);
-                if (INV) v[i] = gf::mul(v[i], scale);
+                v[i] = INV ? gf::mul(scale, v[i]) : gf::canon_w(v[i]);
                 out[nat_rest + c0 + c + (k << kshift)] = v[i];
             });
         }
