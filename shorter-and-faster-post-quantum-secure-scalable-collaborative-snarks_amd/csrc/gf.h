@@ -64,8 +64,9 @@ GF_HD fe neg(const fe &a) { return mk(sub61(0, a.re), sub61(0, a.im)); }
 // coordinate: 16 v_mad_u64_u32 and two reductions (60 VALU instructions on
 // gfx950, against 92 for a three-multiply Karatsuba with three reductions).
 //   x = x1*2^31 + x0 (x0 < 2^31, x1 < 2^30); 2^62 == 2, 2^61 == 1 (mod p)
-GF_HD uint64_t dot2_61(uint64_t a, uint64_t b, uint64_t c, uint64_t d) {
-    // a*b + c*d mod p for canonical a, b, c, d
+// a*b + c*d for canonical a, c and b, d < 2^61 + 2^33 (weak, see below),
+// folded once: the result is congruent mod p and <= p + 7
+GF_HD uint64_t dot2_61_fold(uint64_t a, uint64_t b, uint64_t c, uint64_t d) {
     const uint64_t M31 = 0x7FFFFFFFull;
     const uint32_t a0 = (uint32_t)(a & M31), a1 = (uint32_t)(a >> 31);
     const uint32_t b0 = (uint32_t)(b & M31), b1 = (uint32_t)(b >> 31);
@@ -76,12 +77,20 @@ GF_HD uint64_t dot2_61(uint64_t a, uint64_t b, uint64_t c, uint64_t d) {
     const uint64_t p11 = (uint64_t)a1 * b1 + (uint64_t)c1 * d1;                        // < 2^61
     // value = p11*2^62 + p01*2^31 + p00; p01*2^31 = (p01>>30)*2^61 + (p01 & M30)*2^31
     const uint64_t s = (p11 << 1) + p00 + (p01 >> 30) + ((p01 & 0x3FFFFFFFull) << 31);  // < 2^64
-    const uint64_t r = (s & P) + (s >> 61);                                           // <= p + 7
-    return canon(r);
+    return (s & P) + (s >> 61);                                                       // <= p + 7
 }
+
+// a*b + c*d mod p, canonical
+GF_HD uint64_t dot2_61(uint64_t a, uint64_t b, uint64_t c, uint64_t d) { return canon(dot2_61_fold(a, b, c, d)); }
 
 GF_HD fe mul(const fe &x, const fe &y) {
     return mk(dot2_61(x.re, y.re, x.im ^ P, y.im), dot2_61(x.re, y.im, x.im, y.re));
+}
+
+// The same product left folded once (components <= p + 7) for consumers that
+// take weak values; x's components must be <= p (x.im goes through P ^ x.im).
+GF_HD fe mul_w(const fe &x, const fe &y) {
+    return mk(dot2_61_fold(x.re, y.re, x.im ^ P, y.im), dot2_61_fold(x.re, y.im, x.im, y.re));
 }
 
 // multiply by I: (a + bI) I = -b + aI
@@ -217,7 +226,7 @@ GF_HD fe mul_root_w(const fe &x) {
         constexpr int lg = (M <= 1) ? 0 : (M == 2 ? 1 : M == 4 ? 2 : M == 8 ? 3 : M == 16 ? 4 : M == 32 ? 5 :
                                          M == 64 ? 6 : M == 128 ? 7 : M == 256 ? 8 : M == 512 ? 9 : 10);
         constexpr fe w = pow(root_of_unity(lg), (uint64_t)e);
-        return mul(w, x);
+        return mul_w(w, x);
     }
 }
 

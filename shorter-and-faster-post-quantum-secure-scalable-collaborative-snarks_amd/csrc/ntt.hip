@@ -112,7 +112,7 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
             constexpr int i = decltype(ii)::value;
             constexpr int k1 = brev_c<R1>(i);
             fe x = v[i];
-            if (k1 != 0 && d2 != 0) x = gf::mul(wr[d2 * k1], x);
+            if (k1 != 0 && d2 != 0) x = gf::mul_w(wr[d2 * k1], x);
             Y[(k1 * R2 + d2) * CP + c] = x;
         });
     }
@@ -157,7 +157,7 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
                 constexpr int k2 = decltype(kk)::value;
                 constexpr int i = brev_c<R2>(k2);
                 const uint64_t k = (uint64_t)(k1 + R1 * k2);
-                out[(hi << logNt) + (k << logS) + lo] = gf::mul(t, v[i]);
+                out[(hi << logNt) + (k << logS) + lo] = gf::mul_w(t, v[i]);   // weak: read by the next pass
                 if (k2 + 1 < R2) t = gf::mul(t, step);
             });
         } else {

@@ -3,7 +3,7 @@
 // X[n] = sum_j x[j] w_N^(jn), on random and adversarial inputs (components
 // at the top of the weak range, just above p, both forms of zero); every
 // output must stay weak.  Also mul_root_w against a general multiply, mul
-// with a weak second operand, and canon_w at the range edges.
+// and mul_w with a weak second operand, and canon_w at the range edges.
 #include <cstdio>
 #include <random>
 #include "dft.h"
@@ -83,6 +83,19 @@ int main() {
         if (y.re >= gf::P || y.im >= gf::P || !gf::eq(y, gf::mul(a, gf::canon_w(x)))) ++mbad;
     }
     if (mbad) printf("mul(canonical, weak): %d bad\n", mbad);
+    bad += mbad;
+    // mul_w: the same product folded once (<= p + 7); its first operand may be p
+    mbad = 0;
+    for (int t = 0; t < 200000; ++t) {
+        fe a = gf::mk(rng() % gf::P, rng() % gf::P);
+        if (t % 7 == 0) a.re = gf::P;
+        if (t % 11 == 0) a.im = gf::P;
+        const fe x = gf::mk(component((int)(rng() % 5)), component((int)(rng() % 5)));
+        const fe y = gf::mul_w(a, x);
+        if (y.re > gf::P + 7 || y.im > gf::P + 7 || !gf::eq(gf::canon_w(y), gf::mul(gf::canon_w(a), gf::canon_w(x))))
+            ++mbad;
+    }
+    if (mbad) printf("mul_w(canonical or p, weak): %d bad\n", mbad);
     bad += mbad;
     const uint64_t edges[] = {0, gf::P - 1, gf::P, gf::P + 1, (uint64_t)1 << 61, W - 1, W};
     for (uint64_t x : edges)
