@@ -216,7 +216,8 @@ def batched_leg(reps=5):
     for rep in range(2):
         ctx.sync()
         t0 = time.perf_counter()
-        proofs_seq = [ctx.prove_plain(None, None, 4, cosnark.Transcript(), dev_inputs=d)[0].tobytes() for d in dev]
+        proofs_seq = [ctx.prove_plain(None, None, 4, cosnark.Transcript(), dev_inputs=d, stage_times=False)[0].tobytes()
+                      for d in dev]
         seq.append(time.perf_counter() - t0)
     seq_s = min(seq)
     # batched: graph captured on the first pass
@@ -334,11 +335,11 @@ def main():
         for pb, t in cands:
             ctx.set_ntt_plan(log_cw, pb, t)
             for _ in range(2):
-                ctx.prove_plain(None, None, a.rate, cosnark.Transcript(), dev_inputs=(df, dg, n))
+                ctx.prove_plain(None, None, a.rate, cosnark.Transcript(), dev_inputs=(df, dg, n), stage_times=False)
             ctx.sync()
             t1 = time.perf_counter()
             for _ in range(5):
-                ctx.prove_plain(None, None, a.rate, cosnark.Transcript(), dev_inputs=(df, dg, n))
+                ctx.prove_plain(None, None, a.rate, cosnark.Transcript(), dev_inputs=(df, dg, n), stage_times=False)
             ctx.sync()
             ms = (time.perf_counter() - t1) * 200
             if best_ms is None or ms < best_ms:
@@ -347,18 +348,24 @@ def main():
         if best != cands[0]:
             tuned = "end to end"
     tune_s = time.perf_counter() - t0
-    stage = []
     proof = None
     for _ in range(a.warmup):
-        proof, _ = ctx.prove_plain(None, None, a.rate, cosnark.Transcript(), dev_inputs=(df, dg, n))
+        proof, _ = ctx.prove_plain(None, None, a.rate, cosnark.Transcript(), dev_inputs=(df, dg, n),
+                                   stage_times=False)
     ctx.sync()
+    # the timed proofs record no stage events (a caller that wants only the proof)
     t0 = time.perf_counter()
     for _ in range(a.steps):
-        proof, times = ctx.prove_plain(None, None, a.rate, cosnark.Transcript(), dev_inputs=(df, dg, n))
-        stage.append(times.copy())
+        proof, _ = ctx.prove_plain(None, None, a.rate, cosnark.Transcript(), dev_inputs=(df, dg, n),
+                                   stage_times=False)
     ctx.sync()
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1e3 / a.steps
+    # stage breakdown of the same overlapped proofs, from HIP events, in a run of its own
+    stage = []
+    for _ in range(max(3, a.steps // 4)):
+        _, times = ctx.prove_plain(None, None, a.rate, cosnark.Transcript(), dev_inputs=(df, dg, n))
+        stage.append(times.copy())
     st = np.mean(np.array(stage), axis=0)   # encode, merkle, sumcheck, total (ms, HIP events)
 
     # Stage times for the roofline come from a serial pass (stages one after

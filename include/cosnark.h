@@ -409,7 +409,8 @@ int cosnark_ntt_plan(cosnark_ctx *ctx, int logn, int *passbits, int *tile);
 /* Sets the plan for 2^logn-point transforms (as cosnark_autotune would):
  * passes of at most `passbits` bits (2..8), tiles of `tile` elements
  * (a power of two in 256..4096).  For callers that tune end to end, e.g.
- * the encode beside an overlapped sumcheck. */
+ * the encode beside an overlapped sumcheck.  passbits = tile = 0 restores
+ * the context defaults for that size. */
 int cosnark_set_ntt_plan(cosnark_ctx *ctx, int logn, int passbits, int tile);
 /* Measured throughput of the library's GF(p^2) multiply on this device
  * (independent multiply chains, no memory traffic): the compute ceiling the

@@ -279,8 +279,9 @@ class Context:
                                             _p(polys), _p(r), _p(fin))
         return rc, polys, r, fin
 
-    def prove_plain(self, f, g, rate, ts: Transcript, dev_inputs=None):
-        """Returns (proof bytes, stage times ms [encode, merkle, sumcheck, total])."""
+    def prove_plain(self, f, g, rate, ts: Transcript, dev_inputs=None, stage_times=True):
+        """Returns (proof bytes, stage times ms [encode, merkle, sumcheck, total]);
+        with stage_times=False no stage events are recorded and the times are zero."""
         if dev_inputs is None:
             a = np.ascontiguousarray(f, dtype=np.uint64)
             b = np.ascontiguousarray(g, dtype=np.uint64)
@@ -292,7 +293,7 @@ class Context:
         proof = np.zeros(nb, np.uint8)
         times = np.zeros(4, np.float64)
         self._ck(lib().cosnark_prove_plain(self.h, df.ptr, dg.ptr, n, rate, ctypes.byref(ts), _p(proof),
-                                           _p(times)))
+                                           _p(times) if stage_times else None))
         return proof, times
 
     # ---- WHIR ----

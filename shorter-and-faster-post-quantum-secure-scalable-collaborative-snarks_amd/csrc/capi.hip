@@ -151,6 +151,8 @@ int cosnark_create(int device, cosnark_ctx **out) {
         hipHostMalloc(&ctx->sc_host, sizeof(ScState), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc(&ctx->co_host, CO_HOST_ELEMS * sizeof(fe), hipHostMallocDefault) != hipSuccess)
         return fail(COSNARK_E_NOMEM);
+    if (hipHostGetDevicePointer((void **)&ctx->sc_host_dev, ctx->sc_host, 0) != hipSuccess)
+        return fail(COSNARK_E_HIP);
     if (hipMemset(ctx->sc, 0, sizeof(ScState)) != hipSuccess) return fail(COSNARK_E_HIP);
     if (sumcheck_init(ctx) != COSNARK_OK) return fail(COSNARK_E_HIP);
     if (ntt_init(ctx) != COSNARK_OK) return fail(COSNARK_E_HIP);
@@ -373,10 +375,12 @@ static int sc_stage_state(cosnark_ctx *ctx, const fe &prev, const fe &y, int der
     return sumcheck_state_init(ctx, prev, y, derive_y, s);
 }
 
-// one copy of the device state into the pinned mirror, then results out
+// the device state written into the pinned mirror by one kernel (with a
+// root, if given), then results out
 static int sc_collect(cosnark_ctx *ctx, int m, cosnark_transcript *t, uint64_t *h_polys, uint64_t *h_r,
-                      uint64_t *h_final, hipStream_t s) {
-    HIPCK(hipMemcpyAsync(ctx->sc_host, ctx->sc, sizeof(ScState), hipMemcpyDeviceToHost, s));
+                      uint64_t *h_final, hipStream_t s, const uint8_t *root_src = nullptr) {
+    int rc = sumcheck_export(ctx, root_src, s);
+    if (rc) return rc;
     HIPCK(hipStreamSynchronize(s));
     const ScState &h = *ctx->sc_host;
     if (t) { t->prev[0] = h.tr_prev[m].re; t->prev[1] = h.tr_prev[m].im; }
@@ -433,29 +437,29 @@ int cosnark_prove_plain(cosnark_ctx *ctx, const uint64_t *d_f, const uint64_t *d
     if (rc) return rc;
     // commit (encode + Merkle) on s1, the sumcheck beside it on s2: the
     // transcript never absorbs the root, so the two are independent
+    // stage events only when the caller asks for stage times
     hipStream_t s1 = ctx->stream, s2 = ctx->overlap ? ctx->stream2 : ctx->stream;
     HIPCK(hipEventRecord(ctx->ev[7], ctx->stream));
     if (s1 != ctx->stream) HIPCK(hipStreamWaitEvent(s1, ctx->ev[7], 0));
-    HIPCK(hipEventRecord(ctx->ev[0], s1));
-    bool leaves_done = false;
-    if ((rc = ntt_launch_leaves(ctx, (const fe *)d_f, n, cw, ilog2(cw_n), nd, &leaves_done, s1))) return rc;
-    HIPCK(hipEventRecord(ctx->ev[1], s1));
-    HIPCK(hipStreamWaitEvent(s2, ctx->ev[7], 0));
-    HIPCK(hipEventRecord(ctx->ev[4], s2));
+    if (times) HIPCK(hipEventRecord(ctx->ev[0], s1));
+    int levels_done = 0;
+    if ((rc = ntt_launch_leaves(ctx, (const fe *)d_f, n, cw, ilog2(cw_n), nd, &levels_done, s1))) return rc;
+    if (times) HIPCK(hipEventRecord(ctx->ev[1], s1));
+    if (s2 != ctx->stream) HIPCK(hipStreamWaitEvent(s2, ctx->ev[7], 0));
+    if (times) HIPCK(hipEventRecord(ctx->ev[4], s2));
     if ((rc = sumcheck_launch(ctx, (const fe *)d_f, (const fe *)d_g, n, m, scb, s2))) return rc;
     HIPCK(hipEventRecord(ctx->ev[3], s2));
-    HIPCK(hipEventRecord(ctx->ev[6], s1));
-    // the leaf level came from the encode's last pass where the plan allowed it
-    if ((rc = leaves_done ? merkle_from_leaves_launch(ctx, cw_n / 4, nd, true, s1)
+    if (times) HIPCK(hipEventRecord(ctx->ev[6], s1));
+    // the lowest levels came from the encode's last pass where the plan allowed it
+    if ((rc = levels_done ? merkle_from_leaves_launch(ctx, cw_n / 4, nd, true, s1, levels_done)
                           : merkle_launch(ctx, cw, cw_n, nd, true, s1)))
         return rc;
     HIPCK(hipEventRecord(ctx->ev[2], s1));
-    HIPCK(hipStreamWaitEvent(ctx->stream, ctx->ev[2], 0));
-    HIPCK(hipStreamWaitEvent(ctx->stream, ctx->ev[3], 0));
-    HIPCK(hipEventRecord(ctx->ev[5], ctx->stream));
-    HIPCK(hipMemcpyAsync(ctx->sc->root, nd + 32 * (nodes - 1), 32, hipMemcpyDeviceToDevice, ctx->stream));
+    if (s1 != ctx->stream) HIPCK(hipStreamWaitEvent(ctx->stream, ctx->ev[2], 0));
+    if (s2 != ctx->stream) HIPCK(hipStreamWaitEvent(ctx->stream, ctx->ev[3], 0));
+    if (times) HIPCK(hipEventRecord(ctx->ev[5], ctx->stream));
     std::vector<uint64_t> polys(6 * m), fin(4);
-    rc = sc_collect(ctx, m, t, polys.data(), nullptr, fin.data(), ctx->stream);
+    rc = sc_collect(ctx, m, t, polys.data(), nullptr, fin.data(), ctx->stream, nd + 32 * (nodes - 1));
     memcpy(h_proof, ctx->sc_host->root, 32);
     memcpy(h_proof + 32, polys.data(), 48ull * m);
     memcpy(h_proof + 32 + 48ull * m, fin.data(), 32);

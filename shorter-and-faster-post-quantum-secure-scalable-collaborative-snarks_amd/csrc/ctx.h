@@ -46,6 +46,7 @@ struct cosnark_ctx {
     size_t ntt_tmp_bytes = 0;
     ScState *sc = nullptr;            // device
     ScState *sc_host = nullptr;       // pinned host mirror for copy-in / copy-out
+    ScState *sc_host_dev = nullptr;   // the same memory as a device address
     gf::fe *co_host = nullptr;        // pinned staging, CO_HOST_ELEMS elements (small results, weights)
     gf::fe *sc_partials = nullptr;    // device, 2 x 3 * SC_MAX_BLOCKS elements (ping-pong)
     hipEvent_t ev[8] = {};
@@ -96,23 +97,26 @@ int ntt_launch(cosnark_ctx *ctx, const gf::fe *in, uint64_t in_len, gf::fe *out,
 // writes out_tab[y] (or out + y * 2^logn).  Passes before the last use
 // tmp (batch * 2^logn elements) or, when null, the context's buffer.
 // leaves (optional): the last pass also writes item 0's Merkle leaf digests
-// (BLAKE3 of each 4 consecutive outputs, 2^logn / 4 digests) when the plan
-// allows it; *leaves_done tells whether it did.
+// (BLAKE3 of each 4 consecutive outputs, 2^logn / 4 digests: level 0 of a
+// node array laid out level after level) when the plan allows it;
+// *levels_done receives the number of tree levels written (0 or 1).
 int ntt_launch_batch(cosnark_ctx *ctx, const gf::fe *in, const gf::fe *const *in_tab, uint64_t in_len,
                      gf::fe *out, gf::fe *const *out_tab, int batch, int logn, bool inverse, hipStream_t s,
-                     gf::fe *tmp, uint8_t *leaves = nullptr, bool *leaves_done = nullptr);
+                     gf::fe *tmp, uint8_t *leaves = nullptr, int *levels_done = nullptr);
 // RS encode (forward NTT of in || 0) of 2^logn points with the codeword's
 // Merkle leaf digests written by the last pass where the plan allows it
-// (*leaves_done); otherwise the caller hashes the leaves itself.
-int ntt_launch_leaves(cosnark_ctx *ctx, const gf::fe *in, uint64_t in_len, gf::fe *out, int logn, uint8_t *leaves,
-                      bool *leaves_done, hipStream_t s);
+// (*levels_done, as above); the caller builds the rest of the tree.
+int ntt_launch_leaves(cosnark_ctx *ctx, const gf::fe *in, uint64_t in_len, gf::fe *out, int logn, uint8_t *nodes,
+                      int *levels_done, hipStream_t s);
 int ntt_plan_bits(const cosnark_ctx *ctx, int logn);
 int ntt_plan_tile(const cosnark_ctx *ctx, int logn);
 // Merkle commit of n_elems elements into nodes (all levels).
 int merkle_launch(cosnark_ctx *ctx, const gf::fe *elems, uint64_t n_elems, uint8_t *nodes,
                   bool dup_left, hipStream_t s);
-// levels 1.. of a tree whose `leaves` level-0 digests are in nodes[0 .. leaves)
-int merkle_from_leaves_launch(cosnark_ctx *ctx, uint64_t leaves, uint8_t *nodes, bool dup_left, hipStream_t s);
+// levels `have`.. of a tree whose levels 0 .. have-1 (`leaves` level-0
+// digests first) are already in nodes
+int merkle_from_leaves_launch(cosnark_ctx *ctx, uint64_t leaves, uint8_t *nodes, bool dup_left, hipStream_t s,
+                              int have = 1);
 // distributed_MT tree: first level = column roots of a row-major nrows x Lc
 // matrix (rows nrows..31 zero), then the dup-left tree (2 Lc - 1 digests).
 int merkle_columns_launch(cosnark_ctx *ctx, const gf::fe *cw, int nrows, uint64_t Lc, uint8_t *nodes,
@@ -168,6 +172,9 @@ int sumcheck_batch_run(cosnark_ctx *ctx, const ScBatchPlan &plan, const uint8_t 
 // round 0, or leave unchecked (the caller verifies the claim chain)
 constexpr int SC_CLAIM_CHECK = 0, SC_CLAIM_DERIVE = 1, SC_CLAIM_UNCHECKED = 2;
 int sumcheck_state_init(cosnark_ctx *ctx, const gf::fe &prev, const gf::fe &y, int claim_mode, hipStream_t s);
+// copy the device state (with root_src's 32 bytes as its root, if set) to
+// the pinned host mirror from a kernel
+int sumcheck_export(cosnark_ctx *ctx, const uint8_t *root_src, hipStream_t s);
 // one multi-workgroup round (fold by the previous round's challenge when
 // `fold`); *nb_out = workgroups whose partials the next launch reduces
 int sumcheck_round_launch(cosnark_ctx *ctx, const gf::fe *s1, const gf::fe *s2, gf::fe *d1, gf::fe *d2,

@@ -246,15 +246,16 @@ std::vector<std::pair<MkJob, uint64_t>> tree_stages(int mode, const fe *elems, u
     return st;
 }
 
-// Levels 1.. above `leaves` level-0 digests already in nodes[0 .. leaves).
-std::vector<std::pair<MkJob, uint64_t>> upper_stages(uint64_t leaves, uint8_t *nodes, bool dup_left) {
+// Levels have.. above levels 0 .. have-1 already in nodes (level 0: `leaves`
+// digests).
+std::vector<std::pair<MkJob, uint64_t>> upper_stages(uint64_t leaves, uint8_t *nodes, bool dup_left, int have) {
     std::vector<std::pair<MkJob, uint64_t>> st;
     MkJob J{};
     J.nodes = nodes;
     J.lo = level_offsets(leaves);
     J.dup_left = (int)dup_left;
-    int top = 0;
-    uint64_t count = leaves;
+    int top = have - 1;
+    uint64_t count = leaves >> top;
     while (count > 1) {
         const uint64_t next = count / 2;
         MkJob U = J;
@@ -290,10 +291,13 @@ int merkle_launch(cosnark_ctx *ctx, const fe *elems, uint64_t n_elems, uint8_t *
     return build_tree(ctx, MODE_ELEMS, elems, n_elems / 4, nodes, dup_left, 0, 0, stream);
 }
 
-int merkle_from_leaves_launch(cosnark_ctx *ctx, uint64_t leaves, uint8_t *nodes, bool dup_left, hipStream_t stream) {
+int merkle_from_leaves_launch(cosnark_ctx *ctx, uint64_t leaves, uint8_t *nodes, bool dup_left, hipStream_t stream,
+                              int have) {
     if (!is_pow2(leaves) || leaves > (1ull << 30))
         return set_err(ctx, COSNARK_E_ARG, "merkle: leaf count must be a power of two <= 2^30");
-    for (const auto &st : upper_stages(leaves, nodes, dup_left)) {
+    if (have < 1 || (leaves >> (have - 1)) < 1)
+        return set_err(ctx, COSNARK_E_ARG, "merkle: levels already built out of range");
+    for (const auto &st : upper_stages(leaves, nodes, dup_left, have)) {
         hipLaunchKernelGGL(merkle_block_kernel<MODE_NODES>, dim3((unsigned)st.second), dim3(MK_THREADS), 0, stream,
                            st.first);
         LAUNCHCK();

@@ -284,8 +284,8 @@ int ntt_plan_tile(const cosnark_ctx *ctx, int logn) {
 
 int ntt_launch_batch(cosnark_ctx *ctx, const fe *in, const fe *const *in_tab, uint64_t in_len, fe *out,
                      fe *const *out_tab, int batch, int logn, bool inverse, hipStream_t stream, fe *tmp_own,
-                     uint8_t *leaves, bool *leaves_done) {
-    if (leaves_done) *leaves_done = false;
+                     uint8_t *leaves, int *levels_done) {
+    if (levels_done) *levels_done = 0;
     if (logn < 1 || logn > NTT_LOG_MAX) return set_err(ctx, COSNARK_E_RANGE, "ntt: logn out of range");
     if (batch < 1 || batch > 65535) return set_err(ctx, COSNARK_E_RANGE, "ntt: batch out of range");
     // radix plan: fewest passes of at most `passbits` bits, spread evenly, larger first
@@ -352,7 +352,7 @@ int ntt_launch_batch(cosnark_ctx *ctx, const fe *in, const fe *const *in_tab, ui
             const int r1 = r >= 7 ? 16 : r >= 5 ? 8 : r >= 3 ? 4 : 2;
             if (leaves && m >= 2 && logC >= 2 && (r1 << logC) <= NTT_THREADS) {
                 q.leaves = leaves;
-                if (leaves_done) *leaves_done = true;
+                if (levels_done) *levels_done = 1;
             }
         } else {
             dst = tmp;
@@ -366,10 +366,10 @@ int ntt_launch_batch(cosnark_ctx *ctx, const fe *in, const fe *const *in_tab, ui
     return COSNARK_OK;
 }
 
-int ntt_launch_leaves(cosnark_ctx *ctx, const fe *in, uint64_t in_len, fe *out, int logn, uint8_t *leaves,
-                      bool *leaves_done, hipStream_t stream) {
-    return ntt_launch_batch(ctx, in, nullptr, in_len, out, nullptr, 1, logn, false, stream, nullptr, leaves,
-                            leaves_done);
+int ntt_launch_leaves(cosnark_ctx *ctx, const fe *in, uint64_t in_len, fe *out, int logn, uint8_t *nodes,
+                      int *levels_done, hipStream_t stream) {
+    return ntt_launch_batch(ctx, in, nullptr, in_len, out, nullptr, 1, logn, false, stream, nullptr, nodes,
+                            levels_done);
 }
 
 int ntt_launch(cosnark_ctx *ctx, const fe *in, uint64_t in_len, fe *out, int logn, bool inverse,

@@ -22,6 +22,7 @@
 // quadratics in the challenge still being hashed, so a round costs the hash
 // chain and little else.
 #include <algorithm>
+#include <cstddef>
 #include <cstring>
 #include <vector>
 #include "ctx.h"
@@ -461,6 +462,31 @@ __global__ void sc_init_batch_kernel(const ScJob *tail, const fe *prev, int coun
 
 int sumcheck_init(cosnark_ctx *ctx) {
     (void)ctx;
+    return COSNARK_OK;
+}
+
+// The state (and, when root_src is set, a 32-byte root into st->root) copied
+// into host memory by vector stores: one launch in place of a device-to-
+// device and a device-to-host copy at the end of a proof.
+__global__ __launch_bounds__(256) void sc_export_kernel(const ScState *__restrict__ st,
+                                                        const uint8_t *__restrict__ root_src, ScState *host) {
+    constexpr int W = (int)(sizeof(ScState) / 16);
+    const uint4 *src = reinterpret_cast<const uint4 *>(st);
+    uint4 *dst = reinterpret_cast<uint4 *>(host);
+    const int root_w = (int)(offsetof(ScState, root) / 16);
+    for (int i = threadIdx.x; i < W; i += 256) {
+        if (root_src && (i == root_w || i == root_w + 1))
+            dst[i] = reinterpret_cast<const uint4 *>(root_src)[i - root_w];
+        else
+            dst[i] = src[i];
+    }
+}
+static_assert(sizeof(ScState) % 16 == 0 && offsetof(ScState, root) % 16 == 0, "ScState: 16-byte words");
+
+int sumcheck_export(cosnark_ctx *ctx, const uint8_t *root_src, hipStream_t stream) {
+    hipLaunchKernelGGL(sc_export_kernel, dim3(1), dim3(256), 0, stream, (const ScState *)ctx->sc, root_src,
+                       ctx->sc_host_dev);
+    LAUNCHCK();
     return COSNARK_OK;
 }
 

@@ -145,6 +145,11 @@ int cosnark_ntt_plan(cosnark_ctx *ctx, int logn, int *passbits, int *tile) {
 
 int cosnark_set_ntt_plan(cosnark_ctx *ctx, int logn, int passbits, int tile) {
     if (!ctx || logn < 1 || logn > NTT_LOG_MAX) return COSNARK_E_ARG;
+    if (passbits == 0 && tile == 0) {      // back to the context defaults
+        ctx->ntt_plan[logn].passbits = 0;
+        ctx->ntt_plan[logn].tile = 0;
+        return COSNARK_OK;
+    }
     if (passbits < 2 || passbits > NTT_PASS_LOG_MAX || tile < 256 || tile > NTT_LDS_ELEMS || !is_pow2((uint64_t)tile))
         return set_err(ctx, COSNARK_E_ARG, "set_ntt_plan: passbits in [2, 8], tile a power of two in [256, 4096]");
     ctx->ntt_plan[logn].passbits = passbits;

@@ -296,9 +296,13 @@ int cosnark_whir_commit(cosnark_ctx *ctx, const uint64_t *d_arr, uint64_t n, int
         n * (uint64_t)rate > (1ull << NTT_LOG_MAX))
         return set_err(ctx, COSNARK_E_ARG, "whir_commit: n, rate powers of two, 8 <= n*rate <= 2^28");
     const uint64_t cw_n = n * (uint64_t)rate;
-    int rc = ntt_launch(ctx, (const fe *)d_arr, n, (fe *)d_codeword, ilog2(cw_n), false, ctx->stream);
+    // the encode's last pass writes the tree's lowest levels where its plan allows
+    int levels = 0;
+    int rc = ntt_launch_leaves(ctx, (const fe *)d_arr, n, (fe *)d_codeword, ilog2(cw_n), d_nodes, &levels,
+                               ctx->stream);
     if (rc) return rc;
-    rc = merkle_launch(ctx, (const fe *)d_codeword, cw_n, d_nodes, true, ctx->stream);
+    rc = levels ? merkle_from_leaves_launch(ctx, cw_n / 4, d_nodes, true, ctx->stream, levels)
+                : merkle_launch(ctx, (const fe *)d_codeword, cw_n, d_nodes, true, ctx->stream);
     if (rc) return rc;
     return cosnark_sync(ctx);
 }

@@ -20,6 +20,25 @@ def test_whir_commit_matches_oracle(ctx, logn, rate):
     assert np.array_equal(ctx.download(nodes, (cnt, 32), np.uint8), O.merkle_commit(O.rs_encode(arr, rate), True))
 
 
+@pytest.mark.parametrize("passbits,tile", [(8, 4096), (8, 2048), (6, 4096)])
+def test_whir_commit_fused_tree_levels(ctx, passbits, tile):
+    """A 2^16-point codeword under three NTT plans: radix-256 passes with 16-
+    and 8-column tiles (the last pass writes the leaf digests) and radix-64
+    passes (it does not); every node must match, since the dup-left root
+    alone depends on leaf 0 only."""
+    n, rate = 1 << 14, 4
+    arr = O.synth(0xC05A0013, passbits, n)
+    ctx.set_ntt_plan(16, passbits, tile)
+    try:
+        cw, nodes = ctx.whir_commit(ctx.upload(arr), n, rate)
+    finally:
+        ctx.set_ntt_plan(16, 0, 0)
+    ref_cw = O.rs_encode(arr, rate)
+    assert np.array_equal(ctx.download(cw, (n * rate, 2)), ref_cw)
+    cnt = cosnark.merkle_node_count(n * rate)
+    assert np.array_equal(ctx.download(nodes, (cnt, 32), np.uint8), O.merkle_commit(ref_cw, True))
+
+
 def test_whir_claim_matches_oracle(ctx):
     n = 1 << 12
     arr, z = O.synth(3, 0, n), O.synth(3, 1, 12)
