@@ -402,6 +402,11 @@ def main():
     launches = {"rs_encode": -(-log_cw // passbits), "merkle": mk_launches, "sumcheck": sc_launches}
     names = ["rs_encode", "merkle", "sumcheck"]
     fmul_peak = ctx.fmul_rate()
+    hash_peak = ctx.blake3_rate()
+    # the Merkle stage hashes the internal nodes (the leaves come from the
+    # encode's last pass); serial stage time
+    mk_compress = cw_n // 4 - 1
+    mk_rate = mk_compress / (float(sst[1]) / 1e3)
     # Dominant kernel: the NTT pass (the largest share of device time in the
     # timed steps and the longest serial stage).  Its average launch time
     # comes from the encode stage's HIP events on the commit stream over the
@@ -459,7 +464,13 @@ def main():
                               "frac": round(fmul[dom] / (dom_ms / 1e3) / fmul_peak, 4),
                               "note": "the NTT is bound by 64-bit integer multiply throughput, not HBM; "
                                       "peak = cosnark_fmul_rate, the same GF(p^2) multiply in independent "
-                                      "chains on this GPU, measured in this run"}},
+                                      "chains on this GPU, measured in this run"},
+                     "hash": {"merkle_stage_compress_per_s": round(mk_rate, 1),
+                              "peak_compress_per_s": round(hash_peak, 1),
+                              "frac": round(mk_rate / hash_peak, 4),
+                              "note": "Merkle stage (serial): 2^k - 1 internal-node compressions over its time; "
+                                      "peak = cosnark_blake3_rate, the same compression in independent chains, "
+                                      "measured in this run"}},
     }
     df.free(); dg.free()
     if not a.no_whir:

@@ -416,6 +416,11 @@ int cosnark_set_ntt_plan(cosnark_ctx *ctx, int logn, int passbits, int tile);
  * (independent multiply chains, no memory traffic): the compute ceiling the
  * transform kernels are reported against.  No reference counterpart. */
 int cosnark_fmul_rate(cosnark_ctx *ctx, double *per_s);
+/* Measured throughput of the library's single-block BLAKE3 compression on
+ * this device (independent chains, no memory traffic): the ceiling the
+ * Merkle and leaf-hashing kernels are reported against.  No reference
+ * counterpart. */
+int cosnark_blake3_rate(cosnark_ctx *ctx, double *per_s);
 
 #ifdef __cplusplus
 }

@@ -103,6 +103,7 @@ def lib():
         L.cosnark_autotune.argtypes = [vp, i32, i32, i32, vp]
         L.cosnark_ntt_plan.argtypes = [vp, i32, ctypes.POINTER(i32), ctypes.POINTER(i32)]
         L.cosnark_fmul_rate.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
+        L.cosnark_blake3_rate.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
         L.cosnark_server_create.argtypes = [i32, u64, i32, i32, i32, ctypes.POINTER(vp)]
         L.cosnark_server_submit.argtypes = [vp, vp, vp, u64, vp, ctypes.POINTER(u64)]
         L.cosnark_server_poll.argtypes = [vp, u64, ctypes.POINTER(i32)]
@@ -530,6 +531,11 @@ class Context:
     def fmul_rate(self):
         r = ctypes.c_double(0)
         self._ck(lib().cosnark_fmul_rate(self.h, ctypes.byref(r)))
+        return r.value
+
+    def blake3_rate(self):
+        r = ctypes.c_double(0)
+        self._ck(lib().cosnark_blake3_rate(self.h, ctypes.byref(r)))
         return r.value
 
     def ntt_plan(self, logn):

@@ -75,4 +75,71 @@ B3_HD void compress64(const uint32_t m[16], uint32_t out[8]) {
 #undef B3_ROUND
 #undef B3_G
 
+// ---- one compression on four lanes (device only) ----
+// Lanes 4j..4j+3 of a wave call this together, each with the whole 16-word
+// message.  Lane q holds column q of the state (v[q], v[4+q], v[8+q],
+// v[12+q]): the column half-round is one G per lane, the diagonal half-round
+// rotates rows 1..3 by 1..3 lanes with DPP quad permutes, runs one G per lane
+// and rotates back.  The chain is about 300 instructions instead of about
+// 700 on one lane.  Lane q returns digest words q and q + 4.
+
+// message-word index used by round r at slot i: the permutation
+// (2,6,3,10,7,0,4,13,1,11,12,5,9,14,15,8) applied r times
+struct QuadSchedule {
+    int s[7][16];
+    constexpr QuadSchedule() : s() {
+        constexpr int perm[16] = {2, 6, 3, 10, 7, 0, 4, 13, 1, 11, 12, 5, 9, 14, 15, 8};
+        for (int i = 0; i < 16; ++i) s[0][i] = i;
+        for (int r = 1; r < 7; ++r)
+            for (int i = 0; i < 16; ++i) s[r][i] = s[r - 1][perm[i]];
+    }
+};
+constexpr QuadSchedule QUAD_SCHED{};
+
+template <int CTRL>
+__device__ __forceinline__ uint32_t qperm(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, CTRL, 0xF, 0xF, true);
+}
+// DPP quad_perm control: lane q reads lane p[q] of its quad
+constexpr int qctl(int p0, int p1, int p2, int p3) { return p0 | (p1 << 2) | (p2 << 4) | (p3 << 6); }
+
+// word w_q for lane q, branch-free: k0 / k1 all-ones where bit 0 / bit 1 of q is set
+__device__ __forceinline__ uint32_t qsel(uint32_t k0, uint32_t k1, uint32_t w0, uint32_t w1, uint32_t w2,
+                                         uint32_t w3) {
+    const uint32_t a = (w0 & ~k0) | (w1 & k0), b = (w2 & ~k0) | (w3 & k0);
+    return (a & ~k1) | (b & k1);
+}
+
+__device__ __forceinline__ void qg(uint32_t &a, uint32_t &b, uint32_t &c, uint32_t &d, uint32_t x, uint32_t y) {
+    a = a + b + x; d = rotr(d ^ a, 16);
+    c = c + d;     b = rotr(b ^ c, 12);
+    a = a + b + y; d = rotr(d ^ a, 8);
+    c = c + d;     b = rotr(b ^ c, 7);
+}
+
+__device__ __forceinline__ void compress64_quad(const uint32_t m[16], uint32_t &h_lo, uint32_t &h_hi) {
+    const uint32_t q = threadIdx.x & 3;
+    const uint32_t k0 = 0u - (q & 1), k1 = 0u - (q >> 1);
+    uint32_t a = qsel(k0, k1, IV0, IV1, IV2, IV3);
+    uint32_t b = qsel(k0, k1, IV4, IV5, IV6, IV7);
+    uint32_t c = a;
+    uint32_t d = qsel(k0, k1, 0u, 0u, 64u, FLAGS_ROOT_BLOCK);
+#pragma unroll
+    for (int r = 0; r < 7; ++r) {
+        const int *S = QUAD_SCHED.s[r];
+        qg(a, b, c, d, qsel(k0, k1, m[S[0]], m[S[2]], m[S[4]], m[S[6]]),
+           qsel(k0, k1, m[S[1]], m[S[3]], m[S[5]], m[S[7]]));
+        b = qperm<qctl(1, 2, 3, 0)>(b);
+        c = qperm<qctl(2, 3, 0, 1)>(c);
+        d = qperm<qctl(3, 0, 1, 2)>(d);
+        qg(a, b, c, d, qsel(k0, k1, m[S[8]], m[S[10]], m[S[12]], m[S[14]]),
+           qsel(k0, k1, m[S[9]], m[S[11]], m[S[13]], m[S[15]]));
+        b = qperm<qctl(3, 0, 1, 2)>(b);
+        c = qperm<qctl(2, 3, 0, 1)>(c);
+        d = qperm<qctl(1, 2, 3, 0)>(d);
+    }
+    h_lo = a ^ c;   // word q: v[q] ^ v[8 + q]
+    h_hi = b ^ d;   // word 4 + q: v[4 + q] ^ v[12 + q]
+}
+
 }  // namespace b3

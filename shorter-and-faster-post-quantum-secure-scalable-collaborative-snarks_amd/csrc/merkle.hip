@@ -9,8 +9,9 @@
 // One message per lane.  A workgroup of 256 lanes takes a 1024-node block of
 // its first level: each lane hashes four consecutive nodes and the two
 // levels above them in registers, then the workgroup climbs the remaining
-// eight levels of the block through LDS, writing every node; the block roots
-// feed the next launch.  A 2^22-leaf tree takes three launches.
+// eight levels of the block through LDS, writing every node (levels of at
+// most 64 nodes with four lanes per compression); the block roots feed the
+// next launch.  A 2^22-leaf tree takes three launches.
 //
 // Column mode serves distributed_MT (src/coPCS.cpp:152-183): first-level
 // node j is the root of MT_commit_Blake over column j of a 32-row codeword
@@ -170,6 +171,24 @@ __device__ __forceinline__ void merkle_block(const MkJob &J, uint64_t blk) {
         __syncthreads();
         const int cnt = (1 << block_log) >> s;
         const uint64_t gbase = g0 >> s;
+        if (4 * cnt <= MK_THREADS) {
+            // few nodes: four lanes per compression shortens each level's chain
+            if (tid < 4 * cnt) {
+                const int i = tid >> 2, q = tid & 3;
+                const int ri = dup_left ? 2 * i : 2 * i + 1;
+                uint32_t m[16], lo_w, hi_w;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) { m[j] = src[2 * i].w[j]; m[8 + j] = src[ri].w[j]; }
+                b3::compress64_quad(m, lo_w, hi_w);
+                uint32_t *g = reinterpret_cast<uint32_t *>(nodes + 32 * (lo.off[lvl0 + s] + gbase + i));
+                g[q] = lo_w;
+                g[4 + q] = hi_w;
+                dst[i].w[q] = lo_w;
+                dst[i].w[4 + q] = hi_w;
+            }
+            Digest *t = src; src = dst; dst = t;
+            continue;
+        }
         for (int i = tid; i < cnt; i += MK_THREADS) {
             uint32_t m[16], hh[8];
 #pragma unroll

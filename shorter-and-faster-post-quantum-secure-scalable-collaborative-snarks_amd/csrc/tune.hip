@@ -8,6 +8,7 @@
 // device, with `batch` transforms per launch as the batched prover issues
 // them, and keeps the fastest per size in the context.
 #include <vector>
+#include "blake3.h"
 #include "ctx.h"
 
 using gf::fe;
@@ -33,9 +34,77 @@ __global__ __launch_bounds__(256) void fmul_rate_kernel(fe b, int iters, fe *out
     out[gid] = s;
 }
 
+// Independent chains of single-block BLAKE3 compressions per lane, each
+// digest fed back as the next message's first half: the hashing ceiling the
+// Merkle kernels are measured against.
+constexpr int HR_CHAINS = 2;
+__global__ __launch_bounds__(256) void blake3_rate_kernel(int iters, uint32_t *out) {
+    const uint32_t gid = blockIdx.x * 256 + threadIdx.x;
+    uint32_t m[HR_CHAINS][16];
+#pragma unroll
+    for (int k = 0; k < HR_CHAINS; ++k)
+#pragma unroll
+        for (int i = 0; i < 16; ++i) m[k][i] = gid * 16 + i + 977u * k;
+    for (int it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int k = 0; k < HR_CHAINS; ++k) {
+            uint32_t h[8];
+            b3::compress64(m[k], h);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) m[k][i] = h[i];
+        }
+    }
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < HR_CHAINS; ++k) s ^= m[k][0];
+    out[gid] = s;
+}
+
+template <class Launch>
+int time_best(cosnark_ctx *ctx, Launch launch, float *best_ms) {
+    hipEvent_t e0, e1;
+    HIPCK(hipEventCreate(&e0));
+    HIPCK(hipEventCreate(&e1));
+    float best = 1e30f;
+    for (int rep = 0; rep < 3; ++rep) {
+        HIPCK(hipEventRecord(e0, ctx->stream));
+        launch();
+        LAUNCHCK();
+        HIPCK(hipEventRecord(e1, ctx->stream));
+        HIPCK(hipEventSynchronize(e1));
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (rep > 0 && ms < best) best = ms;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *best_ms = best;
+    return COSNARK_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int cosnark_blake3_rate(cosnark_ctx *ctx, double *per_s) {
+    if (!ctx || !per_s) return COSNARK_E_ARG;
+    HIPCK(hipSetDevice(ctx->device));
+    constexpr unsigned blocks = 8192;
+    constexpr int iters = 32;
+    void *out;
+    int rc = scratch_reserve(ctx, sizeof(uint32_t) * 256 * (size_t)blocks, &out);
+    if (rc) return rc;
+    float best = 0;
+    rc = time_best(
+        ctx,
+        [&]() {
+            hipLaunchKernelGGL(blake3_rate_kernel, dim3(blocks), dim3(256), 0, ctx->stream, iters, (uint32_t *)out);
+        },
+        &best);
+    if (rc) return rc;
+    *per_s = (double)blocks * 256 * iters * HR_CHAINS / (best / 1e3);
+    return COSNARK_OK;
+}
 
 int cosnark_fmul_rate(cosnark_ctx *ctx, double *per_s) {
     if (!ctx || !per_s) return COSNARK_E_ARG;
