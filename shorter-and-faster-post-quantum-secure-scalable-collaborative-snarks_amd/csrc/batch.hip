@@ -16,8 +16,8 @@
 //   stream 2: the sumchecks, one launch per round for every proof still
 //             above the LDS tail, then one tail workgroup per proof;
 //   join; one kernel packs every proof's bytes into one buffer; one copy out.
-// The pass is captured into a hipGraph at the first call and replayed after
-// that.  Every buffer the graph touches belongs to the batch, so later work
+// The commits and the sumchecks are captured into one hipGraph each at the
+// first call and replayed after that, on their two streams.  Every buffer the graph touches belongs to the batch, so later work
 // on the context (which may regrow its own scratch) cannot invalidate it.
 #include <algorithm>
 #include <cstring>
@@ -144,14 +144,17 @@ uint64_t batch_tmp_elems(const BatchCore &C) {
     return t;
 }
 
-int batch_core_enqueue(cosnark_ctx *ctx, const BatchCore &C, const uint8_t *tables, const fe *prev_host,
-                       uint8_t *out_host, fe *tmp, uint64_t tmp_elems, hipStream_t s1, hipStream_t s2,
-                       hipEvent_t fork, hipEvent_t join) {
-    HIPCK(hipMemcpyAsync(C.prev_dev, prev_host, sizeof(fe) * C.count, hipMemcpyHostToDevice, s1));
-    HIPCK(hipEventRecord(fork, s1));
-    HIPCK(hipStreamWaitEvent(s2, fork, 0));
-    int rc = sumcheck_batch_run(ctx, C.sc, tables, s2);
-    if (rc) return rc;
+// The three parts of a pass: the sumchecks (transcripts in first), the
+// commits, and the packing of every proof's bytes with the copy out.
+int batch_enqueue_sumchecks(cosnark_ctx *ctx, const BatchCore &C, const uint8_t *tables, const fe *prev_host,
+                            hipStream_t s) {
+    HIPCK(hipMemcpyAsync(C.prev_dev, prev_host, sizeof(fe) * C.count, hipMemcpyHostToDevice, s));
+    return sumcheck_batch_run(ctx, C.sc, tables, s);
+}
+
+int batch_enqueue_commits(cosnark_ctx *ctx, const BatchCore &C, const uint8_t *tables, fe *tmp, uint64_t tmp_elems,
+                          hipStream_t s1) {
+    int rc;
     for (const auto &G : C.groups) {
         const uint64_t chunk = tmp_elems >> G.logn;
         if (chunk == 0) return set_err(ctx, COSNARK_E_RANGE, "batch: NTT temporary smaller than one codeword");
@@ -162,13 +165,29 @@ int batch_core_enqueue(cosnark_ctx *ctx, const BatchCore &C, const uint8_t *tabl
             if (rc) return rc;
         }
     }
-    if ((rc = merkle_batch_run(ctx, C.mk, tables, s1))) return rc;
-    HIPCK(hipEventRecord(join, s2));
-    HIPCK(hipStreamWaitEvent(s1, join, 0));
+    return merkle_batch_run(ctx, C.mk, tables, s1);
+}
+
+int batch_enqueue_pack(cosnark_ctx *ctx, const BatchCore &C, const uint8_t *tables, uint8_t *out_host,
+                       hipStream_t s1) {
     hipLaunchKernelGGL(pack_kernel, dim3(C.count), dim3(PACK_THREADS), 0, s1, (const PackJob *)(tables + C.pack));
     LAUNCHCK();
     HIPCK(hipMemcpyAsync(out_host, C.out_dev, C.out_bytes, hipMemcpyDeviceToHost, s1));
     return COSNARK_OK;
+}
+
+int batch_core_enqueue(cosnark_ctx *ctx, const BatchCore &C, const uint8_t *tables, const fe *prev_host,
+                       uint8_t *out_host, fe *tmp, uint64_t tmp_elems, hipStream_t s1, hipStream_t s2,
+                       hipEvent_t fork, hipEvent_t join) {
+    // the commits (the longer branch) are issued first, so they start at once
+    HIPCK(hipEventRecord(fork, s1));
+    HIPCK(hipStreamWaitEvent(s2, fork, 0));
+    int rc = batch_enqueue_commits(ctx, C, tables, tmp, tmp_elems, s1);
+    if (rc) return rc;
+    if ((rc = batch_enqueue_sumchecks(ctx, C, tables, prev_host, s2))) return rc;
+    HIPCK(hipEventRecord(join, s2));
+    HIPCK(hipStreamWaitEvent(s1, join, 0));
+    return batch_enqueue_pack(ctx, C, tables, out_host, s1);
 }
 
 // proof i's bytes and transcript out of the copied records; the index of
@@ -198,8 +217,11 @@ struct cosnark_batch {
     uint64_t tmp_elems = 0;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_t0 = nullptr, ev_t1 = nullptr;
     bool use_graph = true;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    // two graphs, one per stream: replaying one graph with both branches ran
+    // them one after the other, so the commits and the sumchecks are
+    // captured apart and joined by events around the launches
+    hipGraph_t graph = nullptr, graph_sc = nullptr;
+    hipGraphExec_t exec = nullptr, exec_sc = nullptr;
 };
 
 namespace {
@@ -209,6 +231,8 @@ void batch_release(cosnark_batch *b) {
     if (b->ctx) (void)hipSetDevice(b->ctx->device);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->exec_sc) (void)hipGraphExecDestroy(b->exec_sc);
+    if (b->graph_sc) (void)hipGraphDestroy(b->graph_sc);
     for (void *p : {(void *)b->f, (void *)b->g, (void *)b->tmp, (void *)b->prev_dev, (void *)b->work,
                     (void *)b->tables, (void *)b->out_dev})
         if (p) (void)hipFree(p);
@@ -225,20 +249,39 @@ int batch_enqueue(cosnark_batch *b) {
                               ctx->stream2, b->ev_fork, b->ev_join);
 }
 
-int batch_capture(cosnark_batch *b) {
-    cosnark_ctx *ctx = b->ctx;
-    HIPCK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    const int rc = batch_enqueue(b);
+template <class Enqueue>
+int capture_one(cosnark_ctx *ctx, hipStream_t s, Enqueue enqueue, hipGraph_t *graph, hipGraphExec_t *exec) {
+    HIPCK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue();
     hipGraph_t g = nullptr;
-    const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    const hipError_t e = hipStreamEndCapture(s, &g);
     if (rc) {
         if (g) (void)hipGraphDestroy(g);
         return rc;
     }
     if (e != hipSuccess) return hip_err(ctx, e, "batch: graph capture");
-    b->graph = g;
-    HIPCK(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+    *graph = g;
+    HIPCK(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
     return COSNARK_OK;
+}
+
+int batch_capture(cosnark_batch *b) {
+    cosnark_ctx *ctx = b->ctx;
+    int rc = capture_one(
+        ctx, ctx->stream,
+        [&]() { return batch_enqueue_commits(ctx, b->core, b->tables, b->tmp, b->tmp_elems, ctx->stream); },
+        &b->graph, &b->exec);
+    if (rc) return rc;
+    rc = capture_one(
+        ctx, ctx->stream2, [&]() { return batch_enqueue_sumchecks(ctx, b->core, b->tables, b->prev_host, ctx->stream2); },
+        &b->graph_sc, &b->exec_sc);
+    if (rc) {   // both graphs or neither
+        (void)hipGraphExecDestroy(b->exec);
+        (void)hipGraphDestroy(b->graph);
+        b->exec = nullptr;
+        b->graph = nullptr;
+    }
+    return rc;
 }
 
 }  // namespace
@@ -343,7 +386,13 @@ int cosnark_batch_prove(cosnark_batch *b, cosnark_transcript *ts, uint8_t *h_pro
     if (b->use_graph && !b->exec && (rc = batch_capture(b))) return rc;
     HIPCK(hipEventRecord(b->ev_t0, ctx->stream));
     if (b->use_graph) {
+        HIPCK(hipEventRecord(b->ev_fork, ctx->stream));
+        HIPCK(hipStreamWaitEvent(ctx->stream2, b->ev_fork, 0));
         HIPCK(hipGraphLaunch(b->exec, ctx->stream));
+        HIPCK(hipGraphLaunch(b->exec_sc, ctx->stream2));
+        HIPCK(hipEventRecord(b->ev_join, ctx->stream2));
+        HIPCK(hipStreamWaitEvent(ctx->stream, b->ev_join, 0));
+        if ((rc = batch_enqueue_pack(ctx, b->core, b->tables, b->out_host, ctx->stream))) return rc;
     } else if ((rc = batch_enqueue(b))) {
         return rc;
     }
