@@ -255,12 +255,39 @@ __device__ __forceinline__ fe quad_eval(const fe &f0, const fe &f1, const fe &f2
     return gf::add(f0, gf::mul(r, gf::add(gf::sub(gf::sub(f1, f0), f2), gf::mul(r, f2))));
 }
 
-__device__ __forceinline__ void wave_sum9(fe *q) {
-#pragma unroll
-    for (int msk = 32; msk >= 1; msk >>= 1)
+// q summed over lanes [0, P) of the wave (P a power of two <= 64; lanes
+// without work hold zeros), every lane of the block ends with the sum
+__device__ __forceinline__ void wave_sum9(fe *q, int P) {
+    for (int msk = 1; msk < P; msk <<= 1)
 #pragma unroll
         for (int t = 0; t < 9; ++t) q[t] = gf::add(q[t], shfl_xor_fe(q[t], msk));
 }
+
+// The nine products of a next-round pair (quad_acc's terms) as
+// A_p(x) * A_p(z), A_p(v) = (v[a] - v[b]) - (v[c] - v[d]) over the pair's four
+// folded values; row holds x at 0..3, z at 4..7 and zero at 8, and off
+// selects x (0) or z (4).  Index 4 in the table stands for zero.
+__device__ __forceinline__ fe quad_operand(const fe *row, int off, int p) {
+    constexpr int8_t T[9][4] = {{2, 0, 4, 4}, {3, 1, 4, 4}, {3, 2, 1, 0}, {0, 4, 4, 4}, {1, 4, 4, 4},
+                                {1, 0, 4, 4}, {2, 4, 4, 4}, {3, 4, 4, 4}, {3, 2, 4, 4}};
+    int ix[4] = {4, 4, 4, 4};
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+        if (p == k) { ix[0] = T[k][0]; ix[1] = T[k][1]; ix[2] = T[k][2]; ix[3] = T[k][3]; }
+    fe v[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) v[j] = row[ix[j] == 4 ? 8 : off + ix[j]];
+    return gf::sub(gf::sub(v[0], v[1]), gf::sub(v[2], v[3]));
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr int SC_SPLIT_MAX = 7;    // groups whose 8 folds and 9 products fit one wave's lanes
+constexpr int SC_SPLIT_NG = 64;    // phases with at most this many groups spread them over lanes
 
 // All rounds from `round0` to m-1 in one workgroup of four waves.  The input
 // (len elements) is v(round0) when round0 == 0, else v(round0 - 1), still to
@@ -271,9 +298,19 @@ __device__ __forceinline__ void wave_sum9(fe *q) {
 // i + 1's sums as quadratics in r[i]; one barrier ends each phase.  Vectors
 // move through global memory (L2), so the workgroup needs no LDS beyond the
 // exchange slots and fits on a CU beside other kernels' workgroups.
+//
+// A lane issues 64-bit multiplies slowly when it is alone on its SIMD, so
+// the compute waves keep per-lane chains short: with many groups (8
+// elements folded into one next-round pair) each lane takes whole groups and
+// only the lanes and waves that hold groups join the reductions; with at
+// most SC_SPLIT_NG groups each wave spreads its share over its lanes, one
+// fold and then one product per lane, exchanged through LDS.
 __device__ __forceinline__ void sc_tail(const ScJob &J) {
     __shared__ fe qpart[2][SC_TAIL_WAVES][9];
+    __shared__ int sh_nw[2];                      // compute waves that wrote qpart[s]
     __shared__ fe sh_r[2];
+    __shared__ fe lfold[SC_TAIL_WAVES][SC_SPLIT_MAX][9];   // per wave: folded x (0..3), z (4..7), zero (8)
+    __shared__ fe lprod[SC_TAIL_WAVES][SC_SPLIT_MAX][9];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const bool hash_wave = wave == 0;
     const int ct = tid - 64;
@@ -287,6 +324,20 @@ __device__ __forceinline__ void sc_tail(const ScJob &J) {
     uint32_t cur = (uint32_t)J.npairs;      // elements in c1 / c2
     fe prev = gf::zero(), y = gf::zero(), rr = gf::zero();
     int claim_mode = 0;
+    // per-thread accumulation, then the sum over the lanes and waves that hold
+    // work items (ng of them, item g on compute thread g mod SC_TAIL_CT)
+    auto publish = [&](fe *q, uint32_t ng, int slot) {
+        const int active = (int)min<uint32_t>(64u, (uint32_t)max<int>(0, (int)ng - 64 * (wave - 1)));
+        if (active > 0) {
+            int P = 1;
+            while (P < active) P <<= 1;
+            wave_sum9(q, P);
+            if (lane == 0)
+#pragma unroll
+                for (int t = 0; t < 9; ++t) qpart[slot][wave - 1][t] = q[t];
+        }
+        if (tid == 64) sh_nw[slot] = (int)min<uint32_t>((uint32_t)SC_TAIL_WAVES, (ng + 63) / 64);
+    };
 
     // ---- before phase 0 ----
     if (hash_wave) {
@@ -313,9 +364,11 @@ __device__ __forceinline__ void sc_tail(const ScJob &J) {
         fe q[9];
 #pragma unroll
         for (int t = 0; t < 9; ++t) q[t] = gf::zero();
+        uint32_t ng;
         if (pending) {
             // round round0's sums in r[round0 - 1], from groups of 4 of the input
-            for (uint32_t g = ct; g < cur / 4; g += SC_TAIL_CT) {
+            ng = cur / 4;
+            for (uint32_t g = ct; g < ng; g += SC_TAIL_CT) {
                 fe x[4], z[4];
 #pragma unroll
                 for (int e = 0; e < 4; ++e) { x[e] = c1[4 * g + e]; z[e] = c2[4 * g + e]; }
@@ -323,17 +376,15 @@ __device__ __forceinline__ void sc_tail(const ScJob &J) {
             }
         } else {
             // round 0's sums directly (constant terms)
-            for (uint32_t j = ct; j < cur / 2; j += SC_TAIL_CT) {
+            ng = cur / 2;
+            for (uint32_t j = ct; j < ng; j += SC_TAIL_CT) {
                 const fe e1 = c1[2 * j], od1 = c1[2 * j + 1], e2 = c2[2 * j], od2 = c2[2 * j + 1];
                 q[0] = gf::add(q[0], gf::mul(gf::sub(od1, e1), gf::sub(od2, e2)));
                 q[3] = gf::add(q[3], gf::mul(e1, e2));
                 q[6] = gf::add(q[6], gf::mul(od1, od2));
             }
         }
-        wave_sum9(q);
-        if (lane == 0)
-#pragma unroll
-            for (int t = 0; t < 9; ++t) qpart[0][wave - 1][t] = q[t];
+        publish(q, ng, 0);
     }
     __syncthreads();
 
@@ -344,12 +395,12 @@ __device__ __forceinline__ void sc_tail(const ScJob &J) {
         if (hash_wave) {
             if (lane < 4) {
                 __builtin_amdgcn_s_setprio(3);
+                const int nw = sh_nw[s];
                 fe f[9];
 #pragma unroll
                 for (int t = 0; t < 9; ++t) {
                     f[t] = qpart[s][0][t];
-#pragma unroll
-                    for (int w = 1; w < SC_TAIL_WAVES; ++w) f[t] = gf::add(f[t], qpart[s][w][t]);
+                    for (int w = 1; w < nw; ++w) f[t] = gf::add(f[t], qpart[s][w][t]);
                 }
                 fe A = f[0], C = f[3], O = f[6];
                 if (fold) {
@@ -363,41 +414,73 @@ __device__ __forceinline__ void sc_tail(const ScJob &J) {
             }
         } else {
             const bool next = i + 1 < m;       // round i + 1 exists: accumulate its sums
-            fe q[9];
-#pragma unroll
-            for (int t = 0; t < 9; ++t) q[t] = gf::zero();
-            if (fold) {
-                // v(i) = fold(v(i - 1), r[i - 1]); groups of 8 -> 4 (the last phase: 4 -> 2)
+            fe *w1 = o1[ob], *w2 = o2[ob];
+            if (fold && next && cur / 8 <= (uint32_t)SC_SPLIT_NG) {
+                // few groups: compute wave cw takes groups cw, cw + 3, ..., seven at a
+                // time: one fold per lane, then one of the nine products per lane,
+                // through the wave's own LDS rows; lanes 0..8 keep the sums
+                const uint32_t ng = cur / 8;
+                const int cw = wave - 1;
+                const uint32_t mine = ng > (uint32_t)cw ? (ng - cw + 2) / 3 : 0;
                 const fe r = sh_r[s];
-                fe *w1 = o1[ob], *w2 = o2[ob];
-                const uint32_t gs = cur >= 8 ? 8 : cur;
-                for (uint32_t g = ct; g < cur / gs; g += SC_TAIL_CT) {
-                    fe x[4], z[4];
-#pragma unroll
-                    for (int e = 0; e < 4; ++e) {
-                        if (2 * e < (int)gs) {
-                            x[e] = fold1(c1[gs * g + 2 * e], c1[gs * g + 2 * e + 1], r);
-                            z[e] = fold1(c2[gs * g + 2 * e], c2[gs * g + 2 * e + 1], r);
-                            w1[(gs / 2) * g + e] = x[e];
-                            w2[(gs / 2) * g + e] = z[e];
-                        }
+                fe acc = gf::zero();
+                for (uint32_t j0 = 0; j0 < mine; j0 += SC_SPLIT_MAX) {
+                    const int nc = (int)min<uint32_t>((uint32_t)SC_SPLIT_MAX, mine - j0);
+                    if (lane < 8 * nc) {
+                        const int gi = lane >> 3, f = lane & 7, e = f & 3;
+                        const uint32_t g = cw + 3 * (j0 + gi);
+                        const fe *src = f < 4 ? c1 : c2;
+                        const fe x = fold1(src[8 * g + 2 * e], src[8 * g + 2 * e + 1], r);
+                        (f < 4 ? w1 : w2)[4 * g + e] = x;
+                        lfold[cw][gi][f] = x;
                     }
-                    if (next) quad_acc(q, x, z);
+                    if (lane < nc) lfold[cw][lane][8] = gf::zero();
+                    wave_lds_sync();
+                    if (lane < 9 * nc) {
+                        const int gi = lane / 9, p = lane % 9;
+                        lprod[cw][gi][p] = gf::mul(quad_operand(lfold[cw][gi], 0, p), quad_operand(lfold[cw][gi], 4, p));
+                    }
+                    wave_lds_sync();
+                    if (lane < 9)
+                        for (int gi = 0; gi < nc; ++gi) acc = gf::add(acc, lprod[cw][gi][lane]);
+                    wave_lds_sync();     // the rows are rewritten by the next chunk
                 }
-            } else if (next) {
-                // round 0 was direct: round 1's sums from groups of 4 of v(0)
-                for (uint32_t g = ct; g < cur / 4; g += SC_TAIL_CT) {
-                    fe x[4], z[4];
+                if (mine > 0 && lane < 9) qpart[s ^ 1][cw][lane] = acc;
+                if (tid == 64) sh_nw[s ^ 1] = (int)min<uint32_t>((uint32_t)SC_TAIL_WAVES, ng);
+            } else {
+                fe q[9];
 #pragma unroll
-                    for (int e = 0; e < 4; ++e) { x[e] = c1[4 * g + e]; z[e] = c2[4 * g + e]; }
-                    quad_acc(q, x, z);
+                for (int t = 0; t < 9; ++t) q[t] = gf::zero();
+                uint32_t ng = 0;
+                if (fold) {
+                    // v(i) = fold(v(i - 1), r[i - 1]); groups of 8 -> 4 (the last phase: 4 -> 2)
+                    const fe r = sh_r[s];
+                    const uint32_t gs = cur >= 8 ? 8 : cur;
+                    ng = cur / gs;
+                    for (uint32_t g = ct; g < ng; g += SC_TAIL_CT) {
+                        fe x[4], z[4];
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) {
+                            if (2 * e < (int)gs) {
+                                x[e] = fold1(c1[gs * g + 2 * e], c1[gs * g + 2 * e + 1], r);
+                                z[e] = fold1(c2[gs * g + 2 * e], c2[gs * g + 2 * e + 1], r);
+                                w1[(gs / 2) * g + e] = x[e];
+                                w2[(gs / 2) * g + e] = z[e];
+                            }
+                        }
+                        if (next) quad_acc(q, x, z);
+                    }
+                } else if (next) {
+                    // round 0 was direct: round 1's sums from groups of 4 of v(0)
+                    ng = cur / 4;
+                    for (uint32_t g = ct; g < ng; g += SC_TAIL_CT) {
+                        fe x[4], z[4];
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) { x[e] = c1[4 * g + e]; z[e] = c2[4 * g + e]; }
+                        quad_acc(q, x, z);
+                    }
                 }
-            }
-            if (next) {
-                wave_sum9(q);
-                if (lane == 0)
-#pragma unroll
-                    for (int t = 0; t < 9; ++t) qpart[s ^ 1][wave - 1][t] = q[t];
+                if (next) publish(q, ng, s ^ 1);
             }
         }
         if (fold) {
