@@ -130,22 +130,31 @@ __global__ __launch_bounds__(WT) void betas_kernel(fe *b, uint64_t nb, const uin
     __syncthreads();
     const uint64_t j0 = ((uint64_t)blockIdx.x * WT + threadIdx.x) * BETA_J;
     if (j0 >= nb) return;
+    // weak arithmetic (gf.h): the running power stays folded once (<= p + 7,
+    // the canonical w first in each product) and the sums are left unreduced
+    // for up to four queries (5 (p + 7) < 2^64), folded in between and made
+    // canonical at the store
     fe acc[BETA_J];
 #pragma unroll
     for (int t = 0; t < BETA_J; ++t) acc[t] = (j0 + t < nb) ? b[j0 + t] : gf::zero();
     for (int i = 0; i < l; ++i) {
         const uint64_t e = ((uint64_t)qq[i] * j0) & Nmask;
-        fe p = gf::mul(cq[i], tw28d(twlo, twhi, (uint32_t)(e << sh)));
+        fe p = gf::mul_w(cq[i], tw28d(twlo, twhi, (uint32_t)(e << sh)));
         const fe w = wq[i];
 #pragma unroll
         for (int t = 0; t < BETA_J; ++t) {
-            acc[t] = gf::add(acc[t], p);
-            if (t + 1 < BETA_J) p = gf::mul(p, w);
+            acc[t].re += p.re;
+            acc[t].im += p.im;
+            if (t + 1 < BETA_J) p = gf::mul_w(w, p);
+        }
+        if ((i & 3) == 3) {
+#pragma unroll
+            for (int t = 0; t < BETA_J; ++t) acc[t] = gf::mk(gf::fold(acc[t].re), gf::fold(acc[t].im));
         }
     }
 #pragma unroll
     for (int t = 0; t < BETA_J; ++t)
-        if (j0 + t < nb) b[j0 + t] = acc[t];
+        if (j0 + t < nb) b[j0 + t] = gf::canon_w(gf::mk(gf::fold(acc[t].re), gf::fold(acc[t].im)));
 }
 
 // ---- host helpers ----------------------------------------------------
