@@ -236,6 +236,89 @@ __global__ __launch_bounds__(CO_THREADS) void bsc_large_kernel(const fe *__restr
     }
 }
 
+// d = 256 with the transforms in registers: the four-step 16 x 16 (natural
+// order in and out, as the NTT passes) on weak values (gf.h, dft.h).  A
+// workgroup takes 8 blocks of both vectors per tile; thread (a, b, col)
+// holds one 16-element column of vector a's block b.  Per block: the
+// inverse transform (unscaled), the twist of coefficient k by w_512^k, the
+// forward transform, so M[k] = d P(w_512 w_256^k) as bsc_large_kernel
+// forms it; thread t then adds point t's products over the 8 blocks (even
+// points from the block values, odd points from the transforms).  Eight
+// barriers per tile of 8 blocks instead of 16 per block, every thread busy.
+constexpr int L256_B = 8, L256_RS = 17;        // blocks per tile, padded row (elements)
+constexpr int L256_LDS = 2 * L256_B * 16 * L256_RS + 2 * 256;   // elements: matrices, w_256 and w_512 tables
+
+template <bool INV>
+__device__ __forceinline__ void four_step256(fe *Mb, int col, const fe *wd, const fe *wt) {
+    fe v[16];
+    // step 1: column n2 = col, 16-point DFTs over n1, twiddle w_256^(n2 k1)
+#pragma unroll
+    for (int n1 = 0; n1 < 16; ++n1) v[n1] = Mb[n1 * L256_RS + col];
+    __syncthreads();
+    dft::dif_w<16, INV>(v);
+    dft::static_for<0, 16>([&](auto ii) {
+        constexpr int i = decltype(ii)::value;
+        constexpr int k1 = dft::brev_c<16>(i);
+        fe x = v[i];
+        if (k1 != 0 && col != 0) {
+            const int e = col * k1;
+            x = gf::mul_w(wd[INV ? (256 - e) & 255 : e], x);
+        }
+        Mb[k1 * L256_RS + col] = x;             // Y[n2][k1] at row k1, column n2
+    });
+    __syncthreads();
+    // step 2: row k1 = col, 16-point DFTs over n2; X[k1 + 16 k2] to natural order
+#pragma unroll
+    for (int n2 = 0; n2 < 16; ++n2) v[n2] = Mb[col * L256_RS + n2];
+    __syncthreads();
+    dft::dif_w<16, INV>(v);
+    dft::static_for<0, 16>([&](auto ii) {
+        constexpr int i = decltype(ii)::value;
+        constexpr int k2 = dft::brev_c<16>(i);
+        fe x = v[i];
+        if (INV) x = gf::mul_w(wt[col + 16 * k2], x);     // twist coefficient k by w_512^k
+        Mb[k2 * L256_RS + col] = x;             // natural k = col + 16 k2 at row k2, column col
+    });
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(CO_THREADS) void bsc_large256_kernel(const fe *__restrict__ p1,
+                                                                  const fe *__restrict__ p2, uint64_t nblocks,
+                                                                  const fe *twlo, const fe *twhi,
+                                                                  fe *__restrict__ part) {
+    extern __shared__ fe S[];
+    fe *wd = S + 2 * L256_B * 16 * L256_RS, *wt = wd + 256;
+    const int t = threadIdx.x;
+    wd[t] = dft::tw28(twlo, twhi, (uint32_t)t << (NTT_LOG_MAX - 8));     // w_256^t
+    wt[t] = dft::tw28(twlo, twhi, (uint32_t)t << (NTT_LOG_MAX - 9));     // w_512^t
+    const int a = t >> 7, bl = (t >> 4) & (L256_B - 1), col = t & 15;
+    fe *Mb = S + (a * L256_B + bl) * 16 * L256_RS;
+    const fe *src = a ? p2 : p1;
+    const int pr = (t >> 4) * L256_RS + (t & 15);   // point t of a block matrix
+    fe acc0 = gf::zero(), acc1 = gf::zero();
+    const uint64_t ntiles = (nblocks + L256_B - 1) / L256_B;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const uint64_t blk = tile * L256_B + bl;
+#pragma unroll
+        for (int n1 = 0; n1 < 16; ++n1)
+            Mb[n1 * L256_RS + col] = blk < nblocks ? src[blk * 256 + n1 * 16 + col] : gf::zero();
+        __syncthreads();
+        // even points: the block values themselves
+#pragma unroll
+        for (int b = 0; b < L256_B; ++b)
+            acc0 = gf::add(acc0, gf::mul(S[b * 16 * L256_RS + pr], S[(L256_B + b) * 16 * L256_RS + pr]));
+        four_step256<true>(Mb, col, wd, wt);
+        four_step256<false>(Mb, col, wd, wt);
+        // odd points: the coset values (weak; the first factor made canonical)
+#pragma unroll
+        for (int b = 0; b < L256_B; ++b)
+            acc1 = gf::add(acc1, gf::mul(gf::canon_w(S[b * 16 * L256_RS + pr]), S[(L256_B + b) * 16 * L256_RS + pr]));
+        __syncthreads();
+    }
+    part[(uint64_t)blockIdx.x * 512 + t] = acc0;
+    part[(uint64_t)blockIdx.x * 512 + 256 + t] = acc1;
+}
+
 // column sums over the workgroups; 16 columns per workgroup, 16 lanes each.
 // Column c = co * d + i is the point w^(ncos i + co) (w of order ncos * d);
 // columns with co > 0 are multiplied by `scale`.
@@ -702,6 +785,8 @@ int copcs_init(cosnark_ctx *) {
                             small_lds) != hipSuccess ||
         hipFuncSetAttribute((const void *)bsc_large_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                             large_lds) != hipSuccess ||
+        hipFuncSetAttribute((const void *)bsc_large256_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)(sizeof(fe) * L256_LDS)) != hipSuccess ||
         hipFuncSetAttribute((const void *)multifold_large_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                             mf_lds) != hipSuccess ||
         hipFuncSetAttribute((const void *)bsc3_small_kernel<16>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -841,6 +926,9 @@ int cosnark_cosumcheck_phase1(cosnark_ctx *ctx, const uint64_t *d_p1, const uint
     if (d <= BSC_SMALL_MAX) {
         const uint64_t tiles = (nblocks + BSC_TB - 1) / BSC_TB;
         nwg = (int)(tiles < BSC_MAX_WG ? tiles : BSC_MAX_WG);
+    } else if (d == 256) {
+        const uint64_t tiles = (nblocks + L256_B - 1) / L256_B;
+        nwg = (int)(tiles < BSC_MAX_WG ? tiles : BSC_MAX_WG);
     } else {
         nwg = (int)(nblocks < BSC_MAX_WG ? nblocks : BSC_MAX_WG);
     }
@@ -856,6 +944,10 @@ int cosnark_cosumcheck_phase1(cosnark_ctx *ctx, const uint64_t *d_p1, const uint
         case 8: launch_bsc_small<8>(a, b, nblocks, part, nwg, s); break;
         case 16: launch_bsc_small<16>(a, b, nblocks, part, nwg, s); break;
         case 32: launch_bsc_small<32>(a, b, nblocks, part, nwg, s); break;
+        case 256:
+            hipLaunchKernelGGL(bsc_large256_kernel, dim3(nwg), dim3(CO_THREADS), sizeof(fe) * L256_LDS, s, a, b,
+                               nblocks, (const fe *)ctx->tw_lo, (const fe *)ctx->tw_hi, part);
+            break;
         default:
             hipLaunchKernelGGL(bsc_large_kernel, dim3(nwg), dim3(CO_THREADS), sizeof(fe) * 2 * d, s, a, b,
                                nblocks, ilog2((uint64_t)d), (const fe *)ctx->tw_lo, (const fe *)ctx->tw_hi,

@@ -70,7 +70,7 @@ def test_pcs_commit_at_share_shape(ctx):
 
 
 @pytest.mark.parametrize("logM,d", [(1, 2), (6, 2), (10, 4), (18, 8), (16, 32), (5, 32), (12, 16), (22, 2),
-                                    (12, 64), (14, 128), (13, 256), (11, 2048), (16, 1024)])
+                                    (12, 64), (14, 128), (13, 256), (8, 256), (18, 256), (11, 2048), (16, 1024)])
 def test_phase1_matches_oracle(ctx, logM, d):
     print(f"cosumcheck phase1 logM={logM} d={d}", flush=True)
     M = 1 << logM
