@@ -496,21 +496,22 @@ __global__ __launch_bounds__(CO_THREADS) void bsc3_small_kernel(const fe *__rest
         __syncthreads();
         products(acc[0]);
         __syncthreads();
-        dft::dif<D, true>(x);                                   // x[i] = D c[brev(i)]
+        // weak arithmetic (gf.h) through the transforms; canonical values in LDS
+        dft::dif_w<D, true>(x);                                 // x[i] = D c[brev(i)]
 #pragma unroll
-        for (int i = 0; i < D; ++i) x[i] = gf::mul(x[i], inv_d);
+        for (int i = 0; i < D; ++i) x[i] = gf::mul_w(inv_d, x[i]);
         dft::static_for<1, 4>([&](auto cc) {
             constexpr int co = decltype(cc)::value;
             fe y[D];
             dft::static_for<0, D>([&](auto ii) {
                 constexpr int i = decltype(ii)::value;
                 constexpr int m = dft::brev_c<D>(i);
-                y[m] = gf::mul_root<4 * D, co * m>(x[i]);
+                y[m] = gf::mul_root_w<4 * D, co * m>(x[i]);
             });
-            dft::dif<D, false>(y);                              // y[i] = P(w_4d^(4 brev(i) + co))
+            dft::dif_w<D, false>(y);                            // y[i] = P(w_4d^(4 brev(i) + co))
             dft::static_for<0, D>([&](auto ii) {
                 constexpr int i = decltype(ii)::value;
-                row[dft::brev_c<D>(i)] = y[i];
+                row[dft::brev_c<D>(i)] = gf::canon_w(y[i]);
             });
             __syncthreads();
             products(acc[co]);
