@@ -446,6 +446,147 @@ __global__ __launch_bounds__(CO_THREADS) void bsc3_kernel(const fe *__restrict__
     }
 }
 
+// Cubic block sum for d = 512 with the transforms in registers.  A
+// 512-point DFT is the four-step 8 x 64 with the 64-point part as 8 x 8
+// (natural order in and out, radix-8 register DFTs on weak values, twiddles
+// w_512^e from an LDS table).  A workgroup takes one block of the four
+// vectors per iteration, 64 threads per vector; thread q keeps the
+// coefficients c[q + 64 n] (n < 8) of its vector in registers after the
+// inverse transform (its last pass leaves exactly those), so each coset
+// s = 1, 2, 3 is a twist by w_2048^(s m) / d and a forward transform from
+// registers.  LDS carries the exchanges between passes and the four
+// vectors' values at each point for the products.  Blocks whose p1 part is
+// zero are skipped, as the reference skips them (src/coSumcheck.cpp:22-28).
+constexpr int C512_RS = 65;                     // padded row of the 8 x 64 exchange
+constexpr int C512_VEC = 8 * C512_RS;           // elements per vector in LDS (>= 512)
+constexpr int C512_LDS = 4 * C512_VEC + 512 + 3 * 512;
+
+// One 512-point transform of the calling thread's column q (64 threads per
+// vector, L = that vector's LDS area): x[n1] = x_in[q + 64 n1] on entry;
+// on return thread q holds X[q + 64 j] in out[j] (j < 8).  Begins by
+// writing L, so L's previous readers must have passed a barrier; every
+// thread of the workgroup calls it.
+template <bool INV>
+__device__ __forceinline__ void t512(fe *x, fe *out, fe *L, int q, const fe *w512) {
+    // pass A: 8 points over n1 for column n2 = q, twiddle w_512^(q k1)
+    dft::dif_w<8, INV>(x);
+    dft::static_for<0, 8>([&](auto ii) {
+        constexpr int i = decltype(ii)::value;
+        constexpr int k1 = dft::brev_c<8>(i);
+        fe y = x[i];
+        if (k1 != 0 && q != 0) {
+            const int e = q * k1;
+            y = gf::mul_w(w512[INV ? (512 - e) & 511 : e], y);
+        }
+        L[k1 * C512_RS + q] = y;
+    });
+    __syncthreads();
+    // pass B: thread (k1, m2): 8 points over m1 of Z[8 m1 + m2], twiddle w_64^(m2 j1), in place
+    {
+        const int k1 = q >> 3, m2 = q & 7;
+        fe v[8];
+#pragma unroll
+        for (int m1 = 0; m1 < 8; ++m1) v[m1] = L[k1 * C512_RS + 8 * m1 + m2];
+        dft::dif_w<8, INV>(v);
+        dft::static_for<0, 8>([&](auto ii) {
+            constexpr int i = decltype(ii)::value;
+            constexpr int j1 = dft::brev_c<8>(i);
+            fe y = v[i];
+            if (j1 != 0 && m2 != 0) {
+                const int e = 8 * m2 * j1;
+                y = gf::mul_w(w512[INV ? (512 - e) & 511 : e], y);
+            }
+            L[k1 * C512_RS + 8 * j1 + m2] = y;
+        });
+    }
+    __syncthreads();
+    // pass C: thread (k1, j1), q = k1 + 8 j1: 8 points over m2; X[q + 64 j2]
+    {
+        const int k1 = q & 7, j1 = q >> 3;
+        fe v[8];
+#pragma unroll
+        for (int m2 = 0; m2 < 8; ++m2) v[m2] = L[k1 * C512_RS + 8 * j1 + m2];
+        dft::dif_w<8, INV>(v);
+        dft::static_for<0, 8>([&](auto ii) {
+            constexpr int i = decltype(ii)::value;
+            out[dft::brev_c<8>(i)] = v[i];
+        });
+    }
+}
+
+__global__ __launch_bounds__(CO_THREADS) void bsc3_512_kernel(const fe *__restrict__ p1, const fe *__restrict__ p2,
+                                                              const fe *__restrict__ p3, const fe *__restrict__ b1,
+                                                              uint64_t nblocks, const fe *twlo, const fe *twhi,
+                                                              fe inv_d, fe *__restrict__ part) {
+    extern __shared__ fe S[];
+    __shared__ int live;
+    fe *w512 = S + 4 * C512_VEC, *tw = w512 + 512;
+    const int t = threadIdx.x, vec = t >> 6, q = t & 63;
+    fe *L = S + vec * C512_VEC;
+    const fe *src = vec == 0 ? p1 : vec == 1 ? p2 : vec == 2 ? p3 : b1;
+    for (int e = t; e < 512; e += CO_THREADS) {
+        w512[e] = dft::tw28(twlo, twhi, (uint32_t)e << (NTT_LOG_MAX - 9));
+#pragma unroll
+        for (int co = 1; co < 4; ++co)      // w_2048^(co e) / d
+            tw[(co - 1) * 512 + e] = gf::mul(inv_d, dft::tw28(twlo, twhi, (uint32_t)(co * e) << (NTT_LOG_MAX - 11)));
+    }
+    fe acc[4][2];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[c][0] = acc[c][1] = gf::zero();
+    // (P1 P2 - P3) B1 at points t and t + 256 from the natural-order rows
+    // (every accumulator takes y or zero, so acc stays in registers)
+    auto products = [&](int co) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int e = t + 256 * j;
+            const fe a = gf::canon_w(S[e]), b = S[C512_VEC + e];
+            const fe x = gf::sub(gf::mul(a, b), gf::canon_w(S[2 * C512_VEC + e]));
+            const fe y = gf::mul(x, S[3 * C512_VEC + e]);
+            dft::static_for<0, 4>([&](auto cc) {
+                constexpr int c = decltype(cc)::value;
+                acc[c][j] = gf::add(acc[c][j], co == c ? y : gf::zero());
+            });
+        }
+    };
+    for (uint64_t blk = blockIdx.x; blk < nblocks; blk += gridDim.x) {
+        fe x[8];
+        int nz = 0;
+#pragma unroll
+        for (int n = 0; n < 8; ++n) {
+            x[n] = src[blk * 512 + q + 64 * n];
+            nz |= (x[n].re | x[n].im) != 0;
+            L[q + 64 * n] = x[n];
+        }
+        if (t == 0) live = 0;
+        __syncthreads();
+        if (vec == 0 && nz) live = 1;
+        __syncthreads();
+        if (!live) continue;                    // uniform: the block's p1 part is zero
+        products(0);                            // coset 0: the block's own values
+        __syncthreads();
+        fe c[8];
+        t512<true>(x, c, L, q, w512);           // c[n] = d * coefficient (q + 64 n)
+#pragma unroll 1
+        for (int co = 1; co < 4; ++co) {
+            fe y[8], z[8];
+#pragma unroll
+            for (int n = 0; n < 8; ++n) y[n] = gf::mul_w(tw[(co - 1) * 512 + q + 64 * n], c[n]);
+            __syncthreads();                    // the previous readers of L are done
+            t512<false>(y, z, L, q, w512);      // z[j] = P(w_2048^(co + 4 (q + 64 j)))
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < 8; ++j) L[q + 64 * j] = z[j];
+            __syncthreads();
+            products(co);
+        }
+        __syncthreads();                        // before the next block's loads
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) part[(uint64_t)blockIdx.x * 2048 + c * 512 + t + 256 * j] = acc[c][j];
+}
+
 // Cubic block sum for d <= 16, transforms in registers: a workgroup takes
 // 64 blocks per tile; wave v (0..3: p1, p2, p3, b1) holds vector v of the
 // tile's blocks, one block per lane.  A lane takes its block's inverse
@@ -792,6 +933,8 @@ int copcs_init(cosnark_ctx *) {
                             mf_lds) != hipSuccess ||
         hipFuncSetAttribute((const void *)bsc3_small_kernel<16>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)(sizeof(fe) * 4 * 64 * 17)) != hipSuccess ||
+        hipFuncSetAttribute((const void *)bsc3_512_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)(sizeof(fe) * C512_LDS)) != hipSuccess ||
         hipFuncSetAttribute((const void *)bsc3_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)(sizeof(fe) * 5 * BSC3_D_MAX + sizeof(int) * (1 << BSC3_TILE_LOG))) != hipSuccess)
         return COSNARK_E_HIP;
@@ -1009,6 +1152,9 @@ int cosnark_cosumcheck3_phase1(cosnark_ctx *ctx, const uint64_t *d_p1, const uin
             case 8: hipLaunchKernelGGL(bsc3_small_kernel<8>, dim3(nwg), dim3(CO_THREADS), lds, s, a1, a2, a3, a4, nb, di, part); break;
             default: hipLaunchKernelGGL(bsc3_small_kernel<16>, dim3(nwg), dim3(CO_THREADS), lds, s, a1, a2, a3, a4, nb, di, part); break;
         }
+    } else if (d == 512) {
+        hipLaunchKernelGGL(bsc3_512_kernel, dim3(nwg), dim3(CO_THREADS), sizeof(fe) * C512_LDS, s, a1, a2, a3, a4,
+                           M / 512, (const fe *)ctx->tw_lo, (const fe *)ctx->tw_hi, di, part);
     } else {
         const size_t lds = sizeof(fe) * 5 * ((size_t)1 << logT) + sizeof(int) * ((size_t)1 << (logT - logd));
         hipLaunchKernelGGL(bsc3_kernel, dim3(nwg), dim3(CO_THREADS), lds, s, a1, a2, a3, a4, M, logd, logT,

@@ -23,7 +23,7 @@ def _vecs(seed, M, d, zero_blocks=()):
 
 @pytest.mark.parametrize("logM,d,zb", [(1, 2, ()), (2, 2, (0,)), (6, 2, (3,)), (10, 4, (0, 7)), (12, 8, ()),
                                        (14, 16, (5,)), (12, 32, ()), (5, 32, (0,)), (13, 64, (2,)), (14, 128, ()),
-                                       (15, 256, (1,)), (16, 512, (0, 3)), (9, 512, ()), (10, 1024, ()),
+                                       (15, 256, (1,)), (16, 512, (0, 3)), (9, 512, ()), (20, 512, (1, 1025, 2047)), (10, 1024, ()),
                                        (18, 1024, (100,)), (18, 16, ()), (5, 16, (1,)), (22, 16, (9,))])
 def test_cubic_phase1_matches_oracle(ctx, logM, d, zb):
     print(f"cosumcheck3 phase1 logM={logM} d={d} zero_blocks={zb}", flush=True)
