@@ -157,6 +157,7 @@ int cosnark_create(int device, cosnark_ctx **out) {
     if (sumcheck_init(ctx) != COSNARK_OK) return fail(COSNARK_E_HIP);
     if (ntt_init(ctx) != COSNARK_OK) return fail(COSNARK_E_HIP);
     if (copcs_init(ctx) != COSNARK_OK) return fail(COSNARK_E_HIP);
+    if (whir_init(ctx) != COSNARK_OK) return fail(COSNARK_E_HIP);
     if (hipDeviceSynchronize() != hipSuccess) return fail(COSNARK_E_HIP);
     *out = ctx;
     return COSNARK_OK;

@@ -149,6 +149,7 @@ int merkle_batch_plan(cosnark_ctx *ctx, int count, const gf::fe *const *elems, c
 int merkle_batch_run(cosnark_ctx *ctx, const MerkleBatchPlan &plan, const uint8_t *tables, hipStream_t s);
 int sumcheck_init(cosnark_ctx *ctx);
 int copcs_init(cosnark_ctx *ctx);
+int whir_init(cosnark_ctx *ctx);
 // Batches of all-Fiat-Shamir product sumchecks (the plain prover's): proof
 // i folds v1[i], v2[i] (n[i] elements) in buf[i] (1.5 n[i] elements) with
 // partials at parts[i] (6 * SC_MAX_BLOCKS elements) and state st[i], whose

@@ -22,8 +22,11 @@
 #include <stdlib.h>
 #include <string.h>
 #include <vector>
+#include <algorithm>
+#include <utility>
 #include "ctx.h"
 #include "blake3.h"
+#include "dft.h"
 
 using gf::fe;
 
@@ -31,8 +34,21 @@ namespace {
 
 constexpr int WK = 4;           // whir_k
 constexpr int WT = 256;         // threads per workgroup
-constexpr int BETA_J = 16;      // b entries per thread in update_betas
 constexpr int WQ_MAX = 256;     // queries per iteration (rate 2 gives 200)
+
+// update_betas in slot form: b[j + J k] for k < BS_K, J = nb / BS_K, is a
+// sum over the D = BS_K N / nb residues u = q' mod D; see betas_slots_kernel
+constexpr int BS_K = 16;                        // outputs per thread
+constexpr int BS_GMAX = 16;                     // residue classes u mod (D / BS_K) taken
+constexpr uint64_t BS_MIN_NB = 1ull << 16;      // smaller b: the direct form
+struct BetaSlots {
+    int ng;                                     // non-empty classes
+    int logD;
+    int u2[BS_GMAX];                            // class g: u = (D / BS_K) u1 + u2[g]
+    // 32-bit fields: the kernel reads them at uniform indices with scalar loads
+    int off[BS_GMAX][BS_K + 1];                 // sorted queries of slot (g, u1): [off[g][u1], off[g][u1 + 1])
+    int slot[WQ_MAX];                           // u1 of sorted query r
+};
 
 __device__ __forceinline__ fe tw28d(const fe *lo, const fe *hi, uint32_t e) {
     return gf::mul(lo[e & (NTT_TW_SIZE - 1)], hi[e >> NTT_TW_BITS]);
@@ -110,9 +126,11 @@ __global__ __launch_bounds__(WT) void axpy_kernel(fe *dst, const fe *__restrict_
 }
 
 // update_betas: b[j] += sum_i c_i w^(q_i' j), w = root of order N = 2^logN.
-// Each thread owns BETA_J consecutive j; per query it starts from
+// Direct form: each thread owns BJ consecutive j; per query it starts from
 // c_i w^(q_i' j0) (exponent reduced mod N, two-table twiddle) and steps by
-// w^(q_i').
+// w^(q_i').  BJ = 16 where b is large; smaller b takes BJ = 4 or 1 so the
+// grid stays wide and the per-query chains short.
+template <int BJ>
 __global__ __launch_bounds__(WT) void betas_kernel(fe *b, uint64_t nb, const uint32_t *__restrict__ qs,
                                                    const fe *__restrict__ cs, int l, int logN,
                                                    const fe *__restrict__ twlo, const fe *__restrict__ twhi) {
@@ -128,33 +146,158 @@ __global__ __launch_bounds__(WT) void betas_kernel(fe *b, uint64_t nb, const uin
         qq[i] = (uint32_t)e;
     }
     __syncthreads();
-    const uint64_t j0 = ((uint64_t)blockIdx.x * WT + threadIdx.x) * BETA_J;
+    const uint64_t j0 = ((uint64_t)blockIdx.x * WT + threadIdx.x) * BJ;
     if (j0 >= nb) return;
     // weak arithmetic (gf.h): the running power stays folded once (<= p + 7,
     // the canonical w first in each product) and the sums are left unreduced
     // for up to four queries (5 (p + 7) < 2^64), folded in between and made
     // canonical at the store
-    fe acc[BETA_J];
+    fe acc[BJ];
 #pragma unroll
-    for (int t = 0; t < BETA_J; ++t) acc[t] = (j0 + t < nb) ? b[j0 + t] : gf::zero();
+    for (int t = 0; t < BJ; ++t) acc[t] = (j0 + t < nb) ? b[j0 + t] : gf::zero();
+#pragma unroll 4
     for (int i = 0; i < l; ++i) {
         const uint64_t e = ((uint64_t)qq[i] * j0) & Nmask;
         fe p = gf::mul_w(cq[i], tw28d(twlo, twhi, (uint32_t)(e << sh)));
         const fe w = wq[i];
 #pragma unroll
-        for (int t = 0; t < BETA_J; ++t) {
+        for (int t = 0; t < BJ; ++t) {
             acc[t].re += p.re;
             acc[t].im += p.im;
-            if (t + 1 < BETA_J) p = gf::mul_w(w, p);
+            if (t + 1 < BJ) p = gf::mul_w(w, p);
         }
         if ((i & 3) == 3) {
 #pragma unroll
-            for (int t = 0; t < BETA_J; ++t) acc[t] = gf::mk(gf::fold(acc[t].re), gf::fold(acc[t].im));
+            for (int t = 0; t < BJ; ++t) acc[t] = gf::mk(gf::fold(acc[t].re), gf::fold(acc[t].im));
         }
     }
 #pragma unroll
-    for (int t = 0; t < BETA_J; ++t)
+    for (int t = 0; t < BJ; ++t)
         if (j0 + t < nb) b[j0 + t] = gf::canon_w(gf::mk(gf::fold(acc[t].re), gf::fold(acc[t].im)));
+}
+
+// P[r WT + a] = w^(q_r' a) for the l sorted queries and a < WT (the slot
+// form's per-thread powers, shared by every workgroup of the launch)
+__global__ __launch_bounds__(WT) void beta_pow_kernel(const uint32_t *__restrict__ qs, int l, int logN,
+                                                      const fe *__restrict__ twlo, const fe *__restrict__ twhi,
+                                                      fe *__restrict__ P) {
+    const int r = blockIdx.x, a = threadIdx.x;
+    if (r >= l) return;
+    const uint64_t Nmask = (1ull << logN) - 1;
+    const uint64_t e = (((uint64_t)qs[r] & Nmask) * (uint64_t)a) & Nmask;
+    P[(size_t)r * WT + a] = tw28d(twlo, twhi, (uint32_t)(e << (NTT_LOG_MAX - logN)));
+}
+
+// update_betas, slot form.  With j = j' + J k (j' < J = nb / K, k < K = BS_K)
+// and D = K N / nb, w^(q j) = w^(q j') w_D^(q k), so
+//   b[j' + J k] += sum_{u < D} t_u w_D^(u k),  t_u = sum_{q_i' = u mod D} c_i w^(q_i' j').
+// Splitting u = R u1 + u2 (R = D / K, u1 < K) gives, per class u2, a K-point
+// DFT over u1 of t_(R u1 + u2) times w_D^(u2 k).  The queries come sorted by
+// (u2, u1) with the bounds in the plan.  The term c_i w^(q_i' j') is
+// G_i P[i][a] with j' = a + JW x (x = workgroup, JW = WT / S), G_i =
+// c_i w^(q_i' JW x) computed per workgroup; a class's terms are loaded eight
+// at a time.  A slot's terms are consecutive in that order, so a running sum
+// in registers is written once per slot, when the (uniform) slot index
+// changes, into the thread's own LDS column (slot u1 at [u1 WT + thread]).
+// S threads share one j' and take every S-th class; their K partial outputs
+// meet in the same LDS afterwards.  A thread does one multiply per query of
+// its classes and one K-point DFT and K twiddles per class, for K outputs;
+// the direct form does about one multiply per query per output.
+template <int S>
+__global__ __launch_bounds__(WT) void betas_slots_kernel(fe *b, uint64_t J, const uint32_t *__restrict__ qs,
+                                                         const fe *__restrict__ cs, int l, int logN,
+                                                         const BetaSlots *__restrict__ plan,
+                                                         const fe *__restrict__ P, const fe *__restrict__ twlo,
+                                                         const fe *__restrict__ twhi) {
+    constexpr int JW = WT / S;
+    __shared__ fe G[WQ_MAX];
+    __shared__ fe tw[BS_GMAX][BS_K];
+    extern __shared__ fe sl[];                  // BS_K x WT slot sums; then (S - 1) x BS_K x JW partials
+    const int sh = NTT_LOG_MAX - logN;
+    const uint64_t Nmask = (1ull << logN) - 1;
+    const uint64_t base = (uint64_t)blockIdx.x * JW;
+    for (int i = threadIdx.x; i < l; i += WT) {
+        const uint64_t e = (((uint64_t)qs[i] & Nmask) * base) & Nmask;
+        G[i] = gf::mul(cs[i], tw28d(twlo, twhi, (uint32_t)(e << sh)));
+    }
+    const int ng = plan->ng, logD = plan->logD;
+    for (int x = threadIdx.x; x < ng * BS_K; x += WT) {
+        const int g = x / BS_K, k = x % BS_K;
+        tw[g][k] = tw28d(twlo, twhi, (uint32_t)(plan->u2[g] * k) << (NTT_LOG_MAX - logD));
+    }
+    __syncthreads();
+    // JW >= 64: a wave lies in one part, so part (and every plan index
+    // derived from it) is uniform; J is a multiple of JW (host), so every
+    // thread has a j' < J
+    const int a = threadIdx.x % JW, part = __builtin_amdgcn_readfirstlane(threadIdx.x / JW);
+    const uint64_t j = base + a;
+    const fe *Pa = P + a;
+    fe *ts = sl + threadIdx.x;
+    fe out[BS_K];
+#pragma unroll
+    for (int k = 0; k < BS_K; ++k) out[k] = gf::zero();
+    for (int g = part; g < ng; g += S) {
+#pragma unroll
+        for (int u = 0; u < BS_K; ++u) ts[u * WT] = gf::zero();
+        const int rs = plan->off[g][0], re = plan->off[g][BS_K];   // re > rs: classes are non-empty
+        int cur = plan->slot[rs];
+        uint64_t sr = 0, si = 0;                // weak running sum of slot cur
+        for (int r0 = rs; r0 < re; r0 += 8) {
+            fe y[8];
+#pragma unroll
+            for (int x = 0; x < 8; ++x)
+                if (r0 + x < re) y[x] = Pa[(size_t)(r0 + x) * WT];
+#pragma unroll
+            for (int x = 0; x < 8; ++x)
+                if (r0 + x < re) {
+                    const fe m = gf::mul_w(G[r0 + x], y[x]);
+                    const int u1 = plan->slot[r0 + x];
+                    if (u1 != cur) {
+                        ts[cur * WT] = gf::mk(sr, si);
+                        cur = u1;
+                        sr = m.re;
+                        si = m.im;
+                    } else {
+                        sr = gf::fold(sr + m.re);
+                        si = gf::fold(si + m.im);
+                    }
+                }
+        }
+        ts[cur * WT] = gf::mk(sr, si);
+        fe t[BS_K];
+#pragma unroll
+        for (int u = 0; u < BS_K; ++u) t[u] = ts[u * WT];
+        dft::dif_w<BS_K, false>(t);             // t[i] = DFT[brev(i)]
+        dft::static_for<0, BS_K>([&](auto ii) {
+            constexpr int i = decltype(ii)::value;
+            constexpr int k = dft::brev_c<BS_K>(i);
+            const fe y = gf::mul_w(tw[g][k], t[i]);
+            out[k] = gf::mk(gf::fold(out[k].re + y.re), gf::fold(out[k].im + y.im));
+        });
+    }
+    if constexpr (S > 1) {
+        fe *red = sl;
+        __syncthreads();                        // every thread's slot column is done
+        if (part > 0) {
+#pragma unroll
+            for (int k = 0; k < BS_K; ++k) red[((part - 1) * BS_K + k) * JW + a] = out[k];
+        }
+        __syncthreads();
+        if (part > 0) return;
+#pragma unroll
+        for (int k = 0; k < BS_K; ++k)
+#pragma unroll
+            for (int q = 1; q < S; ++q) {
+                const fe y = red[((q - 1) * BS_K + k) * JW + a];
+                out[k] = gf::mk(gf::fold(out[k].re + y.re), gf::fold(out[k].im + y.im));
+            }
+    }
+    fe v[BS_K];                                 // all loads before any store
+#pragma unroll
+    for (int k = 0; k < BS_K; ++k) v[k] = b[j + J * k];
+#pragma unroll
+    for (int k = 0; k < BS_K; ++k)
+        b[j + J * k] = gf::canon_w(gf::mk(gf::fold(v[k].re + out[k].re), gf::fold(v[k].im + out[k].im)));
 }
 
 // ---- host helpers ----------------------------------------------------
@@ -176,6 +319,93 @@ std::vector<fe> eq_half(const fe *z, int k0, int bits) {
 }
 
 int ilog2h(uint64_t x) { int l = 0; while ((1ull << l) < x) ++l; return l; }
+
+// Sorts one iteration's update_betas pairs (q_i', c_i) by slot, (u2, u1) of
+// u = q_i' mod D, and fills the slot plan.  Returns false where the slot
+// form does not pay: b below BS_MIN_NB, more than BS_GMAX classes, or more
+// than one class per two queries (each class costs a K-point DFT and K
+// twiddles per thread).  The pairs are permuted either way; the sum over
+// them does not depend on their order.
+static_assert(BS_K == 16, "slot keys pack u1 into four bits");
+bool beta_slots_plan(uint32_t *qs, fe *cs, int l, uint64_t nb, int logN, BetaSlots &bp) {
+    memset(&bp, 0, sizeof bp);
+    static_assert(BS_MIN_NB / BS_K >= WT, "the slot kernel needs J to be a multiple of its workgroup's span");
+    if (l <= 0 || l > WQ_MAX || nb < BS_MIN_NB || (nb & (nb - 1)) || (1ull << logN) < nb) return false;
+    const int logD = logN - ilog2h(nb) + 4, logR = logD - 4;
+    if (logD > NTT_LOG_MAX) return false;
+    const uint64_t Nm = (1ull << logN) - 1, Dm = (1ull << logD) - 1, Rm = (1ull << logR) - 1;
+    std::vector<std::pair<uint32_t, int>> key(l);
+    for (int i = 0; i < l; ++i) {
+        const uint64_t u = ((uint64_t)qs[i] & Nm) & Dm;
+        key[i] = {(uint32_t)(((u & Rm) << 4) | (u >> logR)), i};
+    }
+    std::stable_sort(key.begin(), key.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+    const std::vector<uint32_t> q0(qs, qs + l);
+    const std::vector<fe> c0(cs, cs + l);
+    for (int i = 0; i < l; ++i) {
+        qs[i] = q0[key[i].second];
+        cs[i] = c0[key[i].second];
+    }
+    int ng = 0;
+    for (int i = 0; i < l;) {
+        const uint32_t cls = key[i].first >> 4;
+        int e = i;
+        while (e < l && (key[e].first >> 4) == cls) ++e;
+        if (ng == BS_GMAX) return false;
+        bp.u2[ng] = (int)cls;
+        int r = i;
+        for (int u1 = 0; u1 <= BS_K; ++u1) {
+            while (r < e && (int)(key[r].first & 15) < u1) ++r;
+            bp.off[ng][u1] = r;
+        }
+        for (int x = i; x < e; ++x) bp.slot[x] = (int)(key[x].first & 15);
+        ++ng;
+        i = e;
+    }
+    bp.ng = ng;
+    bp.logD = logD;
+    return 2 * ng <= l;
+}
+
+// update_betas on b (nb entries) for one iteration: the slot form where its
+// plan allows (plan on the device, ng of its classes, P = l x WT powers of
+// scratch), else the direct form with BJ sized to nb
+int betas_launch(cosnark_ctx *ctx, fe *b, uint64_t nb, const uint32_t *dqs, const fe *dc, int l, int logN,
+                 bool slots, int ng, const BetaSlots *dplan, fe *P, hipStream_t s) {
+    const fe *tl = (const fe *)ctx->tw_lo, *th = (const fe *)ctx->tw_hi;
+    if (slots) {
+        hipLaunchKernelGGL(beta_pow_kernel, dim3((unsigned)l), dim3(WT), 0, s, dqs, l, logN, tl, th, P);
+        LAUNCHCK();
+        // S threads per j' while that keeps at least 1024 workgroups (and
+        // S is at most the class count, which the host plan knows)
+        const uint64_t J = nb / BS_K;           // a power of two >= BS_MIN_NB / BS_K >= WT
+        int S = 1;
+        while (S < 4 && S < ng && J * S / WT < 1024) S *= 2;
+        const size_t lds = sizeof(fe) * BS_K * WT;       // slot sums, then the partials (smaller)
+        const unsigned grid = (unsigned)((J * S + WT - 1) / WT);
+        if (S == 1)
+            hipLaunchKernelGGL(betas_slots_kernel<1>, dim3(grid), dim3(WT), lds, s, b, J, dqs, dc, l, logN, dplan,
+                               (const fe *)P, tl, th);
+        else if (S == 2)
+            hipLaunchKernelGGL(betas_slots_kernel<2>, dim3(grid), dim3(WT), lds, s, b, J, dqs, dc, l, logN, dplan,
+                               (const fe *)P, tl, th);
+        else
+            hipLaunchKernelGGL(betas_slots_kernel<4>, dim3(grid), dim3(WT), lds, s, b, J, dqs, dc, l, logN, dplan,
+                               (const fe *)P, tl, th);
+        LAUNCHCK();
+        return COSNARK_OK;
+    }
+    const int bj = nb >= (1ull << 20) ? 16 : nb >= (1ull << 17) ? 4 : 1;
+    const unsigned grid = (unsigned)(((nb + bj - 1) / bj + WT - 1) / WT);
+    if (bj == 16)
+        hipLaunchKernelGGL(betas_kernel<16>, dim3(grid), dim3(WT), 0, s, b, nb, dqs, dc, l, logN, tl, th);
+    else if (bj == 4)
+        hipLaunchKernelGGL(betas_kernel<4>, dim3(grid), dim3(WT), 0, s, b, nb, dqs, dc, l, logN, tl, th);
+    else
+        hipLaunchKernelGGL(betas_kernel<1>, dim3(grid), dim3(WT), 0, s, b, nb, dqs, dc, l, logN, tl, th);
+    LAUNCHCK();
+    return COSNARK_OK;
+}
 
 // get_RS_multifold_segment_tensor (src/coPCS_utils.cpp:333-383) for one
 // whir_k-element segment at position idx of a codeword of n elements
@@ -296,6 +526,18 @@ struct Iter {
 
 }  // namespace
 
+int whir_init(cosnark_ctx *) {
+    const int lds = (int)(sizeof(fe) * BS_K * WT);
+    if (hipFuncSetAttribute((const void *)betas_slots_kernel<1>, hipFuncAttributeMaxDynamicSharedMemorySize, lds) !=
+            hipSuccess ||
+        hipFuncSetAttribute((const void *)betas_slots_kernel<2>, hipFuncAttributeMaxDynamicSharedMemorySize, lds) !=
+            hipSuccess ||
+        hipFuncSetAttribute((const void *)betas_slots_kernel<4>, hipFuncAttributeMaxDynamicSharedMemorySize, lds) !=
+            hipSuccess)
+        return COSNARK_E_HIP;
+    return COSNARK_OK;
+}
+
 extern "C" {
 
 int cosnark_whir_commit(cosnark_ctx *ctx, const uint64_t *d_arr, uint64_t n, int rate, uint64_t *d_codeword,
@@ -413,6 +655,7 @@ int cosnark_whir_open(cosnark_ctx *ctx, const uint64_t *d_arr, uint64_t n, const
     const size_t o_rep = take(16 * rep_total), o_path = take(32 * path_total);
     const size_t o_q = take(4 * qmax * its.size()), o_qs = take(4 * qmax * its.size());
     const size_t o_c = take(16 * qmax * its.size());
+    const size_t o_bs = take(sizeof(BetaSlots) * its.size()), o_P = take(16 * qmax * WT);
     void *base;
     int rc = scratch_reserve(ctx, off, &base);
     if (rc) return rc;
@@ -427,12 +670,16 @@ int cosnark_whir_open(cosnark_ctx *ctx, const uint64_t *d_arr, uint64_t n, const
     uint32_t *dq = (uint32_t *)(B + o_q);      // positions as drawn (gathers)
     uint32_t *dqs = (uint32_t *)(B + o_qs);    // scaled to the new codeword (update_betas)
     fe *dc = (fe *)(B + o_c);
+    BetaSlots *dbs = (BetaSlots *)(B + o_bs);
+    fe *dP = (fe *)(B + o_P);
 
     // host-side parameter staging (kept alive until the final sync)
     std::vector<fe> lo = eq_half((const fe *)h_z, 0, lb), hi = eq_half((const fe *)h_z, lb, hb);
     std::vector<uint32_t> hq(qmax * its.size(), 0);
     std::vector<fe> hc(qmax * its.size(), gf::zero());
     std::vector<fe> ext(64, gf::zero());
+    std::vector<BetaSlots> hbs(its.size());
+    std::vector<char> slots(its.size(), 0);
     for (size_t t = 0; t < its.size(); ++t) {
         const Iter &it = its[t];
         const uint64_t scale = (uint64_t)WK * it.cw_new_n / it.cw_n;   // Whir.cpp:373-374
@@ -442,6 +689,7 @@ int cosnark_whir_open(cosnark_ctx *ctx, const uint64_t *d_arr, uint64_t n, const
             hq[t * qmax + i] = (uint32_t)(it.q[i] * scale);
             hc[t * qmax + i] = c;
         }
+        slots[t] = beta_slots_plan(&hq[t * qmax], &hc[t * qmax], it.l, it.n / 4, ilog2(it.cw_new_n), hbs[t]);
         ext[2 * t] = it.r[0];
         ext[2 * t + 1] = it.r[1];
     }
@@ -454,6 +702,7 @@ int cosnark_whir_open(cosnark_ctx *ctx, const uint64_t *d_arr, uint64_t n, const
     HIPCK(hipMemcpyAsync(dq, hq_raw.data(), 4 * hq_raw.size(), hipMemcpyHostToDevice, s));
     HIPCK(hipMemcpyAsync(dqs, hq.data(), 4 * hq.size(), hipMemcpyHostToDevice, s));
     HIPCK(hipMemcpyAsync(dc, hc.data(), sizeof(fe) * hc.size(), hipMemcpyHostToDevice, s));
+    HIPCK(hipMemcpyAsync(dbs, hbs.data(), sizeof(BetaSlots) * hbs.size(), hipMemcpyHostToDevice, s));
     HIPCK(hipMemcpyAsync(ctx->sc->ext_r, ext.data(), sizeof(fe) * 64, hipMemcpyHostToDevice, s));
     rc = sumcheck_state_init(ctx, gf::zero(), gf::zero(), SC_CLAIM_UNCHECKED, s);
     if (rc) return rc;
@@ -490,14 +739,9 @@ int cosnark_whir_open(cosnark_ctx *ctx, const uint64_t *d_arr, uint64_t n, const
             LAUNCHCK();
         }
         // update_betas on the folded b
-        {
-            const uint64_t nb = it.n / 4;
-            const uint64_t threads = (nb + BETA_J - 1) / BETA_J;
-            hipLaunchKernelGGL(betas_kernel, dim3((unsigned)((threads + WT - 1) / WT)), dim3(WT), 0, s, b_n, nb,
-                               (const uint32_t *)(dqs + t * qmax), (const fe *)(dc + t * qmax), it.l,
-                               ilog2(it.cw_new_n), (const fe *)ctx->tw_lo, (const fe *)ctx->tw_hi);
-            LAUNCHCK();
-        }
+        if ((rc = betas_launch(ctx, b_n, it.n / 4, dqs + t * qmax, dc + t * qmax, it.l, ilog2(it.cw_new_n),
+                               slots[t], hbs[t].ng, dbs + t, dP, s)))
+            return rc;
         arr = arr_n;
         b = b_n;
         cw_old = cw_new;
@@ -645,6 +889,7 @@ int cosnark_whir_batch_open(cosnark_ctx *ctx, int count, const uint64_t *const *
     const size_t STEP_Q = WQ_MAX;
     const size_t o_q = take(4 * STEP_Q * 64), o_qs = take(4 * STEP_Q * 64), o_c = take(16 * STEP_Q * 64);
     const size_t o_z = take(32 * 29 * 64);
+    const size_t o_bs = take(sizeof(BetaSlots) * 64), o_P = take(16 * STEP_Q * WT);
     // replies and paths: bounded by the steps' sizes, reserved below once known
     void *base;
     int rc;
@@ -792,6 +1037,8 @@ int cosnark_whir_batch_open(cosnark_ctx *ctx, int count, const uint64_t *const *
         uint32_t *dq = (uint32_t *)(B + o_q), *dqs = (uint32_t *)(B + o_qs);
         fe *dc = (fe *)(B + o_c);
         uint8_t *dz = (uint8_t *)(B + o_z);
+        BetaSlots *dbs = (BetaSlots *)(B + o_bs);
+        fe *dP = (fe *)(B + o_P);
         fe *rep = (fe *)(B + o_rep);
         uint8_t *paths = (uint8_t *)(B + o_path);
         fe *parts = (fe *)(B + o_parts);
@@ -800,6 +1047,9 @@ int cosnark_whir_batch_open(cosnark_ctx *ctx, int count, const uint64_t *const *
         std::vector<uint32_t> hq(STEP_Q * 64, 0), hqs(STEP_Q * 64, 0);
         std::vector<fe> hc(STEP_Q * 64, gf::zero());
         std::vector<uint8_t> hz(32 * 29 * 64, 0);
+        std::vector<BetaSlots> hbs(64);
+        std::vector<char> slots(64, 0);
+        if (steps.size() > 64) return set_err(ctx, COSNARK_E_RANGE, "whir_batch_open: over 64 steps");
         for (size_t t = 0; t < steps.size(); ++t) {
             const Step &st = steps[t];
             if (st.kind == 0) {
@@ -811,16 +1061,18 @@ int cosnark_whir_batch_open(cosnark_ctx *ctx, int count, const uint64_t *const *
                     hqs[t * STEP_Q + i] = (uint32_t)(st.q[i] * scale);
                     hc[t * STEP_Q + i] = c;
                 }
+                slots[t] = beta_slots_plan(&hqs[t * STEP_Q], &hc[t * STEP_Q], st.l, st.n / st.kappa,
+                                           ilog2(st.cw_new_n), hbs[t]);
             } else if (st.kind == 1) {
                 for (int i = 0; i < st.l; ++i) hq[t * STEP_Q + i] = st.q[i];
             }
         }
-        if (steps.size() > 64) return set_err(ctx, COSNARK_E_RANGE, "whir_batch_open: over 64 steps");
         for (size_t c = 0; c < chains.size(); ++c) memcpy(hz.data() + 32 * 29 * c, chains[c].data(), chains[c].size());
         HIPCK(hipMemcpyAsync(dq, hq.data(), 4 * hq.size(), hipMemcpyHostToDevice, s));
         HIPCK(hipMemcpyAsync(dqs, hqs.data(), 4 * hqs.size(), hipMemcpyHostToDevice, s));
         HIPCK(hipMemcpyAsync(dc, hc.data(), sizeof(fe) * hc.size(), hipMemcpyHostToDevice, s));
         HIPCK(hipMemcpyAsync(dz, hz.data(), hz.size(), hipMemcpyHostToDevice, s));
+        HIPCK(hipMemcpyAsync(dbs, hbs.data(), sizeof(BetaSlots) * hbs.size(), hipMemcpyHostToDevice, s));
         HIPCK(hipMemcpyAsync(ctx->sc->ext_r, ext.data(), sizeof(fe) * 64, hipMemcpyHostToDevice, s));
         if ((rc = sumcheck_state_init(ctx, gf::zero(), gf::zero(), SC_CLAIM_UNCHECKED, s))) return rc;
 
@@ -850,12 +1102,9 @@ int cosnark_whir_batch_open(cosnark_ctx *ctx, int count, const uint64_t *const *
                                    (const uint32_t *)(dq + t * STEP_Q), st.l, st.depth, rep + st.rep_off,
                                    paths + 32 * st.path_off);
                 LAUNCHCK();
-                const uint64_t nb = st.n / st.kappa;
-                const uint64_t threads = (nb + BETA_J - 1) / BETA_J;
-                hipLaunchKernelGGL(betas_kernel, dim3((unsigned)((threads + WT - 1) / WT)), dim3(WT), 0, s, b_n, nb,
-                                   (const uint32_t *)(dqs + t * STEP_Q), (const fe *)(dc + t * STEP_Q), st.l,
-                                   ilog2(st.cw_new_n), (const fe *)ctx->tw_lo, (const fe *)ctx->tw_hi);
-                LAUNCHCK();
+                if ((rc = betas_launch(ctx, b_n, st.n / st.kappa, dqs + t * STEP_Q, dc + t * STEP_Q, st.l,
+                                       ilog2(st.cw_new_n), slots[t], hbs[t].ng, dbs + t, dP, s)))
+                    return rc;
                 const uint64_t cnt = cosnark_merkle_node_count(st.cw_new_n);
                 HIPCK(hipMemcpyAsync(roots.data() + 32 * t, nd_new + 32 * (cnt - 1), 32, hipMemcpyDeviceToHost, s));
                 arr = arr_n;

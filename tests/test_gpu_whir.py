@@ -45,7 +45,8 @@ def test_whir_claim_matches_oracle(ctx):
     assert np.array_equal(ctx.whir_claim(ctx.upload(arr), n, z), O.whir_claim(arr, z))
 
 
-@pytest.mark.parametrize("logn,rate,stop", [(6, 4, 0), (10, 4, 0), (11, 2, 0), (12, 4, 8), (14, 4, 0), (16, 4, 8)])
+@pytest.mark.parametrize("logn,rate,stop", [(6, 4, 0), (10, 4, 0), (11, 2, 0), (12, 4, 8), (14, 4, 0), (16, 4, 8),
+                                            (18, 4, 8), (18, 2, 8), (20, 4, 8)])
 def test_whir_open_matches_oracle(ctx, logn, rate, stop):
     n = 1 << logn
     seed = 0xC05A0003 + logn

@@ -26,7 +26,8 @@ def _gpu(ctx, inp):
 @pytest.mark.parametrize("logs,rate,whir_k", [([10], 4, 4), ([10, 10], 4, 4), ([12, 8, 6, 6], 4, 4),
                                               ([12, 12, 8, 6, 6], 2, 4), ([12, 9], 4, 8), ([8, 6], 4, 2),
                                               ([8, 2], 4, 4), ([11, 7, 5], 2, 4), ([13, 10, 10, 7], 2, 8),
-                                              ([16, 14, 12], 4, 4)])
+                                              ([16, 14, 12], 4, 4), ([18, 16], 4, 4), ([19, 12], 4, 8),
+                                              ([17, 10], 4, 2)])
 def test_batch_open_matches_oracle(ctx, logs, rate, whir_k):
     print(f"whir batch open logs={logs} rate={rate} whir_k={whir_k}", flush=True)
     inp = _inputs(0xC05A0050 + sum(logs), logs, rate, whir_k)
