@@ -100,6 +100,10 @@ int cosnark_create(int device, cosnark_ctx **out) {
         const int v = atoi(t);
         if (v >= 256 && v <= NTT_LDS_ELEMS && is_pow2((uint64_t)v)) ctx->ntt_tile = v;
     }
+    if (const char *t = getenv("COSNARK_NTT_FAN_MIN")) {
+        const int v = atoi(t);
+        if (v >= 1) ctx->ntt_fan_min_blocks = v;
+    }
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return fail(COSNARK_E_HIP);
     if (hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) != hipSuccess) return fail(COSNARK_E_HIP);
     if (const char *e = getenv("COSNARK_SC_BLOCKS")) {

@@ -15,6 +15,7 @@ constexpr int NTT_LDS_ELEMS = 4096;             // elements per workgroup tile
 constexpr int NTT_THREADS = 256;
 constexpr int NTT_RADIX_LOG_MAX = 10;           // radix twiddle table: w_1024^j, j < 1024
 constexpr int NTT_PASS_LOG_MAX = 8;             // radix of one LDS pass <= 256
+constexpr int NTT_FAN_LOG_MAX = 3;              // first-pass fan-out over zero padding up to 8x
 
 // Product-sumcheck device state (one per context).  Transcript value and
 // claim are kept per round: tr_prev[k], tr_y[k] = state before round k.
@@ -52,6 +53,7 @@ struct cosnark_ctx {
     hipEvent_t ev[8] = {};
     int ntt_tile = NTT_LDS_ELEMS;     // elements per NTT workgroup tile (power of two, <= 4096)
     int ntt_passbits = NTT_PASS_LOG_MAX;  // largest radix (bits) of one NTT pass
+    int ntt_fan_min_blocks = 512;     // fan-out first pass only with at least this many workgroups
     // per-size plan (0 = the defaults above); set by the autotuner
     struct { int passbits, tile; } ntt_plan[NTT_LOG_MAX + 1] = {};
     bool overlap = true;              // plain prover: sumcheck beside the commit

@@ -16,7 +16,11 @@
 // pass values are weak (gf.h: unreduced sums, one fold per radix-4 layer);
 // the twiddle products and the last pass's stores are canonical.  Zero padding
 // (Reed-Solomon encoding of n coefficients into rate*n points) is handled by
-// the first pass, which reads only the n real inputs.
+// the first pass, which reads only the n real inputs.  Where padding by
+// E = 2^e would cost a pass, the first pass fans out instead: its digit is
+// e bits wider than its transform, and since only the first R of its E*R
+// inputs are non-zero, X[k0 + E k1] = DFT_R(x[d] w_(ER)^(d k0))[k1]; the
+// workgroup runs the R-point transform once per k0 < E on its twisted tile.
 #include "blake3.h"
 #include "ctx.h"
 #include "dft.h"
@@ -39,6 +43,7 @@ struct NttPass {
     uint64_t in_len;    // elements of `in` that are non-zero inputs
     int logN, logR, logS, logC;
     int last;           // 1: stride-1 pass writing natural order
+    int fan;            // first pass of a transform padded by 2^fan: fan out (see above)
     int npass;
     int rbits[8];       // radix bits per pass, first pass first
     int pre[8];         // pre[t] = rbits[0] + .. + rbits[t-1]
@@ -86,41 +91,12 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
     }
     __syncthreads();
 
-    // ---- step 1: R1-point DFTs over d1 for each (d2, c) ----
+    // fan-out (first pass of a padded forward transform): the pass digit has
+    // LOGR + fan bits, and the steps below run once per k0 < 2^fan
+    const int fan = (LAST || INV) ? 0 : p.fan;
+    const int logNt = LOGR + fan + logS;
     const int items1 = R2 << logC;
-    for (int it = tid; it < items1; it += NTT_THREADS) {
-        int c, d2;
-        uint64_t base;
-        if (!LAST) {
-            c = it & (C - 1);
-            d2 = it >> logC;
-            base = (hi << (LOGR + logS)) + lo0 + c;
-        } else {
-            d2 = it & (R2 - 1);
-            c = it >> LOGR2;
-            base = (rest + (c0 + c) * W0) << LOGR;
-        }
-        fe v[R1];
-#pragma unroll
-        for (int d1 = 0; d1 < R1; ++d1) {
-            const uint64_t d = (uint64_t)(R2 * d1 + d2);
-            const uint64_t pos = base + (LAST ? d : (d << logS));
-            v[d1] = pos < p.in_len ? in[pos] : gf::zero();
-        }
-        dif_w<R1, INV>(v);
-        static_for<0, R1>([&](auto ii) {
-            constexpr int i = decltype(ii)::value;
-            constexpr int k1 = brev_c<R1>(i);
-            fe x = v[i];
-            if (k1 != 0 && d2 != 0) x = gf::mul_w(wr[d2 * k1], x);
-            Y[(k1 * R2 + d2) * CP + c] = x;
-        });
-    }
-    __syncthreads();
-
-    // ---- step 2: R2-point DFTs over d2 for each (k1, c), twiddle, store ----
     const int items2 = R1 << logC;
-    const int logNt = LOGR + logS;
     uint64_t nat_rest = 0;
     int kshift = 0;
     if (LAST) {
@@ -137,6 +113,43 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
     const bool leaves_on = LAST && p.leaves && by == 0 && items2 <= NTT_THREADS;
     fe v[R2];
     int my_c = 0, my_k1 = 0;
+    for (int k0 = 0; k0 < (1 << fan); ++k0) {
+    if (k0 > 0) __syncthreads();   // the previous k0's step 2 is done with Y
+
+    // ---- step 1: R1-point DFTs over d1 for each (d2, c) ----
+    for (int it = tid; it < items1; it += NTT_THREADS) {
+        int c, d2;
+        uint64_t base;
+        if (!LAST) {
+            c = it & (C - 1);
+            d2 = it >> logC;
+            base = (hi << logNt) + lo0 + c;
+        } else {
+            d2 = it & (R2 - 1);
+            c = it >> LOGR2;
+            base = (rest + (c0 + c) * W0) << LOGR;
+        }
+        fe v[R1];
+#pragma unroll
+        for (int d1 = 0; d1 < R1; ++d1) {
+            const uint64_t d = (uint64_t)(R2 * d1 + d2);
+            const uint64_t pos = base + (LAST ? d : (d << logS));
+            v[d1] = pos < p.in_len ? in[pos] : gf::zero();
+            if (k0 > 0)   // twist by w_(E R)^(d k0); inputs are canonical
+                v[d1] = gf::mul_w(tw28(twlo, twhi, (uint32_t)(d * k0) << (NTT_LOG_MAX - LOGR - fan)), v[d1]);
+        }
+        dif_w<R1, INV>(v);
+        static_for<0, R1>([&](auto ii) {
+            constexpr int i = decltype(ii)::value;
+            constexpr int k1 = brev_c<R1>(i);
+            fe x = v[i];
+            if (k1 != 0 && d2 != 0) x = gf::mul_w(wr[d2 * k1], x);
+            Y[(k1 * R2 + d2) * CP + c] = x;
+        });
+    }
+    __syncthreads();
+
+    // ---- step 2: R2-point DFTs over d2 for each (k1, c), twiddle, store ----
     for (int it = tid; it < items2; it += NTT_THREADS) {
         const int c = it & (C - 1), k1 = it >> logC;
         my_c = c;
@@ -145,18 +158,19 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
         for (int d2 = 0; d2 < R2; ++d2) v[d2] = Y[(k1 * R2 + d2) * CP + c];
         dif_w<R2, INV>(v);
         if (!LAST) {
-            // inter-pass twiddle w_{N_t}^(lo k), k = k1 + R1 k2: one table product for
-            // w^(lo k1) and one for w^(lo R1), then a running product over k2
+            // inter-pass twiddle w_{N_t}^(lo k), k = k0 + 2^fan (k1 + R1 k2): one table
+            // product for w^(lo (k0 + 2^fan k1)) and one for w^(lo 2^fan R1), then a
+            // running product over k2
             const uint64_t lo = lo0 + c;
             const int sh = NTT_LOG_MAX - logNt;
-            const uint32_t eb = (uint32_t)((lo * (uint64_t)k1) << sh);
-            const uint32_t es = (uint32_t)((lo * (uint64_t)R1) << sh);
+            const uint32_t eb = (uint32_t)((lo * (uint64_t)(k0 + (k1 << fan))) << sh);
+            const uint32_t es = (uint32_t)((lo * (uint64_t)(R1 << fan)) << sh);
             fe t = tw28(twlo, twhi, eb);
             const fe step = tw28(twlo, twhi, es);
             static_for<0, R2>([&](auto kk) {
                 constexpr int k2 = decltype(kk)::value;
                 constexpr int i = brev_c<R2>(k2);
-                const uint64_t k = (uint64_t)(k1 + R1 * k2);
+                const uint64_t k = (uint64_t)k0 + ((uint64_t)(k1 + R1 * k2) << fan);
                 out[(hi << logNt) + (k << logS) + lo] = gf::mul_w(t, v[i]);   // weak: read by the next pass
                 if (k2 + 1 < R2) t = gf::mul(t, step);
             });
@@ -170,6 +184,7 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
             });
         }
     }
+    }   // k0
     if (leaves_on) {
         // Merkle leaves of the output (MT_commit_Blake, src/merkle_tree.cpp:
         // 211-220: leaf = BLAKE3 of 4 consecutive elements).  A leaf is 4
@@ -291,13 +306,28 @@ int ntt_launch_batch(cosnark_ctx *ctx, const fe *in, const fe *const *in_tab, ui
     // radix plan: fewest passes of at most `passbits` bits, spread evenly, larger first
     const int pmax = ntt_plan_bits(ctx, logn);
     const int tile = ntt_plan_tile(ctx, logn);
-    const int m = (logn + pmax - 1) / pmax;
     const uint64_t N = 1ull << logn;
+    int m = (logn + pmax - 1) / pmax;
+    // a forward transform padded by E = 2^fan (in_len = N / E, E <= 8) fans
+    // out in its first pass where that saves a pass and the first pass keeps
+    // enough workgroups (in_len / tile of them, each running E transforms)
+    int fan = 0;
+    if (!inverse && in_len > 0 && in_len < N && (in_len & (in_len - 1)) == 0 &&
+        (in_len / (uint64_t)tile) >= (uint64_t)ctx->ntt_fan_min_blocks) {
+        const int e = logn - ilog2(in_len);
+        const int mf = (logn - e + pmax - 1) / pmax;
+        if (e <= NTT_FAN_LOG_MAX && mf >= 2 && mf < m) {
+            fan = e;
+            m = mf;
+        }
+    }
     NttPass p{};
     p.in_len = in_len;
     p.logN = logn;
     p.npass = m;
-    for (int t = 0; t < m; ++t) p.rbits[t] = logn / m + (t < logn % m ? 1 : 0);
+    const int lb = logn - fan;
+    for (int t = 0; t < m; ++t) p.rbits[t] = lb / m + (t < lb % m ? 1 : 0);
+    p.rbits[0] += fan;
     p.pre[0] = 0;
     for (int t = 1; t < m; ++t) p.pre[t] = p.pre[t - 1] + p.rbits[t - 1];
     const fe *twlo = inverse ? ctx->itw_lo : ctx->tw_lo;
@@ -315,13 +345,15 @@ int ntt_launch_batch(cosnark_ctx *ctx, const fe *in, const fe *const *in_tab, ui
     }
     int done_bits = 0;
     for (int t = 0; t < m; ++t) {
-        const int r = p.rbits[t];
+        const int r = p.rbits[t];                 // the pass digit's bits
+        const int rd = r - (t == 0 ? fan : 0);    // its transform's
         NttPass q = p;
         q.logR = r;
+        q.fan = t == 0 ? fan : 0;
         const bool last = (t == m - 1);
         q.last = last;
         q.logS = logn - done_bits - r;
-        int logC = ilog2((uint64_t)tile) - r;
+        int logC = ilog2((uint64_t)tile) - rd;
         if (logC < 0) logC = 0;
         if (!last) {
             if (logC > q.logS) logC = q.logS;
@@ -359,7 +391,7 @@ int ntt_launch_batch(cosnark_ctx *ctx, const fe *in, const fe *const *in_tab, ui
             q.out_tab = nullptr;
             q.out_stride = N;
         }
-        int rc = dispatch_pass(ctx, stream, r, blocks, batch, src, dst, q, last, inverse, twlo, twhi, rtab, scale);
+        int rc = dispatch_pass(ctx, stream, rd, blocks, batch, src, dst, q, last, inverse, twlo, twhi, rtab, scale);
         if (rc) return rc;
         done_bits += r;
     }

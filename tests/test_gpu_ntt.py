@@ -44,6 +44,7 @@ def test_ntt_four_pass_matches_oracle(ctx, logn):
 
 
 def test_rs_encode_four_pass_matches_oracle(ctx):
+    # 2^22 -> 2^25: three passes, the first fanning out over the padding by 8
     a = O.synth(0xF1, 0, 1 << 22)
     print("rs_encode 2^22 rate 8", flush=True)
     assert np.array_equal(ctx.rs_encode(a, 8), O.rs_encode(a, 8))
@@ -66,5 +67,30 @@ def test_ntt_many_pass_plans_match_oracle(passbits, logn):
         assert np.array_equal(c.ntt(a), O.fft(a))
         assert np.array_equal(c.ntt(a, inverse=True), O.fft(a, inverse=True))
         assert np.array_equal(c.rs_encode(a[: (1 << logn) // 4], 4), O.rs_encode(a[: (1 << logn) // 4], 4))
+    finally:
+        c.close()
+
+
+@pytest.mark.parametrize("passbits,logn,rate", [(0, 16, 4), (0, 15, 4), (0, 16, 2), (0, 15, 8), (3, 18, 4),
+                                                (2, 12, 4), (5, 15, 8)])
+def test_rs_encode_fan_out_matches_oracle(passbits, logn, rate):
+    """The fan-out first pass (zero padding by 2, 4 or 8 covered by the first
+    pass's digit) at sizes below its default workgroup threshold, with the
+    threshold lowered, at the default and at small pass radices."""
+    import os
+    import cosnark
+    env = {"COSNARK_NTT_FAN_MIN": "1"}
+    if passbits:
+        env["COSNARK_NTT_PASSBITS"] = str(passbits)
+    os.environ.update(env)
+    try:
+        c = cosnark.Context(0)
+    finally:
+        for k in env:
+            del os.environ[k]
+    try:
+        a = O.synth(0x90 + logn + rate, 0, 1 << logn)
+        print(f"rs_encode fan-out passbits={passbits} logn={logn} rate={rate}", flush=True)
+        assert np.array_equal(c.rs_encode(a, rate), O.rs_encode(a, rate))
     finally:
         c.close()
