@@ -598,7 +598,7 @@ __global__ __launch_bounds__(CO_THREADS) void bsc3_512_kernel(const fe *__restri
 template <int D>
 __global__ __launch_bounds__(CO_THREADS) void bsc3_small_kernel(const fe *__restrict__ p1, const fe *__restrict__ p2,
                                                                 const fe *__restrict__ p3, const fe *__restrict__ b1,
-                                                                uint64_t nblocks, fe inv_d, fe *__restrict__ part) {
+                                                                uint64_t nblocks, fe *__restrict__ part) {
     extern __shared__ fe S[];
     constexpr int RP = D + 1, BT = 64, VS = BT * RP, NP = BT * D, J = (NP + CO_THREADS - 1) / CO_THREADS;
     __shared__ int live[BT];
@@ -639,8 +639,9 @@ __global__ __launch_bounds__(CO_THREADS) void bsc3_small_kernel(const fe *__rest
         __syncthreads();
         // weak arithmetic (gf.h) through the transforms; canonical values in LDS
         dft::dif_w<D, true>(x);                                 // x[i] = D c[brev(i)]
+        constexpr int LOGD = D >= 16 ? 4 : D >= 8 ? 3 : D >= 4 ? 2 : 1;
 #pragma unroll
-        for (int i = 0; i < D; ++i) x[i] = gf::mul_w(inv_d, x[i]);
+        for (int i = 0; i < D; ++i) x[i] = gf::mul_pow2_w<61 - LOGD>(x[i]);   // / D: a rotation
         dft::static_for<1, 4>([&](auto cc) {
             constexpr int co = decltype(cc)::value;
             fe y[D];
@@ -1147,10 +1148,10 @@ int cosnark_cosumcheck3_phase1(cosnark_ctx *ctx, const uint64_t *d_p1, const uin
         // the tile's rows, and the four per-thread coset sums at the end
         const size_t lds = sizeof(fe) * std::max<size_t>(4 * 64 * (d + 1), 4 * CO_THREADS);
         switch (d) {
-            case 2: hipLaunchKernelGGL(bsc3_small_kernel<2>, dim3(nwg), dim3(CO_THREADS), lds, s, a1, a2, a3, a4, nb, di, part); break;
-            case 4: hipLaunchKernelGGL(bsc3_small_kernel<4>, dim3(nwg), dim3(CO_THREADS), lds, s, a1, a2, a3, a4, nb, di, part); break;
-            case 8: hipLaunchKernelGGL(bsc3_small_kernel<8>, dim3(nwg), dim3(CO_THREADS), lds, s, a1, a2, a3, a4, nb, di, part); break;
-            default: hipLaunchKernelGGL(bsc3_small_kernel<16>, dim3(nwg), dim3(CO_THREADS), lds, s, a1, a2, a3, a4, nb, di, part); break;
+            case 2: hipLaunchKernelGGL(bsc3_small_kernel<2>, dim3(nwg), dim3(CO_THREADS), lds, s, a1, a2, a3, a4, nb, part); break;
+            case 4: hipLaunchKernelGGL(bsc3_small_kernel<4>, dim3(nwg), dim3(CO_THREADS), lds, s, a1, a2, a3, a4, nb, part); break;
+            case 8: hipLaunchKernelGGL(bsc3_small_kernel<8>, dim3(nwg), dim3(CO_THREADS), lds, s, a1, a2, a3, a4, nb, part); break;
+            default: hipLaunchKernelGGL(bsc3_small_kernel<16>, dim3(nwg), dim3(CO_THREADS), lds, s, a1, a2, a3, a4, nb, part); break;
         }
     } else if (d == 512) {
         hipLaunchKernelGGL(bsc3_512_kernel, dim3(nwg), dim3(CO_THREADS), sizeof(fe) * C512_LDS, s, a1, a2, a3, a4,

@@ -202,6 +202,18 @@ GF_HD fe canon_w(const fe &a) { return mk(canon_w(a.re), canon_w(a.im)); }
 // y < 2^63  ->  y * 2^30 as a weak value: y = h 2^31 + l, 2^61 == 1
 GF_HD uint64_t mul2_30(uint64_t y) { return ((y & 0x7FFFFFFFull) << 30) + (y >> 31); }
 
+// y * 2^K mod p (0 < K < 61) on a weak y: with y = a + b 2^(61-K), a < 2^(61-K),
+// y 2^K = a 2^K + b 2^61 = a 2^K + b.  For y <= 2^61 + 2^32, b <= 2^K + 2^(K-29)
+// and the sum is at most (2^(61-K) - 1) 2^K + 2^K + 2^(K-29) <= 2^61 + 2^31: weak
+template <int K>
+GF_HD uint64_t mul2k_w(uint64_t y) {
+    static_assert(K > 0 && K < 61, "0 < K < 61");
+    return ((y & ((1ull << (61 - K)) - 1)) << K) + (y >> (61 - K));
+}
+// x * 2^K on a weak x (2^K is real; 1 / 2^k = 2^(61 - k)), weak result
+template <int K>
+GF_HD fe mul_pow2_w(const fe &x) { return mk(mul2k_w<K>(x.re), mul2k_w<K>(x.im)); }
+
 // x * w_M^E on a weak x, weak result; E reduced mod M at compile time
 template <int M, int E>
 GF_HD fe mul_root_w(const fe &x) {

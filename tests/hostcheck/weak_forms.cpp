@@ -3,7 +3,8 @@
 // X[n] = sum_j x[j] w_N^(jn), on random and adversarial inputs (components
 // at the top of the weak range, just above p, both forms of zero); every
 // output must stay weak.  Also mul_root_w against a general multiply, mul
-// and mul_w with a weak second operand, and canon_w at the range edges.
+// and mul_w with a weak second operand, canon_w at the range edges, and
+// mul_pow2_w (multiply by 2^K, e.g. 1/d = 2^(61 - log d)) against mul.
 #include <cstdio>
 #include <random>
 #include "dft.h"
@@ -65,6 +66,19 @@ static int check_root(int trials) {
     return bad;
 }
 
+template <int K>
+static int check_pow2(int trials) {
+    int bad = 0;
+    const fe w = gf::mk((uint64_t)1 << K, 0);
+    for (int t = 0; t < trials; ++t) {
+        const fe x = gf::mk(component((int)(rng() % 5)), component((int)(rng() % 5)));
+        const fe y = gf::mul_pow2_w<K>(x);
+        if (y.re > W || y.im > W || !gf::eq(gf::canon_w(y), gf::mul(w, gf::canon_w(x)))) ++bad;
+    }
+    if (bad) printf("mul_pow2_w<%d>: %d bad\n", K, bad);
+    return bad;
+}
+
 int main() {
     int bad = 0;
     bad += check_dft<2, false>(20000) + check_dft<2, true>(20000);
@@ -75,6 +89,8 @@ int main() {
     bad += check_root<16, 6>(50000) + check_root<16, 8>(50000) + check_root<16, 10>(50000);
     bad += check_root<16, 12>(50000) + check_root<16, 14>(50000) + check_root<16, -9>(50000);
     bad += check_root<8, 1>(50000) + check_root<8, 3>(50000) + check_root<8, 5>(50000) + check_root<8, 7>(50000);
+    bad += check_pow2<1>(50000) + check_pow2<30>(50000) + check_pow2<31>(50000) + check_pow2<32>(50000);
+    bad += check_pow2<57>(50000) + check_pow2<58>(50000) + check_pow2<59>(50000) + check_pow2<60>(50000);
     int mbad = 0;
     for (int t = 0; t < 200000; ++t) {
         const fe a = gf::mk(rng() % gf::P, rng() % gf::P);
