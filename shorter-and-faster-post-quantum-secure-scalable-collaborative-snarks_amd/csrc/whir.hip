@@ -645,7 +645,7 @@ int cosnark_whir_open(cosnark_ctx *ctx, const uint64_t *d_arr, uint64_t n, const
     // layout (elements of 16 bytes unless noted)
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) / 256 * 256; return o; };
-    const size_t o_arrA = take(16 * n), o_bA = take(16 * n);
+    const size_t o_arrA = take(16 * (n / 4 > 0 ? n / 4 : 1)), o_bA = take(16 * n);   // arrA: odd iterations' output
     const size_t o_arrB = take(16 * (n / 4)), o_bB = take(16 * (n / 4));
     const size_t o_t1 = take(16 * (n / 2)), o_t2 = take(16 * (n / 2));
     const size_t o_cwX = take(16 * (cw0 / 2)), o_cwY = take(16 * (cw0 / 4 > 8 ? cw0 / 4 : 8));
@@ -708,18 +708,21 @@ int cosnark_whir_open(cosnark_ctx *ctx, const uint64_t *d_arr, uint64_t n, const
     if (rc) return rc;
 
     // arr copy and b = precompute_beta(z)
-    HIPCK(hipMemcpyAsync(arrA, d_arr, 16 * n, hipMemcpyDeviceToDevice, s));
     hipLaunchKernelGGL(beta_kernel, dim3(1024), dim3(WT), 0, s, (const fe *)dlo, (const fe *)dhi, bA, n, lb);
     LAUNCHCK();
 
-    fe *arr = arrA, *b = bA;
+    // iteration 0 reads the caller's array in place (the rounds only read it;
+    // the fold writes the other buffer), then the iterations alternate
+    // (arrB, bB) and (arrA, bA)
+    const fe *arr = (const fe *)d_arr;
+    fe *b = bA;
     const fe *cw_old = (const fe *)d_codeword;
     const uint8_t *nd_old = d_nodes;
     for (size_t t = 0; t < its.size(); ++t) {
         const Iter &it = its[t];
         const int r0 = (int)(2 * t), r1 = r0 + 1;
-        fe *arr_n = (arr == arrA) ? arrB : arrA;
-        fe *b_n = (arr == arrA) ? bB : bA;
+        fe *arr_n = (t % 2 == 0) ? arrB : arrA;
+        fe *b_n = (t % 2 == 0) ? bB : bA;
         // whir_sumcheck: round r0 on (arr, b), round r1 folded by r[r0], fold by r[r1]
         int nb0 = 0, nb1 = 0;
         if ((rc = sumcheck_round_launch(ctx, arr, b, nullptr, nullptr, it.n / 2, r0, false, 0, 0, s, &nb0))) return rc;
