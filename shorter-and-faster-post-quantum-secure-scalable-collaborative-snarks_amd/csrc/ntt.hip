@@ -113,6 +113,7 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
     const bool leaves_on = LAST && p.leaves && by == 0 && items2 <= NTT_THREADS;
     fe v[R2];
     int my_c = 0, my_k1 = 0;
+    fe *tz = Y + (size_t)R * CP;   // fan-out: w_(E R)^(d k0) for the current k0, d < R
     for (int k0 = 0; k0 < (1 << fan); ++k0) {
     if (k0 > 0) __syncthreads();   // the previous k0's step 2 is done with Y
 
@@ -135,8 +136,7 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
             const uint64_t d = (uint64_t)(R2 * d1 + d2);
             const uint64_t pos = base + (LAST ? d : (d << logS));
             v[d1] = pos < p.in_len ? in[pos] : gf::zero();
-            if (k0 > 0)   // twist by w_(E R)^(d k0); inputs are canonical
-                v[d1] = gf::mul_w(tw28(twlo, twhi, (uint32_t)(d * k0) << (NTT_LOG_MAX - LOGR - fan)), v[d1]);
+            if (k0 > 0) v[d1] = gf::mul_w(tz[d], v[d1]);   // twist by w_(E R)^(d k0); inputs are canonical
         }
         dif_w<R1, INV>(v);
         static_for<0, R1>([&](auto ii) {
@@ -148,6 +148,12 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
         });
     }
     __syncthreads();
+
+    // the next k0's twists, while step 2 runs (step 1 is done with tz; the
+    // barrier at the top of the next k0 orders these writes before its reads)
+    if (k0 + 1 < (1 << fan))
+        for (int d = tid; d < R; d += NTT_THREADS)
+            tz[d] = tw28(twlo, twhi, (uint32_t)(d * (k0 + 1)) << (NTT_LOG_MAX - LOGR - fan));
 
     // ---- step 2: R2-point DFTs over d2 for each (k1, c), twiddle, store ----
     for (int it = tid; it < items2; it += NTT_THREADS) {
@@ -249,7 +255,8 @@ int launch_pass(cosnark_ctx *ctx, hipStream_t stream, uint64_t blocks, int batch
                 const NttPass &q, bool last, bool inverse, const fe *twlo, const fe *twhi,
                 const fe *rtab, fe scale) {
     PassKernel k = pass_kernel<R1, R2>(last, inverse);
-    const size_t lds = sizeof(fe) * ((size_t)R1 * R2 * ((1u << q.logC) + Y_PAD + 1));
+    // radix table, padded tile, and for a fan-out pass the R twists
+    const size_t lds = sizeof(fe) * ((size_t)R1 * R2 * ((1u << q.logC) + Y_PAD + 1 + (q.fan ? 1 : 0)));
     hipLaunchKernelGGL(k, dim3((unsigned)blocks, (unsigned)batch), dim3(NTT_THREADS), lds, stream, in, out, q, twlo,
                        twhi, rtab, scale);
     LAUNCHCK();
@@ -277,8 +284,9 @@ int dispatch_pass(cosnark_ctx *ctx, hipStream_t stream, int r, uint64_t blocks, 
 }  // namespace
 
 int ntt_init(cosnark_ctx *ctx) {
-    // the largest tile: 4096 elements + padding + the radix table (~74 KiB)
-    const int bytes = (int)(sizeof(fe) * (NTT_LDS_ELEMS + 256 * (Y_PAD + 1)));
+    // the largest tile: 4096 elements + padding + the radix table + a fan-out
+    // pass's twists (~76 KiB)
+    const int bytes = (int)(sizeof(fe) * (NTT_LDS_ELEMS + 256 * (Y_PAD + 2)));
     for (int r = 1; r <= NTT_PASS_LOG_MAX; ++r)
         for (int last = 0; last < 2; ++last)
             for (int inv = 0; inv < 2; ++inv)
