@@ -184,6 +184,10 @@ int sumcheck_round_launch(cosnark_ctx *ctx, const gf::fe *s1, const gf::fe *s2, 
                           uint64_t npairs, int round, bool fold, int fs_rounds, int nb_prev, hipStream_t s,
                           int *nb_out);
 // finalize `round` and fold both vectors by its challenge into nout elements
+// Rounds `round` and round + 1 with host-drawn challenges (ext_r), fused with
+// both folds: s hold 4 nq elements, d receive nq; finalized on the device.
+int sumcheck_round2_launch(cosnark_ctx *ctx, const gf::fe *s1, const gf::fe *s2, gf::fe *d1, gf::fe *d2, uint64_t nq,
+                           int round, hipStream_t stream);
 int sumcheck_fold_launch(cosnark_ctx *ctx, const gf::fe *s1, const gf::fe *s2, gf::fe *d1, gf::fe *d2, uint64_t nout,
                          int round, int fs_rounds, int nb_prev, hipStream_t s);
 int ntt_init(cosnark_ctx *ctx);

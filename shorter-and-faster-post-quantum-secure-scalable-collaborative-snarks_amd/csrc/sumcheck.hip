@@ -522,6 +522,83 @@ __global__ __launch_bounds__(SC_THREADS) void sc_fold_kernel(
     }
 }
 
+// Two rounds whose challenges are known in advance (round >= fs_rounds: r =
+// ext_r, drawn by the host), fused with both folds: per quadruple j of each
+// vector, round `round`'s terms on its pairs (x0, x1) and (x2, x3), the fold
+// by r0 to (e, o), round + 1's terms on (e, o), and d[j] = fold(e, o, r1).
+// One read of 4 nq elements and one write of nq per vector instead of two
+// reads, a temporary and a fold.  Partials of the two rounds go to part0 and
+// part1; sc_finish2_kernel finalizes both.
+__global__ __launch_bounds__(SC_THREADS) void sc_round2_kernel(const fe *__restrict__ s1, const fe *__restrict__ s2,
+                                                               fe *__restrict__ d1, fe *__restrict__ d2, uint64_t nq,
+                                                               const ScState *__restrict__ st, int round,
+                                                               fe *__restrict__ part0, fe *__restrict__ part1) {
+    __shared__ fe red[3][SC_THREADS / 64];
+    const fe r0 = st->ext_r[round], r1 = st->ext_r[round + 1];
+    fe A0 = gf::zero(), C0 = gf::zero(), O0 = gf::zero(), A1 = gf::zero(), C1 = gf::zero(), O1 = gf::zero();
+    const uint64_t stride = (uint64_t)gridDim.x * SC_THREADS;
+    uint64_t j = (uint64_t)blockIdx.x * SC_THREADS + threadIdx.x;
+    fe x[4], y[4];
+    auto load = [&](uint64_t q) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) { x[i] = s1[4 * q + i]; y[i] = s2[4 * q + i]; }
+    };
+    if (j < nq) load(j);
+    while (j < nq) {
+        const fe x0 = x[0], x1 = x[1], x2 = x[2], x3 = x[3], y0 = y[0], y1 = y[1], y2 = y[2], y3 = y[3];
+        const uint64_t jn = j + stride;
+        if (jn < nq) load(jn);                  // the next quadruple's loads ahead of this one's arithmetic
+        A0 = gf::add(A0, gf::add(gf::mul(gf::sub(x1, x0), gf::sub(y1, y0)), gf::mul(gf::sub(x3, x2), gf::sub(y3, y2))));
+        C0 = gf::add(C0, gf::add(gf::mul(x0, y0), gf::mul(x2, y2)));
+        O0 = gf::add(O0, gf::add(gf::mul(x1, y1), gf::mul(x3, y3)));
+        const fe e1 = fold1(x0, x1, r0), o1 = fold1(x2, x3, r0);
+        const fe e2 = fold1(y0, y1, r0), o2 = fold1(y2, y3, r0);
+        A1 = gf::add(A1, gf::mul(gf::sub(o1, e1), gf::sub(o2, e2)));
+        C1 = gf::add(C1, gf::mul(e1, e2));
+        O1 = gf::add(O1, gf::mul(o1, o2));
+        d1[j] = fold1(e1, o1, r1);
+        d2[j] = fold1(e2, o2, r1);
+        j = jn;
+    }
+    block_reduce3<SC_THREADS>(A0, C0, O0, red);
+    if (threadIdx.x == 0) {
+        part0[3 * blockIdx.x] = A0;
+        part0[3 * blockIdx.x + 1] = C0;
+        part0[3 * blockIdx.x + 2] = O0;
+    }
+    block_reduce3<SC_THREADS>(A1, C1, O1, red);
+    if (threadIdx.x == 0) {
+        part1[3 * blockIdx.x] = A1;
+        part1[3 * blockIdx.x + 1] = C1;
+        part1[3 * blockIdx.x + 2] = O1;
+    }
+}
+
+// Finalize rounds `round` and round + 1 of sc_round2_kernel: both rounds'
+// partials are loaded and reduced together, then the two rounds are
+// finalized in order on the same lanes (the transcript state passes from one
+// to the next in registers).
+__global__ __launch_bounds__(SC_THREADS) void sc_finish2_kernel(ScState *st, int round, int fs_rounds,
+                                                                const fe *part0, const fe *part1, int nb) {
+    __shared__ fe red[3][SC_THREADS / 64];
+    fe A0 = gf::zero(), C0 = gf::zero(), O0 = gf::zero(), A1 = gf::zero(), C1 = gf::zero(), O1 = gf::zero();
+    for (int b = threadIdx.x; b < nb; b += SC_THREADS) {
+        A0 = gf::add(A0, part0[3 * b]);
+        C0 = gf::add(C0, part0[3 * b + 1]);
+        O0 = gf::add(O0, part0[3 * b + 2]);
+        A1 = gf::add(A1, part1[3 * b]);
+        C1 = gf::add(C1, part1[3 * b + 1]);
+        O1 = gf::add(O1, part1[3 * b + 2]);
+    }
+    block_reduce3<SC_THREADS>(A0, C0, O0, red);
+    block_reduce3<SC_THREADS>(A1, C1, O1, red);
+    if (threadIdx.x < 4) {
+        fe prev = st->tr_prev[round], y = st->tr_y[round];
+        finalize_round(st, round, fs_rounds, st->derive_y, prev, y, A0, C0, O0, true);
+        finalize_round(st, round + 1, fs_rounds, st->derive_y, prev, y, A1, C1, O1, true);
+    }
+}
+
 __global__ void sc_init_kernel(ScState *st, fe prev, fe y, int derive_y) {
     st->tr_prev[0] = prev;
     st->tr_y[0] = y;
@@ -602,6 +679,21 @@ int sumcheck_round_launch(cosnark_ctx *ctx, const fe *s1, const fe *s2, fe *d1, 
         hipLaunchKernelGGL(sc_round_kernel<false>, dim3((unsigned)blocks), dim3(SC_THREADS), 0, stream, J);
     LAUNCHCK();
     *nb_out = (int)blocks;
+    return COSNARK_OK;
+}
+
+int sumcheck_round2_launch(cosnark_ctx *ctx, const fe *s1, const fe *s2, fe *d1, fe *d2, uint64_t nq, int round,
+                           hipStream_t stream) {
+    fe *part[2] = {ctx->sc_partials, ctx->sc_partials + 3 * SC_MAX_BLOCKS};
+    uint64_t blocks = (nq + SC_THREADS - 1) / SC_THREADS;
+    if (blocks > 1024) blocks = 1024;
+    if (blocks == 0) blocks = 1;
+    hipLaunchKernelGGL(sc_round2_kernel, dim3((unsigned)blocks), dim3(SC_THREADS), 0, stream, s1, s2, d1, d2, nq,
+                       (const ScState *)ctx->sc, round, part[0], part[1]);
+    LAUNCHCK();
+    hipLaunchKernelGGL(sc_finish2_kernel, dim3(1), dim3(SC_THREADS), 0, stream, ctx->sc, round, 0, (const fe *)part[0],
+                       (const fe *)part[1], (int)blocks);
+    LAUNCHCK();
     return COSNARK_OK;
 }
 

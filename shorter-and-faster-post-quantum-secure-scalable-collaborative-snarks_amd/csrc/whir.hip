@@ -498,6 +498,7 @@ int whir_schedule(const uint64_t *sizes, int count, int whir_k, int *kap, int *a
 // n/2^k entries in (arr_n, b_n); t[0..1] hold n/2, t[2..3] n/4 elements
 int whir_rounds(cosnark_ctx *ctx, const fe *arr, const fe *b, fe *arr_n, fe *b_n, uint64_t n, int k, int r0,
                 fe *const *t, hipStream_t s) {
+    if (k == 2) return sumcheck_round2_launch(ctx, arr, b, arr_n, b_n, n / 4, r0, s);
     int nb = 0, rc;
     if ((rc = sumcheck_round_launch(ctx, arr, b, nullptr, nullptr, n / 2, r0, false, 0, 0, s, &nb))) return rc;
     const fe *s1 = arr, *s2 = b;
@@ -647,7 +648,6 @@ int cosnark_whir_open(cosnark_ctx *ctx, const uint64_t *d_arr, uint64_t n, const
     auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) / 256 * 256; return o; };
     const size_t o_arrA = take(16 * (n / 4 > 0 ? n / 4 : 1)), o_bA = take(16 * n);   // arrA: odd iterations' output
     const size_t o_arrB = take(16 * (n / 4)), o_bB = take(16 * (n / 4));
-    const size_t o_t1 = take(16 * (n / 2)), o_t2 = take(16 * (n / 2));
     const size_t o_cwX = take(16 * (cw0 / 2)), o_cwY = take(16 * (cw0 / 4 > 8 ? cw0 / 4 : 8));
     const size_t o_ndX = take(32 * cosnark_merkle_node_count(cw0 / 2));
     const size_t o_ndY = take(32 * cosnark_merkle_node_count(cw0 / 4 > 8 ? cw0 / 4 : 8));
@@ -661,7 +661,6 @@ int cosnark_whir_open(cosnark_ctx *ctx, const uint64_t *d_arr, uint64_t n, const
     if (rc) return rc;
     char *B = (char *)base;
     fe *arrA = (fe *)(B + o_arrA), *bA = (fe *)(B + o_bA), *arrB = (fe *)(B + o_arrB), *bB = (fe *)(B + o_bB);
-    fe *t1 = (fe *)(B + o_t1), *t2 = (fe *)(B + o_t2);
     fe *cwX = (fe *)(B + o_cwX), *cwY = (fe *)(B + o_cwY);
     uint8_t *ndX = (uint8_t *)(B + o_ndX), *ndY = (uint8_t *)(B + o_ndY);
     fe *dlo = (fe *)(B + o_lo), *dhi = (fe *)(B + o_hi);
@@ -720,14 +719,11 @@ int cosnark_whir_open(cosnark_ctx *ctx, const uint64_t *d_arr, uint64_t n, const
     const uint8_t *nd_old = d_nodes;
     for (size_t t = 0; t < its.size(); ++t) {
         const Iter &it = its[t];
-        const int r0 = (int)(2 * t), r1 = r0 + 1;
+        const int r0 = (int)(2 * t);
         fe *arr_n = (t % 2 == 0) ? arrB : arrA;
         fe *b_n = (t % 2 == 0) ? bB : bA;
-        // whir_sumcheck: round r0 on (arr, b), round r1 folded by r[r0], fold by r[r1]
-        int nb0 = 0, nb1 = 0;
-        if ((rc = sumcheck_round_launch(ctx, arr, b, nullptr, nullptr, it.n / 2, r0, false, 0, 0, s, &nb0))) return rc;
-        if ((rc = sumcheck_round_launch(ctx, arr, b, t1, t2, it.n / 4, r1, true, 0, nb0, s, &nb1))) return rc;
-        if ((rc = sumcheck_fold_launch(ctx, t1, t2, arr_n, b_n, it.n / 4, r1, 0, nb1, s))) return rc;
+        // whir_sumcheck: rounds r0 and r0 + 1 on (arr, b) and both folds, one pass
+        if ((rc = sumcheck_round2_launch(ctx, arr, b, arr_n, b_n, it.n / 4, r0, s))) return rc;
         // Whir_Commit_initial at 2*rate
         fe *cw_new = (t % 2 == 0) ? cwX : cwY;
         uint8_t *nd_new = (t % 2 == 0) ? ndX : ndY;
