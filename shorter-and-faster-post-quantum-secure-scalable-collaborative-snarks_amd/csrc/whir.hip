@@ -407,43 +407,68 @@ int betas_launch(cosnark_ctx *ctx, fe *b, uint64_t nb, const uint32_t *dqs, cons
     return COSNARK_OK;
 }
 
-// get_RS_multifold_segment_tensor (src/coPCS_utils.cpp:333-383) for one
-// whir_k-element segment at position idx of a codeword of n elements
-fe segment_fold(const fe *seg, int seg_n, const fe *r, int nr, uint64_t n, uint64_t idx) {
-    const fe one = gf::one();
-    const fe len_inv = gf::inv(gf::from_u64((uint64_t)seg_n));
-    const int log_fold = ilog2h((uint64_t)seg_n);
-    const int logn = ilog2h(n);
-    fe omega = gf::pow(gf::inv(gf::root_of_unity(logn)), n / (uint64_t)seg_n);
-    fe inv_omegas = gf::pow(gf::inv(gf::root_of_unity(logn)), idx);
-    fe part[64];
-    for (int j = 0; j < seg_n / 2; ++j) {
-        const fe s = gf::add(seg[j], seg[j + seg_n / 2]);
-        const fe d = gf::sub(seg[j], seg[j + seg_n / 2]);
-        part[j] = gf::add(gf::mul(gf::sub(one, r[0]), s), gf::mul(gf::mul(r[0], inv_omegas), d));
-        inv_omegas = gf::mul(inv_omegas, omega);
-    }
-    if (nr > 1) {
-        fe pw[16];
-        for (int i = 0; i < log_fold; ++i) pw[i] = one;
+// get_RS_multifold_segment_tensor (src/coPCS_utils.cpp:333-383) for the
+// queries of one iteration: what depends only on (seg_n, n) -- 1 / seg_n, the
+// roots' inverses and powers -- is computed once, and each query's power
+// w_n^-idx comes from a table of squares (about log2(n) / 2 multiplies
+// instead of inversions and exponentiations per query)
+struct SegFold {
+    int seg_n, log_fold, logn;
+    fe len_inv;          // 1 / seg_n
+    fe omega;            // w_n^-(n / seg_n)
+    fe pw[16];           // level i's step, w_n^-(2^i n / seg_n)
+    fe tinv[64];         // w_n^-(2^k), k < logn
+    SegFold(int seg_n_, uint64_t n) : seg_n(seg_n_), log_fold(ilog2h((uint64_t)seg_n_)), logn(ilog2h(n)) {
         const fe om = gf::inv(gf::root_of_unity(logn));
+        len_inv = gf::inv(gf::from_u64((uint64_t)seg_n));
+        omega = gf::pow(om, n / (uint64_t)seg_n);
+        for (int i = 0; i < 16; ++i) pw[i] = gf::one();
         if (log_fold > 1) pw[1] = gf::mul(om, om);
         for (int i = 2; i < log_fold; ++i) pw[i] = gf::mul(pw[i - 1], pw[i - 1]);
         for (int i = 1; i < log_fold; ++i) pw[i] = gf::pow(pw[i], n / (uint64_t)seg_n);
-        int sn = seg_n;
-        for (int i = 1; i < log_fold; ++i) {
-            sn /= 2;
-            fe pow_omega = gf::pow(gf::inv(gf::root_of_unity(ilog2h(n >> i))), idx);
-            for (int j = 0; j < sn / 2; ++j) {
-                const fe s = gf::add(part[j], part[j + sn / 2]);
-                const fe d = gf::sub(part[j], part[j + sn / 2]);
-                part[j] = gf::add(gf::mul(gf::sub(one, r[i]), s), gf::mul(r[i], gf::mul(pow_omega, d)));
-                pow_omega = gf::mul(pow_omega, pw[i]);
+        tinv[0] = om;
+        for (int k = 1; k < logn && k < 64; ++k) tinv[k] = gf::mul(tinv[k - 1], tinv[k - 1]);
+    }
+    // w_n^-idx (idx < n) from the squares table
+    fe pow_idx(uint64_t idx) const {
+        fe x = gf::one();
+        for (int k = 0; idx; ++k, idx >>= 1)
+            if (idx & 1) x = gf::mul(x, tinv[k]);
+        return x;
+    }
+    // get_RS_multifold_segment_tensor (src/coPCS_utils.cpp:333-383) for one
+    // seg_n-element segment at position idx of a codeword of n elements; the
+    // level-i power w_(n/2^i)^-idx is (w_n^-idx)^(2^i), the roots being
+    // powers of one another
+    fe fold(const fe *seg, const fe *r, int nr, uint64_t idx) const {
+        const fe one = gf::one();
+        const fe w0 = pow_idx(idx);
+        fe inv_omegas = w0;
+        fe part[64];
+        for (int j = 0; j < seg_n / 2; ++j) {
+            const fe s = gf::add(seg[j], seg[j + seg_n / 2]);
+            const fe d = gf::sub(seg[j], seg[j + seg_n / 2]);
+            part[j] = gf::add(gf::mul(gf::sub(one, r[0]), s), gf::mul(gf::mul(r[0], inv_omegas), d));
+            inv_omegas = gf::mul(inv_omegas, omega);
+        }
+        if (nr > 1) {
+            int sn = seg_n;
+            fe wi = w0;
+            for (int i = 1; i < log_fold; ++i) {
+                sn /= 2;
+                wi = gf::mul(wi, wi);            // w_(n/2^i)^-idx
+                fe pow_omega = wi;
+                for (int j = 0; j < sn / 2; ++j) {
+                    const fe s = gf::add(part[j], part[j + sn / 2]);
+                    const fe d = gf::sub(part[j], part[j + sn / 2]);
+                    part[j] = gf::add(gf::mul(gf::sub(one, r[i]), s), gf::mul(r[i], gf::mul(pow_omega, d)));
+                    pow_omega = gf::mul(pow_omega, pw[i]);
+                }
             }
         }
+        return gf::mul(len_inv, part[0]);
     }
-    return gf::mul(len_inv, part[0]);
-}
+};
 
 int query_count(int rate) {   // Whir.cpp:253-258, conjectured_security = false
     return (int)(-100.0 / log2(sqrt(1.0 / (double)rate)));
@@ -787,10 +812,11 @@ int cosnark_whir_open(cosnark_ctx *ctx, const uint64_t *d_arr, uint64_t n, const
         put((uint64_t)it.l);
         for (int i = 0; i < it.l; ++i) put(it.q[i]);
         std::vector<fe> y(it.l);
+        const SegFold sf(WK, it.cw_n);
         for (int i = 0; i < it.l; ++i) {
             const fe *seg = h_rep.data() + it.rep_off + (size_t)i * WK;
             for (int j = 0; j < WK; ++j) putfe(seg[j]);
-            y[i] = segment_fold(seg, WK, it.r, 2, it.cw_n, it.q[i]);
+            y[i] = sf.fold(seg, it.r, 2, it.q[i]);
         }
         for (int i = 0; i < it.l; ++i) putfe(y[i]);
         for (int i = 0; i < it.l * it.depth; ++i) put32(h_paths.data() + 32 * (it.path_off + i));
@@ -1199,10 +1225,11 @@ int cosnark_whir_batch_open(cosnark_ctx *ctx, int count, const uint64_t *const *
                 put((uint64_t)st.l);
                 for (int i = 0; i < st.l; ++i) put(st.q[i]);
                 std::vector<fe> y(st.l);
+                const SegFold sf(st.kappa, st.cw_n);
                 for (int i = 0; i < st.l; ++i) {
                     const fe *seg = h_rep.data() + st.rep_off + (size_t)i * st.kappa;
                     for (int j = 0; j < st.kappa; ++j) putfe(seg[j]);
-                    y[i] = segment_fold(seg, st.kappa, st.r, st.k, st.cw_n, st.q[i]);
+                    y[i] = sf.fold(seg, st.r, st.k, st.q[i]);
                 }
                 for (int i = 0; i < st.l; ++i) putfe(y[i]);
                 for (size_t i = 0; i < (size_t)st.l * st.depth; ++i) put32(h_paths.data() + 32 * (st.path_off + i));
