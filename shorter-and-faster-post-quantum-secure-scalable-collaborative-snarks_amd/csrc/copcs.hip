@@ -872,17 +872,20 @@ void host_fft(std::vector<fe> &a, bool inverse) {
         j ^= bit;
         if (i < j) std::swap(a[i], a[j]);
     }
+    // one table w_n^k (k < n/2; w_n^-k for the inverse): stage `len` uses
+    // every (n/len)-th entry, since root_of_unity(lg) = root_of_unity(logn)^(n/len)
+    std::vector<fe> tw(n > 1 ? n / 2 : 1);
+    fe wn = gf::root_of_unity(logn);
+    if (inverse) wn = gf::inv(wn);
+    tw[0] = gf::one();
+    for (int k = 1; k < n / 2; ++k) tw[k] = gf::mul(tw[k - 1], wn);
     for (int lg = 1; lg <= logn; ++lg) {
-        const int len = 1 << lg;
-        fe wl = gf::root_of_unity(lg);
-        if (inverse) wl = gf::inv(wl);
+        const int len = 1 << lg, step = n >> lg;
         for (int s = 0; s < n; s += len) {
-            fe w = gf::one();
             for (int k = 0; k < len / 2; ++k) {
-                const fe u = a[s + k], v = gf::mul(a[s + k + len / 2], w);
+                const fe u = a[s + k], v = k ? gf::mul(a[s + k + len / 2], tw[k * step]) : a[s + k + len / 2];
                 a[s + k] = gf::add(u, v);
                 a[s + k + len / 2] = gf::sub(u, v);
-                w = gf::mul(w, wl);
             }
         }
     }
