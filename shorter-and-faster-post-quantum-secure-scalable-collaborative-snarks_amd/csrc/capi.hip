@@ -34,6 +34,20 @@ int scratch_reserve(cosnark_ctx *ctx, size_t bytes, void **base) {
     return COSNARK_OK;
 }
 
+int stage_reserve(cosnark_ctx *ctx, size_t bytes, void **base) {
+    if (bytes > ctx->stage_bytes) {
+        // every earlier download through it has been synchronized
+        if (ctx->stage_host) HIPCK(hipHostFree(ctx->stage_host));
+        ctx->stage_host = nullptr;
+        ctx->stage_bytes = 0;
+        const size_t want = bytes > ((size_t)1 << 20) ? bytes : ((size_t)1 << 20);
+        HIPCK(hipHostMalloc(&ctx->stage_host, want, hipHostMallocDefault));
+        ctx->stage_bytes = want;
+    }
+    *base = ctx->stage_host;
+    return COSNARK_OK;
+}
+
 int ntt_tmp_reserve(cosnark_ctx *ctx, size_t bytes, void **base) {
     if (bytes > ctx->ntt_tmp_bytes) {
         // work on either stream may still use the old buffer
@@ -177,6 +191,7 @@ void cosnark_destroy(cosnark_ctx *ctx) {
     (void)hipFree(ctx->sc); (void)hipFree(ctx->sc_partials);
     if (ctx->sc_host) (void)hipHostFree(ctx->sc_host);
     if (ctx->co_host) (void)hipHostFree(ctx->co_host);
+    if (ctx->stage_host) (void)hipHostFree(ctx->stage_host);
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     if (ctx->ntt_tmp) (void)hipFree(ctx->ntt_tmp);
     if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);

@@ -49,6 +49,8 @@ struct cosnark_ctx {
     ScState *sc_host = nullptr;       // pinned host mirror for copy-in / copy-out
     ScState *sc_host_dev = nullptr;   // the same memory as a device address
     gf::fe *co_host = nullptr;        // pinned staging, CO_HOST_ELEMS elements (small results, weights)
+    void *stage_host = nullptr;       // pinned staging for multi-part downloads (grows on demand)
+    size_t stage_bytes = 0;
     gf::fe *sc_partials = nullptr;    // device, 2 x 3 * SC_MAX_BLOCKS elements (ping-pong)
     hipEvent_t ev[8] = {};
     int ntt_tile = NTT_LDS_ELEMS;     // elements per NTT workgroup tile (power of two, <= 4096)
@@ -69,6 +71,8 @@ int hip_err(cosnark_ctx *ctx, hipError_t e, const char *what);
 // ensure the scratch arena holds at least `bytes`; returns base pointer
 int scratch_reserve(cosnark_ctx *ctx, size_t bytes, void **base);
 int ntt_tmp_reserve(cosnark_ctx *ctx, size_t bytes, void **base);
+// pinned host staging of at least `bytes` (kept; replaced only when larger)
+int stage_reserve(cosnark_ctx *ctx, size_t bytes, void **base);
 
 #define HIPCK(expr)                                              \
     do {                                                         \

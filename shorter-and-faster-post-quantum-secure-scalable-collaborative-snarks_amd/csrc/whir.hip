@@ -538,6 +538,30 @@ int whir_rounds(cosnark_ctx *ctx, const fe *arr, const fe *b, fe *arr_n, fe *b_n
     return sumcheck_fold_launch(ctx, s1, s2, arr_n, b_n, n >> k, r0 + k - 1, 0, nb, s);
 }
 
+// Device-to-host copies gathered through the context's pinned staging: each
+// add() records a part; run() enqueues every copy asynchronously, waits
+// once, and copies the parts out.  Pageable destinations would cost a
+// synchronous staged copy each.
+struct Downloads {
+    struct Part { void *dst; const void *src; size_t off, bytes; };
+    std::vector<Part> parts;
+    size_t total = 0;
+    void add(void *dst, const void *src, size_t bytes) {
+        parts.push_back({dst, src, total, bytes});
+        total += (bytes + 255) / 256 * 256;
+    }
+    int run(cosnark_ctx *ctx, hipStream_t s) {
+        void *st;
+        int rc = stage_reserve(ctx, total ? total : 1, &st);
+        if (rc) return rc;
+        for (const Part &p : parts)
+            HIPCK(hipMemcpyAsync((char *)st + p.off, p.src, p.bytes, hipMemcpyDeviceToHost, s));
+        HIPCK(hipStreamSynchronize(s));
+        for (const Part &p : parts) memcpy(p.dst, (const char *)st + p.off, p.bytes);
+        return COSNARK_OK;
+    }
+};
+
 struct Iter {
     uint64_t n;          // |arr| entering the iteration
     int rate;            // rate entering the iteration
@@ -776,17 +800,18 @@ int cosnark_whir_open(cosnark_ctx *ctx, const uint64_t *d_arr, uint64_t n, const
     std::vector<fe> h_rep(rep_total), fin_arr(n_fin), fin_b(n_fin);
     std::vector<uint8_t> h_paths(32 * path_total);
     std::vector<uint8_t> roots(32 * its.size());
+    Downloads dl;
     for (size_t t = 0; t < its.size(); ++t) {
         const uint8_t *nd = (t % 2 == 0) ? ndX : ndY;
         const uint64_t cnt = cosnark_merkle_node_count(its[t].cw_new_n);
-        HIPCK(hipMemcpyAsync(roots.data() + 32 * t, nd + 32 * (cnt - 1), 32, hipMemcpyDeviceToHost, s));
+        dl.add(roots.data() + 32 * t, nd + 32 * (cnt - 1), 32);
     }
-    HIPCK(hipMemcpyAsync(h_rep.data(), rep, sizeof(fe) * rep_total, hipMemcpyDeviceToHost, s));
-    HIPCK(hipMemcpyAsync(h_paths.data(), paths, 32 * path_total, hipMemcpyDeviceToHost, s));
-    HIPCK(hipMemcpyAsync(fin_arr.data(), arr, sizeof(fe) * n_fin, hipMemcpyDeviceToHost, s));
-    HIPCK(hipMemcpyAsync(fin_b.data(), b, sizeof(fe) * n_fin, hipMemcpyDeviceToHost, s));
-    HIPCK(hipMemcpyAsync(ctx->sc_host, ctx->sc, sizeof(ScState), hipMemcpyDeviceToHost, s));
-    HIPCK(hipStreamSynchronize(s));
+    dl.add(h_rep.data(), rep, sizeof(fe) * rep_total);
+    dl.add(h_paths.data(), paths, 32 * path_total);
+    dl.add(fin_arr.data(), arr, sizeof(fe) * n_fin);
+    dl.add(fin_b.data(), b, sizeof(fe) * n_fin);
+    HIPCK(hipMemcpyAsync(ctx->sc_host, ctx->sc, sizeof(ScState), hipMemcpyDeviceToHost, s));   // pinned already
+    if ((rc = dl.run(ctx, s))) return rc;
     const ScState &st = *ctx->sc_host;
 
     // ---- claim chain (the reference's checks) and the record ----
@@ -915,6 +940,7 @@ int cosnark_whir_batch_open(cosnark_ctx *ctx, int count, const uint64_t *const *
     const size_t o_q = take(4 * STEP_Q * 64), o_qs = take(4 * STEP_Q * 64), o_c = take(16 * STEP_Q * 64);
     const size_t o_z = take(32 * 29 * 64);
     const size_t o_bs = take(sizeof(BetaSlots) * 64), o_P = take(16 * STEP_Q * WT);
+    const size_t o_roots = take(32 * 64);   // each iteration's new root, gathered for one download
     // replies and paths: bounded by the steps' sizes, reserved below once known
     void *base;
     int rc;
@@ -1064,6 +1090,7 @@ int cosnark_whir_batch_open(cosnark_ctx *ctx, int count, const uint64_t *const *
         uint8_t *dz = (uint8_t *)(B + o_z);
         BetaSlots *dbs = (BetaSlots *)(B + o_bs);
         fe *dP = (fe *)(B + o_P);
+        uint8_t *droots = (uint8_t *)(B + o_roots);
         fe *rep = (fe *)(B + o_rep);
         uint8_t *paths = (uint8_t *)(B + o_path);
         fe *parts = (fe *)(B + o_parts);
@@ -1131,7 +1158,8 @@ int cosnark_whir_batch_open(cosnark_ctx *ctx, int count, const uint64_t *const *
                                        ilog2(st.cw_new_n), slots[t], hbs[t].ng, dbs + t, dP, s)))
                     return rc;
                 const uint64_t cnt = cosnark_merkle_node_count(st.cw_new_n);
-                HIPCK(hipMemcpyAsync(roots.data() + 32 * t, nd_new + 32 * (cnt - 1), 32, hipMemcpyDeviceToHost, s));
+                // the node slot is reused two steps on: keep the root on the device
+                HIPCK(hipMemcpyAsync(droots + 32 * t, nd_new + 32 * (cnt - 1), 32, hipMemcpyDeviceToDevice, s));
                 arr = arr_n;
                 b = b_n;
                 cw_cur = cw_new;
@@ -1190,15 +1218,17 @@ int cosnark_whir_batch_open(cosnark_ctx *ctx, int count, const uint64_t *const *
         std::vector<fe> h_rep(rep_off + 1), h_parts(PARTS * (eval_off + 1)), fin_arr(fs.n), fin_b(fs.n);
         std::vector<fe> fin_cw(fcw), run_cw(fs.cw_n);
         std::vector<uint8_t> h_paths(32 * (path_off + 1));
-        HIPCK(hipMemcpyAsync(h_rep.data(), rep, sizeof(fe) * (rep_off + 1), hipMemcpyDeviceToHost, s));
-        HIPCK(hipMemcpyAsync(h_paths.data(), paths, 32 * (path_off + 1), hipMemcpyDeviceToHost, s));
-        HIPCK(hipMemcpyAsync(h_parts.data(), parts, sizeof(fe) * h_parts.size(), hipMemcpyDeviceToHost, s));
-        HIPCK(hipMemcpyAsync(fin_arr.data(), arr, sizeof(fe) * fs.n, hipMemcpyDeviceToHost, s));
-        HIPCK(hipMemcpyAsync(fin_b.data(), b, sizeof(fe) * fs.n, hipMemcpyDeviceToHost, s));
-        HIPCK(hipMemcpyAsync(fin_cw.data(), finbuf, sizeof(fe) * fcw, hipMemcpyDeviceToHost, s));
-        HIPCK(hipMemcpyAsync(run_cw.data(), cw_cur, sizeof(fe) * fs.cw_n, hipMemcpyDeviceToHost, s));
-        HIPCK(hipMemcpyAsync(ctx->sc_host, ctx->sc, sizeof(ScState), hipMemcpyDeviceToHost, s));
-        HIPCK(hipStreamSynchronize(s));
+        Downloads dl;
+        dl.add(roots.data(), droots, roots.size());
+        dl.add(h_rep.data(), rep, sizeof(fe) * (rep_off + 1));
+        dl.add(h_paths.data(), paths, 32 * (path_off + 1));
+        dl.add(h_parts.data(), parts, sizeof(fe) * h_parts.size());
+        dl.add(fin_arr.data(), arr, sizeof(fe) * fs.n);
+        dl.add(fin_b.data(), b, sizeof(fe) * fs.n);
+        dl.add(fin_cw.data(), finbuf, sizeof(fe) * fcw);
+        dl.add(run_cw.data(), cw_cur, sizeof(fe) * fs.cw_n);
+        HIPCK(hipMemcpyAsync(ctx->sc_host, ctx->sc, sizeof(ScState), hipMemcpyDeviceToHost, s));   // pinned already
+        if ((rc = dl.run(ctx, s))) return rc;
         const ScState &scs = *ctx->sc_host;
 
         // ---- the record and the reference's checks, in step order ----
