@@ -144,6 +144,7 @@ struct ScJob {
     int nb_in, nb;            // partials in part_in; this job's workgroups
     int round, m, fs_rounds;
     uint32_t first_block;
+    int kind;                 // 0: one round; 1, 2: rounds 0-2 fused (sc_pair01_kernel, sc_fold01_kernel)
 };
 
 // One multi-workgroup round `round`.  FOLD (round > 0): inputs hold
@@ -220,6 +221,136 @@ __device__ __forceinline__ void sc_round(const ScJob &J, uint32_t blk) {
 template <bool FOLD>
 __global__ __launch_bounds__(SC_THREADS) void sc_round_kernel(ScJob J) {
     sc_round<FOLD>(J, blockIdx.x);
+}
+
+// Rounds 0 and 1 from one read of the inputs.  For a quadruple (x0..x3) of
+// each vector, round 1 works on e = x0 + r0 (x1 - x0), o = x2 + r0 (x3 - x2)
+// with r0 still unknown, so each of its three terms is a quadratic in r0,
+// kept as its values at r0 = 0 and r0 = 1 and its r0^2 coefficient (nine
+// products).  Round 0's sums are sums of those: A0 = sum of the C and O r0^2
+// coefficients (the pair differences' products), C0 = sum of their values
+// at 0, O0 = sum of their values at 1.  J.npairs = quadruples; nine
+// partials per workgroup go to part_out.
+__global__ __launch_bounds__(SC_THREADS) void sc_pair01_kernel(ScJob J) {
+    __shared__ fe red[3][SC_THREADS / 64];
+    const fe *__restrict__ s1 = J.s1;
+    const fe *__restrict__ s2 = J.s2;
+    const uint64_t nq = J.npairs;
+    fe q[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) q[i] = gf::zero();
+    const uint64_t stride = (uint64_t)J.nb * SC_THREADS;
+    uint64_t j = (uint64_t)blockIdx.x * SC_THREADS + threadIdx.x;
+    fe x[4], y[4];
+    auto load = [&](uint64_t t) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) { x[i] = s1[4 * t + i]; y[i] = s2[4 * t + i]; }
+    };
+    if (j < nq) load(j);
+    while (j < nq) {
+        const fe x0 = x[0], x1 = x[1], x2 = x[2], x3 = x[3], y0 = y[0], y1 = y[1], y2 = y[2], y3 = y[3];
+        const uint64_t jn = j + stride;
+        if (jn < nq) load(jn);
+        const fe ax = gf::sub(x1, x0), bx = gf::sub(x3, x2), ay = gf::sub(y1, y0), by = gf::sub(y3, y2);
+        // A: (o - e)(o' - e'), o - e = (x2 - x0) + r0 (bx - ax)
+        q[0] = gf::add(q[0], gf::mul(gf::sub(x2, x0), gf::sub(y2, y0)));
+        q[1] = gf::add(q[1], gf::mul(gf::sub(x3, x1), gf::sub(y3, y1)));
+        q[2] = gf::add(q[2], gf::mul(gf::sub(bx, ax), gf::sub(by, ay)));
+        // C: e e' = (x0 + r0 ax)(y0 + r0 ay);  O: o o' = (x2 + r0 bx)(y2 + r0 by)
+        q[3] = gf::add(q[3], gf::mul(x0, y0));
+        q[4] = gf::add(q[4], gf::mul(x1, y1));
+        q[5] = gf::add(q[5], gf::mul(ax, ay));
+        q[6] = gf::add(q[6], gf::mul(x2, y2));
+        q[7] = gf::add(q[7], gf::mul(x3, y3));
+        q[8] = gf::add(q[8], gf::mul(bx, by));
+        j = jn;
+    }
+#pragma unroll
+    for (int g = 0; g < 3; ++g) {
+        block_reduce3<SC_THREADS>(q[3 * g], q[3 * g + 1], q[3 * g + 2], red);
+        if (threadIdx.x == 0) {
+            J.part_out[9 * blockIdx.x + 3 * g] = q[3 * g];
+            J.part_out[9 * blockIdx.x + 3 * g + 1] = q[3 * g + 1];
+            J.part_out[9 * blockIdx.x + 3 * g + 2] = q[3 * g + 2];
+        }
+    }
+}
+
+// Finalizes rounds 0 and 1 from sc_pair01_kernel's partials (in every
+// workgroup; workgroup 0 records them), then folds each 8-tuple of the
+// inputs by r0 and r1 into a pair (e, o) of the quarter-size outputs and
+// forms round 2's sums on it.  J.npairs = round 2's pairs (n / 8).
+__global__ __launch_bounds__(SC_THREADS) void sc_fold01_kernel(ScJob J) {
+    __shared__ fe red[3][SC_THREADS / 64];
+    __shared__ fe sh_r[2];
+    const fe *__restrict__ s1 = J.s1;
+    const fe *__restrict__ s2 = J.s2;
+    fe *__restrict__ d1 = J.d1;
+    fe *__restrict__ d2 = J.d2;
+    const uint64_t npairs = J.npairs;
+    const uint64_t stride = (uint64_t)J.nb * SC_THREADS;
+    uint64_t j = (uint64_t)blockIdx.x * SC_THREADS + threadIdx.x;
+    fe x[8], y[8];
+    auto load = [&](uint64_t t) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) { x[i] = s1[8 * t + i]; y[i] = s2[8 * t + i]; }
+    };
+    if (j < npairs) load(j);   // independent of the challenges: issue first
+    {
+        fe q[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) q[i] = gf::zero();
+        for (int b = threadIdx.x; b < J.nb_in; b += SC_THREADS)
+#pragma unroll
+            for (int i = 0; i < 9; ++i) q[i] = gf::add(q[i], J.part_in[9 * b + i]);
+#pragma unroll
+        for (int g = 0; g < 3; ++g) block_reduce3<SC_THREADS>(q[3 * g], q[3 * g + 1], q[3 * g + 2], red);
+        if (threadIdx.x < 4) {
+            __builtin_amdgcn_s_setprio(3);
+            ScState *st = J.st;
+            const bool writer = blockIdx.x == 0;
+            fe prev = st->tr_prev[0], yv = st->tr_y[0];
+            // round 0 from round 1's quadratics
+            const fe A0 = gf::add(q[5], q[8]), C0 = gf::add(q[3], q[6]), O0 = gf::add(q[4], q[7]);
+            const fe r0 = finalize_round(st, 0, J.fs_rounds, st->derive_y, prev, yv, A0, C0, O0, writer);
+            // a quadratic from its values at 0 and 1 and its square coefficient, at r0
+            auto at = [&](const fe &v0, const fe &v1, const fe &c2) {
+                const fe c1 = gf::sub(gf::sub(v1, v0), c2);
+                return gf::add(v0, gf::mul(r0, gf::add(c1, gf::mul(r0, c2))));
+            };
+            const fe A1 = at(q[0], q[1], q[2]), C1 = at(q[3], q[4], q[5]), O1 = at(q[6], q[7], q[8]);
+            const fe r1 = finalize_round(st, 1, J.fs_rounds, st->derive_y, prev, yv, A1, C1, O1, writer);
+            if (threadIdx.x == 0) { sh_r[0] = r0; sh_r[1] = r1; }
+            __builtin_amdgcn_s_setprio(0);
+        }
+        __syncthreads();
+    }
+    const fe r0 = sh_r[0], r1 = sh_r[1];
+    fe A = gf::zero(), C = gf::zero(), O = gf::zero();
+    while (j < npairs) {
+        fe fx[4], fy[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            fx[k] = fold1(x[2 * k], x[2 * k + 1], r0);
+            fy[k] = fold1(y[2 * k], y[2 * k + 1], r0);
+        }
+        const uint64_t jn = j + stride;
+        const fe e1 = fold1(fx[0], fx[1], r1), o1 = fold1(fx[2], fx[3], r1);
+        const fe e2 = fold1(fy[0], fy[1], r1), o2 = fold1(fy[2], fy[3], r1);
+        d1[2 * j] = e1; d1[2 * j + 1] = o1;
+        d2[2 * j] = e2; d2[2 * j + 1] = o2;
+        A = gf::add(A, gf::mul(gf::sub(o1, e1), gf::sub(o2, e2)));
+        C = gf::add(C, gf::mul(e1, e2));
+        O = gf::add(O, gf::mul(o1, o2));
+        j = jn;
+        if (j < npairs) load(j);
+    }
+    block_reduce3<SC_THREADS>(A, C, O, red);
+    if (threadIdx.x == 0) {
+        J.part_out[3 * blockIdx.x] = A;
+        J.part_out[3 * blockIdx.x + 1] = C;
+        J.part_out[3 * blockIdx.x + 2] = O;
+    }
 }
 
 template <bool FOLD>
@@ -717,7 +848,7 @@ namespace {
 // slot not holding its input first (a folded vector of m elements lands in a
 // slot of at least m), then alternates; it never writes v1 / v2.
 std::vector<ScJob> sumcheck_jobs(const fe *v1, const fe *v2, uint64_t n, int fs_rounds, fe *buf, fe *part0,
-                                 ScState *st, int max_blocks, uint64_t tail_elems) {
+                                 ScState *st, int max_blocks, uint64_t tail_elems, bool fuse01 = false) {
     const int m = ilog2(n);
     fe *a1 = buf, *a2 = buf + n / 2, *b1 = buf + n, *b2 = buf + n + n / 4;
     fe *part[2] = {part0, part0 + 3 * SC_MAX_BLOCKS};
@@ -727,6 +858,42 @@ std::vector<ScJob> sumcheck_jobs(const fe *v1, const fe *v2, uint64_t n, int fs_
     uint64_t len = n;        // elements in s1/s2
     int round = 0, which = 0, nb_prev = 0;
     bool pending = false;    // s1/s2 still need the fold with r[round-1]
+    // rounds 0-2 as two launches where all three are multi-workgroup rounds:
+    // round 2's folded pairs (n / 4 elements) land in outs[1]
+    if (fuse01 && n / 2 > tail_elems) {
+        auto width = [&](uint64_t items) {
+            uint64_t b = (items + SC_THREADS - 1) / SC_THREADS;
+            if (b > (uint64_t)max_blocks) b = max_blocks;
+            return (int)(b ? b : 1);
+        };
+        ScJob A{};
+        A.kind = 1;
+        A.s1 = v1; A.s2 = v2;
+        A.npairs = n / 4;
+        A.st = st;
+        A.part_out = part[1];
+        A.nb = width(n / 4);
+        A.round = 0; A.m = m; A.fs_rounds = fs_rounds;
+        jobs.push_back(A);
+        ScJob B{};
+        B.kind = 2;
+        B.s1 = v1; B.s2 = v2;
+        B.d1 = outs[1][0]; B.d2 = outs[1][1];
+        B.npairs = n / 8;
+        B.st = st;
+        B.part_in = part[1];
+        B.nb_in = A.nb;
+        B.part_out = part[0];
+        B.nb = width(n / 8);
+        B.round = 2; B.m = m; B.fs_rounds = fs_rounds;
+        jobs.push_back(B);
+        s1 = outs[1][0]; s2 = outs[1][1];
+        len = n / 4;
+        round = 3;
+        which = 0;
+        nb_prev = B.nb;
+        pending = true;
+    }
     while (len > tail_elems) {
         const uint64_t after = pending ? len / 2 : len;
         const uint64_t npairs = after / 2;
@@ -772,12 +939,16 @@ std::vector<ScJob> sumcheck_jobs(const fe *v1, const fe *v2, uint64_t n, int fs_
 }  // namespace
 
 int sumcheck_launch(cosnark_ctx *ctx, const fe *v1, const fe *v2, uint64_t n, int fs_rounds, fe *buf,
-                    hipStream_t stream) {
-    const auto jobs =
-        sumcheck_jobs(v1, v2, n, fs_rounds, buf, ctx->sc_partials, ctx->sc, ctx->sc_blocks, ctx->sc_tail_elems);
+                    hipStream_t stream, bool fuse01) {
+    const auto jobs = sumcheck_jobs(v1, v2, n, fs_rounds, buf, ctx->sc_partials, ctx->sc, ctx->sc_blocks,
+                                    ctx->sc_tail_elems, fuse01);
     for (size_t i = 0; i + 1 < jobs.size(); ++i) {
         const ScJob &J = jobs[i];
-        if (J.round > 0)
+        if (J.kind == 1)
+            hipLaunchKernelGGL(sc_pair01_kernel, dim3((unsigned)J.nb), dim3(SC_THREADS), 0, stream, J);
+        else if (J.kind == 2)
+            hipLaunchKernelGGL(sc_fold01_kernel, dim3((unsigned)J.nb), dim3(SC_THREADS), 0, stream, J);
+        else if (J.round > 0)
             hipLaunchKernelGGL(sc_round_kernel<true>, dim3((unsigned)J.nb), dim3(SC_THREADS), 0, stream, J);
         else
             hipLaunchKernelGGL(sc_round_kernel<false>, dim3((unsigned)J.nb), dim3(SC_THREADS), 0, stream, J);
