@@ -35,7 +35,8 @@ def test_sumcheck_matches_oracle(ctx, m):
         assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("m,offset", [(6, 4), (14, 4), (5, 5)])
+# (14, 14) and (16, 15): rounds 0 and 1 draw libc challenges on the fused rounds-0-2 path
+@pytest.mark.parametrize("m,offset", [(6, 4), (14, 4), (5, 5), (14, 14), (16, 15)])
 def test_sumcheck_libc_tail_matches_oracle(ctx, m, offset):
     print(f"sumcheck m={m} offset={offset}", flush=True)
     o, g = _run_both(ctx, m, 0x5D00 + m, offset=offset, libc_seed=1234 + m)
