@@ -424,7 +424,6 @@ int cosnark_product_sumcheck(cosnark_ctx *ctx, const uint64_t *d_v1, const uint6
     if (rc) return rc;
     rc = sc_stage_state(ctx, gf::mk(t->prev[0], t->prev[1]), gf::mk(y[0], y[1]), 0, m, fs_rounds, ctx->stream);
     if (rc) return rc;
-    // alone on the device: rounds 0-2 fused (one read fewer of the inputs)
     rc = sumcheck_launch(ctx, (const fe *)d_v1, (const fe *)d_v2, n, fs_rounds, (fe *)base, ctx->stream, true);
     if (rc) return rc;
     return sc_collect(ctx, m, t, h_polys, h_r, h_final, ctx->stream);
@@ -468,7 +467,7 @@ int cosnark_prove_plain(cosnark_ctx *ctx, const uint64_t *d_f, const uint64_t *d
     if (times) HIPCK(hipEventRecord(ctx->ev[1], s1));
     if (s2 != ctx->stream) HIPCK(hipStreamWaitEvent(s2, ctx->ev[7], 0));
     if (times) HIPCK(hipEventRecord(ctx->ev[4], s2));
-    if ((rc = sumcheck_launch(ctx, (const fe *)d_f, (const fe *)d_g, n, m, scb, s2))) return rc;
+    if ((rc = sumcheck_launch(ctx, (const fe *)d_f, (const fe *)d_g, n, m, scb, s2, true))) return rc;
     HIPCK(hipEventRecord(ctx->ev[3], s2));
     if (times) HIPCK(hipEventRecord(ctx->ev[6], s1));
     // the lowest levels came from the encode's last pass where the plan allowed it

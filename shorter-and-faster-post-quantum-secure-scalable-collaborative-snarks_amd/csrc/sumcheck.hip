@@ -241,16 +241,10 @@ __global__ __launch_bounds__(SC_THREADS) void sc_pair01_kernel(ScJob J) {
     for (int i = 0; i < 9; ++i) q[i] = gf::zero();
     const uint64_t stride = (uint64_t)J.nb * SC_THREADS;
     uint64_t j = (uint64_t)blockIdx.x * SC_THREADS + threadIdx.x;
-    fe x[4], y[4];
-    auto load = [&](uint64_t t) {
-#pragma unroll
-        for (int i = 0; i < 4; ++i) { x[i] = s1[4 * t + i]; y[i] = s2[4 * t + i]; }
-    };
-    if (j < nq) load(j);
     while (j < nq) {
-        const fe x0 = x[0], x1 = x[1], x2 = x[2], x3 = x[3], y0 = y[0], y1 = y[1], y2 = y[2], y3 = y[3];
+        const fe x0 = s1[4 * j], x1 = s1[4 * j + 1], x2 = s1[4 * j + 2], x3 = s1[4 * j + 3];
+        const fe y0 = s2[4 * j], y1 = s2[4 * j + 1], y2 = s2[4 * j + 2], y3 = s2[4 * j + 3];
         const uint64_t jn = j + stride;
-        if (jn < nq) load(jn);
         const fe ax = gf::sub(x1, x0), bx = gf::sub(x3, x2), ay = gf::sub(y1, y0), by = gf::sub(y3, y2);
         // A: (o - e)(o' - e'), o - e = (x2 - x0) + r0 (bx - ax)
         q[0] = gf::add(q[0], gf::mul(gf::sub(x2, x0), gf::sub(y2, y0)));
@@ -290,12 +284,6 @@ __global__ __launch_bounds__(SC_THREADS) void sc_fold01_kernel(ScJob J) {
     const uint64_t npairs = J.npairs;
     const uint64_t stride = (uint64_t)J.nb * SC_THREADS;
     uint64_t j = (uint64_t)blockIdx.x * SC_THREADS + threadIdx.x;
-    fe x[8], y[8];
-    auto load = [&](uint64_t t) {
-#pragma unroll
-        for (int i = 0; i < 8; ++i) { x[i] = s1[8 * t + i]; y[i] = s2[8 * t + i]; }
-    };
-    if (j < npairs) load(j);   // independent of the challenges: issue first
     {
         fe q[9];
 #pragma unroll
@@ -327,23 +315,26 @@ __global__ __launch_bounds__(SC_THREADS) void sc_fold01_kernel(ScJob J) {
     }
     const fe r0 = sh_r[0], r1 = sh_r[1];
     fe A = gf::zero(), C = gf::zero(), O = gf::zero();
-    while (j < npairs) {
-        fe fx[4], fy[4];
+    // one vector's 8-tuple at a time (the scheduling barrier keeps the
+    // second vector's loads after the first's folds): fewer live registers
+    auto fold8 = [&](const fe *v, uint64_t t, fe &e, fe &o) {
+        fe f[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            fx[k] = fold1(x[2 * k], x[2 * k + 1], r0);
-            fy[k] = fold1(y[2 * k], y[2 * k + 1], r0);
-        }
-        const uint64_t jn = j + stride;
-        const fe e1 = fold1(fx[0], fx[1], r1), o1 = fold1(fx[2], fx[3], r1);
-        const fe e2 = fold1(fy[0], fy[1], r1), o2 = fold1(fy[2], fy[3], r1);
+        for (int k = 0; k < 4; ++k) f[k] = fold1(v[8 * t + 2 * k], v[8 * t + 2 * k + 1], r0);
+        e = fold1(f[0], f[1], r1);
+        o = fold1(f[2], f[3], r1);
+    };
+    while (j < npairs) {
+        fe e1, o1, e2, o2;
+        fold8(s1, j, e1, o1);
+        __builtin_amdgcn_sched_barrier(0);
+        fold8(s2, j, e2, o2);
         d1[2 * j] = e1; d1[2 * j + 1] = o1;
         d2[2 * j] = e2; d2[2 * j + 1] = o2;
         A = gf::add(A, gf::mul(gf::sub(o1, e1), gf::sub(o2, e2)));
         C = gf::add(C, gf::mul(e1, e2));
         O = gf::add(O, gf::mul(o1, o2));
-        j = jn;
-        if (j < npairs) load(j);
+        j += stride;
     }
     block_reduce3<SC_THREADS>(A, C, O, red);
     if (threadIdx.x == 0) {
