@@ -361,6 +361,8 @@ int ntt_launch_batch(cosnark_ctx *ctx, const fe *in, const fe *const *in_tab, ui
         const bool last = (t == m - 1);
         q.last = last;
         q.logS = logn - done_bits - r;
+        // step 2's first radix, as pass_kernel_for splits the digit
+        const int r1 = r >= 7 ? 16 : r >= 5 ? 8 : r >= 3 ? 4 : 2;
         int logC = ilog2((uint64_t)tile) - rd;
         if (logC < 0) logC = 0;
         if (!last) {
@@ -368,6 +370,10 @@ int ntt_launch_batch(cosnark_ctx *ctx, const fe *in, const fe *const *in_tab, ui
         } else {
             if (m == 1) logC = 0;
             else if (logC > p.rbits[0]) logC = p.rbits[0];
+            // with leaf digests wanted, narrow the tile until each thread
+            // holds at most one step-2 item (R1 * C <= NTT_THREADS)
+            if (leaves && m >= 2)
+                while (logC > 2 && (r1 << logC) > NTT_THREADS) --logC;
         }
         q.logC = logC;
         if (t > 0) q.in_len = N;
@@ -389,7 +395,6 @@ int ntt_launch_batch(cosnark_ctx *ctx, const fe *in, const fe *const *in_tab, ui
             q.out_stride = N;
             // fused leaf digests: whole leaves per tile (C >= 4) and at most
             // one step-2 item per thread (R1 * C <= NTT_THREADS)
-            const int r1 = r >= 7 ? 16 : r >= 5 ? 8 : r >= 3 ? 4 : 2;
             if (leaves && m >= 2 && logC >= 2 && (r1 << logC) <= NTT_THREADS) {
                 q.leaves = leaves;
                 if (levels_done) *levels_done = 1;

@@ -776,8 +776,11 @@ int cosnark_whir_open(cosnark_ctx *ctx, const uint64_t *d_arr, uint64_t n, const
         // Whir_Commit_initial at 2*rate
         fe *cw_new = (t % 2 == 0) ? cwX : cwY;
         uint8_t *nd_new = (t % 2 == 0) ? ndX : ndY;
-        if ((rc = ntt_launch(ctx, arr_n, it.n / 4, cw_new, ilog2(it.cw_new_n), false, s))) return rc;
-        if ((rc = merkle_launch(ctx, cw_new, it.cw_new_n, nd_new, true, s))) return rc;
+        int lv = 0;
+        if ((rc = ntt_launch_leaves(ctx, arr_n, it.n / 4, cw_new, ilog2(it.cw_new_n), nd_new, &lv, s))) return rc;
+        if ((rc = lv ? merkle_from_leaves_launch(ctx, it.cw_new_n / 4, nd_new, true, s, lv)
+                     : merkle_launch(ctx, cw_new, it.cw_new_n, nd_new, true, s)))
+            return rc;
         // whir_query on the previous codeword
         {
             const int work = it.l * WK + it.l * it.depth;
@@ -1146,8 +1149,12 @@ int cosnark_whir_batch_open(cosnark_ctx *ctx, int count, const uint64_t *const *
                 const int slot = cw_slot == 0 ? 1 : 0;
                 fe *cw_new = cwbuf[slot];
                 uint8_t *nd_new = ndbuf[slot];
-                if ((rc = ntt_launch(ctx, arr_n, st.n / st.kappa, cw_new, ilog2(st.cw_new_n), false, s))) return rc;
-                if ((rc = merkle_launch(ctx, cw_new, st.cw_new_n, nd_new, true, s))) return rc;
+                int lv = 0;
+                if ((rc = ntt_launch_leaves(ctx, arr_n, st.n / st.kappa, cw_new, ilog2(st.cw_new_n), nd_new, &lv, s)))
+                    return rc;
+                if ((rc = lv ? merkle_from_leaves_launch(ctx, st.cw_new_n / 4, nd_new, true, s, lv)
+                             : merkle_launch(ctx, cw_new, st.cw_new_n, nd_new, true, s)))
+                    return rc;
                 const int work = st.l * st.kappa + st.l * st.depth;
                 hipLaunchKernelGGL(gather_kernel, dim3((work + WT - 1) / WT), dim3(WT), 0, s, cw_cur, st.cw_n, st.kappa,
                                    nd_cur, st.zero_src >= 0 ? (const uint8_t *)(dz + 32 * 29 * st.zero_src) : nullptr,

@@ -20,19 +20,21 @@ def test_whir_commit_matches_oracle(ctx, logn, rate):
     assert np.array_equal(ctx.download(nodes, (cnt, 32), np.uint8), O.merkle_commit(O.rs_encode(arr, rate), True))
 
 
-@pytest.mark.parametrize("passbits,tile", [(8, 4096), (8, 2048), (6, 4096)])
-def test_whir_commit_fused_tree_levels(ctx, passbits, tile):
-    """A 2^16-point codeword under three NTT plans: radix-256 passes with 16-
-    and 8-column tiles (the last pass writes the leaf digests) and radix-64
-    passes (it does not); every node must match, since the dup-left root
+@pytest.mark.parametrize("passbits,tile,logn", [(8, 4096, 14), (8, 2048, 14), (6, 4096, 14), (7, 4096, 19)])
+def test_whir_commit_fused_tree_levels(ctx, passbits, tile, logn):
+    """Codewords of 2^16 and 2^21 points under four NTT plans, the last pass
+    writing the leaf digests in each: radix-256 passes with 16- and 8-column
+    tiles; 6 + 5 + 5 bits, whose last pass narrows its 64 columns to 32; and
+    7 + 7 + 7 bits, whose last pass narrows 128 columns to 16 (at most one
+    step-2 item per thread).  Every node must match, since the dup-left root
     alone depends on leaf 0 only."""
-    n, rate = 1 << 14, 4
+    n, rate = 1 << logn, 4
     arr = O.synth(0xC05A0013, passbits, n)
-    ctx.set_ntt_plan(16, passbits, tile)
+    ctx.set_ntt_plan(logn + 2, passbits, tile)
     try:
         cw, nodes = ctx.whir_commit(ctx.upload(arr), n, rate)
     finally:
-        ctx.set_ntt_plan(16, 0, 0)
+        ctx.set_ntt_plan(logn + 2, 0, 0)
     ref_cw = O.rs_encode(arr, rate)
     assert np.array_equal(ctx.download(cw, (n * rate, 2)), ref_cw)
     cnt = cosnark.merkle_node_count(n * rate)
