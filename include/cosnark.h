@@ -154,6 +154,17 @@ uint64_t cosnark_plain_proof_bytes(uint64_t n);
 int cosnark_prove_plain(cosnark_ctx *ctx, const uint64_t *d_f, const uint64_t *d_g,
                         uint64_t n, int rate, cosnark_transcript *ts,
                         uint8_t *h_proof, double *times);
+/* The same proof, keeping the commitment: the codeword (rate*n elements)
+ * is written to d_codeword and every tree level (cosnark_merkle_node_count(
+ * rate*n) digests, level 0 first, root last) to d_nodes, both caller-owned
+ * device buffers, as Whir_Commit_initial (src/Whir.cpp:158-190) leaves the
+ * codeword and MT_commit_Blake (src/merkle_tree.cpp:195-233) the tree for
+ * the openings that follow.  Either may be null (that part then stays in
+ * the context's scratch).  Proof bytes equal cosnark_prove_plain's. */
+int cosnark_prove_plain_ex(cosnark_ctx *ctx, const uint64_t *d_f, const uint64_t *d_g,
+                           uint64_t n, int rate, cosnark_transcript *ts,
+                           uint8_t *h_proof, uint64_t *d_codeword, uint8_t *d_nodes,
+                           double *times);
 /* The verifier of a plain proof's sumcheck (the checks the reference's
  * prover makes, src/Distributed_Sumcheck.cpp:243-248): with y the claimed
  * sum and the transcript t as the prover started it, each round's
@@ -364,6 +375,13 @@ int cosnark_batch_create(cosnark_ctx *ctx, int count, const uint64_t *sizes, int
 void cosnark_batch_destroy(cosnark_batch *b);
 int cosnark_batch_input(cosnark_batch *b, int i, uint64_t **d_f, uint64_t **d_g);
 uint64_t cosnark_batch_proof_offset(const cosnark_batch *b, int i);
+/* Proof i's commitment as the last cosnark_batch_prove left it: its
+ * codeword (rate * sizes[i] elements) and every level of its tree
+ * (cosnark_merkle_node_count(rate * sizes[i]) digests, level 0 first, root
+ * last), device pointers into the batch's memory, valid until the next
+ * prove or the batch's destruction.  They equal cosnark_prove_plain_ex's
+ * for the same input. */
+int cosnark_batch_commitment(cosnark_batch *b, int i, uint64_t **d_codeword, uint8_t **d_nodes);
 int cosnark_batch_set_graph(cosnark_batch *b, int on);
 int cosnark_batch_prove(cosnark_batch *b, cosnark_transcript *ts, uint8_t *h_proofs, double *ms);
 
@@ -381,7 +399,9 @@ int cosnark_batch_prove(cosnark_batch *b, cosnark_transcript *ts, uint8_t *h_pro
  *   submit: d_f, d_g are device arrays on the same device (n elements each,
  *           read during the proof: keep them until wait returns); prev is
  *           the transcript start (null: zero).  Thread-safe.
- *   wait:   blocks until the proof is ready; h_proof receives
+ *   wait:   blocks until the proof is ready (one wait per ticket: a
+ *           second wait on a ticket already being waited for returns
+ *           COSNARK_E_ARG); h_proof receives
  *           cosnark_plain_proof_bytes(n) bytes, identical to
  *           cosnark_prove_plain's for the same inputs and transcript;
  *           latency_ms = completion - submission; releases the ticket.
@@ -391,6 +411,13 @@ typedef struct cosnark_server cosnark_server;
 int cosnark_server_create(int device, uint64_t budget, int rate, int max_log_n, int slots, cosnark_server **out);
 int cosnark_server_submit(cosnark_server *s, const uint64_t *d_f, const uint64_t *d_g, uint64_t n,
                           const uint64_t prev[2], uint64_t *ticket);
+/* submit_ex: as submit, and the request's commitment -- its codeword
+ * (rate * n elements) and every level of its tree (cosnark_merkle_node_count(
+ * rate * n) digests) -- is written to the caller's device buffers d_codeword
+ * and d_nodes (both or neither) instead of the arena, complete when wait
+ * returns, equal to cosnark_prove_plain_ex's. */
+int cosnark_server_submit_ex(cosnark_server *s, const uint64_t *d_f, const uint64_t *d_g, uint64_t n,
+                             const uint64_t prev[2], uint64_t *d_codeword, uint8_t *d_nodes, uint64_t *ticket);
 int cosnark_server_poll(cosnark_server *s, uint64_t ticket, int *done);
 int cosnark_server_wait(cosnark_server *s, uint64_t ticket, uint8_t *h_proof, uint64_t prev_out[2],
                         double *latency_ms);

@@ -592,6 +592,23 @@ static int state_path(const whir_state *S, uint64_t pos, uint8_t *path) {
 
 static int fe_eq(or_fe a, or_fe b) { return a.real == b.real && a.img == b.img; }
 
+/* Thread count of the WHIR restatement's heavy steps: 1 runs the
+ * single-thread functions of this file; more runs their oracle_mt.c twins,
+ * which give the same bytes (tests/test_oracle_mt.py), so a full-size
+ * opening can be checked in seconds. */
+static int whir_threads = 1;
+void or_whir_set_threads(int t) { whir_threads = t < 1 ? 1 : t; }
+
+static void w_encode(const or_fe *arr, uint64_t n, int rate, or_fe *cw) {
+    if (whir_threads > 1) or_rs_encode_mt(arr, n, rate, cw, whir_threads);
+    else or_rs_encode(arr, n, rate, cw);
+}
+
+static void w_merkle(const or_fe *elems, uint64_t n_elems, uint8_t *nodes) {
+    if (whir_threads > 1) or_merkle_commit_mt(elems, n_elems, nodes, 1, whir_threads);
+    else or_merkle_commit(elems, n_elems, nodes, 1);
+}
+
 /* Whir_Iteration (Whir.cpp:354-386) with whir_k = kappa; 0, or a negative
  * status where the reference reports an error */
 static int whir_iteration(whir_state *S, int kappa, rec_t *R, int iter) {
@@ -615,10 +632,10 @@ static int whir_iteration(whir_state *S, int kappa, rec_t *R, int iter) {
     /* Whir_Commit_initial at 2*rate (:158-190) */
     uint64_t ncw_n = S->n * (uint64_t)(2 * S->rate);
     or_fe *ncw = (or_fe *)malloc(sizeof(or_fe) * ncw_n);
-    or_rs_encode(S->arr, S->n, 2 * S->rate, ncw);
+    w_encode(S->arr, S->n, 2 * S->rate, ncw);
     uint64_t nnodes = or_merkle_node_count(ncw_n);
     uint8_t *nmt = (uint8_t *)malloc(32 * nnodes);
-    or_merkle_commit(ncw, ncw_n, nmt, 1);
+    w_merkle(ncw, ncw_n, nmt);
     rec_bytes32(R, nmt + 32 * (nnodes - 1));
 
     /* whir_query (:251-314) on the current codeword */
@@ -657,14 +674,27 @@ static int whir_iteration(whir_state *S, int kappa, rec_t *R, int iter) {
         rec_fe(R, rb);
         or_fe ch = rb;
         or_fe omega = or_root_of_unity(ilog2u(ncw_n));
-        for (int i = 0; i < l; ++i) {
-            if (i > 0) ch = or_mul(ch, rb);
-            or_fe w = or_pow(omega, q[i]), pw = one;
-            for (uint64_t j = 0; j < S->n; ++j) {
-                S->b[j] = or_add(S->b[j], or_mul(ch, pw));
-                pw = or_mul(pw, w);
+        if (whir_threads > 1) {
+            /* b[j] += sum_i rb^(i+1) (w^q_i)^j on all cores */
+            or_fe *chs = (or_fe *)malloc(sizeof(or_fe) * l), *ws = (or_fe *)malloc(sizeof(or_fe) * l);
+            for (int i = 0; i < l; ++i) {
+                if (i > 0) ch = or_mul(ch, rb);
+                chs[i] = ch;
+                ws[i] = or_pow(omega, q[i]);
+                S->K = or_add(S->K, or_mul(ch, y[i]));           /* :383-384 */
             }
-            S->K = or_add(S->K, or_mul(ch, y[i]));               /* :383-384 */
+            or_update_betas_mt(S->b, S->n, chs, ws, l, whir_threads);
+            free(chs); free(ws);
+        } else {
+            for (int i = 0; i < l; ++i) {
+                if (i > 0) ch = or_mul(ch, rb);
+                or_fe w = or_pow(omega, q[i]), pw = one;
+                for (uint64_t j = 0; j < S->n; ++j) {
+                    S->b[j] = or_add(S->b[j], or_mul(ch, pw));
+                    pw = or_mul(pw, w);
+                }
+                S->K = or_add(S->K, or_mul(ch, y[i]));           /* :383-384 */
+            }
         }
     }
     free(q); free(y);
@@ -692,7 +722,7 @@ static int whir_final(whir_state *S, int kappa, rec_t *R) {
     {
         uint64_t ncw_n = S->n * (uint64_t)S->rate;
         or_fe *ncw = (or_fe *)malloc(sizeof(or_fe) * ncw_n);
-        or_rs_encode(S->arr, S->n, S->rate, ncw);
+        w_encode(S->arr, S->n, S->rate, ncw);
         int same = ncw_n == S->cw_n && !memcmp(ncw, S->cw, sizeof(or_fe) * ncw_n);
         free(ncw);
         if (!same) return -3;
@@ -722,9 +752,9 @@ int64_t or_whir_open(const or_fe *arr_in, uint64_t n, const or_fe *z, or_fe K,
     /* the initial commitment (Whir_Commit_initial with the caller's rate) */
     S.cw_n = n * (uint64_t)rate;
     S.cw = (or_fe *)malloc(sizeof(or_fe) * S.cw_n);
-    or_rs_encode(S.arr, n, rate, S.cw);
+    w_encode(S.arr, n, rate, S.cw);
     S.mt = (uint8_t *)malloc(32 * or_merkle_node_count(S.cw_n));
-    or_merkle_commit(S.cw, S.cw_n, S.mt, 1);
+    w_merkle(S.cw, S.cw_n, S.mt);
     S.mt_leaves = S.cw_n / 4;
 
     uint64_t iters_pos = R.n;

@@ -176,6 +176,21 @@ void or_compute_secret_shares(const or_fe *v, uint64_t M, int N, int k, int kk, 
  * N rows of len (rank-major), b N/4 weights; out len elements. */
 void or_sumcheck_aggr(const or_fe *polys, int N, int len, const or_fe *b, or_fe *out);
 
+/* ---- all-cores variants (oracle_mt.c, OpenMP): the same bytes as the
+ * single-thread functions above, `threads` OpenMP threads ---- */
+void or_fft_mt(or_fe *arr, int logn, int inverse, int threads);            /* utils.cpp:904-964, 1583-1673 */
+void or_rs_encode_mt(const or_fe *arr, uint64_t n, int rate, or_fe *cw, int threads);  /* Whir.cpp:158-163 */
+void or_merkle_commit_mt(const or_fe *elems, uint64_t n_elems, uint8_t *nodes, int dup_left, int threads);
+int or_product_sumcheck_mt(or_fe y, const or_fe *v1, const or_fe *v2, uint64_t n, or_fe *prev, or_fe *polys,
+                           or_fe *r, or_fe *final_out, int threads);
+int or_plain_proof_mt(const or_fe *f, const or_fe *g, uint64_t n, int rate, or_fe *prev, uint8_t *proof,
+                      double *stage_ms, int threads);
+void or_update_betas_mt(or_fe *b, uint64_t n, const or_fe *ch, const or_fe *w, int l, int threads);
+int or_max_threads(void);
+/* thread count of or_whir_open / or_whir_batch_open's encodes, trees and
+ * weight updates (1, the default: single-thread) */
+void or_whir_set_threads(int threads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -112,6 +112,16 @@ def oracle():
         L.or_whir_kappas.restype = i32
         L.or_whir_batch_open.argtypes = [i32, vp, vp, vp, vp, vp, vp, vp, i32, i32, vp, u64]
         L.or_whir_batch_open.restype = ctypes.c_int64
+        L.or_fft_mt.argtypes = [vp, i32, i32, i32]
+        L.or_rs_encode_mt.argtypes = [vp, u64, i32, vp, i32]
+        L.or_merkle_commit_mt.argtypes = [vp, u64, vp, i32, i32]
+        L.or_product_sumcheck_mt.argtypes = [_Fe, vp, vp, u64, vp, vp, vp, vp, i32]
+        L.or_product_sumcheck_mt.restype = i32
+        L.or_plain_proof_mt.argtypes = [vp, vp, u64, i32, vp, vp, vp, i32]
+        L.or_plain_proof_mt.restype = i32
+        L.or_max_threads.restype = i32
+        L.or_whir_set_threads.argtypes = [i32]
+        L.or_update_betas_mt.argtypes = [vp, u64, vp, vp, i32, i32]
         _o = L
     return _o
 
@@ -493,3 +503,68 @@ def whir_aggregate_claims(arrays, points):
     oracle().or_whir_aggregate_claims(len(arrs), _ptrs(arrs), n, pts.shape[0], _p(pts), _p(arr), _p(b), _p(K),
                                       _p(bc), _p(ac))
     return arr, b, K[0], bc, ac
+
+
+# ---- all-cores variants (oracle_mt.c): the same bytes as the functions above
+
+def max_threads():
+    return int(oracle().or_max_threads())
+
+
+def fft_mt(arr, inverse=False, threads=None):
+    a = np.ascontiguousarray(arr, dtype=np.uint64).copy()
+    oracle().or_fft_mt(_p(a), a.shape[0].bit_length() - 1, int(inverse), threads or max_threads())
+    return a
+
+
+def rs_encode_mt(arr, rate, threads=None):
+    a = np.ascontiguousarray(arr, dtype=np.uint64)
+    cw = fe_array(a.shape[0] * rate)
+    oracle().or_rs_encode_mt(_p(a), a.shape[0], rate, _p(cw), threads or max_threads())
+    return cw
+
+
+def merkle_commit_mt(elems, dup_left=True, threads=None):
+    e = np.ascontiguousarray(elems, dtype=np.uint64)
+    cnt = int(oracle().or_merkle_node_count(e.shape[0]))
+    nodes = np.zeros((cnt, 32), np.uint8)
+    oracle().or_merkle_commit_mt(_p(e), e.shape[0], _p(nodes), int(dup_left), threads or max_threads())
+    return nodes
+
+
+def product_sumcheck_mt(y, v1, v2, prev, threads=None):
+    """All-Fiat-Shamir product sumcheck on all cores: (status, polys, r, final)."""
+    a = np.ascontiguousarray(v1, dtype=np.uint64)
+    b = np.ascontiguousarray(v2, dtype=np.uint64)
+    n = a.shape[0]
+    m = n.bit_length() - 1
+    st = np.ascontiguousarray(prev, dtype=np.uint64)
+    polys = np.zeros((max(m, 1), 3, 2), np.uint64)
+    r = np.zeros((max(m, 1), 2), np.uint64)
+    fin = np.zeros((2, 2), np.uint64)
+    rc = oracle().or_product_sumcheck_mt(fe(y), _p(a), _p(b), n, _p(st), _p(polys), _p(r), _p(fin),
+                                         threads or max_threads())
+    prev[...] = st
+    return rc, polys[:m], r[:m], fin
+
+
+def plain_proof_mt(f, g, rate, prev=None, threads=None):
+    """plain_proof on all cores: (proof bytes, stage ms [encode, merkle, sumcheck, total])."""
+    a = np.ascontiguousarray(f, dtype=np.uint64)
+    b = np.ascontiguousarray(g, dtype=np.uint64)
+    n = a.shape[0]
+    pv = np.zeros(2, np.uint64) if prev is None else np.ascontiguousarray(prev, dtype=np.uint64)
+    proof = np.zeros(64 + 48 * (n.bit_length() - 1), np.uint8)
+    ms = np.zeros(4, np.float64)
+    rc = oracle().or_plain_proof_mt(_p(a), _p(b), n, rate, _p(pv), _p(proof), _p(ms), threads or max_threads())
+    if rc:
+        raise AssertionError(f"oracle plain proof: round check failed ({rc})")
+    if prev is not None:
+        prev[...] = pv
+    return proof.tobytes(), ms
+
+
+def whir_threads(t):
+    """Thread count of the WHIR restatement's encodes, trees and weight
+    updates (1: single-thread; the bytes are the same either way)."""
+    oracle().or_whir_set_threads(int(t))

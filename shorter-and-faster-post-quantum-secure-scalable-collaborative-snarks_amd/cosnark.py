@@ -67,6 +67,7 @@ def lib():
         L.cosnark_plain_proof_bytes.argtypes = [u64]
         L.cosnark_plain_proof_bytes.restype = u64
         L.cosnark_prove_plain.argtypes = [vp, vp, vp, u64, i32, vp, vp, vp]
+        L.cosnark_prove_plain_ex.argtypes = [vp, vp, vp, u64, i32, vp, vp, vp, vp, vp]
         L.cosnark_whir_commit.argtypes = [vp, vp, u64, i32, vp, vp]
         L.cosnark_whir_claim.argtypes = [vp, vp, u64, vp, vp]
         L.cosnark_whir_open.argtypes = [vp, vp, u64, vp, vp, i32, i32, vp, vp, vp, u64, ctypes.POINTER(u64)]
@@ -98,6 +99,7 @@ def lib():
         L.cosnark_batch_input.argtypes = [vp, i32, ctypes.POINTER(vp), ctypes.POINTER(vp)]
         L.cosnark_batch_proof_offset.argtypes = [vp, i32]
         L.cosnark_batch_proof_offset.restype = u64
+        L.cosnark_batch_commitment.argtypes = [vp, i32, ctypes.POINTER(vp), ctypes.POINTER(vp)]
         L.cosnark_batch_set_graph.argtypes = [vp, i32]
         L.cosnark_batch_prove.argtypes = [vp, vp, vp, ctypes.POINTER(ctypes.c_double)]
         L.cosnark_autotune.argtypes = [vp, i32, i32, i32, vp]
@@ -106,6 +108,7 @@ def lib():
         L.cosnark_blake3_rate.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
         L.cosnark_server_create.argtypes = [i32, u64, i32, i32, i32, ctypes.POINTER(vp)]
         L.cosnark_server_submit.argtypes = [vp, vp, vp, u64, vp, ctypes.POINTER(u64)]
+        L.cosnark_server_submit_ex.argtypes = [vp, vp, vp, u64, vp, vp, vp, ctypes.POINTER(u64)]
         L.cosnark_server_poll.argtypes = [vp, u64, ctypes.POINTER(i32)]
         L.cosnark_server_wait.argtypes = [vp, u64, vp, vp, ctypes.POINTER(ctypes.c_double)]
         L.cosnark_server_stats.argtypes = [vp, vp]
@@ -296,6 +299,23 @@ class Context:
         self._ck(lib().cosnark_prove_plain(self.h, df.ptr, dg.ptr, n, rate, ctypes.byref(ts), _p(proof),
                                            _p(times) if stage_times else None))
         return proof, times
+
+    def prove_plain_keep(self, df, dg, n, rate, ts: Transcript):
+        """cosnark_prove_plain_ex on device inputs: (proof bytes, codeword
+        DevBuf (rate*n elements), nodes DevBuf (every tree level))."""
+        nb = int(lib().cosnark_plain_proof_bytes(n))
+        proof = np.zeros(nb, np.uint8)
+        cw = self.alloc(16 * n * rate)
+        nodes = self.alloc(32 * merkle_node_count(n * rate))
+        self._ck(lib().cosnark_prove_plain_ex(self.h, df.ptr, dg.ptr, n, rate, ctypes.byref(ts), _p(proof),
+                                              cw.ptr, nodes.ptr, None))
+        return proof, cw, nodes
+
+    def download_ptr(self, ptr, shape, dtype=np.uint64):
+        """Download from a raw device address (int)."""
+        out = np.zeros(shape, dtype=dtype)
+        self._ck(lib().cosnark_d2h(self.h, _p(out), ctypes.c_void_p(ptr), out.nbytes))
+        return out
 
     # ---- WHIR ----
     def whir_commit(self, arr_dev, n, rate):
@@ -590,6 +610,16 @@ class Batch:
     def set_graph(self, on):
         self.ctx._ck(lib().cosnark_batch_set_graph(self.h, int(bool(on))))
 
+    def commitment(self, i):
+        """Proof i's codeword (rate*n, 2) and tree nodes (count, 32) as the
+        last prove left them."""
+        c, t = ctypes.c_void_p(), ctypes.c_void_p()
+        self.ctx._ck(lib().cosnark_batch_commitment(self.h, i, ctypes.byref(c), ctypes.byref(t)))
+        cwn = self.sizes[i] * self.rate
+        cw = self.ctx.download_ptr(c.value, (cwn, 2))
+        nodes = self.ctx.download_ptr(t.value, (merkle_node_count(cwn), 32), np.uint8)
+        return cw, nodes
+
     def prove(self, transcripts=None):
         """Returns (list of proof bytes, list of transcripts after, device ms)."""
         n = len(self.sizes)
@@ -631,6 +661,17 @@ class Server:
         rc = lib().cosnark_server_submit(self.h, pf, pg, n, _p(pv), ctypes.byref(t))
         if rc != 0:
             raise CosnarkError(rc, "server_submit failed")
+        return t.value
+
+    def submit_keep(self, d_f, d_g, n, d_cw, d_nodes, prev=(0, 0)):
+        """submit, with the request's codeword and tree written to the
+        device buffers d_cw (rate*n elements) and d_nodes (every level)."""
+        pv = np.ascontiguousarray([int(prev[0]), int(prev[1])], dtype=np.uint64)
+        t = ctypes.c_uint64(0)
+        rc = lib().cosnark_server_submit_ex(self.h, d_f.ptr, d_g.ptr, n, _p(pv), d_cw.ptr, d_nodes.ptr,
+                                            ctypes.byref(t))
+        if rc != 0:
+            raise CosnarkError(rc, "server_submit_ex failed")
         return t.value
 
     def poll(self, ticket):

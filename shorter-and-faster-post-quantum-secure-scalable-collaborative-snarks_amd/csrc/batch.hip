@@ -65,10 +65,11 @@ __global__ __launch_bounds__(PACK_THREADS) void pack_kernel(const PackJob *__res
 
 size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
-size_t batch_work_bytes(uint64_t n, int rate) {
+size_t batch_work_bytes(uint64_t n, int rate, bool with_commit) {
     const uint64_t cwn = n * (uint64_t)rate;
-    return align256(sizeof(fe) * cwn) + align256(32 * cosnark_merkle_node_count(cwn)) +
-           align256(sizeof(fe) * (n + n / 2)) + align256(sizeof(fe) * 6 * SC_MAX_BLOCKS) + align256(sizeof(ScState));
+    const size_t commit = align256(sizeof(fe) * cwn) + align256(32 * cosnark_merkle_node_count(cwn));
+    return (with_commit ? commit : 0) + align256(sizeof(fe) * (n + n / 2)) +
+           align256(sizeof(fe) * 6 * SC_MAX_BLOCKS) + align256(sizeof(ScState));
 }
 
 size_t batch_record_bytes(uint64_t n) { return cosnark_plain_proof_bytes(n) + 32; }
@@ -94,10 +95,15 @@ int batch_core_build(cosnark_ctx *ctx, int count, const BatchProof *pr, int rate
         C.n[i] = n;
         cwn[i] = n * (uint64_t)rate;
         uint8_t *w = (uint8_t *)pr[i].work;
-        cws[i] = (fe *)w;
-        w += align256(sizeof(fe) * cwn[i]);
-        nds[i] = w;
-        w += align256(32 * cosnark_merkle_node_count(cwn[i]));
+        if (pr[i].cw && pr[i].nodes) {
+            cws[i] = pr[i].cw;
+            nds[i] = pr[i].nodes;
+        } else {
+            cws[i] = (fe *)w;
+            w += align256(sizeof(fe) * cwn[i]);
+            nds[i] = w;
+            w += align256(32 * cosnark_merkle_node_count(cwn[i]));
+        }
         bufs[i] = (fe *)w;
         w += align256(sizeof(fe) * (n + n / 2));
         parts[i] = (fe *)w;
@@ -109,6 +115,8 @@ int batch_core_build(cosnark_ctx *ctx, int count, const BatchProof *pr, int rate
         out += batch_record_bytes(n);
     }
     C.out_bytes = out;
+    C.cw = cws;
+    C.nodes = nds;
     // RS encodes: one group per codeword size, launched together
     std::map<int, std::vector<int>> by_log;
     for (int i = 0; i < count; ++i) by_log[ilog2(cwn[i])].push_back(i);
@@ -360,6 +368,14 @@ int cosnark_batch_input(cosnark_batch *b, int i, uint64_t **d_f, uint64_t **d_g)
         return set_err(b ? b->ctx : nullptr, COSNARK_E_ARG, "batch_input: bad argument");
     *d_f = (uint64_t *)((char *)b->f + b->off_in[i]);
     *d_g = (uint64_t *)((char *)b->g + b->off_in[i]);
+    return COSNARK_OK;
+}
+
+int cosnark_batch_commitment(cosnark_batch *b, int i, uint64_t **d_codeword, uint8_t **d_nodes) {
+    if (!b || i < 0 || i >= b->core.count || !d_codeword || !d_nodes)
+        return set_err(b ? b->ctx : nullptr, COSNARK_E_ARG, "batch_commitment: bad argument");
+    *d_codeword = (uint64_t *)b->core.cw[i];
+    *d_nodes = b->core.nodes[i];
     return COSNARK_OK;
 }
 

@@ -437,21 +437,29 @@ uint64_t cosnark_plain_proof_bytes(uint64_t n) { return 32 + 48ull * (uint64_t)i
 
 int cosnark_prove_plain(cosnark_ctx *ctx, const uint64_t *d_f, const uint64_t *d_g, uint64_t n, int rate,
                         cosnark_transcript *t, uint8_t *h_proof, double *times) {
+    return cosnark_prove_plain_ex(ctx, d_f, d_g, n, rate, t, h_proof, nullptr, nullptr, times);
+}
+
+int cosnark_prove_plain_ex(cosnark_ctx *ctx, const uint64_t *d_f, const uint64_t *d_g, uint64_t n, int rate,
+                           cosnark_transcript *t, uint8_t *h_proof, uint64_t *d_codeword, uint8_t *d_nodes,
+                           double *times) {
     if (!ctx || !d_f || !d_g || !t || !h_proof) return set_err(ctx, COSNARK_E_ARG, "prove_plain: null argument");
     if (!is_pow2(n) || n < 4 || rate < 1 || !is_pow2((uint64_t)rate) || n * (uint64_t)rate > (1ull << 28))
         return set_err(ctx, COSNARK_E_ARG, "prove_plain: n, rate must be powers of two, 4 <= n, n*rate <= 2^28");
     const int m = ilog2(n);
     const uint64_t cw_n = n * (uint64_t)rate;
     const uint64_t nodes = cosnark_merkle_node_count(cw_n);
+    // the codeword and the tree go to the caller's buffers when given (the
+    // commitment then stays openable), else to the context's scratch
     const size_t off_cw = 0;
-    const size_t off_nodes = align_up(off_cw + cw_n * sizeof(fe), 256);
-    const size_t off_sc = align_up(off_nodes + nodes * 32, 256);
+    const size_t off_nodes = align_up(off_cw + (d_codeword ? 0 : cw_n * sizeof(fe)), 256);
+    const size_t off_sc = align_up(off_nodes + (d_nodes ? 0 : nodes * 32), 256);
     const size_t total = off_sc + sizeof(fe) * (n + n / 2);
     void *base;
     int rc = scratch_reserve(ctx, total, &base);
     if (rc) return rc;
-    fe *cw = (fe *)((char *)base + off_cw);
-    uint8_t *nd = (uint8_t *)base + off_nodes;
+    fe *cw = d_codeword ? (fe *)d_codeword : (fe *)((char *)base + off_cw);
+    uint8_t *nd = d_nodes ? d_nodes : (uint8_t *)base + off_nodes;
     fe *scb = (fe *)((char *)base + off_sc);
     rc = sc_stage_state(ctx, gf::mk(t->prev[0], t->prev[1]), gf::zero(), 1, m, m, ctx->stream);
     if (rc) return rc;

@@ -210,11 +210,17 @@ struct BatchProof {
     const gf::fe *f, *g;
     uint64_t n;
     void *work;
+    // where the codeword and the tree go when the caller keeps them (both
+    // or neither); work then holds only the sumcheck buffers and state
+    gf::fe *cw = nullptr;
+    uint8_t *nodes = nullptr;
 };
 struct BatchCore {
     int count = 0, rate = 0;
     std::vector<uint64_t> n;
     std::vector<size_t> out_off;     // proof i's record in the output block
+    std::vector<gf::fe *> cw;        // proof i's codeword (rate * n elements)
+    std::vector<uint8_t *> nodes;    // proof i's tree, every level
     size_t out_bytes = 0;
     struct Group {
         int logn, B;
@@ -230,7 +236,9 @@ struct BatchCore {
     uint8_t *out_dev = nullptr;      // device: out_bytes of records
 };
 size_t align256(size_t x);
-size_t batch_work_bytes(uint64_t n, int rate);
+// working memory of one proof; without the codeword and the tree when the
+// caller provides them (BatchProof::cw / nodes)
+size_t batch_work_bytes(uint64_t n, int rate, bool with_commit = true);
 size_t batch_record_bytes(uint64_t n);
 int batch_core_build(cosnark_ctx *ctx, int count, const BatchProof *pr, int rate, gf::fe *prev_dev,
                      uint8_t *out_dev, BatchCore &C);

@@ -100,10 +100,13 @@ struct Request {
     Clock::time_point t_submit, t_done;
     void *work = nullptr;
     size_t work_bytes = 0;
+    fe *cw = nullptr;                   // caller's buffers for the commitment, or null
+    uint8_t *nodes = nullptr;
     std::vector<uint8_t> proof;
     uint64_t prev_out[2] = {0, 0};
     int status = COSNARK_OK;
     bool done = false;
+    bool claimed = false;               // a wait has taken the ticket
 };
 
 struct Slot {
@@ -215,7 +218,7 @@ bool launch(cosnark_server *S, Slot &sl) {
     const int k = (int)take.size();
     std::vector<BatchProof> pr(k);
     for (int i = 0; i < k; ++i) {
-        pr[i] = {take[i]->f, take[i]->g, take[i]->n, take[i]->work};
+        pr[i] = {take[i]->f, take[i]->g, take[i]->n, take[i]->work, take[i]->cw, take[i]->nodes};
         sl.prev_host[i] = take[i]->prev;
     }
     fe *prev_dev = (fe *)sl.shared;
@@ -364,7 +367,13 @@ int cosnark_server_create(int device, uint64_t budget, int rate, int max_log_n, 
 
 int cosnark_server_submit(cosnark_server *S, const uint64_t *d_f, const uint64_t *d_g, uint64_t n,
                           const uint64_t prev[2], uint64_t *ticket) {
+    return cosnark_server_submit_ex(S, d_f, d_g, n, prev, nullptr, nullptr, ticket);
+}
+
+int cosnark_server_submit_ex(cosnark_server *S, const uint64_t *d_f, const uint64_t *d_g, uint64_t n,
+                             const uint64_t prev[2], uint64_t *d_codeword, uint8_t *d_nodes, uint64_t *ticket) {
     if (!S || !d_f || !d_g || !ticket) return COSNARK_E_ARG;
+    if ((d_codeword == nullptr) != (d_nodes == nullptr)) return COSNARK_E_ARG;
     if (!is_pow2(n) || n < 4 || n * (uint64_t)S->rate < 8) return COSNARK_E_ARG;
     if (n > (1ull << S->max_log_n)) return COSNARK_E_RANGE;
     auto *r = new Request;
@@ -372,7 +381,9 @@ int cosnark_server_submit(cosnark_server *S, const uint64_t *d_f, const uint64_t
     r->g = (const fe *)d_g;
     r->n = n;
     if (prev) r->prev = gf::mk(prev[0], prev[1]);
-    r->work_bytes = batch_work_bytes(n, S->rate);
+    r->cw = (fe *)d_codeword;
+    r->nodes = d_nodes;
+    r->work_bytes = batch_work_bytes(n, S->rate, d_codeword == nullptr);
     r->t_submit = Clock::now();
     {
         std::lock_guard<std::mutex> lk(S->mu);
@@ -405,6 +416,10 @@ int cosnark_server_wait(cosnark_server *S, uint64_t ticket, uint8_t *h_proof, ui
     auto it = S->all.find(ticket);
     if (it == S->all.end()) return COSNARK_E_ARG;
     Request *r = it->second;
+    // one wait per ticket: a second waiter would read the request after the
+    // first one deleted it
+    if (r->claimed) return COSNARK_E_ARG;
+    r->claimed = true;
     S->cv_done.wait(lk, [&] { return r->done; });
     if (h_proof && r->status == COSNARK_OK) memcpy(h_proof, r->proof.data(), r->proof.size());
     if (prev_out) {

@@ -863,7 +863,9 @@ std::vector<ScJob> sumcheck_jobs(const fe *v1, const fe *v2, uint64_t n, int fs_
         A.npairs = n / 4;
         A.st = st;
         A.part_out = part[1];
-        A.nb = width(n / 4);
+        // nine partials per workgroup: a slot of 3 * SC_MAX_BLOCKS holds
+        // at most SC_MAX_BLOCKS / 3 workgroups' worth
+        A.nb = std::min(width(n / 4), SC_MAX_BLOCKS / 3);
         A.round = 0; A.m = m; A.fs_rounds = fs_rounds;
         jobs.push_back(A);
         ScJob B{};
