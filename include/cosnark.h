@@ -165,6 +165,23 @@ int cosnark_prove_plain_ex(cosnark_ctx *ctx, const uint64_t *d_f, const uint64_t
                            uint64_t n, int rate, cosnark_transcript *ts,
                            uint8_t *h_proof, uint64_t *d_codeword, uint8_t *d_nodes,
                            double *times);
+/* The plain prover of cosnark_prove_plain_ex with no device allocation.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The caller passes the inputs and both outputs
+ * (d_codeword: rate*n elements, d_nodes: cosnark_merkle_node_count(rate*n)
+ * digests, both required); the NTT temporary and the sumcheck's fold
+ * buffers come from a fixed scratch array of
+ * cosnark_prove_plain_noalloc_max_bytes() bytes (96 MiB) that each GPU holds
+ * until the process ends.  Sizes needing more (rate*n + 1.5 n elements of 16
+ * bytes, rounded up to 256 bytes between the two parts: n = 2^20 at rate 4)
+ * return COSNARK_E_RANGE.  All calls on one GPU share that GPU's scratch:
+ * calls that overlap on one GPU (several host threads or contexts) race.
+ * Proof, codeword and tree equal cosnark_prove_plain_ex's. */
+uint64_t cosnark_prove_plain_noalloc_max_bytes(void);
+int cosnark_prove_plain_noalloc(cosnark_ctx *ctx, const uint64_t *d_f, const uint64_t *d_g,
+                                uint64_t n, int rate, cosnark_transcript *ts, uint8_t *h_proof,
+                                uint64_t *d_codeword, uint8_t *d_nodes);
 /* The verifier of a plain proof's sumcheck (the checks the reference's
  * prover makes, src/Distributed_Sumcheck.cpp:243-248): with y the claimed
  * sum and the transcript t as the prover started it, each round's

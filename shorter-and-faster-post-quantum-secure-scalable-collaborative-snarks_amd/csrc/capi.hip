@@ -64,6 +64,13 @@ int ntt_tmp_reserve(cosnark_ctx *ctx, size_t bytes, void **base) {
 
 static size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
+// Scratch of cosnark_prove_plain_noalloc: a file-scope device array, one
+// copy per GPU, held until the process ends.  Host code never uses the
+// array's name as an address: cosnark_create looks up this device's copy
+// with hipGetSymbolAddress and keeps it in the context.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)96 << 20;
+static __device__ __align__(256) uint8_t scratch[NOALLOC_SCRATCH_BYTES];
+
 // ---------------------------------------------------------------------
 // host side of the libc challenge stream (src/fieldElement.cpp:119-125,
 // 363-368): F::_random() = two 20-digit decimal numbers from random()
@@ -172,6 +179,9 @@ int cosnark_create(int device, cosnark_ctx **out) {
     if (hipHostGetDevicePointer((void **)&ctx->sc_host_dev, ctx->sc_host, 0) != hipSuccess)
         return fail(COSNARK_E_HIP);
     if (hipMemset(ctx->sc, 0, sizeof(ScState)) != hipSuccess) return fail(COSNARK_E_HIP);
+    // this device's copy of the no-allocation entry point's scratch
+    if (hipGetSymbolAddress(&ctx->noalloc_scratch, HIP_SYMBOL(scratch)) != hipSuccess || !ctx->noalloc_scratch)
+        return fail(COSNARK_E_HIP);
     if (sumcheck_init(ctx) != COSNARK_OK) return fail(COSNARK_E_HIP);
     if (ntt_init(ctx) != COSNARK_OK) return fail(COSNARK_E_HIP);
     if (copcs_init(ctx) != COSNARK_OK) return fail(COSNARK_E_HIP);
@@ -440,13 +450,26 @@ int cosnark_prove_plain(cosnark_ctx *ctx, const uint64_t *d_f, const uint64_t *d
     return cosnark_prove_plain_ex(ctx, d_f, d_g, n, rate, t, h_proof, nullptr, nullptr, times);
 }
 
-int cosnark_prove_plain_ex(cosnark_ctx *ctx, const uint64_t *d_f, const uint64_t *d_g, uint64_t n, int rate,
-                           cosnark_transcript *t, uint8_t *h_proof, uint64_t *d_codeword, uint8_t *d_nodes,
-                           double *times) {
+static int prove_plain_args(cosnark_ctx *ctx, const uint64_t *d_f, const uint64_t *d_g, uint64_t n, int rate,
+                            cosnark_transcript *t, uint8_t *h_proof) {
     if (!ctx || !d_f || !d_g || !t || !h_proof) return set_err(ctx, COSNARK_E_ARG, "prove_plain: null argument");
     if (!is_pow2(n) || n < 4 || rate < 1 || !is_pow2((uint64_t)rate) || n * (uint64_t)rate > (1ull << 28))
         return set_err(ctx, COSNARK_E_ARG, "prove_plain: n, rate must be powers of two, 4 <= n, n*rate <= 2^28");
-    const int m = ilog2(n);
+    return COSNARK_OK;
+}
+
+// The plain prover on given buffers: codeword cw (rate n elements), tree nd,
+// sumcheck buffers scb (n + n/2 elements), NTT temporary tmp (rate n
+// elements, or null: the context's own).
+static int prove_plain_core(cosnark_ctx *ctx, const uint64_t *d_f, const uint64_t *d_g, uint64_t n, int rate,
+                            cosnark_transcript *t, uint8_t *h_proof, fe *cw, uint8_t *nd, fe *scb, fe *tmp,
+                            double *times);
+
+int cosnark_prove_plain_ex(cosnark_ctx *ctx, const uint64_t *d_f, const uint64_t *d_g, uint64_t n, int rate,
+                           cosnark_transcript *t, uint8_t *h_proof, uint64_t *d_codeword, uint8_t *d_nodes,
+                           double *times) {
+    int rc = prove_plain_args(ctx, d_f, d_g, n, rate, t, h_proof);
+    if (rc) return rc;
     const uint64_t cw_n = n * (uint64_t)rate;
     const uint64_t nodes = cosnark_merkle_node_count(cw_n);
     // the codeword and the tree go to the caller's buffers when given (the
@@ -456,12 +479,43 @@ int cosnark_prove_plain_ex(cosnark_ctx *ctx, const uint64_t *d_f, const uint64_t
     const size_t off_sc = align_up(off_nodes + (d_nodes ? 0 : nodes * 32), 256);
     const size_t total = off_sc + sizeof(fe) * (n + n / 2);
     void *base;
-    int rc = scratch_reserve(ctx, total, &base);
+    rc = scratch_reserve(ctx, total, &base);
     if (rc) return rc;
     fe *cw = d_codeword ? (fe *)d_codeword : (fe *)((char *)base + off_cw);
     uint8_t *nd = d_nodes ? d_nodes : (uint8_t *)base + off_nodes;
     fe *scb = (fe *)((char *)base + off_sc);
-    rc = sc_stage_state(ctx, gf::mk(t->prev[0], t->prev[1]), gf::zero(), 1, m, m, ctx->stream);
+    return prove_plain_core(ctx, d_f, d_g, n, rate, t, h_proof, cw, nd, scb, nullptr, times);
+}
+
+uint64_t cosnark_prove_plain_noalloc_max_bytes(void) { return NOALLOC_SCRATCH_BYTES; }
+
+int cosnark_prove_plain_noalloc(cosnark_ctx *ctx, const uint64_t *d_f, const uint64_t *d_g, uint64_t n, int rate,
+                                cosnark_transcript *t, uint8_t *h_proof, uint64_t *d_codeword, uint8_t *d_nodes) {
+    int rc = prove_plain_args(ctx, d_f, d_g, n, rate, t, h_proof);
+    if (rc) return rc;
+    if (!d_codeword || !d_nodes) return set_err(ctx, COSNARK_E_ARG, "prove_plain_noalloc: null output buffer");
+    if (!ctx->noalloc_scratch) return set_err(ctx, COSNARK_E_HIP, "prove_plain_noalloc: no scratch on this device");
+    // scratch: the NTT temporary (rate n elements), then the sumcheck's
+    // ping-pong buffers (n + n/2 elements); every byte is written before it
+    // is read, so nothing is cleared
+    const uint64_t cw_n = n * (uint64_t)rate;
+    const size_t off_sc = align_up(cw_n * sizeof(fe), 256);
+    const size_t need = off_sc + sizeof(fe) * (n + n / 2);
+    if (need > NOALLOC_SCRATCH_BYTES)
+        return set_err(ctx, COSNARK_E_RANGE, "prove_plain_noalloc: rate*n + 1.5 n elements exceed the " +
+                                                 std::to_string(NOALLOC_SCRATCH_BYTES >> 20) + " MiB scratch");
+    uint8_t *base = (uint8_t *)ctx->noalloc_scratch;
+    return prove_plain_core(ctx, d_f, d_g, n, rate, t, h_proof, (fe *)d_codeword, d_nodes, (fe *)(base + off_sc),
+                            (fe *)base, nullptr);
+}
+
+static int prove_plain_core(cosnark_ctx *ctx, const uint64_t *d_f, const uint64_t *d_g, uint64_t n, int rate,
+                            cosnark_transcript *t, uint8_t *h_proof, fe *cw, uint8_t *nd, fe *scb, fe *tmp,
+                            double *times) {
+    const int m = ilog2(n);
+    const uint64_t cw_n = n * (uint64_t)rate;
+    const uint64_t nodes = cosnark_merkle_node_count(cw_n);
+    int rc = sc_stage_state(ctx, gf::mk(t->prev[0], t->prev[1]), gf::zero(), 1, m, m, ctx->stream);
     if (rc) return rc;
     // commit (encode + Merkle) on s1, the sumcheck beside it on s2: the
     // transcript never absorbs the root, so the two are independent
@@ -471,7 +525,7 @@ int cosnark_prove_plain_ex(cosnark_ctx *ctx, const uint64_t *d_f, const uint64_t
     if (s1 != ctx->stream) HIPCK(hipStreamWaitEvent(s1, ctx->ev[7], 0));
     if (times) HIPCK(hipEventRecord(ctx->ev[0], s1));
     int levels_done = 0;
-    if ((rc = ntt_launch_leaves(ctx, (const fe *)d_f, n, cw, ilog2(cw_n), nd, &levels_done, s1))) return rc;
+    if ((rc = ntt_launch_leaves(ctx, (const fe *)d_f, n, cw, ilog2(cw_n), nd, &levels_done, s1, tmp))) return rc;
     if (times) HIPCK(hipEventRecord(ctx->ev[1], s1));
     if (s2 != ctx->stream) HIPCK(hipStreamWaitEvent(s2, ctx->ev[7], 0));
     if (times) HIPCK(hipEventRecord(ctx->ev[4], s2));
@@ -486,11 +540,11 @@ int cosnark_prove_plain_ex(cosnark_ctx *ctx, const uint64_t *d_f, const uint64_t
     if (s1 != ctx->stream) HIPCK(hipStreamWaitEvent(ctx->stream, ctx->ev[2], 0));
     if (s2 != ctx->stream) HIPCK(hipStreamWaitEvent(ctx->stream, ctx->ev[3], 0));
     if (times) HIPCK(hipEventRecord(ctx->ev[5], ctx->stream));
-    std::vector<uint64_t> polys(6 * m), fin(4);
-    rc = sc_collect(ctx, m, t, polys.data(), nullptr, fin.data(), ctx->stream, nd + 32 * (nodes - 1));
+    uint64_t polys[6 * 64], fin[4];
+    rc = sc_collect(ctx, m, t, polys, nullptr, fin, ctx->stream, nd + 32 * (nodes - 1));
     memcpy(h_proof, ctx->sc_host->root, 32);
-    memcpy(h_proof + 32, polys.data(), 48ull * m);
-    memcpy(h_proof + 32 + 48ull * m, fin.data(), 32);
+    memcpy(h_proof + 32, polys, 48ull * m);
+    memcpy(h_proof + 32 + 48ull * m, fin, 32);
     if (times) {
         float a = 0, b = 0, c = 0, tot = 0;
         (void)hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]);

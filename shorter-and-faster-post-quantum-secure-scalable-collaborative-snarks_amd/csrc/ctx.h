@@ -59,6 +59,7 @@ struct cosnark_ctx {
     // per-size plan (0 = the defaults above); set by the autotuner
     struct { int passbits, tile; } ntt_plan[NTT_LOG_MAX + 1] = {};
     bool overlap = true;              // plain prover: sumcheck beside the commit
+    void *noalloc_scratch = nullptr;  // this device's copy of the file-scope scratch array (capi.hip)
     int sc_blocks = 256;              // workgroups of a multi-workgroup sumcheck round (<= SC_MAX_BLOCKS)
     uint64_t sc_tail_elems = 4096;    // vectors this short finish in the one-workgroup tail (power of two, >= 4)
 };
@@ -113,7 +114,7 @@ int ntt_launch_batch(cosnark_ctx *ctx, const gf::fe *in, const gf::fe *const *in
 // Merkle leaf digests written by the last pass where the plan allows it
 // (*levels_done, as above); the caller builds the rest of the tree.
 int ntt_launch_leaves(cosnark_ctx *ctx, const gf::fe *in, uint64_t in_len, gf::fe *out, int logn, uint8_t *nodes,
-                      int *levels_done, hipStream_t s);
+                      int *levels_done, hipStream_t s, gf::fe *tmp = nullptr);
 int ntt_plan_bits(const cosnark_ctx *ctx, int logn);
 int ntt_plan_tile(const cosnark_ctx *ctx, int logn);
 // Merkle commit of n_elems elements into nodes (all levels).

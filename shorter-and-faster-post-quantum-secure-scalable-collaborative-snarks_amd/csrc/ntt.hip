@@ -412,8 +412,8 @@ int ntt_launch_batch(cosnark_ctx *ctx, const fe *in, const fe *const *in_tab, ui
 }
 
 int ntt_launch_leaves(cosnark_ctx *ctx, const fe *in, uint64_t in_len, fe *out, int logn, uint8_t *nodes,
-                      int *levels_done, hipStream_t stream) {
-    return ntt_launch_batch(ctx, in, nullptr, in_len, out, nullptr, 1, logn, false, stream, nullptr, nodes,
+                      int *levels_done, hipStream_t stream, fe *tmp) {
+    return ntt_launch_batch(ctx, in, nullptr, in_len, out, nullptr, 1, logn, false, stream, tmp, nodes,
                             levels_done);
 }
 
