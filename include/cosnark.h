@@ -61,6 +61,12 @@ int cosnark_d2d(cosnark_ctx *ctx, void *d_dst, const void *d_src, uint64_t bytes
  * for the few host-side combinations around the kernels (sumcheck masks). */
 void cosnark_field_mul(const uint64_t a[2], const uint64_t b[2], uint64_t out[2]);
 void cosnark_field_add(const uint64_t a[2], const uint64_t b[2], uint64_t out[2]);
+/* blake3_hash(src, dst) (src/Blake3_hash.cpp:5-9): BLAKE3 of one 64-byte
+ * block, 32-byte digest (host) */
+void cosnark_blake3_64(const uint8_t in[64], uint8_t out[32]);
+/* F::_random() (src/fieldElement.cpp:119-125, 363-368): two 20-digit
+ * decimal draws from libc random() on the calling thread, reduced mod p */
+void cosnark_field_random(uint64_t out[2]);
 
 /* ---- synthetic workload inputs ---------------------------------------
  * n elements uniform in GF(p^2): splitmix64-seeded xoshiro256**, each
@@ -249,6 +255,25 @@ int cosnark_whir_rates(const uint64_t *sizes, int count, int rate, int whir_k, i
  * fails (a round, the closing K = <arr, b>, the closing codeword = RS(arr))
  * or where the reference exits (a query beyond the tree, codeword sizes
  * that differ at an aggregation).  whir_k a power of two in [2, 64]. */
+/* _Whir_Open (src/Whir.cpp:469-545): the prover work of one opening of
+ * d_arr (n elements) with explicit weights d_b (n elements; Whir_Open,
+ * :388-412, builds them from its claims) and claim K, at fold factor whir_k
+ * (a power of two in [2, 64]; the reference's global, 8 by default), down to
+ * |arr| <= whir_k.  d_codeword (cw_n elements, the encoding at `rate`) is
+ * opened against its dup-left tree d_nodes (as cosnark_whir_commit writes
+ * it), or, when d_nodes is null, against the tree over zero_leaves all-zero
+ * leaf digests that aggregate_commitments (:547-606) leaves.  Challenges
+ * come from libc random()/rand() in the reference's order.  The record has
+ * cosnark_whir_batch_open's layout (one array: iterations only).
+ * h_challenges receives total_r (log2 n elements: every round's challenge,
+ * then the closing ones) and final_claim the multilinear evaluation of the
+ * initial weights at total_r: the pair _Whir_Open returns (:517-518).
+ * COSNARK_E_CHECK where the reference's checks fail. */
+int cosnark_whir_open_weights(cosnark_ctx *ctx, const uint64_t *d_arr, uint64_t n, const uint64_t *d_b,
+                              const uint64_t K[2], const uint64_t *d_codeword, uint64_t cw_n,
+                              const uint8_t *d_nodes, uint64_t zero_leaves, int rate, int whir_k,
+                              uint64_t *h_record, uint64_t record_cap, uint64_t *words,
+                              uint64_t *h_challenges, uint64_t final_claim[2]);
 /* precompute_beta(r, B), src/utils.cpp:198-243: the 2^m eq weights
  * B[x] = prod_k (bit_k(x) ? r_k : 1 - r_k) of the point h_z (m elements,
  * r_0 on bit 0) into d_out.  1 <= m <= 28. */

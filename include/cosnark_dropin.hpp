@@ -28,7 +28,10 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <algorithm>
+#include <cmath>
 #include <type_traits>
+#include <unordered_map>
 #include <utility>
 #include <vector>
 #include "cosnark.h"
@@ -223,6 +226,37 @@ std::vector<H> open_tree_blake(std::vector<std::vector<H>> &MT_hashes, int pos) 
     return path;
 }
 
+// verify_claim_opt_blake(MT, path, pos, N, visited, ps), :337-383: the
+// reference's verifier walks from pos towards the root until it meets a
+// node an earlier path already visited, marking visited nodes (indices
+// N + pos, halved per level) and adding 32/1024 to the proof-size counter
+// ps per level; it hashes with my_hhash and never compares the result with
+// the root, so it returns true.  This does the same bookkeeping and exits
+// (throws) where the reference does; cosnark_merkle_verify checks paths
+// against the root on the GPU.
+template <class H>
+bool verify_claim_opt_blake(std::vector<std::vector<H>> &MT, const H *path, int pos_element_arr, int N,
+                            std::vector<bool> &visited, double &ps) {
+    (void)path;
+    if (N <= 0 || (N & (-N)) != N) throw error(COSNARK_E_ARG, "verify_claim_opt_blake: N not a power of two");
+    if (MT.empty() || (int)MT[0].size() != N) throw error(COSNARK_E_CHECK, "verify_claim_opt_blake: error in sizes");
+    int pos = pos_element_arr;
+    if (pos < 0 || pos >= (int)MT[0].size()) throw error(COSNARK_E_CHECK, "verify_claim_opt_blake: position out of range");
+    if (visited.size() < 2 * (size_t)N) throw error(COSNARK_E_ARG, "verify_claim_opt_blake: visited shorter than 2N");
+    int pos_element = N + pos_element_arr;
+    for (size_t i = 0; i + 1 < MT.size(); ++i) {
+        if ((pos_element ^ 1) >= 2 * N) throw error(COSNARK_E_CHECK, "verify_claim_opt_blake: node index out of range");
+        if (visited[pos_element ^ 1]) return true;
+        if ((size_t)(pos ^ 1) >= MT[i].size()) throw error(COSNARK_E_CHECK, "verify_claim_opt_blake: error in verification");
+        visited[pos_element ^ 1] = true;
+        pos_element /= 2;
+        pos /= 2;
+        visited[pos_element] = true;
+        ps += 32.0 / 1024.0;
+    }
+    return true;
+}
+
 // ---- product sumcheck ----------------------------------------------------------
 
 // _quadratic_sumcheck(y, v1, v2, N), src/Distributed_Sumcheck.cpp:204-348 at
@@ -251,7 +285,71 @@ std::vector<std::pair<F, std::vector<F>>> _quadratic_sumcheck(F y, std::vector<F
     return {std::make_pair(detail::make<F>(fin[0], fin[1]), rr), std::make_pair(detail::make<F>(fin[2], fin[3]), rr)};
 }
 
-// ---- WHIR, src/Whir.cpp (whir_k = 4) -----------------------------------------------
+// ---- WHIR, src/Whir.cpp ------------------------------------------------------
+
+// The fold factor, the reference's global `int whir_k = 8;` (src/Whir.cpp:6).
+inline int &whir_k() { static int v = 8; return v; }
+
+// get_whir_rates' globals (declared extern at src/Whir.cpp:8-16): the
+// sizes it reads and the schedules it writes.
+inline std::vector<int> &whir_sizes() { static std::vector<int> v; return v; }
+inline std::vector<int> &whir_rates() { static std::vector<int> v; return v; }
+inline std::vector<int> &sorted_indexes() { static std::vector<int> v; return v; }
+inline std::vector<int> &sorted_whir_rates() { static std::vector<int> v; return v; }
+inline std::unordered_map<int, int> &whir_kappas() { static std::unordered_map<int, int> v; return v; }
+
+namespace detail {
+// every level of a dup-left tree over `leaves` all-zero digests: what
+// Whir_Commit, Whir_Commit_Codeword and aggregate_commitments leave (their
+// leaf vector is never filled), level l holding copies of the l-fold
+// self-hash of the zero digest
+template <class H>
+void zero_tree(uint64_t leaves, std::vector<std::vector<H>> &MT) {
+    if (leaves < 1 || !pow2(leaves)) throw error(COSNARK_E_ARG, "zero tree: leaf count not a power of two");
+    const int depth = ilog2(leaves);
+    MT.assign(depth + 1, {});
+    uint8_t cur[32] = {0}, blk[64];
+    for (int l = 0; l <= depth; ++l) {
+        if (l > 0) {
+            std::memcpy(blk, cur, 32);
+            std::memcpy(blk + 32, cur, 32);
+            cosnark_blake3_64(blk, cur);
+        }
+        H h;
+        std::memcpy(&h, cur, 32);
+        MT[l].assign(leaves >> l, h);
+    }
+}
+
+// the query count of whir_query and aggregate_commitments without the
+// conjectured-security option (src/Whir.cpp:253-258, :567-572)
+inline int whir_query_count(int rate) { return (int)(-100.0 / std::log2(std::sqrt(1.0 / (double)rate))); }
+
+// a tree to open on the device: the dup-left tree of the codeword
+// (MT_commit_Blake's levels) or an all-zero-leaf tree, told apart by its
+// leaf digests; returns the zero tree's leaf count (0: nodes holds the tree)
+template <class H>
+uint64_t tree_or_zero(const std::vector<std::vector<H>> &MT, uint64_t cw, std::unique_ptr<Dev> &nodes,
+                      const char *what) {
+    if (MT.empty() || MT[0].empty()) throw error(COSNARK_E_ARG, std::string(what) + ": empty tree");
+    static const uint8_t zero[32] = {0};
+    bool all_zero = true;
+    for (const H &h : MT[0])
+        if (std::memcmp(&h, zero, 32) != 0) { all_zero = false; break; }
+    if (all_zero) return MT[0].size();
+    nodes.reset(new Dev(32 * cosnark_merkle_node_count(cw)));
+    const uint64_t cnt = cosnark_merkle_node_count(cw);
+    if (MT[0].size() != cw / 4) throw error(COSNARK_E_ARG, std::string(what) + ": tree does not match the codeword");
+    uint64_t off = 0;
+    for (size_t l = 0; l < MT.size(); ++l) {
+        if (off + MT[l].size() > cnt) throw error(COSNARK_E_ARG, std::string(what) + ": tree larger than expected");
+        check(cosnark_h2d(ctx(), nodes->u8() + 32 * off, MT[l].data(), 32 * MT[l].size()), "copy in");
+        off += MT[l].size();
+    }
+    if (off != cnt) throw error(COSNARK_E_ARG, std::string(what) + ": tree smaller than expected");
+    return 0;
+}
+}  // namespace detail
 
 // Whir_Commit_initial(arr, codeword, MT, rate, kappa), :158-190: codeword =
 // the rate*|arr| evaluations of arr's coefficients, MT its tree (any kappa)
@@ -271,42 +369,239 @@ void Whir_Commit_initial(std::vector<F> &arr, std::vector<F> &codeword, std::vec
     detail::unpack_levels(h, cw / 4, MT);
 }
 
-// The opening record of the thread's last Whir_Open (cosnark_whir_open's
-// layout), for callers that serialise or verify it.
+// Whir_Commit(arr, codeword, MT, rate, kappa = whir_k), :192-213: the same
+// codeword; MT is the tree the reference leaves, over |codeword| / kappa
+// leaf digests it never fills (all zero).
+template <class F, class H>
+void Whir_Commit(std::vector<F> &arr, std::vector<F> &codeword, std::vector<std::vector<H>> &MT, int rate,
+                 int kappa = whir_k()) {
+    detail::elem_type<F>();
+    detail::digest_type<H>();
+    const uint64_t n = arr.size(), cw = n * (uint64_t)rate;
+    if (!detail::pow2(n) || rate < 1 || !detail::pow2((uint64_t)rate) || kappa < 1 || cw < (uint64_t)kappa)
+        throw error(COSNARK_E_ARG, "Whir_Commit: sizes");
+    detail::Dev a(arr.data(), 16 * n), c(16 * cw);
+    detail::check(cosnark_rs_encode(detail::ctx(), a.u64(), n, rate, c.u64()), "Whir_Commit");
+    codeword.resize(cw);
+    c.get(codeword.data(), 16 * cw);
+    detail::zero_tree(cw / (uint64_t)kappa, MT);
+}
+
+// Whir_Commit_Codeword(codeword, MT, rate), :139-156: MT over
+// |codeword| / whir_k all-zero leaf digests (the codeword is not read).
+template <class F, class H>
+void Whir_Commit_Codeword(std::vector<F> &codeword, std::vector<std::vector<H>> &MT, int) {
+    detail::elem_type<F>();
+    detail::digest_type<H>();
+    detail::zero_tree(codeword.size() / (uint64_t)whir_k(), MT);
+}
+
+// The opening record of the thread's last Whir_Open / _Whir_Open /
+// Whir_Batch_Open (cosnark_whir_batch_open's layout), for callers that
+// serialise or verify it.
 inline std::vector<uint64_t> &last_whir_record() {
     thread_local std::vector<uint64_t> rec;
     return rec;
 }
 
-// Whir_Open(arr, codeword, MT, claims, rate), :388-467, one claim (K, z):
-// the iterations down to |arr| <= 4, challenges from libc random()/rand()
-// in the reference's order.  Throws where the reference's checks exit.
+namespace detail {
+template <class F, class H>
+std::pair<F, std::vector<F>> whir_open_weights(std::vector<F> &arr, std::vector<F> &codeword,
+                                               std::vector<std::vector<H>> &MT, const Dev &b, const F &K, int rate,
+                                               const char *what) {
+    const uint64_t n = arr.size(), cw = codeword.size();
+    if (!pow2(n) || n < 2 || !pow2(cw) || cw < 2 * n) throw error(COSNARK_E_ARG, std::string(what) + ": sizes");
+    std::unique_ptr<Dev> nodes;
+    const uint64_t zl = tree_or_zero(MT, cw, nodes, what);
+    Dev a(arr.data(), 16 * n), c(codeword.data(), 16 * cw);
+    uint64_t k[2], fin[2];
+    std::memcpy(k, &K, 16);
+    auto &rec = last_whir_record();
+    rec.assign(1u << 22, 0);
+    uint64_t words = 0;
+    std::vector<F> r((size_t)ilog2(n));
+    check(cosnark_whir_open_weights(ctx(), a.u64(), n, b.u64(), k, c.u64(), cw, nodes ? nodes->u8() : nullptr, zl,
+                                    rate, whir_k(), rec.data(), rec.size(), &words,
+                                    reinterpret_cast<uint64_t *>(r.data()), fin),
+          what);
+    rec.resize(words);
+    return {make<F>(fin[0], fin[1]), r};
+}
+}  // namespace detail
+
+// Whir_Open(arr, codeword, MT, claims, rate), :388-467, at the global
+// whir_k: the weights are eq(claims[0]'s point); with more claims one
+// F::_random() r batches them, K += r^i K_i and b += r^i eq(claims[1]'s
+// point) for every i >= 1 -- the reference reads claims[1] there (:403), so
+// three or more claims at distinct points fail the first sumcheck check,
+// where the reference exits.  Iterations down to |arr| <= whir_k,
+// challenges from libc random()/rand() in the reference's order.  Throws
+// where the reference's checks exit.  arr, codeword and MT are left as
+// given (the reference folds them).
 template <class F, class H>
 void Whir_Open(std::vector<F> &arr, std::vector<F> &codeword, std::vector<std::vector<H>> &MT,
                std::vector<std::pair<F, std::vector<F>>> claims, int rate) {
     detail::elem_type<F>();
     detail::digest_type<H>();
-    if (claims.size() != 1) throw error(COSNARK_E_ARG, "Whir_Open: one claim only");
-    const uint64_t n = arr.size(), cw = n * (uint64_t)rate, cnt = cosnark_merkle_node_count(cw);
-    if (codeword.size() != cw || MT.empty() || MT[0].size() != cw / 4)
-        throw error(COSNARK_E_ARG, "Whir_Open: codeword / tree do not match arr and rate");
-    detail::Dev a(arr.data(), 16 * n), c(codeword.data(), 16 * cw), nodes(32 * cnt);
-    uint64_t off = 0;
-    for (size_t l = 0; l < MT.size(); ++l) {
-        if (off + MT[l].size() > cnt) throw error(COSNARK_E_ARG, "Whir_Open: tree larger than expected");
-        detail::check(cosnark_h2d(detail::ctx(), nodes.u8() + 32 * off, MT[l].data(), 32 * MT[l].size()), "copy in");
-        off += MT[l].size();
-    }
-    if (off != cnt) throw error(COSNARK_E_ARG, "Whir_Open: tree smaller than expected");
-    uint64_t K[2];
-    std::memcpy(K, &claims[0].first, 16);
-    auto &rec = last_whir_record();
-    rec.assign(1u << 22, 0);
-    uint64_t words = 0;
-    detail::check(cosnark_whir_open(detail::ctx(), a.u64(), n, reinterpret_cast<const uint64_t *>(claims[0].second.data()),
-                                    K, rate, 0, c.u64(), nodes.u8(), rec.data(), rec.size(), &words),
+    if (claims.empty()) throw error(COSNARK_E_ARG, "Whir_Open: no claim");
+    const uint64_t n = arr.size();
+    const int m = detail::ilog2(n);
+    for (auto &cl : claims)
+        if ((int)cl.second.size() != m) throw error(COSNARK_E_ARG, "Whir_Open: claim point size != log2 |arr|");
+    F K = claims[0].first;
+    detail::Dev b(16 * n);
+    detail::check(cosnark_precompute_beta(detail::ctx(), reinterpret_cast<const uint64_t *>(claims[0].second.data()),
+                                          m, b.u64()),
                   "Whir_Open");
-    rec.resize(words);
+    if (claims.size() > 1) {
+        uint64_t r[2], p[2], psum[2] = {0, 0};
+        cosnark_field_random(r);                                    // :399
+        std::memcpy(p, r, 16);
+        for (size_t i = 1; i < claims.size(); ++i) {
+            uint64_t t[2], k[2];
+            std::memcpy(k, &K, 16);
+            cosnark_field_mul(p, reinterpret_cast<const uint64_t *>(&claims[i].first), t);
+            cosnark_field_add(k, t, k);
+            std::memcpy(&K, k, 16);
+            cosnark_field_add(psum, p, psum);
+            cosnark_field_mul(p, r, p);
+        }
+        // b = eq(z_0) + (sum of the powers) eq(z_1)
+        detail::Dev both(32 * n);
+        detail::check(cosnark_d2d(detail::ctx(), both.u8(), b.u8(), 16 * n), "Whir_Open");
+        detail::check(cosnark_precompute_beta(detail::ctx(), reinterpret_cast<const uint64_t *>(claims[1].second.data()),
+                                              m, both.u64() + 2 * n),
+                      "Whir_Open");
+        const uint64_t c[4] = {1, 0, psum[0], psum[1]};
+        detail::check(cosnark_row_combine(detail::ctx(), both.u64(), 2, n, c, b.u64()), "Whir_Open");
+    }
+    detail::whir_open_weights(arr, codeword, MT, b, K, rate, "Whir_Open");
+}
+
+// _Whir_Open(arr, codeword, MT, b, claims, challenges, K, rate, Verify),
+// :469-545, at the global whir_k: returns the final claim, the initial
+// weights' multilinear evaluation at every challenge, with those challenges
+// (total_r).  claims and challenges feed only the reference's verifier-side
+// step 3, which does not exit.  MT may be MT_commit_Blake's tree of the
+// codeword or the all-zero-leaf tree aggregate_commitments leaves.  A failed
+// check throws for either Verify (the reference continues when Verify is
+// false).  arr, codeword, MT and b are left as given (the reference folds
+// them).
+template <class F, class H>
+std::pair<F, std::vector<F>> _Whir_Open(std::vector<F> &arr, std::vector<F> &codeword, std::vector<std::vector<H>> &MT,
+                                        std::vector<F> &b, std::vector<std::vector<std::pair<F, std::vector<F>>>>,
+                                        std::vector<F>, F K, int rate, bool = true) {
+    detail::elem_type<F>();
+    detail::digest_type<H>();
+    if (b.size() != arr.size()) throw error(COSNARK_E_ARG, "_Whir_Open: weights and array sizes differ");
+    detail::Dev db(b.data(), 16 * b.size());
+    return detail::whir_open_weights(arr, codeword, MT, db, K, rate, "_Whir_Open");
+}
+
+// aggregate_commitments(codeword, MT, codewords_aggr, MT_aggr, r, rate),
+// :547-606: codeword = sum_i r[i] codewords_aggr[i]; l = the query count at
+// `rate` positions rand() % (|codeword| / 4) are opened on MT_aggr[0] (the
+// paths feed only the proof-size counter); MT becomes the tree over
+// |codeword| / whir_k all-zero leaf digests.  Throws where the reference
+// exits (codewords of different sizes, a position beyond MT_aggr[0]).
+template <class F, class H>
+void aggregate_commitments(std::vector<F> &codeword, std::vector<std::vector<H>> &MT,
+                           std::vector<std::vector<F>> &codewords_aggr, std::vector<std::vector<std::vector<H>>> &MT_aggr,
+                           std::vector<F> r, int rate) {
+    detail::elem_type<F>();
+    detail::digest_type<H>();
+    const size_t c = codewords_aggr.size();
+    if (!c || r.size() < c || MT_aggr.empty()) throw error(COSNARK_E_ARG, "aggregate_commitments: arguments");
+    const uint64_t L = codewords_aggr[0].size();
+    for (size_t i = 0; i < c; ++i)
+        if (codewords_aggr[i].size() != L) throw error(COSNARK_E_CHECK, "aggregate_commitments: incorrect codeword size");
+    // codeword = sum r_i C_i, 32 rows per combine (the partial sum carried on)
+    detail::Dev acc(16 * L), rows(16 * L * (c < 32 ? c : 32));
+    size_t done = 0;
+    bool first = true;
+    while (done < c) {
+        const size_t take = std::min<size_t>(c - done, first ? 32 : 31);
+        std::vector<uint64_t> coef;
+        size_t row = 0;
+        if (!first) {
+            detail::check(cosnark_d2d(detail::ctx(), rows.u8(), acc.u8(), 16 * L), "aggregate_commitments");
+            coef.push_back(1);
+            coef.push_back(0);
+            ++row;
+        }
+        for (size_t i = 0; i < take; ++i, ++row) {
+            detail::check(cosnark_h2d(detail::ctx(), rows.u8() + 16 * L * row, codewords_aggr[done + i].data(), 16 * L),
+                          "copy in");
+            const uint64_t *ri = reinterpret_cast<const uint64_t *>(&r[done + i]);
+            coef.push_back(ri[0]);
+            coef.push_back(ri[1]);
+        }
+        detail::check(cosnark_row_combine(detail::ctx(), rows.u64(), (int)row, L, coef.data(), acc.u64()),
+                      "aggregate_commitments");
+        done += take;
+        first = false;
+    }
+    codeword.resize(L);
+    acc.get(codeword.data(), 16 * L);
+    const int l = detail::whir_query_count(rate);
+    for (int i = 0; i < l; ++i) {
+        const int q = rand() % (int)(L / 4);                        // :574-575
+        (void)open_tree_blake(MT_aggr[0], q);                       // exits (throws) beyond the tree
+    }
+    detail::zero_tree(L / (uint64_t)whir_k(), MT);
+}
+
+// get_whir_rates(rate), :18-137, on the globals: reads whir_sizes(); writes
+// sorted_indexes() (array i's place in the largest-first order, equal sizes
+// in reverse index order, as the reference's sort gives them), appends the
+// commitment rates in that order to sorted_whir_rates() (the reference
+// never clears it) and copies it to whir_rates(), and sets whir_kappas()
+// (size -> fold factor where the schedule meets that size).  It does not
+// print the schedule as the reference's rank 0 does.
+inline void get_whir_rates(int rate) {
+    const std::vector<int> &ws = whir_sizes();
+    if (ws.empty()) throw error(COSNARK_E_ARG, "get_whir_rates: whir_sizes is empty");
+    const int wk = whir_k();
+    std::vector<std::pair<int, int>> sp(ws.size());
+    for (size_t i = 0; i < ws.size(); ++i) sp[i] = {ws[i], (int)i};
+    std::sort(sp.begin(), sp.end());                                 // :23-25
+    std::vector<int> order;
+    sorted_indexes().assign(ws.size(), 0);
+    for (int i = (int)ws.size() - 1; i >= 0; --i) {
+        order.push_back(sp[i].second);
+        sorted_indexes()[sp[i].second] = (int)ws.size() - 1 - i;
+    }
+    std::vector<uint64_t> sorted;
+    for (int idx : order) sorted.push_back((uint64_t)ws[idx]);
+    std::vector<int> rates(sorted.size());
+    if (cosnark_whir_rates(sorted.data(), (int)sorted.size(), rate, wk, rates.data()))
+        throw error(COSNARK_E_ARG, "get_whir_rates: bad sizes");
+    for (int x : rates) sorted_whir_rates().push_back(x);
+    // the fold factor at each size the schedule meets (:76-123)
+    std::vector<int> s2(sorted.begin(), sorted.end());
+    int size = s2[0];
+    s2.erase(s2.begin());
+    while (!s2.empty() && s2[0] == size) s2.erase(s2.begin());
+    if (s2.empty()) whir_kappas()[size] = wk;
+    bool flag = true;
+    while (!s2.empty()) {
+        if (size / wk > s2[0]) {
+            if (flag) {
+                whir_kappas()[size] = wk;
+                flag = false;
+            }
+            size /= wk;
+        } else {
+            if (flag) whir_kappas()[size] = size / s2[0];
+            size = s2[0];
+            flag = true;
+            const int s = s2[0];
+            while (!s2.empty() && s2[0] == s) s2.erase(s2.begin());
+            if (s2.empty()) whir_kappas()[size] = wk;
+        }
+        if (size <= wk) break;
+    }
+    whir_rates() = sorted_whir_rates();
 }
 
 // ---- coPCS, src/coPCS.cpp and src/coPCS_utils.cpp ----------------------------------
@@ -573,10 +868,10 @@ void tree_in(const std::vector<std::vector<H>> &MT, uint64_t cw, Dev &nodes, con
 }
 }  // namespace detail
 
-// get_whir_rates(rate), src/Whir.cpp:18-137.  The reference reads the global
-// whir_sizes and writes whir_rates; here the sizes (sorted largest first)
-// come in and the rates go out.
-inline std::vector<int> get_whir_rates(const std::vector<int> &sizes, int rate, int whir_k = 4) {
+// get_whir_rates with the sizes (sorted largest first) passed in and the
+// rates returned, for callers that keep no globals: the rates
+// get_whir_rates(rate) appends for the same sizes.
+inline std::vector<int> get_whir_rates(const std::vector<int> &sizes, int rate, int whir_k) {
     std::vector<uint64_t> sz(sizes.begin(), sizes.end());
     std::vector<int> rates(sizes.size());
     if (sizes.empty() || cosnark_whir_rates(sz.data(), (int)sz.size(), rate, whir_k, rates.data()))
@@ -585,7 +880,8 @@ inline std::vector<int> get_whir_rates(const std::vector<int> &sizes, int rate, 
 }
 
 // Whir_Batch_Open(arrays, codewords, MTs, evals, betas, rate, Verify),
-// src/Whir.cpp:706-899, with whir_k the global's value.  The arrays are
+// src/Whir.cpp:706-899, with the global whir_k (or the last argument when
+// it is given).  The arrays are
 // sorted largest first, each committed at get_whir_rates' rate.  The
 // reference consumes its vector arguments as it aggregates them; they are
 // left empty here.  Its record is in last_whir_record().  A failed check
@@ -593,7 +889,8 @@ inline std::vector<int> get_whir_rates(const std::vector<int> &sizes, int rate, 
 template <class F, class H>
 void Whir_Batch_Open(std::vector<std::vector<F>> &arrays, std::vector<std::vector<F>> &codewords,
                      std::vector<std::vector<std::vector<H>>> &MTs, std::vector<F> evals,
-                     std::vector<std::vector<F>> &betas, int rate, bool, int whir_k = 4) {
+                     std::vector<std::vector<F>> &betas, int rate, bool, int wk = 0) {
+    const int whir_k = wk ? wk : cosnark::dropin::whir_k();
     detail::elem_type<F>();
     detail::digest_type<H>();
     const size_t c = arrays.size();

@@ -563,6 +563,8 @@ typedef struct {
     uint64_t mt_leaves;
     or_fe K;
     int rate;
+    or_fe *chal;        /* when set: every challenge of the opening, in order (_Whir_Open's total_r) */
+    int nchal;
 } whir_state;
 
 /* node of level l of a dup-left tree over zero digests: the l-fold
@@ -621,6 +623,7 @@ static int whir_iteration(whir_state *S, int kappa, rec_t *R, int iter) {
         round_poly(S->arr, S->b, L, p);
         if (!fe_eq(or_add(quad_eval(p, zero), quad_eval(p, one)), S->K)) return -10 - iter;
         rr[rd] = or_random_element();
+        if (S->chal) S->chal[S->nchal++] = rr[rd];
         S->K = quad_eval(p, rr[rd]);
         fold(S->arr, L, rr[rd]);
         fold(S->b, L, rr[rd]);
@@ -730,7 +733,11 @@ static int whir_final(whir_state *S, int kappa, rec_t *R) {
     or_fe Kc = S->K;
     for (uint64_t i = 0; i < S->n; ++i) Kc = or_sub(Kc, or_mul(S->arr[i], S->b[i]));
     if (Kc.real != 0 || Kc.img != 0) return -2;
-    for (int i = 0; i < ilog2u(S->n); ++i) rec_fe(R, or_random_element());
+    for (int i = 0; i < ilog2u(S->n); ++i) {
+        const or_fe r = or_random_element();
+        if (S->chal) S->chal[S->nchal++] = r;
+        rec_fe(R, r);
+    }
     return 0;
 }
 
@@ -742,6 +749,7 @@ int64_t or_whir_open(const or_fe *arr_in, uint64_t n, const or_fe *z, or_fe K,
     uint64_t stop = WHIR_K;
     if (stop_log > 0 && (1ULL << stop_log) > stop) stop = 1ULL << stop_log;
     whir_state S;
+    memset(&S, 0, sizeof(S));
     S.n = n;
     S.arr = (or_fe *)malloc(sizeof(or_fe) * n);
     S.b = (or_fe *)malloc(sizeof(or_fe) * n);
@@ -833,15 +841,18 @@ int or_whir_kappas(const uint64_t *sizes, int count, int whir_k, int *kappas, in
  *                    on the running tree, root of the new (zero-digest) tree
  *   final as in Whir_Open, with whir_k as it stands at the end.
  */
+/* Whir_Batch_Open's loop (Whir.cpp:762-899) from the running state S:
+ * iterations, aggregations, the closing steps */
+static int64_t batch_open_from(whir_state *S_, int count, const or_fe *const *arrays, const uint64_t *sizes,
+                               const or_fe *const *codewords, const uint64_t *cw_sizes, const or_fe *const *betas,
+                               int whir_k, uint64_t *record, uint64_t record_cap);
+
 int64_t or_whir_batch_open(int count, const or_fe *const *arrays, const uint64_t *sizes,
                            const or_fe *const *codewords, const uint64_t *cw_sizes, const uint8_t *const *trees,
                            const or_fe *evals, const or_fe *const *betas, int rate, int whir_k,
                            uint64_t *record, uint64_t record_cap) {
-    rec_t R = { record, 0, record_cap, 0 };
-    int kap[64], agg[64];
-    int nk = or_whir_kappas(sizes, count, whir_k, kap, agg, 64);
-    if (nk < 0) return -4;
     whir_state S;
+    memset(&S, 0, sizeof(S));
     S.n = sizes[0];
     S.arr = (or_fe *)malloc(sizeof(or_fe) * S.n);
     S.b = (or_fe *)malloc(sizeof(or_fe) * S.n);
@@ -855,6 +866,58 @@ int64_t or_whir_batch_open(int count, const or_fe *const *arrays, const uint64_t
     S.mt_leaves = S.cw_n / 4;
     S.K = evals[0];                                        /* :714 */
     S.rate = rate;
+    return batch_open_from(&S, count, arrays, sizes, codewords, cw_sizes, betas, whir_k, record, record_cap);
+}
+
+/* _Whir_Open (Whir.cpp:469-545): the iterations of one array arr with its
+ * weights b and claim K down to |arr| <= whir_k, then the closing steps.
+ * The codeword is opened against its dup-left tree (nodes), or, when nodes
+ * is NULL, against the tree over zero_leaves all-zero leaf digests that
+ * aggregate_commitments leaves (:590-605).  chal receives total_r: every
+ * challenge in the order drawn (log2 n of them); final_claim the MLE of the
+ * initial b at total_r (:517-518).  Returns the record's word count (the
+ * layout of or_whir_batch_open's record) or a negative status. */
+int64_t or_whir_open_weights(const or_fe *arr, uint64_t n, const or_fe *b, or_fe K, const or_fe *cw, uint64_t cw_n,
+                             const uint8_t *nodes, uint64_t zero_leaves, int rate, int whir_k, uint64_t *record,
+                             uint64_t record_cap, or_fe *chal, or_fe *final_claim) {
+    whir_state S;
+    memset(&S, 0, sizeof(S));
+    S.n = n;
+    S.arr = (or_fe *)malloc(sizeof(or_fe) * n);
+    S.b = (or_fe *)malloc(sizeof(or_fe) * n);
+    memcpy(S.arr, arr, sizeof(or_fe) * n);
+    memcpy(S.b, b, sizeof(or_fe) * n);
+    S.cw_n = cw_n;
+    S.cw = (or_fe *)malloc(sizeof(or_fe) * cw_n);
+    memcpy(S.cw, cw, sizeof(or_fe) * cw_n);
+    if (nodes) {
+        S.mt = (uint8_t *)malloc(32 * or_merkle_node_count(cw_n));
+        memcpy(S.mt, nodes, 32 * or_merkle_node_count(cw_n));
+        S.mt_leaves = cw_n / 4;
+    } else {
+        S.mt = NULL;
+        S.mt_leaves = zero_leaves;
+    }
+    S.K = K;
+    S.rate = rate;
+    or_fe *own = chal ? NULL : (or_fe *)malloc(sizeof(or_fe) * 64);
+    S.chal = chal ? chal : own;
+    S.nchal = 0;
+    const uint64_t sizes[1] = { n };
+    const int64_t w = batch_open_from(&S, 1, &arr, sizes, &cw, &cw_n, &b, whir_k, record, record_cap);
+    if (w >= 0 && final_claim) *final_claim = or_evaluate_vector(b, n, S.chal);
+    free(own);
+    return w;
+}
+
+static int64_t batch_open_from(whir_state *S_, int count, const or_fe *const *arrays, const uint64_t *sizes,
+                               const or_fe *const *codewords, const uint64_t *cw_sizes, const or_fe *const *betas,
+                               int whir_k, uint64_t *record, uint64_t record_cap) {
+    whir_state S = *S_;
+    rec_t R = { record, 0, record_cap, 0 };
+    int kap[64], agg[64];
+    int nk = or_whir_kappas(sizes, count, whir_k, kap, agg, 64);
+    if (nk < 0) { state_free(&S); return -4; }
 
     uint64_t steps_pos = R.n;
     rec_u64(&R, 0);

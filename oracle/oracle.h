@@ -119,6 +119,14 @@ int or_whir_kappas(const uint64_t *sizes, int count, int whir_k, int *kappas, in
 /* aggregate_claims (src/Whir.cpp:608-668) with arr, b_aggr empty on entry:
  * npoints claim points of log2 n elements each; beta_ch[npoints],
  * arr_ch[narrays] receive the libc-drawn challenges. */
+/* _Whir_Open (src/Whir.cpp:469-545): one array with explicit weights b and
+ * claim K, opened against its dup-left tree (nodes) or, nodes NULL, the
+ * all-zero-leaf tree of zero_leaves leaves that aggregate_commitments
+ * leaves; chal (log2 n) = total_r, final_claim = MLE(b)(total_r); the record
+ * has or_whir_batch_open's layout */
+int64_t or_whir_open_weights(const or_fe *arr, uint64_t n, const or_fe *b, or_fe K, const or_fe *cw, uint64_t cw_n,
+                             const uint8_t *nodes, uint64_t zero_leaves, int rate, int whir_k, uint64_t *record,
+                             uint64_t record_cap, or_fe *chal, or_fe *final_claim);
 void or_whir_aggregate_claims(int narrays, const or_fe *const *arrays, uint64_t n, int npoints, const or_fe *points,
                               or_fe *arr, or_fe *b_aggr, or_fe *K, or_fe *beta_ch, or_fe *arr_ch);
 int64_t or_whir_batch_open(int count, const or_fe *const *arrays, const uint64_t *sizes,

@@ -121,6 +121,8 @@ def oracle():
         L.or_plain_proof_mt.restype = i32
         L.or_max_threads.restype = i32
         L.or_whir_set_threads.argtypes = [i32]
+        L.or_whir_open_weights.argtypes = [vp, u64, vp, _Fe, vp, u64, vp, u64, i32, i32, vp, u64, vp, vp]
+        L.or_whir_open_weights.restype = ctypes.c_int64
         L.or_update_betas_mt.argtypes = [vp, u64, vp, vp, i32, i32]
         _o = L
     return _o
@@ -568,3 +570,22 @@ def whir_threads(t):
     """Thread count of the WHIR restatement's encodes, trees and weight
     updates (1: single-thread; the bytes are the same either way)."""
     oracle().or_whir_set_threads(int(t))
+
+
+def whir_open_weights(arr, b, K, cw, nodes, zero_leaves, rate, whir_k, cap=1 << 22):
+    """_Whir_Open of the oracle: (status or words, record, total_r (log2 n, 2),
+    final claim).  nodes None: the first tree is zero_leaves all-zero leaf
+    digests.  Seed libc first."""
+    a = np.ascontiguousarray(arr, dtype=np.uint64)
+    bb = np.ascontiguousarray(b, dtype=np.uint64)
+    c = np.ascontiguousarray(cw, dtype=np.uint64)
+    nd = None if nodes is None else np.ascontiguousarray(nodes, dtype=np.uint8)
+    n = a.shape[0]
+    rec = np.zeros(cap, np.uint64)
+    chal = np.zeros((max(n.bit_length() - 1, 1), 2), np.uint64)
+    fin = _Fe(0, 0)
+    w = oracle().or_whir_open_weights(_p(a), n, _p(bb), fe(K), _p(c), c.shape[0], None if nd is None else _p(nd),
+                                      zero_leaves, rate, whir_k, _p(rec), cap, _p(chal), ctypes.byref(fin))
+    if w < 0:
+        return int(w), None, None, None
+    return int(w), rec[:w].copy(), chal[:n.bit_length() - 1], unfe(fin)

@@ -89,6 +89,12 @@ def lib():
         L.cosnark_merkle_verify.argtypes = [vp, vp, vp, vp, i32, u64, vp, i32, vp, ctypes.POINTER(u64)]
         L.cosnark_verify_plain.argtypes = [vp, u64, vp, vp, vp, ctypes.POINTER(i32)]
         L.cosnark_whir_aggregate_claims.argtypes = [vp, i32, vp, u64, i32, vp, vp, vp, vp, vp, vp]
+        L.cosnark_whir_open_weights.argtypes = [vp, vp, u64, vp, vp, vp, u64, vp, u64, i32, i32, vp, u64,
+                                                ctypes.POINTER(u64), vp, vp]
+        L.cosnark_field_random.argtypes = [vp]
+        L.cosnark_field_random.restype = None
+        L.cosnark_blake3_64.argtypes = [vp, vp]
+        L.cosnark_blake3_64.restype = None
         L.cosnark_whir_batch_open.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp, vp, i32, i32, vp, u64,
                                               ctypes.POINTER(u64)]
         L.cosnark_sumcheck_aggr.argtypes = [vp, i32, i32, vp, vp]
@@ -438,6 +444,22 @@ class Context:
         for b in ups:
             b.free()
         return rc, rec[:words.value].copy()
+
+    def whir_open_weights(self, arr_dev, n, b_dev, K, cw_dev, cw_n, nodes_dev, zero_leaves, rate, whir_k,
+                          cap=1 << 22):
+        """_Whir_Open (cosnark_whir_open_weights): (status, record, total_r,
+        final claim).  nodes_dev None: the first tree is zero_leaves all-zero
+        leaf digests.  Seed libc first."""
+        KK = np.ascontiguousarray(K, dtype=np.uint64).reshape(2)
+        rec = np.zeros(cap, np.uint64)
+        words = ctypes.c_uint64(0)
+        m = n.bit_length() - 1
+        chal = np.zeros((max(m, 1), 2), np.uint64)
+        fin = np.zeros(2, np.uint64)
+        rc = lib().cosnark_whir_open_weights(self.h, arr_dev.ptr, n, b_dev.ptr, _p(KK), cw_dev.ptr, cw_n,
+                                             None if nodes_dev is None else nodes_dev.ptr, zero_leaves, rate, whir_k,
+                                             _p(rec), cap, ctypes.byref(words), _p(chal), _p(fin))
+        return rc, rec[:words.value].copy(), chal[:m], fin
 
     def whir_aggregate_claims(self, arrays, points):
         """aggregate_claims (src/Whir.cpp:608-668): (arr, b_aggr, K, beta_ch,

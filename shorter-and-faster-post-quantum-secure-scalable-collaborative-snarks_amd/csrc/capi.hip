@@ -4,6 +4,7 @@
 #include <vector>
 #include "ctx.h"
 #include "transcript.h"
+#include "blake3.h"
 
 using gf::fe;
 
@@ -95,6 +96,19 @@ const char *cosnark_version(void) { return "cosnark-hip 0.1 gfx950"; }
 
 void cosnark_field_mul(const uint64_t a[2], const uint64_t b[2], uint64_t out[2]) {
     const fe r = gf::mul(gf::mk(a[0], a[1]), gf::mk(b[0], b[1]));
+    out[0] = r.re;
+    out[1] = r.im;
+}
+
+void cosnark_blake3_64(const uint8_t in[64], uint8_t out[32]) {
+    uint32_t m[16], h[8];
+    memcpy(m, in, 64);                 // little-endian words, as BLAKE3 reads them
+    b3::compress64(m, h);
+    memcpy(out, h, 32);
+}
+
+void cosnark_field_random(uint64_t out[2]) {
+    const fe r = libc_random_element();
     out[0] = r.re;
     out[1] = r.im;
 }

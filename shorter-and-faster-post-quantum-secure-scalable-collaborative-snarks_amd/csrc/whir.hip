@@ -588,6 +588,12 @@ int whir_init(cosnark_ctx *) {
     return COSNARK_OK;
 }
 
+static int whir_batch_open_impl(cosnark_ctx *ctx, int count, const uint64_t *const *d_arrays, const uint64_t *sizes,
+                                const uint64_t *const *d_codewords, const uint64_t *cw_sizes,
+                                const uint8_t *const *d_nodes, const uint64_t *h_evals,
+                                const uint64_t *const *d_betas, int rate, int whir_k, uint64_t *h_record,
+                                uint64_t record_cap, uint64_t *words, uint64_t zero_leaves0, uint64_t *h_challenges);
+
 extern "C" {
 
 int cosnark_whir_commit(cosnark_ctx *ctx, const uint64_t *d_arr, uint64_t n, int rate, uint64_t *d_codeword,
@@ -909,13 +915,47 @@ int cosnark_whir_batch_open(cosnark_ctx *ctx, int count, const uint64_t *const *
                             const uint64_t *const *d_codewords, const uint64_t *cw_sizes,
                             const uint8_t *const *d_nodes, const uint64_t *h_evals, const uint64_t *const *d_betas,
                             int rate, int whir_k, uint64_t *h_record, uint64_t record_cap, uint64_t *words) {
+    return whir_batch_open_impl(ctx, count, d_arrays, sizes, d_codewords, cw_sizes, d_nodes, h_evals, d_betas, rate,
+                                whir_k, h_record, record_cap, words, 0, nullptr);
+}
+
+int cosnark_whir_open_weights(cosnark_ctx *ctx, const uint64_t *d_arr, uint64_t n, const uint64_t *d_b,
+                              const uint64_t K[2], const uint64_t *d_codeword, uint64_t cw_n, const uint8_t *d_nodes,
+                              uint64_t zero_leaves, int rate, int whir_k, uint64_t *h_record, uint64_t record_cap,
+                              uint64_t *words, uint64_t *h_challenges, uint64_t final_claim[2]) {
+    if (!ctx || !d_arr || !d_b || !K || !d_codeword || !h_record || !words || !h_challenges || !final_claim)
+        return set_err(ctx, COSNARK_E_ARG, "whir_open_weights: null argument");
+    if (!d_nodes && (!is_pow2(zero_leaves) || zero_leaves > cw_n))
+        return set_err(ctx, COSNARK_E_ARG, "whir_open_weights: without a tree, zero_leaves a power of two <= cw_n");
+    const uint64_t *arrs[1] = {d_arr}, *cws[1] = {d_codeword}, *bs[1] = {d_b};
+    const uint8_t *nds[1] = {d_nodes};
+    const uint64_t sz[1] = {n}, csz[1] = {cw_n};
+    const int rc = whir_batch_open_impl(ctx, 1, arrs, sz, cws, csz, nds, K, bs, rate, whir_k, h_record, record_cap,
+                                        words, d_nodes ? 0 : zero_leaves, h_challenges);
+    if (rc) return rc;
+    // the closing claim: the initial weights' multilinear extension at
+    // every challenge of the opening (src/Whir.cpp:517-518)
+    return cosnark_whir_claim(ctx, d_b, n, h_challenges, final_claim);
+}
+
+}  // extern "C"
+
+// Whir_Batch_Open's prover work; zero_leaves0 > 0: the first codeword is
+// opened against a tree over that many all-zero leaf digests (what
+// aggregate_commitments leaves) instead of d_nodes[0]; h_challenges
+// (optional): every challenge in the order drawn, then the closing ones.
+static int whir_batch_open_impl(cosnark_ctx *ctx, int count, const uint64_t *const *d_arrays, const uint64_t *sizes,
+                                const uint64_t *const *d_codewords, const uint64_t *cw_sizes,
+                                const uint8_t *const *d_nodes, const uint64_t *h_evals,
+                                const uint64_t *const *d_betas, int rate, int whir_k, uint64_t *h_record,
+                                uint64_t record_cap, uint64_t *words, uint64_t zero_leaves0, uint64_t *h_challenges) {
     if (!ctx || count < 1 || !d_arrays || !sizes || !d_codewords || !cw_sizes || !d_nodes || !h_evals || !d_betas ||
         !h_record || !words)
         return set_err(ctx, COSNARK_E_ARG, "whir_batch_open: null argument");
     if (whir_k < 2 || !is_pow2((uint64_t)whir_k) || whir_k > 64 || rate < 2 || !is_pow2((uint64_t)rate))
         return set_err(ctx, COSNARK_E_ARG, "whir_batch_open: whir_k a power of two in [2, 64], rate >= 2 a power of two");
     for (int i = 0; i < count; ++i) {
-        if (!d_arrays[i] || !d_codewords[i] || !d_nodes[i] || !d_betas[i])
+        if (!d_arrays[i] || !d_codewords[i] || (!d_nodes[i] && !(i == 0 && zero_leaves0)) || !d_betas[i])
             return set_err(ctx, COSNARK_E_ARG, "whir_batch_open: null array, codeword, tree or weights");
         if (!is_pow2(sizes[i]) || sizes[i] < 2 || !is_pow2(cw_sizes[i]) || cw_sizes[i] < 8 ||
             cw_sizes[i] > (1ull << NTT_LOG_MAX) || cw_sizes[i] < 2 * sizes[i])
@@ -971,6 +1011,11 @@ int cosnark_whir_batch_open(cosnark_ctx *ctx, int count, const uint64_t *const *
         uint64_t n = n0, cw_n = cw0;
         int rt = rate, iter = 0, ri = 1, wk = whir_k, zero_cur = -1;
         uint64_t tree_leaves = cw_n / 4;
+        if (zero_leaves0) {                    // the first tree: all-zero leaf digests
+            chains.push_back(zero_chain(ilog2(zero_leaves0)));
+            zero_cur = 0;
+            tree_leaves = zero_leaves0;
+        }
         size_t rep_off = 0, path_off = 0, eval_off = 0;
         while (true) {
             wk = kap[iter];
@@ -1310,12 +1355,21 @@ int cosnark_whir_batch_open(cosnark_ctx *ctx, int count, const uint64_t *const *
         for (uint64_t i = 0; i < fs.n; ++i) Kc = gf::sub(Kc, gf::mul(fin_arr[i], fin_b[i]));
         if (!gf::eq(Kc, gf::zero())) bad = true;
         for (int j = 0; j < fs.k; ++j) putfe(fs.r[j]);
+        if (h_challenges) {                    // total_r: the rounds' challenges, then the closing ones
+            for (int j = 0; j < nround; ++j) { h_challenges[2 * j] = ext[j].re; h_challenges[2 * j + 1] = ext[j].im; }
+            for (int j = 0; j < fs.k; ++j) {
+                h_challenges[2 * (nround + j)] = fs.r[j].re;
+                h_challenges[2 * (nround + j) + 1] = fs.r[j].im;
+            }
+        }
         *words = w;
         if (overflow) return set_err(ctx, COSNARK_E_ARG, "whir_batch_open: record buffer too small");
         if (bad) return set_err(ctx, COSNARK_E_CHECK, "whir_batch_open: claim chain check failed");
     }
     return COSNARK_OK;
 }
+
+extern "C" {
 
 int cosnark_whir_aggregate_claims(cosnark_ctx *ctx, int narrays, const uint64_t *const *d_arrays, uint64_t n,
                                   int npoints, const uint64_t *h_points, uint64_t *d_arr, uint64_t *d_b_aggr,

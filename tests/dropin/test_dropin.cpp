@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <vector>
 #include "cosnark_dropin.hpp"
 #include "../../oracle/oracle.h"
@@ -119,14 +120,219 @@ int main() {
         or_precompute_beta(reinterpret_cast<or_fe *>(z.data()), 12, reinterpret_cast<or_fe *>(beta.data()));
         or_fe K{0, 0};
         for (uint64_t i = 0; i < n; ++i) K = or_add(K, or_mul(O(arr[i]), O(beta[i])));
-        srandom(1234);
-        D::Whir_Open(arr, cw, tree, {{F{K.real, K.img}, z}}, rate);
+        vector<uint8_t> onodes(32 * or_merkle_node_count(n * rate));
+        or_merkle_commit(reinterpret_cast<or_fe *>(ocw.data()), n * rate, onodes.data(), 1);
+        // at the reference's default whir_k = 8 and at 4
+        for (int wk : {8, 4}) {
+            D::whir_k() = wk;
+            srandom(1234);
+            D::Whir_Open(arr, cw, tree, {{F{K.real, K.img}, z}}, rate);
+            vector<uint64_t> orec(1 << 20);
+            vector<F> ochal(12);
+            srandom(1234);
+            const int64_t w = or_whir_open_weights(reinterpret_cast<or_fe *>(arr.data()), n,
+                                                   reinterpret_cast<or_fe *>(beta.data()), K,
+                                                   reinterpret_cast<or_fe *>(ocw.data()), n * rate, onodes.data(), 0,
+                                                   rate, wk, orec.data(), orec.size(),
+                                                   reinterpret_cast<or_fe *>(ochal.data()), nullptr);
+            const auto &rec = D::last_whir_record();
+            CHECK(w > 0 && (int64_t)rec.size() == w && same(rec.data(), orec.data(), 8 * w),
+                  wk == 8 ? "Whir_Open record 2^12, whir_k 8" : "Whir_Open record 2^12, whir_k 4");
+        }
+        // whir_k = 4 with one claim: the same record as the one-array batch open
+        {
+            vector<uint64_t> orec(1 << 20), orec2(1 << 20);
+            const or_fe *pa = reinterpret_cast<or_fe *>(arr.data()), *pc = reinterpret_cast<or_fe *>(ocw.data()),
+                        *pb = reinterpret_cast<or_fe *>(beta.data());
+            const uint8_t *pt = onodes.data();
+            const uint64_t sz = n, cs = n * rate;
+            srandom(77);
+            const int64_t w1 = or_whir_batch_open(1, &pa, &sz, &pc, &cs, &pt, &K, &pb, rate, 4, orec.data(), orec.size());
+            srandom(77);
+            D::whir_k() = 4;
+            D::Whir_Open(arr, cw, tree, {{F{K.real, K.img}, z}}, rate);
+            const auto &rec = D::last_whir_record();
+            CHECK(w1 > 0 && (int64_t)rec.size() == w1 && same(rec.data(), orec.data(), 8 * w1),
+                  "Whir_Open = one-array Whir_Batch_Open at whir_k 4");
+        }
+        // two claims batched with one F::_random() power (Whir.cpp:398-411)
+        {
+            vector<F> z2 = synth(seed, 7, 12), beta2(n);
+            or_precompute_beta(reinterpret_cast<or_fe *>(z2.data()), 12, reinterpret_cast<or_fe *>(beta2.data()));
+            or_fe K2{0, 0};
+            for (uint64_t i = 0; i < n; ++i) K2 = or_add(K2, or_mul(O(arr[i]), O(beta2[i])));
+            D::whir_k() = 8;
+            srandom(555);
+            D::Whir_Open(arr, cw, tree, {{F{K.real, K.img}, z}, {F{K2.real, K2.img}, z2}}, rate);
+            const auto rec = D::last_whir_record();
+            srandom(555);
+            const or_fe r = or_random_element();
+            vector<F> bb(n);
+            for (uint64_t i = 0; i < n; ++i) {
+                const or_fe t = or_add(O(beta[i]), or_mul(r, O(beta2[i])));
+                bb[i] = F{t.real, t.img};
+            }
+            const or_fe KK = or_add(K, or_mul(r, K2));
+            vector<uint64_t> orec(1 << 20);
+            const int64_t w = or_whir_open_weights(reinterpret_cast<or_fe *>(arr.data()), n,
+                                                   reinterpret_cast<or_fe *>(bb.data()), KK,
+                                                   reinterpret_cast<or_fe *>(ocw.data()), n * rate, onodes.data(), 0,
+                                                   rate, 8, orec.data(), orec.size(), nullptr, nullptr);
+            CHECK(w > 0 && (int64_t)rec.size() == w && same(rec.data(), orec.data(), 8 * w), "Whir_Open two claims");
+            // three claims at distinct points: the reference batches claims[1]'s
+            // weights for every i >= 1 (:403), so the first sumcheck check fails
+            vector<F> z3 = synth(seed, 8, 12);
+            bool threw = false;
+            try {
+                D::Whir_Open(arr, cw, tree, {{F{K.real, K.img}, z}, {F{K2.real, K2.img}, z2}, {F{3, 4}, z3}}, rate);
+            } catch (const D::error &e) {
+                threw = e.code == COSNARK_E_CHECK;
+            }
+            CHECK(threw, "Whir_Open three claims: the reference's claims[1] batching fails its check");
+        }
+        // _Whir_Open on the codeword's tree: record, total_r and the final claim
+        {
+            D::whir_k() = 8;
+            srandom(2024);
+            vector<vector<std::pair<F, vector<F>>>> cl;
+            auto fc = D::_Whir_Open(arr, cw, tree, beta, cl, {}, F{K.real, K.img}, rate, true);
+            const auto rec = D::last_whir_record();
+            vector<uint64_t> orec(1 << 20);
+            vector<F> ochal(12);
+            or_fe ofin;
+            srandom(2024);
+            const int64_t w = or_whir_open_weights(reinterpret_cast<or_fe *>(arr.data()), n,
+                                                   reinterpret_cast<or_fe *>(beta.data()), K,
+                                                   reinterpret_cast<or_fe *>(ocw.data()), n * rate, onodes.data(), 0,
+                                                   rate, 8, orec.data(), orec.size(),
+                                                   reinterpret_cast<or_fe *>(ochal.data()), &ofin);
+            CHECK(w > 0 && (int64_t)rec.size() == w && same(rec.data(), orec.data(), 8 * w) && fc.second.size() == 12 &&
+                      same(fc.second.data(), ochal.data(), 16 * 12) && fc.first.real == ofin.real &&
+                      fc.first.img == ofin.img,
+                  "_Whir_Open 2^12 whir_k 8: record, total_r, final claim");
+        }
+    }
+    // Whir_Commit, Whir_Commit_Codeword: codeword and the all-zero-leaf tree
+    {
+        D::whir_k() = 8;
+        const uint64_t n = 1 << 10;
+        vector<F> arr = synth(seed, 9, n), cw;
+        vector<vector<_hash>> mt;
+        D::Whir_Commit(arr, cw, mt, 4);
+        vector<F> ocw(4 * n);
+        or_rs_encode(reinterpret_cast<or_fe *>(arr.data()), n, 4, reinterpret_cast<or_fe *>(ocw.data()));
+        // level l of the zero tree: the l-fold self-hash of 32 zero bytes
+        auto zero_ok = [&](const vector<vector<_hash>> &t, uint64_t leaves) {
+            uint8_t cur[32] = {0}, blk[64];
+            int depth = 0;
+            while ((1ull << depth) < leaves) ++depth;
+            if ((int)t.size() != depth + 1) return false;
+            for (int l = 0; l <= depth; ++l) {
+                if (l) { memcpy(blk, cur, 32); memcpy(blk + 32, cur, 32); or_blake3_hash64(blk, cur); }
+                if (t[l].size() != (leaves >> l)) return false;
+                for (auto &h : t[l]) if (!same(h.arr, cur, 32)) return false;
+            }
+            return true;
+        };
+        CHECK(same(cw.data(), ocw.data(), 16 * cw.size()) && zero_ok(mt, 4 * n / 8), "Whir_Commit codeword and tree");
+        vector<vector<_hash>> mt4;
+        D::Whir_Commit(arr, cw, mt4, 4, 4);
+        CHECK(zero_ok(mt4, n), "Whir_Commit kappa 4 tree");
+        vector<vector<_hash>> mt2;
+        D::Whir_Commit_Codeword(cw, mt2, 4);
+        CHECK(zero_ok(mt2, 4 * n / 8), "Whir_Commit_Codeword tree");
+        // aggregate_commitments over three codewords, then _Whir_Open on its tree
+        vector<vector<F>> arrs, cws;
+        vector<vector<vector<_hash>>> mts;
+        for (int i = 0; i < 3; ++i) {
+            arrs.push_back(synth(seed, 120 + i, n));
+            vector<F> c;
+            vector<vector<_hash>> t;
+            D::Whir_Commit_initial(arrs[i], c, t, 4, 4);
+            cws.push_back(c);
+            mts.push_back(t);
+        }
+        vector<F> rr = synth(seed, 130, 3), acw;
+        vector<vector<_hash>> amt;
+        srandom(99);
+        D::aggregate_commitments(acw, amt, cws, mts, rr, 4);
+        const int after = rand();
+        srandom(99);
+        for (int i = 0; i < 100; ++i) (void)rand();                  // l = 100 at rate 4
+        vector<F> expect(4 * n, F{0, 0});
+        for (int i = 0; i < 3; ++i)
+            for (uint64_t j = 0; j < 4 * n; ++j) {
+                or_fe t = or_add(O(expect[j]), or_mul(O(rr[i]), O(cws[i][j])));
+                expect[j] = F{t.real, t.img};
+            }
+        CHECK(same(acw.data(), expect.data(), 16 * expect.size()) && zero_ok(amt, 4 * n / 8) && after == rand(),
+              "aggregate_commitments codeword, tree, libc draws");
+        // the aggregated array with combined weights, opened against that tree
+        vector<F> aarr(n, F{0, 0}), b = synth(seed, 131, n);
+        for (int i = 0; i < 3; ++i)
+            for (uint64_t j = 0; j < n; ++j) {
+                or_fe t = or_add(O(aarr[j]), or_mul(O(rr[i]), O(arrs[i][j])));
+                aarr[j] = F{t.real, t.img};
+            }
+        or_fe K{0, 0};
+        for (uint64_t i = 0; i < n; ++i) K = or_add(K, or_mul(O(aarr[i]), O(b[i])));
+        srandom(7);
+        vector<vector<std::pair<F, vector<F>>>> cl;
+        auto fc = D::_Whir_Open(aarr, acw, amt, b, cl, {}, F{K.real, K.img}, 4, true);
+        const auto rec = D::last_whir_record();
         vector<uint64_t> orec(1 << 20);
-        srandom(1234);
-        const int64_t w = or_whir_open(reinterpret_cast<or_fe *>(arr.data()), n, reinterpret_cast<or_fe *>(z.data()), K,
-                                       rate, 0, orec.data(), orec.size());
-        const auto &rec = D::last_whir_record();
-        CHECK(w > 0 && (int64_t)rec.size() == w && same(rec.data(), orec.data(), 8 * w), "Whir_Open record 2^12");
+        vector<F> ochal(10);
+        or_fe ofin;
+        srandom(7);
+        const int64_t w = or_whir_open_weights(reinterpret_cast<or_fe *>(aarr.data()), n, reinterpret_cast<or_fe *>(b.data()),
+                                               K, reinterpret_cast<or_fe *>(acw.data()), 4 * n, nullptr, 4 * n / 8, 4, 8,
+                                               orec.data(), orec.size(), reinterpret_cast<or_fe *>(ochal.data()), &ofin);
+        CHECK(w > 0 && (int64_t)rec.size() == w && same(rec.data(), orec.data(), 8 * w) &&
+                  same(fc.second.data(), ochal.data(), 16 * 10) && fc.first.real == ofin.real && fc.first.img == ofin.img,
+              "_Whir_Open on aggregate_commitments' tree (coPCS.cpp:1174-1175 pattern)");
+    }
+    // get_whir_rates(rate) on the globals
+    {
+        D::whir_k() = 8;
+        D::whir_sizes() = {1 << 10, 1 << 12, 1 << 8, 1 << 12};
+        D::sorted_whir_rates().clear();
+        D::whir_kappas().clear();
+        D::get_whir_rates(4);
+        const vector<int> want_idx = {2, 1, 3, 0}, want_rates = {4, 4, 8, 16};
+        std::vector<uint64_t> srt = {1 << 12, 1 << 12, 1 << 10, 1 << 8};
+        vector<int> orates(4);
+        or_whir_rates(srt.data(), 4, 4, 8, orates.data());
+        CHECK(D::sorted_indexes() == want_idx && D::sorted_whir_rates() == want_rates && D::whir_rates() == want_rates &&
+                  orates == want_rates && D::whir_kappas().size() == 3 && D::whir_kappas()[1 << 12] == 4 &&
+                  D::whir_kappas()[1 << 10] == 4 && D::whir_kappas()[1 << 8] == 8,
+              "get_whir_rates(rate): sorted_indexes, rates, kappas");
+        D::whir_sizes() = {1 << 16};
+        D::whir_kappas().clear();
+        D::get_whir_rates(4);
+        CHECK(D::sorted_whir_rates().size() == 5 && D::sorted_whir_rates()[4] == 4 && D::whir_kappas()[1 << 16] == 8,
+              "get_whir_rates appends, one size");
+    }
+    // verify_claim_opt_blake: the reference's bookkeeping
+    {
+        vector<F> e = synth(seed, 140, 32);
+        vector<vector<_hash>> H;
+        D::MT_commit_Blake(e.data(), H, 32);                          // 8 leaves, 4 levels
+        vector<bool> visited(16, false);
+        double ps = 0;
+        auto p5 = D::open_tree_blake(H, 5);
+        bool ok = D::verify_claim_opt_blake(H, p5.data(), 5, 8, visited, ps);
+        // 13 -> siblings 12, 7, 2 marked with the parents 6, 3, 1
+        const vector<int> marked = {1, 2, 3, 6, 7, 12};
+        bool vis_ok = true;
+        for (int i = 0; i < 16; ++i)
+            vis_ok = vis_ok && visited[i] == (std::find(marked.begin(), marked.end(), i) != marked.end());
+        ok = ok && vis_ok && ps == 3 * 32.0 / 1024.0;
+        auto p4 = D::open_tree_blake(H, 4);
+        // 12's sibling 13 is new: one level, then its parent's sibling 7 was visited
+        ok = ok && D::verify_claim_opt_blake(H, p4.data(), 4, 8, visited, ps) && ps == 4 * 32.0 / 1024.0 && visited[13];
+        bool threw = false;
+        try { D::verify_claim_opt_blake(H, p4.data(), 4, 4, visited, ps); } catch (const D::error &) { threw = true; }
+        CHECK(ok && threw, "verify_claim_opt_blake visited / proof size / exits");
     }
     // coPCS commit: Encode + distributed_MT
     {
@@ -269,7 +475,7 @@ int main() {
         const std::vector<int> logs = {12, 10, 8, 8};
         std::vector<int> sizes;
         for (int lg : logs) sizes.push_back(1 << lg);
-        std::vector<int> rates = D::get_whir_rates(sizes, 4);
+        std::vector<int> rates = D::get_whir_rates(sizes, 4, 4);
         std::vector<uint64_t> usz(sizes.begin(), sizes.end());
         std::vector<int> orates(sizes.size());
         or_whir_rates(usz.data(), (int)usz.size(), 4, 4, orates.data());
@@ -310,6 +516,7 @@ int main() {
                                               reinterpret_cast<const or_fe *>(evals.data()), ob.data(), 4, 4,
                                               orec.data(), orec.size());
         srandom(41);
+        D::whir_k() = 4;
         D::Whir_Batch_Open(arrays, codewords, MTs, evals, betas, 4, true);
         const auto &rec = D::last_whir_record();
         CHECK(ow > 0 && (int64_t)rec.size() == ow && same(rec.data(), orec.data(), 8 * ow) && arrays.empty(),
