@@ -390,6 +390,53 @@ int cosnark_share_unpack(uint64_t *h_secret, int n, int kk, uint64_t *h_out);
  * empty and h_out is zero, as in the reference.  nranks a power of two
  * <= 8192. */
 int cosnark_sumcheck_aggr(const uint64_t *h_polys, int nranks, int len, const uint64_t *h_b, uint64_t *h_out);
+/* ---- coSumcheck drivers (src/coSumcheck.cpp:233-448), one rank of N ----
+ * The exchanges of the reference -- sumcheck_aggr's gather to rank 0 and
+ * broadcast of the combination (src/MPI_utils.cpp:2604-2670),
+ * send_final_sumcheck_poly's (:2672-2736) -- are one all-gather supplied by
+ * the caller (MPI, RCCL, or threads in one process): the callback receives
+ * this rank's `elems` elements in `mine` and writes every rank's, rank
+ * after rank (nranks * elems elements), to `all`; it returns 0 on success.
+ * Every rank then forms rank 0's combination itself. */
+typedef int (*cosnark_allgather_fn)(void *user, const uint64_t *mine, uint64_t elems, uint64_t *all);
+/* quadratic_coSumcheck(p1, p2, R, K, K_r, degrees, N, ps), :233-331, as
+ * rank `rank` of nranks (a power of two >= 4: the reference divides by
+ * N/4).  d_p1, d_p2: this rank's shares, M elements each, folded in place
+ * (M / prod(degrees) >= 2 elements remain; the reference resizes them).
+ * h_R[i] (R_sizes[i] = 2 degrees[i] elements, i < nR = ndeg): this rank's
+ * masks, read only (the reference works on its copy).  The transcript ts
+ * absorbs {K, K_r} first (c = hash_to_field({K, K_r})).  Per round: block
+ * sums (cosnark_cosumcheck_phase1) plus c times the mask, the all-gather and
+ * combination with ones(N/4), r = hash_to_field(H), ip_vec, the sparse mask
+ * fold (quadratic_coSumcheck_phase2_sparse, :215-228), the fold by
+ * compute_lagrange_coeff(r, d).  Then {p1[0], p2[0], p1[1], p2[1],
+ * final_eval_r} are gathered and unpacked to N/2 values each.  Outputs:
+ * h_claims (5 elements: the claims' values), h_r (ndeg + 1 elements: the
+ * claims' common point, the closing challenge first), h_ip_vec (optional,
+ * sum of 2 degrees[i] elements: the global ip_vec, round after round), ps
+ * (optional: rank 0 adds the proof size in KiB as the reference does),
+ * check_failures (optional): the reference's printed, not fatal, checks as
+ * a bit mask -- bit i round i's sum != K, bit ndeg the final check. */
+int cosnark_quadratic_cosumcheck(cosnark_ctx *ctx, uint64_t *d_p1, uint64_t *d_p2, uint64_t M,
+                                 const uint64_t *const *h_R, const uint64_t *R_sizes, int nR, const uint64_t K[2],
+                                 const uint64_t K_r[2], const int *degrees, int ndeg, int nranks, int rank,
+                                 cosnark_allgather_fn gather, void *user, cosnark_transcript *ts, uint64_t *h_claims,
+                                 uint64_t *h_r, uint64_t *h_ip_vec, double *ps, int *check_failures);
+/* cubic_coSumcheck(p1, p2, p3, r1, r2, R, K, K_r, degrees, N, ps),
+ * :335-448, as rank `rank` of nranks: as above with b1 = precompute_beta(r1)
+ * (M = 2^nr1 elements, folded with the shares) in the cubic block sums
+ * (cosnark_cosumcheck3_phase1), the combination weighted by b2 =
+ * precompute_beta(r2) (2^nr2 >= N/4), masks of 4 degrees[i] elements,
+ * final values {p1[0], p2[0], p3[0], final_eval_r} (M / prod(degrees) >= 1),
+ * h_claims 4 elements, h_ip_vec sum of 4 degrees[i] elements, and
+ * check_failures bit ndeg + 1 for the reference's verification 2
+ * (b1[0] against the rounds' Lagrange weights). */
+int cosnark_cubic_cosumcheck(cosnark_ctx *ctx, uint64_t *d_p1, uint64_t *d_p2, uint64_t *d_p3, uint64_t M,
+                             const uint64_t *h_r1, int nr1, const uint64_t *h_r2, int nr2, const uint64_t *const *h_R,
+                             const uint64_t *R_sizes, int nR, const uint64_t K[2], const uint64_t K_r[2],
+                             const int *degrees, int ndeg, int nranks, int rank, cosnark_allgather_fn gather,
+                             void *user, cosnark_transcript *ts, uint64_t *h_claims, uint64_t *h_r, uint64_t *h_ip_vec,
+                             double *ps, int *check_failures);
 /* RS_multifold5, src/coPCS_utils.cpp:386-437: folds the n-element codeword
  * d_cw by the fd challenges h_a (fd elements) into n/fd elements at d_out
  * (the reference replaces the codeword; here d_out must not alias d_cw).

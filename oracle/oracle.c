@@ -1323,3 +1323,225 @@ void or_sumcheck_aggr(const or_fe *polys, int N, int len, const or_fe *b, or_fe 
     }
     free(P); free(v);
 }
+
+/* ------------------------------------------------------------------ */
+/* coSumcheck drivers, all N ranks in lockstep (src/coSumcheck.cpp)     */
+/* ------------------------------------------------------------------ */
+
+/* quadratic_coSumcheck_phase2_sparse (coSumcheck.cpp:215-228) on one
+ * rank's masks: s = sum_j L[j] R[iter][j]; the later masks scale by L[0];
+ * R[iter+1][0] += s */
+static or_fe phase2_sparse(or_fe **R, const uint64_t *R_sizes, int nR, const or_fe *L, int iter) {
+    or_fe s = { 0, 0 };
+    for (uint64_t j = 0; j < R_sizes[iter]; ++j) s = or_add(s, or_mul(L[j], R[iter][j]));
+    for (int j = iter + 1; j < nR; ++j)
+        for (uint64_t k = 0; k < R_sizes[j]; ++k) R[j][k] = or_mul(R[j][k], L[0]);
+    if (iter + 1 < nR) R[iter + 1][0] = or_add(R[iter + 1][0], s);
+    return s;
+}
+
+/* the mask of round iter (coSumcheck.cpp:182-193 with stride 2, :48-58 with
+ * stride 4): H[t] += c H_r[t], H_r = R[iter] with H_r[0] += R[j][stride k]
+ * for j > iter */
+static void add_mask(or_fe *H, int len, or_fe **R, const uint64_t *R_sizes, int nR, int iter, or_fe c, int stride) {
+    or_fe h0 = R[iter][0];
+    for (int j = iter + 1; j < nR; ++j)
+        for (uint64_t k = 0; k < R_sizes[j] / (uint64_t)stride; ++k) h0 = or_add(h0, R[j][stride * k]);
+    for (int t = 0; t < len; ++t) H[t] = or_add(H[t], or_mul(c, t == 0 ? h0 : R[iter][t]));
+}
+
+/* ip_vec's update after a round (coSumcheck.cpp:268-276, :390-399) */
+static void ip_update(or_fe *ip, const int *lens, int ndeg, int i, const or_fe *L) {
+    int off = 0;
+    for (int j = 0; j < ndeg; ++j) {
+        for (int k = 0; k < lens[j]; ++k)
+            ip[off + k] = or_mul(ip[off + k], j == i ? L[k] : L[0]);
+        off += lens[j];
+    }
+}
+
+/* send_final_sumcheck_poly's unpack (MPI_utils.cpp:2672-2736): coordinate
+ * i's N shares (rank order) to its N/2 values */
+static void final_unpack(const or_fe *mine, int N, int ncoord, or_fe *H) {
+    or_fe *P = (or_fe *)malloc(sizeof(or_fe) * N);
+    for (int i = 0; i < ncoord; ++i) {
+        for (int m = 0; m < N; ++m) P[m] = mine[(uint64_t)m * ncoord + i];
+        or_unpack_secret_share(P, N, N / 2, H + (uint64_t)i * (N / 2));
+    }
+    free(P);
+}
+
+/* the closing weights: 1 at N = 4, else compute_lagrange_coeff(r, N/4) */
+static void closing_weights(or_fe r, int N, or_fe *L) {
+    if (N / 4 == 1) { L[0].real = 1; L[0].img = 0; }
+    else or_compute_lagrange_coeff(r, N / 4, L);
+}
+
+int or_quadratic_cosumcheck(int N, or_fe **p1, or_fe **p2, uint64_t M, or_fe ***R, const uint64_t *R_sizes, int nR,
+                            or_fe K, or_fe K_r, const int *degrees, int ndeg, or_fe *prev,
+                            or_fe *claims, or_fe *r_out, or_fe *ip_vec, int *failures) {
+    if (N < 4 || (N & (N - 1)) || nR != ndeg) return -1;
+    uint64_t Mc = M;
+    for (int i = 0; i < ndeg; ++i) {
+        if (R_sizes[i] != 2 * (uint64_t)degrees[i] || Mc % (uint64_t)degrees[i]) return -1;
+        Mc /= (uint64_t)degrees[i];
+    }
+    if (Mc < 2) return -1;
+    Mc = M;
+    *failures = 0;
+    or_fe kk[2] = { K, K_r };
+    const or_fe c = or_hash_to_field(prev, kk, 2);                /* :234 */
+    K = or_add(K, or_mul(c, K_r));                                 /* :235 */
+    int *lens = (int *)malloc(sizeof(int) * ndeg), total = 0;
+    for (int i = 0; i < ndeg; ++i) { lens[i] = 2 * degrees[i]; total += lens[i]; }
+    for (int t = 0; t < total; ++t) { ip_vec[t].real = 1; ip_vec[t].img = 0; }  /* :244-247 */
+    or_fe *fin_r = (or_fe *)malloc(sizeof(or_fe) * N);
+    or_fe *ones = (or_fe *)malloc(sizeof(or_fe) * (N / 4));
+    for (int j = 0; j < N / 4; ++j) { ones[j].real = 1; ones[j].img = 0; }
+    for (int i = 0; i < ndeg; ++i) {
+        const int d = degrees[i], len = 2 * d;
+        or_fe *all = (or_fe *)malloc(sizeof(or_fe) * N * len), *H = (or_fe *)malloc(sizeof(or_fe) * len);
+        for (int m = 0; m < N; ++m) {                              /* phase 1 on every rank (:150-198) */
+            or_pcs_sumcheck_phase1(p1[m], p2[m], Mc, d, all + (uint64_t)m * len);
+            add_mask(all + (uint64_t)m * len, len, R[m], R_sizes, nR, i, c, 2);
+        }
+        or_sumcheck_aggr(all, N, len, ones, H);                    /* sumcheck_aggr(H, ones, N) :197 */
+        or_fe eval = { 0, 0 };
+        for (int j = 0; j < len / 2; ++j) eval = or_add(eval, H[2 * j]);   /* :254-257 */
+        if (eval.real != K.real || eval.img != K.img) *failures |= 1 << i;
+        const or_fe r = or_hash_to_field(prev, H, len);            /* :264 */
+        r_out[1 + i] = r;
+        or_fe *L2 = (or_fe *)malloc(sizeof(or_fe) * len);
+        or_compute_lagrange_coeff(r, len, L2);                     /* :267 */
+        ip_update(ip_vec, lens, ndeg, i, L2);
+        K.real = 0; K.img = 0;
+        for (int j = 0; j < len; ++j) K = or_add(K, or_mul(L2[j], H[j]));   /* :280 */
+        for (int m = 0; m < N; ++m) fin_r[m] = phase2_sparse(R[m], R_sizes, nR, L2, i);   /* :283 */
+        for (int m = 0; m < N; ++m) or_pcs_sumcheck_phase2(p1[m], p2[m], Mc, d, r);      /* :287-289 */
+        Mc /= (uint64_t)d;
+        free(L2); free(all); free(H);
+    }
+    /* send_final_sumcheck_poly({p1[0], p2[0], p1[1], p2[1], final_eval_r}, N) (:293) */
+    or_fe *mine = (or_fe *)malloc(sizeof(or_fe) * N * 5), *Hf = (or_fe *)malloc(sizeof(or_fe) * 5 * (N / 2));
+    for (int m = 0; m < N; ++m) {
+        mine[m * 5 + 0] = p1[m][0]; mine[m * 5 + 1] = p2[m][0];
+        mine[m * 5 + 2] = p1[m][1]; mine[m * 5 + 3] = p2[m][1];
+        mine[m * 5 + 4] = fin_r[m];
+    }
+    final_unpack(mine, N, 5, Hf);
+    const or_fe *H0 = Hf, *H1 = Hf + N / 2, *H2 = Hf + N, *H3 = Hf + 3 * (N / 2), *H4 = Hf + 2 * N;
+    for (int i = 0; i < N / 4; ++i)                                /* :298-300 */
+        K = or_sub(K, or_add(or_add(or_mul(H0[i], H1[i]), or_mul(H2[i], H3[i])), or_mul(c, H4[i])));
+    if (K.real || K.img) *failures |= 1 << ndeg;
+    or_fe r = or_hash_to_field(prev, H0, N / 2);                   /* :306-307 */
+    for (int i = 1; i < 5; ++i) r = or_hash_to_field(prev, Hf + (uint64_t)i * (N / 2), N / 2);
+    r_out[0] = r;                                                  /* :309 */
+    or_fe *L = (or_fe *)malloc(sizeof(or_fe) * (N / 4));
+    closing_weights(r, N, L);                                      /* :311-316 */
+    for (int i = 0; i < 5; ++i) {                                  /* :318-325 */
+        or_fe e = { 0, 0 };
+        for (int j = 0; j < N / 4; ++j) e = or_add(e, or_mul(L[j], Hf[(uint64_t)i * (N / 2) + j]));
+        claims[i] = e;
+    }
+    free(L); free(mine); free(Hf); free(fin_r); free(ones); free(lens);
+    return 0;
+}
+
+int or_cubic_cosumcheck(int N, or_fe **p1, or_fe **p2, or_fe **p3, uint64_t M, const or_fe *r1, int nr1,
+                        const or_fe *r2, int nr2, or_fe ***R, const uint64_t *R_sizes, int nR, or_fe K, or_fe K_r,
+                        const int *degrees, int ndeg, or_fe *prev, or_fe *claims, or_fe *r_out, or_fe *ip_vec,
+                        int *failures) {
+    if (N < 4 || (N & (N - 1)) || nR != ndeg || (1ULL << nr1) != M || (1 << nr2) < N / 4) return -1;
+    uint64_t Mc = M;
+    int logsum = 0;
+    for (int i = 0; i < ndeg; ++i) {
+        if (R_sizes[i] != 4 * (uint64_t)degrees[i] || Mc % (uint64_t)degrees[i]) return -1;
+        Mc /= (uint64_t)degrees[i];
+        logsum += ilog2u((uint64_t)degrees[i]);
+    }
+    if (Mc < 1 || logsum > nr1) return -1;
+    Mc = M;
+    *failures = 0;
+    or_fe kk[2] = { K, K_r };
+    const or_fe c = or_hash_to_field(prev, kk, 2);                /* :336 */
+    K = or_add(K, or_mul(c, K_r));                                 /* :337 */
+    or_fe **b1 = (or_fe **)malloc(sizeof(or_fe *) * N);
+    for (int m = 0; m < N; ++m) {                                  /* precompute_beta(r1, b1) :342 */
+        b1[m] = (or_fe *)malloc(sizeof(or_fe) * M);
+        or_precompute_beta(r1, nr1, b1[m]);
+    }
+    or_fe *b2 = (or_fe *)malloc(sizeof(or_fe) << nr2);
+    or_precompute_beta(r2, nr2, b2);                               /* :343 */
+    int *lens = (int *)malloc(sizeof(int) * ndeg), total = 0;
+    for (int i = 0; i < ndeg; ++i) { lens[i] = 4 * degrees[i]; total += lens[i]; }
+    for (int t = 0; t < total; ++t) { ip_vec[t].real = 1; ip_vec[t].img = 0; }
+    or_fe *fin_r = (or_fe *)malloc(sizeof(or_fe) * N);
+    or_fe **LV = (or_fe **)malloc(sizeof(or_fe *) * ndeg);
+    for (int i = 0; i < ndeg; ++i) {
+        const int d = degrees[i], len = 4 * d;
+        or_fe *all = (or_fe *)malloc(sizeof(or_fe) * N * len), *H = (or_fe *)malloc(sizeof(or_fe) * len);
+        for (int m = 0; m < N; ++m) {                              /* :9-62 on every rank */
+            or_cosumcheck3_phase1(p1[m], p2[m], p3[m], b1[m], Mc, d, all + (uint64_t)m * len);
+            add_mask(all + (uint64_t)m * len, len, R[m], R_sizes, nR, i, c, 4);
+        }
+        or_sumcheck_aggr(all, N, len, b2, H);                      /* sumcheck_aggr(H, b2, N) :61 */
+        or_fe eval = { 0, 0 };
+        for (int j = 0; j < len / 4; ++j) eval = or_add(eval, H[4 * j]);   /* :358-361 */
+        if (eval.real != K.real || eval.img != K.img) *failures |= 1 << i;
+        const or_fe r = or_hash_to_field(prev, H, len);            /* :367 */
+        r_out[1 + i] = r;
+        or_fe *L4 = (or_fe *)malloc(sizeof(or_fe) * len);
+        or_compute_lagrange_coeff(r, len, L4);                     /* :371 */
+        ip_update(ip_vec, lens, ndeg, i, L4);
+        K.real = 0; K.img = 0;
+        for (int j = 0; j < len; ++j) K = or_add(K, or_mul(L4[j], H[j]));   /* :387 */
+        for (int m = 0; m < N; ++m) fin_r[m] = phase2_sparse(R[m], R_sizes, nR, L4, i);   /* :388 */
+        LV[i] = (or_fe *)malloc(sizeof(or_fe) * d);
+        or_compute_lagrange_coeff(r, d, LV[i]);                    /* :390 */
+        for (int m = 0; m < N; ++m) or_cosumcheck3_phase2(p1[m], p2[m], p3[m], b1[m], Mc, d, LV[i]);   /* :393 */
+        Mc /= (uint64_t)d;
+        free(L4); free(all); free(H);
+    }
+    /* send_final_sumcheck_poly({p1[0], p2[0], p3[0], final_eval_r}, N) (:398) */
+    or_fe *mine = (or_fe *)malloc(sizeof(or_fe) * N * 4), *Hf = (or_fe *)malloc(sizeof(or_fe) * 4 * (N / 2));
+    for (int m = 0; m < N; ++m) {
+        mine[m * 4 + 0] = p1[m][0]; mine[m * 4 + 1] = p2[m][0];
+        mine[m * 4 + 2] = p3[m][0]; mine[m * 4 + 3] = fin_r[m];
+    }
+    final_unpack(mine, N, 4, Hf);
+    const or_fe *H0 = Hf, *H1 = Hf + N / 2, *H2 = Hf + N, *H3 = Hf + 3 * (N / 2);
+    for (int i = 0; i < N / 4; ++i)                                /* :403-405 */
+        K = or_sub(K, or_mul(b2[i], or_add(or_mul(b1[0][0], or_sub(or_mul(H0[i], H1[i]), H2[i])), or_mul(c, H3[i]))));
+    if (K.real || K.img) *failures |= 1 << ndeg;
+    or_fe r = or_hash_to_field(prev, H0, N / 2);                   /* :410: H[3] is not absorbed */
+    r = or_hash_to_field(prev, H1, N / 2);
+    r = or_hash_to_field(prev, H2, N / 2);
+    r_out[0] = r;
+    or_fe *L = (or_fe *)malloc(sizeof(or_fe) * (N / 4));
+    closing_weights(r, N, L);                                      /* :413-418 */
+    for (int i = 0; i < 4; ++i) {                                  /* :420-427 */
+        or_fe e = { 0, 0 };
+        for (int j = 0; j < N / 4; ++j) e = or_add(e, or_mul(L[j], Hf[(uint64_t)i * (N / 2) + j]));
+        claims[i] = e;
+    }
+    /* verification 2: b1[0] = prod_i <beta(r1 slice i), L_i> (:429-443) */
+    {
+        or_fe y = { 1, 0 };
+        int ctr = 0;
+        for (int i = 0; i < ndeg; ++i) {
+            const int lg = ilog2u((uint64_t)degrees[i]);
+            or_fe *tb = (or_fe *)malloc(sizeof(or_fe) << lg);
+            or_precompute_beta(r1 + ctr, lg, tb);
+            ctr += lg;
+            or_fe s = { 0, 0 };
+            for (int j = 0; j < degrees[i]; ++j) s = or_add(s, or_mul(tb[j], LV[i][j]));
+            y = or_mul(y, s);
+            free(tb);
+        }
+        if (y.real != b1[0][0].real || y.img != b1[0][0].img) *failures |= 1 << (ndeg + 1);
+    }
+    for (int i = 0; i < ndeg; ++i) free(LV[i]);
+    for (int m = 0; m < N; ++m) free(b1[m]);
+    free(LV); free(b1); free(b2); free(L); free(mine); free(Hf); free(fin_r); free(lens);
+    return 0;
+}

@@ -184,6 +184,25 @@ void or_compute_secret_shares(const or_fe *v, uint64_t M, int N, int k, int kk, 
  * N rows of len (rank-major), b N/4 weights; out len elements. */
 void or_sumcheck_aggr(const or_fe *polys, int N, int len, const or_fe *b, or_fe *out);
 
+/* ---- coSumcheck drivers (src/coSumcheck.cpp:233-448), all N ranks in
+ * lockstep: rank m's shares p1[m], p2[m] (, p3[m]) of M elements are folded
+ * in place, R[m][i] (R_sizes[i] elements: 2 d_i quadratic, 4 d_i cubic) are
+ * rank m's masks (modified as the reference modifies its copy); the
+ * gathers of sumcheck_aggr and send_final_sumcheck_poly read the other
+ * ranks' values; prev is the transcript every rank shares.  Outputs: the
+ * claims (5 quadratic, 4 cubic), r_out = sumcheck_r (ndeg + 1: the closing
+ * challenge first), ip_vec (sum of the R sizes) and the reference's
+ * printed (not fatal) check failures as a bit mask: bit i round i's
+ * eval != K, bit ndeg the final check, bit ndeg + 1 the cubic's
+ * verification 2.  N >= 4 a power of two; 0 or -1. */
+int or_quadratic_cosumcheck(int N, or_fe **p1, or_fe **p2, uint64_t M, or_fe ***R, const uint64_t *R_sizes, int nR,
+                            or_fe K, or_fe K_r, const int *degrees, int ndeg, or_fe *prev,
+                            or_fe *claims, or_fe *r_out, or_fe *ip_vec, int *failures);
+int or_cubic_cosumcheck(int N, or_fe **p1, or_fe **p2, or_fe **p3, uint64_t M, const or_fe *r1, int nr1,
+                        const or_fe *r2, int nr2, or_fe ***R, const uint64_t *R_sizes, int nR, or_fe K, or_fe K_r,
+                        const int *degrees, int ndeg, or_fe *prev, or_fe *claims, or_fe *r_out, or_fe *ip_vec,
+                        int *failures);
+
 /* ---- all-cores variants (oracle_mt.c, OpenMP): the same bytes as the
  * single-thread functions above, `threads` OpenMP threads ---- */
 void or_fft_mt(or_fe *arr, int logn, int inverse, int threads);            /* utils.cpp:904-964, 1583-1673 */

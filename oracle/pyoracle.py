@@ -123,6 +123,12 @@ def oracle():
         L.or_whir_set_threads.argtypes = [i32]
         L.or_whir_open_weights.argtypes = [vp, u64, vp, _Fe, vp, u64, vp, u64, i32, i32, vp, u64, vp, vp]
         L.or_whir_open_weights.restype = ctypes.c_int64
+        L.or_quadratic_cosumcheck.argtypes = [i32, vp, vp, u64, vp, vp, i32, _Fe, _Fe, vp, i32, vp, vp, vp, vp,
+                                              ctypes.POINTER(i32)]
+        L.or_quadratic_cosumcheck.restype = i32
+        L.or_cubic_cosumcheck.argtypes = [i32, vp, vp, vp, u64, vp, i32, vp, i32, vp, vp, i32, _Fe, _Fe, vp, i32, vp,
+                                          vp, vp, vp, ctypes.POINTER(i32)]
+        L.or_cubic_cosumcheck.restype = i32
         L.or_update_betas_mt.argtypes = [vp, u64, vp, vp, i32, i32]
         _o = L
     return _o
@@ -589,3 +595,56 @@ def whir_open_weights(arr, b, K, cw, nodes, zero_leaves, rate, whir_k, cap=1 << 
     if w < 0:
         return int(w), None, None, None
     return int(w), rec[:w].copy(), chal[:n.bit_length() - 1], unfe(fin)
+
+
+def _mask_ptrs(R):
+    """R[m][i] arrays -> or_fe*** (rank m, iteration i), kept alive by the caller."""
+    rows = [(ctypes.c_void_p * len(Rm))(*[a.ctypes.data for a in Rm]) for Rm in R]
+    top = (ctypes.c_void_p * len(R))(*[ctypes.cast(r, ctypes.c_void_p).value for r in rows])
+    return top, rows
+
+
+def quadratic_cosumcheck(p1, p2, R, K, K_r, degrees, prev):
+    """quadratic_coSumcheck over N ranks in lockstep.  p1, p2: lists of N
+    share vectors (copied); R: N lists of masks (copied).  Returns (status,
+    claims (5, 2), sumcheck_r (ndeg + 1, 2), ip_vec, failures); prev updated."""
+    N = len(p1)
+    a = [np.ascontiguousarray(x, dtype=np.uint64).copy() for x in p1]
+    b = [np.ascontiguousarray(x, dtype=np.uint64).copy() for x in p2]
+    RR = [[np.ascontiguousarray(x, dtype=np.uint64).copy() for x in Rm] for Rm in R]
+    top, keep = _mask_ptrs(RR)
+    sizes = np.ascontiguousarray([x.shape[0] for x in RR[0]], dtype=np.uint64)
+    deg = np.ascontiguousarray(degrees, dtype=np.int32)
+    claims = np.zeros((5, 2), np.uint64)
+    r = np.zeros((len(degrees) + 1, 2), np.uint64)
+    ip = np.zeros((int(sizes.sum()), 2), np.uint64)
+    fails = ctypes.c_int(0)
+    st = np.ascontiguousarray(prev, dtype=np.uint64)
+    rc = oracle().or_quadratic_cosumcheck(N, _ptrs(a), _ptrs(b), a[0].shape[0], top, _p(sizes), len(RR[0]), fe(K),
+                                          fe(K_r), _p(deg), len(degrees), _p(st), _p(claims), _p(r), _p(ip),
+                                          ctypes.byref(fails))
+    prev[...] = st
+    return rc, claims, r, ip, fails.value
+
+
+def cubic_cosumcheck(p1, p2, p3, r1, r2, R, K, K_r, degrees, prev):
+    """cubic_coSumcheck over N ranks in lockstep: (status, claims (4, 2),
+    sumcheck_r, ip_vec, failures); prev updated."""
+    N = len(p1)
+    a, b, c = ([np.ascontiguousarray(x, dtype=np.uint64).copy() for x in v] for v in (p1, p2, p3))
+    RR = [[np.ascontiguousarray(x, dtype=np.uint64).copy() for x in Rm] for Rm in R]
+    top, keep = _mask_ptrs(RR)
+    sizes = np.ascontiguousarray([x.shape[0] for x in RR[0]], dtype=np.uint64)
+    deg = np.ascontiguousarray(degrees, dtype=np.int32)
+    rr1 = np.ascontiguousarray(r1, dtype=np.uint64)
+    rr2 = np.ascontiguousarray(r2, dtype=np.uint64)
+    claims = np.zeros((4, 2), np.uint64)
+    r = np.zeros((len(degrees) + 1, 2), np.uint64)
+    ip = np.zeros((int(sizes.sum()), 2), np.uint64)
+    fails = ctypes.c_int(0)
+    st = np.ascontiguousarray(prev, dtype=np.uint64)
+    rc = oracle().or_cubic_cosumcheck(N, _ptrs(a), _ptrs(b), _ptrs(c), a[0].shape[0], _p(rr1), rr1.shape[0],
+                                      _p(rr2), rr2.shape[0], top, _p(sizes), len(RR[0]), fe(K), fe(K_r), _p(deg),
+                                      len(degrees), _p(st), _p(claims), _p(r), _p(ip), ctypes.byref(fails))
+    prev[...] = st
+    return rc, claims, r, ip, fails.value

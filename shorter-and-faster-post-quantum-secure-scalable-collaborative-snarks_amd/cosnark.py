@@ -30,6 +30,10 @@ class CosnarkError(RuntimeError):
 
 _lib = None
 
+# cosnark_allgather_fn: (user, mine, elems, all) -> 0 on success
+ALLGATHER_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint64,
+                                ctypes.POINTER(ctypes.c_uint64))
+
 
 def lib():
     global _lib
@@ -91,6 +95,12 @@ def lib():
         L.cosnark_whir_aggregate_claims.argtypes = [vp, i32, vp, u64, i32, vp, vp, vp, vp, vp, vp]
         L.cosnark_whir_open_weights.argtypes = [vp, vp, u64, vp, vp, vp, u64, vp, u64, i32, i32, vp, u64,
                                                 ctypes.POINTER(u64), vp, vp]
+        L.cosnark_quadratic_cosumcheck.argtypes = [vp, vp, vp, u64, vp, vp, i32, vp, vp, vp, i32, i32, i32,
+                                                   ALLGATHER_FN, vp, vp, vp, vp, vp,
+                                                   ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i32)]
+        L.cosnark_cubic_cosumcheck.argtypes = [vp, vp, vp, vp, u64, vp, i32, vp, i32, vp, vp, i32, vp, vp, vp, i32,
+                                               i32, i32, ALLGATHER_FN, vp, vp, vp, vp, vp,
+                                               ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i32)]
         L.cosnark_field_random.argtypes = [vp]
         L.cosnark_field_random.restype = None
         L.cosnark_blake3_64.argtypes = [vp, vp]
@@ -461,6 +471,51 @@ class Context:
                                              _p(rec), cap, ctypes.byref(words), _p(chal), _p(fin))
         return rc, rec[:words.value].copy(), chal[:m], fin
 
+    def _masks(self, R):
+        RR = [np.ascontiguousarray(x, dtype=np.uint64) for x in R]
+        ptrs = (ctypes.c_void_p * len(RR))(*[x.ctypes.data for x in RR])
+        sizes = np.ascontiguousarray([x.shape[0] for x in RR], dtype=np.uint64)
+        return RR, ptrs, sizes
+
+    def quadratic_cosumcheck(self, d_p1, d_p2, M, R, K, K_r, degrees, nranks, rank, gather, ts):
+        """quadratic_coSumcheck as rank `rank` of nranks; gather: an
+        ALLGATHER_FN.  Returns (claims (5, 2), r (ndeg + 1, 2), ip_vec, ps,
+        check-failure mask).  d_p1, d_p2 are folded in place."""
+        RR, ptrs, sizes = self._masks(R)
+        deg = np.ascontiguousarray(degrees, dtype=np.int32)
+        KK = np.ascontiguousarray(K, dtype=np.uint64).reshape(2)
+        Kr = np.ascontiguousarray(K_r, dtype=np.uint64).reshape(2)
+        claims = np.zeros((5, 2), np.uint64)
+        r = np.zeros((len(degrees) + 1, 2), np.uint64)
+        ip = np.zeros((int(sizes.sum()), 2), np.uint64)
+        ps = ctypes.c_double(0)
+        fails = ctypes.c_int(0)
+        self._ck(lib().cosnark_quadratic_cosumcheck(self.h, d_p1.ptr, d_p2.ptr, M, ptrs, _p(sizes), len(RR), _p(KK),
+                                                    _p(Kr), _p(deg), len(degrees), nranks, rank, gather, None,
+                                                    ctypes.byref(ts), _p(claims), _p(r), _p(ip), ctypes.byref(ps),
+                                                    ctypes.byref(fails)))
+        return claims, r, ip, ps.value, fails.value
+
+    def cubic_cosumcheck(self, d_p1, d_p2, d_p3, M, r1, r2, R, K, K_r, degrees, nranks, rank, gather, ts):
+        """cubic_coSumcheck as rank `rank` of nranks: (claims (4, 2), r, ip_vec,
+        ps, check-failure mask)."""
+        RR, ptrs, sizes = self._masks(R)
+        deg = np.ascontiguousarray(degrees, dtype=np.int32)
+        rr1 = np.ascontiguousarray(r1, dtype=np.uint64)
+        rr2 = np.ascontiguousarray(r2, dtype=np.uint64)
+        KK = np.ascontiguousarray(K, dtype=np.uint64).reshape(2)
+        Kr = np.ascontiguousarray(K_r, dtype=np.uint64).reshape(2)
+        claims = np.zeros((4, 2), np.uint64)
+        r = np.zeros((len(degrees) + 1, 2), np.uint64)
+        ip = np.zeros((int(sizes.sum()), 2), np.uint64)
+        ps = ctypes.c_double(0)
+        fails = ctypes.c_int(0)
+        self._ck(lib().cosnark_cubic_cosumcheck(self.h, d_p1.ptr, d_p2.ptr, d_p3.ptr, M, _p(rr1), rr1.shape[0],
+                                                _p(rr2), rr2.shape[0], ptrs, _p(sizes), len(RR), _p(KK), _p(Kr),
+                                                _p(deg), len(degrees), nranks, rank, gather, None, ctypes.byref(ts),
+                                                _p(claims), _p(r), _p(ip), ctypes.byref(ps), ctypes.byref(fails)))
+        return claims, r, ip, ps.value, fails.value
+
     def whir_aggregate_claims(self, arrays, points):
         """aggregate_claims (src/Whir.cpp:608-668): (arr, b_aggr, K, beta_ch,
         arr_ch) for host arrays of equal size and claim points (npoints,
@@ -775,3 +830,33 @@ def verify_plain(proof, n, y, ts=None):
     bad = ctypes.c_int(0)
     rc = lib().cosnark_verify_plain(_p(pr), n, _p(yy), ctypes.byref(t), _p(r), ctypes.byref(bad))
     return rc, r, bad.value
+
+
+class ThreadAllGather:
+    """An all-gather among `n` threads of one process, for running n ranks'
+    coSumcheck drivers side by side (each thread holds its own Context).
+    callback(rank) gives the ALLGATHER_FN for that rank's thread."""
+
+    def __init__(self, n):
+        import threading
+        self.n = n
+        self.barrier = threading.Barrier(n)
+        self.parts = [None] * n
+        self._keep = []
+
+    def callback(self, rank):
+        def fn(user, mine, elems, all_out):
+            try:
+                self.parts[rank] = np.ctypeslib.as_array(mine, shape=(2 * elems,)).copy()
+                self.barrier.wait()
+                out = np.ctypeslib.as_array(all_out, shape=(2 * elems * self.n,))
+                for m in range(self.n):
+                    out[2 * elems * m: 2 * elems * (m + 1)] = self.parts[m]
+                self.barrier.wait()          # everyone has read before the next exchange writes
+                return 0
+            except Exception:
+                self.barrier.abort()
+                return 1
+        cb = ALLGATHER_FN(fn)
+        self._keep.append(cb)
+        return cb

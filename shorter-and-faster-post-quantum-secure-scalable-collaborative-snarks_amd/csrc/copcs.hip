@@ -37,6 +37,7 @@
 #include <cstring>
 #include <vector>
 #include "ctx.h"
+#include "transcript.h"
 #include "dft.h"
 
 using gf::fe;
@@ -1334,6 +1335,302 @@ int cosnark_row_combine(cosnark_ctx *ctx, const uint64_t *d_rows, int nrows, uin
                        (const fe *)d_rows, nrows, len, w, (fe *)d_out);
     LAUNCHCK();
     return cosnark_sync(ctx);
+}
+
+}  // extern "C"
+
+// ---- coSumcheck drivers (src/coSumcheck.cpp:233-448), one rank ------------
+//
+// The rank's block-sum rounds run on the GPU (phase 1, fold); the masks, the
+// Lagrange weights, the transcript and rank 0's share arithmetic run on the
+// host.  The reference's gathers to rank 0 followed by a broadcast of the
+// combined result (sumcheck_aggr, src/MPI_utils.cpp:2604-2670;
+// send_final_sumcheck_poly, :2672-2736) are one all-gather through the
+// caller's callback: every rank then forms the same combination itself.
+
+namespace {
+
+struct CoDriver {
+    cosnark_ctx *ctx;
+    int N, rank;
+    cosnark_allgather_fn gather;
+    void *user;
+    cosnark_transcript *ts;
+    std::vector<std::vector<fe>> R;
+    int stride;                          // mask stride: 2 quadratic, 4 cubic
+    fe c;
+    std::vector<fe> ip;                  // ip_vec, the rounds' vectors back to back
+    std::vector<int> lens;
+    int fails = 0;
+    double ps = 0;
+
+    fe absorb(const fe *x, size_t n) {
+        fe prev = gf::mk(ts->prev[0], ts->prev[1]);
+        for (size_t i = 0; i < n; ++i) prev = ts::hash_F(prev, x[i]);
+        ts->prev[0] = prev.re;
+        ts->prev[1] = prev.im;
+        return prev;
+    }
+    // H[t] += c H_r[t] (coSumcheck.cpp:182-193 / :48-58)
+    void mask(std::vector<fe> &H, int iter) const {
+        fe h0 = R[iter][0];
+        for (size_t j = iter + 1; j < R.size(); ++j)
+            for (size_t k = 0; k < R[j].size() / stride; ++k) h0 = gf::add(h0, R[j][stride * k]);
+        for (size_t t = 0; t < H.size(); ++t) H[t] = gf::add(H[t], gf::mul(c, t == 0 ? h0 : R[iter][t]));
+    }
+    // every rank's values of `mine`, rank-major
+    int allgather(const std::vector<fe> &mine, std::vector<fe> &all) {
+        all.assign((size_t)N * mine.size(), gf::zero());
+        const int rc = gather(user, (const uint64_t *)mine.data(), mine.size(), (uint64_t *)all.data());
+        return rc ? set_err(ctx, COSNARK_E_ARG, "coSumcheck: the all-gather callback failed") : COSNARK_OK;
+    }
+    // sumcheck_aggr(H, b, N): the gathered polynomials combined with b
+    int aggr(std::vector<fe> &H, const std::vector<fe> &b) {
+        std::vector<fe> all;
+        int rc = allgather(H, all);
+        if (rc) return rc;
+        if (cosnark_sumcheck_aggr((const uint64_t *)all.data(), N, (int)H.size(), (const uint64_t *)b.data(),
+                                  (uint64_t *)H.data()))
+            return set_err(ctx, COSNARK_E_ARG, "coSumcheck: sumcheck_aggr sizes");
+        if (rank == 0) ps += (double)H.size() * 16 / 1024.0;
+        return COSNARK_OK;
+    }
+    // one round's bookkeeping after the aggregated polynomial H of `len`
+    // entries is known: the check, the challenge, ip_vec, K, the sparse mask
+    // fold (coSumcheck.cpp:215-228); returns r
+    fe round(const std::vector<fe> &H, int i, fe &K, fe &final_eval_r) {
+        fe eval = gf::zero();
+        for (size_t j = 0; j < H.size() / stride; ++j) eval = gf::add(eval, H[stride * j]);
+        if (!gf::eq(eval, K)) fails |= 1 << i;
+        const fe r = absorb(H.data(), H.size());
+        const std::vector<fe> L = lagrange_weights(r, (int)H.size());
+        size_t off = 0;
+        for (size_t j = 0; j < lens.size(); ++j) {
+            for (int k = 0; k < lens[j]; ++k) ip[off + k] = gf::mul(ip[off + k], (int)j == i ? L[k] : L[0]);
+            off += lens[j];
+        }
+        K = gf::zero();
+        for (size_t j = 0; j < H.size(); ++j) K = gf::add(K, gf::mul(L[j], H[j]));
+        fe s = gf::zero();
+        for (size_t j = 0; j < R[i].size(); ++j) s = gf::add(s, gf::mul(L[j], R[i][j]));
+        for (size_t j = i + 1; j < R.size(); ++j)
+            for (auto &x : R[j]) x = gf::mul(x, L[0]);
+        if ((size_t)i + 1 < R.size()) R[i + 1][0] = gf::add(R[i + 1][0], s);
+        final_eval_r = s;
+        return r;
+    }
+    // send_final_sumcheck_poly(s, N): coordinate i's N shares -> N/2 values
+    int final_values(const std::vector<fe> &mine, std::vector<std::vector<fe>> &H) {
+        std::vector<fe> all;
+        int rc = allgather(mine, all);
+        if (rc) return rc;
+        H.assign(mine.size(), std::vector<fe>(N / 2));
+        std::vector<fe> P(N);
+        for (size_t i = 0; i < mine.size(); ++i) {
+            for (int m = 0; m < N; ++m) P[m] = all[(size_t)m * mine.size() + i];
+            cosnark_share_unpack((uint64_t *)P.data(), N, N / 2, (uint64_t *)H[i].data());
+        }
+        if (rank == 0) ps += (double)mine.size() * (N / 2) * 16 / 1024.0;
+        return COSNARK_OK;
+    }
+    std::vector<fe> closing(const fe &r) const {
+        return N / 4 == 1 ? std::vector<fe>{gf::one()} : lagrange_weights(r, N / 4);
+    }
+};
+
+int codriver_args(cosnark_ctx *ctx, uint64_t M, const uint64_t *const *h_R, const uint64_t *R_sizes, int nR,
+                  const int *degrees, int ndeg, int nranks, int rank, cosnark_allgather_fn gather, void *ts,
+                  int rmul, int dmax, int final_len, const char *what) {
+    if (!ctx || !h_R || !R_sizes || !degrees || !gather || !ts) return set_err(ctx, COSNARK_E_ARG, std::string(what) + ": null argument");
+    if (nranks < 4 || nranks > CO_HOST_ELEMS || !is_pow2((uint64_t)nranks) || rank < 0 || rank >= nranks)
+        return set_err(ctx, COSNARK_E_ARG, std::string(what) + ": nranks a power of two >= 4 (the reference divides by N/4)");
+    if (ndeg < 1 || nR != ndeg || !is_pow2(M)) return set_err(ctx, COSNARK_E_ARG, std::string(what) + ": one mask per round, M a power of two");
+    uint64_t Mc = M;
+    for (int i = 0; i < ndeg; ++i) {
+        if (degrees[i] < 2 || degrees[i] > dmax || !is_pow2((uint64_t)degrees[i]) || Mc < (uint64_t)degrees[i])
+            return set_err(ctx, COSNARK_E_ARG, std::string(what) + ": degrees powers of two within the vector");
+        if (R_sizes[i] != (uint64_t)rmul * degrees[i] || !h_R[i])
+            return set_err(ctx, COSNARK_E_ARG, std::string(what) + ": mask i must hold the round's polynomial size");
+        Mc /= (uint64_t)degrees[i];
+    }
+    if (Mc < (uint64_t)final_len) return set_err(ctx, COSNARK_E_ARG, std::string(what) + ": vectors folded below the final values");
+    return COSNARK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cosnark_quadratic_cosumcheck(cosnark_ctx *ctx, uint64_t *d_p1, uint64_t *d_p2, uint64_t M,
+                                 const uint64_t *const *h_R, const uint64_t *R_sizes, int nR, const uint64_t K_in[2],
+                                 const uint64_t K_r[2], const int *degrees, int ndeg, int nranks, int rank,
+                                 cosnark_allgather_fn gather, void *user, cosnark_transcript *ts, uint64_t *h_claims,
+                                 uint64_t *h_r, uint64_t *h_ip_vec, double *ps, int *check_failures) {
+    int rc = codriver_args(ctx, M, h_R, R_sizes, nR, degrees, ndeg, nranks, rank, gather, ts, 2, BSC_D_MAX, 2,
+                           "quadratic_coSumcheck");
+    if (rc) return rc;
+    if (!d_p1 || !d_p2 || !K_in || !K_r || !h_claims || !h_r)
+        return set_err(ctx, COSNARK_E_ARG, "quadratic_coSumcheck: null argument");
+    CoDriver D{ctx, nranks, rank, gather, user, ts, {}, 2, gf::zero(), {}, {}};
+    for (int i = 0; i < nR; ++i) D.R.emplace_back((const fe *)h_R[i], (const fe *)h_R[i] + R_sizes[i]);
+    fe K = gf::mk(K_in[0], K_in[1]);
+    const fe kk[2] = {K, gf::mk(K_r[0], K_r[1])};
+    D.c = D.absorb(kk, 2);                                               // :234
+    K = gf::add(K, gf::mul(D.c, kk[1]));                                 // :235
+    for (int i = 0; i < ndeg; ++i) D.lens.push_back(2 * degrees[i]);
+    for (int l : D.lens) D.ip.insert(D.ip.end(), (size_t)l, gf::one()); // :244-247
+    const std::vector<fe> ones(nranks / 4, gf::one());
+    std::vector<fe> rs;
+    fe final_eval_r = gf::zero();
+    uint64_t Mc = M;
+    for (int i = 0; i < ndeg; ++i) {
+        const int d = degrees[i];
+        std::vector<fe> H(2 * d);
+        if ((rc = cosnark_cosumcheck_phase1(ctx, d_p1, d_p2, Mc, d, (uint64_t *)H.data()))) return rc;
+        D.mask(H, i);
+        if ((rc = D.aggr(H, ones))) return rc;                           // :197
+        const fe r = D.round(H, i, K, final_eval_r);                     // :252-284
+        rs.push_back(r);
+        const uint64_t rr[2] = {r.re, r.im};
+        if ((rc = cosnark_cosumcheck_phase2(ctx, d_p1, d_p2, Mc, d, rr))) return rc;   // :286-289
+        Mc /= (uint64_t)d;
+    }
+    fe v[4];
+    HIPCK(hipMemcpy(v, d_p1, 2 * sizeof(fe), hipMemcpyDeviceToHost));
+    HIPCK(hipMemcpy(v + 2, d_p2, 2 * sizeof(fe), hipMemcpyDeviceToHost));
+    std::vector<std::vector<fe>> H;
+    if ((rc = D.final_values({v[0], v[2], v[1], v[3], final_eval_r}, H))) return rc;   // :293
+    for (int i = 0; i < nranks / 4; ++i)                                 // :298-300
+        K = gf::sub(K, gf::add(gf::add(gf::mul(H[0][i], H[1][i]), gf::mul(H[2][i], H[3][i])), gf::mul(D.c, H[4][i])));
+    if (!gf::eq(K, gf::zero())) D.fails |= 1 << ndeg;
+    fe r = D.absorb(H[0].data(), H[0].size());                           // :306-307
+    for (size_t i = 1; i < H.size(); ++i) r = D.absorb(H[i].data(), H[i].size());
+    const std::vector<fe> L = D.closing(r);                              // :311-316
+    for (size_t i = 0; i < H.size(); ++i) {                              // :318-325
+        fe e = gf::zero();
+        for (int j = 0; j < nranks / 4; ++j) e = gf::add(e, gf::mul(L[j], H[i][j]));
+        h_claims[2 * i] = e.re;
+        h_claims[2 * i + 1] = e.im;
+    }
+    rs.insert(rs.begin(), r);
+    memcpy(h_r, rs.data(), sizeof(fe) * rs.size());
+    if (h_ip_vec) memcpy(h_ip_vec, D.ip.data(), sizeof(fe) * D.ip.size());
+    if (ps) *ps += D.ps;
+    if (check_failures) *check_failures = D.fails;
+    return COSNARK_OK;
+}
+
+int cosnark_cubic_cosumcheck(cosnark_ctx *ctx, uint64_t *d_p1, uint64_t *d_p2, uint64_t *d_p3, uint64_t M,
+                             const uint64_t *h_r1, int nr1, const uint64_t *h_r2, int nr2, const uint64_t *const *h_R,
+                             const uint64_t *R_sizes, int nR, const uint64_t K_in[2], const uint64_t K_r[2],
+                             const int *degrees, int ndeg, int nranks, int rank, cosnark_allgather_fn gather,
+                             void *user, cosnark_transcript *ts, uint64_t *h_claims, uint64_t *h_r, uint64_t *h_ip_vec,
+                             double *ps, int *check_failures) {
+    int rc = codriver_args(ctx, M, h_R, R_sizes, nR, degrees, ndeg, nranks, rank, gather, ts, 4, BSC3_D_MAX, 1,
+                           "cubic_coSumcheck");
+    if (rc) return rc;
+    if (!d_p1 || !d_p2 || !d_p3 || !h_r1 || !h_r2 || !K_in || !K_r || !h_claims || !h_r)
+        return set_err(ctx, COSNARK_E_ARG, "cubic_coSumcheck: null argument");
+    if (nr1 < 1 || nr1 > NTT_LOG_MAX || (1ull << nr1) != M || nr2 < 0 || nr2 > 13 || (1 << nr2) < nranks / 4)
+        return set_err(ctx, COSNARK_E_ARG, "cubic_coSumcheck: |p| = 2^|r1| and 2^|r2| >= N/4");
+    int logsum = 0;
+    for (int i = 0; i < ndeg; ++i) logsum += ilog2((uint64_t)degrees[i]);
+    if (logsum > nr1) return set_err(ctx, COSNARK_E_ARG, "cubic_coSumcheck: degrees beyond |r1|");
+    CoDriver D{ctx, nranks, rank, gather, user, ts, {}, 4, gf::zero(), {}, {}};
+    for (int i = 0; i < nR; ++i) D.R.emplace_back((const fe *)h_R[i], (const fe *)h_R[i] + R_sizes[i]);
+    fe K = gf::mk(K_in[0], K_in[1]);
+    const fe kk[2] = {K, gf::mk(K_r[0], K_r[1])};
+    D.c = D.absorb(kk, 2);                                               // :336
+    K = gf::add(K, gf::mul(D.c, kk[1]));                                 // :337
+    // b1 = precompute_beta(r1) on the device (M elements); b2 on the host
+    uint64_t *d_b1 = nullptr;
+    HIPCK(hipMalloc(&d_b1, sizeof(fe) * M));
+    struct Free { uint64_t *p; ~Free() { if (p) (void)hipFree(p); } } fb{d_b1};
+    if ((rc = cosnark_precompute_beta(ctx, h_r1, nr1, d_b1))) return rc;  // :342
+    std::vector<fe> b2(1u << nr2);                                       // :343
+    {
+        b2[0] = gf::one();
+        const fe *r2 = (const fe *)h_r2;
+        for (int i = 0; i < nr2; ++i) {                                  // precompute_beta, r2[0] on bit 0
+            const fe ri = r2[nr2 - 1 - i];
+            for (size_t j = (size_t)1 << i; j-- > 0;) {
+                const fe t = gf::mul(ri, b2[j]);
+                b2[2 * j] = gf::sub(b2[j], t);
+                b2[2 * j + 1] = t;
+            }
+        }
+    }
+    for (int i = 0; i < ndeg; ++i) D.lens.push_back(4 * degrees[i]);
+    for (int l : D.lens) D.ip.insert(D.ip.end(), (size_t)l, gf::one());
+    std::vector<fe> rs;
+    std::vector<std::vector<fe>> LV;
+    fe final_eval_r = gf::zero();
+    uint64_t Mc = M;
+    for (int i = 0; i < ndeg; ++i) {
+        const int d = degrees[i];
+        std::vector<fe> H(4 * d);
+        if ((rc = cosnark_cosumcheck3_phase1(ctx, d_p1, d_p2, d_p3, d_b1, Mc, d, (uint64_t *)H.data()))) return rc;
+        D.mask(H, i);
+        if ((rc = D.aggr(H, b2))) return rc;                             // :61
+        const fe r = D.round(H, i, K, final_eval_r);                     // :356-388
+        rs.push_back(r);
+        LV.push_back(lagrange_weights(r, d));                            // :390-392
+        if ((rc = cosnark_cosumcheck3_fold(ctx, d_p1, d_p2, d_p3, d_b1, Mc, d, (const uint64_t *)LV.back().data())))
+            return rc;                                                   // :393
+        Mc /= (uint64_t)d;
+    }
+    fe v[4];
+    HIPCK(hipMemcpy(v, d_p1, sizeof(fe), hipMemcpyDeviceToHost));
+    HIPCK(hipMemcpy(v + 1, d_p2, sizeof(fe), hipMemcpyDeviceToHost));
+    HIPCK(hipMemcpy(v + 2, d_p3, sizeof(fe), hipMemcpyDeviceToHost));
+    HIPCK(hipMemcpy(v + 3, d_b1, sizeof(fe), hipMemcpyDeviceToHost));
+    const fe b10 = v[3];
+    std::vector<std::vector<fe>> H;
+    if ((rc = D.final_values({v[0], v[1], v[2], final_eval_r}, H))) return rc;   // :398
+    for (int i = 0; i < nranks / 4; ++i)                                 // :403-405
+        K = gf::sub(K, gf::mul(b2[i], gf::add(gf::mul(b10, gf::sub(gf::mul(H[0][i], H[1][i]), H[2][i])),
+                                              gf::mul(D.c, H[3][i]))));
+    if (!gf::eq(K, gf::zero())) D.fails |= 1 << ndeg;
+    fe r = D.absorb(H[0].data(), H[0].size());                           // :410: H[3] is not absorbed
+    r = D.absorb(H[1].data(), H[1].size());
+    r = D.absorb(H[2].data(), H[2].size());
+    const std::vector<fe> L = D.closing(r);
+    for (size_t i = 0; i < H.size(); ++i) {                              // :420-427
+        fe e = gf::zero();
+        for (int j = 0; j < nranks / 4; ++j) e = gf::add(e, gf::mul(L[j], H[i][j]));
+        h_claims[2 * i] = e.re;
+        h_claims[2 * i + 1] = e.im;
+    }
+    // verification 2 (:429-443): b1[0] = prod_i <beta(r1 slice i), L_i>
+    {
+        fe y = gf::one();
+        int ctr = 0;
+        const fe *r1 = (const fe *)h_r1;
+        for (int i = 0; i < ndeg; ++i) {
+            const int lg = ilog2((uint64_t)degrees[i]);
+            std::vector<fe> tb((size_t)1 << lg);
+            tb[0] = gf::one();
+            for (int a = 0; a < lg; ++a) {
+                const fe ri = r1[ctr + lg - 1 - a];
+                for (size_t j = (size_t)1 << a; j-- > 0;) {
+                    const fe t = gf::mul(ri, tb[j]);
+                    tb[2 * j] = gf::sub(tb[j], t);
+                    tb[2 * j + 1] = t;
+                }
+            }
+            ctr += lg;
+            fe s = gf::zero();
+            for (int j = 0; j < degrees[i]; ++j) s = gf::add(s, gf::mul(tb[j], LV[i][j]));
+            y = gf::mul(y, s);
+        }
+        if (!gf::eq(y, b10)) D.fails |= 1 << (ndeg + 1);
+    }
+    rs.insert(rs.begin(), r);
+    memcpy(h_r, rs.data(), sizeof(fe) * rs.size());
+    if (h_ip_vec) memcpy(h_ip_vec, D.ip.data(), sizeof(fe) * D.ip.size());
+    if (ps) *ps += D.ps;
+    if (check_failures) *check_failures = D.fails;
+    return COSNARK_OK;
 }
 
 }  // extern "C"
