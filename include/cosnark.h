@@ -304,6 +304,24 @@ int cosnark_whir_batch_open(cosnark_ctx *ctx, int count, const uint64_t *const *
  * powers of two; rate*L <= 2^28. */
 int cosnark_copcs_encode(cosnark_ctx *ctx, const uint64_t *d_rows, int nrows, uint64_t L, int rate,
                          int log_rate_offset, uint64_t *d_codewords);
+/* PCS_commit at one rank (src/coPCS.cpp:236-250): prepare_data
+ * (:76-128) lays this rank's n_shares shares and its random mask values out
+ * in rows of B = n_shares / 16 elements (BATCH_FACTOR / 2,
+ * src/New_PCS.cpp:9): each row takes B - q0 shares and then q0 random
+ * values until the shares run out (the rest of that row's share part is
+ * zero), and one more row takes random values only (B of them while at
+ * least B + 1 shares exceed the random values used, else B - (n_shares -
+ * used)); q0 is the reference's _queries[0] (333 at rate 4).  The rows
+ * (*nrows of them, at most 32) are written to d_rows, encoded with
+ * cosnark_copcs_encode at `rate` and log_rate_offset into d_codewords
+ * (*nrows x rate B) and committed with cosnark_distributed_mt into d_nodes
+ * (2 rate B - 1 digests).  d_rows holds 32 B elements, d_codewords 32 rate B.
+ * COSNARK_E_RANGE where the reference exits (more than 32 rows needed). */
+int cosnark_pcs_commit(cosnark_ctx *ctx, const uint64_t *d_shares, uint64_t n_shares, const uint64_t *d_random,
+                       uint64_t n_random, int q0, int rate, int log_rate_offset, uint64_t *d_rows,
+                       uint64_t *d_codewords, uint8_t *d_nodes, int *nrows);
+/* prepare_data's row count and row size for these sizes, without a GPU */
+int cosnark_pcs_prepare_rows(uint64_t n_shares, uint64_t n_random, int q0, int *nrows, uint64_t *batch_size);
 /* The segments and folds of delegate_checks (src/coPCS.cpp:740-760):
  * for each query q < l, segment q = d_cw[h_idx[q] + s * (n/fd)], s < fd,
  * and h_y[q] = RS_multifold_segment(segment q, h_L, n, h_idx[q])

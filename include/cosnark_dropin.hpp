@@ -51,6 +51,26 @@ inline cosnark_transcript &transcript() {                               // src/F
     thread_local cosnark_transcript t{{0, 0}};
     return t;
 }
+// This rank and the exchange the coSumcheck drivers use in place of the
+// reference's MPI gathers and broadcasts (MPI_Comm_rank and
+// sumcheck_aggr / send_final_sumcheck_poly, src/MPI_utils.cpp:2604-2736):
+// an all-gather over the N ranks, e.g. MPI_Allgather of 2 * elems
+// MPI_UINT64_T.  Per thread, so threads of one process can act as ranks.
+struct Comm {
+    int rank = 0;
+    cosnark_allgather_fn gather = nullptr;
+    void *user = nullptr;
+};
+inline Comm &comm() {
+    thread_local Comm c;
+    return c;
+}
+// ip_vec (src/coSumcheck.cpp:231), written by the coSumcheck drivers
+template <class F>
+std::vector<std::vector<F>> &ip_vec() {
+    thread_local std::vector<std::vector<F>> v;
+    return v;
+}
 
 namespace detail {
 
@@ -630,6 +650,98 @@ void Encode(std::vector<std::vector<F>> &Codeword, int, std::vector<int> rate) {
     }
 }
 
+// prepare_parameters' globals (src/coPCS.cpp:18-24, src/New_PCS.cpp:3-16)
+inline std::vector<int> &_degrees() { static std::vector<int> v; return v; }
+inline std::vector<int> &_queries() { static std::vector<int> v; return v; }
+inline std::vector<std::vector<int>> &_rates() { static std::vector<std::vector<int>> v; return v; }
+inline std::vector<int> &delegation_size() { static std::vector<int> v; return v; }
+inline int &randomness() { static int v = 0; return v; }
+inline int &max_delegation_size() { static int v = 0; return v; }
+inline int &Security_Level() { static int v = 100; return v; }     // src/New_PCS.cpp:3
+constexpr int BATCH_FACTOR = 32;                                   // src/New_PCS.cpp:9
+
+// prepare_parameters(rates, rounds, N, M), src/coPCS.cpp:38-70: get_degrees
+// (rounds, 2 (4 M / N)) and get_queries(rounds, rates) (src/New_PCS.cpp:
+// 23-76) append to _degrees() and _queries(); rate_offset() multiplies by
+// rates[i][0] for i >= 1; delegation_size() gets next_pow2(d_i q_i) for
+// i >= 2; randomness() adds q_i d_i; max_delegation_size() the largest
+// delegation size.  The reference's globals accumulate across calls, as
+// here; its rank 0 prints them, this does not.
+inline void prepare_parameters(std::vector<std::vector<int>> rates, int rounds, int N, int M) {
+    if (rounds < 3 || (int)rates.size() < rounds || N < 1) throw error(COSNARK_E_ARG, "prepare_parameters: arguments");
+    // get_degrees (New_PCS.cpp:53-76)
+    int log_size = (int)std::log2((double)(2 * (4 * (long long)M / N)));
+    _degrees().push_back(BATCH_FACTOR);
+    log_size -= (int)std::log2((double)BATCH_FACTOR);
+    _degrees().push_back(8);
+    log_size -= 3;
+    const int deg = log_size / (rounds - 2);
+    for (int i = 1; i < rounds - 2; ++i) _degrees().push_back(1 << deg);
+    _degrees().push_back(1 << (log_size - deg * (rounds - 3)));
+    // get_queries (New_PCS.cpp:23-47)
+    const double SL = Security_Level();
+    for (int i = 0; i < rounds; ++i) {
+        if (i <= 1) {
+            _queries().push_back((int)(SL / (-std::log2(1 - 0.25 * (1.0 - 1.0 / (double)rates[0][0])))));
+        } else {
+            const double r0 = rates[i][0], r1 = rates[i][1];
+            const int l1 = (int)(SL / (-std::log2(1 - 0.5 * 0.5 * (1.0 - 1.0 / r0) * (1 - std::sqrt(1.0 / r1)))));
+            const int l2 = (int)(SL / (-std::log2(1 - 0.5 * 0.5 * (1.0 - 1.0 / r0))) +
+                                 SL / (-std::log2(std::sqrt(1.0 / r1))));
+            _queries().push_back(l1 > l2 ? l2 : l1);
+        }
+    }
+    _rates() = rates;
+    for (size_t i = 1; i < rates.size(); ++i) rate_offset() *= (unsigned)rates[i][0];   // :44-46
+    auto next_pow2 = [](size_t x) -> int {                                               // src/utils.cpp:20-25
+        if ((1 << (int)std::log2((double)x)) == (int)x) return (int)x;
+        return 1 << (1 + (int)std::log2((double)x));
+    };
+    for (size_t i = 2; i < _degrees().size(); ++i)
+        delegation_size().push_back(next_pow2((size_t)_degrees()[i] * _queries()[i]));
+    for (size_t i = 0; i < _degrees().size(); ++i) randomness() += _queries()[i] * _degrees()[i];
+    if (delegation_size().empty()) throw error(COSNARK_E_ARG, "prepare_parameters: fewer than three rounds");
+    max_delegation_size() = delegation_size()[0];
+    for (int d : delegation_size())
+        if (max_delegation_size() <= d) max_delegation_size() = d;
+}
+
+// PCS_commit(C, Codeword, Commit_data, Shares, Random_Shares, rate, N),
+// src/coPCS.cpp:236-250, one rank: prepare_data's rows (_queries()[0]
+// random values closing each, one random-only row) into Commit_data, their
+// encodings at rate[0][0] (with rate_offset()) into Codeword, distributed_MT
+// into C; Shares and Random_Shares are cleared as the reference clears them.
+template <class F, class MTT>
+void PCS_commit(MTT &C, std::vector<std::vector<F>> &Codeword, std::vector<std::vector<F>> &Commit_data,
+                std::vector<F> &Shares, std::vector<F> &Random_Shares, std::vector<std::vector<int>> rate, int N) {
+    detail::elem_type<F>();
+    if (_queries().empty() || rate.empty() || rate[0].empty())
+        throw error(COSNARK_E_ARG, "PCS_commit: call prepare_parameters first");
+    const uint64_t n = Shares.size(), nr = Random_Shares.size(), B = n / (BATCH_FACTOR / 2);
+    const int r = rate[0][0];
+    detail::Dev s(Shares.data(), 16 * n), rn(Random_Shares.data(), 16 * (nr ? nr : 1));
+    detail::Dev rows(16 * 32 * B), cws(16 * 32 * B * r), nodes(32 * (2 * B * r - 1));
+    int c = 0;
+    detail::check(cosnark_pcs_commit(detail::ctx(), s.u64(), n, rn.u64(), nr, _queries()[0], r,
+                                     detail::ilog2(rate_offset()), rows.u64(), cws.u64(), nodes.u8(), &c),
+                  "PCS_commit");
+    Commit_data.resize(BATCH_FACTOR);
+    Codeword.assign(BATCH_FACTOR, {});
+    for (int i = 0; i < c; ++i) {
+        Commit_data[i].resize(B);
+        detail::check(cosnark_d2h(detail::ctx(), Commit_data[i].data(), rows.u8() + 16 * B * i, 16 * B), "copy out");
+        Codeword[i].resize(B * r);
+        detail::check(cosnark_d2h(detail::ctx(), Codeword[i].data(), cws.u8() + 16 * B * r * i, 16 * B * r), "copy out");
+    }
+    std::vector<uint8_t> h(32 * (2 * B * r - 1));
+    nodes.get(h.data(), h.size());
+    C.Base_MT.clear();
+    detail::unpack_levels(h, B * r, C.Base_MT);
+    if (N == 1) C.Root_MT.assign(1, {C.Base_MT.back()[0]});
+    Shares.clear();
+    Random_Shares.clear();
+}
+
 // distributed_MT(Codeword, Com, N), src/coPCS.cpp:152-230, one rank: Base_MT
 // is the tree over the column roots; at N = 1 Root_MT is that one root.
 // MTT is the reference's MT struct (Base_MT, Root_MT of vector<vector<_hash>>).
@@ -817,6 +929,116 @@ void cubic_coSumcheck_phase2(std::vector<F> &p1, std::vector<F> &p2, std::vector
     b.get(p2.data(), 16 * (M / d));
     c.get(p3.data(), 16 * (M / d));
     e.get(b1.data(), 16 * (M / d));
+}
+
+namespace detail {
+template <class F>
+void split_ip_vec(const std::vector<F> &flat, const std::vector<int> &degrees, int mult) {
+    auto &ip = ip_vec<F>();
+    ip.assign(degrees.size(), {});
+    size_t off = 0;
+    for (size_t i = 0; i < degrees.size(); ++i) {
+        ip[i].assign(flat.begin() + off, flat.begin() + off + (size_t)mult * degrees[i]);
+        off += (size_t)mult * degrees[i];
+    }
+}
+template <class F>
+std::vector<const uint64_t *> mask_ptrs(const std::vector<std::vector<F>> &R, std::vector<uint64_t> &sizes) {
+    std::vector<const uint64_t *> p;
+    for (auto &x : R) {
+        p.push_back(reinterpret_cast<const uint64_t *>(x.data()));
+        sizes.push_back(x.size());
+    }
+    return p;
+}
+}  // namespace detail
+
+// quadratic_coSumcheck(p1, p2, R, K, K_r, degrees, N, ps), src/coSumcheck.cpp:
+// 233-331, as comm().rank of N (N >= 4) exchanging through comm().gather:
+// returns the five claims with their common point; p1 and p2 are folded
+// and resized as the reference leaves them, ip_vec<F>() is set, rank 0
+// adds the proof size to ps.  The reference's printed check failures are
+// not errors there; here they are counted in last_check_failures().
+inline int &last_check_failures() { thread_local int v = 0; return v; }
+template <class F>
+std::vector<std::pair<F, std::vector<F>>> quadratic_coSumcheck(std::vector<F> &p1, std::vector<F> &p2,
+                                                               std::vector<std::vector<F>> R, F K, F K_r,
+                                                               std::vector<int> degrees, int N, double &ps) {
+    detail::elem_type<F>();
+    const uint64_t M = p1.size();
+    if (p2.size() != M || !comm().gather) throw error(COSNARK_E_ARG, "quadratic_coSumcheck: sizes or no comm().gather");
+    detail::Dev a(p1.data(), 16 * M), b(p2.data(), 16 * M);
+    std::vector<uint64_t> sizes;
+    auto ptrs = detail::mask_ptrs(R, sizes);
+    uint64_t k[2], kr[2];
+    std::memcpy(k, &K, 16);
+    std::memcpy(kr, &K_r, 16);
+    uint64_t total = 0;
+    for (int d : degrees) total += 2 * (uint64_t)d;
+    std::vector<F> claims(5), r(degrees.size() + 1), ip(total);
+    int fails = 0;
+    detail::check(cosnark_quadratic_cosumcheck(detail::ctx(), a.u64(), b.u64(), M, ptrs.data(), sizes.data(),
+                                               (int)R.size(), k, kr, degrees.data(), (int)degrees.size(), N,
+                                               comm().rank, comm().gather, comm().user, &transcript(),
+                                               reinterpret_cast<uint64_t *>(claims.data()),
+                                               reinterpret_cast<uint64_t *>(r.data()),
+                                               reinterpret_cast<uint64_t *>(ip.data()), &ps, &fails),
+                  "quadratic_coSumcheck");
+    last_check_failures() = fails;
+    uint64_t Mc = M;
+    for (int d : degrees) Mc /= (uint64_t)d;
+    p1.resize(Mc);
+    p2.resize(Mc);
+    a.get(p1.data(), 16 * Mc);
+    b.get(p2.data(), 16 * Mc);
+    detail::split_ip_vec(ip, degrees, 2);
+    std::vector<std::pair<F, std::vector<F>>> out;
+    for (auto &c : claims) out.push_back({c, r});
+    return out;
+}
+
+// cubic_coSumcheck(p1, p2, p3, r1, r2, R, K, K_r, degrees, N, ps),
+// src/coSumcheck.cpp:335-448, as comm().rank of N: the four claims with
+// their common point; p1, p2, p3 folded and resized; ip_vec<F>() set.
+template <class F>
+std::vector<std::pair<F, std::vector<F>>> cubic_coSumcheck(std::vector<F> &p1, std::vector<F> &p2, std::vector<F> &p3,
+                                                           std::vector<F> &r1, std::vector<F> &r2,
+                                                           std::vector<std::vector<F>> R, F K, F K_r,
+                                                           std::vector<int> degrees, int N, double &ps) {
+    detail::elem_type<F>();
+    const uint64_t M = p1.size();
+    if (p2.size() != M || p3.size() != M || !comm().gather)
+        throw error(COSNARK_E_ARG, "cubic_coSumcheck: sizes or no comm().gather");
+    detail::Dev a(p1.data(), 16 * M), b(p2.data(), 16 * M), c(p3.data(), 16 * M);
+    std::vector<uint64_t> sizes;
+    auto ptrs = detail::mask_ptrs(R, sizes);
+    uint64_t k[2], kr[2];
+    std::memcpy(k, &K, 16);
+    std::memcpy(kr, &K_r, 16);
+    uint64_t total = 0;
+    for (int d : degrees) total += 4 * (uint64_t)d;
+    std::vector<F> claims(4), r(degrees.size() + 1), ip(total);
+    int fails = 0;
+    detail::check(cosnark_cubic_cosumcheck(detail::ctx(), a.u64(), b.u64(), c.u64(), M,
+                                           reinterpret_cast<const uint64_t *>(r1.data()), (int)r1.size(),
+                                           reinterpret_cast<const uint64_t *>(r2.data()), (int)r2.size(), ptrs.data(),
+                                           sizes.data(), (int)R.size(), k, kr, degrees.data(), (int)degrees.size(), N,
+                                           comm().rank, comm().gather, comm().user, &transcript(),
+                                           reinterpret_cast<uint64_t *>(claims.data()),
+                                           reinterpret_cast<uint64_t *>(r.data()),
+                                           reinterpret_cast<uint64_t *>(ip.data()), &ps, &fails),
+                  "cubic_coSumcheck");
+    last_check_failures() = fails;
+    uint64_t Mc = M;
+    for (int d : degrees) Mc /= (uint64_t)d;
+    for (auto *v : {&p1, &p2, &p3}) v->resize(Mc);
+    a.get(p1.data(), 16 * Mc);
+    b.get(p2.data(), 16 * Mc);
+    c.get(p3.data(), 16 * Mc);
+    detail::split_ip_vec(ip, degrees, 4);
+    std::vector<std::pair<F, std::vector<F>>> out;
+    for (auto &x : claims) out.push_back({x, r});
+    return out;
 }
 
 // unpack_secret_share(secret, _k, k), src/MPI_utils.cpp:2769-2785 (host):

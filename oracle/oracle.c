@@ -1545,3 +1545,46 @@ int or_cubic_cosumcheck(int N, or_fe **p1, or_fe **p2, or_fe **p3, uint64_t M, c
     free(LV); free(b1); free(b2); free(L); free(mine); free(Hf); free(fin_r); free(lens);
     return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* PCS_commit at one rank: src/coPCS.cpp:76-128, 236-250               */
+/* ------------------------------------------------------------------ */
+
+/* prepare_data(Commit_data, Shares, Random) with BATCH_FACTOR = 32
+ * (src/New_PCS.cpp:9) and _queries[0] = q0: rows (32 x B, B = n/16) are
+ * filled as the reference fills them; returns the non-empty row count, or
+ * -1 where the reference exits or reads past its inputs */
+int or_prepare_data(const or_fe *shares, uint64_t n, const or_fe *rnd, uint64_t nr, int q0, or_fe *rows) {
+    const uint64_t B = n / 16, q = (uint64_t)q0;
+    if (B <= q) return -1;
+    memset(rows, 0, sizeof(or_fe) * 32 * B);
+    uint64_t data_ctr = 0, rand_ctr = 0;
+    int last_batch = 0, done = 0;
+    for (int i = 0; i < 32; ++i) {                         /* :84-104 */
+        for (uint64_t k = 0; k < B - q; ++k) {
+            rows[i * B + k] = shares[data_ctr++];
+            if (data_ctr == n) break;
+        }
+        for (uint64_t k = B - q; k < B; ++k) {
+            if (rand_ctr >= nr) return -1;
+            rows[i * B + k] = rnd[rand_ctr++];
+        }
+        if (data_ctr == n) { last_batch = i; done = 1; break; }
+    }
+    if (!done) return -1;                                  /* :105-108 */
+    last_batch++;                                          /* :109 */
+    if (last_batch >= 32) return -1;
+    if (n - rand_ctr > B || rand_ctr > n) {                /* :112 (unsigned) */
+        for (uint64_t i = 0; i < B; ++i) {
+            if (rand_ctr >= nr) return -1;
+            rows[last_batch * B + i] = rnd[rand_ctr++];
+            if (rand_ctr == nr) break;
+        }
+    } else {
+        for (uint64_t i = 0; i < B - (n - rand_ctr); ++i) {
+            if (rand_ctr >= nr) return -1;
+            rows[last_batch * B + i] = rnd[rand_ctr++];
+        }
+    }
+    return last_batch + 1;
+}

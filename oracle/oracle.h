@@ -203,6 +203,11 @@ int or_cubic_cosumcheck(int N, or_fe **p1, or_fe **p2, or_fe **p3, uint64_t M, c
                         const int *degrees, int ndeg, or_fe *prev, or_fe *claims, or_fe *r_out, or_fe *ip_vec,
                         int *failures);
 
+/* prepare_data (src/coPCS.cpp:76-128), BATCH_FACTOR 32, _queries[0] = q0:
+ * rows (32 x n/16) as the reference fills them; the non-empty row count or
+ * -1 where it exits or reads past its inputs */
+int or_prepare_data(const or_fe *shares, uint64_t n, const or_fe *rnd, uint64_t nr, int q0, or_fe *rows);
+
 /* ---- all-cores variants (oracle_mt.c, OpenMP): the same bytes as the
  * single-thread functions above, `threads` OpenMP threads ---- */
 void or_fft_mt(or_fe *arr, int logn, int inverse, int threads);            /* utils.cpp:904-964, 1583-1673 */

@@ -123,6 +123,8 @@ def oracle():
         L.or_whir_set_threads.argtypes = [i32]
         L.or_whir_open_weights.argtypes = [vp, u64, vp, _Fe, vp, u64, vp, u64, i32, i32, vp, u64, vp, vp]
         L.or_whir_open_weights.restype = ctypes.c_int64
+        L.or_prepare_data.argtypes = [vp, u64, vp, u64, i32, vp]
+        L.or_prepare_data.restype = i32
         L.or_quadratic_cosumcheck.argtypes = [i32, vp, vp, u64, vp, vp, i32, _Fe, _Fe, vp, i32, vp, vp, vp, vp,
                                               ctypes.POINTER(i32)]
         L.or_quadratic_cosumcheck.restype = i32
@@ -648,3 +650,22 @@ def cubic_cosumcheck(p1, p2, p3, r1, r2, R, K, K_r, degrees, prev):
                                       len(degrees), _p(st), _p(claims), _p(r), _p(ip), ctypes.byref(fails))
     prev[...] = st
     return rc, claims, r, ip, fails.value
+
+
+def prepare_data(shares, rnd, q0):
+    """prepare_data: (row count or -1, rows (32, n/16, 2))."""
+    a = np.ascontiguousarray(shares, dtype=np.uint64)
+    r = np.ascontiguousarray(rnd, dtype=np.uint64)
+    n = a.shape[0]
+    rows = np.zeros((32, n // 16, 2), np.uint64)
+    c = oracle().or_prepare_data(_p(a), n, _p(r), r.shape[0], q0, _p(rows))
+    return int(c), rows
+
+
+def pcs_commit(shares, rnd, q0, rate, log_rate_offset):
+    """PCS_commit at one rank: (row count, rows, codewords (rows, rate B, 2),
+    Base_MT nodes)."""
+    c, rows = prepare_data(shares, rnd, q0)
+    assert c > 0
+    cw = copcs_encode(rows[:c], rate, log_rate_offset)
+    return c, rows[:c], cw, distributed_mt(cw)

@@ -101,6 +101,8 @@ def lib():
         L.cosnark_cubic_cosumcheck.argtypes = [vp, vp, vp, vp, u64, vp, i32, vp, i32, vp, vp, i32, vp, vp, vp, i32,
                                                i32, i32, ALLGATHER_FN, vp, vp, vp, vp, vp,
                                                ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i32)]
+        L.cosnark_pcs_commit.argtypes = [vp, vp, u64, vp, u64, i32, i32, i32, vp, vp, vp, ctypes.POINTER(i32)]
+        L.cosnark_pcs_prepare_rows.argtypes = [u64, u64, i32, ctypes.POINTER(i32), ctypes.POINTER(u64)]
         L.cosnark_field_random.argtypes = [vp]
         L.cosnark_field_random.restype = None
         L.cosnark_blake3_64.argtypes = [vp, vp]
@@ -470,6 +472,23 @@ class Context:
                                              None if nodes_dev is None else nodes_dev.ptr, zero_leaves, rate, whir_k,
                                              _p(rec), cap, ctypes.byref(words), _p(chal), _p(fin))
         return rc, rec[:words.value].copy(), chal[:m], fin
+
+    def pcs_commit(self, d_shares, n, d_random, nr, q0, rate, log_rate_offset, download=True):
+        """PCS_commit at one rank: (rows count, rows, codewords, Base_MT
+        nodes) as host arrays (download=False: the device buffers)."""
+        B = n // 16
+        rows = self.alloc(16 * 32 * B)
+        cws = self.alloc(16 * 32 * B * rate)
+        nodes = self.alloc(32 * (2 * B * rate - 1))
+        c = ctypes.c_int(0)
+        self._ck(lib().cosnark_pcs_commit(self.h, d_shares.ptr, n, d_random.ptr, nr, q0, rate, log_rate_offset,
+                                          rows.ptr, cws.ptr, nodes.ptr, ctypes.byref(c)))
+        if not download:
+            return c.value, rows, cws, nodes
+        out = (c.value, self.download(rows, (c.value, B, 2)), self.download(cws, (c.value, B * rate, 2)),
+               self.download(nodes, (2 * B * rate - 1, 32), np.uint8))
+        rows.free(); cws.free(); nodes.free()
+        return out
 
     def _masks(self, R):
         RR = [np.ascontiguousarray(x, dtype=np.uint64) for x in R]

@@ -1634,3 +1634,92 @@ int cosnark_cubic_cosumcheck(cosnark_ctx *ctx, uint64_t *d_p1, uint64_t *d_p2, u
 }
 
 }  // extern "C"
+
+// ---- PCS_commit at one rank (src/coPCS.cpp:236-250) ------------------------
+
+namespace {
+
+constexpr int PCS_BATCH_FACTOR = 32;     // BATCH_FACTOR, src/New_PCS.cpp:9
+
+// prepare_data's layout (src/coPCS.cpp:76-128) as copies: rows of B =
+// n / (BATCH_FACTOR / 2) elements; each holds B - q0 shares then q0 random
+// values until the shares run out (the row where they do is zero-filled
+// after its last share), and the next row takes random values only
+struct Seg {
+    int row;
+    uint64_t col, src, len;
+    bool random;
+};
+
+int pcs_prepare_plan(uint64_t n, uint64_t nr, uint64_t q0, std::vector<Seg> &segs, int *nrows, uint64_t *B_out) {
+    const uint64_t B = n / (PCS_BATCH_FACTOR / 2);
+    if (B <= q0) return COSNARK_E_ARG;
+    uint64_t data = 0, rnd = 0;
+    int last = -1;
+    for (int i = 0; i < PCS_BATCH_FACTOR; ++i) {
+        const uint64_t take = std::min<uint64_t>(B - q0, n - data);
+        segs.push_back({i, 0, data, take, false});
+        data += take;
+        segs.push_back({i, B - q0, rnd, q0, true});
+        rnd += q0;
+        if (data == n) { last = i; break; }
+    }
+    if (last < 0) return COSNARK_E_RANGE;                   // more batches needed (the reference exits, :106-109)
+    ++last;                                                 // :110
+    if (last >= PCS_BATCH_FACTOR) return COSNARK_E_RANGE;   // no row left for the random row
+    // :113-126; the reference compares Shares.size() - rand_ctr as an
+    // unsigned value, so more random values used than shares is case 1
+    uint64_t cnt;
+    if (rnd > n || n - rnd > B) {
+        if (nr <= rnd) return COSNARK_E_ARG;                // it would read past Random
+        cnt = std::min<uint64_t>(B, nr - rnd);              // stops when Random runs out
+    } else {
+        cnt = B - (n - rnd);
+    }
+    segs.push_back({last, 0, rnd, cnt, true});
+    if (rnd + cnt > nr) return COSNARK_E_ARG;               // the reference reads past Random there
+    *nrows = last + 1;
+    *B_out = B;
+    return COSNARK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cosnark_pcs_prepare_rows(uint64_t n_shares, uint64_t n_random, int q0, int *nrows, uint64_t *batch_size) {
+    if (!nrows || !batch_size || q0 < 0 || !is_pow2(n_shares) || n_shares < PCS_BATCH_FACTOR / 2) return COSNARK_E_ARG;
+    std::vector<Seg> segs;
+    return pcs_prepare_plan(n_shares, n_random, (uint64_t)q0, segs, nrows, batch_size);
+}
+
+int cosnark_pcs_commit(cosnark_ctx *ctx, const uint64_t *d_shares, uint64_t n_shares, const uint64_t *d_random,
+                       uint64_t n_random, int q0, int rate, int log_rate_offset, uint64_t *d_rows,
+                       uint64_t *d_codewords, uint8_t *d_nodes, int *nrows) {
+    if (!ctx || !d_shares || !d_random || !d_rows || !d_codewords || !d_nodes || !nrows)
+        return set_err(ctx, COSNARK_E_ARG, "pcs_commit: null argument");
+    if (!is_pow2(n_shares) || n_shares < PCS_BATCH_FACTOR / 2 || q0 < 0 || rate < 1 || !is_pow2((uint64_t)rate))
+        return set_err(ctx, COSNARK_E_ARG, "pcs_commit: shares a power of two >= 16, rate a power of two");
+    std::vector<Seg> segs;
+    uint64_t B = 0;
+    int rows = 0;
+    int rc = pcs_prepare_plan(n_shares, n_random, (uint64_t)q0, segs, &rows, &B);
+    if (rc == COSNARK_E_RANGE) return set_err(ctx, rc, "pcs_commit: the shares do not fit BATCH_FACTOR rows");
+    if (rc) return set_err(ctx, rc, "pcs_commit: q0 >= the row size, or too few random values");
+    if (B * (uint64_t)rate > (1ull << NTT_LOG_MAX)) return set_err(ctx, COSNARK_E_RANGE, "pcs_commit: codeword too long");
+    hipStream_t s = ctx->stream;
+    // prepare_data: rows zero-filled, then the share and random segments
+    HIPCK(hipMemsetAsync(d_rows, 0, sizeof(fe) * B * rows, s));
+    for (const Seg &g : segs) {
+        if (!g.len) continue;
+        const uint64_t *src = (g.random ? d_random : d_shares) + 2 * g.src;
+        HIPCK(hipMemcpyAsync(d_rows + 2 * (g.row * B + g.col), src, sizeof(fe) * g.len, hipMemcpyDeviceToDevice, s));
+    }
+    // Encode (:131-147), then distributed_MT at one rank (:152-183)
+    if ((rc = cosnark_copcs_encode(ctx, d_rows, rows, B, rate, log_rate_offset, d_codewords))) return rc;
+    if ((rc = cosnark_distributed_mt(ctx, d_codewords, rows, B * (uint64_t)rate, d_nodes))) return rc;
+    *nrows = rows;
+    return COSNARK_OK;
+}
+
+}  // extern "C"

@@ -7,6 +7,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <algorithm>
+#include <condition_variable>
+#include <mutex>
+#include <thread>
 #include <vector>
 #include "cosnark_dropin.hpp"
 #include "../../oracle/oracle.h"
@@ -33,6 +36,32 @@ static vector<F> synth(uint64_t seed, uint64_t stream, uint64_t n) {
 }
 static bool same(const void *a, const void *b, size_t bytes) { return std::memcmp(a, b, bytes) == 0; }
 static or_fe O(const F &x) { return or_fe{x.real, x.img}; }
+
+// an all-gather among the threads of this process acting as ranks
+struct ThreadGather {
+    int n;
+    std::mutex mu;
+    std::condition_variable cv;
+    int arrived = 0;
+    long gen = 0;
+    std::vector<std::vector<uint64_t>> parts;
+    explicit ThreadGather(int n_) : n(n_), parts(n_) {}
+    void barrier() {
+        std::unique_lock<std::mutex> lk(mu);
+        const long g = gen;
+        if (++arrived == n) { arrived = 0; ++gen; cv.notify_all(); }
+        else cv.wait(lk, [&] { return gen != g; });
+    }
+    static int fn(void *user, const uint64_t *mine, uint64_t elems, uint64_t *all) {
+        auto *T = static_cast<std::pair<ThreadGather *, int> *>(user);
+        ThreadGather &G = *T->first;
+        G.parts[T->second].assign(mine, mine + 2 * elems);
+        G.barrier();
+        for (int m = 0; m < G.n; ++m) std::memcpy(all + 2 * elems * m, G.parts[m].data(), 16 * elems);
+        G.barrier();
+        return 0;
+    }
+};
 
 int main() {
     const uint64_t seed = 0xC05A0040;
@@ -547,6 +576,166 @@ int main() {
                   same(arr.data(), oarr.data(), 16 * 256) && same(b_aggr.data(), ob2.data(), 16 * 256) &&
                   K.real == oK.real && K.img == oK.img,
               "aggregate_claims 3 arrays x 2 points");
+    }
+    // prepare_parameters with the reference's test rates (src/main.cpp:54)
+    {
+        D::rate_offset() = 1;                  // the reference's initial value (src/coPCS.cpp:22)
+        D::prepare_parameters({{4}, {4}, {4, 4}, {4, 4}}, 4, 4, 1 << 20);
+        const vector<int> wd = {32, 8, 64, 128}, wq = {333, 333, 433, 433}, wdel = {32768, 65536};
+        CHECK(D::_degrees() == wd && D::_queries() == wq && D::rate_offset() == 64 && D::delegation_size() == wdel &&
+                  D::randomness() == 96456 && D::max_delegation_size() == 65536,
+              "prepare_parameters degrees, queries, rate_offset, delegation sizes");
+    }
+    // PCS_commit at one rank: prepare_data + Encode + distributed_MT
+    {
+        const uint64_t n = 1 << 14, B = n / 16;
+        vector<F> shares = synth(seed, 150, n), rnd = synth(seed, 151, 9016);
+        vector<F> sh0 = shares, rn0 = rnd;
+        MT C;
+        vector<vector<F>> cw, cd;
+        D::PCS_commit(C, cw, cd, shares, rnd, {{4}, {4}, {4, 4}, {4, 4}}, 1);
+        vector<F> orows(32 * B);
+        const int c = or_prepare_data(reinterpret_cast<or_fe *>(sh0.data()), n, reinterpret_cast<or_fe *>(rn0.data()),
+                                      rn0.size(), 333, reinterpret_cast<or_fe *>(orows.data()));
+        vector<F> ocw((size_t)c * 4 * B);
+        or_copcs_encode(reinterpret_cast<or_fe *>(orows.data()), c, B, 4, 6, reinterpret_cast<or_fe *>(ocw.data()));
+        vector<uint8_t> base(32 * (2 * 4 * B - 1));
+        or_distributed_mt(reinterpret_cast<or_fe *>(ocw.data()), c, 4 * B, base.data());
+        bool ok = c == 25 && shares.empty() && rnd.empty() && cd.size() == 32 && cd[c].empty() && cw[c].empty();
+        for (int i = 0; ok && i < c; ++i)
+            ok = same(cd[i].data(), &orows[i * B], 16 * B) && same(cw[i].data(), &ocw[(size_t)i * 4 * B], 64 * B);
+        uint64_t off = 0;
+        for (size_t l = 0; ok && l < C.Base_MT.size(); ++l) {
+            ok = same(C.Base_MT[l].data(), base.data() + 32 * off, 32 * C.Base_MT[l].size());
+            off += C.Base_MT[l].size();
+        }
+        CHECK(ok && off == 2 * 4 * B - 1, "PCS_commit 2^14 shares: rows, codewords, Base_MT");
+    }
+    // quadratic / cubic coSumcheck over 4 ranks (threads), against the oracle
+    {
+        const int N = 4;
+        const uint64_t M = 1 << 10;
+        const std::vector<int> degrees = {16, 8, 4};
+        vector<F> s1 = synth(seed, 160, M), s2 = synth(seed, 161, M), s3 = synth(seed, 162, M);
+        vector<F> pads = synth(seed, 163, 3 * M);
+        vector<vector<F>> p1(N, vector<F>(M)), p2(N, vector<F>(M)), p3(N, vector<F>(M));
+        vector<F> sh(N * M);
+        auto share = [&](const vector<F> &sec, const F *pd, vector<vector<F>> &out) {
+            or_compute_secret_shares(reinterpret_cast<const or_fe *>(sec.data()), M, N, 1, 2,
+                                     reinterpret_cast<const or_fe *>(pd), reinterpret_cast<or_fe *>(sh.data()));
+            for (int m = 0; m < N; ++m) std::copy(sh.begin() + m * M, sh.begin() + (m + 1) * M, out[m].begin());
+        };
+        share(s1, pads.data(), p1);
+        share(s2, pads.data() + M, p2);
+        share(s3, pads.data() + 2 * M, p3);
+        or_fe K{0, 0};
+        for (uint64_t i = 0; i < M; ++i) K = or_add(K, or_mul(O(s1[i]), O(s2[i])));
+        vector<vector<vector<F>>> R(N);
+        for (int m = 0; m < N; ++m)
+            for (int d : degrees) R[m].push_back(vector<F>(2 * d, F{0, 0}));
+        // the oracle over all ranks
+        vector<vector<F>> q1 = p1, q2 = p2;
+        vector<or_fe *> a1, a2;
+        vector<vector<or_fe *>> rr(N);
+        vector<or_fe **> rp;
+        vector<vector<vector<F>>> Rc = R;
+        for (int m = 0; m < N; ++m) {
+            a1.push_back(reinterpret_cast<or_fe *>(q1[m].data()));
+            a2.push_back(reinterpret_cast<or_fe *>(q2[m].data()));
+            for (auto &x : Rc[m]) rr[m].push_back(reinterpret_cast<or_fe *>(x.data()));
+        }
+        for (int m = 0; m < N; ++m) rp.push_back(rr[m].data());
+        std::vector<uint64_t> rs = {32, 16, 8};
+        vector<F> oclaims(5), or_(4), oip(56);
+        or_fe prev{0, 0};
+        int ofails = -1;
+        const int orc = or_quadratic_cosumcheck(N, a1.data(), a2.data(), M, rp.data(), rs.data(), 3, K, or_fe{0, 0},
+                                                degrees.data(), 3, &prev, reinterpret_cast<or_fe *>(oclaims.data()),
+                                                reinterpret_cast<or_fe *>(or_.data()),
+                                                reinterpret_cast<or_fe *>(oip.data()), &ofails);
+        ThreadGather G(N);
+        std::vector<std::pair<ThreadGather *, int>> who;
+        for (int m = 0; m < N; ++m) who.push_back({&G, m});
+        std::vector<std::vector<std::pair<F, vector<F>>>> got(N);
+        std::vector<int> fails(N, -1);
+        std::vector<double> ps(N, 0);
+        std::vector<bool> ipok(N, false);
+        std::vector<std::thread> th;
+        for (int m = 0; m < N; ++m)
+            th.emplace_back([&, m] {
+                D::comm() = D::Comm{m, &ThreadGather::fn, &who[m]};
+                D::transcript() = cosnark_transcript{{0, 0}};
+                vector<F> x1 = p1[m], x2 = p2[m];
+                got[m] = D::quadratic_coSumcheck(x1, x2, R[m], F{K.real, K.img}, F{0, 0}, degrees, N, ps[m]);
+                fails[m] = D::last_check_failures();
+                auto &ip = D::ip_vec<F>();
+                ipok[m] = ip.size() == 3 && ip[0].size() == 32 && same(ip[0].data(), oip.data(), 16 * 32) &&
+                          same(ip[2].data(), &oip[48], 16 * 8) && x1.size() == 2;
+            });
+        for (auto &t : th) t.join();
+        bool ok = orc == 0 && ofails == 0;
+        for (int m = 0; ok && m < N; ++m) {
+            ok = fails[m] == 0 && ipok[m] && got[m].size() == 5 && (ps[m] > 0) == (m == 0);
+            for (int i = 0; ok && i < 5; ++i)
+                ok = same(&got[m][i].first, &oclaims[i], 16) && got[m][i].second.size() == 4 &&
+                     same(got[m][i].second.data(), or_.data(), 16 * 4);
+        }
+        CHECK(ok, "quadratic_coSumcheck over 4 ranks (threads) = the oracle, checks pass");
+        // cubic: b1 = eq(r1), b2 = eq(r2) (one slot at N = 4)
+        vector<F> r1 = synth(seed, 170, 10), r2 = synth(seed, 171, 1);
+        vector<F> b1(M);
+        or_precompute_beta(reinterpret_cast<or_fe *>(r1.data()), 10, reinterpret_cast<or_fe *>(b1.data()));
+        or_fe b20 = or_sub(or_fe{1, 0}, O(r2[0]));
+        or_fe K3{0, 0};
+        for (uint64_t i = 0; i < M; ++i)
+            K3 = or_add(K3, or_mul(or_mul(or_sub(or_mul(O(s1[i]), O(s2[i])), O(s3[i])), O(b1[i])), b20));
+        vector<vector<vector<F>>> R3(N);
+        for (int m = 0; m < N; ++m)
+            for (int d : {16, 16, 4}) R3[m].push_back(vector<F>(4 * d, F{0, 0}));
+        const std::vector<int> deg3 = {16, 16, 4};
+        vector<vector<F>> c1 = p1, c2 = p2, c3 = p3;
+        vector<or_fe *> e1, e2, e3;
+        vector<vector<or_fe *>> rr3(N);
+        vector<or_fe **> rp3;
+        vector<vector<vector<F>>> Rc3 = R3;
+        for (int m = 0; m < N; ++m) {
+            e1.push_back(reinterpret_cast<or_fe *>(c1[m].data()));
+            e2.push_back(reinterpret_cast<or_fe *>(c2[m].data()));
+            e3.push_back(reinterpret_cast<or_fe *>(c3[m].data()));
+            for (auto &x : Rc3[m]) rr3[m].push_back(reinterpret_cast<or_fe *>(x.data()));
+        }
+        for (int m = 0; m < N; ++m) rp3.push_back(rr3[m].data());
+        std::vector<uint64_t> rs3 = {64, 64, 16};
+        vector<F> oc3(4), or3(4), oip3(144);
+        or_fe prev3{0, 0};
+        int of3 = -1;
+        const int orc3 = or_cubic_cosumcheck(N, e1.data(), e2.data(), e3.data(), M,
+                                             reinterpret_cast<or_fe *>(r1.data()), 10,
+                                             reinterpret_cast<or_fe *>(r2.data()), 1, rp3.data(), rs3.data(), 3, K3,
+                                             or_fe{0, 0}, deg3.data(), 3, &prev3,
+                                             reinterpret_cast<or_fe *>(oc3.data()),
+                                             reinterpret_cast<or_fe *>(or3.data()),
+                                             reinterpret_cast<or_fe *>(oip3.data()), &of3);
+        std::vector<std::vector<std::pair<F, vector<F>>>> got3(N);
+        std::vector<int> f3(N, -1);
+        std::vector<std::thread> th3;
+        for (int m = 0; m < N; ++m)
+            th3.emplace_back([&, m] {
+                D::comm() = D::Comm{m, &ThreadGather::fn, &who[m]};
+                D::transcript() = cosnark_transcript{{0, 0}};
+                vector<F> x1 = p1[m], x2 = p2[m], x3 = p3[m], rr1 = r1, rr2 = r2;
+                double pp = 0;
+                got3[m] = D::cubic_coSumcheck(x1, x2, x3, rr1, rr2, R3[m], F{K3.real, K3.img}, F{0, 0}, deg3, N, pp);
+                f3[m] = D::last_check_failures();
+            });
+        for (auto &t : th3) t.join();
+        ok = orc3 == 0 && of3 == 0;
+        for (int m = 0; ok && m < N; ++m) {
+            ok = f3[m] == 0 && got3[m].size() == 4;
+            for (int i = 0; ok && i < 4; ++i)
+                ok = same(&got3[m][i].first, &oc3[i], 16) && same(got3[m][i].second.data(), or3.data(), 16 * 4);
+        }
+        CHECK(ok, "cubic_coSumcheck over 4 ranks (threads) = the oracle, checks pass");
     }
     // RS_multifold5
     {
