@@ -249,6 +249,8 @@ def batched_leg(ctx, reps=10, warm=10):
     sizes = cosnark.cfg4_sizes()
     t0 = time.perf_counter()
     ctx.autotune(14, 22, batch=8)
+    mk_default = ctx.merkle_schedule()
+    mk_b, mk_ms = ctx.autotune_merkle(sizes)    # a codeword of 4n elements has n leaves
     tune_s = time.perf_counter() - t0
     b = ctx.batch(sizes, 4)
     dev = []
@@ -278,6 +280,7 @@ def batched_leg(ctx, reps=10, warm=10):
     bat_s = min(walls)
     same = proofs_b == proofs_seq
     b.close()
+    ctx.set_merkle_schedule(mk_default)      # the later legs keep the default schedule
     for f, g, _ in dev:
         f.free(); g.free()
     return {"workload": "64 proofs, n = 2^e with e uniform on 12..20 (seed 0xC05A0004), RS blowup 4",
@@ -288,14 +291,20 @@ def batched_leg(ctx, reps=10, warm=10):
             "sequential_plain_ms": round(seq_s * 1e3, 3),
             "speedup_vs_sequential": round(seq_s / bat_s, 2),
             "autotune_s": round(tune_s, 2),
+            "merkle_schedule": {"tuned_block_log": mk_b, "default_block_log": mk_default,
+                                "trees_ms": round(mk_ms, 4),
+                                "note": "2^b first-level nodes per workgroup, b levels per launch, fan-in 2"},
             "proofs_identical_to_sequential": bool(same)}
 
 
-def serving_leg(ctx, requests=1000, rate_hz=2000.0):
+def serving_leg(ctx, requests=1000, rate_hz=2000.0, sweep=(500.0, 1000.0, 2000.0, 3000.0, 4000.0, 6000.0, 8000.0, 12000.0)):
     """Config 5: Poisson arrivals at rate_hz of proof requests of 2^e
     elements, e uniform on 14..22 (seed 0xC05A0005); witnesses device-resident
     (one f, g pair per size); HBM budget 16 GiB, 3 slots.  Every served proof
-    is checked against the plain prover's proof of the same witness."""
+    is checked against the plain prover's proof of the same witness.  The
+    same stream is then offered at each rate of `sweep` (a fresh server per
+    rate, about a second of arrivals each) up to and past saturation, with
+    p50 / p99 latency per offered rate."""
     seed = 0xC05A0005
     pool, plain = {}, {}
     for e in range(14, 23):
@@ -303,29 +312,42 @@ def serving_leg(ctx, requests=1000, rate_hz=2000.0):
         f, g = cosnark.synth(seed, 2 * e, n), cosnark.synth(seed, 2 * e + 1, n)
         pool[e] = (ctx.upload(f), ctx.upload(g))
         plain[e] = ctx.prove_plain(None, None, 4, cosnark.Transcript(), dev_inputs=(pool[e][0], pool[e][1], n))[0].tobytes()
-    exps = [14 + int(v) % 9 for v in cosnark.synth(seed, 3, requests)[:, 0]]
-    gaps = np.random.default_rng(seed).exponential(1.0 / rate_hz, requests)
-    arrivals = np.cumsum(gaps)
-    srv = cosnark.Server(16 << 30, rate=4, max_log_n=22, slots=3)
-    for e in (14, 18, 22):      # warm the server's kernels and tables
-        srv.wait(srv.submit(pool[e][0], pool[e][1], 1 << e), 1 << e)
-    tickets = []
-    t0 = time.perf_counter()
-    for i, e in enumerate(exps):
-        while time.perf_counter() - t0 < arrivals[i]:
-            pass
-        tickets.append(srv.submit(pool[e][0], pool[e][1], 1 << e))
-    lat, ok = [], 0
-    for t, e in zip(tickets, exps):
-        proof, _, ms = srv.wait(t, 1 << e)
-        lat.append(ms)
-        ok += proof == plain[e]
-    wall = time.perf_counter() - t0
-    st = srv.stats()
-    srv.close()
+
+    def run(rate, count):
+        exps = [14 + int(v) % 9 for v in cosnark.synth(seed, 3, count)[:, 0]]
+        gaps = np.random.default_rng(seed).exponential(1.0 / rate, count)
+        arrivals = np.cumsum(gaps)
+        srv = cosnark.Server(16 << 30, rate=4, max_log_n=22, slots=3)
+        for e in (14, 18, 22):      # warm the server's kernels and tables
+            srv.wait(srv.submit(pool[e][0], pool[e][1], 1 << e), 1 << e)
+        tickets = []
+        t0 = time.perf_counter()
+        for i, e in enumerate(exps):
+            while time.perf_counter() - t0 < arrivals[i]:
+                pass
+            tickets.append(srv.submit(pool[e][0], pool[e][1], 1 << e))
+        lat, ok = [], 0
+        for t, e in zip(tickets, exps):
+            proof, _, ms = srv.wait(t, 1 << e)
+            lat.append(ms)
+            ok += proof == plain[e]
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+        srv.close()
+        lat = np.array(lat)
+        return exps, wall, lat, ok, st
+
+    exps, wall, lat, ok, st = run(rate_hz, requests)
+    curve = []
+    for r in sweep:
+        cnt = int(min(max(r * 1.0, 250), 4000))
+        _, w, l, k, _ = run(r, cnt)
+        curve.append({"offered_per_s": r, "requests": cnt, "throughput_proofs_per_s": round(cnt / w, 1),
+                      "p50_ms": round(float(np.percentile(l, 50)), 3), "p99_ms": round(float(np.percentile(l, 99)), 3),
+                      "proofs_identical_to_plain": f"{k}/{cnt}"})
     for a, b in pool.values():
         a.free(); b.free()
-    lat = np.array(lat)
+    sat = max(c["throughput_proofs_per_s"] for c in curve)
     return {"workload": f"{requests} requests, Poisson arrivals at {rate_hz:g}/s, n = 2^e with e uniform on 14..22 "
                         "(seed 0xC05A0005), RS blowup 4, HBM budget 16 GiB, 3 slots",
             "offered_per_s": rate_hz, "throughput_proofs_per_s": round(requests / wall, 1),
@@ -333,7 +355,8 @@ def serving_leg(ctx, requests=1000, rate_hz=2000.0):
             "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
             "mean_ms": round(float(lat.mean()), 3), "passes": st["passes"], "max_in_flight": st["max_inflight"],
             "arena_peak_gib": round(st["arena_peak_bytes"] / 2**30, 3),
-            "proofs_identical_to_plain": f"{ok}/{requests}"}
+            "proofs_identical_to_plain": f"{ok}/{requests}",
+            "saturation_proofs_per_s": sat, "sweep": curve}
 
 
 def main():

@@ -546,6 +546,18 @@ int cosnark_ntt_plan(cosnark_ctx *ctx, int logn, int *passbits, int *tile);
  * the encode beside an overlapped sumcheck.  passbits = tile = 0 restores
  * the context defaults for that size. */
 int cosnark_set_ntt_plan(cosnark_ctx *ctx, int logn, int passbits, int tile);
+/* The Merkle schedule (no reference counterpart): a workgroup takes 2^b
+ * first-level nodes of a tree and climbs b levels in LDS before the next
+ * launch (b in [6, 10]; 10 by default), so a tree of 2^k leaves takes about
+ * k / (b + 1) launches.  Fan-in stays 2 (the reference's binary dup-left
+ * tree) and every b builds the same tree.  cosnark_autotune_merkle times a
+ * batch of trees of the given leaf counts (built in shared launches, as the
+ * batched prover builds them) under each b and keeps the fastest in ctx;
+ * run it before creating batches. */
+int cosnark_autotune_merkle(cosnark_ctx *ctx, const uint64_t *leaf_counts, int count, int *best_block_log,
+                            double *best_ms);
+int cosnark_merkle_schedule(cosnark_ctx *ctx, int *block_log);
+int cosnark_set_merkle_schedule(cosnark_ctx *ctx, int block_log);
 /* Measured throughput of the library's GF(p^2) multiply on this device
  * (independent multiply chains, no memory traffic): the compute ceiling the
  * transform kernels are reported against.  No reference counterpart. */

@@ -122,6 +122,9 @@ def lib():
         L.cosnark_batch_prove.argtypes = [vp, vp, vp, ctypes.POINTER(ctypes.c_double)]
         L.cosnark_autotune.argtypes = [vp, i32, i32, i32, vp]
         L.cosnark_ntt_plan.argtypes = [vp, i32, ctypes.POINTER(i32), ctypes.POINTER(i32)]
+        L.cosnark_autotune_merkle.argtypes = [vp, vp, i32, ctypes.POINTER(i32), ctypes.POINTER(ctypes.c_double)]
+        L.cosnark_merkle_schedule.argtypes = [vp, ctypes.POINTER(i32)]
+        L.cosnark_set_merkle_schedule.argtypes = [vp, i32]
         L.cosnark_fmul_rate.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
         L.cosnark_blake3_rate.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
         L.cosnark_server_create.argtypes = [i32, u64, i32, i32, i32, ctypes.POINTER(vp)]
@@ -661,6 +664,23 @@ class Context:
 
     def set_ntt_plan(self, logn, passbits, tile):
         self._ck(lib().cosnark_set_ntt_plan(self.h, logn, passbits, tile))
+
+    def autotune_merkle(self, leaf_counts):
+        """Time the Merkle schedules (2^b first-level nodes per workgroup,
+        b in 6..10) on a batch of trees of these leaf counts and keep the
+        fastest; returns (b, ms)."""
+        arr = np.ascontiguousarray(leaf_counts, dtype=np.uint64)
+        b, ms = ctypes.c_int(0), ctypes.c_double(0)
+        self._ck(lib().cosnark_autotune_merkle(self.h, _p(arr), len(arr), ctypes.byref(b), ctypes.byref(ms)))
+        return b.value, ms.value
+
+    def merkle_schedule(self):
+        b = ctypes.c_int(0)
+        self._ck(lib().cosnark_merkle_schedule(self.h, ctypes.byref(b)))
+        return b.value
+
+    def set_merkle_schedule(self, block_log):
+        self._ck(lib().cosnark_set_merkle_schedule(self.h, block_log))
 
     def batch(self, sizes, rate):
         return Batch(self, sizes, rate)

@@ -235,7 +235,7 @@ LevelOffsets level_offsets(uint64_t leaves) {
 // nodes, then one per up to ten further levels until the root.  Returns the
 // (job, workgroups) of each launch.
 std::vector<std::pair<MkJob, uint64_t>> tree_stages(int mode, const fe *elems, uint64_t leaves, uint8_t *nodes,
-                                                    bool dup_left, uint64_t col_len, int col_rows) {
+                                                    bool dup_left, uint64_t col_len, int col_rows, uint64_t cap) {
     std::vector<std::pair<MkJob, uint64_t>> st;
     MkJob J{};
     J.elems = elems;
@@ -245,7 +245,7 @@ std::vector<std::pair<MkJob, uint64_t>> tree_stages(int mode, const fe *elems, u
     J.col_rows = col_rows;
     J.dup_left = (int)dup_left;
     J.lvl0 = 0;
-    J.block_log = ilog2(leaves < (uint64_t)MK_BLOCK ? leaves : (uint64_t)MK_BLOCK);
+    J.block_log = ilog2(leaves < cap ? leaves : cap);
     uint64_t blocks = leaves >> J.block_log;
     st.push_back({J, blocks});
     int top = J.block_log;                            // highest level written
@@ -256,7 +256,7 @@ std::vector<std::pair<MkJob, uint64_t>> tree_stages(int mode, const fe *elems, u
         U.elems = nullptr;
         U.in_nodes = nodes + 32 * J.lo.off[top];
         U.lvl0 = top + 1;
-        U.block_log = ilog2(next < (uint64_t)MK_BLOCK ? next : (uint64_t)MK_BLOCK);
+        U.block_log = ilog2(next < cap ? next : cap);
         const uint64_t nb = next >> U.block_log;
         st.push_back({U, nb});
         top += 1 + U.block_log;
@@ -267,7 +267,8 @@ std::vector<std::pair<MkJob, uint64_t>> tree_stages(int mode, const fe *elems, u
 
 // Levels have.. above levels 0 .. have-1 already in nodes (level 0: `leaves`
 // digests).
-std::vector<std::pair<MkJob, uint64_t>> upper_stages(uint64_t leaves, uint8_t *nodes, bool dup_left, int have) {
+std::vector<std::pair<MkJob, uint64_t>> upper_stages(uint64_t leaves, uint8_t *nodes, bool dup_left, int have,
+                                                     uint64_t cap) {
     std::vector<std::pair<MkJob, uint64_t>> st;
     MkJob J{};
     J.nodes = nodes;
@@ -280,7 +281,7 @@ std::vector<std::pair<MkJob, uint64_t>> upper_stages(uint64_t leaves, uint8_t *n
         MkJob U = J;
         U.in_nodes = nodes + 32 * J.lo.off[top];
         U.lvl0 = top + 1;
-        U.block_log = ilog2(next < (uint64_t)MK_BLOCK ? next : (uint64_t)MK_BLOCK);
+        U.block_log = ilog2(next < cap ? next : cap);
         const uint64_t nb = next >> U.block_log;
         st.push_back({U, nb});
         top += 1 + U.block_log;
@@ -289,9 +290,12 @@ std::vector<std::pair<MkJob, uint64_t>> upper_stages(uint64_t leaves, uint8_t *n
     return st;
 }
 
+// first-level nodes per workgroup: the context's Merkle schedule
+uint64_t block_cap(const cosnark_ctx *ctx) { return 1ull << ctx->mk_block_log; }
+
 int build_tree(cosnark_ctx *ctx, int mode, const fe *elems, uint64_t leaves, uint8_t *nodes, bool dup_left,
                uint64_t col_len, int col_rows, hipStream_t stream) {
-    const auto st = tree_stages(mode, elems, leaves, nodes, dup_left, col_len, col_rows);
+    const auto st = tree_stages(mode, elems, leaves, nodes, dup_left, col_len, col_rows, block_cap(ctx));
     for (size_t i = 0; i < st.size(); ++i) {
         auto k = i > 0 ? merkle_block_kernel<MODE_NODES>
                        : mode == MODE_ELEMS ? merkle_block_kernel<MODE_ELEMS> : merkle_block_kernel<MODE_COLUMNS>;
@@ -316,7 +320,7 @@ int merkle_from_leaves_launch(cosnark_ctx *ctx, uint64_t leaves, uint8_t *nodes,
         return set_err(ctx, COSNARK_E_ARG, "merkle: leaf count must be a power of two <= 2^30");
     if (have < 1 || (leaves >> (have - 1)) < 1)
         return set_err(ctx, COSNARK_E_ARG, "merkle: levels already built out of range");
-    for (const auto &st : upper_stages(leaves, nodes, dup_left, have)) {
+    for (const auto &st : upper_stages(leaves, nodes, dup_left, have, block_cap(ctx))) {
         hipLaunchKernelGGL(merkle_block_kernel<MODE_NODES>, dim3((unsigned)st.second), dim3(MK_THREADS), 0, stream,
                            st.first);
         LAUNCHCK();
@@ -342,7 +346,7 @@ int merkle_batch_plan(cosnark_ctx *ctx, int count, const fe *const *elems, const
     for (int i = 0; i < count; ++i) {
         if (!is_pow2(n_elems[i]) || n_elems[i] < 8 || n_elems[i] > (1ull << 30))
             return set_err(ctx, COSNARK_E_ARG, "merkle batch: n_elems must be powers of two in [8, 2^30]");
-        per[i] = tree_stages(MODE_ELEMS, elems[i], n_elems[i] / 4, nodes[i], dup_left, 0, 0);
+        per[i] = tree_stages(MODE_ELEMS, elems[i], n_elems[i] / 4, nodes[i], dup_left, 0, 0, block_cap(ctx));
         nst = std::max(nst, per[i].size());
     }
     plan.launches.clear();

@@ -1,4 +1,5 @@
-// tune.hip -- the NTT radix plan per transform size, picked by timing.
+// tune.hip -- the NTT radix plan per transform size and the Merkle schedule,
+// picked by timing.
 //
 // A 2^L-point transform runs as ceil(L / b) LDS passes of radix at most 2^b,
 // each workgroup staging a tile of T elements.  Every (b, T) computes the
@@ -203,6 +204,87 @@ int cosnark_autotune(cosnark_ctx *ctx, int logn_lo, int logn_hi, int batch, doub
     (void)hipEventDestroy(e1);
     cleanup();
     return rc;
+}
+
+// The Merkle schedule: how many first-level nodes one workgroup takes (2^b),
+// which sets how many levels it climbs in LDS before the next launch (b) and
+// how many launches a tree needs.  Fan-in stays 2: the reference's binary,
+// dup-left tree (src/merkle_tree.cpp:267-299).  Every b builds the same
+// tree; the tuner times a batch of trees of the given leaf counts, built in
+// shared launches as the batched prover builds them, under each b.
+int cosnark_autotune_merkle(cosnark_ctx *ctx, const uint64_t *leaf_counts, int count, int *best_block_log,
+                            double *best_ms) {
+    if (!ctx || !leaf_counts || count < 1 || count > 4096) return set_err(ctx, COSNARK_E_ARG, "autotune_merkle: arguments");
+    HIPCK(hipSetDevice(ctx->device));
+    std::vector<uint64_t> n_elems(count);
+    uint64_t total = 0, nodes_total = 0;
+    for (int i = 0; i < count; ++i) {
+        if (!is_pow2(leaf_counts[i]) || leaf_counts[i] < 2 || leaf_counts[i] > (1ull << 26))
+            return set_err(ctx, COSNARK_E_ARG, "autotune_merkle: leaf counts powers of two in [2, 2^26]");
+        n_elems[i] = 4 * leaf_counts[i];
+        total += n_elems[i];
+        nodes_total += cosnark_merkle_node_count(n_elems[i]);
+    }
+    fe *el = nullptr;
+    uint8_t *nd = nullptr, *tables = nullptr;
+    auto cleanup = [&]() {
+        for (void *p : {(void *)el, (void *)nd, (void *)tables})
+            if (p) (void)hipFree(p);
+    };
+    if (hipMalloc(&el, sizeof(fe) * total) != hipSuccess || hipMalloc(&nd, 32 * nodes_total) != hipSuccess) {
+        cleanup();
+        return set_err(ctx, COSNARK_E_NOMEM, "autotune_merkle: allocation failed");
+    }
+    if (hipMemsetAsync(el, 0, sizeof(fe) * total, ctx->stream) != hipSuccess) {
+        cleanup();
+        return set_err(ctx, COSNARK_E_HIP, "autotune_merkle: memset failed");
+    }
+    std::vector<const fe *> ep(count);
+    std::vector<uint8_t *> np(count);
+    uint64_t eo = 0, no = 0;
+    for (int i = 0; i < count; ++i) {
+        ep[i] = el + eo;
+        np[i] = nd + 32 * no;
+        eo += n_elems[i];
+        no += cosnark_merkle_node_count(n_elems[i]);
+    }
+    const int saved = ctx->mk_block_log;
+    float best = 1e30f;
+    int bb = saved, rc = COSNARK_OK;
+    for (int b = 6; b <= 10 && rc == COSNARK_OK; ++b) {
+        ctx->mk_block_log = b;
+        TableBuf tb;
+        MerkleBatchPlan plan;
+        if ((rc = merkle_batch_plan(ctx, count, ep.data(), n_elems.data(), np.data(), true, tb, plan))) break;
+        if (tables) (void)hipFree(tables);
+        tables = nullptr;
+        if (hipMalloc(&tables, tb.host.size()) != hipSuccess ||
+            hipMemcpy(tables, tb.host.data(), tb.host.size(), hipMemcpyHostToDevice) != hipSuccess) {
+            rc = set_err(ctx, COSNARK_E_NOMEM, "autotune_merkle: tables");
+            break;
+        }
+        float t = 0;
+        rc = time_best(ctx, [&]() { (void)merkle_batch_run(ctx, plan, tables, ctx->stream); }, &t);
+        if (rc == COSNARK_OK && t < best) { best = t; bb = b; }
+    }
+    ctx->mk_block_log = rc == COSNARK_OK ? bb : saved;
+    if (best_block_log) *best_block_log = ctx->mk_block_log;
+    if (best_ms) *best_ms = best;
+    (void)hipStreamSynchronize(ctx->stream);
+    cleanup();
+    return rc;
+}
+
+int cosnark_merkle_schedule(cosnark_ctx *ctx, int *block_log) {
+    if (!ctx || !block_log) return COSNARK_E_ARG;
+    *block_log = ctx->mk_block_log;
+    return COSNARK_OK;
+}
+
+int cosnark_set_merkle_schedule(cosnark_ctx *ctx, int block_log) {
+    if (!ctx || block_log < 6 || block_log > 10) return set_err(ctx, COSNARK_E_ARG, "set_merkle_schedule: 6 <= block_log <= 10");
+    ctx->mk_block_log = block_log;
+    return COSNARK_OK;
 }
 
 int cosnark_ntt_plan(cosnark_ctx *ctx, int logn, int *passbits, int *tile) {

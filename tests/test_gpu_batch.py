@@ -124,6 +124,22 @@ def test_autotuned_plans_give_the_same_proofs(ctx):
     tuned.close()
 
 
+@pytest.mark.parametrize("block_log", [7, 9])
+def test_merkle_schedule_gives_the_same_proofs(ctx, block_log):
+    """Batched proofs under a non-default Merkle schedule (shared launches
+    of all trees' levels) equal the plain prover's."""
+    sizes = [1 << 10, 1 << 12, 1 << 16, 1 << 18]
+    c = cosnark.Context(0)
+    c.set_merkle_schedule(block_log)
+    b = c.batch(sizes, 4)
+    ins = _fill(b, 0xC05A0030 + block_log)
+    proofs, _, _ = b.prove()
+    for i, (f, g) in enumerate(ins):
+        assert proofs[i] == _plain(ctx, f, g, 4)[0], f"proof {i} (n={sizes[i]}, b={block_log})"
+    b.close()
+    c.close()
+
+
 def test_batch_argument_errors(ctx):
     lib = cosnark.lib()
     import ctypes
