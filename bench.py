@@ -466,11 +466,19 @@ def main():
         nxt = cnt // 2
         cnt = nxt >> min(nxt.bit_length() - 1, 10)
         mk_launches += 1
-    sc_len, sc_launches, pending = n, 1, False
-    while sc_len > 4096:
-        sc_len = sc_len // 2 if pending else sc_len
-        pending = True
+    # sumcheck launches of the plain prover: rounds 0-1 in one launch, then
+    # two rounds per launch while the folded vector exceeds the tail size
+    # (1024 elements unless COSNARK_SC_TAIL says otherwise), then the tail
+    sc_tail = int(os.environ.get("COSNARK_SC_TAIL", "1024"))
+    sc_len, sc_launches = n, 1
+    if n // 2 > sc_tail:
         sc_launches += 1
+        while True:
+            quad = sc_len // 4 > sc_tail
+            sc_launches += 1
+            sc_len //= 4
+            if not quad:
+                break
     launches = {"rs_encode": -(-log_cw // passbits), "merkle": mk_launches, "sumcheck": sc_launches}
     names = ["rs_encode", "merkle", "sumcheck"]
     fmul_peak = ctx.fmul_rate()

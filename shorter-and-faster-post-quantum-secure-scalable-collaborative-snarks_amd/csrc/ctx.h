@@ -61,12 +61,13 @@ struct cosnark_ctx {
     struct { int passbits, tile; } ntt_plan[NTT_LOG_MAX + 1] = {};
     bool overlap = true;              // plain prover: sumcheck beside the commit
     void *noalloc_scratch = nullptr;  // this device's copy of the file-scope scratch array (capi.hip)
-    int sc_blocks = 256;              // workgroups of a multi-workgroup sumcheck round (<= SC_MAX_BLOCKS)
-    uint64_t sc_tail_elems = 4096;    // vectors this short finish in the one-workgroup tail (power of two, >= 4)
+    int sc_blocks = 512;              // workgroups of a multi-workgroup sumcheck round (<= SC_MAX_BLOCKS)
+    uint64_t sc_tail_elems = 1024;    // vectors this short finish in the one-workgroup tail (power of two, >= 4)
+    int sc_wide_blocks = 256;         // workgroups of the fused rounds 0-2 (<= SC_MAX_BLOCKS / 3)
 };
 
 constexpr int CO_HOST_ELEMS = 8192;     // pinned staging for block-sumcheck results and weights
-constexpr int SC_MAX_BLOCKS = 2048;    // partial slots per round: the most workgroups a sumcheck round may use
+constexpr int SC_MAX_BLOCKS = 4096;    // partial slots per round: the most workgroups a sumcheck round may use
 
 int set_err(cosnark_ctx *ctx, int code, const std::string &msg);
 int hip_err(cosnark_ctx *ctx, hipError_t e, const char *what);

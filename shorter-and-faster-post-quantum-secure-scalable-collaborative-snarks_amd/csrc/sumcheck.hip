@@ -8,19 +8,21 @@
 // y = H(r), fold v[j] = r (v[2j+1] - v[2j]) + v[2j] (:257-261).
 //
 // Here b is formed as sum o1 o2 - a - c (o = v[2j+1]), three products per
-// pair.  Round i's evaluation is fused with the fold by r[i-1]: one pass
-// reads four elements per vector, writes the two folded ones and sums the
-// round's three terms.  Each workgroup writes its partial sums; the next
-// round's kernel starts by reducing the previous round's partials in every
-// workgroup (a few KiB from L2) and running the transcript hashes
-// redundantly, on four lanes, so no atomics, no last-workgroup election and
-// no host round trip sit between rounds.  Transcript values are kept per
-// round (tr_prev[k], tr_y[k] = state before round k) so a kernel never reads
-// a slot it writes.  Once a vector is small (sc_tail_elems) one workgroup
-// runs all remaining rounds: its wave 0 runs the transcript while waves 1..3
-// fold by the challenge just drawn and form the next round's sums as
-// quadratics in the challenge still being hashed, so a round costs the hash
-// chain and little else.
+// pair.  While the vectors are long, rounds go two per launch: a round's
+// three sums and the next round's sums as quadratics in the challenge not
+// yet drawn (nine values) are formed in one pass, so a launch finalizes two
+// rounds and folds by both challenges (sc_pair01_kernel, sc_fold2_kernel).
+// Each workgroup writes its partial sums; the next launch starts by reducing
+// the previous launch's partials in wave 0 of every workgroup (a few KiB from
+// L2, summed with DPP exchanges) and running the transcript hashes
+// redundantly, on four lanes, while its other waves' loads are in flight, so
+// no atomics, no last-workgroup election and no host round trip sit between
+// launches.  Transcript values are kept per round (tr_prev[k], tr_y[k] =
+// state before round k) so a kernel never reads a slot it writes.  Once a
+// vector is small (sc_tail_elems) one workgroup runs all remaining rounds:
+// its wave 0 runs the transcript while waves 1..3 fold by the challenge just
+// drawn and form the next round's sums as quadratics in the challenge still
+// being hashed, so a round costs the hash chain and little else.
 #include <algorithm>
 #include <cstddef>
 #include <cstring>
@@ -42,17 +44,70 @@ __device__ __forceinline__ fe fold1(const fe &e, const fe &o, const fe &r) {
     return gf::add(gf::mul(r, gf::sub(o, e)), e);
 }
 
-__device__ __forceinline__ fe shfl_xor_fe(const fe &v, int m) {
-    return gf::mk(__shfl_xor(v.re, m), __shfl_xor(v.im, m));
+// One 32-bit DPP move per half of each coordinate: lane i reads the lane
+// that CTRL names (quad permutes and row mirrors, all rows and banks on).
+template <int CTRL>
+__device__ __forceinline__ uint64_t dpp64(uint64_t x) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)x, CTRL, 0xF, 0xF, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(x >> 32), CTRL, 0xF, 0xF, false);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+template <int CTRL>
+__device__ __forceinline__ fe dpp_fe(const fe &v) {
+    return gf::mk(dpp64<CTRL>(v.re), dpp64<CTRL>(v.im));
+}
+
+__device__ __forceinline__ uint64_t readlane64(uint64_t x, int lane) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, lane);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), lane);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ fe readlane_fe(const fe &v, int lane) {
+    return gf::mk(readlane64(v.re, lane), readlane64(v.im, lane));
+}
+
+// x < 2^64 -> x mod p up to one p (<= p + 7)
+__device__ __forceinline__ uint64_t fold61(uint64_t x) { return (x & gf::P) + (x >> 61); }
+
+// v[t] summed over the 64 lanes of the wave, on every lane.  All lanes must
+// be active and v canonical.  Within each row of 16: DPP exchanges with lane
+// i ^ 1, i ^ 2 (quad permutes), then 7 - i and 15 - i (half-row and row
+// mirrors, which pair each lane with one of the other half once the halves
+// are uniform); the four row sums are then read out of lanes 0, 16, 32, 48.
+// The adds stay unreduced while they cannot wrap (8 canonical values sum
+// below 2^64) with one Mersenne fold before the last row step and one before
+// the cross-row sum.  No LDS round trips, unlike ds_bpermute shuffles.
+template <int K>
+__device__ __forceinline__ void wave_allsum(fe *v) {
+    uint64_t re[K], im[K];
+#pragma unroll
+    for (int t = 0; t < K; ++t) { re[t] = v[t].re; im[t] = v[t].im; }
+#pragma unroll
+    for (int t = 0; t < K; ++t) { re[t] += dpp64<0xB1>(re[t]); im[t] += dpp64<0xB1>(im[t]); }    // quad_perm(1, 0, 3, 2)
+#pragma unroll
+    for (int t = 0; t < K; ++t) { re[t] += dpp64<0x4E>(re[t]); im[t] += dpp64<0x4E>(im[t]); }    // quad_perm(2, 3, 0, 1)
+#pragma unroll
+    for (int t = 0; t < K; ++t) { re[t] += dpp64<0x141>(re[t]); im[t] += dpp64<0x141>(im[t]); }  // row_half_mirror
+#pragma unroll
+    for (int t = 0; t < K; ++t) {
+        re[t] = fold61(re[t]); im[t] = fold61(im[t]);
+        re[t] += dpp64<0x140>(re[t]); im[t] += dpp64<0x140>(im[t]);                            // row_mirror
+        re[t] = fold61(re[t]); im[t] = fold61(im[t]);
+    }
+#pragma unroll
+    for (int t = 0; t < K; ++t) {
+        const uint64_t sr = readlane64(re[t], 0) + readlane64(re[t], 16) + readlane64(re[t], 32) + readlane64(re[t], 48);
+        const uint64_t si = readlane64(im[t], 0) + readlane64(im[t], 16) + readlane64(im[t], 32) + readlane64(im[t], 48);
+        v[t] = gf::mk(gf::canon(fold61(sr)), gf::canon(fold61(si)));
+    }
 }
 
 __device__ __forceinline__ void wave_reduce3(fe &a, fe &b, fe &c) {
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) {
-        a = gf::add(a, shfl_xor_fe(a, m));
-        b = gf::add(b, shfl_xor_fe(b, m));
-        c = gf::add(c, shfl_xor_fe(c, m));
-    }
+    fe v[3] = {a, b, c};
+    wave_allsum<3>(v);
+    a = v[0]; b = v[1]; c = v[2];
 }
 
 // Block-wide sum of (A, C, O); the result is valid in wave 0.  Starts and
@@ -70,6 +125,47 @@ __device__ __forceinline__ void block_reduce3(fe &A, fe &C, fe &O, fe (*red)[NT 
             A = gf::add(A, red[0][w]); C = gf::add(C, red[1][w]); O = gf::add(O, red[2][w]);
         }
     }
+}
+
+// Block-wide sums of v[0..K) written to out[0..K): one DPP sum per wave,
+// one barrier, then threads 0..K-1 add the waves' sums and store.  All
+// threads must call it.
+template <int K, int NT>
+__device__ __forceinline__ void block_partials(fe *v, fe *out, fe (*red)[K]) {
+    wave_allsum<K>(v);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int t = 0; t < K; ++t) red[wv][t] = v[t];
+    __syncthreads();
+    if (threadIdx.x < K) {
+        fe a = red[0][threadIdx.x];
+        for (int w = 1; w < NT / 64; ++w) a = gf::add(a, red[w][threadIdx.x]);
+        out[threadIdx.x] = a;
+    }
+}
+
+// Wave 0: v[0..K) = sum over b < nb_in of part[K b + t], on every lane.
+// A lane loads U sets before adding any, so a sweep costs one memory latency.
+template <int K>
+__device__ __forceinline__ void wave_partials(const fe *part, int nb_in, fe *v) {
+    constexpr int U = K > 3 ? 2 : 4;
+#pragma unroll
+    for (int t = 0; t < K; ++t) v[t] = gf::zero();
+    for (int b0 = threadIdx.x & 63; b0 < nb_in; b0 += 64 * U) {
+        fe x[U][K];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int b = b0 + 64 * u;
+#pragma unroll
+            for (int t = 0; t < K; ++t) x[u][t] = b < nb_in ? part[K * b + t] : gf::zero();
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int t = 0; t < K; ++t) v[t] = gf::add(v[t], x[u][t]);
+    }
+    wave_allsum<K>(v);
 }
 
 // Round `round`'s polynomial from (A = sum d1 d2, C = sum e1 e2,
@@ -108,25 +204,23 @@ __device__ fe finalize_round(ScState *st, int round, int fs_rounds, int claim_mo
 }
 
 // Reduce round `round`'s workgroup partials and finalize it in this
-// workgroup: (*sh_r, *sh_prev, *sh_y) receive the challenge and the state
-// after the round, visible to all threads on return.
-template <int NT>
-__device__ void finish_round(ScState *st, int round, int fs_rounds, const fe *part_in, int nb_in, fe (*red)[NT / 64],
-                             fe *sh_r, fe *sh_prev, fe *sh_y, bool writer) {
-    fe A = gf::zero(), C = gf::zero(), O = gf::zero();
-    for (int b = threadIdx.x; b < nb_in; b += NT) {
-        A = gf::add(A, part_in[3 * b]);
-        C = gf::add(C, part_in[3 * b + 1]);
-        O = gf::add(O, part_in[3 * b + 2]);
-    }
-    block_reduce3<NT>(A, C, O, red);
-    if (threadIdx.x < 4) {
-        // the transcript chain is serial: let it issue ahead of co-resident waves
-        __builtin_amdgcn_s_setprio(3);
+// workgroup: wave 0 sums the partials (its transcript state loads issue
+// first), lanes 0..3 run the round; (*sh_r, *sh_prev, *sh_y) receive the
+// challenge and the state after the round, visible to all threads on return.
+__device__ void finish_round(ScState *st, int round, int fs_rounds, const fe *part_in, int nb_in, fe *sh_r,
+                             fe *sh_prev, fe *sh_y, bool writer) {
+    if (threadIdx.x < 64) {
         fe prev = st->tr_prev[round], y = st->tr_y[round];
-        const fe r = finalize_round(st, round, fs_rounds, st->derive_y, prev, y, A, C, O, writer);
-        if (threadIdx.x == 0) { *sh_r = r; *sh_prev = prev; *sh_y = y; }
-        __builtin_amdgcn_s_setprio(0);
+        const int claim = st->derive_y;
+        fe v[3];
+        wave_partials<3>(part_in, nb_in, v);
+        if (threadIdx.x < 4) {
+            // the transcript chain is serial: let it issue ahead of co-resident waves
+            __builtin_amdgcn_s_setprio(3);
+            const fe r = finalize_round(st, round, fs_rounds, claim, prev, y, v[0], v[1], v[2], writer);
+            if (threadIdx.x == 0) { *sh_r = r; *sh_prev = prev; *sh_y = y; }
+            __builtin_amdgcn_s_setprio(0);
+        }
     }
     __syncthreads();
 }
@@ -144,7 +238,7 @@ struct ScJob {
     int nb_in, nb;            // partials in part_in; this job's workgroups
     int round, m, fs_rounds;
     uint32_t first_block;
-    int kind;                 // 0: one round; 1, 2: rounds 0-2 fused (sc_pair01_kernel, sc_fold01_kernel)
+    int kind;                 // 0: one round; 1: sc_pair01_kernel; 2, 3: sc_fold2_kernel<false>, <true>
 };
 
 // One multi-workgroup round `round`.  FOLD (round > 0): inputs hold
@@ -153,7 +247,7 @@ struct ScJob {
 // round's per-workgroup partials to part_out.
 template <bool FOLD>
 __device__ __forceinline__ void sc_round(const ScJob &J, uint32_t blk) {
-    __shared__ fe red[3][SC_THREADS / 64];
+    __shared__ fe red[SC_THREADS / 64][3];
     __shared__ fe sh_r, sh_prev, sh_y;
     const fe *__restrict__ s1 = J.s1;
     const fe *__restrict__ s2 = J.s2;
@@ -182,7 +276,7 @@ __device__ __forceinline__ void sc_round(const ScJob &J, uint32_t blk) {
     load(j0);
     fe r = gf::zero();
     if (FOLD) {
-        finish_round<SC_THREADS>(st, round - 1, fs_rounds, part_in, nb_in, red, &sh_r, &sh_prev, &sh_y, blk == 0);
+        finish_round(st, round - 1, fs_rounds, part_in, nb_in, &sh_r, &sh_prev, &sh_y, blk == 0);
         r = sh_r;
     }
     fe A = gf::zero(), C = gf::zero(), O = gf::zero();
@@ -210,145 +304,13 @@ __device__ __forceinline__ void sc_round(const ScJob &J, uint32_t blk) {
         j0 += stride;
         if (j0 < npairs) load(j0);
     }
-    block_reduce3<SC_THREADS>(A, C, O, red);
-    if (threadIdx.x == 0) {
-        part_out[3 * blk] = A;
-        part_out[3 * blk + 1] = C;
-        part_out[3 * blk + 2] = O;
-    }
+    fe v[3] = {A, C, O};
+    block_partials<3, SC_THREADS>(v, part_out + 3 * blk, red);
 }
 
 template <bool FOLD>
 __global__ __launch_bounds__(SC_THREADS) void sc_round_kernel(ScJob J) {
     sc_round<FOLD>(J, blockIdx.x);
-}
-
-// Rounds 0 and 1 from one read of the inputs.  For a quadruple (x0..x3) of
-// each vector, round 1 works on e = x0 + r0 (x1 - x0), o = x2 + r0 (x3 - x2)
-// with r0 still unknown, so each of its three terms is a quadratic in r0,
-// kept as its values at r0 = 0 and r0 = 1 and its r0^2 coefficient (nine
-// products).  Round 0's sums are sums of those: A0 = sum of the C and O r0^2
-// coefficients (the pair differences' products), C0 = sum of their values
-// at 0, O0 = sum of their values at 1.  J.npairs = quadruples; nine
-// partials per workgroup go to part_out.
-__global__ __launch_bounds__(SC_THREADS) void sc_pair01_kernel(ScJob J) {
-    __shared__ fe red[3][SC_THREADS / 64];
-    const fe *__restrict__ s1 = J.s1;
-    const fe *__restrict__ s2 = J.s2;
-    const uint64_t nq = J.npairs;
-    fe q[9];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) q[i] = gf::zero();
-    const uint64_t stride = (uint64_t)J.nb * SC_THREADS;
-    uint64_t j = (uint64_t)blockIdx.x * SC_THREADS + threadIdx.x;
-    while (j < nq) {
-        const fe x0 = s1[4 * j], x1 = s1[4 * j + 1], x2 = s1[4 * j + 2], x3 = s1[4 * j + 3];
-        const fe y0 = s2[4 * j], y1 = s2[4 * j + 1], y2 = s2[4 * j + 2], y3 = s2[4 * j + 3];
-        const uint64_t jn = j + stride;
-        const fe ax = gf::sub(x1, x0), bx = gf::sub(x3, x2), ay = gf::sub(y1, y0), by = gf::sub(y3, y2);
-        // A: (o - e)(o' - e'), o - e = (x2 - x0) + r0 (bx - ax)
-        q[0] = gf::add(q[0], gf::mul(gf::sub(x2, x0), gf::sub(y2, y0)));
-        q[1] = gf::add(q[1], gf::mul(gf::sub(x3, x1), gf::sub(y3, y1)));
-        q[2] = gf::add(q[2], gf::mul(gf::sub(bx, ax), gf::sub(by, ay)));
-        // C: e e' = (x0 + r0 ax)(y0 + r0 ay);  O: o o' = (x2 + r0 bx)(y2 + r0 by)
-        q[3] = gf::add(q[3], gf::mul(x0, y0));
-        q[4] = gf::add(q[4], gf::mul(x1, y1));
-        q[5] = gf::add(q[5], gf::mul(ax, ay));
-        q[6] = gf::add(q[6], gf::mul(x2, y2));
-        q[7] = gf::add(q[7], gf::mul(x3, y3));
-        q[8] = gf::add(q[8], gf::mul(bx, by));
-        j = jn;
-    }
-#pragma unroll
-    for (int g = 0; g < 3; ++g) {
-        block_reduce3<SC_THREADS>(q[3 * g], q[3 * g + 1], q[3 * g + 2], red);
-        if (threadIdx.x == 0) {
-            J.part_out[9 * blockIdx.x + 3 * g] = q[3 * g];
-            J.part_out[9 * blockIdx.x + 3 * g + 1] = q[3 * g + 1];
-            J.part_out[9 * blockIdx.x + 3 * g + 2] = q[3 * g + 2];
-        }
-    }
-}
-
-// Finalizes rounds 0 and 1 from sc_pair01_kernel's partials (in every
-// workgroup; workgroup 0 records them), then folds each 8-tuple of the
-// inputs by r0 and r1 into a pair (e, o) of the quarter-size outputs and
-// forms round 2's sums on it.  J.npairs = round 2's pairs (n / 8).
-__global__ __launch_bounds__(SC_THREADS) void sc_fold01_kernel(ScJob J) {
-    __shared__ fe red[3][SC_THREADS / 64];
-    __shared__ fe sh_r[2];
-    const fe *__restrict__ s1 = J.s1;
-    const fe *__restrict__ s2 = J.s2;
-    fe *__restrict__ d1 = J.d1;
-    fe *__restrict__ d2 = J.d2;
-    const uint64_t npairs = J.npairs;
-    const uint64_t stride = (uint64_t)J.nb * SC_THREADS;
-    uint64_t j = (uint64_t)blockIdx.x * SC_THREADS + threadIdx.x;
-    {
-        fe q[9];
-#pragma unroll
-        for (int i = 0; i < 9; ++i) q[i] = gf::zero();
-        for (int b = threadIdx.x; b < J.nb_in; b += SC_THREADS)
-#pragma unroll
-            for (int i = 0; i < 9; ++i) q[i] = gf::add(q[i], J.part_in[9 * b + i]);
-#pragma unroll
-        for (int g = 0; g < 3; ++g) block_reduce3<SC_THREADS>(q[3 * g], q[3 * g + 1], q[3 * g + 2], red);
-        if (threadIdx.x < 4) {
-            __builtin_amdgcn_s_setprio(3);
-            ScState *st = J.st;
-            const bool writer = blockIdx.x == 0;
-            fe prev = st->tr_prev[0], yv = st->tr_y[0];
-            // round 0 from round 1's quadratics
-            const fe A0 = gf::add(q[5], q[8]), C0 = gf::add(q[3], q[6]), O0 = gf::add(q[4], q[7]);
-            const fe r0 = finalize_round(st, 0, J.fs_rounds, st->derive_y, prev, yv, A0, C0, O0, writer);
-            // a quadratic from its values at 0 and 1 and its square coefficient, at r0
-            auto at = [&](const fe &v0, const fe &v1, const fe &c2) {
-                const fe c1 = gf::sub(gf::sub(v1, v0), c2);
-                return gf::add(v0, gf::mul(r0, gf::add(c1, gf::mul(r0, c2))));
-            };
-            const fe A1 = at(q[0], q[1], q[2]), C1 = at(q[3], q[4], q[5]), O1 = at(q[6], q[7], q[8]);
-            const fe r1 = finalize_round(st, 1, J.fs_rounds, st->derive_y, prev, yv, A1, C1, O1, writer);
-            if (threadIdx.x == 0) { sh_r[0] = r0; sh_r[1] = r1; }
-            __builtin_amdgcn_s_setprio(0);
-        }
-        __syncthreads();
-    }
-    const fe r0 = sh_r[0], r1 = sh_r[1];
-    fe A = gf::zero(), C = gf::zero(), O = gf::zero();
-    // one vector's 8-tuple at a time (the scheduling barrier keeps the
-    // second vector's loads after the first's folds): fewer live registers
-    auto fold8 = [&](const fe *v, uint64_t t, fe &e, fe &o) {
-        fe f[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) f[k] = fold1(v[8 * t + 2 * k], v[8 * t + 2 * k + 1], r0);
-        e = fold1(f[0], f[1], r1);
-        o = fold1(f[2], f[3], r1);
-    };
-    while (j < npairs) {
-        fe e1, o1, e2, o2;
-        fold8(s1, j, e1, o1);
-        __builtin_amdgcn_sched_barrier(0);
-        fold8(s2, j, e2, o2);
-        d1[2 * j] = e1; d1[2 * j + 1] = o1;
-        d2[2 * j] = e2; d2[2 * j + 1] = o2;
-        A = gf::add(A, gf::mul(gf::sub(o1, e1), gf::sub(o2, e2)));
-        C = gf::add(C, gf::mul(e1, e2));
-        O = gf::add(O, gf::mul(o1, o2));
-        j += stride;
-    }
-    block_reduce3<SC_THREADS>(A, C, O, red);
-    if (threadIdx.x == 0) {
-        J.part_out[3 * blockIdx.x] = A;
-        J.part_out[3 * blockIdx.x + 1] = C;
-        J.part_out[3 * blockIdx.x + 2] = O;
-    }
-}
-
-template <bool FOLD>
-__global__ __launch_bounds__(SC_THREADS) void sc_round_batch_kernel(const ScJob *__restrict__ jobs,
-                                                                     const uint16_t *__restrict__ blk_job) {
-    const ScJob &J = jobs[blk_job[blockIdx.x]];
-    sc_round<FOLD>(J, blockIdx.x - J.first_block);
 }
 
 // A round's sums as quadratics in the challenge r not yet drawn: for each of
@@ -377,12 +339,157 @@ __device__ __forceinline__ fe quad_eval(const fe &f0, const fe &f1, const fe &f2
     return gf::add(f0, gf::mul(r, gf::add(gf::sub(gf::sub(f1, f0), f2), gf::mul(r, f2))));
 }
 
-// q summed over lanes [0, P) of the wave (P a power of two <= 64; lanes
-// without work hold zeros), every lane of the block ends with the sum
-__device__ __forceinline__ void wave_sum9(fe *q, int P) {
-    for (int msk = 1; msk < P; msk <<= 1)
+// Rounds 0 and 1 from one read of the inputs: quad_acc over each quadruple
+// (x0..x3) of each vector gives round 1's three terms as quadratics in the
+// challenge r0 not yet drawn, and round 0's sums are sums of those
+// coefficients: A0 = q5 + q8 (the pair differences' products), C0 = q3 + q6
+// (values at r0 = 0), O0 = q4 + q7 (values at r0 = 1).  J.npairs =
+// quadruples; nine partials per workgroup go to part_out.
+__global__ __launch_bounds__(SC_THREADS) void sc_pair01_kernel(ScJob J) {
+    __shared__ fe red[SC_THREADS / 64][9];
+    const fe *__restrict__ s1 = J.s1;
+    const fe *__restrict__ s2 = J.s2;
+    const uint64_t nq = J.npairs;
+    fe q[9];
 #pragma unroll
-        for (int t = 0; t < 9; ++t) q[t] = gf::add(q[t], shfl_xor_fe(q[t], msk));
+    for (int i = 0; i < 9; ++i) q[i] = gf::zero();
+    const uint64_t stride = (uint64_t)J.nb * SC_THREADS;
+    for (uint64_t j = (uint64_t)blockIdx.x * SC_THREADS + threadIdx.x; j < nq; j += stride) {
+        fe x[4], y[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) { x[e] = s1[4 * j + e]; y[e] = s2[4 * j + e]; }
+        quad_acc(q, x, y);
+    }
+    block_partials<9, SC_THREADS>(q, J.part_out + 9 * blockIdx.x, red);
+}
+
+// Two rounds per launch.  The input v(i) (J.round = i) comes with nine
+// partials per workgroup of the previous launch: round i + 1's sums as
+// quadratics in r_i (quad_acc's layout), from which round i's sums follow as
+// in sc_pair01_kernel.  In every workgroup wave 0 reduces them and finalizes
+// rounds i and i + 1 (r_i, then r_{i+1}; workgroup 0 records them) while
+// the other waves' first loads are in flight; then groups of v(i) are folded
+// by r_i and r_{i+1} into v(i + 2), which is written, and on v(i + 2) the
+// kernel forms
+//   QUAD = false: round i + 2's sums (three partials per workgroup): a lane
+//     folds an 8-tuple of each vector into a pair; J.npairs = pairs of
+//     v(i + 2);
+//   QUAD = true: round i + 3's sums as quadratics in r_{i+2} (nine
+//     partials, the layout this kernel reads): four lanes fold a 16-tuple
+//     (4 elements each) into a quadruple of v(i + 2), exchange it by DPP, and
+//     lane p of the four forms the three products at r = p (p = 0, 1, and
+//     p = 2 for the r^2 coefficient); J.npairs = quadruples of v(i + 2).
+// A chain pair01, QUAD, ..., QUAD, direct thus covers two rounds per launch
+// and ends like any multi-workgroup round: v(k) in memory, round k's sums in
+// three partials per workgroup.
+template <bool QUAD>
+__global__ __launch_bounds__(SC_THREADS) void sc_fold2_kernel(ScJob J) {
+    constexpr int K = QUAD ? 9 : 3;          // partials per workgroup written
+    constexpr int NL = QUAD ? 4 : 8;         // elements of v(i) per lane and vector
+    __shared__ fe red[SC_THREADS / 64][K];
+    __shared__ fe sh_r[2];
+    const fe *__restrict__ s1 = J.s1;
+    const fe *__restrict__ s2 = J.s2;
+    fe *__restrict__ d1 = J.d1;
+    fe *__restrict__ d2 = J.d2;
+    const uint64_t lanes = QUAD ? 4 * J.npairs : J.npairs;   // lane-items
+    const uint64_t stride = (uint64_t)J.nb * SC_THREADS;     // a multiple of 4: a lane keeps its place in its quad
+    const int round = J.round;
+    uint64_t t = (uint64_t)blockIdx.x * SC_THREADS + threadIdx.x;
+    fe a[NL], b[NL];
+    auto load = [&](uint64_t u) {
+#pragma unroll
+        for (int e = 0; e < NL; ++e) { a[e] = s1[NL * u + e]; b[e] = s2[NL * u + e]; }
+    };
+    if (t < lanes) load(t);
+    if (threadIdx.x < 64) {
+        ScState *st = J.st;
+        fe prev = st->tr_prev[round], yv = st->tr_y[round];
+        const int claim = st->derive_y;
+        fe q[9];
+        wave_partials<9>(J.part_in, J.nb_in, q);
+        if (threadIdx.x < 4) {
+            __builtin_amdgcn_s_setprio(3);
+            const bool writer = blockIdx.x == 0;
+            const fe A0 = gf::add(q[5], q[8]), C0 = gf::add(q[3], q[6]), O0 = gf::add(q[4], q[7]);
+            const fe r0 = finalize_round(st, round, J.fs_rounds, claim, prev, yv, A0, C0, O0, writer);
+            const fe A1 = quad_eval(q[0], q[1], q[2], r0), C1 = quad_eval(q[3], q[4], q[5], r0),
+                     O1 = quad_eval(q[6], q[7], q[8], r0);
+            const fe r1 = finalize_round(st, round + 1, J.fs_rounds, claim, prev, yv, A1, C1, O1, writer);
+            if (threadIdx.x == 0) { sh_r[0] = r0; sh_r[1] = r1; }
+            __builtin_amdgcn_s_setprio(0);
+        }
+    }
+    __syncthreads();
+    const fe r0 = sh_r[0], r1 = sh_r[1];
+    fe v[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = gf::zero();
+    if (QUAD) {
+        const uint32_t p = threadIdx.x & 3;
+        fe acc[3] = {gf::zero(), gf::zero(), gf::zero()};   // A, C, O at r = p
+        while (t < lanes) {
+            // this lane's element of the quadruple: v(i + 2)[t]
+            const fe x = fold1(fold1(a[0], a[1], r0), fold1(a[2], a[3], r0), r1);
+            const fe z = fold1(fold1(b[0], b[1], r0), fold1(b[2], b[3], r0), r1);
+            d1[t] = x;
+            d2[t] = z;
+            t += stride;
+            if (t < lanes) load(t);
+            // e(p), o(p) of the next-round pair (x0, x1), (x2, x3): p = 0: x0, x2;
+            // p = 1: x1, x3; p = 2: x1 - x0, x3 - x2 (lane 3's values are unused)
+            const fe ex = dpp_fe<0xD4>(x), oxh = dpp_fe<0xFE>(x);      // quad_perm(0,1,1,3), (2,3,3,3)
+            const fe ez = dpp_fe<0xD4>(z), ozh = dpp_fe<0xFE>(z);
+            const fe x0 = dpp_fe<0x00>(x), z0 = dpp_fe<0x00>(z);       // quad_perm(0,0,0,0)
+            const bool inf = p == 2;
+            const fe e1 = inf ? gf::sub(ex, x0) : ex, o1 = inf ? gf::sub(oxh, x) : oxh;
+            const fe e2 = inf ? gf::sub(ez, z0) : ez, o2 = inf ? gf::sub(ozh, z) : ozh;
+            acc[0] = gf::add(acc[0], gf::mul(gf::sub(o1, e1), gf::sub(o2, e2)));
+            acc[1] = gf::add(acc[1], gf::mul(e1, e2));
+            acc[2] = gf::add(acc[2], gf::mul(o1, o2));
+        }
+        // A(p) -> v[p], C(p) -> v[3 + p], O(p) -> v[6 + p]
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            v[k] = p == (uint32_t)k ? acc[0] : gf::zero();
+            v[3 + k] = p == (uint32_t)k ? acc[1] : gf::zero();
+            v[6 + k] = p == (uint32_t)k ? acc[2] : gf::zero();
+        }
+    } else {
+        while (t < lanes) {
+            fe f[4], g[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                f[k] = fold1(a[2 * k], a[2 * k + 1], r0);
+                g[k] = fold1(b[2 * k], b[2 * k + 1], r0);
+            }
+            const fe e1 = fold1(f[0], f[1], r1), o1 = fold1(f[2], f[3], r1);
+            const fe e2 = fold1(g[0], g[1], r1), o2 = fold1(g[2], g[3], r1);
+            d1[2 * t] = e1; d1[2 * t + 1] = o1;
+            d2[2 * t] = e2; d2[2 * t + 1] = o2;
+            t += stride;
+            if (t < lanes) load(t);
+            v[0] = gf::add(v[0], gf::mul(gf::sub(o1, e1), gf::sub(o2, e2)));
+            v[1] = gf::add(v[1], gf::mul(e1, e2));
+            v[2] = gf::add(v[2], gf::mul(o1, o2));
+        }
+    }
+    block_partials<K, SC_THREADS>(v, J.part_out + K * blockIdx.x, red);
+}
+
+template <bool FOLD>
+__global__ __launch_bounds__(SC_THREADS) void sc_round_batch_kernel(const ScJob *__restrict__ jobs,
+                                                                     const uint16_t *__restrict__ blk_job) {
+    const ScJob &J = jobs[blk_job[blockIdx.x]];
+    sc_round<FOLD>(J, blockIdx.x - J.first_block);
+}
+
+// q summed over the wave (lanes without work hold zeros; P, the power of two
+// covering the lanes with work, is not needed by the all-lane sum), every
+// lane ends with the sum
+__device__ __forceinline__ void wave_sum9(fe *q, int P) {
+    (void)P;
+    wave_allsum<9>(q);
 }
 
 // The nine products of a next-round pair (quad_acc's terms) as
@@ -633,9 +740,8 @@ __global__ __launch_bounds__(SC_TAIL_THREADS) void sc_tail_batch_kernel(const Sc
 __global__ __launch_bounds__(SC_THREADS) void sc_fold_kernel(
     const fe *__restrict__ s1, const fe *__restrict__ s2, fe *__restrict__ d1, fe *__restrict__ d2, uint64_t nout,
     int round, int fs_rounds, ScState *st, const fe *part_in, int nb_in) {
-    __shared__ fe red[3][SC_THREADS / 64];
     __shared__ fe sh_r, sh_prev, sh_y;
-    finish_round<SC_THREADS>(st, round, fs_rounds, part_in, nb_in, red, &sh_r, &sh_prev, &sh_y, blockIdx.x == 0);
+    finish_round(st, round, fs_rounds, part_in, nb_in, &sh_r, &sh_prev, &sh_y, blockIdx.x == 0);
     const fe r = sh_r;
     for (uint64_t j = (uint64_t)blockIdx.x * SC_THREADS + threadIdx.x; j < nout; j += (uint64_t)gridDim.x * SC_THREADS) {
         const fe a0 = s1[2 * j], a1 = s1[2 * j + 1], b0 = s2[2 * j], b1 = s2[2 * j + 1];
@@ -839,7 +945,8 @@ namespace {
 // slot not holding its input first (a folded vector of m elements lands in a
 // slot of at least m), then alternates; it never writes v1 / v2.
 std::vector<ScJob> sumcheck_jobs(const fe *v1, const fe *v2, uint64_t n, int fs_rounds, fe *buf, fe *part0,
-                                 ScState *st, int max_blocks, uint64_t tail_elems, bool fuse01 = false) {
+                                 ScState *st, int max_blocks, uint64_t tail_elems, int wide_blocks,
+                                 bool fuse01 = false) {
     const int m = ilog2(n);
     fe *a1 = buf, *a2 = buf + n / 2, *b1 = buf + n, *b2 = buf + n + n / 4;
     fe *part[2] = {part0, part0 + 3 * SC_MAX_BLOCKS};
@@ -849,12 +956,18 @@ std::vector<ScJob> sumcheck_jobs(const fe *v1, const fe *v2, uint64_t n, int fs_
     uint64_t len = n;        // elements in s1/s2
     int round = 0, which = 0, nb_prev = 0;
     bool pending = false;    // s1/s2 still need the fold with r[round-1]
-    // rounds 0-2 as two launches where all three are multi-workgroup rounds:
-    // round 2's folded pairs (n / 4 elements) land in outs[1]
+    const fe *pin = part[1];  // partials the next launch reads
+    // rounds 0, 1, ... two per launch while the vectors are long: sc_pair01
+    // (rounds 0 and 1 as nine partials), then sc_fold2 launches, each
+    // finalizing two rounds and quartering the vectors; the last one (its
+    // output at most tail_elems) leaves round k's sums in three partials
     if (fuse01 && n / 2 > tail_elems) {
+        // these launches stream the long vectors: as many workgroups as
+        // keep every SIMD's loads in flight (wide_blocks), not the later
+        // rounds' count, whose partials every workgroup re-reads
         auto width = [&](uint64_t items) {
             uint64_t b = (items + SC_THREADS - 1) / SC_THREADS;
-            if (b > (uint64_t)max_blocks) b = max_blocks;
+            if (b > (uint64_t)wide_blocks) b = wide_blocks;
             return (int)(b ? b : 1);
         };
         ScJob A{};
@@ -868,23 +981,37 @@ std::vector<ScJob> sumcheck_jobs(const fe *v1, const fe *v2, uint64_t n, int fs_
         A.nb = std::min(width(n / 4), SC_MAX_BLOCKS / 3);
         A.round = 0; A.m = m; A.fs_rounds = fs_rounds;
         jobs.push_back(A);
-        ScJob B{};
-        B.kind = 2;
-        B.s1 = v1; B.s2 = v2;
-        B.d1 = outs[1][0]; B.d2 = outs[1][1];
-        B.npairs = n / 8;
-        B.st = st;
-        B.part_in = part[1];
-        B.nb_in = A.nb;
-        B.part_out = part[0];
-        B.nb = width(n / 8);
-        B.round = 2; B.m = m; B.fs_rounds = fs_rounds;
-        jobs.push_back(B);
-        s1 = outs[1][0]; s2 = outs[1][1];
-        len = n / 4;
-        round = 3;
-        which = 0;
-        nb_prev = B.nb;
+        int nb_in = A.nb, w = 1, i = 0;
+        uint64_t L = n;     // elements of v(i)
+        for (;;) {
+            const bool quad = L / 4 > tail_elems;
+            ScJob B{};
+            B.kind = quad ? 3 : 2;
+            B.s1 = s1; B.s2 = s2;
+            B.d1 = outs[w][0]; B.d2 = outs[w][1];
+            B.npairs = quad ? L / 16 : L / 8;
+            B.st = st;
+            B.part_in = pin;
+            B.nb_in = nb_in;
+            B.part_out = pin == part[0] ? part[1] : part[0];
+            // lanes: four per quadruple of v(i + 2) (QUAD), one per pair (direct)
+            B.nb = quad ? std::min(width(L / 4), SC_MAX_BLOCKS / 3) : width(L / 8);
+            B.round = i; B.m = m; B.fs_rounds = fs_rounds;
+            jobs.push_back(B);
+            s1 = B.d1; s2 = B.d2;
+            pin = B.part_out;
+            nb_in = B.nb;
+            w ^= 1;
+            L /= 4;
+            i += 2;
+            if (!quad) break;
+        }
+        // v(i) is in s1 / s2, round i's sums in pin: the next launch
+        // finalizes round i and folds by r_i
+        len = L;
+        round = i + 1;
+        which = w;
+        nb_prev = nb_in;
         pending = true;
     }
     while (len > tail_elems) {
@@ -896,8 +1023,9 @@ std::vector<ScJob> sumcheck_jobs(const fe *v1, const fe *v2, uint64_t n, int fs_
         J.s1 = s1; J.s2 = s2;
         J.npairs = npairs;
         J.st = st;
-        J.part_in = part[(round + 1) & 1];
-        J.part_out = part[round & 1];
+        J.part_in = pin;
+        J.part_out = pin == part[0] ? part[1] : part[0];
+        pin = J.part_out;
         J.nb_in = pending ? nb_prev : 0;
         J.nb = (int)blocks;
         J.round = round;
@@ -920,7 +1048,7 @@ std::vector<ScJob> sumcheck_jobs(const fe *v1, const fe *v2, uint64_t n, int fs_
     T.e1 = outs[which ^ 1][0]; T.e2 = outs[which ^ 1][1];
     T.npairs = len;
     T.st = st;
-    T.part_in = part[(round + 1) & 1];
+    T.part_in = pin;
     T.nb_in = nb_prev;
     T.round = round;
     T.m = m;
@@ -934,13 +1062,15 @@ std::vector<ScJob> sumcheck_jobs(const fe *v1, const fe *v2, uint64_t n, int fs_
 int sumcheck_launch(cosnark_ctx *ctx, const fe *v1, const fe *v2, uint64_t n, int fs_rounds, fe *buf,
                     hipStream_t stream, bool fuse01) {
     const auto jobs = sumcheck_jobs(v1, v2, n, fs_rounds, buf, ctx->sc_partials, ctx->sc, ctx->sc_blocks,
-                                    ctx->sc_tail_elems, fuse01);
+                                    ctx->sc_tail_elems, ctx->sc_wide_blocks, fuse01);
     for (size_t i = 0; i + 1 < jobs.size(); ++i) {
         const ScJob &J = jobs[i];
         if (J.kind == 1)
             hipLaunchKernelGGL(sc_pair01_kernel, dim3((unsigned)J.nb), dim3(SC_THREADS), 0, stream, J);
         else if (J.kind == 2)
-            hipLaunchKernelGGL(sc_fold01_kernel, dim3((unsigned)J.nb), dim3(SC_THREADS), 0, stream, J);
+            hipLaunchKernelGGL(sc_fold2_kernel<false>, dim3((unsigned)J.nb), dim3(SC_THREADS), 0, stream, J);
+        else if (J.kind == 3)
+            hipLaunchKernelGGL(sc_fold2_kernel<true>, dim3((unsigned)J.nb), dim3(SC_THREADS), 0, stream, J);
         else if (J.round > 0)
             hipLaunchKernelGGL(sc_round_kernel<true>, dim3((unsigned)J.nb), dim3(SC_THREADS), 0, stream, J);
         else
@@ -961,7 +1091,8 @@ int sumcheck_batch_plan(cosnark_ctx *ctx, int count, const fe *const *v1, const 
     size_t nr = 0;
     for (int i = 0; i < count; ++i) {
         const int m = ilog2(n[i]);
-        per[i] = sumcheck_jobs(v1[i], v2[i], n[i], m, buf[i], parts[i], st[i], ctx->sc_blocks, ctx->sc_tail_elems);
+        per[i] = sumcheck_jobs(v1[i], v2[i], n[i], m, buf[i], parts[i], st[i], ctx->sc_blocks, ctx->sc_tail_elems,
+                               ctx->sc_wide_blocks);
         nr = std::max(nr, per[i].size() - 1);
     }
     plan.rounds.clear();
