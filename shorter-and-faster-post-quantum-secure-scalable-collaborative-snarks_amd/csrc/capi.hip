@@ -149,6 +149,10 @@ int cosnark_create(int device, cosnark_ctx **out) {
         const int b = atoi(e);
         if (b >= 1 && b <= SC_MAX_BLOCKS / 3) ctx->sc_wide_blocks = b;
     }
+    if (const char *e = getenv("COSNARK_SC_WIDE_SHARED")) {
+        const int b = atoi(e);
+        if (b >= 1 && b <= SC_MAX_BLOCKS / 3) ctx->sc_wide_shared = b;
+    }
     if (const char *e = getenv("COSNARK_SC_TAIL")) {
         const long long v = atoll(e);
         if (v >= 4 && v <= (1 << 20) && is_pow2((uint64_t)v)) ctx->sc_tail_elems = (uint64_t)v;

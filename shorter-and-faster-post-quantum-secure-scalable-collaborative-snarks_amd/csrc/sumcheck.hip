@@ -517,6 +517,7 @@ __device__ __forceinline__ void wave_lds_sync() {
 
 constexpr int SC_SPLIT_MAX = 7;    // groups whose 8 folds and 9 products fit one wave's lanes
 constexpr int SC_SPLIT_NG = 64;    // phases with at most this many groups spread them over lanes
+constexpr int SC_TAIL_LDS = 512;   // folded vectors up to this many elements stay in LDS
 
 // All rounds from `round0` to m-1 in one workgroup of four waves.  The input
 // (len elements) is v(round0) when round0 == 0, else v(round0 - 1), still to
@@ -524,9 +525,11 @@ constexpr int SC_SPLIT_NG = 64;    // phases with at most this many groups sprea
 // finalizes round i = round0 + k on wave 0 (lanes 0..3: sums, claim check,
 // transcript hashes) while waves 1..3 fold the current vector by r[i - 1]
 // into the next buffer (d, e, d, ...: never the input) and accumulate round
-// i + 1's sums as quadratics in r[i]; one barrier ends each phase.  Vectors
-// move through global memory (L2), so the workgroup needs no LDS beyond the
-// exchange slots and fits on a CU beside other kernels' workgroups.
+// i + 1's sums as quadratics in r[i]; one barrier ends each phase.  A folded
+// vector of at most SC_TAIL_LDS elements stays in LDS (32 KiB for both
+// slots), longer ones go through global memory (L2).  Lanes 4g..4g+3 of
+// wave 0 evaluate quadratic g at the last challenge in parallel before lanes
+// 0..3 hash.
 //
 // A lane issues 64-bit multiplies slowly when it is alone on its SIMD, so
 // the compute waves keep per-lane chains short: with many groups (8
@@ -540,6 +543,7 @@ __device__ __forceinline__ void sc_tail(const ScJob &J) {
     __shared__ fe sh_r[2];
     __shared__ fe lfold[SC_TAIL_WAVES][SC_SPLIT_MAX][9];   // per wave: folded x (0..3), z (4..7), zero (8)
     __shared__ fe lprod[SC_TAIL_WAVES][SC_SPLIT_MAX][9];
+    __shared__ fe lvec[2][2][SC_TAIL_LDS];                 // [slot][vector]: short folded vectors
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const bool hash_wave = wave == 0;
     const int ct = tid - 64;
@@ -551,6 +555,11 @@ __device__ __forceinline__ void sc_tail(const ScJob &J) {
     fe *const o2[2] = {J.d2, J.e2};
     int ob = 0;
     uint32_t cur = (uint32_t)J.npairs;      // elements in c1 / c2
+    // the fold of the current vector (cur / 2 elements) goes to slot ob: LDS
+    // when it fits (the next phase reads it back at LDS latency), else the
+    // global ping-pong buffers
+    auto out1 = [&]() -> fe * { return cur / 2 <= (uint32_t)SC_TAIL_LDS ? lvec[ob][0] : o1[ob]; };
+    auto out2 = [&]() -> fe * { return cur / 2 <= (uint32_t)SC_TAIL_LDS ? lvec[ob][1] : o2[ob]; };
     fe prev = gf::zero(), y = gf::zero(), rr = gf::zero();
     int claim_mode = 0;
     // per-thread accumulation, then the sum over the lanes and waves that hold
@@ -587,6 +596,7 @@ __device__ __forceinline__ void sc_tail(const ScJob &J) {
                 rr = finalize_round(st, round0 - 1, fs_rounds, claim_mode, prev, y, A, C, O, true);
                 __builtin_amdgcn_s_setprio(0);
             }
+            rr = readlane_fe(rr, 0);      // the wave's lanes 0..11 evaluate with it
         }
         if (lane == 0) sh_r[0] = rr;
     } else {
@@ -622,28 +632,31 @@ __device__ __forceinline__ void sc_tail(const ScJob &J) {
         const int i = round0 + k, s = k & 1;
         const bool fold = pending || k > 0;    // the current vector is v(i - 1)
         if (hash_wave) {
+            // lanes 4g..4g+3 sum quadratic g (A, C, O) over the compute waves
+            // and evaluate it at the last challenge, in parallel; lanes 0..3
+            // then run the round
+            __builtin_amdgcn_s_setprio(3);
+            fe E = gf::zero();
+            if (lane < 12) {
+                const int g = lane >> 2, nw = sh_nw[s];
+                fe f0 = qpart[s][0][3 * g], f1 = qpart[s][0][3 * g + 1], f2 = qpart[s][0][3 * g + 2];
+                for (int w = 1; w < nw; ++w) {
+                    f0 = gf::add(f0, qpart[s][w][3 * g]);
+                    f1 = gf::add(f1, qpart[s][w][3 * g + 1]);
+                    f2 = gf::add(f2, qpart[s][w][3 * g + 2]);
+                }
+                E = fold ? quad_eval(f0, f1, f2, rr) : f0;
+            }
+            const fe A = readlane_fe(E, 0), C = readlane_fe(E, 4), O = readlane_fe(E, 8);
             if (lane < 4) {
-                __builtin_amdgcn_s_setprio(3);
-                const int nw = sh_nw[s];
-                fe f[9];
-#pragma unroll
-                for (int t = 0; t < 9; ++t) {
-                    f[t] = qpart[s][0][t];
-                    for (int w = 1; w < nw; ++w) f[t] = gf::add(f[t], qpart[s][w][t]);
-                }
-                fe A = f[0], C = f[3], O = f[6];
-                if (fold) {
-                    A = quad_eval(f[0], f[1], f[2], rr);
-                    C = quad_eval(f[3], f[4], f[5], rr);
-                    O = quad_eval(f[6], f[7], f[8], rr);
-                }
                 rr = finalize_round(st, i, fs_rounds, claim_mode, prev, y, A, C, O, true);
                 if (lane == 0) sh_r[s ^ 1] = rr;
-                __builtin_amdgcn_s_setprio(0);
             }
+            rr = readlane_fe(rr, 0);
+            __builtin_amdgcn_s_setprio(0);
         } else {
             const bool next = i + 1 < m;       // round i + 1 exists: accumulate its sums
-            fe *w1 = o1[ob], *w2 = o2[ob];
+            fe *w1 = out1(), *w2 = out2();
             if (fold && next && cur / 8 <= (uint32_t)SC_SPLIT_NG) {
                 // few groups: compute wave cw takes groups cw, cw + 3, ..., seven at a
                 // time: one fold per lane, then one of the nine products per lane,
@@ -713,8 +726,8 @@ __device__ __forceinline__ void sc_tail(const ScJob &J) {
             }
         }
         if (fold) {
-            c1 = o1[ob];
-            c2 = o2[ob];
+            c1 = out1();
+            c2 = out2();
             ob ^= 1;
             cur /= 2;
         }
@@ -1061,8 +1074,10 @@ std::vector<ScJob> sumcheck_jobs(const fe *v1, const fe *v2, uint64_t n, int fs_
 
 int sumcheck_launch(cosnark_ctx *ctx, const fe *v1, const fe *v2, uint64_t n, int fs_rounds, fe *buf,
                     hipStream_t stream, bool fuse01) {
+    // beside the commit (stream2) the long-vector launches take fewer workgroups
+    const int wide = stream == ctx->stream2 ? ctx->sc_wide_shared : ctx->sc_wide_blocks;
     const auto jobs = sumcheck_jobs(v1, v2, n, fs_rounds, buf, ctx->sc_partials, ctx->sc, ctx->sc_blocks,
-                                    ctx->sc_tail_elems, ctx->sc_wide_blocks, fuse01);
+                                    ctx->sc_tail_elems, wide, fuse01);
     for (size_t i = 0; i + 1 < jobs.size(); ++i) {
         const ScJob &J = jobs[i];
         if (J.kind == 1)
