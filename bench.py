@@ -348,6 +348,12 @@ def serving_leg(ctx, requests=1000, rate_hz=2000.0, sweep=(500.0, 1000.0, 2000.0
     for a, b in pool.values():
         a.free(); b.free()
     sat = max(c["throughput_proofs_per_s"] for c in curve)
+    # where the tail of the latency distribution comes from: per request size
+    by_size = {}
+    for e in sorted(set(exps)):
+        le = np.array([l for l, x in zip(lat, exps) if x == e])
+        by_size[f"2^{e}"] = {"requests": int(le.size), "p50_ms": round(float(np.percentile(le, 50)), 3),
+                             "p99_ms": round(float(np.percentile(le, 99)), 3)}
     return {"workload": f"{requests} requests, Poisson arrivals at {rate_hz:g}/s, n = 2^e with e uniform on 14..22 "
                         "(seed 0xC05A0005), RS blowup 4, HBM budget 16 GiB, 3 slots",
             "offered_per_s": rate_hz, "throughput_proofs_per_s": round(requests / wall, 1),
@@ -356,7 +362,7 @@ def serving_leg(ctx, requests=1000, rate_hz=2000.0, sweep=(500.0, 1000.0, 2000.0
             "mean_ms": round(float(lat.mean()), 3), "passes": st["passes"], "max_in_flight": st["max_inflight"],
             "arena_peak_gib": round(st["arena_peak_bytes"] / 2**30, 3),
             "proofs_identical_to_plain": f"{ok}/{requests}",
-            "saturation_proofs_per_s": sat, "sweep": curve}
+            "saturation_proofs_per_s": sat, "latency_by_size": by_size, "sweep": curve}
 
 
 def main():

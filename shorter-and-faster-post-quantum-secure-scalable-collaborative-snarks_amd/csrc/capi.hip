@@ -145,6 +145,10 @@ int cosnark_create(int device, cosnark_ctx **out) {
         const int b = atoi(e);
         if (b >= 1 && b <= SC_MAX_BLOCKS) ctx->sc_blocks = b;
     }
+    if (const char *e = getenv("COSNARK_SC_BATCH_BLOCKS")) {
+        const int b = atoi(e);
+        if (b >= 1 && b <= SC_MAX_BLOCKS) ctx->sc_batch_blocks = b;
+    }
     if (const char *e = getenv("COSNARK_SC_WIDE")) {
         const int b = atoi(e);
         if (b >= 1 && b <= SC_MAX_BLOCKS / 3) ctx->sc_wide_blocks = b;

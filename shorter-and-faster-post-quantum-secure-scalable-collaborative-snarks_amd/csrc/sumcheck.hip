@@ -1106,8 +1106,8 @@ int sumcheck_batch_plan(cosnark_ctx *ctx, int count, const fe *const *v1, const 
     size_t nr = 0;
     for (int i = 0; i < count; ++i) {
         const int m = ilog2(n[i]);
-        per[i] = sumcheck_jobs(v1[i], v2[i], n[i], m, buf[i], parts[i], st[i], ctx->sc_blocks, ctx->sc_tail_elems,
-                               ctx->sc_wide_blocks);
+        per[i] = sumcheck_jobs(v1[i], v2[i], n[i], m, buf[i], parts[i], st[i], ctx->sc_batch_blocks,
+                               ctx->sc_tail_elems, ctx->sc_wide_blocks);
         nr = std::max(nr, per[i].size() - 1);
     }
     plan.rounds.clear();
