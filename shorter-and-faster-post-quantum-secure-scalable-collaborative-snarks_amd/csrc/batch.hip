@@ -11,13 +11,17 @@
 // A batch is planned once for a list of sizes: device slots for every
 // proof's f and g, codeword, tree, sumcheck buffers and state, and the launch
 // tables.  One proving pass is
-//   stream 1: transcripts in; RS encode, one launch per NTT pass for all
-//             proofs of one size; Merkle, one launch per stage for all trees;
+//   streams 1 and 3: RS encode, one launch per NTT pass for all proofs of
+//             one size; Merkle, one launch per stage for all of the lane's
+//             trees;
 //   stream 2: the sumchecks, one launch per round for every proof still
 //             above the LDS tail, then one tail workgroup per proof;
 //   join; one kernel packs every proof's bytes into one buffer; one copy out.
-// The commits and the sumchecks are captured into one hipGraph each at the
-// first call and replayed after that, on their two streams.  Every buffer the graph touches belongs to the batch, so later work
+// The commits go in two lanes on two streams (the largest codewords on one,
+// the rest on the other, balanced by elements encoded), each lane hashing
+// its own trees.  The commit lanes and the sumchecks are captured into one
+// hipGraph each at the first call and replayed after that, on their three
+// streams.  Every buffer the graph touches belongs to the batch, so later work
 // on the context (which may regrow its own scratch) cannot invalidate it.
 #include <algorithm>
 #include <cstring>
@@ -75,7 +79,7 @@ size_t batch_work_bytes(uint64_t n, int rate, bool with_commit) {
 size_t batch_record_bytes(uint64_t n) { return cosnark_plain_proof_bytes(n) + 32; }
 
 int batch_core_build(cosnark_ctx *ctx, int count, const BatchProof *pr, int rate, fe *prev_dev, uint8_t *out_dev,
-                     BatchCore &C) {
+                     BatchCore &C, bool two_lanes) {
     C.count = count;
     C.rate = rate;
     C.n.resize(count);
@@ -130,8 +134,44 @@ int batch_core_build(cosnark_ctx *ctx, int count, const BatchProof *pr, int rate
         }
         C.groups.push_back(G);
     }
-    int rc = merkle_batch_plan(ctx, count, cws.data(), cwn.data(), nds.data(), true, C.tb, C.mk);
-    if (rc) return rc;
+    // Two commit lanes (two streams): the groups, largest first, each to the
+    // lane with less work so far (work ~ elements encoded).  The large
+    // groups fill the device by themselves; the small ones, a few
+    // workgroups per launch, then run beside them instead of one after
+    // another.  Each lane hashes its own trees.
+    C.glane.assign(C.groups.size(), 0);
+    C.two_lanes = false;
+    if (two_lanes && C.groups.size() >= 2) {
+        std::vector<int> order(C.groups.size());
+        for (size_t g = 0; g < order.size(); ++g) order[g] = (int)g;
+        auto work = [&](int g) { return (uint64_t)C.groups[g].B << C.groups[g].logn; };
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work(a) > work(b); });
+        uint64_t load[2] = {0, 0};
+        for (int g : order) {
+            const int l = load[1] < load[0] ? 1 : 0;
+            C.glane[g] = l;
+            load[l] += work(g);
+        }
+        for (int l : C.glane) C.two_lanes = C.two_lanes || l == 1;
+    }
+    std::vector<int> lane_of(count, 0);
+    for (size_t g = 0; g < C.groups.size(); ++g)
+        for (int i : by_log[C.groups[g].logn]) lane_of[i] = C.glane[g];
+    int rc = COSNARK_OK;
+    C.mk1 = MerkleBatchPlan{};
+    for (int l = 0; l < (C.two_lanes ? 2 : 1); ++l) {
+        std::vector<const fe *> e;
+        std::vector<uint64_t> ne;
+        std::vector<uint8_t *> nd;
+        for (int i = 0; i < count; ++i)
+            if (lane_of[i] == l) {
+                e.push_back(cws[i]);
+                ne.push_back(cwn[i]);
+                nd.push_back(nds[i]);
+            }
+        rc = merkle_batch_plan(ctx, (int)e.size(), e.data(), ne.data(), nd.data(), true, C.tb, l ? C.mk1 : C.mk);
+        if (rc) return rc;
+    }
     rc = sumcheck_batch_plan(ctx, count, fin.data(), gin.data(), C.n.data(), bufs.data(), parts.data(), sts.data(),
                              prev_dev, C.tb, C.sc);
     if (rc) return rc;
@@ -146,9 +186,10 @@ int batch_core_build(cosnark_ctx *ctx, int count, const BatchProof *pr, int rate
     return COSNARK_OK;
 }
 
-uint64_t batch_tmp_elems(const BatchCore &C) {
+uint64_t batch_tmp_elems(const BatchCore &C, int lane) {
     uint64_t t = 0;
-    for (const auto &G : C.groups) t = std::max(t, (uint64_t)G.B << G.logn);
+    for (size_t g = 0; g < C.groups.size(); ++g)
+        if (C.glane.empty() ? lane == 0 : C.glane[g] == lane) t = std::max(t, (uint64_t)C.groups[g].B << C.groups[g].logn);
     return t;
 }
 
@@ -161,9 +202,11 @@ int batch_enqueue_sumchecks(cosnark_ctx *ctx, const BatchCore &C, const uint8_t 
 }
 
 int batch_enqueue_commits(cosnark_ctx *ctx, const BatchCore &C, const uint8_t *tables, fe *tmp, uint64_t tmp_elems,
-                          hipStream_t s1) {
+                          hipStream_t s1, int lane) {
     int rc;
-    for (const auto &G : C.groups) {
+    for (size_t gi = 0; gi < C.groups.size(); ++gi) {
+        if ((C.glane.empty() ? 0 : C.glane[gi]) != lane) continue;
+        const auto &G = C.groups[gi];
         const uint64_t chunk = tmp_elems >> G.logn;
         if (chunk == 0) return set_err(ctx, COSNARK_E_RANGE, "batch: NTT temporary smaller than one codeword");
         for (int c0 = 0; c0 < G.B; c0 += (int)chunk) {
@@ -173,7 +216,7 @@ int batch_enqueue_commits(cosnark_ctx *ctx, const BatchCore &C, const uint8_t *t
             if (rc) return rc;
         }
     }
-    return merkle_batch_run(ctx, C.mk, tables, s1);
+    return merkle_batch_run(ctx, lane ? C.mk1 : C.mk, tables, s1);
 }
 
 int batch_enqueue_pack(cosnark_ctx *ctx, const BatchCore &C, const uint8_t *tables, uint8_t *out_host,
@@ -186,15 +229,24 @@ int batch_enqueue_pack(cosnark_ctx *ctx, const BatchCore &C, const uint8_t *tabl
 
 int batch_core_enqueue(cosnark_ctx *ctx, const BatchCore &C, const uint8_t *tables, const fe *prev_host,
                        uint8_t *out_host, fe *tmp, uint64_t tmp_elems, hipStream_t s1, hipStream_t s2,
-                       hipEvent_t fork, hipEvent_t join) {
+                       hipEvent_t fork, hipEvent_t join, uint64_t tmp_elems1, hipStream_t s3, hipEvent_t join3) {
+    const bool lane1 = C.two_lanes;
+    if (lane1 && (!s3 || !join3 || tmp_elems1 == 0))
+        return set_err(ctx, COSNARK_E_ARG, "batch: two commit lanes need a third stream and their temporary");
     // the commits (the longer branch) are issued first, so they start at once
     HIPCK(hipEventRecord(fork, s1));
     HIPCK(hipStreamWaitEvent(s2, fork, 0));
-    int rc = batch_enqueue_commits(ctx, C, tables, tmp, tmp_elems, s1);
+    if (lane1) HIPCK(hipStreamWaitEvent(s3, fork, 0));
+    int rc = batch_enqueue_commits(ctx, C, tables, tmp, tmp_elems, s1, 0);
     if (rc) return rc;
+    if (lane1 && (rc = batch_enqueue_commits(ctx, C, tables, tmp + tmp_elems, tmp_elems1, s3, 1))) return rc;
     if ((rc = batch_enqueue_sumchecks(ctx, C, tables, prev_host, s2))) return rc;
     HIPCK(hipEventRecord(join, s2));
     HIPCK(hipStreamWaitEvent(s1, join, 0));
+    if (lane1) {
+        HIPCK(hipEventRecord(join3, s3));
+        HIPCK(hipStreamWaitEvent(s1, join3, 0));
+    }
     return batch_enqueue_pack(ctx, C, tables, out_host, s1);
 }
 
@@ -222,14 +274,14 @@ struct cosnark_batch {
     std::vector<size_t> off_in;
     fe *f = nullptr, *g = nullptr, *tmp = nullptr, *prev_dev = nullptr, *prev_host = nullptr;
     uint8_t *work = nullptr, *tables = nullptr, *out_dev = nullptr, *out_host = nullptr;
-    uint64_t tmp_elems = 0;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_t0 = nullptr, ev_t1 = nullptr;
+    uint64_t tmp_elems = 0, tmp_elems1 = 0;   // NTT temporary of commit lanes 0 and 1 (lane 1's after lane 0's)
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_join3 = nullptr, ev_t0 = nullptr, ev_t1 = nullptr;
     bool use_graph = true;
-    // two graphs, one per stream: replaying one graph with both branches ran
-    // them one after the other, so the commits and the sumchecks are
+    // one graph per stream: replaying one graph with several branches ran
+    // them one after the other, so the commit lanes and the sumchecks are
     // captured apart and joined by events around the launches
-    hipGraph_t graph = nullptr, graph_sc = nullptr;
-    hipGraphExec_t exec = nullptr, exec_sc = nullptr;
+    hipGraph_t graph = nullptr, graph_sc = nullptr, graph1 = nullptr;
+    hipGraphExec_t exec = nullptr, exec_sc = nullptr, exec1 = nullptr;
 };
 
 namespace {
@@ -241,12 +293,14 @@ void batch_release(cosnark_batch *b) {
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->exec_sc) (void)hipGraphExecDestroy(b->exec_sc);
     if (b->graph_sc) (void)hipGraphDestroy(b->graph_sc);
+    if (b->exec1) (void)hipGraphExecDestroy(b->exec1);
+    if (b->graph1) (void)hipGraphDestroy(b->graph1);
     for (void *p : {(void *)b->f, (void *)b->g, (void *)b->tmp, (void *)b->prev_dev, (void *)b->work,
                     (void *)b->tables, (void *)b->out_dev})
         if (p) (void)hipFree(p);
     if (b->out_host) (void)hipHostFree(b->out_host);
     if (b->prev_host) (void)hipHostFree(b->prev_host);
-    for (hipEvent_t e : {b->ev_fork, b->ev_join, b->ev_t0, b->ev_t1})
+    for (hipEvent_t e : {b->ev_fork, b->ev_join, b->ev_join3, b->ev_t0, b->ev_t1})
         if (e) (void)hipEventDestroy(e);
     delete b;
 }
@@ -254,7 +308,7 @@ void batch_release(cosnark_batch *b) {
 int batch_enqueue(cosnark_batch *b) {
     cosnark_ctx *ctx = b->ctx;
     return batch_core_enqueue(ctx, b->core, b->tables, b->prev_host, b->out_host, b->tmp, b->tmp_elems, ctx->stream,
-                              ctx->stream2, b->ev_fork, b->ev_join);
+                              ctx->stream2, b->ev_fork, b->ev_join, b->tmp_elems1, ctx->stream3, b->ev_join3);
 }
 
 template <class Enqueue>
@@ -275,19 +329,33 @@ int capture_one(cosnark_ctx *ctx, hipStream_t s, Enqueue enqueue, hipGraph_t *gr
 
 int batch_capture(cosnark_batch *b) {
     cosnark_ctx *ctx = b->ctx;
+    auto drop = [](hipGraph_t &g, hipGraphExec_t &e) {
+        if (e) (void)hipGraphExecDestroy(e);
+        if (g) (void)hipGraphDestroy(g);
+        e = nullptr;
+        g = nullptr;
+    };
     int rc = capture_one(
         ctx, ctx->stream,
-        [&]() { return batch_enqueue_commits(ctx, b->core, b->tables, b->tmp, b->tmp_elems, ctx->stream); },
+        [&]() { return batch_enqueue_commits(ctx, b->core, b->tables, b->tmp, b->tmp_elems, ctx->stream, 0); },
         &b->graph, &b->exec);
     if (rc) return rc;
-    rc = capture_one(
-        ctx, ctx->stream2, [&]() { return batch_enqueue_sumchecks(ctx, b->core, b->tables, b->prev_host, ctx->stream2); },
-        &b->graph_sc, &b->exec_sc);
-    if (rc) {   // both graphs or neither
-        (void)hipGraphExecDestroy(b->exec);
-        (void)hipGraphDestroy(b->graph);
-        b->exec = nullptr;
-        b->graph = nullptr;
+    if (b->core.two_lanes)
+        rc = capture_one(
+            ctx, ctx->stream3,
+            [&]() {
+                return batch_enqueue_commits(ctx, b->core, b->tables, b->tmp + b->tmp_elems, b->tmp_elems1,
+                                             ctx->stream3, 1);
+            },
+            &b->graph1, &b->exec1);
+    if (!rc)
+        rc = capture_one(
+            ctx, ctx->stream2,
+            [&]() { return batch_enqueue_sumchecks(ctx, b->core, b->tables, b->prev_host, ctx->stream2); },
+            &b->graph_sc, &b->exec_sc);
+    if (rc) {   // every graph or none
+        drop(b->graph, b->exec);
+        drop(b->graph1, b->exec1);
     }
     return rc;
 }
@@ -327,6 +395,7 @@ int cosnark_batch_create(cosnark_ctx *ctx, int count, const uint64_t *sizes, int
         return nomem();
     if (hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->ev_join3, hipEventDisableTiming) != hipSuccess ||
         hipEventCreate(&b->ev_t0) != hipSuccess || hipEventCreate(&b->ev_t1) != hipSuccess)
         return fail(hip_err(ctx, hipErrorUnknown, "batch_create: events"));
     std::vector<BatchProof> pr(count);
@@ -337,14 +406,19 @@ int cosnark_batch_create(cosnark_ctx *ctx, int count, const uint64_t *sizes, int
         pr[i].work = b->work + off_work[i];
         b->prev_host[i] = gf::zero();
     }
-    int rc = batch_core_build(ctx, count, pr.data(), rate, b->prev_dev, b->out_dev, b->core);
+    int rc = batch_core_build(ctx, count, pr.data(), rate, b->prev_dev, b->out_dev, b->core, true);
     if (rc) return fail(rc);
-    // one NTT temporary for every group, capped at 2^26 elements (groups run
-    // in chunks that fit)
-    uint64_t maxn = 0;
-    for (const auto &G : b->core.groups) maxn = std::max<uint64_t>(maxn, 1ull << G.logn);
-    b->tmp_elems = std::max(maxn, std::min<uint64_t>(batch_tmp_elems(b->core), 1ull << 26));
-    if (hipMalloc(&b->tmp, sizeof(fe) * b->tmp_elems) != hipSuccess ||
+    // one NTT temporary per commit lane, each capped at 2^26 elements (a
+    // lane's groups run in chunks that fit)
+    auto lane_tmp = [&](int lane) {
+        uint64_t maxn = 0;
+        for (size_t g = 0; g < b->core.groups.size(); ++g)
+            if (b->core.glane[g] == lane) maxn = std::max<uint64_t>(maxn, 1ull << b->core.groups[g].logn);
+        return maxn ? std::max(maxn, std::min<uint64_t>(batch_tmp_elems(b->core, lane), 1ull << 26)) : 0;
+    };
+    b->tmp_elems = lane_tmp(0);
+    b->tmp_elems1 = b->core.two_lanes ? lane_tmp(1) : 0;
+    if (hipMalloc(&b->tmp, sizeof(fe) * (b->tmp_elems + b->tmp_elems1)) != hipSuccess ||
         hipMalloc(&b->tables, b->core.tb.host.size()) != hipSuccess)
         return nomem();
     if (hipMemcpy(b->tables, b->core.tb.host.data(), b->core.tb.host.size(), hipMemcpyHostToDevice) != hipSuccess)
@@ -359,6 +433,7 @@ void cosnark_batch_destroy(cosnark_batch *b) {
         (void)hipSetDevice(b->ctx->device);
         (void)hipStreamSynchronize(b->ctx->stream);
         (void)hipStreamSynchronize(b->ctx->stream2);
+        (void)hipStreamSynchronize(b->ctx->stream3);
     }
     batch_release(b);
 }
@@ -402,12 +477,21 @@ int cosnark_batch_prove(cosnark_batch *b, cosnark_transcript *ts, uint8_t *h_pro
     if (b->use_graph && !b->exec && (rc = batch_capture(b))) return rc;
     HIPCK(hipEventRecord(b->ev_t0, ctx->stream));
     if (b->use_graph) {
+        const bool lane1 = b->exec1 != nullptr;
         HIPCK(hipEventRecord(b->ev_fork, ctx->stream));
         HIPCK(hipStreamWaitEvent(ctx->stream2, b->ev_fork, 0));
-        HIPCK(hipGraphLaunch(b->exec, ctx->stream));
+        if (lane1) HIPCK(hipStreamWaitEvent(ctx->stream3, b->ev_fork, 0));
+        // the sumchecks' long chain of dependent rounds first: a graph
+        // launch takes host time, and the later graphs' kernels start later
         HIPCK(hipGraphLaunch(b->exec_sc, ctx->stream2));
+        if (lane1) HIPCK(hipGraphLaunch(b->exec1, ctx->stream3));
+        HIPCK(hipGraphLaunch(b->exec, ctx->stream));
         HIPCK(hipEventRecord(b->ev_join, ctx->stream2));
         HIPCK(hipStreamWaitEvent(ctx->stream, b->ev_join, 0));
+        if (lane1) {
+            HIPCK(hipEventRecord(b->ev_join3, ctx->stream3));
+            HIPCK(hipStreamWaitEvent(ctx->stream, b->ev_join3, 0));
+        }
         if ((rc = batch_enqueue_pack(ctx, b->core, b->tables, b->out_host, ctx->stream))) return rc;
     } else if ((rc = batch_enqueue(b))) {
         return rc;

@@ -141,6 +141,7 @@ int cosnark_create(int device, cosnark_ctx **out) {
     }
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return fail(COSNARK_E_HIP);
     if (hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) != hipSuccess) return fail(COSNARK_E_HIP);
+    if (hipStreamCreateWithFlags(&ctx->stream3, hipStreamNonBlocking) != hipSuccess) return fail(COSNARK_E_HIP);
     if (const char *e = getenv("COSNARK_SC_BLOCKS")) {
         const int b = atoi(e);
         if (b >= 1 && b <= SC_MAX_BLOCKS) ctx->sc_blocks = b;
@@ -231,6 +232,7 @@ void cosnark_destroy(cosnark_ctx *ctx) {
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     if (ctx->ntt_tmp) (void)hipFree(ctx->ntt_tmp);
     if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
+    if (ctx->stream3) (void)hipStreamDestroy(ctx->stream3);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

@@ -35,6 +35,7 @@ struct cosnark_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipStream_t stream2 = nullptr;    // concurrent stage (sumcheck beside the commit)
+    hipStream_t stream3 = nullptr;    // batched prover: the second lane of commits
     std::string err;
     // twiddle tables (device): omega_{2^28}^(e & mask), omega_{2^28}^((e >> 14) << 14),
     // and the same for the inverse root; radix tables omega_1024^j (j < 512).
@@ -233,7 +234,10 @@ struct BatchCore {
         size_t in_tab, out_tab;      // table offsets: input and codeword pointers
     };
     std::vector<Group> groups;
-    MerkleBatchPlan mk;
+    std::vector<int> glane;          // per group: commit lane 0 or 1
+    bool two_lanes = false;          // lane 1 holds some groups (runs on its own stream)
+    MerkleBatchPlan mk;              // lane 0's trees (every tree with one lane)
+    MerkleBatchPlan mk1;             // lane 1's trees
     ScBatchPlan sc;
     size_t pack = 0;
     TableBuf tb;
@@ -246,14 +250,19 @@ size_t align256(size_t x);
 size_t batch_work_bytes(uint64_t n, int rate, bool with_commit = true);
 size_t batch_record_bytes(uint64_t n);
 int batch_core_build(cosnark_ctx *ctx, int count, const BatchProof *pr, int rate, gf::fe *prev_dev,
-                     uint8_t *out_dev, BatchCore &C);
-uint64_t batch_tmp_elems(const BatchCore &C);
+                     uint8_t *out_dev, BatchCore &C, bool two_lanes = false);
+// NTT temporary that lane `lane`'s groups need at once (0 for an empty lane)
+uint64_t batch_tmp_elems(const BatchCore &C, int lane = 0);
 // The proving pass: prev_host (pinned) -> prev_dev; sumchecks on s2; NTT
 // groups (in chunks of tmp_elems) and trees on s1; join; records -> out_host
 // (pinned).  tables: the device copy of C.tb.
 int batch_core_enqueue(cosnark_ctx *ctx, const BatchCore &C, const uint8_t *tables, const gf::fe *prev_host,
                        uint8_t *out_host, gf::fe *tmp, uint64_t tmp_elems, hipStream_t s1, hipStream_t s2,
-                       hipEvent_t fork, hipEvent_t join);
+                       hipEvent_t fork, hipEvent_t join, uint64_t tmp_elems1 = 0, hipStream_t s3 = nullptr,
+                       hipEvent_t join3 = nullptr);
+// one lane's commits: its NTT groups (in chunks of tmp_elems) and its trees
+int batch_enqueue_commits(cosnark_ctx *ctx, const BatchCore &C, const uint8_t *tables, gf::fe *tmp,
+                          uint64_t tmp_elems, hipStream_t s, int lane);
 // proof i's bytes (to proofs[i] when non-null), transcript and self-check
 // flag out of the copied records; returns the first failed proof or -1
 int batch_unpack(const BatchCore &C, const uint8_t *out_host, uint8_t *const *proofs, cosnark_transcript *ts,
