@@ -588,44 +588,55 @@ __global__ __launch_bounds__(CO_THREADS) void bsc3_512_kernel(const fe *__restri
         for (int j = 0; j < 2; ++j) part[(uint64_t)blockIdx.x * 2048 + c * 512 + t + 256 * j] = acc[c][j];
 }
 
-// Cubic block sum for d <= 16, transforms in registers: a workgroup takes
-// 64 blocks per tile; wave v (0..3: p1, p2, p3, b1) holds vector v of the
-// tile's blocks, one block per lane.  A lane takes its block's inverse
-// transform once, then for each coset s = 1, 2, 3 the d-point transform of
-// the coefficients twisted by w_4d^(s m) / d (compile-time roots); coset 0
-// is the block itself.  Each coset's values meet in LDS, where thread t
-// forms (P1 P2 - P4) P3 for the points k = t mod d of blocks t/d + 256j/d
-// (64 d pairs per coset: for d = 2 threads 128..255 form none).
+// Cubic block sum for d <= 16, transforms in registers.  Every wave works
+// on its own: it takes 16 blocks, lane l = 16 v + b holding vector v (p1,
+// p2, p3, b1) of block b.  A lane takes its block's inverse transform once,
+// then for each coset s = 1, 2, 3 the d-point transform of the coefficients
+// twisted by w_4d^(s m) / d (compile-time roots); coset 0 is the block
+// itself.  Each coset's values meet in the wave's own LDS rows, so the waves
+// synchronise only with themselves (no workgroup barrier until the final
+// column sums) and one wave's transforms overlap another's products.  Lane
+// l forms (P1 P2 - P4) P3 for point k = l mod d of blocks l/d + (64/d) j
+// of its wave.
+__device__ __forceinline__ void co_wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int D>
-__global__ __launch_bounds__(CO_THREADS) void bsc3_small_kernel(const fe *__restrict__ p1, const fe *__restrict__ p2,
-                                                                const fe *__restrict__ p3, const fe *__restrict__ b1,
-                                                                uint64_t nblocks, fe *__restrict__ part) {
+__global__ __launch_bounds__(CO_THREADS) void bsc3_wave_kernel(const fe *__restrict__ p1, const fe *__restrict__ p2,
+                                                               const fe *__restrict__ p3, const fe *__restrict__ b1,
+                                                               uint64_t nblocks, fe *__restrict__ part) {
     extern __shared__ fe S[];
-    constexpr int RP = D + 1, BT = 64, VS = BT * RP, NP = BT * D, J = (NP + CO_THREADS - 1) / CO_THREADS;
-    __shared__ int live[BT];
-    const int t = threadIdx.x, lane = t & (BT - 1), vec = t / BT;
+    constexpr int WB = 16, RP = D + 1, VS = WB * RP, WS = 4 * VS;   // blocks per wave, row pitch, vector and wave spans
+    constexpr int NP = WB * D, J = (NP + 63) / 64;                   // (block, point) pairs per coset and wave
+    __shared__ int live[CO_THREADS / 64][WB];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int vec = lane / WB, b = lane & (WB - 1);
     const fe *src = vec == 0 ? p1 : vec == 1 ? p2 : vec == 2 ? p3 : b1;
-    fe *row = S + vec * VS + lane * RP;
-    const int k = t & (D - 1);
+    fe *rows = S + w * WS;
+    fe *row = rows + vec * VS + b * RP;
+    const int k = lane & (D - 1);
     fe acc[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) acc[c] = gf::zero();
-    // (P1 P2 - P4) P3 at point k of blocks t/D + j * 256/D
     auto products = [&](fe &a) {
 #pragma unroll
         for (int j = 0; j < J; ++j) {
-            const int p = t + CO_THREADS * j;       // (block, point) pair; 128 pairs for d = 2
+            const int p = lane + 64 * j;
             if (p >= NP) break;
-            const int b = p / D;
-            if (live[b]) {
-                const fe *c = S + b * RP + k;
+            const int bb = p / D;
+            if (live[w][bb]) {
+                const fe *c = rows + bb * RP + k;
                 a = gf::add(a, gf::mul(gf::sub(gf::mul(c[0], c[VS]), c[2 * VS]), c[3 * VS]));
             }
         }
     };
-    const uint64_t ntiles = (nblocks + BT - 1) / BT;
-    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const uint64_t blk = tile * BT + lane;
+    const uint64_t ngroups = (nblocks + WB - 1) / WB;
+    const uint64_t gw = (uint64_t)blockIdx.x * (CO_THREADS / 64) + w, nw = (uint64_t)gridDim.x * (CO_THREADS / 64);
+    for (uint64_t grp = gw; grp < ngroups; grp += nw) {
+        const uint64_t blk = grp * WB + b;
         fe x[D];
         int nz = 0;
 #pragma unroll
@@ -634,12 +645,11 @@ __global__ __launch_bounds__(CO_THREADS) void bsc3_small_kernel(const fe *__rest
             nz |= (x[i].re | x[i].im) != 0;
             row[i] = x[i];
         }
-        if (vec == 0) live[lane] = nz;     // blocks whose p1 part is zero are skipped
-        __syncthreads();
+        if (vec == 0) live[w][b] = nz;     // blocks whose p1 part is zero are skipped
+        co_wave_sync();
         products(acc[0]);
-        __syncthreads();
-        // weak arithmetic (gf.h) through the transforms; canonical values in LDS
-        dft::dif_w<D, true>(x);                                 // x[i] = D c[brev(i)]
+        co_wave_sync();                     // the rows are rewritten below
+        dft::dif_w<D, true>(x);             // x[i] = D c[brev(i)]
         constexpr int LOGD = D >= 16 ? 4 : D >= 8 ? 3 : D >= 4 ? 2 : 1;
 #pragma unroll
         for (int i = 0; i < D; ++i) x[i] = gf::mul_pow2_w<61 - LOGD>(x[i]);   // / D: a rotation
@@ -651,19 +661,18 @@ __global__ __launch_bounds__(CO_THREADS) void bsc3_small_kernel(const fe *__rest
                 constexpr int m = dft::brev_c<D>(i);
                 y[m] = gf::mul_root_w<4 * D, co * m>(x[i]);
             });
-            dft::dif_w<D, false>(y);                            // y[i] = P(w_4d^(4 brev(i) + co))
+            dft::dif_w<D, false>(y);        // y[i] = P(w_4d^(4 brev(i) + co))
             dft::static_for<0, D>([&](auto ii) {
                 constexpr int i = decltype(ii)::value;
                 row[dft::brev_c<D>(i)] = gf::canon_w(y[i]);
             });
-            __syncthreads();
+            co_wave_sync();
             products(acc[co]);
-            __syncthreads();
+            co_wave_sync();
         });
     }
-    // column (co, k) adds the threads with t mod D = k (S holds at least
-    // 4 * CO_THREADS elements: the launch sizes it for this as well)
-    static_assert(4 * BT * RP >= 4 * CO_THREADS || D < 4, "launch size covers the reduction");
+    // column (co, k) adds the threads with t mod D = k
+    __syncthreads();
 #pragma unroll
     for (int c = 0; c < 4; ++c) S[c * CO_THREADS + t] = acc[c];
     __syncthreads();
@@ -936,7 +945,7 @@ int copcs_init(cosnark_ctx *) {
                             (int)(sizeof(fe) * L256_LDS)) != hipSuccess ||
         hipFuncSetAttribute((const void *)multifold_large_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                             mf_lds) != hipSuccess ||
-        hipFuncSetAttribute((const void *)bsc3_small_kernel<16>, hipFuncAttributeMaxDynamicSharedMemorySize,
+        hipFuncSetAttribute((const void *)bsc3_wave_kernel<16>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)(sizeof(fe) * 4 * 64 * 17)) != hipSuccess ||
         hipFuncSetAttribute((const void *)bsc3_512_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)(sizeof(fe) * C512_LDS)) != hipSuccess ||
@@ -1152,10 +1161,10 @@ int cosnark_cosumcheck3_phase1(cosnark_ctx *ctx, const uint64_t *d_p1, const uin
         // the tile's rows, and the four per-thread coset sums at the end
         const size_t lds = sizeof(fe) * std::max<size_t>(4 * 64 * (d + 1), 4 * CO_THREADS);
         switch (d) {
-            case 2: hipLaunchKernelGGL(bsc3_small_kernel<2>, dim3(nwg), dim3(CO_THREADS), lds, s, a1, a2, a3, a4, nb, part); break;
-            case 4: hipLaunchKernelGGL(bsc3_small_kernel<4>, dim3(nwg), dim3(CO_THREADS), lds, s, a1, a2, a3, a4, nb, part); break;
-            case 8: hipLaunchKernelGGL(bsc3_small_kernel<8>, dim3(nwg), dim3(CO_THREADS), lds, s, a1, a2, a3, a4, nb, part); break;
-            default: hipLaunchKernelGGL(bsc3_small_kernel<16>, dim3(nwg), dim3(CO_THREADS), lds, s, a1, a2, a3, a4, nb, part); break;
+            case 2: hipLaunchKernelGGL(bsc3_wave_kernel<2>, dim3(nwg), dim3(CO_THREADS), lds, s, a1, a2, a3, a4, nb, part); break;
+            case 4: hipLaunchKernelGGL(bsc3_wave_kernel<4>, dim3(nwg), dim3(CO_THREADS), lds, s, a1, a2, a3, a4, nb, part); break;
+            case 8: hipLaunchKernelGGL(bsc3_wave_kernel<8>, dim3(nwg), dim3(CO_THREADS), lds, s, a1, a2, a3, a4, nb, part); break;
+            default: hipLaunchKernelGGL(bsc3_wave_kernel<16>, dim3(nwg), dim3(CO_THREADS), lds, s, a1, a2, a3, a4, nb, part); break;
         }
     } else if (d == 512) {
         hipLaunchKernelGGL(bsc3_512_kernel, dim3(nwg), dim3(CO_THREADS), sizeof(fe) * C512_LDS, s, a1, a2, a3, a4,
