@@ -354,11 +354,22 @@ __global__ __launch_bounds__(SC_THREADS) void sc_pair01_kernel(ScJob J) {
 #pragma unroll
     for (int i = 0; i < 9; ++i) q[i] = gf::zero();
     const uint64_t stride = (uint64_t)J.nb * SC_THREADS;
-    for (uint64_t j = (uint64_t)blockIdx.x * SC_THREADS + threadIdx.x; j < nq; j += stride) {
-        fe x[4], y[4];
+    // the next quadruples' loads are issued before this one's products
+    uint64_t j = (uint64_t)blockIdx.x * SC_THREADS + threadIdx.x;
+    fe x[4], y[4];
+    if (j < nq)
 #pragma unroll
         for (int e = 0; e < 4; ++e) { x[e] = s1[4 * j + e]; y[e] = s2[4 * j + e]; }
+    while (j < nq) {
+        const uint64_t jn = j + stride;
+        fe xn[4], yn[4];
+        if (jn < nq)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) { xn[e] = s1[4 * jn + e]; yn[e] = s2[4 * jn + e]; }
         quad_acc(q, x, y);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) { x[e] = xn[e]; y[e] = yn[e]; }
+        j = jn;
     }
     block_partials<9, SC_THREADS>(q, J.part_out + 9 * blockIdx.x, red);
 }
