@@ -168,6 +168,7 @@ int whir_init(cosnark_ctx *ctx);
 // transcript starts from prev[i] and whose claim comes from round 0.
 struct ScBatchPlan {
     struct Launch {
+        int kind;                 // 0: one round (fold: FOLD form); 1: rounds 0-1; 2, 3: two rounds (direct, quadratic)
         bool fold;
         uint32_t blocks;
         size_t jobs, blk_job;     // table offsets

@@ -345,7 +345,7 @@ __device__ __forceinline__ fe quad_eval(const fe &f0, const fe &f1, const fe &f2
 // coefficients: A0 = q5 + q8 (the pair differences' products), C0 = q3 + q6
 // (values at r0 = 0), O0 = q4 + q7 (values at r0 = 1).  J.npairs =
 // quadruples; nine partials per workgroup go to part_out.
-__global__ __launch_bounds__(SC_THREADS) void sc_pair01_kernel(ScJob J) {
+__device__ __forceinline__ void sc_pair01(const ScJob &J, uint32_t blk) {
     __shared__ fe red[SC_THREADS / 64][9];
     const fe *__restrict__ s1 = J.s1;
     const fe *__restrict__ s2 = J.s2;
@@ -355,7 +355,7 @@ __global__ __launch_bounds__(SC_THREADS) void sc_pair01_kernel(ScJob J) {
     for (int i = 0; i < 9; ++i) q[i] = gf::zero();
     const uint64_t stride = (uint64_t)J.nb * SC_THREADS;
     // the next quadruples' loads are issued before this one's products
-    uint64_t j = (uint64_t)blockIdx.x * SC_THREADS + threadIdx.x;
+    uint64_t j = (uint64_t)blk * SC_THREADS + threadIdx.x;
     fe x[4], y[4];
     if (j < nq)
 #pragma unroll
@@ -371,7 +371,16 @@ __global__ __launch_bounds__(SC_THREADS) void sc_pair01_kernel(ScJob J) {
         for (int e = 0; e < 4; ++e) { x[e] = xn[e]; y[e] = yn[e]; }
         j = jn;
     }
-    block_partials<9, SC_THREADS>(q, J.part_out + 9 * blockIdx.x, red);
+    block_partials<9, SC_THREADS>(q, J.part_out + 9 * blk, red);
+}
+
+__global__ __launch_bounds__(SC_THREADS) void sc_pair01_kernel(ScJob J) { sc_pair01(J, blockIdx.x); }
+
+// a batch launch: workgroup b works on job blk_job[b], block b - first_block of it
+__global__ __launch_bounds__(SC_THREADS) void sc_pair01_batch_kernel(const ScJob *__restrict__ jobs,
+                                                                    const uint16_t *__restrict__ blk_job) {
+    const ScJob &J = jobs[blk_job[blockIdx.x]];
+    sc_pair01(J, blockIdx.x - J.first_block);
 }
 
 // Two rounds per launch.  The input v(i) (J.round = i) comes with nine
@@ -394,7 +403,7 @@ __global__ __launch_bounds__(SC_THREADS) void sc_pair01_kernel(ScJob J) {
 // and ends like any multi-workgroup round: v(k) in memory, round k's sums in
 // three partials per workgroup.
 template <bool QUAD>
-__global__ __launch_bounds__(SC_THREADS) void sc_fold2_kernel(ScJob J) {
+__device__ __forceinline__ void sc_fold2(const ScJob &J, uint32_t blk) {
     constexpr int K = QUAD ? 9 : 3;          // partials per workgroup written
     constexpr int NL = QUAD ? 4 : 8;         // elements of v(i) per lane and vector
     __shared__ fe red[SC_THREADS / 64][K];
@@ -406,7 +415,7 @@ __global__ __launch_bounds__(SC_THREADS) void sc_fold2_kernel(ScJob J) {
     const uint64_t lanes = QUAD ? 4 * J.npairs : J.npairs;   // lane-items
     const uint64_t stride = (uint64_t)J.nb * SC_THREADS;     // a multiple of 4: a lane keeps its place in its quad
     const int round = J.round;
-    uint64_t t = (uint64_t)blockIdx.x * SC_THREADS + threadIdx.x;
+    uint64_t t = (uint64_t)blk * SC_THREADS + threadIdx.x;
     fe a[NL], b[NL];
     auto load = [&](uint64_t u) {
 #pragma unroll
@@ -421,7 +430,7 @@ __global__ __launch_bounds__(SC_THREADS) void sc_fold2_kernel(ScJob J) {
         wave_partials<9>(J.part_in, J.nb_in, q);
         if (threadIdx.x < 4) {
             __builtin_amdgcn_s_setprio(3);
-            const bool writer = blockIdx.x == 0;
+            const bool writer = blk == 0;
             const fe A0 = gf::add(q[5], q[8]), C0 = gf::add(q[3], q[6]), O0 = gf::add(q[4], q[7]);
             const fe r0 = finalize_round(st, round, J.fs_rounds, claim, prev, yv, A0, C0, O0, writer);
             const fe A1 = quad_eval(q[0], q[1], q[2], r0), C1 = quad_eval(q[3], q[4], q[5], r0),
@@ -485,7 +494,19 @@ __global__ __launch_bounds__(SC_THREADS) void sc_fold2_kernel(ScJob J) {
             v[2] = gf::add(v[2], gf::mul(o1, o2));
         }
     }
-    block_partials<K, SC_THREADS>(v, J.part_out + K * blockIdx.x, red);
+    block_partials<K, SC_THREADS>(v, J.part_out + K * blk, red);
+}
+
+template <bool QUAD>
+__global__ __launch_bounds__(SC_THREADS) void sc_fold2_kernel(ScJob J) {
+    sc_fold2<QUAD>(J, blockIdx.x);
+}
+
+template <bool QUAD>
+__global__ __launch_bounds__(SC_THREADS) void sc_fold2_batch_kernel(const ScJob *__restrict__ jobs,
+                                                                   const uint16_t *__restrict__ blk_job) {
+    const ScJob &J = jobs[blk_job[blockIdx.x]];
+    sc_fold2<QUAD>(J, blockIdx.x - J.first_block);
 }
 
 template <bool FOLD>
@@ -1118,28 +1139,36 @@ int sumcheck_batch_plan(cosnark_ctx *ctx, int count, const fe *const *v1, const 
     for (int i = 0; i < count; ++i) {
         const int m = ilog2(n[i]);
         per[i] = sumcheck_jobs(v1[i], v2[i], n[i], m, buf[i], parts[i], st[i], ctx->sc_batch_blocks,
-                               ctx->sc_tail_elems, ctx->sc_wide_blocks);
+                               ctx->sc_tail_elems, ctx->sc_batch_blocks, true);
         nr = std::max(nr, per[i].size() - 1);
     }
     plan.rounds.clear();
     plan.count = count;
     plan.prev = prev;
+    // step r holds every proof's r-th launch; proofs at one step may need
+    // different kernels (a first-rounds launch, a two-round launch of either
+    // form, a single round), so a step is one launch per kernel present
     for (size_t r = 0; r < nr; ++r) {
-        std::vector<ScJob> jobs;
-        std::vector<uint16_t> map;
-        for (int i = 0; i < count; ++i) {
-            if (r + 1 >= per[i].size()) continue;
-            ScJob J = per[i][r];
-            J.first_block = (uint32_t)map.size();
-            for (int b = 0; b < J.nb; ++b) map.push_back((uint16_t)jobs.size());
-            jobs.push_back(J);
-        }
-        if (jobs.size() > 0xFFFF) return set_err(ctx, COSNARK_E_RANGE, "sumcheck batch: too many proofs");
-        ScBatchPlan::Launch L{r > 0, (uint32_t)map.size(), tb.reserve(sizeof(ScJob) * jobs.size()), 0};
-        L.blk_job = tb.reserve(sizeof(uint16_t) * map.size());
-        memcpy(tb.at<ScJob>(L.jobs), jobs.data(), sizeof(ScJob) * jobs.size());
-        memcpy(tb.at<uint16_t>(L.blk_job), map.data(), sizeof(uint16_t) * map.size());
-        plan.rounds.push_back(L);
+        for (int kind = 0; kind < 4; ++kind)
+            for (int fold = 0; fold < (kind == 0 ? 2 : 1); ++fold) {
+                std::vector<ScJob> jobs;
+                std::vector<uint16_t> map;
+                for (int i = 0; i < count; ++i) {
+                    if (r + 1 >= per[i].size()) continue;
+                    ScJob J = per[i][r];
+                    if (J.kind != kind || (kind == 0 && (J.round > 0) != (fold == 1))) continue;
+                    J.first_block = (uint32_t)map.size();
+                    for (int b = 0; b < J.nb; ++b) map.push_back((uint16_t)jobs.size());
+                    jobs.push_back(J);
+                }
+                if (jobs.empty()) continue;
+                if (jobs.size() > 0xFFFF) return set_err(ctx, COSNARK_E_RANGE, "sumcheck batch: too many proofs");
+                ScBatchPlan::Launch L{kind, fold == 1, (uint32_t)map.size(), tb.reserve(sizeof(ScJob) * jobs.size()), 0};
+                L.blk_job = tb.reserve(sizeof(uint16_t) * map.size());
+                memcpy(tb.at<ScJob>(L.jobs), jobs.data(), sizeof(ScJob) * jobs.size());
+                memcpy(tb.at<uint16_t>(L.blk_job), map.data(), sizeof(uint16_t) * map.size());
+                plan.rounds.push_back(L);
+            }
     }
     plan.tail = tb.reserve(sizeof(ScJob) * count);
     for (int i = 0; i < count; ++i) tb.at<ScJob>(plan.tail)[i] = per[i].back();
@@ -1152,12 +1181,18 @@ int sumcheck_batch_run(cosnark_ctx *ctx, const ScBatchPlan &plan, const uint8_t 
                        plan.count);
     LAUNCHCK();
     for (const auto &L : plan.rounds) {
-        if (L.fold)
-            hipLaunchKernelGGL(sc_round_batch_kernel<true>, dim3(L.blocks), dim3(SC_THREADS), 0, stream,
-                               (const ScJob *)(tables + L.jobs), (const uint16_t *)(tables + L.blk_job));
-        else
-            hipLaunchKernelGGL(sc_round_batch_kernel<false>, dim3(L.blocks), dim3(SC_THREADS), 0, stream,
-                               (const ScJob *)(tables + L.jobs), (const uint16_t *)(tables + L.blk_job));
+        const ScJob *jobs = (const ScJob *)(tables + L.jobs);
+        const uint16_t *map = (const uint16_t *)(tables + L.blk_job);
+        switch (L.kind) {
+            case 1: hipLaunchKernelGGL(sc_pair01_batch_kernel, dim3(L.blocks), dim3(SC_THREADS), 0, stream, jobs, map); break;
+            case 2: hipLaunchKernelGGL(sc_fold2_batch_kernel<false>, dim3(L.blocks), dim3(SC_THREADS), 0, stream, jobs, map); break;
+            case 3: hipLaunchKernelGGL(sc_fold2_batch_kernel<true>, dim3(L.blocks), dim3(SC_THREADS), 0, stream, jobs, map); break;
+            default:
+                if (L.fold)
+                    hipLaunchKernelGGL(sc_round_batch_kernel<true>, dim3(L.blocks), dim3(SC_THREADS), 0, stream, jobs, map);
+                else
+                    hipLaunchKernelGGL(sc_round_batch_kernel<false>, dim3(L.blocks), dim3(SC_THREADS), 0, stream, jobs, map);
+        }
         LAUNCHCK();
     }
     hipLaunchKernelGGL(sc_tail_batch_kernel, dim3(plan.count), dim3(SC_TAIL_THREADS), 0, stream, tail);
