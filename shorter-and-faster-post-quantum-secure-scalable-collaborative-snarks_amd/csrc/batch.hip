@@ -14,8 +14,8 @@
 //   streams 1 and 3: RS encode, one launch per NTT pass for all proofs of
 //             one size; Merkle, one launch per stage for all of the lane's
 //             trees;
-//   stream 2: the sumchecks, one launch per round for every proof still
-//             above the LDS tail, then one tail workgroup per proof;
+//   stream 2: the sumchecks, two rounds per launch for every proof whose
+//             vectors are still long, then one tail workgroup per proof;
 //   join; one kernel packs every proof's bytes into one buffer; one copy out.
 // The commits go in two lanes on two streams (the largest codewords on one,
 // the rest on the other, balanced by elements encoded), each lane hashing

@@ -203,9 +203,9 @@ int sumcheck_fold_launch(cosnark_ctx *ctx, const gf::fe *s1, const gf::fe *s2, g
                          int round, int fs_rounds, int nb_prev, hipStream_t s);
 int ntt_init(cosnark_ctx *ctx);
 // Product sumcheck rounds; state in ctx->sc (initialised by the caller).
-// scratch buffers: a1,a2,b1,b2 each of n/2 elements.
-// fuse01: rounds 0-2 as two launches (sc_pair01_kernel, sc_fold01_kernel);
-// the batch job builder keeps one launch per round
+// buf: n + n/2 elements (ping n/2 per vector, pong n/4 per vector).
+// fuse01: two rounds per launch while the vectors are long (sc_pair01_kernel,
+// then sc_fold2_kernel), as the batch plan also does
 int sumcheck_launch(cosnark_ctx *ctx, const gf::fe *v1, const gf::fe *v2, uint64_t n, int fs_rounds,
                     gf::fe *buf, hipStream_t s, bool fuse01 = false);
 
