@@ -56,7 +56,13 @@ constexpr int Y_PAD = 1;   // one padding element per LDS row keeps 16-byte lane
 // (four-step) transform: step 1 does R2*C in-register R1-point DFTs over
 // stride-R2 inputs and the w_R^(d2 k1) twiddle, step 2 does R1*C R2-point
 // DFTs, applies the inter-pass twiddle w_{N_t}^(lo k) and stores.
-template <int R1, int R2, bool LAST, bool INV>
+//
+// SPLIT: every thread holds at most one step-1 and one step-2 item, and the
+// LDS exchange moves the real words first and the imaginary words second
+// through one buffer of 8-byte words.  That halves the tile's LDS (about
+// 42 KiB for the 4096-element tile instead of 76), so three workgroups fit
+// on a CU instead of two, for two more barriers per pass.
+template <int R1, int R2, bool LAST, bool INV, bool SPLIT>
 __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
     const fe *in0, fe *out0, NttPass p, const fe *__restrict__ twlo, const fe *__restrict__ twhi,
     const fe *__restrict__ rtab, fe scale) {
@@ -65,7 +71,8 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
     constexpr int LOGR2 = (R2 >= 16) ? 4 : (R2 >= 8) ? 3 : (R2 >= 4) ? 2 : (R2 >= 2) ? 1 : 0;
     extern __shared__ fe ntt_lds[];
     fe *wr = ntt_lds;          // w_R^e, e < R
-    fe *Y = ntt_lds + R;       // (R * C + R * Y_PAD) elements
+    fe *Y = ntt_lds + R;       // (R * C + R * Y_PAD) elements (SPLIT: as many 8-byte words)
+    uint64_t *Yw = reinterpret_cast<uint64_t *>(Y);
     const int tid = threadIdx.x;
     const int logC = p.logC, C = 1 << logC;
     const int CP = C + Y_PAD;   // padded LDS row length
@@ -93,7 +100,9 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
 
     // fan-out (first pass of a padded forward transform): the pass digit has
     // LOGR + fan bits, and the steps below run once per k0 < 2^fan
-    const int fan = (LAST || INV) ? 0 : p.fan;
+    // (the host never picks the split exchange for a fan-out pass: the k0
+    // loop around it costs the split kernel half its occupancy in registers)
+    const int fan = (LAST || INV || SPLIT) ? 0 : p.fan;
     const int logNt = LOGR + fan + logS;
     const int items1 = R2 << logC;
     const int items2 = R1 << logC;
@@ -113,12 +122,15 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
     const bool leaves_on = LAST && p.leaves && by == 0 && items2 <= NTT_THREADS;
     fe v[R2];
     int my_c = 0, my_k1 = 0;
-    fe *tz = Y + (size_t)R * CP;   // fan-out: w_(E R)^(d k0) for the current k0, d < R
+    // fan-out: w_(E R)^(d k0) for the current k0, d < R (R * CP is even)
+    fe *tz = SPLIT ? Y + (size_t)R * CP / 2 : Y + (size_t)R * CP;
     for (int k0 = 0; k0 < (1 << fan); ++k0) {
     if (k0 > 0) __syncthreads();   // the previous k0's step 2 is done with Y
 
     // ---- step 1: R1-point DFTs over d1 for each (d2, c) ----
-    for (int it = tid; it < items1; it += NTT_THREADS) {
+    // item -> x[i] = the DFT output k1 = brev(i) after the w_R^(d2 k1)
+    // twiddle, bound for exchange row k1 * R2 + d2, column c (*off = d2 * CP + c)
+    auto step1 = [&](int it, fe *x, int *off) {
         int c, d2;
         uint64_t base;
         if (!LAST) {
@@ -130,38 +142,35 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
             c = it >> LOGR2;
             base = (rest + (c0 + c) * W0) << LOGR;
         }
-        fe v[R1];
 #pragma unroll
         for (int d1 = 0; d1 < R1; ++d1) {
             const uint64_t d = (uint64_t)(R2 * d1 + d2);
             const uint64_t pos = base + (LAST ? d : (d << logS));
-            v[d1] = pos < p.in_len ? in[pos] : gf::zero();
-            if (k0 > 0) v[d1] = gf::mul_w(tz[d], v[d1]);   // twist by w_(E R)^(d k0); inputs are canonical
+            x[d1] = pos < p.in_len ? in[pos] : gf::zero();
+            if (k0 > 0) x[d1] = gf::mul_w(tz[d], x[d1]);   // twist by w_(E R)^(d k0); inputs are canonical
         }
-        dif_w<R1, INV>(v);
+        dif_w<R1, INV>(x);
         static_for<0, R1>([&](auto ii) {
             constexpr int i = decltype(ii)::value;
             constexpr int k1 = brev_c<R1>(i);
-            fe x = v[i];
-            if (k1 != 0 && d2 != 0) x = gf::mul_w(wr[d2 * k1], x);
-            Y[(k1 * R2 + d2) * CP + c] = x;
+            if (k1 != 0 && d2 != 0) x[i] = gf::mul_w(wr[d2 * k1], x[i]);
         });
-    }
-    __syncthreads();
-
-    // the next k0's twists, while step 2 runs (step 1 is done with tz; the
-    // barrier at the top of the next k0 orders these writes before its reads)
-    if (k0 + 1 < (1 << fan))
-        for (int d = tid; d < R; d += NTT_THREADS)
-            tz[d] = tw28(twlo, twhi, (uint32_t)(d * (k0 + 1)) << (NTT_LOG_MAX - LOGR - fan));
-
+        *off = d2 * CP + c;
+    };
+    // the next k0's twists, while step 2 runs (called once step 1 is done
+    // with tz; the barrier at the top of the next k0 orders these writes
+    // before its reads)
+    auto next_twists = [&]() {
+        if (k0 + 1 < (1 << fan))
+            for (int d = tid; d < R; d += NTT_THREADS)
+                tz[d] = tw28(twlo, twhi, (uint32_t)(d * (k0 + 1)) << (NTT_LOG_MAX - LOGR - fan));
+    };
     // ---- step 2: R2-point DFTs over d2 for each (k1, c), twiddle, store ----
-    for (int it = tid; it < items2; it += NTT_THREADS) {
+    // (v holds the item's exchange column: v[d2] = row k1 * R2 + d2, column c)
+    auto step2 = [&](int it) {
         const int c = it & (C - 1), k1 = it >> logC;
         my_c = c;
         my_k1 = k1;
-#pragma unroll
-        for (int d2 = 0; d2 < R2; ++d2) v[d2] = Y[(k1 * R2 + d2) * CP + c];
         dif_w<R2, INV>(v);
         if (!LAST) {
             // inter-pass twiddle w_{N_t}^(lo k), k = k0 + 2^fan (k1 + R1 k2): one table
@@ -189,6 +198,55 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
                 out[nat_rest + c0 + c + (k << kshift)] = v[i];
             });
         }
+    };
+
+    if constexpr (!SPLIT) {
+        for (int it = tid; it < items1; it += NTT_THREADS) {
+            fe x[R1];
+            int off;
+            step1(it, x, &off);
+            static_for<0, R1>([&](auto ii) {
+                constexpr int i = decltype(ii)::value;
+                Y[brev_c<R1>(i) * R2 * CP + off] = x[i];
+            });
+        }
+        __syncthreads();
+        next_twists();
+        for (int it = tid; it < items2; it += NTT_THREADS) {
+            const int c = it & (C - 1), k1 = it >> logC;
+#pragma unroll
+            for (int d2 = 0; d2 < R2; ++d2) v[d2] = Y[(k1 * R2 + d2) * CP + c];
+            step2(it);
+        }
+    } else {
+        // at most one item of each step per thread (items1, items2 <= NTT_THREADS)
+        const bool act1 = tid < items1, act2 = tid < items2;
+        fe x[R1];
+        int off = 0;
+        if (act1) step1(tid, x, &off);
+        const int c2 = tid & (C - 1), k12 = tid >> logC;
+        if (act1)
+            static_for<0, R1>([&](auto ii) {
+                constexpr int i = decltype(ii)::value;
+                Yw[brev_c<R1>(i) * R2 * CP + off] = x[i].re;
+            });
+        __syncthreads();
+        next_twists();
+        if (act2)
+#pragma unroll
+            for (int d2 = 0; d2 < R2; ++d2) v[d2].re = Yw[(k12 * R2 + d2) * CP + c2];
+        __syncthreads();
+        if (act1)
+            static_for<0, R1>([&](auto ii) {
+                constexpr int i = decltype(ii)::value;
+                Yw[brev_c<R1>(i) * R2 * CP + off] = x[i].im;
+            });
+        __syncthreads();
+        if (act2) {
+#pragma unroll
+            for (int d2 = 0; d2 < R2; ++d2) v[d2].im = Yw[(k12 * R2 + d2) * CP + c2];
+            step2(tid);
+        }
     }
     }   // k0
     if (leaves_on) {
@@ -198,54 +256,67 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
         // restage them as Z[k][c] in the exchange buffer (after every
         // thread's step-2 reads of it), then each thread hashes whole leaves.
         // The condition is uniform over the workgroup, so are the barriers.
-        __syncthreads();
-        if (tid < items2) {
-            static_for<0, R2>([&](auto ii) {
-                constexpr int i = decltype(ii)::value;
-                constexpr int k2 = brev_c<R2>(i);
-                Y[(my_k1 + R1 * k2) * CP + my_c] = v[i];
-            });
-        }
-        __syncthreads();
-        const int qpr = C / 4, nleaf = (R * C) / 4;
-        for (int l = tid; l < nleaf; l += NTT_THREADS) {
-            const int k = l / qpr, q = l % qpr;
-            uint32_t m[16], h[8];
-            const fe *src = Y + k * CP + 4 * q;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                m[4 * e] = (uint32_t)src[e].re;
-                m[4 * e + 1] = (uint32_t)(src[e].re >> 32);
-                m[4 * e + 2] = (uint32_t)src[e].im;
-                m[4 * e + 3] = (uint32_t)(src[e].im >> 32);
+        // SPLIT halves the buffer, so it restages and hashes rows k < R/2
+        // (k2 < R2/2) first and the rest second.
+        constexpr int NH = SPLIT ? 2 : 1, RH = R / NH;
+        const int qpr = C / 4, nleaf = (RH * C) / 4;
+        static_for<0, NH>([&](auto hh) {
+            constexpr int h = decltype(hh)::value;
+            __syncthreads();   // the exchange buffer is free (step-2 reads, or the previous half's hashes)
+            if (tid < items2) {
+                static_for<0, R2>([&](auto ii) {
+                    constexpr int i = decltype(ii)::value;
+                    constexpr int k2 = brev_c<R2>(i);
+                    const int hc = decltype(hh)::value;
+                    if (k2 / (R2 / NH) == hc) Y[(my_k1 + R1 * k2 - hc * RH) * CP + my_c] = v[i];
+                });
             }
-            b3::compress64(m, h);
-            const uint64_t pos = nat_rest + c0 + 4 * q + ((uint64_t)k << kshift);
-            uint4 *dst = reinterpret_cast<uint4 *>(p.leaves + 32 * (pos >> 2));
-            dst[0] = make_uint4(h[0], h[1], h[2], h[3]);
-            dst[1] = make_uint4(h[4], h[5], h[6], h[7]);
-        }
+            __syncthreads();
+            for (int l = tid; l < nleaf; l += NTT_THREADS) {
+                const int k = l / qpr, q = l % qpr;
+                uint32_t m[16], hv[8];
+                const fe *src = Y + k * CP + 4 * q;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    m[4 * e] = (uint32_t)src[e].re;
+                    m[4 * e + 1] = (uint32_t)(src[e].re >> 32);
+                    m[4 * e + 2] = (uint32_t)src[e].im;
+                    m[4 * e + 3] = (uint32_t)(src[e].im >> 32);
+                }
+                b3::compress64(m, hv);
+                const uint64_t pos = nat_rest + c0 + 4 * q + ((uint64_t)(k + h * RH) << kshift);
+                uint4 *dst = reinterpret_cast<uint4 *>(p.leaves + 32 * (pos >> 2));
+                dst[0] = make_uint4(hv[0], hv[1], hv[2], hv[3]);
+                dst[1] = make_uint4(hv[4], hv[5], hv[6], hv[7]);
+            }
+        });
     }
 }
 
 using PassKernel = void (*)(const fe *, fe *, NttPass, const fe *, const fe *, const fe *, fe);
 
-template <int R1, int R2>
-PassKernel pass_kernel(bool last, bool inverse) {
-    return last ? (inverse ? ntt_pass_kernel<R1, R2, true, true> : ntt_pass_kernel<R1, R2, true, false>)
-                : (inverse ? ntt_pass_kernel<R1, R2, false, true> : ntt_pass_kernel<R1, R2, false, false>);
+template <int R1, int R2, bool SPLIT>
+PassKernel pass_kernel_s(bool last, bool inverse) {
+    return last ? (inverse ? ntt_pass_kernel<R1, R2, true, true, SPLIT> : ntt_pass_kernel<R1, R2, true, false, SPLIT>)
+                : (inverse ? ntt_pass_kernel<R1, R2, false, true, SPLIT>
+                           : ntt_pass_kernel<R1, R2, false, false, SPLIT>);
 }
 
-PassKernel pass_kernel_for(int r, bool last, bool inverse) {
+template <int R1, int R2>
+PassKernel pass_kernel(bool last, bool inverse, bool split) {
+    return split ? pass_kernel_s<R1, R2, true>(last, inverse) : pass_kernel_s<R1, R2, false>(last, inverse);
+}
+
+PassKernel pass_kernel_for(int r, bool last, bool inverse, bool split) {
     switch (r) {
-        case 1: return pass_kernel<2, 1>(last, inverse);
-        case 2: return pass_kernel<2, 2>(last, inverse);
-        case 3: return pass_kernel<4, 2>(last, inverse);
-        case 4: return pass_kernel<4, 4>(last, inverse);
-        case 5: return pass_kernel<8, 4>(last, inverse);
-        case 6: return pass_kernel<8, 8>(last, inverse);
-        case 7: return pass_kernel<16, 8>(last, inverse);
-        case 8: return pass_kernel<16, 16>(last, inverse);
+        case 1: return pass_kernel<2, 1>(last, inverse, split);
+        case 2: return pass_kernel<2, 2>(last, inverse, split);
+        case 3: return pass_kernel<4, 2>(last, inverse, split);
+        case 4: return pass_kernel<4, 4>(last, inverse, split);
+        case 5: return pass_kernel<8, 4>(last, inverse, split);
+        case 6: return pass_kernel<8, 8>(last, inverse, split);
+        case 7: return pass_kernel<16, 8>(last, inverse, split);
+        case 8: return pass_kernel<16, 16>(last, inverse, split);
         default: return nullptr;
     }
 }
@@ -254,9 +325,13 @@ template <int R1, int R2>
 int launch_pass(cosnark_ctx *ctx, hipStream_t stream, uint64_t blocks, int batch, const fe *in, fe *out,
                 const NttPass &q, bool last, bool inverse, const fe *twlo, const fe *twhi,
                 const fe *rtab, fe scale) {
-    PassKernel k = pass_kernel<R1, R2>(last, inverse);
-    // radix table, padded tile, and for a fan-out pass the R twists
-    const size_t lds = sizeof(fe) * ((size_t)R1 * R2 * ((1u << q.logC) + Y_PAD + 1 + (q.fan ? 1 : 0)));
+    // the split exchange needs at most one item of each step per thread
+    const bool split = ctx->ntt_split && !q.fan && ((R1 << q.logC) <= NTT_THREADS) &&
+                       ((R2 << q.logC) <= NTT_THREADS);
+    PassKernel k = pass_kernel<R1, R2>(last, inverse, split);
+    // radix table, padded tile (split: 8-byte words), and for a fan-out pass the R twists
+    const size_t R = (size_t)R1 * R2, CP = (1u << q.logC) + Y_PAD;
+    const size_t lds = sizeof(fe) * (R + (q.fan ? R : 0)) + (split ? 8 : 16) * R * CP;
     hipLaunchKernelGGL(k, dim3((unsigned)blocks, (unsigned)batch), dim3(NTT_THREADS), lds, stream, in, out, q, twlo,
                        twhi, rtab, scale);
     LAUNCHCK();
@@ -290,8 +365,9 @@ int ntt_init(cosnark_ctx *ctx) {
     for (int r = 1; r <= NTT_PASS_LOG_MAX; ++r)
         for (int last = 0; last < 2; ++last)
             for (int inv = 0; inv < 2; ++inv)
-                HIPCK(hipFuncSetAttribute((const void *)pass_kernel_for(r, last, inv),
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+                for (int split = 0; split < 2; ++split)
+                    HIPCK(hipFuncSetAttribute((const void *)pass_kernel_for(r, last, inv, split),
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
     return COSNARK_OK;
 }
 

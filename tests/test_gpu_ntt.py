@@ -94,3 +94,37 @@ def test_rs_encode_fan_out_matches_oracle(passbits, logn, rate):
         assert np.array_equal(c.rs_encode(a, rate), O.rs_encode(a, rate))
     finally:
         c.close()
+
+
+@pytest.mark.parametrize("split", [0, 1])
+@pytest.mark.parametrize("passbits,tile,logn", [(8, 4096, 20), (8, 2048, 18), (7, 4096, 17), (6, 1024, 16),
+                                                (4, 512, 13)])
+def test_ntt_exchange_forms_match_oracle(split, passbits, tile, logn):
+    """Both LDS exchanges of a pass: whole 16-byte elements (COSNARK_NTT_SPLIT=0)
+    and real words then imaginary words through a half-size buffer (the
+    default where each thread holds at most one item per step).  Forward,
+    inverse, an RS encode and a WHIR commit (leaf digests from the last pass,
+    restaged in two halves under the split form) under each plan."""
+    import os
+    import cosnark
+    os.environ["COSNARK_NTT_SPLIT"] = str(split)
+    try:
+        c = cosnark.Context(0)
+    finally:
+        del os.environ["COSNARK_NTT_SPLIT"]
+    try:
+        print(f"ntt exchange split={split} passbits={passbits} tile={tile} logn={logn}", flush=True)
+        for lg in (logn, logn + 2):
+            c.set_ntt_plan(lg, passbits, tile)
+        a = O.synth(0x60 + logn + passbits, split, 1 << logn)
+        assert np.array_equal(c.ntt(a), O.fft(a))
+        assert np.array_equal(c.ntt(a, inverse=True), O.fft(a, inverse=True))
+        q = a[: (1 << logn) // 4]
+        assert np.array_equal(c.rs_encode(q, 4), O.rs_encode(q, 4))
+        cw, nodes = c.whir_commit(c.upload(a), 1 << logn, 4)
+        ref_cw = O.rs_encode(a, 4)
+        assert np.array_equal(c.download(cw, ((4 << logn), 2)), ref_cw)
+        cnt = cosnark.merkle_node_count(4 << logn)
+        assert np.array_equal(c.download(nodes, (cnt, 32), np.uint8), O.merkle_commit(ref_cw, True))
+    finally:
+        c.close()
