@@ -136,6 +136,7 @@ int cosnark_create(int device, cosnark_ctx **out) {
         if (v >= 256 && v <= NTT_LDS_ELEMS && is_pow2((uint64_t)v)) ctx->ntt_tile = v;
     }
     if (const char *t = getenv("COSNARK_NTT_SPLIT")) ctx->ntt_split = atoi(t) != 0;
+    if (const char *t = getenv("COSNARK_NTT_DEFER")) ctx->ntt_defer = atoi(t) != 0;
     if (const char *t = getenv("COSNARK_NTT_FAN_MIN")) {
         const int v = atoi(t);
         if (v >= 1) ctx->ntt_fan_min_blocks = v;

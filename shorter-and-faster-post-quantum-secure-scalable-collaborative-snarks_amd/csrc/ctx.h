@@ -58,6 +58,7 @@ struct cosnark_ctx {
     int ntt_passbits = NTT_PASS_LOG_MAX;  // largest radix (bits) of one NTT pass
     int ntt_fan_min_blocks = 512;     // fan-out first pass only with at least this many workgroups
     int ntt_split = 1;                // NTT passes exchange real and imaginary words in turn where they can
+    int ntt_defer = 1;                // NTT plans of 3+ passes leave inter-pass twiddles to the consuming pass
     int mk_block_log = 10;            // Merkle schedule: 2^this first-level nodes per workgroup (6..10)
     // per-size plan (0 = the defaults above); set by the autotuner
     struct { int passbits, tile; } ntt_plan[NTT_LOG_MAX + 1] = {};

@@ -45,10 +45,29 @@ struct NttPass {
     int last;           // 1: stride-1 pass writing natural order
     int fan;            // first pass of a transform padded by 2^fan: fan out (see above)
     int npass;
+    int pass;           // this pass's index
+    int defer;          // inter-pass twiddles applied by the consuming pass (see below)
+    int ut_elems;       // elements of the per-tile input-twiddle table (0 if none)
+    int ut_lo;          // 1: that table also holds w_N^(k_0 lo) per column (R1 x C values)
     int rbits[8];       // radix bits per pass, first pass first
     int pre[8];         // pre[t] = rbits[0] + .. + rbits[t-1]
     uint8_t *leaves;    // last pass, batch item 0: Merkle leaf digests of the output (or null)
 };
+
+// loads and stores through global-address-space pointers (global_load /
+// global_store rather than flat accesses, which also count against the LDS
+// wait counter)
+using gfe = __attribute__((address_space(1))) fe;
+__device__ __forceinline__ fe gload(const gfe *p) {
+    fe r;
+    r.re = p->re;
+    r.im = p->im;
+    return r;
+}
+__device__ __forceinline__ void gstore(gfe *p, const fe &v) {
+    p->re = v.re;
+    p->im = v.im;
+}
 
 constexpr int Y_PAD = 1;   // one padding element per LDS row keeps 16-byte lanes off shared banks
 
@@ -57,29 +76,47 @@ constexpr int Y_PAD = 1;   // one padding element per LDS row keeps 16-byte lane
 // stride-R2 inputs and the w_R^(d2 k1) twiddle, step 2 does R1*C R2-point
 // DFTs, applies the inter-pass twiddle w_{N_t}^(lo k) and stores.
 //
+// Deferred twiddles (plans of three or more passes).  Pass t's inter-pass
+// twiddle on the element at (previous frequencies, k_t, lo) is
+// w_N^(k_t 2^pre[t] lo).  Instead of multiplying it in at pass t's store, a
+// later pass applies it: in pass j's tile every earlier frequency k_t is
+// fixed, so the pending factor on input digit d and column lo is
+// w_N^(E (d S + lo)) with E = sum_t k_t 2^pre[t] (the digit-reversed hi).
+// Its d-dependent part is w_N^(E S R2 d1) on step 1's inputs (a per-tile
+// table of R1 values) times w_N^(E S d2), which is constant along step 1's
+// DFT and so joins the w_R^(d2 k1) twiddle (a per-tile table of R values
+// replacing w_R's); its lo part is left for the next pass.  The last pass's
+// columns are values of k_0, so k_0's share of the last digit's factor,
+// w_N^(k_0 lo), is applied one pass earlier, folded into that pass's input
+// table (R1 x C values).  Per element that costs about one multiply and a
+// table read where the running product cost two.
+//
 // SPLIT: every thread holds at most one step-1 and one step-2 item, and the
 // LDS exchange moves the real words first and the imaginary words second
 // through one buffer of 8-byte words.  That halves the tile's LDS (about
 // 42 KiB for the 4096-element tile instead of 76), so three workgroups fit
 // on a CU instead of two, for two more barriers per pass.
-template <int R1, int R2, bool LAST, bool INV, bool SPLIT>
-__global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
-    const fe *in0, fe *out0, NttPass p, const fe *__restrict__ twlo, const fe *__restrict__ twhi,
+// DEFER (non-last passes; p.defer set): the pass stores without the
+// inter-pass twiddle and, past the first pass, applies the pending ones.  The
+// last pass takes p.defer at run time.
+template <int R1, int R2, bool LAST, bool INV, bool SPLIT, bool DEFER>
+__device__ __forceinline__ void ntt_pass_body(
+    const fe *in0, fe *out0, const NttPass &p, const fe *__restrict__ twlo, const fe *__restrict__ twhi,
     const fe *__restrict__ rtab, fe scale) {
     constexpr int R = R1 * R2;
     constexpr int LOGR = (R >= 256) ? 8 : (R >= 128) ? 7 : (R >= 64) ? 6 : (R >= 32) ? 5 : (R >= 16) ? 4 : (R >= 8) ? 3 : (R >= 4) ? 2 : (R >= 2) ? 1 : 0;
     constexpr int LOGR2 = (R2 >= 16) ? 4 : (R2 >= 8) ? 3 : (R2 >= 4) ? 2 : (R2 >= 2) ? 1 : 0;
     extern __shared__ fe ntt_lds[];
-    fe *wr = ntt_lds;          // w_R^e, e < R
-    fe *Y = ntt_lds + R;       // (R * C + R * Y_PAD) elements (SPLIT: as many 8-byte words)
+    fe *wr = ntt_lds;          // step 1's twiddle for (d2, k1) at d2 * R1 + k1
+    fe *ut = ntt_lds + R;      // deferred twiddles: step 1's input factors (p.ut_elems)
+    fe *Y = ut + p.ut_elems;   // (R * C + R * Y_PAD) elements (SPLIT: as many 8-byte words)
     uint64_t *Yw = reinterpret_cast<uint64_t *>(Y);
     const int tid = threadIdx.x;
     const int logC = p.logC, C = 1 << logC;
     const int CP = C + Y_PAD;   // padded LDS row length
     const uint32_t by = blockIdx.y;
-    const fe *in = p.in_tab ? p.in_tab[by] : in0 + by * p.in_stride;
-    fe *out = p.out_tab ? p.out_tab[by] : out0 + by * p.out_stride;
-    for (int j = tid; j < R; j += NTT_THREADS) wr[j] = rtab[j << (NTT_RADIX_LOG_MAX - LOGR)];
+    const gfe *in = (const gfe *)(p.in_tab ? p.in_tab[by] : in0 + by * p.in_stride);
+    gfe *out = (gfe *)(p.out_tab ? p.out_tab[by] : out0 + by * p.out_stride);
 
     const uint64_t bid = blockIdx.x;
     // block -> positions
@@ -95,6 +132,46 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
         rest = bid >> cblocks_log;
         c0 = (bid & ((1ull << cblocks_log) - 1)) << logC;
         W0 = 1ull << (p.logN - p.rbits[0] - p.rbits[p.npass - 1]);
+    }
+    // E of the deferred twiddles: the earlier passes' frequencies held in
+    // hi (non-last) or rest (last; k_0 is the column there and its share was
+    // applied one pass earlier), digit-reversed
+    const bool consumer = (LAST || DEFER) && p.defer && p.pass >= 1;
+    uint64_t E = 0, k_0 = 0;
+    if (consumer) {
+        uint64_t x = LAST ? rest : hi;
+        for (int t = LAST ? p.npass - 2 : p.pass - 1; t >= (LAST ? 1 : 0); --t) {
+            const uint64_t dgt = x & ((1ull << p.rbits[t]) - 1);
+            x >>= p.rbits[t];
+            E += dgt << p.pre[t];
+            k_0 = dgt;   // the last digit read (t = 0) is the top one
+        }
+    }
+    if (!consumer) {
+        for (int j = tid; j < R; j += NTT_THREADS) {
+            const int d2 = j / R1, k1 = j % R1;
+            wr[j] = rtab[((d2 * k1) & (R - 1)) << (NTT_RADIX_LOG_MAX - LOGR)];
+        }
+    } else {
+        // w_R^(d2 k1) w_N^(E S d2), and w_N^(E S R2 d1) (times w_N^(k_0 lo) for
+        // the table of R1 x C values), as powers of the order-2^28 root
+        const uint32_t m28 = (1u << NTT_LOG_MAX) - 1;
+        const int sN = NTT_LOG_MAX - p.logN;
+        const uint32_t es = (uint32_t)((E << logS) << sN);   // w_N^(E S) as a power of w_28
+        // Each entry is a product of factors that many lanes of a wave share
+        // (w_N^(E S d2): R1 lanes; w_N^(E S R2 d1): C lanes) or that come from
+        // the small radix table, so the table loads touch few cache lines.
+        for (int j = tid; j < R; j += NTT_THREADS) {
+            const uint32_t d2 = j / R1, k1 = j % R1;
+            const fe v = tw28(twlo, twhi, (d2 * es) & m28);
+            wr[j] = gf::mul(v, rtab[((d2 * k1) & (R - 1)) << (NTT_RADIX_LOG_MAX - LOGR)]);
+        }
+        for (int j = tid; j < p.ut_elems; j += NTT_THREADS) {
+            const uint32_t d1 = p.ut_lo ? j >> logC : j;
+            const fe u = tw28(twlo, twhi, (d1 * R2 * es) & m28);
+            ut[j] = p.ut_lo ? gf::mul(u, tw28(twlo, twhi, (uint32_t)((k_0 * (lo0 + (j & (C - 1)))) << sN) & m28))
+                            : u;
+        }
     }
     __syncthreads();
 
@@ -122,6 +199,8 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
     const bool leaves_on = LAST && p.leaves && by == 0 && items2 <= NTT_THREADS;
     fe v[R2];
     int my_c = 0, my_k1 = 0;
+    // zero padding (only the first pass reads fewer than 2^logN inputs)
+    const bool padded = p.in_len < (1ull << p.logN);
     // fan-out: w_(E R)^(d k0) for the current k0, d < R (R * CP is even)
     fe *tz = SPLIT ? Y + (size_t)R * CP / 2 : Y + (size_t)R * CP;
     for (int k0 = 0; k0 < (1 << fan); ++k0) {
@@ -142,18 +221,39 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
             c = it >> LOGR2;
             base = (rest + (c0 + c) * W0) << LOGR;
         }
+        if (padded) {
 #pragma unroll
-        for (int d1 = 0; d1 < R1; ++d1) {
-            const uint64_t d = (uint64_t)(R2 * d1 + d2);
-            const uint64_t pos = base + (LAST ? d : (d << logS));
-            x[d1] = pos < p.in_len ? in[pos] : gf::zero();
-            if (k0 > 0) x[d1] = gf::mul_w(tz[d], x[d1]);   // twist by w_(E R)^(d k0); inputs are canonical
+            for (int d1 = 0; d1 < R1; ++d1) {
+                const uint64_t d = (uint64_t)(R2 * d1 + d2);
+                const uint64_t pos = base + (LAST ? d : (d << logS));
+                x[d1] = pos < p.in_len ? gload(in + pos) : gf::zero();
+                if (k0 > 0) x[d1] = gf::mul_w(tz[d], x[d1]);   // twist by w_(E R)^(d k0); inputs are canonical
+            }
+        } else {
+#pragma unroll
+            for (int d1 = 0; d1 < R1; ++d1) {
+                const uint64_t d = (uint64_t)(R2 * d1 + d2);
+                x[d1] = gload(in + base + (LAST ? d : (d << logS)));
+            }
+        }
+        // deferred twiddles (inputs weak <= p + 7 from the previous pass's
+        // store), in a block of their own after every load is issued
+        if (consumer) {
+            if (p.ut_lo) {
+#pragma unroll
+                for (int d1 = 0; d1 < R1; ++d1) x[d1] = gf::mul_w(ut[(d1 << logC) + c], x[d1]);
+            } else {
+#pragma unroll
+                for (int d1 = 1; d1 < R1; ++d1) x[d1] = gf::mul_w(ut[d1], x[d1]);
+            }
         }
         dif_w<R1, INV>(x);
         static_for<0, R1>([&](auto ii) {
             constexpr int i = decltype(ii)::value;
             constexpr int k1 = brev_c<R1>(i);
-            if (k1 != 0 && d2 != 0) x[i] = gf::mul_w(wr[d2 * k1], x[i]);
+            // (where a pass may consume deferred twiddles, column k1 = 0 is
+            // multiplied too: by 1 unless it does)
+            if ((k1 != 0 || LAST || DEFER) && d2 != 0) x[i] = gf::mul_w(wr[d2 * R1 + k1], x[i]);
         });
         *off = d2 * CP + c;
     };
@@ -172,7 +272,15 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
         my_c = c;
         my_k1 = k1;
         dif_w<R2, INV>(v);
-        if (!LAST) {
+        if constexpr (!LAST && DEFER) {
+            // the inter-pass twiddle is left to the next passes; weak stores
+            static_for<0, R2>([&](auto kk) {
+                constexpr int k2 = decltype(kk)::value;
+                constexpr int i = brev_c<R2>(k2);
+                const uint64_t k = (uint64_t)k0 + ((uint64_t)(k1 + R1 * k2) << fan);
+                gstore(out + (hi << logNt) + (k << logS) + lo0 + c, v[i]);
+            });
+        } else if constexpr (!LAST) {
             // inter-pass twiddle w_{N_t}^(lo k), k = k0 + 2^fan (k1 + R1 k2): one table
             // product for w^(lo (k0 + 2^fan k1)) and one for w^(lo 2^fan R1), then a
             // running product over k2
@@ -186,7 +294,7 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
                 constexpr int k2 = decltype(kk)::value;
                 constexpr int i = brev_c<R2>(k2);
                 const uint64_t k = (uint64_t)k0 + ((uint64_t)(k1 + R1 * k2) << fan);
-                out[(hi << logNt) + (k << logS) + lo] = gf::mul_w(t, v[i]);   // weak: read by the next pass
+                gstore(out + (hi << logNt) + (k << logS) + lo, gf::mul_w(t, v[i]));   // weak: read by the next pass
                 if (k2 + 1 < R2) t = gf::mul(t, step);
             });
         } else {
@@ -195,7 +303,7 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
                 constexpr int k2 = brev_c<R2>(i);
                 const uint64_t k = (uint64_t)(k1 + R1 * k2);
                 v[i] = INV ? gf::mul(scale, v[i]) : gf::canon_w(v[i]);
-                out[nat_rest + c0 + c + (k << kshift)] = v[i];
+                gstore(out + nat_rest + c0 + c + (k << kshift), v[i]);
             });
         }
     };
@@ -258,7 +366,7 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
         // The condition is uniform over the workgroup, so are the barriers.
         // SPLIT halves the buffer, so it restages and hashes rows k < R/2
         // (k2 < R2/2) first and the rest second.
-        constexpr int NH = SPLIT ? 2 : 1, RH = R / NH;
+        constexpr int NH = (SPLIT && R2 >= 2) ? 2 : 1, RH = R / NH;
         const int qpr = C / 4, nleaf = (RH * C) / 4;
         static_for<0, NH>([&](auto hh) {
             constexpr int h = decltype(hh)::value;
@@ -293,30 +401,54 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
     }
 }
 
+// The split form's LDS lets three workgroups share a CU, so its kernels are
+// held to the registers of three waves per SIMD (168 VGPRs); the whole-element
+// form's LDS allows two.
+template <int R1, int R2, bool LAST, bool INV, bool DEFER>
+__global__ __launch_bounds__(NTT_THREADS, 3) void ntt_pass_split_kernel(
+    const fe *in0, fe *out0, NttPass p, const fe *__restrict__ twlo, const fe *__restrict__ twhi,
+    const fe *__restrict__ rtab, fe scale) {
+    ntt_pass_body<R1, R2, LAST, INV, true, DEFER>(in0, out0, p, twlo, twhi, rtab, scale);
+}
+
+template <int R1, int R2, bool LAST, bool INV, bool DEFER>
+__global__ __launch_bounds__(NTT_THREADS, 2) void ntt_pass_kernel(
+    const fe *in0, fe *out0, NttPass p, const fe *__restrict__ twlo, const fe *__restrict__ twhi,
+    const fe *__restrict__ rtab, fe scale) {
+    ntt_pass_body<R1, R2, LAST, INV, false, DEFER>(in0, out0, p, twlo, twhi, rtab, scale);
+}
+
 using PassKernel = void (*)(const fe *, fe *, NttPass, const fe *, const fe *, const fe *, fe);
 
+template <int R1, int R2, bool LAST, bool INV, bool SPLIT, bool DEFER>
+constexpr PassKernel pass_fn() {
+    if constexpr (SPLIT) return ntt_pass_split_kernel<R1, R2, LAST, INV, DEFER>;
+    else return ntt_pass_kernel<R1, R2, LAST, INV, DEFER>;
+}
+
 template <int R1, int R2, bool SPLIT>
-PassKernel pass_kernel_s(bool last, bool inverse) {
-    return last ? (inverse ? ntt_pass_kernel<R1, R2, true, true, SPLIT> : ntt_pass_kernel<R1, R2, true, false, SPLIT>)
-                : (inverse ? ntt_pass_kernel<R1, R2, false, true, SPLIT>
-                           : ntt_pass_kernel<R1, R2, false, false, SPLIT>);
+PassKernel pass_kernel_s(bool last, bool inverse, bool defer) {
+    if (last) return inverse ? pass_fn<R1, R2, true, true, SPLIT, false>() : pass_fn<R1, R2, true, false, SPLIT, false>();
+    if (defer) return inverse ? pass_fn<R1, R2, false, true, SPLIT, true>() : pass_fn<R1, R2, false, false, SPLIT, true>();
+    return inverse ? pass_fn<R1, R2, false, true, SPLIT, false>() : pass_fn<R1, R2, false, false, SPLIT, false>();
 }
 
 template <int R1, int R2>
-PassKernel pass_kernel(bool last, bool inverse, bool split) {
-    return split ? pass_kernel_s<R1, R2, true>(last, inverse) : pass_kernel_s<R1, R2, false>(last, inverse);
+PassKernel pass_kernel(bool last, bool inverse, bool split, bool defer) {
+    return split ? pass_kernel_s<R1, R2, true>(last, inverse, defer)
+                 : pass_kernel_s<R1, R2, false>(last, inverse, defer);
 }
 
-PassKernel pass_kernel_for(int r, bool last, bool inverse, bool split) {
+PassKernel pass_kernel_for(int r, bool last, bool inverse, bool split, bool defer) {
     switch (r) {
-        case 1: return pass_kernel<2, 1>(last, inverse, split);
-        case 2: return pass_kernel<2, 2>(last, inverse, split);
-        case 3: return pass_kernel<4, 2>(last, inverse, split);
-        case 4: return pass_kernel<4, 4>(last, inverse, split);
-        case 5: return pass_kernel<8, 4>(last, inverse, split);
-        case 6: return pass_kernel<8, 8>(last, inverse, split);
-        case 7: return pass_kernel<16, 8>(last, inverse, split);
-        case 8: return pass_kernel<16, 16>(last, inverse, split);
+        case 1: return pass_kernel<2, 1>(last, inverse, split, defer);
+        case 2: return pass_kernel<2, 2>(last, inverse, split, defer);
+        case 3: return pass_kernel<4, 2>(last, inverse, split, defer);
+        case 4: return pass_kernel<4, 4>(last, inverse, split, defer);
+        case 5: return pass_kernel<8, 4>(last, inverse, split, defer);
+        case 6: return pass_kernel<8, 8>(last, inverse, split, defer);
+        case 7: return pass_kernel<16, 8>(last, inverse, split, defer);
+        case 8: return pass_kernel<16, 16>(last, inverse, split, defer);
         default: return nullptr;
     }
 }
@@ -326,12 +458,12 @@ int launch_pass(cosnark_ctx *ctx, hipStream_t stream, uint64_t blocks, int batch
                 const NttPass &q, bool last, bool inverse, const fe *twlo, const fe *twhi,
                 const fe *rtab, fe scale) {
     // the split exchange needs at most one item of each step per thread
-    const bool split = ctx->ntt_split && !q.fan && ((R1 << q.logC) <= NTT_THREADS) &&
+    const bool split = ctx->ntt_split && !q.fan && R2 >= 2 && ((R1 << q.logC) <= NTT_THREADS) &&
                        ((R2 << q.logC) <= NTT_THREADS);
-    PassKernel k = pass_kernel<R1, R2>(last, inverse, split);
+    PassKernel k = pass_kernel<R1, R2>(last, inverse, split, q.defer != 0);
     // radix table, padded tile (split: 8-byte words), and for a fan-out pass the R twists
     const size_t R = (size_t)R1 * R2, CP = (1u << q.logC) + Y_PAD;
-    const size_t lds = sizeof(fe) * (R + (q.fan ? R : 0)) + (split ? 8 : 16) * R * CP;
+    const size_t lds = sizeof(fe) * (R + (size_t)q.ut_elems + (q.fan ? R : 0)) + (split ? 8 : 16) * R * CP;
     hipLaunchKernelGGL(k, dim3((unsigned)blocks, (unsigned)batch), dim3(NTT_THREADS), lds, stream, in, out, q, twlo,
                        twhi, rtab, scale);
     LAUNCHCK();
@@ -360,14 +492,15 @@ int dispatch_pass(cosnark_ctx *ctx, hipStream_t stream, int r, uint64_t blocks, 
 
 int ntt_init(cosnark_ctx *ctx) {
     // the largest tile: 4096 elements + padding + the radix table + a fan-out
-    // pass's twists (~76 KiB)
-    const int bytes = (int)(sizeof(fe) * (NTT_LDS_ELEMS + 256 * (Y_PAD + 2)));
+    // pass's twists or a deferred-twiddle table of up to 1024 values (~92 KiB)
+    const int bytes = (int)(sizeof(fe) * (NTT_LDS_ELEMS + 256 * (Y_PAD + 2) + 1024));
     for (int r = 1; r <= NTT_PASS_LOG_MAX; ++r)
         for (int last = 0; last < 2; ++last)
             for (int inv = 0; inv < 2; ++inv)
                 for (int split = 0; split < 2; ++split)
-                    HIPCK(hipFuncSetAttribute((const void *)pass_kernel_for(r, last, inv, split),
-                                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+                    for (int defer = 0; defer < 2; ++defer)
+                        HIPCK(hipFuncSetAttribute((const void *)pass_kernel_for(r, last, inv, split, defer),
+                                                  hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
     return COSNARK_OK;
 }
 
@@ -418,6 +551,13 @@ int ntt_launch_batch(cosnark_ctx *ctx, const fe *in, const fe *const *in_tab, ui
     const fe *twhi = inverse ? ctx->itw_hi : ctx->tw_hi;
     const fe *rtab = inverse ? ctx->irtw : ctx->rtw;
     const fe scale = gf::inv(gf::from_u64(N));
+    // deferred inter-pass twiddles: plans of three or more passes whose
+    // transforms all have at least 4 bits (step 2 radix >= 4, so the table of
+    // the pass before the last stays within 1024 values)
+    bool defer = ctx->ntt_defer && m >= 3;
+    for (int t = 0; t < m; ++t)
+        if (p.rbits[t] - (t == 0 ? fan : 0) < 4) defer = false;
+    p.defer = defer ? 1 : 0;
     // The last pass scatters rows into natural order, so it cannot run in
     // place: passes 0..m-2 work in a context-owned temporary (one N-element
     // slice per batch item) and the last one writes `out`.  (A single pass
@@ -452,6 +592,9 @@ int ntt_launch_batch(cosnark_ctx *ctx, const fe *in, const fe *const *in_tab, ui
                 while (logC > 2 && (r1 << logC) > NTT_THREADS) --logC;
         }
         q.logC = logC;
+        q.pass = t;
+        q.ut_lo = (defer && t >= 1 && t == m - 2) ? 1 : 0;
+        q.ut_elems = (defer && t >= 1) ? (q.ut_lo ? (r1 << logC) : r1) : 0;
         if (t > 0) q.in_len = N;
         const uint64_t blocks = N >> (r + logC);
         const fe *src;

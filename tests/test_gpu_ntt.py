@@ -96,24 +96,30 @@ def test_rs_encode_fan_out_matches_oracle(passbits, logn, rate):
         c.close()
 
 
-@pytest.mark.parametrize("split", [0, 1])
+@pytest.mark.parametrize("split,defer", [(0, 0), (1, 0), (0, 1), (1, 1)])
 @pytest.mark.parametrize("passbits,tile,logn", [(8, 4096, 20), (8, 2048, 18), (7, 4096, 17), (6, 1024, 16),
-                                                (4, 512, 13)])
-def test_ntt_exchange_forms_match_oracle(split, passbits, tile, logn):
-    """Both LDS exchanges of a pass: whole 16-byte elements (COSNARK_NTT_SPLIT=0)
-    and real words then imaginary words through a half-size buffer (the
-    default where each thread holds at most one item per step).  Forward,
-    inverse, an RS encode and a WHIR commit (leaf digests from the last pass,
-    restaged in two halves under the split form) under each plan."""
+                                                (4, 512, 13), (5, 4096, 19)])
+def test_ntt_exchange_and_twiddle_forms_match_oracle(split, defer, passbits, tile, logn):
+    """Both LDS exchanges of a pass -- whole 16-byte elements
+    (COSNARK_NTT_SPLIT=0) and real words then imaginary words through a
+    half-size buffer (the default where each thread holds at most one item per
+    step) -- and both inter-pass twiddle schemes: a running product at each
+    pass's store (COSNARK_NTT_DEFER=0) and the factors left to the consuming
+    passes (the default for plans of three or more passes of at least 4 bits;
+    19 bits at 5 per pass takes four passes).  Forward, inverse, an RS encode
+    and a WHIR commit (leaf digests from the last pass, restaged in two halves
+    under the split form) under each plan."""
     import os
     import cosnark
-    os.environ["COSNARK_NTT_SPLIT"] = str(split)
+    env = {"COSNARK_NTT_SPLIT": str(split), "COSNARK_NTT_DEFER": str(defer)}
+    os.environ.update(env)
     try:
         c = cosnark.Context(0)
     finally:
-        del os.environ["COSNARK_NTT_SPLIT"]
+        for k in env:
+            del os.environ[k]
     try:
-        print(f"ntt exchange split={split} passbits={passbits} tile={tile} logn={logn}", flush=True)
+        print(f"ntt exchange split={split} defer={defer} passbits={passbits} tile={tile} logn={logn}", flush=True)
         for lg in (logn, logn + 2):
             c.set_ntt_plan(lg, passbits, tile)
         a = O.synth(0x60 + logn + passbits, split, 1 << logn)
