@@ -68,6 +68,16 @@ __device__ __forceinline__ void gstore(gfe *p, const fe &v) {
     p->re = v.re;
     p->im = v.im;
 }
+// A wave-uniform base plus a 32-bit per-lane element offset (< 2^28, so its
+// byte offset fits 32 bits): one offset register per access instead of a
+// 64-bit address per access, and the base stays in scalar registers.
+using gbyte = __attribute__((address_space(1))) uint8_t;
+__device__ __forceinline__ fe gload_at(const gfe *base, uint32_t off) {
+    return gload((const gfe *)((const gbyte *)base + 16u * off));
+}
+__device__ __forceinline__ void gstore_at(gfe *base, uint32_t off, const fe &v) {
+    gstore((gfe *)((gbyte *)base + 16u * off), v);
+}
 
 constexpr int Y_PAD = 1;   // one padding element per LDS row keeps 16-byte lanes off shared banks
 
@@ -212,14 +222,20 @@ __device__ __forceinline__ void ntt_pass_body(
     auto step1 = [&](int it, fe *x, int *off) {
         int c, d2;
         uint64_t base;
+        uint64_t ubase;      // the workgroup's part of base (wave-uniform)
+        uint32_t loff;       // the lane's part of the input offset at d1 = 0
         if (!LAST) {
             c = it & (C - 1);
             d2 = it >> logC;
-            base = (hi << logNt) + lo0 + c;
+            ubase = (hi << logNt) + lo0;
+            base = ubase + c;
+            loff = (uint32_t)c + ((uint32_t)d2 << logS);
         } else {
             d2 = it & (R2 - 1);
             c = it >> LOGR2;
+            ubase = (rest + c0 * W0) << LOGR;
             base = (rest + (c0 + c) * W0) << LOGR;
+            loff = (uint32_t)((c * W0) << LOGR) + (uint32_t)d2;
         }
         if (padded) {
 #pragma unroll
@@ -232,8 +248,8 @@ __device__ __forceinline__ void ntt_pass_body(
         } else {
 #pragma unroll
             for (int d1 = 0; d1 < R1; ++d1) {
-                const uint64_t d = (uint64_t)(R2 * d1 + d2);
-                x[d1] = gload(in + base + (LAST ? d : (d << logS)));
+                const uint64_t ud = (uint64_t)(R2 * d1);
+                x[d1] = gload_at(in + ubase + (LAST ? ud : (ud << logS)), loff);
             }
         }
         // deferred twiddles (inputs weak <= p + 7 from the previous pass's
@@ -277,8 +293,8 @@ __device__ __forceinline__ void ntt_pass_body(
             static_for<0, R2>([&](auto kk) {
                 constexpr int k2 = decltype(kk)::value;
                 constexpr int i = brev_c<R2>(k2);
-                const uint64_t k = (uint64_t)k0 + ((uint64_t)(k1 + R1 * k2) << fan);
-                gstore(out + (hi << logNt) + (k << logS) + lo0 + c, v[i]);
+                const uint64_t uk = (uint64_t)k0 + ((uint64_t)(R1 * k2) << fan);
+                gstore_at(out + (hi << logNt) + (uk << logS) + lo0, (uint32_t)c + ((uint32_t)k1 << (fan + logS)), v[i]);
             });
         } else if constexpr (!LAST) {
             // inter-pass twiddle w_{N_t}^(lo k), k = k0 + 2^fan (k1 + R1 k2): one table
@@ -293,17 +309,19 @@ __device__ __forceinline__ void ntt_pass_body(
             static_for<0, R2>([&](auto kk) {
                 constexpr int k2 = decltype(kk)::value;
                 constexpr int i = brev_c<R2>(k2);
-                const uint64_t k = (uint64_t)k0 + ((uint64_t)(k1 + R1 * k2) << fan);
-                gstore(out + (hi << logNt) + (k << logS) + lo, gf::mul_w(t, v[i]));   // weak: read by the next pass
+                const uint64_t uk = (uint64_t)k0 + ((uint64_t)(R1 * k2) << fan);
+                // weak: read by the next pass
+                gstore_at(out + (hi << logNt) + (uk << logS) + lo0, (uint32_t)c + ((uint32_t)k1 << (fan + logS)),
+                          gf::mul_w(t, v[i]));
                 if (k2 + 1 < R2) t = gf::mul(t, step);
             });
         } else {
             static_for<0, R2>([&](auto ii) {
                 constexpr int i = decltype(ii)::value;
                 constexpr int k2 = brev_c<R2>(i);
-                const uint64_t k = (uint64_t)(k1 + R1 * k2);
+                const uint64_t uk = (uint64_t)(R1 * k2);
                 v[i] = INV ? gf::mul(scale, v[i]) : gf::canon_w(v[i]);
-                gstore(out + nat_rest + c0 + c + (k << kshift), v[i]);
+                gstore_at(out + nat_rest + c0 + (uk << kshift), (uint32_t)c + ((uint32_t)k1 << kshift), v[i]);
             });
         }
     };
