@@ -502,12 +502,13 @@ def main():
     per_launch_bytes = alg[dom] / launches[dom]
     per_launch_s = dom_ms / 1e3 / launches[dom]
     achieved = per_launch_bytes / per_launch_s / 1e9
-    kernel = "ntt_pass_kernel"
+    # (the 16 x 16 passes over 16 columns use the split-exchange kernel)
+    kernel = "ntt_pass_split_kernel"
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     if os.path.exists(tfile):
         with open(tfile) as fh:
-            tj = json.load(fh).get(f"ntt_pass_kernel n=2^{log_cw} rate-{a.rate} encode")
+            tj = json.load(fh).get(f"{kernel} n=2^{log_cw} rate-{a.rate} encode")
         if tj and tj.get("passes") == launches[dom]:
             traffic = tj["bytes_per_launch"]
 
