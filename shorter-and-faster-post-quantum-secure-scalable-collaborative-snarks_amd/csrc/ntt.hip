@@ -44,6 +44,7 @@ struct NttPass {
     int logN, logR, logS, logC;
     int last;           // 1: stride-1 pass writing natural order
     int fan;            // first pass of a transform padded by 2^fan: fan out (see above)
+    int fan_blk;        // fan-out with k0 from the block index (one k0 per workgroup, see below)
     int npass;
     int pass;           // this pass's index
     int defer;          // inter-pass twiddles applied by the consuming pass (see below)
@@ -128,7 +129,15 @@ __device__ __forceinline__ void ntt_pass_body(
     const gfe *in = (const gfe *)(p.in_tab ? p.in_tab[by] : in0 + by * p.in_stride);
     gfe *out = (gfe *)(p.out_tab ? p.out_tab[by] : out0 + by * p.out_stride);
 
-    const uint64_t bid = blockIdx.x;
+    // A fan-out pass in block form gives each of the 2^fan values of k0 a
+    // workgroup of its own (block = tile * 2^fan + k0): the twists
+    // w_(E R)^(d k0) are then one more per-tile factor on step 1's inputs,
+    // applied like the deferred twiddles, and the split exchange needs no k0
+    // loop around it.  The tile's inputs are read by each of its 2^fan
+    // workgroups, which run side by side.
+    const bool fan_blk = !LAST && !INV && p.fan_blk;
+    const uint32_t k0_blk = fan_blk ? (uint32_t)(blockIdx.x & ((1u << p.fan) - 1)) : 0u;
+    const uint64_t bid = fan_blk ? (uint64_t)(blockIdx.x >> p.fan) : (uint64_t)blockIdx.x;
     // block -> positions
     uint64_t hi = 0, lo0 = 0;              // non-last: pos = hi*(R*S) + d*S + lo0 + c
     uint64_t rest = 0, c0 = 0, W0 = 0;      // last: row(c) = rest + (c0 + c)*W0, pos = row*R + d
@@ -146,7 +155,7 @@ __device__ __forceinline__ void ntt_pass_body(
     // E of the deferred twiddles: the earlier passes' frequencies held in
     // hi (non-last) or rest (last; k_0 is the column there and its share was
     // applied one pass earlier), digit-reversed
-    const bool consumer = (LAST || DEFER) && p.defer && p.pass >= 1;
+    const bool consumer = ((LAST || DEFER) && p.defer && p.pass >= 1) || (DEFER && fan_blk);
     uint64_t E = 0, k_0 = 0;
     if (consumer) {
         uint64_t x = LAST ? rest : hi;
@@ -167,7 +176,8 @@ __device__ __forceinline__ void ntt_pass_body(
         // the table of R1 x C values), as powers of the order-2^28 root
         const uint32_t m28 = (1u << NTT_LOG_MAX) - 1;
         const int sN = NTT_LOG_MAX - p.logN;
-        const uint32_t es = (uint32_t)((E << logS) << sN);   // w_N^(E S) as a power of w_28
+        // w_N^(E S) as a power of w_28; for a fan-out block, w_(E R)^(k0)
+        const uint32_t es = fan_blk ? (k0_blk << (NTT_LOG_MAX - LOGR - p.fan)) : (uint32_t)((E << logS) << sN);
         // Each entry is a product of factors that many lanes of a wave share
         // (w_N^(E S d2): R1 lanes; w_N^(E S R2 d1): C lanes) or that come from
         // the small radix table, so the table loads touch few cache lines.
@@ -186,10 +196,12 @@ __device__ __forceinline__ void ntt_pass_body(
     __syncthreads();
 
     // fan-out (first pass of a padded forward transform): the pass digit has
-    // LOGR + fan bits, and the steps below run once per k0 < 2^fan
-    // (the host never picks the split exchange for a fan-out pass: the k0
-    // loop around it costs the split kernel half its occupancy in registers)
-    const int fan = (LAST || INV || SPLIT) ? 0 : p.fan;
+    // LOGR + fan bits, and the steps below run once per k0 < 2^fan, or once
+    // for this block's k0 in block form (the only fan-out the split exchange
+    // takes: the k0 loop around it costs the split kernel half its occupancy
+    // in registers)
+    const int fan = (LAST || INV || (SPLIT && !fan_blk)) ? 0 : p.fan;
+    const int nk0 = fan_blk ? 1 : (1 << fan);
     const int logNt = LOGR + fan + logS;
     const int items1 = R2 << logC;
     const int items2 = R1 << logC;
@@ -213,8 +225,9 @@ __device__ __forceinline__ void ntt_pass_body(
     const bool padded = p.in_len < (1ull << p.logN);
     // fan-out: w_(E R)^(d k0) for the current k0, d < R (R * CP is even)
     fe *tz = SPLIT ? Y + (size_t)R * CP / 2 : Y + (size_t)R * CP;
-    for (int k0 = 0; k0 < (1 << fan); ++k0) {
-    if (k0 > 0) __syncthreads();   // the previous k0's step 2 is done with Y
+    for (int kk = 0; kk < nk0; ++kk) {
+    const int k0 = fan_blk ? (int)k0_blk : kk;
+    if (kk > 0) __syncthreads();   // the previous k0's step 2 is done with Y
 
     // ---- step 1: R1-point DFTs over d1 for each (d2, c) ----
     // item -> x[i] = the DFT output k1 = brev(i) after the w_R^(d2 k1)
@@ -243,7 +256,8 @@ __device__ __forceinline__ void ntt_pass_body(
                 const uint64_t d = (uint64_t)(R2 * d1 + d2);
                 const uint64_t pos = base + (LAST ? d : (d << logS));
                 x[d1] = pos < p.in_len ? gload(in + pos) : gf::zero();
-                if (k0 > 0) x[d1] = gf::mul_w(tz[d], x[d1]);   // twist by w_(E R)^(d k0); inputs are canonical
+                // twist by w_(E R)^(d k0) (block form: in the deferred tables); inputs are canonical
+                if (!fan_blk && k0 > 0) x[d1] = gf::mul_w(tz[d], x[d1]);
             }
         } else {
 #pragma unroll
@@ -277,7 +291,7 @@ __device__ __forceinline__ void ntt_pass_body(
     // with tz; the barrier at the top of the next k0 orders these writes
     // before its reads)
     auto next_twists = [&]() {
-        if (k0 + 1 < (1 << fan))
+        if (kk + 1 < nk0)
             for (int d = tid; d < R; d += NTT_THREADS)
                 tz[d] = tw28(twlo, twhi, (uint32_t)(d * (k0 + 1)) << (NTT_LOG_MAX - LOGR - fan));
     };
@@ -476,12 +490,12 @@ int launch_pass(cosnark_ctx *ctx, hipStream_t stream, uint64_t blocks, int batch
                 const NttPass &q, bool last, bool inverse, const fe *twlo, const fe *twhi,
                 const fe *rtab, fe scale) {
     // the split exchange needs at most one item of each step per thread
-    const bool split = ctx->ntt_split && !q.fan && R2 >= 2 && ((R1 << q.logC) <= NTT_THREADS) &&
+    const bool split = ctx->ntt_split && (!q.fan || q.fan_blk) && R2 >= 2 && ((R1 << q.logC) <= NTT_THREADS) &&
                        ((R2 << q.logC) <= NTT_THREADS);
     PassKernel k = pass_kernel<R1, R2>(last, inverse, split, q.defer != 0);
     // radix table, padded tile (split: 8-byte words), and for a fan-out pass the R twists
     const size_t R = (size_t)R1 * R2, CP = (1u << q.logC) + Y_PAD;
-    const size_t lds = sizeof(fe) * (R + (size_t)q.ut_elems + (q.fan ? R : 0)) + (split ? 8 : 16) * R * CP;
+    const size_t lds = sizeof(fe) * (R + (size_t)q.ut_elems + (q.fan && !q.fan_blk ? R : 0)) + (split ? 8 : 16) * R * CP;
     hipLaunchKernelGGL(k, dim3((unsigned)blocks, (unsigned)batch), dim3(NTT_THREADS), lds, stream, in, out, q, twlo,
                        twhi, rtab, scale);
     LAUNCHCK();
@@ -613,8 +627,15 @@ int ntt_launch_batch(cosnark_ctx *ctx, const fe *in, const fe *const *in_tab, ui
         q.pass = t;
         q.ut_lo = (defer && t >= 1 && t == m - 2) ? 1 : 0;
         q.ut_elems = (defer && t >= 1) ? (q.ut_lo ? (r1 << logC) : r1) : 0;
+        // a fan-out first pass takes the block form (one k0 per workgroup,
+        // the twists in the deferred tables) where the split exchange fits
+        // (the step radices of the transform's rd bits, as pass_kernel_for splits them)
+        const int t1 = rd >= 7 ? 16 : rd >= 5 ? 8 : rd >= 3 ? 4 : 2, t2 = 1 << (rd - ilog2((uint64_t)t1));
+        q.fan_blk = (t == 0 && fan && defer && ctx->ntt_split && t2 >= 2 && (t1 << logC) <= NTT_THREADS &&
+                     (t2 << logC) <= NTT_THREADS) ? 1 : 0;
+        if (q.fan_blk) q.ut_elems = t1;
         if (t > 0) q.in_len = N;
-        const uint64_t blocks = N >> (r + logC);
+        const uint64_t blocks = (N >> (r + logC)) << (q.fan_blk ? fan : 0);
         const fe *src;
         fe *dst;
         if (t == 0) {

@@ -134,3 +134,42 @@ def test_ntt_exchange_and_twiddle_forms_match_oracle(split, defer, passbits, til
         assert np.array_equal(c.download(nodes, (cnt, 32), np.uint8), O.merkle_commit(ref_cw, True))
     finally:
         c.close()
+
+
+@pytest.mark.parametrize("split", [0, 1])
+@pytest.mark.parametrize("logn,rate,passbits,tile", [(17, 4, 6, 1024), (17, 4, 6, 4096), (15, 8, 5, 512),
+                                                     (15, 8, 5, 2048)])
+def test_rs_encode_fan_out_deferred_matches_oracle(split, logn, rate, passbits, tile):
+    """Fan-out first passes in plans of three passes (deferred twiddles):
+    with the split exchange and narrow enough tiles the first pass runs in
+    block form (one k0 per workgroup, the twists in the per-tile tables):
+    2^19 points as (2 + 6) + 6 + 5 bits at tile 1024, 2^18 as (3 + 5) + 5 + 5
+    at tile 512; at wider tiles (and without the split exchange) the first
+    pass runs its k0 loop inside the workgroup.  The 2^25-point encode of
+    test_rs_encode_four_pass_matches_oracle takes the block form at the
+    default plan, (3 + 8) + 7 + 7 bits.  The codeword
+    and every tree node (leaf digests from the last pass) against the
+    oracle."""
+    import os
+    import cosnark
+    env = {"COSNARK_NTT_FAN_MIN": "1", "COSNARK_NTT_SPLIT": str(split)}
+    os.environ.update(env)
+    try:
+        c = cosnark.Context(0)
+    finally:
+        for k in env:
+            del os.environ[k]
+    try:
+        n = 1 << logn
+        lg = logn + rate.bit_length() - 1
+        print(f"fan-out deferred split={split} logn={logn} rate={rate} passbits={passbits} tile={tile}", flush=True)
+        c.set_ntt_plan(lg, passbits, tile)
+        a = O.synth(0xFA + logn + passbits, rate, n)
+        ref_cw = O.rs_encode(a, rate)
+        assert np.array_equal(c.rs_encode(a, rate), ref_cw)
+        cw, nodes = c.whir_commit(c.upload(a), n, rate)
+        assert np.array_equal(c.download(cw, (n * rate, 2)), ref_cw)
+        cnt = cosnark.merkle_node_count(n * rate)
+        assert np.array_equal(c.download(nodes, (cnt, 32), np.uint8), O.merkle_commit(ref_cw, True))
+    finally:
+        c.close()
