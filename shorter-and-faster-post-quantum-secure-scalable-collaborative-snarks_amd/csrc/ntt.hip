@@ -218,9 +218,14 @@ __device__ __forceinline__ void ntt_pass_body(
     }
     // the fused leaf hashing below needs the last item's outputs after the
     // loop (it is enabled only when each thread has at most one item)
-    const bool leaves_on = LAST && p.leaves && by == 0 && items2 <= NTT_THREADS;
-    fe v[R2];
-    int my_c = 0, my_k1 = 0;
+    // Where step 1's transform is twice step 2's (R1 = 2 R2: radix 128 as
+    // 16 x 8), the split form gives each thread two step-2 items, the same
+    // element count as its one step-1 item, so neither step idles half the
+    // threads.
+    constexpr int MI2 = (SPLIT && R1 == 2 * R2) ? 2 : 1;
+    const bool leaves_on = LAST && p.leaves && by == 0 && items2 <= MI2 * NTT_THREADS;
+    fe vv[MI2][R2];
+    int my_c[MI2] = {}, my_k1[MI2] = {};
     // zero padding (only the first pass reads fewer than 2^logN inputs)
     const bool padded = p.in_len < (1ull << p.logN);
     // fan-out: w_(E R)^(d k0) for the current k0, d < R (R * CP is even)
@@ -296,11 +301,12 @@ __device__ __forceinline__ void ntt_pass_body(
                 tz[d] = tw28(twlo, twhi, (uint32_t)(d * (k0 + 1)) << (NTT_LOG_MAX - LOGR - fan));
     };
     // ---- step 2: R2-point DFTs over d2 for each (k1, c), twiddle, store ----
-    // (v holds the item's exchange column: v[d2] = row k1 * R2 + d2, column c)
-    auto step2 = [&](int it) {
+    // (v holds the item's exchange column: v[d2] = row k1 * R2 + d2, column c;
+    // q: the thread's item slot)
+    auto step2 = [&](int it, fe *v, int q) {
         const int c = it & (C - 1), k1 = it >> logC;
-        my_c = c;
-        my_k1 = k1;
+        my_c[q] = c;
+        my_k1[q] = k1;
         dif_w<R2, INV>(v);
         if constexpr (!LAST && DEFER) {
             // the inter-pass twiddle is left to the next passes; weak stores
@@ -355,16 +361,16 @@ __device__ __forceinline__ void ntt_pass_body(
         for (int it = tid; it < items2; it += NTT_THREADS) {
             const int c = it & (C - 1), k1 = it >> logC;
 #pragma unroll
-            for (int d2 = 0; d2 < R2; ++d2) v[d2] = Y[(k1 * R2 + d2) * CP + c];
-            step2(it);
+            for (int d2 = 0; d2 < R2; ++d2) vv[0][d2] = Y[(k1 * R2 + d2) * CP + c];
+            step2(it, vv[0], 0);
         }
     } else {
-        // at most one item of each step per thread (items1, items2 <= NTT_THREADS)
-        const bool act1 = tid < items1, act2 = tid < items2;
+        // at most one step-1 item and MI2 step-2 items per thread
+        // (items1 <= NTT_THREADS, items2 <= MI2 * NTT_THREADS)
+        const bool act1 = tid < items1;
         fe x[R1];
         int off = 0;
         if (act1) step1(tid, x, &off);
-        const int c2 = tid & (C - 1), k12 = tid >> logC;
         if (act1)
             static_for<0, R1>([&](auto ii) {
                 constexpr int i = decltype(ii)::value;
@@ -372,9 +378,15 @@ __device__ __forceinline__ void ntt_pass_body(
             });
         __syncthreads();
         next_twists();
-        if (act2)
+        static_for<0, MI2>([&](auto qq) {
+            constexpr int q = decltype(qq)::value;
+            const int it = tid + q * NTT_THREADS;
+            if (it < items2) {
+                const int c2 = it & (C - 1), k12 = it >> logC;
 #pragma unroll
-            for (int d2 = 0; d2 < R2; ++d2) v[d2].re = Yw[(k12 * R2 + d2) * CP + c2];
+                for (int d2 = 0; d2 < R2; ++d2) vv[q][d2].re = Yw[(k12 * R2 + d2) * CP + c2];
+            }
+        });
         __syncthreads();
         if (act1)
             static_for<0, R1>([&](auto ii) {
@@ -382,11 +394,16 @@ __device__ __forceinline__ void ntt_pass_body(
                 Yw[brev_c<R1>(i) * R2 * CP + off] = x[i].im;
             });
         __syncthreads();
-        if (act2) {
+        static_for<0, MI2>([&](auto qq) {
+            constexpr int q = decltype(qq)::value;
+            const int it = tid + q * NTT_THREADS;
+            if (it < items2) {
+                const int c2 = it & (C - 1), k12 = it >> logC;
 #pragma unroll
-            for (int d2 = 0; d2 < R2; ++d2) v[d2].im = Yw[(k12 * R2 + d2) * CP + c2];
-            step2(tid);
-        }
+                for (int d2 = 0; d2 < R2; ++d2) vv[q][d2].im = Yw[(k12 * R2 + d2) * CP + c2];
+                step2(it, vv[q], q);
+            }
+        });
     }
     }   // k0
     if (leaves_on) {
@@ -403,14 +420,17 @@ __device__ __forceinline__ void ntt_pass_body(
         static_for<0, NH>([&](auto hh) {
             constexpr int h = decltype(hh)::value;
             __syncthreads();   // the exchange buffer is free (step-2 reads, or the previous half's hashes)
-            if (tid < items2) {
-                static_for<0, R2>([&](auto ii) {
-                    constexpr int i = decltype(ii)::value;
-                    constexpr int k2 = brev_c<R2>(i);
-                    const int hc = decltype(hh)::value;
-                    if (k2 / (R2 / NH) == hc) Y[(my_k1 + R1 * k2 - hc * RH) * CP + my_c] = v[i];
-                });
-            }
+            static_for<0, MI2>([&](auto qq) {
+                constexpr int q = decltype(qq)::value;
+                if (tid + q * NTT_THREADS < items2) {
+                    static_for<0, R2>([&](auto ii) {
+                        constexpr int i = decltype(ii)::value;
+                        constexpr int k2 = brev_c<R2>(i);
+                        const int hc = decltype(hh)::value;
+                        if (k2 / (R2 / NH) == hc) Y[(my_k1[q] + R1 * k2 - hc * RH) * CP + my_c[q]] = vv[q][i];
+                    });
+                }
+            });
             __syncthreads();
             for (int l = tid; l < nleaf; l += NTT_THREADS) {
                 const int k = l / qpr, q = l % qpr;
@@ -490,8 +510,9 @@ int launch_pass(cosnark_ctx *ctx, hipStream_t stream, uint64_t blocks, int batch
                 const NttPass &q, bool last, bool inverse, const fe *twlo, const fe *twhi,
                 const fe *rtab, fe scale) {
     // the split exchange needs at most one item of each step per thread
-    const bool split = ctx->ntt_split && (!q.fan || q.fan_blk) && R2 >= 2 && ((R1 << q.logC) <= NTT_THREADS) &&
-                       ((R2 << q.logC) <= NTT_THREADS);
+    // (two step-2 items per thread where R1 = 2 R2)
+    const bool split = ctx->ntt_split && (!q.fan || q.fan_blk) && R2 >= 2 &&
+                       ((R1 << q.logC) <= (R1 == 2 * R2 ? 2 : 1) * NTT_THREADS) && ((R2 << q.logC) <= NTT_THREADS);
     PassKernel k = pass_kernel<R1, R2>(last, inverse, split, q.defer != 0);
     // radix table, padded tile (split: 8-byte words), and for a fan-out pass the R twists
     const size_t R = (size_t)R1 * R2, CP = (1u << q.logC) + Y_PAD;
@@ -501,6 +522,10 @@ int launch_pass(cosnark_ctx *ctx, hipStream_t stream, uint64_t blocks, int batch
     LAUNCHCK();
     return COSNARK_OK;
 }
+
+// step-2 items one thread may hold in a pass whose transform has rd bits:
+// two where the split exchange applies and R1 = 2 R2 (odd rd >= 3)
+int items2_per_thread(const cosnark_ctx *ctx, int rd) { return (ctx->ntt_split && (rd & 1) && rd >= 3) ? 2 : 1; }
 
 int dispatch_pass(cosnark_ctx *ctx, hipStream_t stream, int r, uint64_t blocks, int batch, const fe *in, fe *out,
                   const NttPass &q, bool last, bool inverse, const fe *twlo, const fe *twhi, const fe *rtab,
@@ -619,9 +644,10 @@ int ntt_launch_batch(cosnark_ctx *ctx, const fe *in, const fe *const *in_tab, ui
             if (m == 1) logC = 0;
             else if (logC > p.rbits[0]) logC = p.rbits[0];
             // with leaf digests wanted, narrow the tile until each thread
-            // holds at most one step-2 item (R1 * C <= NTT_THREADS)
+            // holds at most one step-2 item (R1 * C <= NTT_THREADS; two in
+            // the split form where R1 = 2 R2)
             if (leaves && m >= 2)
-                while (logC > 2 && (r1 << logC) > NTT_THREADS) --logC;
+                while (logC > 2 && (r1 << logC) > items2_per_thread(ctx, rd) * NTT_THREADS) --logC;
         }
         q.logC = logC;
         q.pass = t;
@@ -631,8 +657,8 @@ int ntt_launch_batch(cosnark_ctx *ctx, const fe *in, const fe *const *in_tab, ui
         // the twists in the deferred tables) where the split exchange fits
         // (the step radices of the transform's rd bits, as pass_kernel_for splits them)
         const int t1 = rd >= 7 ? 16 : rd >= 5 ? 8 : rd >= 3 ? 4 : 2, t2 = 1 << (rd - ilog2((uint64_t)t1));
-        q.fan_blk = (t == 0 && fan && defer && ctx->ntt_split && t2 >= 2 && (t1 << logC) <= NTT_THREADS &&
-                     (t2 << logC) <= NTT_THREADS) ? 1 : 0;
+        q.fan_blk = (t == 0 && fan && defer && ctx->ntt_split && t2 >= 2 &&
+                     (t1 << logC) <= items2_per_thread(ctx, rd) * NTT_THREADS && (t2 << logC) <= NTT_THREADS) ? 1 : 0;
         if (q.fan_blk) q.ut_elems = t1;
         if (t > 0) q.in_len = N;
         const uint64_t blocks = (N >> (r + logC)) << (q.fan_blk ? fan : 0);
@@ -652,8 +678,9 @@ int ntt_launch_batch(cosnark_ctx *ctx, const fe *in, const fe *const *in_tab, ui
             q.out_tab = out_tab;
             q.out_stride = N;
             // fused leaf digests: whole leaves per tile (C >= 4) and at most
-            // one step-2 item per thread (R1 * C <= NTT_THREADS)
-            if (leaves && m >= 2 && logC >= 2 && (r1 << logC) <= NTT_THREADS) {
+            // one step-2 item per thread (two in the split form where R1 =
+            // 2 R2; the pass then takes that form, see launch_pass)
+            if (leaves && m >= 2 && logC >= 2 && (r1 << logC) <= items2_per_thread(ctx, rd) * NTT_THREADS) {
                 q.leaves = leaves;
                 if (levels_done) *levels_done = 1;
             }
