@@ -218,11 +218,12 @@ __device__ __forceinline__ void ntt_pass_body(
     }
     // the fused leaf hashing below needs the last item's outputs after the
     // loop (it is enabled only when each thread has at most one item)
-    // Where step 1's transform is twice step 2's (R1 = 2 R2: radix 128 as
-    // 16 x 8), the split form gives each thread two step-2 items, the same
-    // element count as its one step-1 item, so neither step idles half the
-    // threads.
-    constexpr int MI2 = (SPLIT && R1 == 2 * R2) ? 2 : 1;
+    // The split form gives each thread up to 16 elements in each step:
+    // 16 / R1 step-1 items and 16 / R2 step-2 items (one each for 16 x 16,
+    // one and two for 16 x 8, two and two for 8 x 8, ...), so a 4096-element
+    // tile keeps every thread busy in both steps.
+    constexpr int MI1 = (SPLIT && R1 < 16) ? 16 / R1 : 1;
+    constexpr int MI2 = (SPLIT && R2 < 16) ? 16 / R2 : 1;
     const bool leaves_on = LAST && p.leaves && by == 0 && items2 <= MI2 * NTT_THREADS;
     fe vv[MI2][R2];
     int my_c[MI2] = {}, my_k1[MI2] = {};
@@ -365,17 +366,22 @@ __device__ __forceinline__ void ntt_pass_body(
             step2(it, vv[0], 0);
         }
     } else {
-        // at most one step-1 item and MI2 step-2 items per thread
-        // (items1 <= NTT_THREADS, items2 <= MI2 * NTT_THREADS)
-        const bool act1 = tid < items1;
-        fe x[R1];
-        int off = 0;
-        if (act1) step1(tid, x, &off);
-        if (act1)
-            static_for<0, R1>([&](auto ii) {
-                constexpr int i = decltype(ii)::value;
-                Yw[brev_c<R1>(i) * R2 * CP + off] = x[i].re;
-            });
+        // at most MI1 step-1 and MI2 step-2 items per thread
+        // (items1 <= MI1 * NTT_THREADS, items2 <= MI2 * NTT_THREADS)
+        fe x[MI1][R1];
+        int off[MI1] = {};
+        static_for<0, MI1>([&](auto qq) {
+            constexpr int q = decltype(qq)::value;
+            if (tid + q * NTT_THREADS < items1) step1(tid + q * NTT_THREADS, x[q], &off[q]);
+        });
+        static_for<0, MI1>([&](auto qq) {
+            constexpr int q = decltype(qq)::value;
+            if (tid + q * NTT_THREADS < items1)
+                static_for<0, R1>([&](auto ii) {
+                    constexpr int i = decltype(ii)::value;
+                    Yw[brev_c<R1>(i) * R2 * CP + off[q]] = x[q][i].re;
+                });
+        });
         __syncthreads();
         next_twists();
         static_for<0, MI2>([&](auto qq) {
@@ -388,11 +394,14 @@ __device__ __forceinline__ void ntt_pass_body(
             }
         });
         __syncthreads();
-        if (act1)
-            static_for<0, R1>([&](auto ii) {
-                constexpr int i = decltype(ii)::value;
-                Yw[brev_c<R1>(i) * R2 * CP + off] = x[i].im;
-            });
+        static_for<0, MI1>([&](auto qq) {
+            constexpr int q = decltype(qq)::value;
+            if (tid + q * NTT_THREADS < items1)
+                static_for<0, R1>([&](auto ii) {
+                    constexpr int i = decltype(ii)::value;
+                    Yw[brev_c<R1>(i) * R2 * CP + off[q]] = x[q][i].im;
+                });
+        });
         __syncthreads();
         static_for<0, MI2>([&](auto qq) {
             constexpr int q = decltype(qq)::value;
@@ -510,9 +519,10 @@ int launch_pass(cosnark_ctx *ctx, hipStream_t stream, uint64_t blocks, int batch
                 const NttPass &q, bool last, bool inverse, const fe *twlo, const fe *twhi,
                 const fe *rtab, fe scale) {
     // the split exchange needs at most one item of each step per thread
-    // (two step-2 items per thread where R1 = 2 R2)
+    // (up to 16 elements per thread in each step: 16 / R1 step-1 items,
+    // 16 / R2 step-2 items)
     const bool split = ctx->ntt_split && (!q.fan || q.fan_blk) && R2 >= 2 &&
-                       ((R1 << q.logC) <= (R1 == 2 * R2 ? 2 : 1) * NTT_THREADS) && ((R2 << q.logC) <= NTT_THREADS);
+                       ((R2 << q.logC) <= (16 / R1) * NTT_THREADS) && ((R1 << q.logC) <= (16 / R2) * NTT_THREADS);
     PassKernel k = pass_kernel<R1, R2>(last, inverse, split, q.defer != 0);
     // radix table, padded tile (split: 8-byte words), and for a fan-out pass the R twists
     const size_t R = (size_t)R1 * R2, CP = (1u << q.logC) + Y_PAD;
@@ -524,8 +534,12 @@ int launch_pass(cosnark_ctx *ctx, hipStream_t stream, uint64_t blocks, int batch
 }
 
 // step-2 items one thread may hold in a pass whose transform has rd bits:
-// two where the split exchange applies and R1 = 2 R2 (odd rd >= 3)
-int items2_per_thread(const cosnark_ctx *ctx, int rd) { return (ctx->ntt_split && (rd & 1) && rd >= 3) ? 2 : 1; }
+// 16 / R2 where the split exchange applies (rd >= 2), else one
+int items2_per_thread(const cosnark_ctx *ctx, int rd) {
+    if (!ctx->ntt_split || rd < 2) return 1;
+    const int r1 = rd >= 7 ? 16 : rd >= 5 ? 8 : rd >= 3 ? 4 : 2;
+    return 16 / (1 << (rd - ilog2((uint64_t)r1)));
+}
 
 int dispatch_pass(cosnark_ctx *ctx, hipStream_t stream, int r, uint64_t blocks, int batch, const fe *in, fe *out,
                   const NttPass &q, bool last, bool inverse, const fe *twlo, const fe *twhi, const fe *rtab,
@@ -658,7 +672,8 @@ int ntt_launch_batch(cosnark_ctx *ctx, const fe *in, const fe *const *in_tab, ui
         // (the step radices of the transform's rd bits, as pass_kernel_for splits them)
         const int t1 = rd >= 7 ? 16 : rd >= 5 ? 8 : rd >= 3 ? 4 : 2, t2 = 1 << (rd - ilog2((uint64_t)t1));
         q.fan_blk = (t == 0 && fan && defer && ctx->ntt_split && t2 >= 2 &&
-                     (t1 << logC) <= items2_per_thread(ctx, rd) * NTT_THREADS && (t2 << logC) <= NTT_THREADS) ? 1 : 0;
+                     (t1 << logC) <= items2_per_thread(ctx, rd) * NTT_THREADS &&
+                     (t2 << logC) <= (16 / t1) * NTT_THREADS) ? 1 : 0;
         if (q.fan_blk) q.ut_elems = t1;
         if (t > 0) q.in_len = N;
         const uint64_t blocks = (N >> (r + logC)) << (q.fan_blk ? fan : 0);
