@@ -7,20 +7,23 @@
 //
 // Design: a 2^L-point transform is split into passes of radix 2^r (r <= 8).
 // Pass t transforms one index digit of stride S_t = 2^(L - r_0 - .. - r_t)
-// for every (prefix, suffix) pair: a workgroup stages R x C elements
-// (R = 2^r points for C adjacent columns, C*R = 4096) in LDS, runs radix-2
-// decimation-in-time stages there, multiplies by the inter-pass twiddle
-// w_{N_t}^(lo*k) and writes back in place.  The last pass (stride 1) writes
-// in natural order; its workgroup takes C rows whose leading output digit
-// is consecutive, so each store is a C-element contiguous run.  Inside a
-// pass values are weak (gf.h: unreduced sums, one fold per radix-4 layer);
-// the twiddle products and the last pass's stores are canonical.  Zero padding
-// (Reed-Solomon encoding of n coefficients into rate*n points) is handled by
-// the first pass, which reads only the n real inputs.  Where padding by
-// E = 2^e would cost a pass, the first pass fans out instead: its digit is
-// e bits wider than its transform, and since only the first R of its E*R
-// inputs are non-zero, X[k0 + E k1] = DFT_R(x[d] w_(ER)^(d k0))[k1]; the
-// workgroup runs the R-point transform once per k0 < E on its twisted tile.
+// for every (prefix, suffix) pair: a 256-thread workgroup stages R x C
+// elements (R = 2^r points for C adjacent columns, C*R <= 4096) and runs the
+// R-point DFT as a four-step R1 x R2 in registers (step 1, twiddle, LDS
+// exchange, step 2).  The inter-pass twiddle w_{N_t}^(lo*k) is either
+// applied at the store (two-pass plans) or left to the passes that read the
+// output (deferred twiddles, see ntt_pass_body).  The last pass (stride 1)
+// writes in natural order; its workgroup takes C rows whose leading output
+// digit is consecutive, so each store is a C-element contiguous run, and it
+// can hash the Merkle leaves of its tile.  Inside a pass values are weak
+// (gf.h: unreduced sums, one fold per radix-4 layer); the last pass's stores
+// are canonical.  Zero padding (Reed-Solomon encoding of n coefficients into
+// rate*n points) is handled by the first pass, which reads only the n real
+// inputs.  Where padding by E = 2^e would cost a pass, the first pass fans
+// out instead: its digit is e bits wider than its transform, and since only
+// the first R of its E*R inputs are non-zero, X[k0 + E k1] =
+// DFT_R(x[d] w_(ER)^(d k0))[k1]; the R-point transform runs once per
+// k0 < E, in one workgroup per k0 (block form) or in a loop.
 #include "blake3.h"
 #include "ctx.h"
 #include "dft.h"
