@@ -137,6 +137,7 @@ int cosnark_create(int device, cosnark_ctx **out) {
     }
     if (const char *t = getenv("COSNARK_NTT_SPLIT")) ctx->ntt_split = atoi(t) != 0;
     if (const char *t = getenv("COSNARK_NTT_DEFER")) ctx->ntt_defer = atoi(t) != 0;
+    if (const char *t = getenv("COSNARK_NTT_LEAF_LEVELS")) ctx->ntt_leaf_levels = atoi(t) >= 3 ? 3 : 1;
     if (const char *t = getenv("COSNARK_NTT_FAN_MIN")) {
         const int v = atoi(t);
         if (v >= 1) ctx->ntt_fan_min_blocks = v;

@@ -56,6 +56,7 @@ struct NttPass {
     int rbits[8];       // radix bits per pass, first pass first
     int pre[8];         // pre[t] = rbits[0] + .. + rbits[t-1]
     uint8_t *leaves;    // last pass, batch item 0: Merkle leaf digests of the output (or null)
+    int leaf_lv;        // tree levels the last pass writes at `leaves` (1, or 3: levels 1 and 2 too)
 };
 
 // loads and stores through global-address-space pointers (global_load /
@@ -427,8 +428,22 @@ __device__ __forceinline__ void ntt_pass_body(
         // The condition is uniform over the workgroup, so are the barriers.
         // SPLIT halves the buffer, so it restages and hashes rows k < R/2
         // (k2 < R2/2) first and the rest second.
+        //
+        // With p.leaf_lv == 3 the pass also writes tree levels 1 and 2.  The
+        // tree hashes a node's left child twice (src/merkle_tree.cpp:287-292),
+        // so level-1 node j is BLAKE3(leaf 2j || leaf 2j) and level-2 node i
+        // depends on leaf 4i alone.  The host sets it only where a half holds
+        // one unit of two consecutive leaves per thread (R C = 16 x 256), a
+        // row holds whole groups of four leaves (C a multiple of 16) and rows
+        // start at multiples of four leaves: a thread hashes its two leaves
+        // and the level-1 node of the first, and after the second half the
+        // level-2 nodes go one per thread, the odd units' lanes taking the
+        // second half's from their even neighbour by DPP.
         constexpr int NH = (SPLIT && R2 >= 2) ? 2 : 1, RH = R / NH;
         const int qpr = C / 4, nleaf = (RH * C) / 4;
+        const bool lv3 = NH == 2 && p.leaf_lv == 3;
+        const uint64_t n_lf = 1ull << (p.logN - 2);    // leaves: level 1 starts there, level 2 at 3/2 of it
+        uint32_t carry[8];                              // first half: the unit's level-1 digest
         static_for<0, NH>([&](auto hh) {
             constexpr int h = decltype(hh)::value;
             __syncthreads();   // the exchange buffer is free (step-2 reads, or the previous half's hashes)
@@ -444,10 +459,7 @@ __device__ __forceinline__ void ntt_pass_body(
                 }
             });
             __syncthreads();
-            for (int l = tid; l < nleaf; l += NTT_THREADS) {
-                const int k = l / qpr, q = l % qpr;
-                uint32_t m[16], hv[8];
-                const fe *src = Y + k * CP + 4 * q;
+            auto leaf_msg = [&](const fe *src, uint32_t *m) {
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     m[4 * e] = (uint32_t)src[e].re;
@@ -455,11 +467,61 @@ __device__ __forceinline__ void ntt_pass_body(
                     m[4 * e + 2] = (uint32_t)src[e].im;
                     m[4 * e + 3] = (uint32_t)(src[e].im >> 32);
                 }
-                b3::compress64(m, hv);
-                const uint64_t pos = nat_rest + c0 + 4 * q + ((uint64_t)(k + h * RH) << kshift);
-                uint4 *dst = reinterpret_cast<uint4 *>(p.leaves + 32 * (pos >> 2));
-                dst[0] = make_uint4(hv[0], hv[1], hv[2], hv[3]);
-                dst[1] = make_uint4(hv[4], hv[5], hv[6], hv[7]);
+            };
+            auto self_pair = [](const uint32_t *d, uint32_t *out) {
+                uint32_t m[16];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) { m[j] = d[j]; m[8 + j] = d[j]; }
+                b3::compress64(m, out);
+            };
+            if (lv3) {
+                const int upr = qpr / 2, k = tid / upr, up = tid % upr;
+                const fe *src = Y + k * CP + 8 * up;
+                uint32_t m[16], h0[8], h1[8], l1[8];
+                leaf_msg(src, m);
+                b3::compress64(m, h0);
+                leaf_msg(src + 4, m);
+                b3::compress64(m, h1);
+                const uint64_t li = (nat_rest + c0 + 8 * up + ((uint64_t)(k + h * RH) << kshift)) >> 2;
+                uint4 *dst = reinterpret_cast<uint4 *>(p.leaves + 32 * li);
+                dst[0] = make_uint4(h0[0], h0[1], h0[2], h0[3]);
+                dst[1] = make_uint4(h0[4], h0[5], h0[6], h0[7]);
+                dst[2] = make_uint4(h1[0], h1[1], h1[2], h1[3]);
+                dst[3] = make_uint4(h1[4], h1[5], h1[6], h1[7]);
+                self_pair(h0, l1);
+                uint4 *d1 = reinterpret_cast<uint4 *>(p.leaves + 32 * (n_lf + (li >> 1)));
+                d1[0] = make_uint4(l1[0], l1[1], l1[2], l1[3]);
+                d1[1] = make_uint4(l1[4], l1[5], l1[6], l1[7]);
+                if (h == 0) {
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) carry[j] = l1[j];
+                } else {
+                    // lane 2i + 1 reads lane 2i's level-1 digest (quad_perm 0, 0, 2, 2)
+                    const bool odd = (up & 1) != 0;
+                    uint32_t in[8], l2[8];
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        const uint32_t nb = (uint32_t)__builtin_amdgcn_mov_dpp((int)l1[j], 0xA0, 0xF, 0xF, false);
+                        in[j] = odd ? nb : carry[j];
+                    }
+                    self_pair(in, l2);
+                    const int row = odd ? k + RH : k;
+                    const uint64_t i2 = (nat_rest + c0 + 8 * (up & ~1) + ((uint64_t)row << kshift)) >> 4;
+                    uint4 *d2 = reinterpret_cast<uint4 *>(p.leaves + 32 * (n_lf + (n_lf >> 1) + i2));
+                    d2[0] = make_uint4(l2[0], l2[1], l2[2], l2[3]);
+                    d2[1] = make_uint4(l2[4], l2[5], l2[6], l2[7]);
+                }
+            } else {
+                for (int l = tid; l < nleaf; l += NTT_THREADS) {
+                    const int k = l / qpr, q = l % qpr;
+                    uint32_t m[16], hv[8];
+                    leaf_msg(Y + k * CP + 4 * q, m);
+                    b3::compress64(m, hv);
+                    const uint64_t pos = nat_rest + c0 + 4 * q + ((uint64_t)(k + h * RH) << kshift);
+                    uint4 *dst = reinterpret_cast<uint4 *>(p.leaves + 32 * (pos >> 2));
+                    dst[0] = make_uint4(hv[0], hv[1], hv[2], hv[3]);
+                    dst[1] = make_uint4(hv[4], hv[5], hv[6], hv[7]);
+                }
             }
         });
     }
@@ -517,15 +579,19 @@ PassKernel pass_kernel_for(int r, bool last, bool inverse, bool split, bool defe
     }
 }
 
+// the split exchange needs at most one item of each step per thread (up to
+// 16 elements per thread in each step: 16 / R1 step-1 items, 16 / R2 step-2
+// items)
+bool pass_split(const cosnark_ctx *ctx, int R1, int R2, const NttPass &q) {
+    return ctx->ntt_split && (!q.fan || q.fan_blk) && R2 >= 2 && ((R2 << q.logC) <= (16 / R1) * NTT_THREADS) &&
+           ((R1 << q.logC) <= (16 / R2) * NTT_THREADS);
+}
+
 template <int R1, int R2>
 int launch_pass(cosnark_ctx *ctx, hipStream_t stream, uint64_t blocks, int batch, const fe *in, fe *out,
                 const NttPass &q, bool last, bool inverse, const fe *twlo, const fe *twhi,
                 const fe *rtab, fe scale) {
-    // the split exchange needs at most one item of each step per thread
-    // (up to 16 elements per thread in each step: 16 / R1 step-1 items,
-    // 16 / R2 step-2 items)
-    const bool split = ctx->ntt_split && (!q.fan || q.fan_blk) && R2 >= 2 &&
-                       ((R2 << q.logC) <= (16 / R1) * NTT_THREADS) && ((R1 << q.logC) <= (16 / R2) * NTT_THREADS);
+    const bool split = pass_split(ctx, R1, R2, q);
     PassKernel k = pass_kernel<R1, R2>(last, inverse, split, q.defer != 0);
     // radix table, padded tile (split: 8-byte words), and for a fan-out pass the R twists
     const size_t R = (size_t)R1 * R2, CP = (1u << q.logC) + Y_PAD;
@@ -700,7 +766,16 @@ int ntt_launch_batch(cosnark_ctx *ctx, const fe *in, const fe *const *in_tab, ui
             // 2 R2; the pass then takes that form, see launch_pass)
             if (leaves && m >= 2 && logC >= 2 && (r1 << logC) <= items2_per_thread(ctx, rd) * NTT_THREADS) {
                 q.leaves = leaves;
-                if (levels_done) *levels_done = 1;
+                q.leaf_lv = 1;
+                // levels 1 and 2 as well (dup-left trees): the split form with
+                // one two-leaf unit per thread in each half (R C = 4096), whole
+                // four-leaf groups per row (C >= 16) and rows at multiples of
+                // four leaves (logn - r >= 4)
+                const int r2 = 1 << (r - ilog2((uint64_t)r1));
+                if (ctx->ntt_leaf_levels >= 3 && pass_split(ctx, r1, r2, q) && ((r1 * r2) << logC) == 16 * NTT_THREADS &&
+                    logC >= 4 && logn - r >= 4)
+                    q.leaf_lv = 3;
+                if (levels_done) *levels_done = q.leaf_lv;
             }
         } else {
             dst = tmp;

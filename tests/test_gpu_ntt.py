@@ -173,3 +173,36 @@ def test_rs_encode_fan_out_deferred_matches_oracle(split, logn, rate, passbits, 
         assert np.array_equal(c.download(nodes, (cnt, 32), np.uint8), O.merkle_commit(ref_cw, True))
     finally:
         c.close()
+
+
+@pytest.mark.parametrize("leaf_levels", [1, 3])
+@pytest.mark.parametrize("passbits,tile,logn", [(8, 4096, 20), (7, 4096, 17), (6, 4096, 16), (8, 2048, 18),
+                                                (8, 4096, 12)])
+def test_leaf_pass_tree_levels_match_oracle(leaf_levels, passbits, tile, logn):
+    """The last NTT pass writes the leaf digests, and with
+    COSNARK_NTT_LEAF_LEVELS=3 (the default) tree levels 1 and 2 as well where
+    a half-tile holds one two-leaf unit per thread (radix 16 x 16 over 16
+    columns, 16 x 8 over 32, 8 x 8 over 64); 2048-element tiles and a
+    two-pass 2^14-point encode keep the leaves alone.  Every codeword element
+    and every node of the WHIR commit's tree against the oracle."""
+    import os
+    import cosnark
+    os.environ["COSNARK_NTT_LEAF_LEVELS"] = str(leaf_levels)
+    try:
+        c = cosnark.Context(0)
+    finally:
+        del os.environ["COSNARK_NTT_LEAF_LEVELS"]
+    try:
+        print(f"leaf pass levels={leaf_levels} passbits={passbits} tile={tile} logn={logn}", flush=True)
+        c.set_ntt_plan(logn + 2, passbits, tile)
+        a = O.synth(0x1EAF + logn + passbits, leaf_levels, 1 << logn)
+        cw, nodes = c.whir_commit(c.upload(a), 1 << logn, 4)
+        ref_cw = O.rs_encode(a, 4)
+        assert np.array_equal(c.download(cw, ((4 << logn), 2)), ref_cw)
+        cnt = cosnark.merkle_node_count(4 << logn)
+        got = c.download(nodes, (cnt, 32), np.uint8)
+        ref = O.merkle_commit(ref_cw, True)
+        bad = np.nonzero(np.any(got != ref, axis=1))[0]
+        assert bad.size == 0, f"{bad.size} nodes differ, first {bad[:4]}"
+    finally:
+        c.close()
