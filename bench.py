@@ -53,21 +53,50 @@ def parse():
     return ap.parse_args()
 
 
-def leaves_fused(logn, passbits, tile):
-    """Whether the NTT plan's last pass writes the leaf digests (the rule of
-    ntt_launch_batch: at least two passes, whole leaves per tile after the
-    tile narrows to at most one step-2 item per thread)."""
+def leaf_levels(logn, passbits, tile):
+    """Tree levels the NTT plan's last pass writes (the rule of
+    ntt_launch_batch): 0 with fewer than two passes or no whole leaves per
+    tile (after the tile narrows to the step-2 items a thread may hold: 16 /
+    R2 under the split exchange); 3 (leaf digests, levels 1 and 2) where the
+    split form holds one two-leaf unit per thread in each half-tile (R C =
+    4096), rows hold whole four-leaf groups (C >= 16) and rows start at
+    multiples of four leaves; else 1 (leaf digests).  COSNARK_NTT_SPLIT=0
+    and COSNARK_NTT_LEAF_LEVELS=1 change it as they change the library."""
+    split = os.environ.get("COSNARK_NTT_SPLIT", "1") != "0"
+    allow3 = int(os.environ.get("COSNARK_NTT_LEAF_LEVELS", "3")) >= 3
     m = -(-logn // passbits)
     if m < 2:
-        return False
+        return 0
     rb = [logn // m + (1 if t < logn % m else 0) for t in range(m)]
     r = rb[-1]
     r1 = 16 if r >= 7 else 8 if r >= 5 else 4 if r >= 3 else 2
+    r2 = 1 << (r - (r1.bit_length() - 1))
+    items2 = 16 // r2 if (split and r >= 2) else 1
     logc = max(0, tile.bit_length() - 1 - r)
     logc = min(logc, rb[0])
-    while logc > 2 and (r1 << logc) > 256:
+    while logc > 2 and (r1 << logc) > items2 * 256:
         logc -= 1
-    return logc >= 2 and (r1 << logc) <= 256
+    if not (logc >= 2 and (r1 << logc) <= items2 * 256):
+        return 0
+    sp = split and r2 >= 2 and (r2 << logc) <= (16 // r1) * 256 and (r1 << logc) <= (16 // r2) * 256
+    if allow3 and sp and ((r1 * r2) << logc) == 4096 and logc >= 4 and logn - r >= 4:
+        return 3
+    return 1
+
+
+def merkle_launches(leaves, have, cap_log=10):
+    """Launches of the tree builder above `have` levels already written
+    (0: from the elements), as merkle.hip's tree_stages / upper_stages."""
+    if have == 0:
+        cnt = leaves >> min(leaves.bit_length() - 1, cap_log)
+        n = 1
+    else:
+        cnt, n = leaves >> (have - 1), 0
+    while cnt > 1:
+        nxt = cnt // 2
+        cnt = nxt >> min(nxt.bit_length() - 1, cap_log)
+        n += 1
+    return n
 
 
 def cpu_info():
@@ -455,23 +484,22 @@ def main():
     # (inputs read once, outputs written once) over its launches
     cw_n = n * a.rate
     # The plain prover's last NTT pass writes the Merkle leaf digests (one
-    # per 4 codeword elements) when its plan allows (leaves_fused), so the
-    # encode stage writes them and the Merkle stage reads them instead of the
-    # codeword; the tree stage then writes the internal nodes.
-    fused = leaves_fused(log_cw, *ctx.ntt_plan(log_cw))
+    # per 4 codeword elements) when its plan allows, and where its tile
+    # allows tree levels 1 and 2 too (leaf_levels); the Merkle stage then
+    # reads the highest of those levels (or the codeword) and writes the
+    # internal nodes above it.
+    lv = leaf_levels(log_cw, *ctx.ntt_plan(log_cw))
     leaves = cw_n // 4
+    enc_nodes = {0: 0, 1: leaves, 3: leaves + leaves // 2 + leaves // 4}[lv]
+    mk_nodes = 2 * leaves - 1 - enc_nodes
     alg = {
-        "rs_encode": 16 * n + 16 * cw_n + (32 * leaves if fused else 0),
-        "merkle": (32 * leaves if fused else 16 * cw_n + 32 * leaves) + 32 * (leaves - 1),
+        "rs_encode": 16 * n + 16 * cw_n + 32 * enc_nodes,
+        "merkle": (32 * (leaves >> (lv - 1)) if lv else 16 * cw_n) + 32 * mk_nodes,
         "sumcheck": 96 * (n - 1),
     }
     fmul = {"rs_encode": (cw_n // 2) * log_cw + cw_n, "merkle": 0, "sumcheck": 6 * (n - 1)}
     passbits = ctx.ntt_plan(log_cw)[0]
-    cnt, mk_launches = (cw_n // 4) >> 10, 1
-    while cnt > 1:
-        nxt = cnt // 2
-        cnt = nxt >> min(nxt.bit_length() - 1, 10)
-        mk_launches += 1
+    mk_launches = merkle_launches(leaves, lv)
     # sumcheck launches of the plain prover: rounds 0-1 in one launch, then
     # two rounds per launch while the folded vector exceeds the tail size
     # (1024 elements unless COSNARK_SC_TAIL says otherwise), then the tail
@@ -489,9 +517,9 @@ def main():
     names = ["rs_encode", "merkle", "sumcheck"]
     fmul_peak = ctx.fmul_rate()
     hash_peak = ctx.blake3_rate()
-    # the Merkle stage hashes the internal nodes (the leaves come from the
-    # encode's last pass); serial stage time
-    mk_compress = cw_n // 4 - 1
+    # the Merkle stage hashes the nodes the encode's last pass left to it
+    # (all of them where the plan writes no leaves); serial stage time
+    mk_compress = mk_nodes
     mk_rate = mk_compress / (float(sst[1]) / 1e3)
     # Dominant kernel: the NTT pass (the largest share of device time in the
     # timed steps and the longest serial stage).  Its average launch time
