@@ -60,8 +60,11 @@ def leaf_levels(logn, passbits, tile):
     R2 under the split exchange); 3 (leaf digests, levels 1 and 2) where the
     split form holds one two-leaf unit per thread in each half-tile (R C =
     4096), rows hold whole four-leaf groups (C >= 16) and rows start at
-    multiples of four leaves; else 1 (leaf digests).  COSNARK_NTT_SPLIT=0
-    and COSNARK_NTT_LEAF_LEVELS=1 change it as they change the library."""
+    multiples of four leaves; else 1 (leaf digests).  The plain prover asks
+    for at most these three levels (the whole tree from the last pass, which
+    WHIR's commits take, lengthens the pass more than it saves beside the
+    sumcheck).  COSNARK_NTT_SPLIT=0 and COSNARK_NTT_LEAF_LEVELS=1 change it
+    as they change the library."""
     split = os.environ.get("COSNARK_NTT_SPLIT", "1") != "0"
     allow3 = int(os.environ.get("COSNARK_NTT_LEAF_LEVELS", "3")) >= 3
     m = -(-logn // passbits)

@@ -137,7 +137,10 @@ int cosnark_create(int device, cosnark_ctx **out) {
     }
     if (const char *t = getenv("COSNARK_NTT_SPLIT")) ctx->ntt_split = atoi(t) != 0;
     if (const char *t = getenv("COSNARK_NTT_DEFER")) ctx->ntt_defer = atoi(t) != 0;
-    if (const char *t = getenv("COSNARK_NTT_LEAF_LEVELS")) ctx->ntt_leaf_levels = atoi(t) >= 3 ? 3 : 1;
+    if (const char *t = getenv("COSNARK_NTT_LEAF_LEVELS")) {
+        const int v = atoi(t);
+        ctx->ntt_leaf_levels = v > 3 ? 64 : v == 3 ? 3 : 1;
+    }
     if (const char *t = getenv("COSNARK_NTT_FAN_MIN")) {
         const int v = atoi(t);
         if (v >= 1) ctx->ntt_fan_min_blocks = v;
@@ -556,7 +559,9 @@ static int prove_plain_core(cosnark_ctx *ctx, const uint64_t *d_f, const uint64_
     if (s1 != ctx->stream) HIPCK(hipStreamWaitEvent(s1, ctx->ev[7], 0));
     if (times) HIPCK(hipEventRecord(ctx->ev[0], s1));
     int levels_done = 0;
-    if ((rc = ntt_launch_leaves(ctx, (const fe *)d_f, n, cw, ilog2(cw_n), nd, &levels_done, s1, tmp))) return rc;
+    // levels 1 and 2 at most from the encode: the whole tree in its last
+    // pass lengthens it more than it saves beside the sumcheck
+    if ((rc = ntt_launch_leaves(ctx, (const fe *)d_f, n, cw, ilog2(cw_n), nd, &levels_done, s1, tmp, 3))) return rc;
     if (times) HIPCK(hipEventRecord(ctx->ev[1], s1));
     if (s2 != ctx->stream) HIPCK(hipStreamWaitEvent(s2, ctx->ev[7], 0));
     if (times) HIPCK(hipEventRecord(ctx->ev[4], s2));

@@ -59,7 +59,7 @@ struct cosnark_ctx {
     int ntt_fan_min_blocks = 512;     // fan-out first pass only with at least this many workgroups
     int ntt_split = 1;                // NTT passes exchange real and imaginary words in turn where they can
     int ntt_defer = 1;                // NTT plans of 3+ passes leave inter-pass twiddles to the consuming pass
-    int ntt_leaf_levels = 3;          // tree levels a leaf-hashing last NTT pass may write (1 or 3)
+    int ntt_leaf_levels = 64;         // most tree levels a leaf-hashing last NTT pass writes (1, 3, or > 3: all)
     int mk_block_log = 10;            // Merkle schedule: 2^this first-level nodes per workgroup (6..10)
     // per-size plan (0 = the defaults above); set by the autotuner
     struct { int passbits, tile; } ntt_plan[NTT_LOG_MAX + 1] = {};
@@ -114,16 +114,17 @@ int ntt_launch(cosnark_ctx *ctx, const gf::fe *in, uint64_t in_len, gf::fe *out,
 // leaves (optional): the last pass also writes item 0's Merkle leaf digests
 // (BLAKE3 of each 4 consecutive outputs, 2^logn / 4 digests: level 0 of a
 // node array laid out level after level) when the plan allows it;
-// *levels_done receives the number of tree levels written (0, 1 or 3; levels
-// 1 and 2 of a tree that hashes left children twice).
+// *levels_done receives the number of tree levels written: 0, 1, 3 or all
+// logn - 1 of them (levels above the leaves only for a tree that hashes left
+// children twice), at most max_levels (and ctx->ntt_leaf_levels).
 int ntt_launch_batch(cosnark_ctx *ctx, const gf::fe *in, const gf::fe *const *in_tab, uint64_t in_len,
                      gf::fe *out, gf::fe *const *out_tab, int batch, int logn, bool inverse, hipStream_t s,
-                     gf::fe *tmp, uint8_t *leaves = nullptr, int *levels_done = nullptr);
+                     gf::fe *tmp, uint8_t *leaves = nullptr, int *levels_done = nullptr, int max_levels = 64);
 // RS encode (forward NTT of in || 0) of 2^logn points with the codeword's
 // Merkle leaf digests written by the last pass where the plan allows it
 // (*levels_done, as above); the caller builds the rest of the tree.
 int ntt_launch_leaves(cosnark_ctx *ctx, const gf::fe *in, uint64_t in_len, gf::fe *out, int logn, uint8_t *nodes,
-                      int *levels_done, hipStream_t s, gf::fe *tmp = nullptr);
+                      int *levels_done, hipStream_t s, gf::fe *tmp = nullptr, int max_levels = 64);
 int ntt_plan_bits(const cosnark_ctx *ctx, int logn);
 int ntt_plan_tile(const cosnark_ctx *ctx, int logn);
 // Merkle commit of n_elems elements into nodes (all levels).

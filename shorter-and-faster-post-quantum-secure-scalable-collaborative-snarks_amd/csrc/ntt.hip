@@ -56,7 +56,8 @@ struct NttPass {
     int rbits[8];       // radix bits per pass, first pass first
     int pre[8];         // pre[t] = rbits[0] + .. + rbits[t-1]
     uint8_t *leaves;    // last pass, batch item 0: Merkle leaf digests of the output (or null)
-    int leaf_lv;        // tree levels the last pass writes at `leaves` (1, or 3: levels 1 and 2 too)
+    int leaf_lv;        // tree levels the last pass writes at `leaves`: 1, 3 (levels 1 and 2 too), or
+                        // more than 3: every level up to the root
 };
 
 // loads and stores through global-address-space pointers (global_load /
@@ -141,7 +142,14 @@ __device__ __forceinline__ void ntt_pass_body(
     // workgroups, which run side by side.
     const bool fan_blk = !LAST && !INV && p.fan_blk;
     const uint32_t k0_blk = fan_blk ? (uint32_t)(blockIdx.x & ((1u << p.fan) - 1)) : 0u;
-    const uint64_t bid = fan_blk ? (uint64_t)(blockIdx.x >> p.fan) : (uint64_t)blockIdx.x;
+    uint64_t bid = fan_blk ? (uint64_t)(blockIdx.x >> p.fan) : (uint64_t)blockIdx.x;
+    // A last pass that climbs the whole tree takes its tiles in bit-reversed
+    // order: the tiles whose level-2 nodes carry the longest chains (column
+    // bases with the most trailing zeros) then start first instead of last.
+    if (LAST && p.leaf_lv > 3) {
+        const int tl = p.logN - LOGR - logC;
+        bid = tl > 0 ? (uint64_t)(__brev((uint32_t)bid) >> (32 - tl)) : 0;
+    }
     // block -> positions
     uint64_t hi = 0, lo0 = 0;              // non-last: pos = hi*(R*S) + d*S + lo0 + c
     uint64_t rest = 0, c0 = 0, W0 = 0;      // last: row(c) = rest + (c0 + c)*W0, pos = row*R + d
@@ -429,7 +437,7 @@ __device__ __forceinline__ void ntt_pass_body(
         // SPLIT halves the buffer, so it restages and hashes rows k < R/2
         // (k2 < R2/2) first and the rest second.
         //
-        // With p.leaf_lv == 3 the pass also writes tree levels 1 and 2.  The
+        // With p.leaf_lv >= 3 the pass also writes tree levels 1 and 2.  The
         // tree hashes a node's left child twice (src/merkle_tree.cpp:287-292),
         // so level-1 node j is BLAKE3(leaf 2j || leaf 2j) and level-2 node i
         // depends on leaf 4i alone.  The host sets it only where a half holds
@@ -441,7 +449,7 @@ __device__ __forceinline__ void ntt_pass_body(
         // second half's from their even neighbour by DPP.
         constexpr int NH = (SPLIT && R2 >= 2) ? 2 : 1, RH = R / NH;
         const int qpr = C / 4, nleaf = (RH * C) / 4;
-        const bool lv3 = NH == 2 && p.leaf_lv == 3;
+        const bool lv3 = NH == 2 && p.leaf_lv >= 3;
         const uint64_t n_lf = 1ull << (p.logN - 2);    // leaves: level 1 starts there, level 2 at 3/2 of it
         uint32_t carry[8];                              // first half: the unit's level-1 digest
         static_for<0, NH>([&](auto hh) {
@@ -510,6 +518,26 @@ __device__ __forceinline__ void ntt_pass_body(
                     uint4 *d2 = reinterpret_cast<uint4 *>(p.leaves + 32 * (n_lf + (n_lf >> 1) + i2));
                     d2[0] = make_uint4(l2[0], l2[1], l2[2], l2[3]);
                     d2[1] = make_uint4(l2[4], l2[5], l2[6], l2[7]);
+                    if (p.leaf_lv > 3) {
+                        // the rest of the tree: node j of level 2 + s is the
+                        // chain from level-2 node j 2^s, so the lane holding
+                        // level-2 node i2 climbs while i2 has s trailing
+                        // zeros, up to the root (lanes leave the loop at
+                        // their own level; nothing below waits on them)
+                        uint64_t off = n_lf + (n_lf >> 1), cnt = n_lf >> 2, j = i2;
+                        while (cnt > 1 && (j & 1) == 0) {
+                            off += cnt;
+                            cnt >>= 1;
+                            j >>= 1;
+                            uint32_t nx[8];
+                            self_pair(l2, nx);
+#pragma unroll
+                            for (int w = 0; w < 8; ++w) l2[w] = nx[w];
+                            uint4 *dn = reinterpret_cast<uint4 *>(p.leaves + 32 * (off + j));
+                            dn[0] = make_uint4(l2[0], l2[1], l2[2], l2[3]);
+                            dn[1] = make_uint4(l2[4], l2[5], l2[6], l2[7]);
+                        }
+                    }
                 }
             } else {
                 for (int l = tid; l < nleaf; l += NTT_THREADS) {
@@ -656,8 +684,9 @@ int ntt_plan_tile(const cosnark_ctx *ctx, int logn) {
 
 int ntt_launch_batch(cosnark_ctx *ctx, const fe *in, const fe *const *in_tab, uint64_t in_len, fe *out,
                      fe *const *out_tab, int batch, int logn, bool inverse, hipStream_t stream, fe *tmp_own,
-                     uint8_t *leaves, int *levels_done) {
+                     uint8_t *leaves, int *levels_done, int max_levels) {
     if (levels_done) *levels_done = 0;
+    const int lv_cap = max_levels < ctx->ntt_leaf_levels ? max_levels : ctx->ntt_leaf_levels;
     if (logn < 1 || logn > NTT_LOG_MAX) return set_err(ctx, COSNARK_E_RANGE, "ntt: logn out of range");
     if (batch < 1 || batch > 65535) return set_err(ctx, COSNARK_E_RANGE, "ntt: batch out of range");
     // radix plan: fewest passes of at most `passbits` bits, spread evenly, larger first
@@ -772,9 +801,9 @@ int ntt_launch_batch(cosnark_ctx *ctx, const fe *in, const fe *const *in_tab, ui
                 // four-leaf groups per row (C >= 16) and rows at multiples of
                 // four leaves (logn - r >= 4)
                 const int r2 = 1 << (r - ilog2((uint64_t)r1));
-                if (ctx->ntt_leaf_levels >= 3 && pass_split(ctx, r1, r2, q) && ((r1 * r2) << logC) == 16 * NTT_THREADS &&
+                if (lv_cap >= 3 && pass_split(ctx, r1, r2, q) && ((r1 * r2) << logC) == 16 * NTT_THREADS &&
                     logC >= 4 && logn - r >= 4)
-                    q.leaf_lv = 3;
+                    q.leaf_lv = lv_cap > 3 ? logn - 1 : 3;
                 if (levels_done) *levels_done = q.leaf_lv;
             }
         } else {
@@ -790,9 +819,9 @@ int ntt_launch_batch(cosnark_ctx *ctx, const fe *in, const fe *const *in_tab, ui
 }
 
 int ntt_launch_leaves(cosnark_ctx *ctx, const fe *in, uint64_t in_len, fe *out, int logn, uint8_t *nodes,
-                      int *levels_done, hipStream_t stream, fe *tmp) {
+                      int *levels_done, hipStream_t stream, fe *tmp, int max_levels) {
     return ntt_launch_batch(ctx, in, nullptr, in_len, out, nullptr, 1, logn, false, stream, tmp, nodes,
-                            levels_done);
+                            levels_done, max_levels);
 }
 
 int ntt_launch(cosnark_ctx *ctx, const fe *in, uint64_t in_len, fe *out, int logn, bool inverse,

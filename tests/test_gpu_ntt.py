@@ -175,15 +175,18 @@ def test_rs_encode_fan_out_deferred_matches_oracle(split, logn, rate, passbits, 
         c.close()
 
 
-@pytest.mark.parametrize("leaf_levels", [1, 3])
+@pytest.mark.parametrize("leaf_levels", [1, 3, 64])
 @pytest.mark.parametrize("passbits,tile,logn", [(8, 4096, 20), (7, 4096, 17), (6, 4096, 16), (8, 2048, 18),
-                                                (8, 4096, 12)])
+                                                (8, 4096, 12), (8, 4096, 3)])
 def test_leaf_pass_tree_levels_match_oracle(leaf_levels, passbits, tile, logn):
-    """The last NTT pass writes the leaf digests, and with
-    COSNARK_NTT_LEAF_LEVELS=3 (the default) tree levels 1 and 2 as well where
-    a half-tile holds one two-leaf unit per thread (radix 16 x 16 over 16
-    columns, 16 x 8 over 32, 8 x 8 over 64); 2048-element tiles and a
-    two-pass 2^14-point encode keep the leaves alone.  Every codeword element
+    """The last NTT pass writes the leaf digests; with
+    COSNARK_NTT_LEAF_LEVELS=3 tree levels 1 and 2 as well where a half-tile
+    holds one two-leaf unit per thread (radix 16 x 16 over 16 columns, 16 x 8
+    over 32, 8 x 8 over 64), and with more (the default) every level up to
+    the root, each lane climbing from its level-2 node while the node index
+    has trailing zeros (16 x 16 columns: one level-2 node per row; 32 and 64
+    columns: two and four).  2048-element tiles keep the leaves alone; a
+    2^5-point encode is one pass and hashes nothing.  Every codeword element
     and every node of the WHIR commit's tree against the oracle."""
     import os
     import cosnark
