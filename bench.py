@@ -589,9 +589,10 @@ def main():
                      "hash": {"merkle_stage_compress_per_s": round(mk_rate, 1),
                               "peak_compress_per_s": round(hash_peak, 1),
                               "frac": round(mk_rate / hash_peak, 4),
-                              "note": "Merkle stage (serial): 2^k - 1 internal-node compressions over its time; "
-                                      "peak = cosnark_blake3_rate, the same compression in independent chains, "
-                                      "measured in this run"}},
+                              "note": "Merkle stage (serial): the compressions of the tree levels the encode's "
+                                      "last pass left to it (levels 3 and up for the bench's plan) over its time, "
+                                      "whose upper levels are latency chains; peak = cosnark_blake3_rate, the same "
+                                      "compression in independent chains, measured in this run"}},
     }
     df.free(); dg.free()
     if not a.no_whir:
