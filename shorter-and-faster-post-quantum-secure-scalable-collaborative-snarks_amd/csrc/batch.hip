@@ -124,14 +124,28 @@ int batch_core_build(cosnark_ctx *ctx, int count, const BatchProof *pr, int rate
     // RS encodes: one group per codeword size, launched together
     std::map<int, std::vector<int>> by_log;
     for (int i = 0; i < count; ++i) by_log[ilog2(cwn[i])].push_back(i);
+    // Where a group's plan allows it, the last NTT pass of each encode also
+    // writes its tree's leaf digests and levels 1 and 2 (the rule of
+    // ntt_launch_batch, asked in a dry run), and that tree starts above them.
+    std::vector<int> have(count, 0);
     for (auto &kv : by_log) {
-        BatchCore::Group G{kv.first, (int)kv.second.size(), C.n[kv.second[0]], 0, 0};
+        BatchCore::Group G{kv.first, (int)kv.second.size(), C.n[kv.second[0]], 0, 0, 0, 0};
         G.in_tab = C.tb.reserve(sizeof(fe *) * G.B);
         G.out_tab = C.tb.reserve(sizeof(fe *) * G.B);
+        G.lv_tab = C.tb.reserve(sizeof(uint8_t *) * G.B);
         for (int j = 0; j < G.B; ++j) {
             C.tb.at<const fe *>(G.in_tab)[j] = fin[kv.second[j]];
             C.tb.at<fe *>(G.out_tab)[j] = cws[kv.second[j]];
+            C.tb.at<uint8_t *>(G.lv_tab)[j] = nds[kv.second[j]];
         }
+        if (ctx->batch_leaf_levels > 0) {
+            uint8_t *const probe[1] = {nds[kv.second[0]]};
+            int rc = ntt_launch_batch(ctx, nullptr, (const fe *const *)fin.data(), G.in_len, nullptr, cws.data(), G.B,
+                                      G.logn, false, nullptr, nullptr, nullptr, &G.levels, ctx->batch_leaf_levels,
+                                      probe, true);
+            if (rc) return rc;
+        }
+        for (int i : kv.second) have[i] = G.levels;
         C.groups.push_back(G);
     }
     // Two commit lanes (two streams): the groups, largest first, each to the
@@ -163,13 +177,16 @@ int batch_core_build(cosnark_ctx *ctx, int count, const BatchProof *pr, int rate
         std::vector<const fe *> e;
         std::vector<uint64_t> ne;
         std::vector<uint8_t *> nd;
+        std::vector<int> hv;
         for (int i = 0; i < count; ++i)
             if (lane_of[i] == l) {
                 e.push_back(cws[i]);
                 ne.push_back(cwn[i]);
                 nd.push_back(nds[i]);
+                hv.push_back(have[i]);
             }
-        rc = merkle_batch_plan(ctx, (int)e.size(), e.data(), ne.data(), nd.data(), true, C.tb, l ? C.mk1 : C.mk);
+        rc = merkle_batch_plan(ctx, (int)e.size(), e.data(), ne.data(), nd.data(), true, C.tb, l ? C.mk1 : C.mk,
+                               hv.data());
         if (rc) return rc;
     }
     rc = sumcheck_batch_plan(ctx, count, fin.data(), gin.data(), C.n.data(), bufs.data(), parts.data(), sts.data(),
@@ -211,9 +228,15 @@ int batch_enqueue_commits(cosnark_ctx *ctx, const BatchCore &C, const uint8_t *t
         if (chunk == 0) return set_err(ctx, COSNARK_E_RANGE, "batch: NTT temporary smaller than one codeword");
         for (int c0 = 0; c0 < G.B; c0 += (int)chunk) {
             const int nb = (int)std::min<uint64_t>(chunk, (uint64_t)(G.B - c0));
+            int lv = 0;
             rc = ntt_launch_batch(ctx, nullptr, (const fe *const *)(tables + G.in_tab) + c0, G.in_len, nullptr,
-                                  (fe *const *)(tables + G.out_tab) + c0, nb, G.logn, false, s1, tmp);
+                                  (fe *const *)(tables + G.out_tab) + c0, nb, G.logn, false, s1, tmp, nullptr, &lv,
+                                  G.levels, G.levels ? (uint8_t *const *)(tables + G.lv_tab) + c0 : nullptr);
             if (rc) return rc;
+            // the trees were planned from the levels the encode writes
+            if (lv != G.levels)
+                return set_err(ctx, COSNARK_E_ARG, "batch: the NTT plan of 2^" + std::to_string(G.logn) +
+                                                         " points changed after the batch was planned");
         }
     }
     return merkle_batch_run(ctx, lane ? C.mk1 : C.mk, tables, s1);

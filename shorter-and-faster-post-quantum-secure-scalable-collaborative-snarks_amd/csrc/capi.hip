@@ -137,6 +137,10 @@ int cosnark_create(int device, cosnark_ctx **out) {
     }
     if (const char *t = getenv("COSNARK_NTT_SPLIT")) ctx->ntt_split = atoi(t) != 0;
     if (const char *t = getenv("COSNARK_NTT_DEFER")) ctx->ntt_defer = atoi(t) != 0;
+    if (const char *t = getenv("COSNARK_BATCH_LEAF_LEVELS")) {
+        const int v = atoi(t);
+        ctx->batch_leaf_levels = v > 3 ? 64 : v == 3 ? 3 : v >= 1 ? 1 : 0;
+    }
     if (const char *t = getenv("COSNARK_NTT_LEAF_LEVELS")) {
         const int v = atoi(t);
         ctx->ntt_leaf_levels = v > 3 ? 64 : v == 3 ? 3 : 1;

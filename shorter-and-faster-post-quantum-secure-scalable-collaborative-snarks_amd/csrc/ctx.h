@@ -60,6 +60,7 @@ struct cosnark_ctx {
     int ntt_split = 1;                // NTT passes exchange real and imaginary words in turn where they can
     int ntt_defer = 1;                // NTT plans of 3+ passes leave inter-pass twiddles to the consuming pass
     int ntt_leaf_levels = 64;         // most tree levels a leaf-hashing last NTT pass writes (1, 3, or > 3: all)
+    int batch_leaf_levels = 3;        // the same for the batched prover's encodes (0: none)
     int mk_block_log = 10;            // Merkle schedule: 2^this first-level nodes per workgroup (6..10)
     // per-size plan (0 = the defaults above); set by the autotuner
     struct { int passbits, tile; } ntt_plan[NTT_LOG_MAX + 1] = {};
@@ -117,9 +118,12 @@ int ntt_launch(cosnark_ctx *ctx, const gf::fe *in, uint64_t in_len, gf::fe *out,
 // *levels_done receives the number of tree levels written: 0, 1, 3 or all
 // logn - 1 of them (levels above the leaves only for a tree that hashes left
 // children twice), at most max_levels (and ctx->ntt_leaf_levels).
+// leaves_tab (a device array of batch node arrays) gives every item its
+// tree in place of `leaves`.  dry: only *levels_done, nothing launched.
 int ntt_launch_batch(cosnark_ctx *ctx, const gf::fe *in, const gf::fe *const *in_tab, uint64_t in_len,
                      gf::fe *out, gf::fe *const *out_tab, int batch, int logn, bool inverse, hipStream_t s,
-                     gf::fe *tmp, uint8_t *leaves = nullptr, int *levels_done = nullptr, int max_levels = 64);
+                     gf::fe *tmp, uint8_t *leaves = nullptr, int *levels_done = nullptr, int max_levels = 64,
+                     uint8_t *const *leaves_tab = nullptr, bool dry = false);
 // RS encode (forward NTT of in || 0) of 2^logn points with the codeword's
 // Merkle leaf digests written by the last pass where the plan allows it
 // (*levels_done, as above); the caller builds the rest of the tree.
@@ -161,8 +165,10 @@ struct MerkleBatchPlan {
     };
     std::vector<Launch> launches;
 };
+// have (optional): per tree, the levels already in nodes (0: from the elements)
 int merkle_batch_plan(cosnark_ctx *ctx, int count, const gf::fe *const *elems, const uint64_t *n_elems,
-                      uint8_t *const *nodes, bool dup_left, TableBuf &tb, MerkleBatchPlan &plan);
+                      uint8_t *const *nodes, bool dup_left, TableBuf &tb, MerkleBatchPlan &plan,
+                      const int *have = nullptr);
 int merkle_batch_run(cosnark_ctx *ctx, const MerkleBatchPlan &plan, const uint8_t *tables, hipStream_t s);
 int sumcheck_init(cosnark_ctx *ctx);
 int copcs_init(cosnark_ctx *ctx);
@@ -238,6 +244,8 @@ struct BatchCore {
         int logn, B;
         uint64_t in_len;
         size_t in_tab, out_tab;      // table offsets: input and codeword pointers
+        size_t lv_tab;               // table offset: node array pointers (leaf digests from the last pass)
+        int levels;                  // tree levels the group's last NTT pass writes
     };
     std::vector<Group> groups;
     std::vector<int> glane;          // per group: commit lane 0 or 1

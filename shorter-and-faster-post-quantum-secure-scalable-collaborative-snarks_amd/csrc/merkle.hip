@@ -340,34 +340,45 @@ int merkle_columns_launch(cosnark_ctx *ctx, const fe *cw, int nrows, uint64_t Lc
 // ---- batches of trees -----------------------------------------------------
 
 int merkle_batch_plan(cosnark_ctx *ctx, int count, const fe *const *elems, const uint64_t *n_elems,
-                      uint8_t *const *nodes, bool dup_left, TableBuf &tb, MerkleBatchPlan &plan) {
+                      uint8_t *const *nodes, bool dup_left, TableBuf &tb, MerkleBatchPlan &plan, const int *have) {
     std::vector<std::vector<std::pair<MkJob, uint64_t>>> per(count);
+    std::vector<int> mode0(count);
     size_t nst = 0;
     for (int i = 0; i < count; ++i) {
         if (!is_pow2(n_elems[i]) || n_elems[i] < 8 || n_elems[i] > (1ull << 30))
             return set_err(ctx, COSNARK_E_ARG, "merkle batch: n_elems must be powers of two in [8, 2^30]");
-        per[i] = tree_stages(MODE_ELEMS, elems[i], n_elems[i] / 4, nodes[i], dup_left, 0, 0, block_cap(ctx));
+        const uint64_t leaves = n_elems[i] / 4;
+        const int h = have ? have[i] : 0;
+        if (h < 0 || (h > 0 && (leaves >> (h - 1)) < 1))
+            return set_err(ctx, COSNARK_E_ARG, "merkle batch: levels already built out of range");
+        // a tree whose lowest levels are in place starts above them
+        per[i] = h ? upper_stages(leaves, nodes[i], dup_left, h, block_cap(ctx))
+                   : tree_stages(MODE_ELEMS, elems[i], leaves, nodes[i], dup_left, 0, 0, block_cap(ctx));
+        mode0[i] = h ? MODE_NODES : MODE_ELEMS;
         nst = std::max(nst, per[i].size());
     }
     plan.launches.clear();
+    // stage s of every tree: one launch per kernel mode present
     for (size_t s = 0; s < nst; ++s) {
-        std::vector<MkJob> jobs;
-        std::vector<uint16_t> map;
-        for (int i = 0; i < count; ++i) {
-            if (s >= per[i].size()) continue;
-            MkJob J = per[i][s].first;
-            J.first_block = (uint32_t)map.size();
-            for (uint64_t b = 0; b < per[i][s].second; ++b) map.push_back((uint16_t)jobs.size());
-            jobs.push_back(J);
+        for (int mode : {MODE_ELEMS, MODE_NODES}) {
+            std::vector<MkJob> jobs;
+            std::vector<uint16_t> map;
+            for (int i = 0; i < count; ++i) {
+                if (s >= per[i].size() || (s == 0 ? mode0[i] : MODE_NODES) != mode) continue;
+                MkJob J = per[i][s].first;
+                J.first_block = (uint32_t)map.size();
+                for (uint64_t b = 0; b < per[i][s].second; ++b) map.push_back((uint16_t)jobs.size());
+                jobs.push_back(J);
+            }
+            if (jobs.empty()) continue;
+            if (jobs.size() > 0xFFFF || map.size() > 0x7FFFFFFFu)
+                return set_err(ctx, COSNARK_E_RANGE, "merkle batch: too many trees or blocks in one launch");
+            MerkleBatchPlan::Launch L{mode, (uint32_t)map.size(), tb.reserve(sizeof(MkJob) * jobs.size()), 0};
+            L.blk_job = tb.reserve(sizeof(uint16_t) * map.size());
+            memcpy(tb.at<MkJob>(L.jobs), jobs.data(), sizeof(MkJob) * jobs.size());
+            memcpy(tb.at<uint16_t>(L.blk_job), map.data(), sizeof(uint16_t) * map.size());
+            plan.launches.push_back(L);
         }
-        if (jobs.size() > 0xFFFF || map.size() > 0x7FFFFFFFu)
-            return set_err(ctx, COSNARK_E_RANGE, "merkle batch: too many trees or blocks in one launch");
-        MerkleBatchPlan::Launch L{s == 0 ? MODE_ELEMS : MODE_NODES, (uint32_t)map.size(),
-                                  tb.reserve(sizeof(MkJob) * jobs.size()), 0};
-        L.blk_job = tb.reserve(sizeof(uint16_t) * map.size());
-        memcpy(tb.at<MkJob>(L.jobs), jobs.data(), sizeof(MkJob) * jobs.size());
-        memcpy(tb.at<uint16_t>(L.blk_job), map.data(), sizeof(uint16_t) * map.size());
-        plan.launches.push_back(L);
     }
     return COSNARK_OK;
 }

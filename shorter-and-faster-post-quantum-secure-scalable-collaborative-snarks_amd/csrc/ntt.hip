@@ -56,6 +56,7 @@ struct NttPass {
     int rbits[8];       // radix bits per pass, first pass first
     int pre[8];         // pre[t] = rbits[0] + .. + rbits[t-1]
     uint8_t *leaves;    // last pass, batch item 0: Merkle leaf digests of the output (or null)
+    uint8_t *const *leaves_tab;  // last pass: item y's node array leaves_tab[y] (null: `leaves` for item 0)
     int leaf_lv;        // tree levels the last pass writes at `leaves`: 1, 3 (levels 1 and 2 too), or
                         // more than 3: every level up to the root
 };
@@ -236,7 +237,8 @@ __device__ __forceinline__ void ntt_pass_body(
     // tile keeps every thread busy in both steps.
     constexpr int MI1 = (SPLIT && R1 < 16) ? 16 / R1 : 1;
     constexpr int MI2 = (SPLIT && R2 < 16) ? 16 / R2 : 1;
-    const bool leaves_on = LAST && p.leaves && by == 0 && items2 <= MI2 * NTT_THREADS;
+    uint8_t *const lvp = !LAST ? nullptr : p.leaves_tab ? p.leaves_tab[by] : by == 0 ? p.leaves : nullptr;
+    const bool leaves_on = LAST && lvp && items2 <= MI2 * NTT_THREADS;
     fe vv[MI2][R2];
     int my_c[MI2] = {}, my_k1[MI2] = {};
     // zero padding (only the first pass reads fewer than 2^logN inputs)
@@ -491,13 +493,13 @@ __device__ __forceinline__ void ntt_pass_body(
                 leaf_msg(src + 4, m);
                 b3::compress64(m, h1);
                 const uint64_t li = (nat_rest + c0 + 8 * up + ((uint64_t)(k + h * RH) << kshift)) >> 2;
-                uint4 *dst = reinterpret_cast<uint4 *>(p.leaves + 32 * li);
+                uint4 *dst = reinterpret_cast<uint4 *>(lvp + 32 * li);
                 dst[0] = make_uint4(h0[0], h0[1], h0[2], h0[3]);
                 dst[1] = make_uint4(h0[4], h0[5], h0[6], h0[7]);
                 dst[2] = make_uint4(h1[0], h1[1], h1[2], h1[3]);
                 dst[3] = make_uint4(h1[4], h1[5], h1[6], h1[7]);
                 self_pair(h0, l1);
-                uint4 *d1 = reinterpret_cast<uint4 *>(p.leaves + 32 * (n_lf + (li >> 1)));
+                uint4 *d1 = reinterpret_cast<uint4 *>(lvp + 32 * (n_lf + (li >> 1)));
                 d1[0] = make_uint4(l1[0], l1[1], l1[2], l1[3]);
                 d1[1] = make_uint4(l1[4], l1[5], l1[6], l1[7]);
                 if (h == 0) {
@@ -515,7 +517,7 @@ __device__ __forceinline__ void ntt_pass_body(
                     self_pair(in, l2);
                     const int row = odd ? k + RH : k;
                     const uint64_t i2 = (nat_rest + c0 + 8 * (up & ~1) + ((uint64_t)row << kshift)) >> 4;
-                    uint4 *d2 = reinterpret_cast<uint4 *>(p.leaves + 32 * (n_lf + (n_lf >> 1) + i2));
+                    uint4 *d2 = reinterpret_cast<uint4 *>(lvp + 32 * (n_lf + (n_lf >> 1) + i2));
                     d2[0] = make_uint4(l2[0], l2[1], l2[2], l2[3]);
                     d2[1] = make_uint4(l2[4], l2[5], l2[6], l2[7]);
                     if (p.leaf_lv > 3) {
@@ -533,7 +535,7 @@ __device__ __forceinline__ void ntt_pass_body(
                             self_pair(l2, nx);
 #pragma unroll
                             for (int w = 0; w < 8; ++w) l2[w] = nx[w];
-                            uint4 *dn = reinterpret_cast<uint4 *>(p.leaves + 32 * (off + j));
+                            uint4 *dn = reinterpret_cast<uint4 *>(lvp + 32 * (off + j));
                             dn[0] = make_uint4(l2[0], l2[1], l2[2], l2[3]);
                             dn[1] = make_uint4(l2[4], l2[5], l2[6], l2[7]);
                         }
@@ -546,7 +548,7 @@ __device__ __forceinline__ void ntt_pass_body(
                     leaf_msg(Y + k * CP + 4 * q, m);
                     b3::compress64(m, hv);
                     const uint64_t pos = nat_rest + c0 + 4 * q + ((uint64_t)(k + h * RH) << kshift);
-                    uint4 *dst = reinterpret_cast<uint4 *>(p.leaves + 32 * (pos >> 2));
+                    uint4 *dst = reinterpret_cast<uint4 *>(lvp + 32 * (pos >> 2));
                     dst[0] = make_uint4(hv[0], hv[1], hv[2], hv[3]);
                     dst[1] = make_uint4(hv[4], hv[5], hv[6], hv[7]);
                 }
@@ -684,8 +686,9 @@ int ntt_plan_tile(const cosnark_ctx *ctx, int logn) {
 
 int ntt_launch_batch(cosnark_ctx *ctx, const fe *in, const fe *const *in_tab, uint64_t in_len, fe *out,
                      fe *const *out_tab, int batch, int logn, bool inverse, hipStream_t stream, fe *tmp_own,
-                     uint8_t *leaves, int *levels_done, int max_levels) {
+                     uint8_t *leaves, int *levels_done, int max_levels, uint8_t *const *leaves_tab, bool dry) {
     if (levels_done) *levels_done = 0;
+    const bool want_leaves = leaves || leaves_tab;
     const int lv_cap = max_levels < ctx->ntt_leaf_levels ? max_levels : ctx->ntt_leaf_levels;
     if (logn < 1 || logn > NTT_LOG_MAX) return set_err(ctx, COSNARK_E_RANGE, "ntt: logn out of range");
     if (batch < 1 || batch > 65535) return set_err(ctx, COSNARK_E_RANGE, "ntt: batch out of range");
@@ -732,7 +735,7 @@ int ntt_launch_batch(cosnark_ctx *ctx, const fe *in, const fe *const *in_tab, ui
     // slice per batch item) and the last one writes `out`.  (A single pass
     // is one workgroup per item: reads finish before its writes start.)
     fe *tmp = tmp_own;
-    if (m >= 2 && !tmp) {
+    if (m >= 2 && !tmp && !dry) {
         int rc = ntt_tmp_reserve(ctx, (sizeof(fe) << logn) * (size_t)batch, (void **)&tmp);
         if (rc) return rc;
     }
@@ -758,7 +761,7 @@ int ntt_launch_batch(cosnark_ctx *ctx, const fe *in, const fe *const *in_tab, ui
             // with leaf digests wanted, narrow the tile until each thread
             // holds at most one step-2 item (R1 * C <= NTT_THREADS; two in
             // the split form where R1 = 2 R2)
-            if (leaves && m >= 2)
+            if (want_leaves && m >= 2)
                 while (logC > 2 && (r1 << logC) > items2_per_thread(ctx, rd) * NTT_THREADS) --logC;
         }
         q.logC = logC;
@@ -793,8 +796,9 @@ int ntt_launch_batch(cosnark_ctx *ctx, const fe *in, const fe *const *in_tab, ui
             // fused leaf digests: whole leaves per tile (C >= 4) and at most
             // one step-2 item per thread (two in the split form where R1 =
             // 2 R2; the pass then takes that form, see launch_pass)
-            if (leaves && m >= 2 && logC >= 2 && (r1 << logC) <= items2_per_thread(ctx, rd) * NTT_THREADS) {
+            if (want_leaves && m >= 2 && logC >= 2 && (r1 << logC) <= items2_per_thread(ctx, rd) * NTT_THREADS) {
                 q.leaves = leaves;
+                q.leaves_tab = leaves_tab;
                 q.leaf_lv = 1;
                 // levels 1 and 2 as well (dup-left trees): the split form with
                 // one two-leaf unit per thread in each half (R C = 4096), whole
@@ -810,6 +814,10 @@ int ntt_launch_batch(cosnark_ctx *ctx, const fe *in, const fe *const *in_tab, ui
             dst = tmp;
             q.out_tab = nullptr;
             q.out_stride = N;
+        }
+        if (dry) {
+            done_bits += r;
+            continue;
         }
         int rc = dispatch_pass(ctx, stream, rd, blocks, batch, src, dst, q, last, inverse, twlo, twhi, rtab, scale);
         if (rc) return rc;
