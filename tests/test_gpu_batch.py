@@ -147,3 +147,37 @@ def test_batch_argument_errors(ctx):
     for sizes, rate in (([3], 4), ([1 << 27], 4), ([4], 1), ([16], 3)):
         arr = np.ascontiguousarray(sizes, dtype=np.uint64)
         assert lib.cosnark_batch_create(ctx.h, len(sizes), cosnark._p(arr), rate, ctypes.byref(h)) == cosnark.E_ARG
+
+
+@pytest.mark.parametrize("levels", [0, 1, 64])
+def test_batch_tree_levels_from_the_encode(levels):
+    """Each group's last NTT pass writes its proofs' leaf digests and tree
+    levels 1 and 2 by default (COSNARK_BATCH_LEAF_LEVELS=3, covered by the
+    64-proof commitment test); here the other settings -- no levels (trees
+    from the codewords), the leaf digests alone, the whole tree -- on a mixed
+    batch: every codeword and tree node against the oracle, and every proof
+    equal to the plain prover's."""
+    import os
+    os.environ["COSNARK_BATCH_LEAF_LEVELS"] = str(levels)
+    try:
+        c = cosnark.Context(0)
+    finally:
+        del os.environ["COSNARK_BATCH_LEAF_LEVELS"]
+    try:
+        sizes = [1 << 12, 1 << 14, 1 << 14, 1 << 16, 1 << 11, 1 << 15]
+        print(f"batch tree levels={levels} sizes={sizes}", flush=True)
+        b = c.batch(sizes, 4)
+        ins = _fill(b, 0xB1E0 + levels)
+        proofs, _, _ = b.prove()
+        for i, n in enumerate(sizes):
+            f, g = ins[i]
+            ref_cw = O.rs_encode(f, 4)
+            ref_nodes = O.merkle_commit(ref_cw, True)
+            cw, nodes = b.commitment(i)
+            assert np.array_equal(cw, ref_cw), f"proof {i} codeword"
+            bad = np.nonzero(np.any(nodes != ref_nodes, axis=1))[0]
+            assert bad.size == 0, f"proof {i} (n={n}): {bad.size} nodes differ, first {bad[:4]}"
+            assert proofs[i] == _plain(c, f, g, 4)[0]
+        b.close()
+    finally:
+        c.close()
