@@ -137,6 +137,14 @@ int cosnark_create(int device, cosnark_ctx **out) {
     }
     if (const char *t = getenv("COSNARK_NTT_SPLIT")) ctx->ntt_split = atoi(t) != 0;
     if (const char *t = getenv("COSNARK_NTT_DEFER")) ctx->ntt_defer = atoi(t) != 0;
+    if (const char *t = getenv("COSNARK_MK_BLOCK_LOG")) {
+        const int v = atoi(t);
+        if (v >= 6 && v <= 10) ctx->mk_block_log = v;
+    }
+    if (const char *t = getenv("COSNARK_MK_REG_LOG")) {
+        const int v = atoi(t);
+        if (v >= 0 && v <= 2) ctx->mk_reg_log = v;
+    }
     if (const char *t = getenv("COSNARK_BATCH_LEAF_LEVELS")) {
         const int v = atoi(t);
         ctx->batch_leaf_levels = v > 3 ? 64 : v == 3 ? 3 : v >= 1 ? 1 : 0;

@@ -62,6 +62,7 @@ struct cosnark_ctx {
     int ntt_leaf_levels = 64;         // most tree levels a leaf-hashing last NTT pass writes (1, 3, or > 3: all)
     int batch_leaf_levels = 3;        // the same for the batched prover's encodes (0: none)
     int mk_block_log = 10;            // Merkle schedule: 2^this first-level nodes per workgroup (6..10)
+    int mk_reg_log = 2;               // Merkle: levels a lane climbs in registers (0..2; 2^this nodes per lane)
     // per-size plan (0 = the defaults above); set by the autotuner
     struct { int passbits, tile; } ntt_plan[NTT_LOG_MAX + 1] = {};
     bool overlap = true;              // plain prover: sumcheck beside the commit

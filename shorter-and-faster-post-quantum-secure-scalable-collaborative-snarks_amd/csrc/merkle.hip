@@ -57,7 +57,7 @@ __device__ __forceinline__ void hash_pair(const uint32_t l[8], const uint32_t r[
 // MODE_NODES: the first level is level `lvl0` computed from level lvl0-1.
 // MODE_COLUMNS: the first level holds the column roots of a row-major matrix
 // of `col_rows` rows of `col_len` elements (dup-left column trees only).
-// A lane owns 2^RL consecutive first-level nodes (RL = min(2, block_log)) and
+// A lane owns 2^RL consecutive first-level nodes (RL = J.reg_log) and
 // climbs RL levels in registers; the workgroup then climbs the remaining
 // block_log - RL levels through LDS.  All input loads of a lane are issued
 // before its first compression.
@@ -72,6 +72,7 @@ struct MkJob {
     LevelOffsets lo;
     uint64_t col_len;
     int lvl0, block_log, dup_left, col_rows;
+    int reg_log;                // RL: first-level nodes per lane 2^RL, 0 <= RL <= 2, block_log - RL <= 8
     uint32_t first_block;
 };
 
@@ -86,7 +87,7 @@ __device__ __forceinline__ void merkle_block(const MkJob &J, uint64_t blk) {
     const int lvl0 = J.lvl0, block_log = J.block_log, dup_left = J.dup_left, col_rows = J.col_rows;
     const uint64_t col_len = J.col_len;
     const int tid = threadIdx.x;
-    const int RL = block_log < 2 ? block_log : 2;
+    const int RL = J.reg_log;
     const int per = 1 << RL;                            // first-level nodes per lane
     const int lanes = (1 << block_log) >> RL;           // active lanes
     const uint64_t g0 = blk << block_log;
@@ -221,6 +222,15 @@ __global__ __launch_bounds__(MK_THREADS) void merkle_batch_kernel(const MkJob *_
 
 namespace {
 
+// levels a lane climbs in registers: `want` (the context's schedule) within
+// 0..min(2, block_log), and at least block_log - 8 (at most 256 lanes)
+int reg_levels(int block_log, int want) {
+    int rl = want < 2 ? want : 2;
+    if (rl > block_log) rl = block_log;
+    if (rl < block_log - 8) rl = block_log - 8;
+    return rl < 0 ? 0 : rl;
+}
+
 LevelOffsets level_offsets(uint64_t leaves) {
     LevelOffsets lo{};
     uint64_t off = 0;
@@ -235,7 +245,8 @@ LevelOffsets level_offsets(uint64_t leaves) {
 // nodes, then one per up to ten further levels until the root.  Returns the
 // (job, workgroups) of each launch.
 std::vector<std::pair<MkJob, uint64_t>> tree_stages(int mode, const fe *elems, uint64_t leaves, uint8_t *nodes,
-                                                    bool dup_left, uint64_t col_len, int col_rows, uint64_t cap) {
+                                                    bool dup_left, uint64_t col_len, int col_rows, uint64_t cap,
+                                                    int want_rl) {
     std::vector<std::pair<MkJob, uint64_t>> st;
     MkJob J{};
     J.elems = elems;
@@ -246,6 +257,7 @@ std::vector<std::pair<MkJob, uint64_t>> tree_stages(int mode, const fe *elems, u
     J.dup_left = (int)dup_left;
     J.lvl0 = 0;
     J.block_log = ilog2(leaves < cap ? leaves : cap);
+    J.reg_log = reg_levels(J.block_log, want_rl);
     uint64_t blocks = leaves >> J.block_log;
     st.push_back({J, blocks});
     int top = J.block_log;                            // highest level written
@@ -257,6 +269,7 @@ std::vector<std::pair<MkJob, uint64_t>> tree_stages(int mode, const fe *elems, u
         U.in_nodes = nodes + 32 * J.lo.off[top];
         U.lvl0 = top + 1;
         U.block_log = ilog2(next < cap ? next : cap);
+        U.reg_log = reg_levels(U.block_log, want_rl);
         const uint64_t nb = next >> U.block_log;
         st.push_back({U, nb});
         top += 1 + U.block_log;
@@ -268,7 +281,7 @@ std::vector<std::pair<MkJob, uint64_t>> tree_stages(int mode, const fe *elems, u
 // Levels have.. above levels 0 .. have-1 already in nodes (level 0: `leaves`
 // digests).
 std::vector<std::pair<MkJob, uint64_t>> upper_stages(uint64_t leaves, uint8_t *nodes, bool dup_left, int have,
-                                                     uint64_t cap) {
+                                                     uint64_t cap, int want_rl) {
     std::vector<std::pair<MkJob, uint64_t>> st;
     MkJob J{};
     J.nodes = nodes;
@@ -282,6 +295,7 @@ std::vector<std::pair<MkJob, uint64_t>> upper_stages(uint64_t leaves, uint8_t *n
         U.in_nodes = nodes + 32 * J.lo.off[top];
         U.lvl0 = top + 1;
         U.block_log = ilog2(next < cap ? next : cap);
+        U.reg_log = reg_levels(U.block_log, want_rl);
         const uint64_t nb = next >> U.block_log;
         st.push_back({U, nb});
         top += 1 + U.block_log;
@@ -295,7 +309,7 @@ uint64_t block_cap(const cosnark_ctx *ctx) { return 1ull << ctx->mk_block_log; }
 
 int build_tree(cosnark_ctx *ctx, int mode, const fe *elems, uint64_t leaves, uint8_t *nodes, bool dup_left,
                uint64_t col_len, int col_rows, hipStream_t stream) {
-    const auto st = tree_stages(mode, elems, leaves, nodes, dup_left, col_len, col_rows, block_cap(ctx));
+    const auto st = tree_stages(mode, elems, leaves, nodes, dup_left, col_len, col_rows, block_cap(ctx), ctx->mk_reg_log);
     for (size_t i = 0; i < st.size(); ++i) {
         auto k = i > 0 ? merkle_block_kernel<MODE_NODES>
                        : mode == MODE_ELEMS ? merkle_block_kernel<MODE_ELEMS> : merkle_block_kernel<MODE_COLUMNS>;
@@ -320,7 +334,7 @@ int merkle_from_leaves_launch(cosnark_ctx *ctx, uint64_t leaves, uint8_t *nodes,
         return set_err(ctx, COSNARK_E_ARG, "merkle: leaf count must be a power of two <= 2^30");
     if (have < 1 || (leaves >> (have - 1)) < 1)
         return set_err(ctx, COSNARK_E_ARG, "merkle: levels already built out of range");
-    for (const auto &st : upper_stages(leaves, nodes, dup_left, have, block_cap(ctx))) {
+    for (const auto &st : upper_stages(leaves, nodes, dup_left, have, block_cap(ctx), ctx->mk_reg_log)) {
         hipLaunchKernelGGL(merkle_block_kernel<MODE_NODES>, dim3((unsigned)st.second), dim3(MK_THREADS), 0, stream,
                            st.first);
         LAUNCHCK();
@@ -352,8 +366,9 @@ int merkle_batch_plan(cosnark_ctx *ctx, int count, const fe *const *elems, const
         if (h < 0 || (h > 0 && (leaves >> (h - 1)) < 1))
             return set_err(ctx, COSNARK_E_ARG, "merkle batch: levels already built out of range");
         // a tree whose lowest levels are in place starts above them
-        per[i] = h ? upper_stages(leaves, nodes[i], dup_left, h, block_cap(ctx))
-                   : tree_stages(MODE_ELEMS, elems[i], leaves, nodes[i], dup_left, 0, 0, block_cap(ctx));
+        per[i] = h ? upper_stages(leaves, nodes[i], dup_left, h, block_cap(ctx), ctx->mk_reg_log)
+                   : tree_stages(MODE_ELEMS, elems[i], leaves, nodes[i], dup_left, 0, 0, block_cap(ctx),
+                                 ctx->mk_reg_log);
         mode0[i] = h ? MODE_NODES : MODE_ELEMS;
         nst = std::max(nst, per[i].size());
     }

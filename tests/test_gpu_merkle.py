@@ -59,3 +59,39 @@ def test_merkle_autotune_keeps_a_valid_schedule():
         c.autotune_merkle([3])
     assert c.merkle_schedule() == b
     c.close()
+
+
+@pytest.mark.parametrize("block_log,reg_log", [(8, 0), (9, 1), (9, 0), (6, 0), (10, 2)])
+def test_merkle_register_levels_match_oracle(block_log, reg_log):
+    """The Merkle schedule's register levels (COSNARK_MK_REG_LOG: 2^RL
+    first-level nodes per lane, at least block_log - 8): trees from the
+    elements (both parent forms) and from level 2 (a WHIR commit and the
+    plain prover's commitment, levels 1 and 2 from the encode) against the
+    oracle."""
+    import os
+    import cosnark
+    env = {"COSNARK_MK_BLOCK_LOG": str(block_log), "COSNARK_MK_REG_LOG": str(reg_log),
+           "COSNARK_NTT_LEAF_LEVELS": "3"}
+    os.environ.update(env)
+    try:
+        c = cosnark.Context(0)
+    finally:
+        for k in env:
+            del os.environ[k]
+    try:
+        print(f"merkle register levels b={block_log} rl={reg_log}", flush=True)
+        for dup in (True, False):
+            e = O.synth(0xE9 + block_log, reg_log, 1 << 18)
+            assert np.array_equal(c.merkle_commit(e, dup), O.merkle_commit(e, dup))
+        a = O.synth(0xEA, block_log, 1 << 16)
+        cw, nodes = c.whir_commit(c.upload(a), 1 << 16, 4)
+        ref_cw = O.rs_encode(a, 4)
+        got = c.download(nodes, (cosnark.merkle_node_count(4 << 16), 32), np.uint8)
+        assert np.array_equal(got, O.merkle_commit(ref_cw, True))
+        f, g = O.synth(0xEB, 0, 1 << 16), O.synth(0xEB, 1, 1 << 16)
+        df, dg = c.upload(f), c.upload(g)
+        proof, cwb, ndb = c.prove_plain_keep(df, dg, 1 << 16, 4, cosnark.Transcript())
+        got = c.download(ndb, (cosnark.merkle_node_count(4 << 16), 32), np.uint8)
+        assert np.array_equal(got, O.merkle_commit(O.rs_encode(f, 4), True))
+    finally:
+        c.close()
