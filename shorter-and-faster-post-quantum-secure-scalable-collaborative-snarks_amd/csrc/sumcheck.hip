@@ -347,29 +347,50 @@ __device__ __forceinline__ fe quad_eval(const fe &f0, const fe &f1, const fe &f2
 // quadruples; nine partials per workgroup go to part_out.
 __device__ __forceinline__ void sc_pair01(const ScJob &J, uint32_t blk) {
     __shared__ fe red[SC_THREADS / 64][9];
+    // per wave, the 64 quadruples of each vector it takes in one step, padded
+    // one element in five so the quadruple reads spread over the banks
+    __shared__ fe stage[SC_THREADS / 64][2][320];
     const fe *__restrict__ s1 = J.s1;
     const fe *__restrict__ s2 = J.s2;
     const uint64_t nq = J.npairs;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    fe *const xs = stage[wave][0], *const ys = stage[wave][1];
     fe q[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) q[i] = gf::zero();
     const uint64_t stride = (uint64_t)J.nb * SC_THREADS;
-    // the next quadruples' loads are issued before this one's products
-    uint64_t j = (uint64_t)blk * SC_THREADS + threadIdx.x;
-    fe x[4], y[4];
-    if (j < nq)
+    // A wave takes 64 consecutive quadruples (256 elements of each vector) a
+    // step, loaded with consecutive lanes on consecutive elements (each load
+    // instruction one contiguous KiB) and regrouped through its LDS slice, so
+    // lane l gets quadruple l; the next step's loads are issued first.
+    uint64_t wb = (uint64_t)blk * SC_THREADS + (uint64_t)wave * 64;   // the wave's first quadruple
+    fe xc[4], yc[4];
+    auto load = [&](uint64_t base) {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) { x[e] = s1[4 * j + e]; y[e] = s2[4 * j + e]; }
-    while (j < nq) {
-        const uint64_t jn = j + stride;
-        fe xn[4], yn[4];
-        if (jn < nq)
+        for (int e = 0; e < 4; ++e) {
+            const uint64_t i = 4 * base + lane + 64 * e;
+            xc[e] = i < 4 * nq ? s1[i] : gf::zero();
+            yc[e] = i < 4 * nq ? s2[i] : gf::zero();
+        }
+    };
+    if (wb < nq) load(wb);
+    while (wb < nq) {
 #pragma unroll
-            for (int e = 0; e < 4; ++e) { xn[e] = s1[4 * jn + e]; yn[e] = s2[4 * jn + e]; }
-        quad_acc(q, x, y);
+        for (int e = 0; e < 4; ++e) {
+            const int i = lane + 64 * e;
+            xs[i + (i >> 2)] = xc[e];
+            ys[i + (i >> 2)] = yc[e];
+        }
+        const uint64_t wn = wb + stride;
+        if (wn < nq) load(wn);
+        fe x[4], y[4];
 #pragma unroll
-        for (int e = 0; e < 4; ++e) { x[e] = xn[e]; y[e] = yn[e]; }
-        j = jn;
+        for (int k = 0; k < 4; ++k) {
+            x[k] = xs[5 * lane + k];
+            y[k] = ys[5 * lane + k];
+        }
+        if (wb + lane < nq) quad_acc(q, x, y);
+        wb = wn;
     }
     block_partials<9, SC_THREADS>(q, J.part_out + 9 * blk, red);
 }
