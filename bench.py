@@ -47,6 +47,7 @@ def parse():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving (config 5) leg")
     ap.add_argument("--no-tune", action="store_true", help="keep the default NTT radix plan")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="serving leg arrival rate (requests/s)")
+    ap.add_argument("--serve-slots", type=int, default=4, help="serving leg: passes in flight (stream pairs)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="first run three 256 MiB device-to-device copies (a kernel of known traffic, for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -329,10 +330,11 @@ def batched_leg(ctx, reps=10, warm=10):
             "proofs_identical_to_sequential": bool(same)}
 
 
-def serving_leg(ctx, requests=1000, rate_hz=2000.0, sweep=(500.0, 1000.0, 2000.0, 3000.0, 4000.0, 6000.0, 8000.0, 12000.0)):
+def serving_leg(ctx, requests=1000, rate_hz=2000.0, sweep=(500.0, 1000.0, 2000.0, 3000.0, 4000.0, 6000.0, 8000.0, 12000.0),
+                slots=4):
     """Config 5: Poisson arrivals at rate_hz of proof requests of 2^e
     elements, e uniform on 14..22 (seed 0xC05A0005); witnesses device-resident
-    (one f, g pair per size); HBM budget 16 GiB, 3 slots.  Every served proof
+    (one f, g pair per size); HBM budget 16 GiB, `slots` passes in flight.  Every served proof
     is checked against the plain prover's proof of the same witness.  The
     same stream is then offered at each rate of `sweep` (a fresh server per
     rate, about a second of arrivals each) up to and past saturation, with
@@ -349,7 +351,7 @@ def serving_leg(ctx, requests=1000, rate_hz=2000.0, sweep=(500.0, 1000.0, 2000.0
         exps = [14 + int(v) % 9 for v in cosnark.synth(seed, 3, count)[:, 0]]
         gaps = np.random.default_rng(seed).exponential(1.0 / rate, count)
         arrivals = np.cumsum(gaps)
-        srv = cosnark.Server(16 << 30, rate=4, max_log_n=22, slots=3)
+        srv = cosnark.Server(16 << 30, rate=4, max_log_n=22, slots=slots)
         for e in (14, 18, 22):      # warm the server's kernels and tables
             srv.wait(srv.submit(pool[e][0], pool[e][1], 1 << e), 1 << e)
         tickets = []
@@ -387,7 +389,7 @@ def serving_leg(ctx, requests=1000, rate_hz=2000.0, sweep=(500.0, 1000.0, 2000.0
         by_size[f"2^{e}"] = {"requests": int(le.size), "p50_ms": round(float(np.percentile(le, 50)), 3),
                              "p99_ms": round(float(np.percentile(le, 99)), 3)}
     return {"workload": f"{requests} requests, Poisson arrivals at {rate_hz:g}/s, n = 2^e with e uniform on 14..22 "
-                        "(seed 0xC05A0005), RS blowup 4, HBM budget 16 GiB, 3 slots",
+                        f"(seed 0xC05A0005), RS blowup 4, HBM budget 16 GiB, {slots} slots",
             "offered_per_s": rate_hz, "throughput_proofs_per_s": round(requests / wall, 1),
             "elements_per_s": round(sum(1 << e for e in exps) / wall, 1),
             "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
@@ -604,7 +606,7 @@ def main():
     if not a.no_batch:
         out["batched_cfg4"] = batched_leg(ctx)
     if not a.no_serve:
-        out["serving_cfg5"] = serving_leg(ctx, rate_hz=a.serve_rate)
+        out["serving_cfg5"] = serving_leg(ctx, rate_hz=a.serve_rate, slots=a.serve_slots)
     if not a.no_cpu:
         c = cpu_baseline(f, g, a.rate, proof)
         out["cpu_baseline"] = {"value": round(c["mt_ms"], 1), "unit": "ms per rank share", "cores": c["threads"],
