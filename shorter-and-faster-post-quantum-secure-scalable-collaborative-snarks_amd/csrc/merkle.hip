@@ -269,8 +269,10 @@ std::vector<std::pair<MkJob, uint64_t>> tree_stages(int mode, const fe *elems, u
         U.in_nodes = nodes + 32 * J.lo.off[top];
         U.lvl0 = top + 1;
         U.block_log = ilog2(next < cap ? next : cap);
-        U.reg_log = reg_levels(U.block_log, want_rl);
         const uint64_t nb = next >> U.block_log;
+        // a one-workgroup stage is a latency chain: one node per lane, the
+        // most lanes at its first level
+        U.reg_log = reg_levels(U.block_log, nb == 1 ? 0 : want_rl);
         st.push_back({U, nb});
         top += 1 + U.block_log;
         count = nb;
@@ -295,8 +297,10 @@ std::vector<std::pair<MkJob, uint64_t>> upper_stages(uint64_t leaves, uint8_t *n
         U.in_nodes = nodes + 32 * J.lo.off[top];
         U.lvl0 = top + 1;
         U.block_log = ilog2(next < cap ? next : cap);
-        U.reg_log = reg_levels(U.block_log, want_rl);
         const uint64_t nb = next >> U.block_log;
+        // a one-workgroup stage is a latency chain: one node per lane, the
+        // most lanes at its first level
+        U.reg_log = reg_levels(U.block_log, nb == 1 ? 0 : want_rl);
         st.push_back({U, nb});
         top += 1 + U.block_log;
         count = nb;
