@@ -139,10 +139,11 @@ int batch_core_build(cosnark_ctx *ctx, int count, const BatchProof *pr, int rate
             C.tb.at<uint8_t *>(G.lv_tab)[j] = nds[kv.second[j]];
         }
         if (ctx->batch_leaf_levels > 0) {
-            uint8_t *const probe[1] = {nds[kv.second[0]]};
-            int rc = ntt_launch_batch(ctx, nullptr, (const fe *const *)fin.data(), G.in_len, nullptr, cws.data(), G.B,
-                                      G.logn, false, nullptr, nullptr, nullptr, &G.levels, ctx->batch_leaf_levels,
-                                      probe, true);
+            // a dry run launches and reads nothing: only the plan and the
+            // presence of a node table count
+            static uint8_t *const any_tab[1] = {nullptr};
+            int rc = ntt_launch_batch(ctx, nullptr, nullptr, G.in_len, nullptr, nullptr, G.B, G.logn, false, nullptr,
+                                      nullptr, nullptr, &G.levels, ctx->batch_leaf_levels, any_tab, true);
             if (rc) return rc;
         }
         for (int i : kv.second) have[i] = G.levels;
