@@ -20,7 +20,7 @@
 // launches.  Transcript values are kept per round (tr_prev[k], tr_y[k] =
 // state before round k) so a kernel never reads a slot it writes.  Once a
 // vector is small (sc_tail_elems) one workgroup runs all remaining rounds:
-// its wave 0 runs the transcript while waves 1..3 fold by the challenge just
+// its wave 0 runs the transcript while waves 1..7 fold by the challenge just
 // drawn and form the next round's sums as quadratics in the challenge still
 // being hashed, so a round costs the hash chain and little else.
 #include <algorithm>
@@ -35,7 +35,7 @@ using gf::fe;
 namespace {
 
 constexpr int SC_THREADS = 256;
-constexpr int SC_TAIL_THREADS = 256;    // wave 0: transcript; waves 1..3: folds and sums
+constexpr int SC_TAIL_THREADS = 512;    // wave 0: transcript; waves 1..7: folds and sums
 constexpr int SC_TAIL_CT = SC_TAIL_THREADS - 64;
 constexpr int SC_TAIL_WAVES = SC_TAIL_CT / 64;
 constexpr int SC_ITEMS = 1;             // pairs per thread per sweep in the multi-workgroup rounds
@@ -569,14 +569,14 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 constexpr int SC_SPLIT_MAX = 7;    // groups whose 8 folds and 9 products fit one wave's lanes
-constexpr int SC_SPLIT_NG = 64;    // phases with at most this many groups spread them over lanes
+constexpr int SC_SPLIT_NG = 128;   // phases with at most this many groups spread them over lanes
 constexpr int SC_TAIL_LDS = 512;   // folded vectors up to this many elements stay in LDS
 
 // All rounds from `round0` to m-1 in one workgroup of four waves.  The input
 // (len elements) is v(round0) when round0 == 0, else v(round0 - 1), still to
 // be folded by r[round0 - 1], which is finalized here from part_in.  Phase k
 // finalizes round i = round0 + k on wave 0 (lanes 0..3: sums, claim check,
-// transcript hashes) while waves 1..3 fold the current vector by r[i - 1]
+// transcript hashes) while waves 1..7 fold the current vector by r[i - 1]
 // into the next buffer (d, e, d, ...: never the input) and accumulate round
 // i + 1's sums as quadratics in r[i]; one barrier ends each phase.  A folded
 // vector of at most SC_TAIL_LDS elements stays in LDS (32 KiB for both
@@ -711,19 +711,20 @@ __device__ __forceinline__ void sc_tail(const ScJob &J) {
             const bool next = i + 1 < m;       // round i + 1 exists: accumulate its sums
             fe *w1 = out1(), *w2 = out2();
             if (fold && next && cur / 8 <= (uint32_t)SC_SPLIT_NG) {
-                // few groups: compute wave cw takes groups cw, cw + 3, ..., seven at a
+                // few groups: compute wave cw takes groups cw, cw + 7, ..., seven at a
                 // time: one fold per lane, then one of the nine products per lane,
                 // through the wave's own LDS rows; lanes 0..8 keep the sums
                 const uint32_t ng = cur / 8;
                 const int cw = wave - 1;
-                const uint32_t mine = ng > (uint32_t)cw ? (ng - cw + 2) / 3 : 0;
+                const uint32_t mine =
+                    ng > (uint32_t)cw ? (ng - cw + SC_TAIL_WAVES - 1) / SC_TAIL_WAVES : 0;
                 const fe r = sh_r[s];
                 fe acc = gf::zero();
                 for (uint32_t j0 = 0; j0 < mine; j0 += SC_SPLIT_MAX) {
                     const int nc = (int)min<uint32_t>((uint32_t)SC_SPLIT_MAX, mine - j0);
                     if (lane < 8 * nc) {
                         const int gi = lane >> 3, f = lane & 7, e = f & 3;
-                        const uint32_t g = cw + 3 * (j0 + gi);
+                        const uint32_t g = cw + SC_TAIL_WAVES * (j0 + gi);
                         const fe *src = f < 4 ? c1 : c2;
                         const fe x = fold1(src[8 * g + 2 * e], src[8 * g + 2 * e + 1], r);
                         (f < 4 ? w1 : w2)[4 * g + e] = x;
