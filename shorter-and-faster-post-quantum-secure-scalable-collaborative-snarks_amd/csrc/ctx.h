@@ -70,7 +70,7 @@ struct cosnark_ctx {
     int sc_blocks = 512;              // workgroups of a multi-workgroup sumcheck round (<= SC_MAX_BLOCKS)
     int sc_batch_blocks = 256;        // the same per proof in a batched round (many proofs share the launch)
     uint64_t sc_tail_elems = 1024;    // vectors this short finish in the one-workgroup tail (power of two, >= 4)
-    int sc_wide_blocks = 512;         // workgroups of the two-round launches on long vectors (<= SC_MAX_BLOCKS / 3)
+    int sc_wide_blocks = 256;         // workgroups of the two-round launches on long vectors (<= SC_MAX_BLOCKS / 3)
     int sc_wide_shared = 128;         // the same beside the commit (sumcheck on stream2): leaves the NTT its CUs
 };
 
